@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the fractional-step hot path on one MI355X.
+
+Workload (default): the 256x128x128 stretched curvilinear LES channel with an
+immersed sphere (tanh beta=2, sinusoidal bottom, Re=5000, Cs^2=0.01), seeded
+solenoidal k^-5/3 velocity.  One step = one fractional step in the
+reference's order (solvers.c:648-1385): eddy viscosity, Pressure_Gradient,
+4-stage RungeKutta momentum (4 Formfunction_2 evaluations), VolumeFlux,
+Poisson RHS, Krylov-multigrid solve to ||b-A phi||/||b|| <= 1e-8 from a zero
+initial guess, UpdatePressure, Projection, Contra2Cart, Divergence.
+
+Prints one JSON line.  value = RHS throughput in Gcell/s of Formfunction_2
+(FP64, interior cells / mean kernel time from HIP events on the library's
+stream); the Poisson time-to-1e-8 in FP64 and in mixed FP32/FP64, the step
+times of both modes and the roofline of the dominant kernel are reported in
+the same line.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vfs-geophysics-3.1_amd"))
+
+HBM_PEAK_GBS = 8000.0
+RHS_BYTES_PER_CELL = {1: 424, 0: 416}   # Formfunction_2 compulsory traffic, LES on/off
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class HipEvents:
+    """hipEvent timing on the null stream (the library's default stream)."""
+
+    def __init__(self):
+        self.h = C.CDLL("libamdhip64.so")
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.h.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e):
+        assert self.h.hipEventRecord(e, C.c_void_p(0)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.h.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def setup(name, N=None):
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    cfg, spec = W.make_config(name, N)
+    ctx = V.Ctx(cfg)
+    coor = W.grid(spec)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    if spec["immersed"]:
+        ctx.nvert_sphere(*W.SPHERE)
+    nvert = ctx.nvert().host()
+    metrics = {n: ctx.metric(n).host() for n in ("cent", "icsi", "jeta", "kzet")}
+    ucat, ucont = W.fields(cfg, spec, metrics, nvert)
+    n_unknowns = ctx.poisson_setup()
+    return cfg, spec, ctx, ucat, ucont, n_unknowns
+
+
+class State:
+    def __init__(self, ctx, ucat, ucont):
+        import vfsgpu as V
+        self.uc = V.DeviceArray.from_host(ucont)
+        self.u = V.DeviceArray.from_host(ucat)
+        self.u0 = ctx.vec()
+        self.nut = ctx.sca()
+        self.p = ctx.sca()
+        self.dp = ctx.vec()
+        self.rhs = ctx.vec()
+        self.phi = ctx.sca()
+        self.b = ctx.sca()
+
+
+def fractional_step(ctx, cfg, s, precision, rtol=1e-8, timers=None):
+    from vfsgpu import workloads as W
+    if cfg["les"]:
+        ctx.eddy_viscosity(s.u, W.CS_LES, s.nut)
+    ctx.pressure_gradient(s.p, s.dp)
+    s.u0.copy_from(s.uc)
+    for alfa in (0.25, 1 / 3.0, 0.5, 1.0):
+        ctx.rk_stage(s.u0, s.uc, s.u, s.nut, s.dp, alfa, s.rhs)
+    ctx.volume_flux(s.uc)
+    ctx.poisson_rhs(s.uc, s.b)
+    ctx.sync()
+    t0 = time.perf_counter()
+    s.phi.upload(np.zeros(s.phi.n))
+    it, rel = ctx.poisson_solve(s.b, s.phi, rtol=rtol, precision=precision, maxit=1000)
+    ctx.sync()
+    t1 = time.perf_counter()
+    ctx.update_pressure(s.p, s.phi)
+    ctx.projection(s.phi, s.uc)
+    ctx.contra2cart(s.uc, s.u)
+    maxdiv = ctx.divergence(s.uc)
+    if timers is not None:
+        timers.append(dict(poisson_s=t1 - t0, iters=it, relres=rel, maxdiv=maxdiv))
+    return it, rel, maxdiv
+
+
+def time_rhs(ctx, cfg, s, reps):
+    """Mean Formfunction_2 time (ms) from HIP events over `reps` calls."""
+    ev = HipEvents()
+    a, b = ev.event(), ev.event()
+    ctx.contra2cart(s.uc, s.u)
+    for _ in range(3):
+        ctx.formfunction2(s.uc, s.u, s.nut, s.rhs, 1.0)
+    ctx.sync()
+    ev.record(a)
+    for _ in range(reps):
+        ctx.formfunction2(s.uc, s.u, s.nut, s.rhs, 1.0)
+    ev.record(b)
+    return ev.elapsed_ms(a, b) / reps
+
+
+def cpu_baseline(name, budget_s):
+    """The CPU oracle's Formfunction_2 on a bounded sample of the same
+    generator (single thread)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_py import Oracle
+    from vfsgpu import workloads as W
+    N = (128, 64, 64) if name != "C1" else (64, 64, 64)
+    cfg, spec = W.make_config(name, N)
+    coor = W.grid(spec)
+    o = Oracle(cfg, coor)
+    if spec["immersed"]:
+        o.nvert_sphere(*W.SPHERE)
+    ucat, ucont = W.fields(cfg, spec, o.g, o.nvert)
+    o.contra2cart(ucont, ucat)
+    nut = o.sca()
+    if cfg["les"]:
+        o.eddy_viscosity(ucat, W.CS_LES, nut)
+    rhs = o.vec()
+    reps, t = 0, 0.0
+    while t < budget_s and reps < 50:
+        t0 = time.perf_counter()
+        o.formfunction2(ucont, ucat, nut, rhs, 1.0)
+        t += time.perf_counter() - t0
+        reps += 1
+    cells = N[0] * N[1] * N[2]
+    return dict(value=round(cells * reps / t / 1e9, 5), unit="Gcell/s", cores=1, kind="port",
+                sample=f"{reps} Formfunction_2 evaluations of the {name} generator at {N[0]}x{N[1]}x{N[2]} "
+                       f"cells, FP64, one thread ({t:.1f} s)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="C2")
+    ap.add_argument("--rhs-reps", type=int, default=20)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              f"(the multi-GPU k-slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    cfg, spec, ctx, ucat, ucont, nunk = setup(args.config)
+    Ni, Nj, Nk = spec["N"]
+    cells = Ni * Nj * Nk
+    log(f"[bench] {args.config} {Ni}x{Nj}x{Nk} cells={cells} unknowns={nunk}")
+    s = State(ctx, ucat, ucont)
+
+    res = {}
+    for mode, prec in (("fp64", 0), ("mixed", 1)):
+        st = State(ctx, ucat, ucont)
+        for _ in range(args.warmup):
+            fractional_step(ctx, cfg, st, prec)
+        ctx.sync()
+        timers = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fractional_step(ctx, cfg, st, prec, timers=timers)
+        ctx.sync()
+        t1 = time.perf_counter()
+        res[mode] = dict(step_ms=(t1 - t0) / args.steps * 1e3,
+                         poisson_s=float(np.median([x["poisson_s"] for x in timers])),
+                         iters=int(np.median([x["iters"] for x in timers])),
+                         relres=max(x["relres"] for x in timers),
+                         maxdiv=max(x["maxdiv"] for x in timers))
+        log(f"[bench] {mode}: {res[mode]}")
+
+    ctx.contra2cart(s.uc, s.u)
+    if cfg["les"]:
+        from vfsgpu import workloads as W
+        ctx.eddy_viscosity(s.u, W.CS_LES, s.nut)
+    rhs_ms = time_rhs(ctx, cfg, s, args.rhs_reps)
+    gcells = cells / (rhs_ms * 1e-3) / 1e9
+    bytes_per_cell = RHS_BYTES_PER_CELL[cfg["les"]]
+    achieved = bytes_per_cell * cells / (rhs_ms * 1e-3) / 1e9
+    log(f"[bench] Formfunction_2 {rhs_ms:.3f} ms -> {gcells:.2f} Gcell/s, {achieved:.0f} GB/s")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.config, args.cpu_budget)
+        log(f"[bench] cpu baseline {cpu}")
+
+    out = {
+        "metric": "RHS Gcell/s + Poisson time-to-1e-8 residual, FP64 vs mixed FP32, 1 GPU",
+        "value": round(gcells, 3),
+        "unit": "Gcell/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["mixed"]["step_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.config} {Ni}x{Nj}x{Nk} curvilinear channel"
+                               + (" LES + immersed sphere" if cfg["les"] else ""),
+                   "cells": cells, "poisson_unknowns": nunk,
+                   "step": "LES + Pressure_Gradient + RK4 (4x Formfunction_2) + VolumeFlux + Poisson to 1e-8 "
+                           "+ UpdatePressure + Projection + Contra2Cart + Divergence"},
+        "rhs_ms": round(rhs_ms, 4),
+        "poisson": {"fp64_s_to_1e-8": round(res["fp64"]["poisson_s"], 5),
+                    "mixed_s_to_1e-8": round(res["mixed"]["poisson_s"], 5),
+                    "fp64_iters": res["fp64"]["iters"], "mixed_iters": res["mixed"]["iters"],
+                    "max_relres": max(res["fp64"]["relres"], res["mixed"]["relres"]),
+                    "max_div_after_projection": max(res["fp64"]["maxdiv"], res["mixed"]["maxdiv"])},
+        "step_ms": {"fp64": round(res["fp64"]["step_ms"], 3), "mixed": round(res["mixed"]["step_ms"], 3),
+                    "speedup_mixed_vs_fp64": round(res["fp64"]["step_ms"] / res["mixed"]["step_ms"], 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "Formfunction_2", "bytes_per_cell": bytes_per_cell},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

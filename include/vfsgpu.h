@@ -1,0 +1,162 @@
+/*
+ * vfsgpu.h -- C ABI of the MI355X (gfx950) implementation of the
+ * VFS-Geophysics 3.1 fractional-step hot path: the curvilinear momentum
+ * right-hand side and the pressure projection.
+ *
+ * Conventions (shared by every entry point):
+ *  - A context (vfs_ctx) stands for the reference's UserCtx: grid sizes,
+ *    flags, metrics and the nvert mask live in it, on the device.
+ *  - Field arguments are DEVICE pointers in the reference's single-rank DMDA
+ *    layout: sizes mx = IM+1, my = JM+1, mz = KM+1 (IM = node count), scalar
+ *    s[(k*my + j)*mx + i], vector (Cmpnts) v[3*((k*my + j)*mx + i) + c] with
+ *    c = x,y,z.  A host wrapper can hand DAVecGetArray storage straight
+ *    through after copying it to the device.
+ *  - Return value 0 = success, < 0 = error (vfs_last_error() explains); no
+ *    entry point calls exit().  Work is enqueued on the context's stream;
+ *    entry points that return scalars synchronize that stream.
+ *  - All arithmetic of the drop-in path is IEEE FP64; the FP32 forms are
+ *    explicit options (precision argument of vfs_poisson_solve).
+ */
+#ifndef VFSGPU_H
+#define VFSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vfs_ctx vfs_ctx;
+
+/* Run-time flags of the reference that the hot path reads (main.c:1602-2110,
+ * init.c:52-69, bcs.dat via init.c:561-566).  bctype: 1 = no-slip wall,
+ * 7 = periodic (the matching *_periodic flag must be set). */
+typedef struct {
+    int mx, my, mz;
+    int i_periodic, j_periodic, k_periodic;
+    int bctype[6];
+    int immersed;        /* -imm: 0 no IB, 1 CURVIB masks              */
+    int les;             /* -les 1: constant-coefficient Smagorinsky    */
+    int second_order;    /* -second_order                              */
+    int dpdz_set;        /* k-periodic forcing from mean_pressure_gradient */
+    double ren, dt, st;  /* Reynolds number, time step, Strouhal factor */
+    double mean_pressure_gradient;
+} vfs_config;
+
+/* ---- context, memory, stream ------------------------------------------ */
+/* Replaces the UserCtx set-up of MG_Initial (init.c:270-930). */
+int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out);
+int vfs_ctx_destroy(vfs_ctx *ctx);
+const char *vfs_last_error(void);
+int vfs_set_stream(vfs_ctx *ctx, void *hip_stream);
+int vfs_synchronize(vfs_ctx *ctx);
+int vfs_device_malloc(size_t bytes, void **ptr);   /* zero-filled */
+int vfs_device_free(void *ptr);
+int vfs_memcpy_htod(void *dst, const void *src, size_t bytes);
+int vfs_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int vfs_memcpy_dtod(void *dst, const void *src, size_t bytes);
+
+/* ---- geometry ---------------------------------------------------------- */
+/* FormMetrics (metrics.c:4-1346, NEWMETRIC): cell and face metrics from node
+ * coordinates coor (vector field, nodes i<mx-1, j<my-1, k<mz-1). */
+int vfs_build_metrics(vfs_ctx *ctx, const double *coor);
+/* Copy one metric array out: which = 0..16 in the order csi, eta, zet, aj,
+ * icsi, ieta, izet, iaj, jcsi, jeta, jzet, jaj, kcsi, keta, kzet, kaj, cent. */
+int vfs_get_metric(vfs_ctx *ctx, int which, double *dst);
+/* Replace the context's nvert mask (the reference's user->Nvert). */
+int vfs_set_nvert(vfs_ctx *ctx, const double *nvert);
+int vfs_get_nvert(vfs_ctx *ctx, double *dst);
+/* IB mask of a sphere from its signed distance at cell centres with the
+ * reference's 3x3x3 dilation and relabelling (ibm.c:224, 319-341, 508-516):
+ * 3 solid, 1 IB node, 0 fluid.  Stored in the context and copied to nvert_out
+ * when that is not NULL. */
+int vfs_nvert_sphere(vfs_ctx *ctx, double cx, double cy, double cz, double r, double *nvert_out);
+
+/* ---- momentum ---------------------------------------------------------- */
+/* Contra2Cart_2 (rhs.c:57-741; called through Contra2Cart, rhs.c:27).
+ * ucont: in/out (periodic ghosts refreshed), ucat: in/out (values at cells
+ * with nvert >= 0.1 are inputs and are kept, solid cells are zeroed). */
+int vfs_contra2cart(vfs_ctx *ctx, double *ucont, double *ucat);
+/* Compute_eddy_viscosity_LES (les.c:1135-1252) with constant Cs
+ * (Compute_Smagorinsky_Constant_1, les.c:67-77: Cs = 0.01 for -les 1). */
+int vfs_eddy_viscosity(vfs_ctx *ctx, const double *ucat, double cs, double *nu_t);
+/* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
+int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
+/* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */
+int vfs_ib_bc(vfs_ctx *ctx, double *ucont, double *ucat);
+/* Formfunction_2 (momentum.c:456-1979): rhs += scale * F(ucont, ucat, nu_t)
+ * on interior faces, masked faces and boundary planes set to zero.  ucont is
+ * in/out (its periodic ghosts are rewritten as in momentum.c:642-669).
+ * nu_t may be NULL when les == 0. */
+int vfs_formfunction2(vfs_ctx *ctx, double *ucont, const double *ucat,
+                      const double *nu_t, double *rhs, double scale);
+/* FormFunction_SNES (momentum.c:2197-2301): rhs = -(U - U_o)/dt
+ * + 0.5 FF2(U) + 0.5 rhs_o - dP.  U is not modified; lucont receives the
+ * ghosted, IB-corrected flux the RHS was evaluated on; ucat in/out. */
+int vfs_formfunction_snes(vfs_ctx *ctx, const double *U, const double *U_o,
+                          const double *rhs_o, const double *dP, const double *nu_t,
+                          double *lucont, double *ucat, double *rhs);
+/* One stage of RungeKutta (rhs.c:1637-1659):
+ * Contra2Cart; U = alfa*dt*st*(FF2(U) - dP) + U0; Contra2Cart.
+ * rhs_scratch receives FF2(U) - dP. */
+int vfs_rk_stage(vfs_ctx *ctx, const double *u0, double *ucont, double *ucat,
+                 const double *nu_t, const double *dP, double alfa, double *rhs_scratch);
+/* Calc_Minimum_dt (momentum.c:104-195): CFL-1 time step. */
+int vfs_min_dt(vfs_ctx *ctx, const double *ucont, double *dt_out);
+
+/* ---- pressure projection ------------------------------------------------ */
+/* setup_lidx2 + PoissonLHSNew (poisson.c:182-329, 971-1682): numbers the
+ * unknowns (returned in *n_unknowns) and builds the 19-point coefficients
+ * (with the 1e-10 drop rule) and the multigrid hierarchy, from the context's
+ * metrics and nvert. */
+int vfs_poisson_setup(vfs_ctx *ctx, int *n_unknowns);
+/* Unknown numbering on the DA layout (gid, -1 where no unknown). */
+int vfs_poisson_gid(vfs_ctx *ctx, double *gid);
+/* The 19 coefficients per cell after the drop rule, slot-major
+ * coef[m*P + cell], slot order of poisson.c:951-969. */
+int vfs_poisson_coefficients(vfs_ctx *ctx, double *coef);
+/* y = A x on the DA layout (FP64 operator). */
+int vfs_poisson_apply(vfs_ctx *ctx, const double *x, double *y);
+/* VolumeFlux (poisson.c:3570-3910 when immersed, 3306-3500 otherwise):
+ * net IB flux correction of ucont in place; flux/area after correction. */
+int vfs_volume_flux(vfs_ctx *ctx, double *ucont, double *flux, double *area);
+/* PoissonRHS2_hypre + Remove_Nullspace_Scale (poisson_hypre.c:257-399,
+ * 110-192): b on the DA layout, zero outside the unknowns. */
+int vfs_poisson_rhs(vfs_ctx *ctx, const double *ucont, double *b);
+/* Krylov + multigrid solve of A phi = b to ||b - A phi||/||b|| <= rtol in the
+ * FP64 operator, then Remove_Nullspace and Convert_Phi2_Phi
+ * (poisson_hypre.c:498-630, 194-255; poisson.c:92-180).
+ * precision 0: FP64 throughout; 1: FP32 stencil/smoother/Krylov inside an
+ * FP64 defect-correction loop (FP64 residual, dot products, updates).
+ * phi: in = initial guess on the DA layout, out = solution (ghosts set). */
+int vfs_poisson_solve(vfs_ctx *ctx, const double *b, double *phi, double rtol,
+                      int precision, int maxit, int *iters, double *relres);
+/* UpdatePressure (poisson.c:3144-3304): p, phi in/out. */
+int vfs_update_pressure(vfs_ctx *ctx, double *p, double *phi);
+/* Projection (poisson.c:2707-3073) without the trailing Contra2Cart. */
+int vfs_projection(vfs_ctx *ctx, const double *phi, double *ucont);
+/* Divergence (main.c:1047-1177): max |div U| * aj over unmasked cells;
+ * div (may be NULL) receives the per-cell field. */
+int vfs_divergence(vfs_ctx *ctx, const double *ucont, double *div, double *maxdiv);
+
+/* ---- synthetic workloads (host-side generators) -------------------------- */
+double vfs_wl_uniform(uint64_t seed, uint64_t counter);
+int vfs_wl_channel_grid(int IM, int JM, int KM, double Lx, double Ly, double Lz,
+                        double beta, double amp, double *coor_host);
+int vfs_wl_poiseuille(int mx, int my, int mz, const int per[3], double Ly, double noise,
+                      uint64_t seed, const double *cent, const double *nvert,
+                      const double *icsi, const double *jeta, const double *kzet,
+                      double *ucat, double *ucont);
+int vfs_wl_turbulent(int mx, int my, int mz, const int per[3], double Lx, double Ly, double Lz,
+                     int nmodes, int nmax, double urms, uint64_t seed, const double *cent,
+                     const double *nvert, const double *icsi, const double *jeta,
+                     const double *kzet, double *ucat, double *ucont);
+int vfs_wl_random_scalar(int mx, int my, int mz, double amp, uint64_t seed,
+                         const double *nvert, const double *aj, double *out);
+int vfs_wl_random_vector(int mx, int my, int mz, double amp, uint64_t seed, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

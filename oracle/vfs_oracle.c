@@ -1,0 +1,1633 @@
+/*
+ * CPU oracle: single-threaded FP64 restatement of the VFS-Geophysics
+ * fractional-step hot path.  TEST INFRASTRUCTURE ONLY (see vfs_oracle.h).
+ *
+ * Every routine follows the reference's loop order and operation order
+ * (file:line cited per function), so that it reproduces the reference's
+ * floating-point results on the same inputs.  Branches for options that no
+ * supported configuration exercises (levelset, rans, skew, clark, weno,
+ * wall functions, inflow/outflow bctypes, ii/jj/kk_periodic) are rejected
+ * with a negative return code instead of being restated.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  No FMA
+ * contraction, so every a*b+c rounds twice exactly as the reference does.
+ */
+#include "vfs_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define DIMS  const int mx = c->mx, my = c->my, mz = c->mz; (void)mx; (void)my; (void)mz
+#define IX(k, j, i) ((((size_t)(k)) * (size_t)my + (size_t)(j)) * (size_t)mx + (size_t)(i))
+#define VX(a, id) ((a)[3 * (id) + 0])
+#define VY(a, id) ((a)[3 * (id) + 1])
+#define VZ(a, id) ((a)[3 * (id) + 2])
+
+/* Periodic ghost source (the `flag/a/b/c` idiom, e.g. momentum.c:642-669).
+ * Sources are never themselves flagged, so in-place copies are order-free. */
+static int per_src(const vo_cfg *c, int i, int j, int k, int *a, int *b, int *cc)
+{
+    DIMS;
+    int flag = 0;
+    *a = i; *b = j; *cc = k;
+    if (c->i_periodic && i == 0) *a = mx - 2, flag = 1;
+    else if (c->i_periodic && i == mx - 1) *a = 1, flag = 1;
+    if (c->j_periodic && j == 0) *b = my - 2, flag = 1;
+    else if (c->j_periodic && j == my - 1) *b = 1, flag = 1;
+    if (c->k_periodic && k == 0) *cc = mz - 2, flag = 1;
+    else if (c->k_periodic && k == mz - 1) *cc = 1, flag = 1;
+    return flag;
+}
+
+static void per_copy_vec(const vo_cfg *c, double *v)
+{
+    DIMS;
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int a, b, cc;
+                if (per_src(c, i, j, k, &a, &b, &cc)) {
+                    size_t d = IX(k, j, i), s = IX(cc, b, a);
+                    VX(v, d) = VX(v, s); VY(v, d) = VY(v, s); VZ(v, d) = VZ(v, s);
+                }
+            }
+}
+
+static void per_copy_sca(const vo_cfg *c, double *v)
+{
+    DIMS;
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int a, b, cc;
+                if (per_src(c, i, j, k, &a, &b, &cc)) v[IX(k, j, i)] = v[IX(cc, b, a)];
+            }
+}
+
+/* Supported boundary set: every face is either no-slip (1) or periodic (7,
+ * with the matching i/j/k_periodic flag). */
+static int check_cfg(const vo_cfg *c)
+{
+    if (c->mx < 7 || c->my < 7 || c->mz < 7) return -2;
+    const int per[3] = {c->i_periodic, c->j_periodic, c->k_periodic};
+    for (int f = 0; f < 6; f++) {
+        int bt = c->bctype[f];
+        if (per[f / 2]) { if (bt != 7) return -3; }
+        else if (bt != 1) return -3;
+    }
+    if (c->immersed != 0 && c->immersed != 1) return -4;
+    return 0;
+}
+
+static void copy3(double *d, size_t di, const double *s, size_t si)
+{
+    d[3 * di] = s[3 * si]; d[3 * di + 1] = s[3 * si + 1]; d[3 * di + 2] = s[3 * si + 2];
+}
+
+/* ------------------------------------------------------------------------ */
+/* metrics.c:4-1346 FormMetrics, NEWMETRIC branch                           */
+/* ------------------------------------------------------------------------ */
+int vo_form_metrics(const vo_cfg *c, const double *coor, vo_geom *g)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+#define CO(k, j, i, comp) coor[3 * IX(k, j, i) + (comp)]
+    /* cell centres, metrics.c:80-99 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                for (int q = 0; q < 3; q++)
+                    g->cent[3 * IX(k, j, i) + q] = 0.125 *
+                        (CO(k, j, i, q) + CO(k, j - 1, i, q) + CO(k - 1, j, i, q) + CO(k - 1, j - 1, i, q) +
+                         CO(k, j, i - 1, q) + CO(k, j - 1, i - 1, q) + CO(k - 1, j, i - 1, q) + CO(k - 1, j - 1, i - 1, q));
+
+    /* cell Jacobian and csi/eta/zet, metrics.c:186-251 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                double d[3][3]; /* d[comp][dir]: dir 0=c, 1=e, 2=z */
+                for (int q = 0; q < 3; q++) {
+                    d[q][0] = 0.25 * (CO(k, j, i, q) + CO(k, j - 1, i, q) + CO(k - 1, j, i, q) + CO(k - 1, j - 1, i, q) -
+                                      CO(k, j, i - 1, q) - CO(k, j - 1, i - 1, q) - CO(k - 1, j, i - 1, q) - CO(k - 1, j - 1, i - 1, q));
+                    d[q][1] = 0.25 * (CO(k, j, i, q) + CO(k, j, i - 1, q) + CO(k - 1, j, i, q) + CO(k - 1, j, i - 1, q) -
+                                      CO(k, j - 1, i, q) - CO(k, j - 1, i - 1, q) - CO(k - 1, j - 1, i, q) - CO(k - 1, j - 1, i - 1, q));
+                    d[q][2] = 0.25 * (CO(k, j, i, q) + CO(k, j - 1, i, q) + CO(k, j, i - 1, q) + CO(k, j - 1, i - 1, q) -
+                                      CO(k - 1, j, i, q) - CO(k - 1, j - 1, i, q) - CO(k - 1, j, i - 1, q) - CO(k - 1, j - 1, i - 1, q));
+                }
+                double dxdc = d[0][0], dydc = d[1][0], dzdc = d[2][0];
+                double dxde = d[0][1], dyde = d[1][1], dzde = d[2][1];
+                double dxdz = d[0][2], dydz = d[1][2], dzdz = d[2][2];
+                size_t id = IX(k, j, i);
+                double a = dxdc * (dyde * dzdz - dzde * dydz) - dydc * (dxde * dzdz - dzde * dxdz) +
+                           dzdc * (dxde * dydz - dyde * dxdz);
+                g->aj[id] = 1. / a;
+                VX(g->csi, id) = dyde * dzdz - dzde * dydz;
+                VY(g->csi, id) = -dxde * dzdz + dzde * dxdz;
+                VZ(g->csi, id) = dxde * dydz - dyde * dxdz;
+                VX(g->eta, id) = dydz * dzdc - dzdz * dydc;
+                VY(g->eta, id) = -dxdz * dzdc + dzdz * dxdc;
+                VZ(g->eta, id) = dxdz * dydc - dydz * dxdc;
+                VX(g->zet, id) = dydc * dzde - dzdc * dyde;
+                VY(g->zet, id) = -dxdc * dzde + dzdc * dxde;
+                VZ(g->zet, id) = dxdc * dyde - dydc * dxde;
+            }
+#undef CO
+    /* boundary mirror, metrics.c:253-334, in the reference's plane order */
+    for (int side = 0; side < 6; side++) {
+        for (int k = 0; k < mz; k++)
+            for (int j = 0; j < my; j++)
+                for (int i = 0; i < mx; i++) {
+                    int si = i, sj = j, sk = k;
+                    if (side == 0) { if (i != 0) continue; si = 1; }
+                    if (side == 1) { if (i != mx - 1) continue; si = mx - 2; }
+                    if (side == 2) { if (j != 0) continue; sj = 1; }
+                    if (side == 3) { if (j != my - 1) continue; sj = my - 2; }
+                    if (side == 4) { if (k != 0) continue; sk = 1; }
+                    if (side == 5) { if (k != mz - 1) continue; sk = mz - 2; }
+                    size_t d = IX(k, j, i), s = IX(sk, sj, si);
+                    copy3(g->csi, d, g->csi, s); copy3(g->eta, d, g->eta, s);
+                    copy3(g->zet, d, g->zet, s); g->aj[d] = g->aj[s];
+                }
+    }
+    /* periodic ghosts of the cell metrics, metrics.c:354-389 */
+    per_copy_vec(c, g->csi); per_copy_vec(c, g->eta); per_copy_vec(c, g->zet); per_copy_sca(c, g->aj);
+
+    /* face metrics: averages of the two cells, harmonic-mean Jacobian
+     * (metrics.c:581-584 i-faces, 689-692 j-faces, 797-800 k-faces) */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 0; i < mx - 1; i++) {
+                size_t id = IX(k, j, i), ip = IX(k, j, i + 1);
+                for (int q = 0; q < 3; q++) {
+                    g->icsi[3 * id + q] = 0.5 * g->csi[3 * id + q] + 0.5 * g->csi[3 * ip + q];
+                    g->ieta[3 * id + q] = 0.5 * g->eta[3 * id + q] + 0.5 * g->eta[3 * ip + q];
+                    g->izet[3 * id + q] = 0.5 * g->zet[3 * id + q] + 0.5 * g->zet[3 * ip + q];
+                }
+                g->iaj[id] = 2. / (1. / g->aj[id] + 1. / g->aj[ip]);
+            }
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 0; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i), jp = IX(k, j + 1, i);
+                for (int q = 0; q < 3; q++) {
+                    g->jcsi[3 * id + q] = 0.5 * g->csi[3 * id + q] + 0.5 * g->csi[3 * jp + q];
+                    g->jeta[3 * id + q] = 0.5 * g->eta[3 * id + q] + 0.5 * g->eta[3 * jp + q];
+                    g->jzet[3 * id + q] = 0.5 * g->zet[3 * id + q] + 0.5 * g->zet[3 * jp + q];
+                }
+                g->jaj[id] = 2. / (1. / g->aj[id] + 1. / g->aj[jp]);
+            }
+    for (int k = 0; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i), kp = IX(k + 1, j, i);
+                for (int q = 0; q < 3; q++) {
+                    g->kcsi[3 * id + q] = 0.5 * g->csi[3 * id + q] + 0.5 * g->csi[3 * kp + q];
+                    g->keta[3 * id + q] = 0.5 * g->eta[3 * id + q] + 0.5 * g->eta[3 * kp + q];
+                    g->kzet[3 * id + q] = 0.5 * g->zet[3 * id + q] + 0.5 * g->zet[3 * kp + q];
+                }
+                g->kaj[id] = 2. / (1 / g->aj[id] + 1 / g->aj[kp]);
+            }
+    /* final periodic copy of all sixteen arrays, metrics.c:1214-1250 */
+    if (c->i_periodic || c->j_periodic || c->k_periodic) {
+        double *v[12] = {g->csi, g->eta, g->zet, g->icsi, g->ieta, g->izet,
+                         g->jcsi, g->jeta, g->jzet, g->kcsi, g->keta, g->kzet};
+        double *s[4] = {g->aj, g->iaj, g->jaj, g->kaj};
+        for (int n = 0; n < 12; n++) per_copy_vec(c, v[n]);
+        for (int n = 0; n < 4; n++) per_copy_sca(c, s[n]);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* ibm.c:319-341, 508-536: mask from an analytic sphere                    */
+/* ------------------------------------------------------------------------ */
+int vo_nvert_sphere(const vo_cfg *c, const vo_geom *g, double cx, double cy,
+                    double cz, double r, double *nvert)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    size_t P = (size_t)mx * my * mz;
+    for (size_t n = 0; n < P; n++) nvert[n] = 0.;
+    /* inside test at cell centres: 4 = inside this body (ibm.c:224) */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                double dx = VX(g->cent, id) - cx, dy = VY(g->cent, id) - cy, dz = VZ(g->cent, id) - cz;
+                double d = sqrt(dx * dx + dy * dy + dz * dz) - r;
+                if (d < 0.) nvert[id] = 4.;
+            }
+    /* near-boundary dilation over the clamped 3x3x3 box (ibm.c:319-341) */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                size_t id = IX(k, j, i);
+                if (nvert[id] < 0) nvert[id] = 0;
+                int ip = (i < mx - 1 ? (i + 1) : (i)), im = (i > 0 ? (i - 1) : (i));
+                int jp = (j < my - 1 ? (j + 1) : (j)), jm = (j > 0 ? (j - 1) : (j));
+                int kp = (k < mz - 1 ? (k + 1) : (k)), km = (k > 0 ? (k - 1) : (k));
+                if ((int)(nvert[id] + 0.5) != 4) {
+                    for (int kk = km; kk < kp + 1; kk++)
+                        for (int jj = jm; jj < jp + 1; jj++)
+                            for (int ii = im; ii < ip + 1; ii++)
+                                if ((int)(nvert[IX(kk, jj, ii)] + 0.5) == 4)
+                                    nvert[id] = nvert[id] > 2 ? nvert[id] : 2;
+                }
+            }
+    /* back to 3 (solid) / 1 (IB node) on interior cells (ibm.c:508-516) */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if ((int)(nvert[id] + 0.5) == 2) nvert[id] = 1;
+                if ((int)(nvert[id] + 0.5) == 4) nvert[id] = 3;
+            }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rhs.c:57-741 Contra2Cart_2 (bctype 1 / periodic subset)                  */
+/* ------------------------------------------------------------------------ */
+int vo_contra2cart(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   double *ucont, double *ucat)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    per_copy_vec(c, ucont); /* rhs.c:121-148 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (!(nvert[id] < 0.1)) continue;
+                double m00 = VX(g->csi, id), m01 = VY(g->csi, id), m02 = VZ(g->csi, id);
+                double m10 = VX(g->eta, id), m11 = VY(g->eta, id), m12 = VZ(g->eta, id);
+                double m20 = VX(g->zet, id), m21 = VY(g->zet, id), m22 = VZ(g->zet, id);
+                double q0 = 0.5 * (VX(ucont, IX(k, j, i - 1)) + VX(ucont, id));
+                double q1 = 0.5 * (VY(ucont, IX(k, j - 1, i)) + VY(ucont, id));
+                double q2 = 0.5 * (VZ(ucont, IX(k - 1, j, i)) + VZ(ucont, id));
+                double det = m00 * (m11 * m22 - m12 * m21) - m01 * (m10 * m22 - m12 * m20) +
+                             m02 * (m10 * m21 - m11 * m20);
+                double det0 = q0 * (m11 * m22 - m12 * m21) - q1 * (m01 * m22 - m02 * m21) +
+                              q2 * (m01 * m12 - m02 * m11);
+                double det1 = -q0 * (m10 * m22 - m12 * m20) + q1 * (m00 * m22 - m02 * m20) -
+                              q2 * (m00 * m12 - m02 * m10);
+                double det2 = q0 * (m10 * m21 - m11 * m20) - q1 * (m00 * m21 - m01 * m20) +
+                              q2 * (m00 * m11 - m01 * m10);
+                VX(ucat, id) = det0 / det;
+                VY(ucat, id) = det1 / det;
+                VZ(ucat, id) = det2 / det;
+            }
+    per_copy_vec(c, ucat); /* rhs.c:246-279 */
+    /* boundary loop reads the snapshot lucat and writes ucat (rhs.c:294-674) */
+    size_t P = (size_t)mx * my * mz;
+    double *L = (double *)malloc(3 * P * sizeof(double));
+    if (!L) return -1;
+    memcpy(L, ucat, 3 * P * sizeof(double));
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                size_t id = IX(k, j, i);
+                if ((int)(nvert[id] + 0.1) == 3) { VX(ucat, id) = VY(ucat, id) = VZ(ucat, id) = 0; continue; }
+                size_t s = (size_t)-1;
+                if (i == 0 && c->bctype[0] == 1 && (j != 0 && k != 0)) s = IX(k, j, i + 1);
+                if (i == mx - 1 && c->bctype[1] == 1 && (j != 0 && k != 0)) s = IX(k, j, i - 1);
+                if (j == 0 && c->bctype[2] == 1 && (i != 0 && k != 0)) s = IX(k, j + 1, i);
+                if (j == my - 1 && c->bctype[3] == 1 && (i != 0 && k != 0)) s = IX(k, j - 1, i);
+                if (k == 0 && c->bctype[4] == 1 && (i != 0 && j != 0)) s = IX(k + 1, j, i);
+                if (k == mz - 1 && c->bctype[5] == 1 && (i != 0 && j != 0)) s = IX(k - 1, j, i);
+                if (s != (size_t)-1) {
+                    VX(ucat, id) = -1 * VX(L, s); VY(ucat, id) = -1 * VY(L, s); VZ(ucat, id) = -1 * VZ(L, s);
+                }
+                /* 101220 corner zeroing (rhs.c:668-673) */
+                if (i != 0 && i != mx - 1 && j != 0 && j != my - 2 && k != 0 && k != mz - 2) {
+                    int z = 0;
+                    if (c->bctype[0] <= 1 && c->bctype[2] <= 1 && (i == 1 && j == 1)) z = 1;
+                    if (c->bctype[1] <= 1 && c->bctype[2] <= 1 && (i == mx - 2 && j == 1)) z = 1;
+                    if (c->bctype[0] <= 1 && c->bctype[3] <= 1 && (i == 1 && j == my - 2)) z = 1;
+                    if (c->bctype[1] <= 1 && c->bctype[3] <= 1 && (i == mx - 2 && j == my - 2)) z = 1;
+                    if (z) VX(ucat, id) = VY(ucat, id) = VZ(ucat, id) = 0;
+                }
+            }
+    free(L);
+    per_copy_vec(c, ucat); /* rhs.c:704-740 */
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* k-omega.c:48-278 face gradients, 305-423 cell gradient, 597-610 chain     */
+/* ------------------------------------------------------------------------ */
+#define SOLID 0.1 /* k-omega.c:11 */
+#define U3(k, j, i, q) ucat[3 * IX(k, j, i) + (q)]
+#define NV(k, j, i) nvert[IX(k, j, i)]
+
+static void du_i(const vo_cfg *c, int i, int j, int k, const double *ucat, const double *nvert,
+                 double dc[3], double de[3], double dz[3])
+{
+    DIMS;
+    for (int q = 0; q < 3; q++) dc[q] = U3(k, j, i + 1, q) - U3(k, j, i, q);
+    if (NV(k, j + 1, i) > SOLID || NV(k, j + 1, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j, i + 1, q) + U3(k, j, i, q) - U3(k, j - 1, i + 1, q) - U3(k, j - 1, i, q)) * 0.5;
+    else if (NV(k, j - 1, i) > SOLID || NV(k, j - 1, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i + 1, q) + U3(k, j + 1, i, q) - U3(k, j, i + 1, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i + 1, q) + U3(k, j + 1, i, q) - U3(k, j - 1, i + 1, q) - U3(k, j - 1, i, q)) * 0.25;
+    if (NV(k + 1, j, i) > SOLID || NV(k + 1, j, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k, j, i + 1, q) + U3(k, j, i, q) - U3(k - 1, j, i + 1, q) - U3(k - 1, j, i, q)) * 0.5;
+    else if (NV(k - 1, j, i) > SOLID || NV(k - 1, j, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i + 1, q) + U3(k + 1, j, i, q) - U3(k, j, i + 1, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i + 1, q) + U3(k + 1, j, i, q) - U3(k - 1, j, i + 1, q) - U3(k - 1, j, i, q)) * 0.25;
+}
+
+static void du_j(const vo_cfg *c, int i, int j, int k, const double *ucat, const double *nvert,
+                 double dc[3], double de[3], double dz[3])
+{
+    DIMS;
+    if (NV(k, j, i + 1) > SOLID || NV(k, j + 1, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j + 1, i, q) + U3(k, j, i, q) - U3(k, j + 1, i - 1, q) - U3(k, j, i - 1, q)) * 0.5;
+    else if (NV(k, j, i - 1) > SOLID || NV(k, j + 1, i - 1) > SOLID)
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j + 1, i + 1, q) + U3(k, j, i + 1, q) - U3(k, j + 1, i, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j + 1, i + 1, q) + U3(k, j, i + 1, q) - U3(k, j + 1, i - 1, q) - U3(k, j, i - 1, q)) * 0.25;
+    for (int q = 0; q < 3; q++) de[q] = U3(k, j + 1, i, q) - U3(k, j, i, q);
+    if (NV(k + 1, j, i) > SOLID || NV(k + 1, j + 1, i) > SOLID)
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k, j + 1, i, q) + U3(k, j, i, q) - U3(k - 1, j + 1, i, q) - U3(k - 1, j, i, q)) * 0.5;
+    else if (NV(k - 1, j, i) > SOLID || NV(k - 1, j + 1, i) > SOLID)
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j + 1, i, q) + U3(k + 1, j, i, q) - U3(k, j + 1, i, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j + 1, i, q) + U3(k + 1, j, i, q) - U3(k - 1, j + 1, i, q) - U3(k - 1, j, i, q)) * 0.25;
+}
+
+static void du_k(const vo_cfg *c, int i, int j, int k, const double *ucat, const double *nvert,
+                 double dc[3], double de[3], double dz[3])
+{
+    DIMS;
+    if (NV(k, j, i + 1) > SOLID || NV(k + 1, j, i + 1) > SOLID)
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k + 1, j, i, q) + U3(k, j, i, q) - U3(k + 1, j, i - 1, q) - U3(k, j, i - 1, q)) * 0.5;
+    else if (NV(k, j, i - 1) > SOLID || NV(k + 1, j, i - 1) > SOLID)
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k + 1, j, i + 1, q) + U3(k, j, i + 1, q) - U3(k + 1, j, i, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k + 1, j, i + 1, q) + U3(k, j, i + 1, q) - U3(k + 1, j, i - 1, q) - U3(k, j, i - 1, q)) * 0.25;
+    if (NV(k, j + 1, i) > SOLID || NV(k + 1, j + 1, i) > SOLID)
+        for (int q = 0; q < 3; q++) de[q] = (U3(k + 1, j, i, q) + U3(k, j, i, q) - U3(k + 1, j - 1, i, q) - U3(k, j - 1, i, q)) * 0.5;
+    else if (NV(k, j - 1, i) > SOLID || NV(k + 1, j - 1, i) > SOLID)
+        for (int q = 0; q < 3; q++) de[q] = (U3(k + 1, j + 1, i, q) + U3(k, j + 1, i, q) - U3(k + 1, j, i, q) - U3(k, j, i, q)) * 0.5;
+    else
+        for (int q = 0; q < 3; q++) de[q] = (U3(k + 1, j + 1, i, q) + U3(k, j + 1, i, q) - U3(k + 1, j - 1, i, q) - U3(k, j - 1, i, q)) * 0.25;
+    for (int q = 0; q < 3; q++) dz[q] = U3(k + 1, j, i, q) - U3(k, j, i, q);
+}
+
+/* k-omega.c:305-423 (note the k==0 quirk at 391) */
+static void du_center(const vo_cfg *c, int i, int j, int k, const double *ucat, const double *nvert,
+                      double dc[3], double de[3], double dz[3])
+{
+    DIMS;
+    if (NV(k, j, i + 1) > SOLID || (!c->i_periodic && i == mx - 2))
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i, q) - U3(k, j, i - 1, q));
+    else if (NV(k, j, i - 1) > SOLID || (!c->i_periodic && i == 1))
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i + 1, q) - U3(k, j, i, q));
+    else
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i + 1, q) - U3(k, j, i - 1, q)) * 0.5;
+    if (NV(k, j + 1, i) > SOLID || (!c->j_periodic && j == my - 2))
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j, i, q) - U3(k, j - 1, i, q));
+    else if (NV(k, j - 1, i) > SOLID || (!c->j_periodic && j == 1))
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i, q) - U3(k, j, i, q));
+    else
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i, q) - U3(k, j - 1, i, q)) * 0.5;
+    if (NV(k + 1, j, i) > SOLID || (!c->k_periodic && k == mz - 2))
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k, j, i, q) - U3(k - 1, j, i, q));
+    else if (NV(k - 1, j, i) > SOLID || (!c->k_periodic && k == 0))
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i, q) - U3(k, j, i, q));
+    else
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i, q) - U3(k - 1, j, i, q)) * 0.5;
+}
+
+/* ------------------------------------------------------------------------ */
+/* les.c:1135-1252 Compute_eddy_viscosity_LES                               */
+/* ------------------------------------------------------------------------ */
+int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                      const double *ucat, double cs, double *nu_t)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (nvert[id] > 1.1) { nu_t[id] = 0; continue; }
+                double ajc = g->aj[id];
+                double csi0 = VX(g->csi, id), csi1 = VY(g->csi, id), csi2 = VZ(g->csi, id);
+                double eta0 = VX(g->eta, id), eta1 = VY(g->eta, id), eta2 = VZ(g->eta, id);
+                double zet0 = VX(g->zet, id), zet1 = VY(g->zet, id), zet2 = VZ(g->zet, id);
+                double dc[3], de[3], dz[3];
+                du_center(c, i, j, k, ucat, nvert, dc, de, dz);
+                double du_dx = (dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0) * ajc;
+                double du_dy = (dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1) * ajc;
+                double du_dz = (dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2) * ajc;
+                double dv_dx = (dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0) * ajc;
+                double dv_dy = (dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1) * ajc;
+                double dv_dz = (dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2) * ajc;
+                double dw_dx = (dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0) * ajc;
+                double dw_dy = (dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1) * ajc;
+                double dw_dz = (dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2) * ajc;
+                double Sxx = 0.5 * (du_dx + du_dx), Sxy = 0.5 * (du_dy + dv_dx), Sxz = 0.5 * (du_dz + dw_dx);
+                double Syx = Sxy, Syy = 0.5 * (dv_dy + dv_dy), Syz = 0.5 * (dv_dz + dw_dy);
+                double Szx = Sxz, Szy = Syz, Szz = 0.5 * (dw_dz + dw_dz);
+                double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
+                                          Szx * Szx + Szy * Szy + Szz * Szz));
+                double filter = pow(1. / g->aj[id], 1. / 3.);
+                nu_t[id] = cs * pow(filter, 2.0) * Sabs;
+            }
+    per_copy_sca(c, nu_t); /* les.c:1215-1243 */
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* momentum.c:197-440 Pressure_Gradient                                      */
+/* ------------------------------------------------------------------------ */
+#define IB1(x) ((int)((x) + 0.5) == 1)
+int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         double *p, double *dp)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    if (c->k_periodic && !c->dpdz_set) return -5; /* flux forcing not restated */
+    per_copy_sca(c, p);
+    size_t P = (size_t)mx * my * mz;
+    memset(dp, 0, 3 * P * sizeof(double));
+#define PP(k, j, i) p[IX(k, j, i)]
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                const double *ic = g->icsi + 3 * id, *ie = g->ieta + 3 * id, *iz = g->izet + 3 * id;
+                const double *jc = g->jcsi + 3 * id, *je = g->jeta + 3 * id, *jz = g->jzet + 3 * id;
+                const double *kc = g->kcsi + 3 * id, *ke = g->keta + 3 * id, *kz = g->kzet + 3 * id;
+                double g11_i = (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]);
+                double g12_i = (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]);
+                double g13_i = (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]);
+                double g21_j = (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]);
+                double g22_j = (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]);
+                double g23_j = (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]);
+                double g31_k = (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]);
+                double g32_k = (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]);
+                double g33_k = (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
+                double r1 = 1.0, r2 = 1.0, r3 = 1.0;
+                double dpdc, dpde, dpdz;
+                const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
+
+                if ((i == 0 || i == mx - 2) && ip) dpdc = PP(k, j, 1) - PP(k, j, i);
+                else dpdc = PP(k, j, i + 1) - PP(k, j, i);
+                if (IB1(NV(k, j + 1, i)) || IB1(NV(k, j + 1, i + 1)) || (j == my - 2 && !jp))
+                    dpde = (PP(k, j, i) - PP(k, j - 1, i) + PP(k, j, i + 1) - PP(k, j - 1, i + 1)) * 0.5;
+                else if (IB1(NV(k, j - 1, i)) || IB1(NV(k, j - 1, i + 1)) || (j == 1 && !jp))
+                    dpde = (PP(k, j + 1, i) - PP(k, j, i) + PP(k, j + 1, i + 1) - PP(k, j, i + 1)) * 0.5;
+                else dpde = (PP(k, j + 1, i) - PP(k, j - 1, i) + PP(k, j + 1, i + 1) - PP(k, j - 1, i + 1)) * 0.25;
+                if (IB1(NV(k + 1, j, i)) || IB1(NV(k + 1, j, i + 1)) || (k == mz - 2 && !kp))
+                    dpdz = (PP(k, j, i) - PP(k - 1, j, i) + PP(k, j, i + 1) - PP(k - 1, j, i + 1)) * 0.5;
+                else if (IB1(NV(k - 1, j, i)) || IB1(NV(k - 1, j, i + 1)) || (k == 1 && !kp))
+                    dpdz = (PP(k + 1, j, i) - PP(k, j, i) + PP(k + 1, j, i + 1) - PP(k, j, i + 1)) * 0.5;
+                else dpdz = (PP(k + 1, j, i) - PP(k - 1, j, i) + PP(k + 1, j, i + 1) - PP(k - 1, j, i + 1)) * 0.25;
+                VX(dp, id) = (dpdc * g11_i + dpde * g12_i + dpdz * g13_i) * g->iaj[id] / r1;
+
+                if (IB1(NV(k, j, i + 1)) || IB1(NV(k, j + 1, i + 1)) || (i == mx - 2 && !ip))
+                    dpdc = (PP(k, j, i) - PP(k, j, i - 1) + PP(k, j + 1, i) - PP(k, j + 1, i - 1)) * 0.5;
+                else if (IB1(NV(k, j, i - 1)) || IB1(NV(k, j + 1, i - 1)) || (i == 1 && !ip))
+                    dpdc = (PP(k, j, i + 1) - PP(k, j, i) + PP(k, j + 1, i + 1) - PP(k, j + 1, i)) * 0.5;
+                else dpdc = (PP(k, j, i + 1) - PP(k, j, i - 1) + PP(k, j + 1, i + 1) - PP(k, j + 1, i - 1)) * 0.25;
+                if ((j == 0 || j == my - 2) && jp) dpde = PP(k, 1, i) - PP(k, j, i);
+                else dpde = PP(k, j + 1, i) - PP(k, j, i);
+                if (IB1(NV(k + 1, j, i)) || IB1(NV(k + 1, j + 1, i)) || (k == mz - 2 && !kp))
+                    dpdz = (PP(k, j, i) - PP(k - 1, j, i) + PP(k, j + 1, i) - PP(k - 1, j + 1, i)) * 0.5;
+                else if (IB1(NV(k - 1, j, i)) || IB1(NV(k - 1, j + 1, i)) || (k == 1 && !kp))
+                    dpdz = (PP(k + 1, j, i) - PP(k, j, i) + PP(k + 1, j + 1, i) - PP(k, j + 1, i)) * 0.5;
+                else dpdz = (PP(k + 1, j, i) - PP(k - 1, j, i) + PP(k + 1, j + 1, i) - PP(k - 1, j + 1, i)) * 0.25;
+                VY(dp, id) = (dpdc * g21_j + dpde * g22_j + dpdz * g23_j) * g->jaj[id] / r2;
+
+                if (IB1(NV(k, j, i + 1)) || IB1(NV(k + 1, j, i + 1)) || (i == mx - 2 && !ip))
+                    dpdc = (PP(k, j, i) - PP(k, j, i - 1) + PP(k + 1, j, i) - PP(k + 1, j, i - 1)) * 0.5;
+                else if (IB1(NV(k, j, i - 1)) || IB1(NV(k + 1, j, i - 1)) || (i == 1 && !ip))
+                    dpdc = (PP(k, j, i + 1) - PP(k, j, i) + PP(k + 1, j, i + 1) - PP(k + 1, j, i)) * 0.5;
+                else dpdc = (PP(k, j, i + 1) - PP(k, j, i - 1) + PP(k + 1, j, i + 1) - PP(k + 1, j, i - 1)) * 0.25;
+                if (IB1(NV(k, j + 1, i)) || IB1(NV(k + 1, j + 1, i)) || (j == my - 2 && !jp))
+                    dpde = (PP(k, j, i) - PP(k, j - 1, i) + PP(k + 1, j, i) - PP(k + 1, j - 1, i)) * 0.5;
+                else if (IB1(NV(k, j - 1, i)) || IB1(NV(k + 1, j - 1, i)) || (j == 1 && !jp))
+                    dpde = (PP(k, j + 1, i) - PP(k, j, i) + PP(k + 1, j + 1, i) - PP(k + 1, j, i)) * 0.5;
+                else dpde = (PP(k, j + 1, i) - PP(k, j - 1, i) + PP(k + 1, j + 1, i) - PP(k + 1, j - 1, i)) * 0.25;
+                if ((k == 0 || k == mz - 2) && kp) dpdz = PP(1, j, i) - PP(k, j, i);
+                else dpdz = (PP(k + 1, j, i) - PP(k, j, i));
+                if (kp) {
+                    double dzz = 1. / g->kaj[id] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
+                    dpdz += c->mean_pressure_gradient * dzz;
+                }
+                VZ(dp, id) = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * g->kaj[id] / r3;
+
+                if (i == 0 || NV(k, j, i) + NV(k, j, i + 1) > 0.1 || (!ip && i == mx - 2)) VX(dp, id) = 0;
+                if (j == 0 || NV(k, j, i) + NV(k, j + 1, i) > 0.1 || (!jp && j == my - 2)) VY(dp, id) = 0;
+                if (k == 0 || NV(k, j, i) + NV(k + 1, j, i) > 0.1 || (!kp && k == mz - 2)) VZ(dp, id) = 0;
+            }
+#undef PP
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* momentum.c:1982-2195 IB_BC                                                */
+/* ------------------------------------------------------------------------ */
+int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert, double *ucont, double *ucat)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    per_copy_vec(c, ucat); /* momentum.c:2050-2075 */
+    if (c->immersed) {
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) {
+                    size_t id = IX(k, j, i);
+                    double f = 1.0, ucx, ucy, ucz;
+                    const double *ic = g->icsi + 3 * id, *je = g->jeta + 3 * id, *kz = g->kzet + 3 * id;
+                    if ((int)nvert[id] == 1) {
+                        ucx = (U3(k, j, i, 0) + U3(k, j, i + 1, 0)) * 0.5;
+                        ucy = (U3(k, j, i, 1) + U3(k, j, i + 1, 1)) * 0.5;
+                        ucz = (U3(k, j, i, 2) + U3(k, j, i + 1, 2)) * 0.5;
+                        VX(ucont, id) = (ucx * ic[0] + ucy * ic[1] + ucz * ic[2]) * f;
+                        ucx = (U3(k, j + 1, i, 0) + U3(k, j, i, 0)) * 0.5;
+                        ucy = (U3(k, j + 1, i, 1) + U3(k, j, i, 1)) * 0.5;
+                        ucz = (U3(k, j + 1, i, 2) + U3(k, j, i, 2)) * 0.5;
+                        VY(ucont, id) = (ucx * je[0] + ucy * je[1] + ucz * je[2]) * f;
+                        ucx = (U3(k + 1, j, i, 0) + U3(k, j, i, 0)) * 0.5;
+                        ucy = (U3(k + 1, j, i, 1) + U3(k, j, i, 1)) * 0.5;
+                        ucz = (U3(k + 1, j, i, 2) + U3(k, j, i, 2)) * 0.5;
+                        VZ(ucont, id) = (ucx * kz[0] + ucy * kz[1] + ucz * kz[2]) * f;
+                    }
+                    if ((int)(NV(k, j, i + 1)) == 1) {
+                        ucx = (U3(k, j, i, 0) + U3(k, j, i + 1, 0)) * 0.5;
+                        ucy = (U3(k, j, i, 1) + U3(k, j, i + 1, 1)) * 0.5;
+                        ucz = (U3(k, j, i, 2) + U3(k, j, i + 1, 2)) * 0.5;
+                        VX(ucont, id) = (ucx * ic[0] + ucy * ic[1] + ucz * ic[2]) * f;
+                    }
+                    if ((int)(NV(k, j + 1, i)) == 1) {
+                        ucx = (U3(k, j + 1, i, 0) + U3(k, j, i, 0)) * 0.5;
+                        ucy = (U3(k, j + 1, i, 1) + U3(k, j, i, 1)) * 0.5;
+                        ucz = (U3(k, j + 1, i, 2) + U3(k, j, i, 2)) * 0.5;
+                        VY(ucont, id) = (ucx * je[0] + ucy * je[1] + ucz * je[2]) * f;
+                    }
+                    if ((int)(NV(k + 1, j, i)) == 1) {
+                        ucx = (U3(k + 1, j, i, 0) + U3(k, j, i, 0)) * 0.5;
+                        ucy = (U3(k + 1, j, i, 1) + U3(k, j, i, 1)) * 0.5;
+                        ucz = (U3(k + 1, j, i, 2) + U3(k, j, i, 2)) * 0.5;
+                        VZ(ucont, id) = (ucx * kz[0] + ucy * kz[1] + ucz * kz[2]) * f;
+                    }
+                }
+    }
+    /* periodic face copies, momentum.c:2167-2181 */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                if (c->i_periodic && i == 0) VX(ucont, IX(k, j, 0)) = VX(ucont, IX(k, j, mx - 2));
+                if (c->i_periodic && i == mx - 1) VX(ucont, IX(k, j, mx - 1)) = VX(ucont, IX(k, j, 1));
+                if (c->j_periodic && j == 0) VY(ucont, IX(k, 0, i)) = VY(ucont, IX(k, my - 2, i));
+                if (c->j_periodic && j == my - 1) VY(ucont, IX(k, my - 1, i)) = VY(ucont, IX(k, 1, i));
+                if (c->k_periodic && k == 0) VZ(ucont, IX(0, j, i)) = VZ(ucont, IX(mz - 2, j, i));
+                if (c->k_periodic && k == mz - 1) VZ(ucont, IX(mz - 1, j, i)) = VZ(ucont, IX(1, j, i));
+            }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* momentum.c:456-1979 Formfunction_2                                        */
+/* ------------------------------------------------------------------------ */
+int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                     double *ucont, const double *ucat, const double *nu_t,
+                     double *rhs, double scale)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
+    size_t P = (size_t)mx * my * mz;
+    double *buf = (double *)calloc(21 * P, sizeof(double));
+    if (!buf) return -1;
+    double *div1 = buf, *div2 = buf + 3 * P, *div3 = buf + 6 * P;
+    double *visc1 = buf + 9 * P, *visc2 = buf + 12 * P, *visc3 = buf + 15 * P, *fp = buf + 18 * P;
+
+    per_copy_vec(c, ucont); /* momentum.c:642-669 */
+
+    for (int dir = 0; dir < 3; dir++) {
+        const double *fc = dir == 0 ? g->icsi : dir == 1 ? g->jcsi : g->kcsi;
+        const double *fe = dir == 0 ? g->ieta : dir == 1 ? g->jeta : g->keta;
+        const double *fz = dir == 0 ? g->izet : dir == 1 ? g->jzet : g->kzet;
+        const double *faj = dir == 0 ? g->iaj : dir == 1 ? g->jaj : g->kaj;
+        double *dv = dir == 0 ? div1 : dir == 1 ? div2 : div3;
+        double *vs = dir == 0 ? visc1 : dir == 1 ? visc2 : visc3;
+        /* face loops: i-faces momentum.c:672-931, j 934-1166, k 1169-1409 */
+        for (int k = (dir == 2 ? 0 : 1); k < mz - 1; k++)
+            for (int j = (dir == 1 ? 0 : 1); j < my - 1; j++)
+                for (int i = (dir == 0 ? 0 : 1); i < mx - 1; i++) {
+                    size_t id = IX(k, j, i);
+                    double ajc = faj[id];
+                    double csi0 = fc[3 * id], csi1 = fc[3 * id + 1], csi2 = fc[3 * id + 2];
+                    double eta0 = fe[3 * id], eta1 = fe[3 * id + 1], eta2 = fe[3 * id + 2];
+                    double zet0 = fz[3 * id], zet1 = fz[3 * id + 1], zet2 = fz[3 * id + 2];
+                    double dc[3], de[3], dz[3];
+                    if (dir == 0) du_i(c, i, j, k, ucat, nvert, dc, de, dz);
+                    else if (dir == 1) du_j(c, i, j, k, ucat, nvert, dc, de, dz);
+                    else du_k(c, i, j, k, ucat, nvert, dc, de, dz);
+                    double g11, g21, g31, nrm0, nrm1, nrm2;
+                    if (dir == 0) {
+                        g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2;
+                        g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
+                        g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
+                        nrm0 = csi0, nrm1 = csi1, nrm2 = csi2;
+                    } else if (dir == 1) {
+                        g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2;
+                        g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
+                        g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
+                        nrm0 = eta0, nrm1 = eta1, nrm2 = eta2;
+                    } else {
+                        g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2;
+                        g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
+                        g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
+                        nrm0 = zet0, nrm1 = zet1, nrm2 = zet2;
+                    }
+                    double r11 = dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0;
+                    double r21 = dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0;
+                    double r31 = dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0;
+                    double r12 = dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1;
+                    double r22 = dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1;
+                    double r32 = dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1;
+                    double r13 = dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2;
+                    double r23 = dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2;
+                    double r33 = dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2;
+
+                    /* convective scheme selection */
+                    int n = dir == 0 ? i : dir == 1 ? j : k;
+                    int m = dir == 0 ? mx : dir == 1 ? my : mz;
+                    int per = dir == 0 ? ip : dir == 1 ? jp : kp;
+                    int L = n - 1, R = n + 2;
+#define NB(t) (dir == 0 ? NV(k, j, t) : dir == 1 ? NV(k, t, i) : NV(t, j, i))
+#define UB(t, q) (dir == 0 ? U3(k, j, t, q) : dir == 1 ? U3(k, t, i, q) : U3(t, j, i, q))
+#define UC(t) (dir == 0 ? ucont[3 * IX(k, j, t) + 0] : dir == 1 ? ucont[3 * IX(k, t, i) + 1] : ucont[3 * IX(t, j, i) + 2])
+                    if (n == 0 || n == m - 2) {
+                        if (per) L = m - 3, R = 2;
+                        else L = n, R = n + 1;
+                    } else if (NB(L) + NB(R) > 0.1) L = n, R = n + 1;
+                    if (c->second_order) L = n, R = n + 1;
+                    double ucon_f = UC(n);
+                    double ucon = ucon_f;
+                    if (per && (n == 0 || n == m - 2)) ucon = UC(m - 2);
+                    double up = -0.5 * (ucon + fabs(ucon));
+                    double um = -0.5 * (ucon - fabs(ucon));
+                    double d3v[3];
+                    if (c->immersed && n != m - 2 && NB(n) > 0.1) {
+                        for (int q = 0; q < 3; q++)
+                            d3v[q] = um * (0.125 * (-UB(n + 2, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
+                                     up * (0.125 * (-UB(n, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+                    } else if (c->immersed && n != 0 && NB(n + 1) > 0.1) {
+                        for (int q = 0; q < 3; q++)
+                            d3v[q] = um * (0.125 * (-UB(n + 1, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
+                                     up * (0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+                    } else if (c->second_order) {
+                        for (int q = 0; q < 3; q++) d3v[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
+                    } else {
+                        for (int q = 0; q < 3; q++)
+                            d3v[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
+                    }
+                    if (NB(n) + NB(n + 1) > 0.1) {
+                        if (!c->immersed) d3v[0] = d3v[1] = d3v[2] = 0;
+                    }
+                    dv[3 * id] = d3v[0]; dv[3 * id + 1] = d3v[1]; dv[3 * id + 2] = d3v[2];
+
+                    double nu = 1. / c->ren, nut = 0;
+                    double v0, v1, v2;
+                    if (c->les) {
+                        if ((n == 0 && !per) || NB(n) > 0.1) nut = dir == 0 ? nu_t[IX(k, j, i + 1)] : dir == 1 ? nu_t[IX(k, j + 1, i)] : nu_t[IX(k + 1, j, i)];
+                        else if ((n == m - 2 && !per) || NB(n + 1) > 0.1) nut = nu_t[id];
+                        else nut = 0.5 * (nu_t[id] + (dir == 0 ? nu_t[IX(k, j, i + 1)] : dir == 1 ? nu_t[IX(k, j + 1, i)] : nu_t[IX(k + 1, j, i)]));
+                        v0 = (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * nrm0 + r21 * nrm1 + r31 * nrm2) * ajc * (nut);
+                        v1 = (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * nrm0 + r22 * nrm1 + r32 * nrm2) * ajc * (nut);
+                        v2 = (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * nrm0 + r23 * nrm1 + r33 * nrm2) * ajc * (nut);
+                    } else {
+                        v0 = v1 = v2 = 0;
+                    }
+                    v0 += (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * nrm0 + r21 * nrm1 + r31 * nrm2) * ajc * (nu);
+                    v1 += (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * nrm0 + r22 * nrm1 + r32 * nrm2) * ajc * (nu);
+                    v2 += (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * nrm0 + r23 * nrm1 + r33 * nrm2) * ajc * (nu);
+                    vs[3 * id] = v0; vs[3 * id + 1] = v1; vs[3 * id + 2] = v2;
+#undef NB
+#undef UB
+#undef UC
+                }
+    }
+    /* periodic copies of the face arrays, momentum.c:1463-1504 */
+    per_copy_vec(c, div1); per_copy_vec(c, div2); per_copy_vec(c, div3);
+    per_copy_vec(c, visc1); per_copy_vec(c, visc2); per_copy_vec(c, visc3);
+
+    /* cell divergence with 4th-order deferred correction, momentum.c:1506-1636 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                size_t im = IX(k, j, i - 1), jm = IX(k, j - 1, i), km = IX(k - 1, j, i);
+                double dv[3], d4[3] = {0, 0, 0}, f[3];
+                for (int q = 0; q < 3; q++)
+                    dv[q] = (div1[3 * id + q] - div1[3 * im + q] + div2[3 * id + q] - div2[3 * jm + q] +
+                             div3[3 * id + q] - div3[3 * km + q]);
+                int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
+                double den_i = 3., den_j = 3., den_k = 3.;
+                if (i == 1) { if (ip) iL = mx - 3; else iR = i, iL = i - 1, den_i = 1.; }
+                else if (i == 2 || i == mx - 3) { if (!ip) iR = i, iL = i - 1, den_i = 1.; }
+                else if (i == mx - 2) { if (ip) iR = 1; else iR = i, iL = i - 1, den_i = 1.; }
+                if (j == 1) { if (jp) jL = my - 3; else jR = j, jL = j - 1, den_j = 1.; }
+                else if (j == 2 || j == my - 3) { if (!jp) jR = j, jL = j - 1, den_j = 1.; }
+                else if (j == my - 2) { if (jp) jR = 1; else jR = j, jL = j - 1, den_j = 1.; }
+                if (k == 1) { if (kp) kL = mz - 3; else kR = k, kL = k - 1, den_k = 1.; }
+                else if (k == 2 || k == mz - 3) { if (!kp) kR = k, kL = k - 1, den_k = 1.; }
+                else if (k == mz - 2) { if (kp) kR = 1; else kR = k, kL = k - 1, den_k = 1.; }
+                if (NV(k, j, i - 1) + NV(k, j, i) + NV(k, j, i + 1) > 0.1) iR = i, iL = i - 1, den_i = 1.;
+                if (NV(k, j - 1, i) + NV(k, j, i) + NV(k, j + 1, i) > 0.1) jR = j, jL = j - 1, den_j = 1.;
+                if (NV(k - 1, j, i) + NV(k, j, i) + NV(k + 1, j, i) > 0.1) kR = k, kL = k - 1, den_k = 1.;
+                if (c->second_order) {
+                    for (int q = 0; q < 3; q++) f[q] = dv[q];
+                } else {
+                    size_t a1 = IX(k, j, iR), b1 = IX(k, j, iL), a2 = IX(k, jR, i), b2 = IX(k, jL, i);
+                    size_t a3 = IX(kR, j, i), b3 = IX(kL, j, i);
+                    for (int q = 0; q < 3; q++) {
+                        d4[q] += (div1[3 * a1 + q] - div1[3 * b1 + q]) * (1. / den_i);
+                        d4[q] += (div2[3 * a2 + q] - div2[3 * b2 + q]) * (1. / den_j);
+                        d4[q] += (div3[3 * a3 + q] - div3[3 * b3 + q]) * (1. / den_k);
+                        f[q] = 9. / 8. * dv[q] + -1. / 8. * d4[q];
+                    }
+                }
+                for (int q = 0; q < 3; q++)
+                    f[q] += (visc1[3 * id + q] - visc1[3 * im + q] + visc2[3 * id + q] - visc2[3 * jm + q] +
+                             visc3[3 * id + q] - visc3[3 * km + q]) / 1.0;
+                fp[3 * id] = f[0]; fp[3 * id + 1] = f[1]; fp[3 * id + 2] = f[2];
+            }
+    per_copy_vec(c, fp); /* momentum.c:1643-1671 */
+
+    /* contravariant projection back onto faces, momentum.c:1673-1693 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i), a = IX(k, j, i + 1), b = IX(k, j + 1, i), cc = IX(k + 1, j, i);
+                const double *cs = g->csi, *et = g->eta, *ze = g->zet;
+                rhs[3 * id] += scale * (0.5 * (cs[3 * id] * fp[3 * id] + cs[3 * id + 1] * fp[3 * id + 1] + cs[3 * id + 2] * fp[3 * id + 2]) +
+                                        0.5 * (cs[3 * a] * fp[3 * a] + cs[3 * a + 1] * fp[3 * a + 1] + cs[3 * a + 2] * fp[3 * a + 2])) * g->iaj[id];
+                rhs[3 * id + 1] += scale * (0.5 * (et[3 * id] * fp[3 * id] + et[3 * id + 1] * fp[3 * id + 1] + et[3 * id + 2] * fp[3 * id + 2]) +
+                                            0.5 * (et[3 * b] * fp[3 * b] + et[3 * b + 1] * fp[3 * b + 1] + et[3 * b + 2] * fp[3 * b + 2])) * g->jaj[id];
+                rhs[3 * id + 2] += scale * (0.5 * (ze[3 * id] * fp[3 * id] + ze[3 * id + 1] * fp[3 * id + 1] + ze[3 * id + 2] * fp[3 * id + 2]) +
+                                            0.5 * (ze[3 * cc] * fp[3 * cc] + ze[3 * cc + 1] * fp[3 * cc + 1] + ze[3 * cc + 2] * fp[3 * cc + 2])) * g->kaj[id];
+                if (NV(k, j, i) + NV(k, j, i + 1) > 0.1 || (!ip && i == mx - 2)) rhs[3 * id] = 0;
+                if (NV(k, j, i) + NV(k, j + 1, i) > 0.1 || (!jp && j == my - 2)) rhs[3 * id + 1] = 0;
+                if (NV(k, j, i) + NV(k + 1, j, i) > 0.1 || (!kp && k == mz - 2)) rhs[3 * id + 2] = 0;
+            }
+    /* boundary planes, momentum.c:1828-1900 */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++) {
+            size_t a = IX(k, j, 0);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++) {
+            if (!ip) rhs[3 * IX(k, j, mx - 2)] = 0;
+            size_t a = IX(k, j, mx - 1);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    for (int k = 0; k < mz; k++)
+        for (int i = 0; i < mx; i++) {
+            size_t a = IX(k, 0, i);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    for (int k = 0; k < mz; k++)
+        for (int i = 0; i < mx; i++) {
+            if (!jp) rhs[3 * IX(k, my - 2, i) + 1] = 0;
+            size_t a = IX(k, my - 1, i);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+            size_t a = IX(0, j, i);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    for (int j = 0; j < my; j++)
+        for (int i = 0; i < mx; i++) {
+            if (!kp) rhs[3 * IX(mz - 2, j, i) + 2] = 0;
+            size_t a = IX(mz - 1, j, i);
+            rhs[3 * a] = rhs[3 * a + 1] = rhs[3 * a + 2] = 0;
+        }
+    free(buf);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* momentum.c:2197-2301 FormFunction_SNES (time_coeff()==1 branch)          */
+/* ------------------------------------------------------------------------ */
+int vo_formfunction_snes(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         const double *U, const double *U_o, const double *rhs_o,
+                         const double *dP, const double *nu_t, double *lucont,
+                         double *ucat, double *rhs)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    size_t P = (size_t)mx * my * mz;
+    double *Ug = (double *)malloc(3 * P * sizeof(double));
+    if (!Ug) return -1;
+    memcpy(Ug, U, 3 * P * sizeof(double));
+    /* no-slip boundary fluxes (momentum.c:2224-2232) */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                if (i == 0 && c->bctype[0] == 1) VX(Ug, IX(k, j, i)) = 0;
+                if (i == mx - 1 && c->bctype[1] == 1) VX(Ug, IX(k, j, i - 1)) = 0;
+                if (j == 0 && c->bctype[2] == 1) VY(Ug, IX(k, j, i)) = 0;
+                if (j == my - 1 && c->bctype[3] == 1) VY(Ug, IX(k, j - 1, i)) = 0;
+                if (k == 0 && c->bctype[4] == 1) VZ(Ug, IX(k, j, i)) = 0;
+                if (k == mz - 1 && c->bctype[5] == 1) VZ(Ug, IX(k - 1, j, i)) = 0;
+            }
+    memcpy(lucont, Ug, 3 * P * sizeof(double));
+    rc = vo_contra2cart(c, g, nvert, lucont, ucat);
+    if (!rc) rc = vo_ib_bc(c, g, nvert, lucont, ucat);
+    if (rc) { free(Ug); return rc; }
+    const double dt = c->dt;
+    for (size_t n = 0; n < 3 * P; n++) {
+        double r = 0;
+        r += -1. / dt * Ug[n];
+        r += 1. / dt * U_o[n];
+        rhs[n] = r;
+    }
+    rc = vo_formfunction2(c, g, nvert, lucont, ucat, nu_t, rhs, 0.5);
+    for (size_t n = 0; n < 3 * P; n++) {
+        rhs[n] += 0.5 * rhs_o[n];
+        rhs[n] += -1 * dP[n];
+    }
+    free(Ug);
+    return rc;
+}
+
+/* momentum.c:104-195 Calc_Minimum_dt */
+double vo_min_dt(const vo_cfg *c, const vo_geom *g, const double *ucont)
+{
+    DIMS;
+    double ldt = 1.e7;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                double a;
+                a = fabs(1. / VX(ucont, id) / g->aj[id]); ldt = a < ldt ? a : ldt;
+                a = fabs(1. / VY(ucont, id) / g->aj[id]); ldt = a < ldt ? a : ldt;
+                a = fabs(1. / VZ(ucont, id) / g->aj[id]); ldt = a < ldt ? a : ldt;
+            }
+    return ldt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Pressure projection                                                       */
+/* ------------------------------------------------------------------------ */
+#define POISSON_THRESHOLD 0.1 /* poisson.c:11 */
+
+int vo_setup_gid(const vo_cfg *c, const double *nvert, double *gid)
+{
+    DIMS;
+    size_t P = (size_t)mx * my * mz;
+    for (size_t n = 0; n < P; n++) gid[n] = -1;
+    int cnt = 0;
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                if (i == 0 || i == mx - 1 || j == 0 || j == my - 1 || k == 0 || k == mz - 1) continue;
+                if ((int)NV(k, j, i) > POISSON_THRESHOLD) continue;
+                gid[IX(k, j, i)] = (double)cnt++;
+            }
+    per_copy_sca(c, gid);
+    return cnt;
+}
+
+/* coefficient slots, poisson.c:951-969 */
+enum { CP, EP, WP, NP, SP, TP, BP, NE, SE, NW, SW, TN, BN, TS, BS, TE, BE, TW, BW };
+
+/* g_ab fields with the periodic index map, poisson.c:1096-1132 */
+static double gfield(const vo_cfg *c, const vo_geom *g, int which, int k, int j, int i)
+{
+    DIMS;
+    int a, b, cc;
+    per_src(c, i, j, k, &a, &b, &cc);
+    size_t s = IX(cc, b, a);
+    const double *x, *y, *w;
+    switch (which) {
+    case 11: x = g->icsi; y = g->icsi; w = g->iaj; break;
+    case 12: x = g->ieta; y = g->icsi; w = g->iaj; break;
+    case 13: x = g->izet; y = g->icsi; w = g->iaj; break;
+    case 21: x = g->jcsi; y = g->jeta; w = g->jaj; break;
+    case 22: x = g->jeta; y = g->jeta; w = g->jaj; break;
+    case 23: x = g->jzet; y = g->jeta; w = g->jaj; break;
+    case 31: x = g->kcsi; y = g->kzet; w = g->kaj; break;
+    case 32: x = g->keta; y = g->kzet; w = g->kaj; break;
+    default: x = g->kzet; y = g->kzet; w = g->kaj; break;
+    }
+    return (x[3 * s] * y[3 * s] + x[3 * s + 1] * y[3 * s + 1] + x[3 * s + 2] * y[3 * s + 2]) * w[s];
+}
+
+/* PoissonLHSNew row body for one fluid cell, poisson.c:1160-1555 */
+static void stencil_row(const vo_cfg *c, const vo_geom *g, const double *nvert, int i, int j, int k, double vol[19])
+{
+    DIMS;
+    const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
+    const double th = POISSON_THRESHOLD, r = 1.0;
+    for (int m = 0; m < 19; m++) vol[m] = 0.;
+#define G(w, kk, jj, ii) gfield(c, g, w, kk, jj, ii)
+    /* i+1 - i */
+    if (NV(k, j, i + 1) < th && (i != mx - 2 || ip)) {
+        vol[CP] -= G(11, k, j, i) / r;
+        vol[EP] += G(11, k, j, i) / r;
+        if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j - 1, i) + NV(k, j - 1, i + 1) < th && j != 1) {
+                vol[CP] += G(12, k, j, i) * 0.5 / r; vol[EP] += G(12, k, j, i) * 0.5 / r;
+                vol[SP] -= G(12, k, j, i) * 0.5 / r; vol[SE] -= G(12, k, j, i) * 0.5 / r;
+            }
+        } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k, j - 1, i + 1) > th) {
+            if (NV(k, j + 1, i) + NV(k, j + 1, i + 1) < th) {
+                vol[NP] += G(12, k, j, i) * 0.5 / r; vol[NE] += G(12, k, j, i) * 0.5 / r;
+                vol[CP] -= G(12, k, j, i) * 0.5 / r; vol[EP] -= G(12, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[NP] += G(12, k, j, i) * 0.25 / r; vol[NE] += G(12, k, j, i) * 0.25 / r;
+            vol[SP] -= G(12, k, j, i) * 0.25 / r; vol[SE] -= G(12, k, j, i) * 0.25 / r;
+        }
+        if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j, i + 1) < th && k != 1) {
+                vol[CP] += G(13, k, j, i) * 0.5 / r; vol[EP] += G(13, k, j, i) * 0.5 / r;
+                vol[BP] -= G(13, k, j, i) * 0.5 / r; vol[BE] -= G(13, k, j, i) * 0.5 / r;
+            }
+        } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j, i + 1) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j, i + 1) < th) {
+                vol[TP] += G(13, k, j, i) * 0.5 / r; vol[TE] += G(13, k, j, i) * 0.5 / r;
+                vol[CP] -= G(13, k, j, i) * 0.5 / r; vol[EP] -= G(13, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[TP] += G(13, k, j, i) * 0.25 / r; vol[TE] += G(13, k, j, i) * 0.25 / r;
+            vol[BP] -= G(13, k, j, i) * 0.25 / r; vol[BE] -= G(13, k, j, i) * 0.25 / r;
+        }
+    }
+    /* i - i-1 */
+    if (NV(k, j, i - 1) < th && (i != 1 || ip)) {
+        vol[CP] -= G(11, k, j, i - 1) / r;
+        vol[WP] += G(11, k, j, i - 1) / r;
+        if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k, j + 1, i - 1) > th) {
+            if (NV(k, j - 1, i) + NV(k, j - 1, i - 1) < th && j != 1) {
+                vol[CP] -= G(12, k, j, i - 1) * 0.5 / r; vol[WP] -= G(12, k, j, i - 1) * 0.5 / r;
+                vol[SP] += G(12, k, j, i - 1) * 0.5 / r; vol[SW] += G(12, k, j, i - 1) * 0.5 / r;
+            }
+        } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k, j - 1, i - 1) > th) {
+            if (NV(k, j + 1, i) + NV(k, j + 1, i - 1) < th) {
+                vol[NP] -= G(12, k, j, i - 1) * 0.5 / r; vol[NW] -= G(12, k, j, i - 1) * 0.5 / r;
+                vol[CP] += G(12, k, j, i - 1) * 0.5 / r; vol[WP] += G(12, k, j, i - 1) * 0.5 / r;
+            }
+        } else {
+            vol[NP] -= G(12, k, j, i - 1) * 0.25 / r; vol[NW] -= G(12, k, j, i - 1) * 0.25 / r;
+            vol[SP] += G(12, k, j, i - 1) * 0.25 / r; vol[SW] += G(12, k, j, i - 1) * 0.25 / r;
+        }
+        if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j, i - 1) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j, i - 1) < th && k != 1) {
+                vol[CP] -= G(13, k, j, i - 1) * 0.5 / r; vol[WP] -= G(13, k, j, i - 1) * 0.5 / r;
+                vol[BP] += G(13, k, j, i - 1) * 0.5 / r; vol[BW] += G(13, k, j, i - 1) * 0.5 / r;
+            }
+        } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j, i - 1) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j, i - 1) < th) {
+                vol[TP] -= G(13, k, j, i - 1) * 0.5 / r; vol[TW] -= G(13, k, j, i - 1) * 0.5 / r;
+                vol[CP] += G(13, k, j, i - 1) * 0.5 / r; vol[WP] += G(13, k, j, i - 1) * 0.5 / r;
+            }
+        } else {
+            vol[TP] -= G(13, k, j, i - 1) * 0.25 / r; vol[TW] -= G(13, k, j, i - 1) * 0.25 / r;
+            vol[BP] += G(13, k, j, i - 1) * 0.25 / r; vol[BW] += G(13, k, j, i - 1) * 0.25 / r;
+        }
+    }
+    /* j+1 - j */
+    if (NV(k, j + 1, i) < th && (j != my - 2 || jp)) {
+        if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k, j + 1, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k, j + 1, i - 1) < th && i != 1) {
+                vol[CP] += G(21, k, j, i) * 0.5 / r; vol[NP] += G(21, k, j, i) * 0.5 / r;
+                vol[WP] -= G(21, k, j, i) * 0.5 / r; vol[NW] -= G(21, k, j, i) * 0.5 / r;
+            }
+        } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k, j + 1, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k, j + 1, i + 1) < th) {
+                vol[EP] += G(21, k, j, i) * 0.5 / r; vol[NE] += G(21, k, j, i) * 0.5 / r;
+                vol[CP] -= G(21, k, j, i) * 0.5 / r; vol[NP] -= G(21, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[EP] += G(21, k, j, i) * 0.25 / r; vol[NE] += G(21, k, j, i) * 0.25 / r;
+            vol[WP] -= G(21, k, j, i) * 0.25 / r; vol[NW] -= G(21, k, j, i) * 0.25 / r;
+        }
+        vol[CP] -= G(22, k, j, i) / r;
+        vol[NP] += G(22, k, j, i) / r;
+        if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j + 1, i) < th && k != 1) {
+                vol[CP] += G(23, k, j, i) * 0.5 / r; vol[NP] += G(23, k, j, i) * 0.5 / r;
+                vol[BP] -= G(23, k, j, i) * 0.5 / r; vol[BN] -= G(23, k, j, i) * 0.5 / r;
+            }
+        } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j + 1, i) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j + 1, i) < th) {
+                vol[TP] += G(23, k, j, i) * 0.5 / r; vol[TN] += G(23, k, j, i) * 0.5 / r;
+                vol[CP] -= G(23, k, j, i) * 0.5 / r; vol[NP] -= G(23, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[TP] += G(23, k, j, i) * 0.25 / r; vol[TN] += G(23, k, j, i) * 0.25 / r;
+            vol[BP] -= G(23, k, j, i) * 0.25 / r; vol[BN] -= G(23, k, j, i) * 0.25 / r;
+        }
+    }
+    /* j - j-1 */
+    if (NV(k, j - 1, i) < th && (j != 1 || jp)) {
+        if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k, j - 1, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k, j - 1, i - 1) < th && i != 1) {
+                vol[CP] -= G(21, k, j - 1, i) * 0.5 / r; vol[SP] -= G(21, k, j - 1, i) * 0.5 / r;
+                vol[WP] += G(21, k, j - 1, i) * 0.5 / r; vol[SW] += G(21, k, j - 1, i) * 0.5 / r;
+            }
+        } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k, j - 1, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k, j - 1, i + 1) < th) {
+                vol[EP] -= G(21, k, j - 1, i) * 0.5 / r; vol[SE] -= G(21, k, j - 1, i) * 0.5 / r;
+                vol[CP] += G(21, k, j - 1, i) * 0.5 / r; vol[SP] += G(21, k, j - 1, i) * 0.5 / r;
+            }
+        } else {
+            vol[EP] -= G(21, k, j - 1, i) * 0.25 / r; vol[SE] -= G(21, k, j - 1, i) * 0.25 / r;
+            vol[WP] += G(21, k, j - 1, i) * 0.25 / r; vol[SW] += G(21, k, j - 1, i) * 0.25 / r;
+        }
+        vol[CP] -= G(22, k, j - 1, i) / r;
+        vol[SP] += G(22, k, j - 1, i) / r;
+        if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j - 1, i) > th) {
+            if (NV(k - 1, j, i) + NV(k - 1, j - 1, i) < th && k != 1) {
+                vol[CP] -= G(23, k, j - 1, i) * 0.5 / r; vol[SP] -= G(23, k, j - 1, i) * 0.5 / r;
+                vol[BP] += G(23, k, j - 1, i) * 0.5 / r; vol[BS] += G(23, k, j - 1, i) * 0.5 / r;
+            }
+        } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j - 1, i) > th) {
+            if (NV(k + 1, j, i) + NV(k + 1, j - 1, i) < th) {
+                vol[TP] -= G(23, k, j - 1, i) * 0.5 / r; vol[TS] -= G(23, k, j - 1, i) * 0.5 / r;
+                vol[CP] += G(23, k, j - 1, i) * 0.5 / r; vol[SP] += G(23, k, j - 1, i) * 0.5 / r;
+            }
+        } else {
+            vol[TP] -= G(23, k, j - 1, i) * 0.25 / r; vol[TS] -= G(23, k, j - 1, i) * 0.25 / r;
+            vol[BP] += G(23, k, j - 1, i) * 0.25 / r; vol[BS] += G(23, k, j - 1, i) * 0.25 / r;
+        }
+    }
+    /* k+1 - k */
+    if (NV(k + 1, j, i) < th && (k != mz - 2 || kp)) {
+        if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k + 1, j, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k + 1, j, i - 1) < th && i != 1) {
+                vol[CP] += G(31, k, j, i) * 0.5 / r; vol[TP] += G(31, k, j, i) * 0.5 / r;
+                vol[WP] -= G(31, k, j, i) * 0.5 / r; vol[TW] -= G(31, k, j, i) * 0.5 / r;
+            }
+        } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k + 1, j, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k + 1, j, i + 1) < th) {
+                vol[EP] += G(31, k, j, i) * 0.5 / r; vol[TE] += G(31, k, j, i) * 0.5 / r;
+                vol[CP] -= G(31, k, j, i) * 0.5 / r; vol[TP] -= G(31, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[EP] += G(31, k, j, i) * 0.25 / r; vol[TE] += G(31, k, j, i) * 0.25 / r;
+            vol[WP] -= G(31, k, j, i) * 0.25 / r; vol[TW] -= G(31, k, j, i) * 0.25 / r;
+        }
+        if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k + 1, j + 1, i) > th) {
+            if (NV(k, j - 1, i) + NV(k + 1, j - 1, i) < th && j != 1) {
+                vol[CP] += G(32, k, j, i) * 0.5 / r; vol[TP] += G(32, k, j, i) * 0.5 / r;
+                vol[SP] -= G(32, k, j, i) * 0.5 / r; vol[TS] -= G(32, k, j, i) * 0.5 / r;
+            }
+        } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k + 1, j - 1, i) > th) {
+            if (NV(k, j + 1, i) + NV(k + 1, j + 1, i) < th) {
+                vol[NP] += G(32, k, j, i) * 0.5 / r; vol[TN] += G(32, k, j, i) * 0.5 / r;
+                vol[CP] -= G(32, k, j, i) * 0.5 / r; vol[TP] -= G(32, k, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[NP] += G(32, k, j, i) * 0.25 / r; vol[TN] += G(32, k, j, i) * 0.25 / r;
+            vol[SP] -= G(32, k, j, i) * 0.25 / r; vol[TS] -= G(32, k, j, i) * 0.25 / r;
+        }
+        vol[CP] -= G(33, k, j, i) / r;
+        vol[TP] += G(33, k, j, i) / r;
+    }
+    /* k - k-1 */
+    if (NV(k - 1, j, i) < th && (k != 1 || kp)) {
+        if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k - 1, j, i + 1) > th) {
+            if (NV(k, j, i - 1) + NV(k - 1, j, i - 1) < th && i != 1) {
+                vol[CP] -= G(31, k - 1, j, i) * 0.5 / r; vol[BP] -= G(31, k - 1, j, i) * 0.5 / r;
+                vol[WP] += G(31, k - 1, j, i) * 0.5 / r; vol[BW] += G(31, k - 1, j, i) * 0.5 / r;
+            }
+        } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k - 1, j, i - 1) > th) {
+            if (NV(k, j, i + 1) + NV(k - 1, j, i + 1) < th) {
+                vol[EP] -= G(31, k - 1, j, i) * 0.5 / r; vol[BE] -= G(31, k - 1, j, i) * 0.5 / r;
+                vol[CP] += G(31, k - 1, j, i) * 0.5 / r; vol[BP] += G(31, k - 1, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[EP] -= G(31, k - 1, j, i) * 0.25 / r; vol[BE] -= G(31, k - 1, j, i) * 0.25 / r;
+            vol[WP] += G(31, k - 1, j, i) * 0.25 / r; vol[BW] += G(31, k - 1, j, i) * 0.25 / r;
+        }
+        if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k - 1, j + 1, i) > th) {
+            if (NV(k, j - 1, i) + NV(k - 1, j - 1, i) < th && j != 1) {
+                vol[CP] -= G(32, k - 1, j, i) * 0.5 / r; vol[BP] -= G(32, k - 1, j, i) * 0.5 / r;
+                vol[SP] += G(32, k - 1, j, i) * 0.5 / r; vol[BS] += G(32, k - 1, j, i) * 0.5 / r;
+            }
+        } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k - 1, j - 1, i) > th) {
+            if (NV(k, j + 1, i) + NV(k - 1, j + 1, i) < th) {
+                vol[NP] -= G(32, k - 1, j, i) * 0.5 / r; vol[BN] -= G(32, k - 1, j, i) * 0.5 / r;
+                vol[CP] += G(32, k - 1, j, i) * 0.5 / r; vol[BP] += G(32, k - 1, j, i) * 0.5 / r;
+            }
+        } else {
+            vol[NP] -= G(32, k - 1, j, i) * 0.25 / r; vol[BN] -= G(32, k - 1, j, i) * 0.25 / r;
+            vol[SP] += G(32, k - 1, j, i) * 0.25 / r; vol[BS] += G(32, k - 1, j, i) * 0.25 / r;
+        }
+        vol[CP] -= G(33, k - 1, j, i) / r;
+        vol[BP] += G(33, k - 1, j, i) / r;
+    }
+#undef G
+}
+
+/* neighbour offsets of the 19 slots (di, dj, dk) */
+static const int SLOT[19][3] = {
+    {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}, {0, 1, 1}, {0, 1, -1}, {0, -1, 1},
+    {0, -1, -1}, {1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+
+int vo_poisson_stencil(const vo_cfg *c, const vo_geom *g, const double *nvert, double *coef)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    size_t P = (size_t)mx * my * mz;
+    memset(coef, 0, 19 * P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (NV(k, j, i) > POISSON_THRESHOLD) continue;
+                stencil_row(c, g, nvert, i, j, k, coef + 19 * id);
+            }
+    return 0;
+}
+
+int vo_poisson_lhs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   const double *gid, int *rowptr, int *col, double *val)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    int nrow = 0, nnz = 0;
+    rowptr[0] = 0;
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int row = (int)gid[IX(k, j, i)];
+                if (i == 0 || i == mx - 1 || j == 0 || j == my - 1 || k == 0 || k == mz - 1) continue;
+                double vol[19];
+                int idx[19];
+                int start = nnz;
+                if (NV(k, j, i) > 0.1 && row >= 0) {
+                    col[nnz] = row; val[nnz] = 1.0; nnz++;
+                } else if (NV(k, j, i) > POISSON_THRESHOLD) {
+                    continue;
+                } else {
+                    stencil_row(c, g, nvert, i, j, k, vol);
+                    for (int m = 0; m < 19; m++)
+                        idx[m] = (int)gid[IX(k + SLOT[m][2], j + SLOT[m][1], i + SLOT[m][0])];
+                    /* drop rule, poisson.c:1604-1608; INSERT_VALUES replaces duplicates */
+                    for (int m = 0; m < 19; m++) {
+                        if ((fabs(vol[m]) > 1.e-10 && idx[m] >= 0) || m == CP) {
+                            int found = -1;
+                            for (int q = start; q < nnz; q++) if (col[q] == idx[m]) found = q;
+                            if (found >= 0) val[found] = vol[m];
+                            else { col[nnz] = idx[m]; val[nnz] = vol[m]; nnz++; }
+                        }
+                    }
+                }
+                if (row != nrow) return -6; /* rows are emitted in gid order */
+                nrow++;
+                rowptr[nrow] = nnz;
+            }
+    return nnz;
+}
+
+int vo_poisson_rhs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   const double *ucont, double *b)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
+    size_t P = (size_t)mx * my * mz;
+    memset(b, 0, P * sizeof(double));
+    const double coeff = 1.0; /* time_coeff(), poisson.c:17-25 */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (NV(k, j, i) >= 0.1) continue;
+                double v = 0;
+                v -= VX(ucont, id);
+                if (i == 1 && ip) v += VX(ucont, IX(k, j, mx - 2));
+                else v += VX(ucont, IX(k, j, i - 1));
+                v -= VY(ucont, id);
+                if (j == 1 && jp) v += VY(ucont, IX(k, my - 2, i));
+                else v += VY(ucont, IX(k, j - 1, i));
+                v -= VZ(ucont, id);
+                if (k == 1 && kp) v += VZ(ucont, IX(mz - 2, j, i));
+                else v += VZ(ucont, IX(k - 1, j, i));
+                v *= -1.0 / c->dt * c->st * coeff;
+                b[id] = v;
+            }
+    /* Remove_Nullspace_Scale, poisson_hypre.c:110-192 */
+    double sum = 0, sum_aj = 0;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (NV(k, j, i) < 0.1) { sum += b[id]; sum_aj += g->aj[id]; }
+            }
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (NV(k, j, i) < 0.1) b[id] += -sum * g->aj[id] / sum_aj;
+            }
+    return 0;
+}
+
+/* poisson.c:3570-3910 VolumeFlux, flg=0 */
+int vo_volume_flux(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   double *ucont, double *flux_out, double *area_out)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const double eps = 1.e-8, ibmval = 1.1;
+    double flux = 0, area = 0;
+#define AREA(v, id) sqrt(VX(v, id) * VX(v, id) + VY(v, id) * VY(v, id) + VZ(v, id) * VZ(v, id))
+    for (int pass = 0; pass < 3; pass++) {
+        double corr = 0;
+        if (pass == 1) corr = area > 1.e-15 ? flux / area : 0;
+        if (pass != 1) flux = 0, area = 0;
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) {
+                    size_t id = IX(k, j, i);
+                    const double *ar[3] = {g->csi, g->eta, g->zet};
+                    double nb[3] = {NV(k, j, i + 1), NV(k, j + 1, i), NV(k + 1, j, i)};
+                    int inb[3] = {i < mx - 2, j < my - 2, k < mz - 2};
+                    for (int q = 0; q < 3; q++) {
+                        double *u = &ucont[3 * id + q];
+                        double a = AREA(ar[q], id);
+                        /* fluid cell, IB neighbour: outflow into the body */
+                        if (NV(k, j, i) < 0.1 && nb[q] > 0.1 && nb[q] < ibmval && inb[q]) {
+                            if (pass == 0) {
+                                if (fabs(*u) > eps) { flux += *u; area += a; }
+                                else *u = 0.;
+                            } else if (pass == 1) {
+                                if (fabs(*u) > eps) *u -= a * corr;
+                            } else { flux += *u; area += a; }
+                        }
+                    }
+                    for (int q = 0; q < 3; q++) {
+                        double *u = &ucont[3 * id + q];
+                        double a = AREA(ar[q], id);
+                        if (NV(k, j, i) > 0.1 && NV(k, j, i) < ibmval && nb[q] < 0.1 && inb[q]) {
+                            if (pass == 0) {
+                                if (fabs(*u) > eps) { flux -= *u; area += a; }
+                                else *u = 0.;
+                            } else if (pass == 1) {
+                                if (fabs(*u) > eps) *u += a * corr;
+                            } else { flux -= *u; area += a; }
+                        }
+                    }
+                }
+    }
+#undef AREA
+    if (flux_out) *flux_out = flux;
+    if (area_out) *area_out = area;
+    return 0;
+}
+
+int vo_csr_apply(const vo_cfg *c, const double *gid, const int *rowptr,
+                 const int *col, const double *val, const double *x, double *y)
+{
+    DIMS;
+    size_t P = (size_t)mx * my * mz;
+    int n = 0;
+    for (size_t q = 0; q < P; q++) if (gid[q] >= 0) n = (int)gid[q] + 1 > n ? (int)gid[q] + 1 : n;
+    double *xc = (double *)malloc((size_t)(n + 1) * sizeof(double));
+    if (!xc) return -1;
+    /* compress (interior cells only; periodic ghosts share the gid) */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                int r = (int)gid[IX(k, j, i)];
+                if (r >= 0) xc[r] = x[IX(k, j, i)];
+            }
+    memset(y, 0, P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                int r = (int)gid[IX(k, j, i)];
+                if (r < 0) continue;
+                double s = 0;
+                for (int q = rowptr[r]; q < rowptr[r + 1]; q++) s += val[q] * xc[col[q]];
+                y[IX(k, j, i)] = s;
+            }
+    free(xc);
+    return 0;
+}
+
+/* GMRES(m) with Jacobi right preconditioning, modified Gram-Schmidt */
+static double dot(const double *a, const double *b, int n)
+{
+    double s = 0;
+    for (int q = 0; q < n; q++) s += a[q] * b[q];
+    return s;
+}
+
+static void spmv(int n, const int *rp, const int *col, const double *val, const double *x, double *y)
+{
+    for (int r = 0; r < n; r++) {
+        double s = 0;
+        for (int q = rp[r]; q < rp[r + 1]; q++) s += val[q] * x[col[q]];
+        y[r] = s;
+    }
+}
+
+int vo_poisson_solve(const vo_cfg *c, const double *nvert, const double *gid,
+                     const int *rowptr, const int *col, const double *val,
+                     const double *b, double rtol, int maxit, double *phi, double *relres)
+{
+    DIMS;
+    size_t P = (size_t)mx * my * mz;
+    int n = 0;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) if (gid[IX(k, j, i)] >= 0) n++;
+    const int M = 80;
+    double *bc = calloc(n, sizeof(double)), *x = calloc(n, sizeof(double)), *dinv = calloc(n, sizeof(double));
+    double *V = calloc((size_t)(M + 1) * n, sizeof(double)), *w = calloc(n, sizeof(double)), *z = calloc(n, sizeof(double));
+    double *H = calloc((size_t)(M + 1) * M, sizeof(double)), *cs = calloc(M, sizeof(double));
+    double *sn = calloc(M, sizeof(double)), *s = calloc(M + 1, sizeof(double)), *yv = calloc(M, sizeof(double));
+    if (!bc || !x || !dinv || !V || !w || !z || !H || !cs || !sn || !s || !yv) return -1;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                int r = (int)gid[IX(k, j, i)];
+                if (r >= 0) bc[r] = b[IX(k, j, i)];
+            }
+    for (int r = 0; r < n; r++) {
+        double d = 1;
+        for (int q = rowptr[r]; q < rowptr[r + 1]; q++) if (col[q] == r) d = val[q];
+        dinv[r] = d != 0 ? 1. / d : 1.;
+    }
+    double bn = sqrt(dot(bc, bc, n));
+    if (bn == 0) bn = 1;
+    int it = 0;
+    double res = 0;
+    while (it < maxit) {
+        spmv(n, rowptr, col, val, x, w);
+        for (int r = 0; r < n; r++) V[r] = bc[r] - w[r];
+        double beta = sqrt(dot(V, V, n));
+        res = beta / bn;
+        if (res <= rtol) break;
+        for (int r = 0; r < n; r++) V[r] /= beta;
+        for (int q = 0; q <= M; q++) s[q] = 0;
+        s[0] = beta;
+        int kk;
+        for (kk = 0; kk < M && it < maxit; kk++, it++) {
+            for (int r = 0; r < n; r++) z[r] = dinv[r] * V[(size_t)kk * n + r];
+            spmv(n, rowptr, col, val, z, w);
+            for (int q = 0; q <= kk; q++) {
+                double h = dot(w, V + (size_t)q * n, n);
+                H[q * M + kk] = h;
+                for (int r = 0; r < n; r++) w[r] -= h * V[(size_t)q * n + r];
+            }
+            double hn = sqrt(dot(w, w, n));
+            H[(kk + 1) * M + kk] = hn;
+            if (hn > 0) for (int r = 0; r < n; r++) V[(size_t)(kk + 1) * n + r] = w[r] / hn;
+            for (int q = 0; q < kk; q++) {
+                double t = cs[q] * H[q * M + kk] + sn[q] * H[(q + 1) * M + kk];
+                H[(q + 1) * M + kk] = -sn[q] * H[q * M + kk] + cs[q] * H[(q + 1) * M + kk];
+                H[q * M + kk] = t;
+            }
+            double a = H[kk * M + kk], bb = H[(kk + 1) * M + kk];
+            double den = sqrt(a * a + bb * bb);
+            cs[kk] = den > 0 ? a / den : 1; sn[kk] = den > 0 ? bb / den : 0;
+            H[kk * M + kk] = den; H[(kk + 1) * M + kk] = 0;
+            s[kk + 1] = -sn[kk] * s[kk]; s[kk] = cs[kk] * s[kk];
+            if (fabs(s[kk + 1]) / bn <= rtol * 0.5) { kk++; it++; break; }
+        }
+        for (int q = kk - 1; q >= 0; q--) {
+            double t = s[q];
+            for (int p = q + 1; p < kk; p++) t -= H[q * M + p] * yv[p];
+            yv[q] = t / H[q * M + q];
+        }
+        for (int r = 0; r < n; r++) {
+            double t = 0;
+            for (int q = 0; q < kk; q++) t += V[(size_t)q * n + r] * yv[q];
+            x[r] += dinv[r] * t;
+        }
+    }
+    spmv(n, rowptr, col, val, x, w);
+    double rn = 0;
+    for (int r = 0; r < n; r++) rn += (bc[r] - w[r]) * (bc[r] - w[r]);
+    res = sqrt(rn) / bn;
+    if (relres) *relres = res;
+    /* Remove_Nullspace: plain mean over nvert<0.1 cells, poisson_hypre.c:194-255 */
+    double sum = 0;
+    int cnt = 0;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                if (NV(k, j, i) < 0.1) { sum += x[(int)gid[IX(k, j, i)]]; cnt++; }
+    double v = -sum / (double)cnt;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                if (NV(k, j, i) < 0.1) x[(int)gid[IX(k, j, i)]] += v;
+    /* Convert_Phi2_Phi, poisson.c:92-180 */
+    memset(phi, 0, P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                if ((int)NV(k, j, i) > POISSON_THRESHOLD) phi[IX(k, j, i)] = 0;
+                else phi[IX(k, j, i)] = x[(int)gid[IX(k, j, i)]];
+            }
+    per_copy_sca(c, phi);
+    free(bc); free(x); free(dinv); free(V); free(w); free(z); free(H); free(cs); free(sn); free(s); free(yv);
+    return it;
+}
+
+/* poisson.c:3144-3304 UpdatePressure */
+int vo_update_pressure(const vo_cfg *c, const double *nvert, double *p, double *phi)
+{
+    DIMS;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (nvert[id] < 0.1) p[id] += phi[id];
+                if (nvert[id] > 1.1) p[id] = 0;
+            }
+    per_copy_sca(c, p);
+    per_copy_sca(c, phi);
+    return 0;
+}
+
+/* poisson.c:2707-3073 Projection */
+int vo_projection(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                  const double *phi, double *ucont)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
+    const double th = POISSON_THRESHOLD, coeff = 1.0, r1 = 1.0, r2 = 1.0, r3 = 1.0;
+#define PH(k, j, i) phi[IX(k, j, i)]
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                double dpdc, dpde, dpdz;
+                if (i < mx - 2 || (ip && i == mx - 2)) {
+                    dpdc = PH(k, j, i + 1) - PH(k, j, i);
+                    if (i == mx - 2 && ip) dpdc = PH(k, j, 1) - PH(k, j, i);
+                    dpde = 0.; dpdz = 0.;
+                    if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k, j + 1, i + 1) > th) {
+                        if (NV(k, j - 1, i) + NV(k, j - 1, i + 1) < th && (j != 1))
+                            dpde = (PH(k, j, i) + PH(k, j, i + 1) - PH(k, j - 1, i) - PH(k, j - 1, i + 1)) * 0.5;
+                    } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k, j - 1, i + 1) > th) {
+                        if (NV(k, j + 1, i) + NV(k, j + 1, i + 1) < th)
+                            dpde = (PH(k, j + 1, i) + PH(k, j + 1, i + 1) - PH(k, j, i) - PH(k, j, i + 1)) * 0.5;
+                    } else dpde = (PH(k, j + 1, i) + PH(k, j + 1, i + 1) - PH(k, j - 1, i) - PH(k, j - 1, i + 1)) * 0.25;
+                    if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j, i + 1) > th) {
+                        if (NV(k - 1, j, i) + NV(k - 1, j, i + 1) < th && (k != 1))
+                            dpdz = (PH(k, j, i) + PH(k, j, i + 1) - PH(k - 1, j, i) - PH(k - 1, j, i + 1)) * 0.5;
+                    } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j, i + 1) > th) {
+                        if (NV(k + 1, j, i) + NV(k + 1, j, i + 1) < th)
+                            dpdz = (PH(k + 1, j, i) + PH(k + 1, j, i + 1) - PH(k, j, i) - PH(k, j, i + 1)) * 0.5;
+                    } else dpdz = (PH(k + 1, j, i) + PH(k + 1, j, i + 1) - PH(k - 1, j, i) - PH(k - 1, j, i + 1)) * 0.25;
+                    if ((NV(k, j, i) + NV(k, j, i + 1)) < th) {
+                        const double *ic = g->icsi + 3 * id, *ie = g->ieta + 3 * id, *iz = g->izet + 3 * id;
+                        double a = g->iaj[id];
+                        ucont[3 * id] -= (dpdc * (ic[0] * ic[0] + ic[1] * ic[1] + ic[2] * ic[2]) * a +
+                                          dpde * (ie[0] * ic[0] + ie[1] * ic[1] + ie[2] * ic[2]) * a +
+                                          dpdz * (iz[0] * ic[0] + iz[1] * ic[1] + iz[2] * ic[2]) * a) *
+                                         c->dt * c->st / coeff / r1;
+                    }
+                }
+                if (j < my - 2 || (jp && j == my - 2)) {
+                    dpdc = 0.; dpdz = 0.;
+                    if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k, j + 1, i + 1) > th) {
+                        if (NV(k, j, i - 1) + NV(k, j + 1, i - 1) < th && (i != 1))
+                            dpdc = (PH(k, j, i) + PH(k, j + 1, i) - PH(k, j, i - 1) - PH(k, j + 1, i - 1)) * 0.5;
+                    } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k, j + 1, i - 1) > th) {
+                        if (NV(k, j, i + 1) + NV(k, j + 1, i + 1) < th)
+                            dpdc = (PH(k, j, i + 1) + PH(k, j + 1, i + 1) - PH(k, j, i) - PH(k, j + 1, i)) * 0.5;
+                    } else dpdc = (PH(k, j, i + 1) + PH(k, j + 1, i + 1) - PH(k, j, i - 1) - PH(k, j + 1, i - 1)) * 0.25;
+                    if (j == my - 2 && jp) dpde = PH(k, 1, i) - PH(k, j, i);
+                    else dpde = PH(k, j + 1, i) - PH(k, j, i);
+                    if ((k == mz - 2 && !kp) || NV(k + 1, j, i) + NV(k + 1, j + 1, i) > th) {
+                        if (NV(k - 1, j, i) + NV(k - 1, j + 1, i) < th && (k != 1))
+                            dpdz = (PH(k, j, i) + PH(k, j + 1, i) - PH(k - 1, j, i) - PH(k - 1, j + 1, i)) * 0.5;
+                    } else if ((k == 1 && !kp) || NV(k - 1, j, i) + NV(k - 1, j + 1, i) > th) {
+                        if (NV(k + 1, j, i) + NV(k + 1, j + 1, i) < th)
+                            dpdz = (PH(k + 1, j, i) + PH(k + 1, j + 1, i) - PH(k, j, i) - PH(k, j + 1, i)) * 0.5;
+                    } else dpdz = (PH(k + 1, j, i) + PH(k + 1, j + 1, i) - PH(k - 1, j, i) - PH(k - 1, j + 1, i)) * 0.25;
+                    if ((NV(k, j, i) + NV(k, j + 1, i)) < th || (int)(NV(k, j, i) + NV(k, j + 1, i)) == 5) {
+                        const double *jc = g->jcsi + 3 * id, *je = g->jeta + 3 * id, *jz = g->jzet + 3 * id;
+                        double a = g->jaj[id];
+                        ucont[3 * id + 1] -= (dpdc * (jc[0] * je[0] + jc[1] * je[1] + jc[2] * je[2]) * a +
+                                              dpde * (je[0] * je[0] + je[1] * je[1] + je[2] * je[2]) * a +
+                                              dpdz * (jz[0] * je[0] + jz[1] * je[1] + jz[2] * je[2]) * a) *
+                                             c->dt * c->st / coeff / r2;
+                    }
+                }
+                if (k < mz - 2 || (kp && k == mz - 2)) {
+                    dpdc = 0.; dpde = 0.;
+                    if ((i == mx - 2 && !ip) || NV(k, j, i + 1) + NV(k + 1, j, i + 1) > th) {
+                        if (NV(k, j, i - 1) + NV(k + 1, j, i - 1) < th && (i != 1))
+                            dpdc = (PH(k, j, i) + PH(k + 1, j, i) - PH(k, j, i - 1) - PH(k + 1, j, i - 1)) * 0.5;
+                    } else if ((i == 1 && !ip) || NV(k, j, i - 1) + NV(k + 1, j, i - 1) > th) {
+                        if (NV(k, j, i + 1) + NV(k + 1, j, i + 1) < th)
+                            dpdc = (PH(k, j, i + 1) + PH(k + 1, j, i + 1) - PH(k, j, i) - PH(k + 1, j, i)) * 0.5;
+                    } else dpdc = (PH(k, j, i + 1) + PH(k + 1, j, i + 1) - PH(k, j, i - 1) - PH(k + 1, j, i - 1)) * 0.25;
+                    if ((j == my - 2 && !jp) || NV(k, j + 1, i) + NV(k + 1, j + 1, i) > th) {
+                        if (NV(k, j - 1, i) + NV(k + 1, j - 1, i) < th && (j != 1))
+                            dpde = (PH(k, j, i) + PH(k + 1, j, i) - PH(k, j - 1, i) - PH(k + 1, j - 1, i)) * 0.5;
+                    } else if ((j == 1 && !jp) || NV(k, j - 1, i) + NV(k + 1, j - 1, i) > th) {
+                        if (NV(k, j + 1, i) + NV(k + 1, j + 1, i) < th)
+                            dpde = (PH(k, j + 1, i) + PH(k + 1, j + 1, i) - PH(k, j, i) - PH(k + 1, j, i)) * 0.5;
+                    } else dpde = (PH(k, j + 1, i) + PH(k + 1, j + 1, i) - PH(k, j - 1, i) - PH(k + 1, j - 1, i)) * 0.25;
+                    if (k == mz - 2 && kp) dpdz = PH(1, j, i) - PH(k, j, i);
+                    else dpdz = PH(k + 1, j, i) - PH(k, j, i);
+                    if ((NV(k, j, i) + NV(k + 1, j, i)) < th) {
+                        const double *kc = g->kcsi + 3 * id, *ke = g->keta + 3 * id, *kz = g->kzet + 3 * id;
+                        double a = g->kaj[id];
+                        ucont[3 * id + 2] -= (dpdc * (kc[0] * kz[0] + kc[1] * kz[1] + kc[2] * kz[2]) * a +
+                                              dpde * (ke[0] * kz[0] + ke[1] * kz[1] + ke[2] * kz[2]) * a +
+                                              dpdz * (kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]) * a) *
+                                             c->dt * c->st / coeff / r3;
+                    }
+                }
+            }
+#undef PH
+    /* per-component periodic face copies, poisson.c:3003-3043 */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int fi = 0, fj = 0, fk = 0, a = i, b = j, cc = k;
+                if (ip && i == 0) a = mx - 2, fi = 1; else if (ip && i == mx - 1) a = 1, fi = 1;
+                if (jp && j == 0) b = my - 2, fj = 1; else if (jp && j == my - 1) b = 1, fj = 1;
+                if (kp && k == 0) cc = mz - 2, fk = 1; else if (kp && k == mz - 1) cc = 1, fk = 1;
+                size_t d = IX(k, j, i), s = IX(cc, b, a);
+                if (fi) VX(ucont, d) = VX(ucont, s);
+                if (fj) VY(ucont, d) = VY(ucont, s);
+                if (fk) VZ(ucont, d) = VZ(ucont, s);
+            }
+    return 0;
+}
+
+/* main.c:1047-1177 Divergence */
+double vo_divergence(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                     const double *ucont, double *div)
+{
+    DIMS;
+    size_t P = (size_t)mx * my * mz;
+    memset(div, 0, P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                double m = fabs((VX(ucont, id) - VX(ucont, IX(k, j, i - 1)) + VY(ucont, id) - VY(ucont, IX(k, j - 1, i)) +
+                                 VZ(ucont, id) - VZ(ucont, IX(k - 1, j, i))) * g->aj[id]);
+                if (NV(k, j, i) + NV(k + 1, j, i) + NV(k - 1, j, i) + NV(k, j + 1, i) + NV(k, j - 1, i) +
+                    NV(k, j, i + 1) + NV(k, j, i - 1) > 0.1) m = 0.;
+                div[id] = m;
+            }
+    double mx_ = div[0];
+    for (size_t n = 0; n < P; n++) if (div[n] > mx_) mx_ = div[n];
+    return mx_;
+}
+
+/* rhs.c:1637-1659: one Runge-Kutta stage with coefficient alfa:
+ * Contra2Cart; Rhs = FF2(U) - dP; U = alfa*dt*st*Rhs + U0; Contra2Cart. */
+int vo_rk_stage(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                const double *u0, double *ucont, double *ucat, const double *nu_t,
+                const double *dP, double alfa, double *rhs)
+{
+    DIMS;
+    size_t P = (size_t)mx * my * mz;
+    int rc = vo_contra2cart(c, g, nvert, ucont, ucat);
+    if (rc) return rc;
+    memset(rhs, 0, 3 * P * sizeof(double));
+    rc = vo_formfunction2(c, g, nvert, ucont, ucat, nu_t, rhs, 1.0);
+    if (rc) return rc;
+    double a = alfa * c->dt * c->st;
+    for (size_t n = 0; n < 3 * P; n++) {
+        rhs[n] += -1 * dP[n];
+        ucont[n] = a * rhs[n] + u0[n];
+    }
+    return vo_contra2cart(c, g, nvert, ucont, ucat);
+}

@@ -1,0 +1,142 @@
+/*
+ * CPU oracle for the VFS-Geophysics fractional-step hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain-C, single-threaded FP64
+ * restatement of the reference routines, written from reading the reference
+ * source as text.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it, and only as the checker.  The product path
+ * (libvfsgpu.so) never links, loads or calls anything in this directory.
+ *
+ * Layout contract (identical to the reference DMDA single-rank arrays):
+ *   sizes mx = IM+1, my = JM+1, mz = KM+1 (IM = node count),
+ *   scalar field  s[(k*my + j)*mx + i]
+ *   vector field  v[3*((k*my + j)*mx + i) + c],  c = x,y,z  (Cmpnts AoS)
+ *   cells 1..m-2 are interior, 0 and m-1 are ghosts; periodic directions use
+ *   the i_periodic convention (ghost 0 == cell m-2, ghost m-1 == cell 1).
+ */
+#ifndef VFS_ORACLE_H
+#define VFS_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int mx, my, mz;
+    int i_periodic, j_periodic, k_periodic;
+    int bctype[6];
+    int immersed;          /* 0: no IB, 1: CURVIB masks (poisson.c:3570 VolumeFlux) */
+    int les;               /* 0/1: constant-Cs Smagorinsky                          */
+    int second_order;      /* momentum.c -second_order                              */
+    int dpdz_set;          /* 1: k-periodic forcing uses mean_pressure_gradient     */
+    double ren, dt, st;
+    double mean_pressure_gradient;
+} vo_cfg;
+
+typedef struct {
+    double *csi, *eta, *zet, *aj;
+    double *icsi, *ieta, *izet, *iaj;
+    double *jcsi, *jeta, *jzet, *jaj;
+    double *kcsi, *keta, *kzet, *kaj;
+    double *cent;
+} vo_geom;
+
+/* metrics.c:4-1346 (NEWMETRIC). coor: node coordinates on the DA layout
+ * (nodes i<mx-1, j<my-1, k<mz-1). All geom arrays must be zero-initialised. */
+int vo_form_metrics(const vo_cfg *c, const double *coor, vo_geom *g);
+
+/* ibm.c:319-341, 508-536: nvert mask from a sphere signed distance at cell
+ * centres: 3 solid, 1 fluid cell with a solid cell in its 3x3x3 box, 0 fluid. */
+int vo_nvert_sphere(const vo_cfg *c, const vo_geom *g, double cx, double cy,
+                    double cz, double r, double *nvert);
+
+/* rhs.c:57-741 Contra2Cart_2.  ucont: in/out (periodic ghosts), ucat: in/out. */
+int vo_contra2cart(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   double *ucont, double *ucat);
+
+/* les.c:1135-1252 Compute_eddy_viscosity_LES with a constant Cs (les.c:67-77). */
+int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                      const double *ucat, double cs, double *nu_t);
+
+/* momentum.c:197-440 Pressure_Gradient. p: in/out (periodic ghosts). */
+int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         double *p, double *dp);
+
+/* momentum.c:1982-2195 IB_BC. ucont in/out, ucat in/out (periodic ghosts). */
+int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert,
+             double *ucont, double *ucat);
+
+/* momentum.c:456-1979 Formfunction_2: rhs += scale * F(ucont, ucat).
+ * ucont: in/out (periodic ghosts are rewritten, momentum.c:642-669). */
+int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                     double *ucont, const double *ucat, const double *nu_t,
+                     double *rhs, double scale);
+
+/* momentum.c:2197-2301 FormFunction_SNES.  U: candidate Ucont (not modified);
+ * lucont: scratch out (the ghosted/IB-corrected flux the RHS saw);
+ * ucat: in/out (IB-node values are inputs). */
+int vo_formfunction_snes(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         const double *U, const double *U_o, const double *rhs_o,
+                         const double *dP, const double *nu_t, double *lucont,
+                         double *ucat, double *rhs);
+
+/* momentum.c:104-195 Calc_Minimum_dt: CFL-1 time step. */
+double vo_min_dt(const vo_cfg *c, const vo_geom *g, const double *ucont);
+
+/* ---- pressure projection ---- */
+/* poisson.c:182-329 setup_lidx2; returns the number of unknowns. gid: DA layout. */
+int vo_setup_gid(const vo_cfg *c, const double *nvert, double *gid);
+
+/* poisson.c:971-1682 PoissonLHSNew as a CSR matrix over the unknowns.
+ * rowptr[n+1], col[19n], val[19n] preallocated; returns nnz. */
+int vo_poisson_lhs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   const double *gid, int *rowptr, int *col, double *val);
+
+/* The 19 stencil coefficients per cell before the drop rule (index map
+ * poisson.c:951-969).  coef[19*P] on the DA layout, zero where no row. */
+int vo_poisson_stencil(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                       double *coef);
+
+/* poisson_hypre.c:257-399 PoissonRHS2_hypre + Remove_Nullspace_Scale (110-192)
+ * b on the DA layout (zero outside unknowns). */
+int vo_poisson_rhs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   const double *ucont, double *b);
+
+/* poisson.c:3570-3910 VolumeFlux (immersed==1, flg=0). ucont in/out. */
+int vo_volume_flux(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                   double *ucont, double *flux_out, double *area_out);
+
+/* FP64 restarted GMRES on the assembled CSR to rtol, then Remove_Nullspace
+ * (poisson_hypre.c:194-255) and Convert_Phi2_Phi (poisson.c:92-180).
+ * b, phi on the DA layout.  Returns iterations, *relres the true residual. */
+int vo_poisson_solve(const vo_cfg *c, const double *nvert, const double *gid,
+                     const int *rowptr, const int *col, const double *val,
+                     const double *b, double rtol, int maxit, double *phi,
+                     double *relres);
+
+/* y = A x on the DA layout with the assembled CSR. */
+int vo_csr_apply(const vo_cfg *c, const double *gid, const int *rowptr,
+                 const int *col, const double *val, const double *x, double *y);
+
+/* poisson.c:3144-3304 UpdatePressure.  p, phi in/out (periodic ghosts). */
+int vo_update_pressure(const vo_cfg *c, const double *nvert, double *p, double *phi);
+
+/* poisson.c:2707-3073 Projection (without the trailing Contra2Cart).
+ * phi read-only (periodic ghosts must be set), ucont in/out. */
+int vo_projection(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                  const double *phi, double *ucont);
+
+/* main.c:1047-1177 Divergence: returns max |div U| * aj. */
+double vo_divergence(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                     const double *ucont, double *div);
+
+/* rhs.c:1637-1659 one Runge-Kutta stage (alfa = 0.25, 1/3, 0.5, 1 in turn):
+ * U = alfa*dt*st*(FF2(U) - dP) + U0.  ucont, ucat in/out. */
+int vo_rk_stage(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                const double *u0, double *ucont, double *ucat, const double *nu_t,
+                const double *dP, double alfa, double *rhs_scratch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

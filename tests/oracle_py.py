@@ -1,0 +1,191 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker that GPU results are
+compared against; nothing in the product imports this module.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "liboracle.so")
+
+
+class VoCfg(C.Structure):
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+                ("i_periodic", C.c_int), ("j_periodic", C.c_int), ("k_periodic", C.c_int),
+                ("bctype", C.c_int * 6), ("immersed", C.c_int), ("les", C.c_int),
+                ("second_order", C.c_int), ("dpdz_set", C.c_int),
+                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+                ("mean_pressure_gradient", C.c_double)]
+
+
+GEOM_FIELDS = ["csi", "eta", "zet", "aj", "icsi", "ieta", "izet", "iaj",
+               "jcsi", "jeta", "jzet", "jaj", "kcsi", "keta", "kzet", "kaj", "cent"]
+VECTOR_GEOM = {"csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jzet",
+               "kcsi", "keta", "kzet", "cent"}
+
+
+class VoGeom(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in GEOM_FIELDS]
+
+
+def _build():
+    src = os.path.join(ORACLE_DIR, "vfs_oracle.c")
+    if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(src):
+        subprocess.check_call(["make", "-s", "-C", ORACLE_DIR])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _build()
+        _lib = C.CDLL(LIB)
+        d = C.c_double
+        for name in ["vo_min_dt", "vo_divergence"]:
+            getattr(_lib, name).restype = d
+    return _lib
+
+
+def ptr(a):
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(C.POINTER(C.c_double))
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(C.POINTER(C.c_int))
+    raise TypeError(a.dtype)
+
+
+def make_cfg(d):
+    c = VoCfg()
+    for k, v in d.items():
+        if k == "bctype":
+            for n in range(6):
+                c.bctype[n] = int(v[n])
+        elif hasattr(c, k):
+            setattr(c, k, v)
+    return c
+
+
+class Oracle:
+    """Holds a configuration plus metrics and wraps every oracle routine."""
+
+    def __init__(self, cfg: dict, coor: np.ndarray):
+        self.cfgd = dict(cfg)
+        self.c = make_cfg(cfg)
+        self.mx, self.my, self.mz = cfg["mx"], cfg["my"], cfg["mz"]
+        self.P = self.mx * self.my * self.mz
+        self.g = {}
+        for n in GEOM_FIELDS:
+            self.g[n] = np.zeros(self.P * (3 if n in VECTOR_GEOM else 1))
+        self.geom = VoGeom(*[ptr(self.g[n]) for n in GEOM_FIELDS])
+        rc = lib().vo_form_metrics(C.byref(self.c), ptr(coor), C.byref(self.geom))
+        if rc:
+            raise RuntimeError(f"vo_form_metrics rc={rc}")
+        self.nvert = np.zeros(self.P)
+
+    def vec(self):
+        return np.zeros(3 * self.P)
+
+    def sca(self):
+        return np.zeros(self.P)
+
+    def nvert_sphere(self, cx, cy, cz, r):
+        rc = lib().vo_nvert_sphere(C.byref(self.c), C.byref(self.geom), C.c_double(cx), C.c_double(cy),
+                                   C.c_double(cz), C.c_double(r), ptr(self.nvert))
+        assert rc == 0, rc
+        return self.nvert
+
+    def _call(self, name, *args):
+        rc = getattr(lib(), name)(C.byref(self.c), *args)
+        if isinstance(rc, int) and rc < 0:
+            raise RuntimeError(f"{name} rc={rc}")
+        return rc
+
+    def contra2cart(self, ucont, ucat):
+        self._call("vo_contra2cart", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat))
+
+    def eddy_viscosity(self, ucat, cs, nu_t):
+        self._call("vo_eddy_viscosity", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.c_double(cs), ptr(nu_t))
+
+    def pressure_gradient(self, p, dp):
+        self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
+
+    def ib_bc(self, ucont, ucat):
+        self._call("vo_ib_bc", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat))
+
+    def formfunction2(self, ucont, ucat, nu_t, rhs, scale):
+        self._call("vo_formfunction2", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat),
+                   ptr(nu_t), ptr(rhs), C.c_double(scale))
+
+    def formfunction_snes(self, U, U_o, rhs_o, dP, nu_t, lucont, ucat, rhs):
+        self._call("vo_formfunction_snes", C.byref(self.geom), ptr(self.nvert), ptr(U), ptr(U_o),
+                   ptr(rhs_o), ptr(dP), ptr(nu_t), ptr(lucont), ptr(ucat), ptr(rhs))
+
+    def rk_stage(self, u0, ucont, ucat, nu_t, dP, alfa, rhs):
+        self._call("vo_rk_stage", C.byref(self.geom), ptr(self.nvert), ptr(u0), ptr(ucont), ptr(ucat),
+                   ptr(nu_t), ptr(dP), C.c_double(alfa), ptr(rhs))
+
+    def min_dt(self, ucont):
+        return lib().vo_min_dt(C.byref(self.c), C.byref(self.geom), ptr(ucont))
+
+    def setup_gid(self):
+        self.gid = self.sca()
+        self.n = self._call("vo_setup_gid", ptr(self.nvert), ptr(self.gid))
+        return self.gid
+
+    def poisson_lhs(self):
+        if not hasattr(self, "gid"):
+            self.setup_gid()
+        n = self.n
+        rp = np.zeros(n + 1, dtype=np.int32)
+        col = np.zeros(19 * n + 1, dtype=np.int32)
+        val = np.zeros(19 * n + 1)
+        nnz = self._call("vo_poisson_lhs", C.byref(self.geom), ptr(self.nvert), ptr(self.gid),
+                         ptr(rp), ptr(col), ptr(val))
+        self.csr = (rp, col[:nnz].copy(), val[:nnz].copy())
+        return self.csr
+
+    def poisson_stencil(self):
+        coef = np.zeros(19 * self.P)
+        self._call("vo_poisson_stencil", C.byref(self.geom), ptr(self.nvert), ptr(coef))
+        return coef
+
+    def poisson_rhs(self, ucont, b):
+        self._call("vo_poisson_rhs", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(b))
+
+    def volume_flux(self, ucont):
+        f = C.c_double()
+        a = C.c_double()
+        self._call("vo_volume_flux", C.byref(self.geom), ptr(self.nvert), ptr(ucont), C.byref(f), C.byref(a))
+        return f.value, a.value
+
+    def poisson_solve(self, b, phi, rtol=1e-13, maxit=20000):
+        if not hasattr(self, "csr"):
+            self.poisson_lhs()
+        rp, col, val = self.csr
+        rel = C.c_double()
+        it = self._call("vo_poisson_solve", ptr(self.nvert), ptr(self.gid), ptr(rp), ptr(col), ptr(val),
+                        ptr(b), C.c_double(rtol), C.c_int(maxit), ptr(phi), C.byref(rel))
+        return it, rel.value
+
+    def csr_apply(self, x, y):
+        rp, col, val = self.csr
+        self._call("vo_csr_apply", ptr(self.gid), ptr(rp), ptr(col), ptr(val), ptr(x), ptr(y))
+
+    def update_pressure(self, p, phi):
+        self._call("vo_update_pressure", ptr(self.nvert), ptr(p), ptr(phi))
+
+    def projection(self, phi, ucont):
+        self._call("vo_projection", C.byref(self.geom), ptr(self.nvert), ptr(phi), ptr(ucont))
+
+    def divergence(self, ucont):
+        div = self.sca()
+        m = lib().vo_divergence(C.byref(self.c), C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(div))
+        return m, div

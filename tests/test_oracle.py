@@ -1,0 +1,387 @@
+"""Pins the CPU oracle against what the mathematics fixes.
+
+The reference ships no tests, fixtures or golden vectors and cannot be built
+here (it needs PETSc 3.1, HYPRE 2.11 and MPI), so each oracle routine is
+checked against closed forms, invariants and brute force chosen so that a
+plausible restatement mistake (a dropped term, a wrong sign or index, a
+transposed operand) fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from cases import oracle_case
+from conftest import rel_err
+from oracle_py import Oracle
+from vfsgpu import workloads as W
+import vfsgpu as V
+
+
+def uniform_case(N=(12, 10, 8), immersed=0, les=0, all_periodic=False):
+    Ni, Nj, Nk = N
+    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=1 if all_periodic else 0, k_periodic=1,
+               bctype=[7, 7, 7, 7, 7, 7] if all_periodic else [7, 7, 1, 1, 7, 7], immersed=immersed, les=les,
+               second_order=0, dpdz_set=1, ren=180.0, dt=1e-3, st=1.0, mean_pressure_gradient=0.0)
+    coor = V.channel_grid(Ni + 1, Nj + 1, Nk + 1, W.LX, W.LY, W.LZ, 0.0, 0.0)
+    return cfg, coor, Oracle(cfg, coor)
+
+
+def arr(o, a, comps=3):
+    return a.reshape(o.mz, o.my, o.mx, comps) if comps > 1 else a.reshape(o.mz, o.my, o.mx)
+
+
+# ---------------------------------------------------------------------------
+# metrics (metrics.c)
+# ---------------------------------------------------------------------------
+def test_metrics_uniform_closed_form():
+    cfg, coor, o = uniform_case()
+    Ni, Nj, Nk = cfg["mx"] - 2, cfg["my"] - 2, cfg["mz"] - 2
+    dx, dy, dz = W.LX / Ni, W.LY / Nj, W.LZ / Nk
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    csi, eta, zet = arr(o, o.g["csi"]), arr(o, o.g["eta"]), arr(o, o.g["zet"])
+    aj = arr(o, o.g["aj"], 1)
+    assert rel_err(csi[inner], np.broadcast_to([dy * dz, 0, 0], csi[inner].shape)) < 1e-14
+    assert rel_err(eta[inner], np.broadcast_to([0, dz * dx, 0], eta[inner].shape)) < 1e-14
+    assert rel_err(zet[inner], np.broadcast_to([0, 0, dx * dy], zet[inner].shape)) < 1e-14
+    assert rel_err(aj[inner], np.full(aj[inner].shape, 1 / (dx * dy * dz))) < 1e-13
+    # face metrics of a uniform grid equal the cell metrics; harmonic mean too
+    for f, ref in (("icsi", [dy * dz, 0, 0]), ("jeta", [0, dz * dx, 0]), ("kzet", [0, 0, dx * dy])):
+        a = arr(o, o.g[f])[inner]
+        assert rel_err(a, np.broadcast_to(ref, a.shape)) < 1e-14, f
+    for f in ("iaj", "jaj", "kaj"):
+        a = arr(o, o.g[f], 1)[inner]
+        assert rel_err(a, np.full(a.shape, 1 / (dx * dy * dz))) < 1e-13, f
+
+
+def test_metrics_jacobian_is_inverse_cell_volume():
+    """On the stretched, wavy C2 grid, 1/aj of every cell equals the volume of
+    its hexahedron from the divergence theorem (independent formula)."""
+    cfg, spec = W.make_config("C2", (10, 8, 6))
+    coor = W.grid(spec)
+    o = Oracle(cfg, coor)
+    X = coor.reshape(o.mz, o.my, o.mx, 3)
+    aj = arr(o, o.g["aj"], 1)
+    for (k, j, i) in [(1, 1, 1), (3, 4, 5), (6, 8, 10), (2, 7, 3)]:
+        c = np.array([[[X[k - 1 + a, j - 1 + b, i - 1 + q] for q in (0, 1)] for b in (0, 1)] for a in (0, 1)])
+        # trilinear hexahedron volume via the 8-point Gauss rule of det(J)
+        vol = 0.0
+        g = 1 / math.sqrt(3)
+        for s in (-g, g):
+            for t in (-g, g):
+                for u in (-g, g):
+                    r, e, z = (s + 1) / 2, (t + 1) / 2, (u + 1) / 2
+                    def d(f):
+                        return f
+                    Jm = np.zeros((3, 3))
+                    # derivatives of the trilinear map w.r.t. (r, e, z)
+                    Jm[:, 0] = ((1 - e) * (1 - z) * (c[0, 0, 1] - c[0, 0, 0]) + e * (1 - z) * (c[0, 1, 1] - c[0, 1, 0]) +
+                                (1 - e) * z * (c[1, 0, 1] - c[1, 0, 0]) + e * z * (c[1, 1, 1] - c[1, 1, 0]))
+                    Jm[:, 1] = ((1 - r) * (1 - z) * (c[0, 1, 0] - c[0, 0, 0]) + r * (1 - z) * (c[0, 1, 1] - c[0, 0, 1]) +
+                                (1 - r) * z * (c[1, 1, 0] - c[1, 0, 0]) + r * z * (c[1, 1, 1] - c[1, 0, 1]))
+                    Jm[:, 2] = ((1 - r) * (1 - e) * (c[1, 0, 0] - c[0, 0, 0]) + r * (1 - e) * (c[1, 0, 1] - c[0, 0, 1]) +
+                                (1 - r) * e * (c[1, 1, 0] - c[0, 1, 0]) + r * e * (c[1, 1, 1] - c[0, 1, 1]))
+                    vol += np.linalg.det(Jm) / 8
+        # the reference's centred 8-node differences equal the Jacobian at the
+        # cell centre, exact for the volume of a parallelepiped and second-order
+        # accurate for a smooth hexahedron
+        assert abs(1 / aj[k, j, i] - vol) / vol < 2e-3
+
+
+def test_metrics_ghost_mirror_and_periodic():
+    cfg, spec = W.make_config("C2", (10, 8, 6))
+    o = Oracle(cfg, W.grid(spec))
+    csi = arr(o, o.g["csi"])
+    aj = arr(o, o.g["aj"], 1)
+    # periodic in i and k: ghost 0 == cell m-2, ghost m-1 == cell 1
+    assert np.array_equal(csi[3, 4, 0], csi[3, 4, o.mx - 2])
+    assert np.array_equal(aj[0, 4, 3], aj[o.mz - 2, 4, 3])
+    # mirrored in j (metrics.c:288-310)
+    assert np.array_equal(aj[3, 0, 3], aj[3, 1, 3]) and np.array_equal(aj[3, o.my - 1, 3], aj[3, o.my - 2, 3])
+    # face value is the average of the two cells, face Jacobian the harmonic mean
+    icsi = arr(o, o.g["icsi"])
+    iaj = arr(o, o.g["iaj"], 1)
+    assert np.allclose(icsi[2, 3, 4], 0.5 * csi[2, 3, 4] + 0.5 * csi[2, 3, 5], rtol=0, atol=0)
+    assert iaj[2, 3, 4] == 2.0 / (1.0 / aj[2, 3, 4] + 1.0 / aj[2, 3, 5])
+
+
+# ---------------------------------------------------------------------------
+# mask (ibm.c)
+# ---------------------------------------------------------------------------
+def test_nvert_sphere_brute_force():
+    cfg, spec = W.make_config("C2", (24, 16, 12))
+    coor = W.grid(spec)
+    o = Oracle(cfg, coor)
+    nv = arr(o, o.nvert_sphere(*W.SPHERE).copy(), 1)
+    X = coor.reshape(o.mz, o.my, o.mx, 3)
+    cent = 0.125 * (X[1:, 1:, 1:] + X[1:, :-1, 1:] + X[:-1, 1:, 1:] + X[:-1, :-1, 1:] +
+                    X[1:, 1:, :-1] + X[1:, :-1, :-1] + X[:-1, 1:, :-1] + X[:-1, :-1, :-1])
+    cx, cy, cz, r = W.SPHERE
+    inside = np.zeros((o.mz, o.my, o.mx), bool)
+    d = np.sqrt((cent[..., 0] - cx) ** 2 + (cent[..., 1] - cy) ** 2 + (cent[..., 2] - cz) ** 2) - r
+    inside[1:-1, 1:-1, 1:-1] = d[:-1, :-1, :-1][: o.mz - 2, : o.my - 2, : o.mx - 2] < 0
+    ref = np.zeros((o.mz, o.my, o.mx))
+    ref[inside] = 3
+    for k in range(1, o.mz - 1):
+        for j in range(1, o.my - 1):
+            for i in range(1, o.mx - 1):
+                if not inside[k, j, i] and inside[k - 1:k + 2, j - 1:j + 2, i - 1:i + 2].any():
+                    ref[k, j, i] = 1
+    assert inside.sum() > 4
+    assert np.array_equal(nv, ref)
+
+
+# ---------------------------------------------------------------------------
+# Contra2Cart (rhs.c)
+# ---------------------------------------------------------------------------
+def test_contra2cart_cramer_brute_force():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (12, 10, 8))
+    u = ucat.copy()
+    o.contra2cart(ucont.copy(), u)
+    U = arr(o, u)
+    Uc = arr(o, ucont)
+    csi, eta, zet = arr(o, o.g["csi"]), arr(o, o.g["eta"]), arr(o, o.g["zet"])
+    nv = arr(o, o.nvert, 1)
+    for (k, j, i) in [(1, 1, 1), (4, 5, 6), (8, 10, 12), (3, 2, 7)]:
+        if nv[k, j, i] >= 0.1:
+            continue
+        M = np.array([csi[k, j, i], eta[k, j, i], zet[k, j, i]])
+        q = np.array([0.5 * (Uc[k, j, i - 1, 0] + Uc[k, j, i, 0]), 0.5 * (Uc[k, j - 1, i, 1] + Uc[k, j, i, 1]),
+                      0.5 * (Uc[k - 1, j, i, 2] + Uc[k, j, i, 2])])
+        assert rel_err(U[k, j, i], np.linalg.solve(M, q)) < 1e-12
+
+
+def test_contra2cart_noslip_ghosts_and_linearity():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C1", (10, 8, 6))
+    u1 = ucat.copy(); o.contra2cart(ucont.copy(), u1)
+    U = arr(o, u1)
+    # no-slip mirror: ghost = -first interior cell (rhs.c:554-567)
+    assert np.array_equal(U[3, 0, 4], -U[3, 1, 4]) and np.array_equal(U[3, o.my - 1, 4], -U[3, o.my - 2, 4])
+    # periodic ghosts
+    assert np.array_equal(U[3, 4, 0], U[3, 4, o.mx - 2])
+    # linear in ucont
+    u2 = ucat.copy(); o.contra2cart(2.5 * ucont, u2)
+    assert rel_err(u2, 2.5 * u1) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# Formfunction_2 (momentum.c)
+# ---------------------------------------------------------------------------
+def test_ff2_poiseuille_closed_form():
+    """u = 1-(y-1)^2: convection cancels and the viscous flux difference is
+    exact for quadratics, so interior rhs.x = nu * u'' * dx*dy*dz * csi.x*iaj
+    = -2 nu dy dz; rhs.y = rhs.z = 0."""
+    Ni, Nj, Nk = 16, 16, 16
+    cfg, coor, o = uniform_case((Ni, Nj, Nk))
+    dx, dy, dz = W.LX / Ni, W.LY / Nj, W.LZ / Nk
+    ucat, ucont = V.poiseuille(cfg, W.LY, 0.0, 1, o.g["cent"], o.nvert, o.g["icsi"], o.g["jeta"], o.g["kzet"])
+    u = ucat.copy(); uc = ucont.copy()
+    o.contra2cart(uc, u)
+    rhs = o.vec()
+    o.formfunction2(uc, u, o.sca(), rhs, 1.0)
+    R = arr(o, rhs)
+    nu = 1.0 / cfg["ren"]
+    interior = R[1:-1, 3:-3, 1:-1]
+    assert rel_err(interior[..., 0], np.full(interior[..., 0].shape, -2 * nu * dy * dz)) < 1e-10
+    assert np.abs(R[..., 1]).max() < 1e-15 and np.abs(R[..., 2]).max() < 1e-15
+
+
+def test_ff2_uniform_flow_has_zero_rhs():
+    """Triply periodic uniform grid, constant velocity: every face flux is the
+    same, so divergence, 4th-order correction and viscous terms vanish."""
+    cfg, coor, o = uniform_case((8, 8, 8), all_periodic=True)
+    ucat = np.zeros(3 * o.P)
+    U = np.array([1.0, -0.3, 0.7])
+    ucat.reshape(-1, 3)[:] = U
+    ucont = np.zeros(3 * o.P)
+    for f, q in (("icsi", 0), ("jeta", 1), ("kzet", 2)):
+        A = o.g[f].reshape(-1, 3)
+        ucont.reshape(-1, 3)[:, q] = A @ U
+    u = ucat.copy()
+    o.contra2cart(ucont, u)
+    rhs = o.vec()
+    o.formfunction2(ucont, u, o.sca(), rhs, 1.0)
+    assert np.abs(rhs).max() < 1e-14
+
+
+def test_ff2_accumulates_and_scales():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (12, 10, 8))
+    u = ucat.copy(); uc = ucont.copy()
+    o.contra2cart(uc, u)
+    nut = o.sca(); o.eddy_viscosity(u, W.CS_LES, nut)
+    r1 = o.vec(); o.formfunction2(uc, u, nut, r1, 1.0)
+    r2 = np.ones(3 * o.P); o.formfunction2(uc, u, nut, r2, 0.5)
+    R1, R2 = arr(o, r1), arr(o, r2)
+    m = R1 != 0
+    # rhs += scale*F on unmasked interior faces
+    assert rel_err(R2[m], 1.0 + 0.5 * R1[m]) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# LES (les.c)
+# ---------------------------------------------------------------------------
+def test_les_linear_shear_closed_form():
+    Ni, Nj, Nk = 10, 12, 8
+    cfg, coor, o = uniform_case((Ni, Nj, Nk), les=1)
+    dx, dy, dz = W.LX / Ni, W.LY / Nj, W.LZ / Nk
+    gamma = 0.7
+    ucat = np.zeros(3 * o.P)
+    cent = o.g["cent"].reshape(-1, 3)
+    ucat.reshape(-1, 3)[:, 0] = gamma * cent[:, 1]
+    U = arr(o, ucat)
+    U[:, 0] = -U[:, 1]; U[:, -1] = -U[:, -2]
+    U[:, :, 0] = U[:, :, -2]; U[:, :, -1] = U[:, :, 1]   # periodic ghosts
+    U[0] = U[-2]; U[-1] = U[1]
+    nut = o.sca()
+    o.eddy_viscosity(ucat, 0.01, nut)
+    N = arr(o, nut, 1)[1:-1, 2:-2, 1:-1]
+    ref = 0.01 * (dx * dy * dz) ** (2 / 3) * abs(gamma)
+    assert rel_err(N, np.full(N.shape, ref)) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Pressure gradient (momentum.c:197-440)
+# ---------------------------------------------------------------------------
+def test_pressure_gradient_linear_pressure():
+    """p = a*y on a uniform grid: dP.y = g22 * a*dy * jaj = a * dz*dx, dP.x = dP.z = 0."""
+    Ni, Nj, Nk = 10, 12, 8
+    cfg, coor, o = uniform_case((Ni, Nj, Nk))
+    dx, dy, dz = W.LX / Ni, W.LY / Nj, W.LZ / Nk
+    a = 0.37
+    p = a * o.g["cent"].reshape(-1, 3)[:, 1].copy()
+    dp = o.vec()
+    o.pressure_gradient(p, dp)
+    D = arr(o, dp)
+    inner = D[1:-1, 1:-2, 1:-1]
+    assert rel_err(inner[..., 1], np.full(inner[..., 1].shape, a * dz * dx)) < 1e-12
+    assert np.abs(D[..., 0]).max() < 1e-14 and np.abs(D[..., 2]).max() < 1e-14
+    assert np.all(D[:, o.my - 2, :, 1] == 0)
+
+
+# ---------------------------------------------------------------------------
+# Poisson operator (poisson.c:971-1682)
+# ---------------------------------------------------------------------------
+def test_poisson_uniform_reduces_to_7_point():
+    Ni, Nj, Nk = 8, 6, 5
+    cfg, coor, o = uniform_case((Ni, Nj, Nk))
+    dx, dy, dz = W.LX / Ni, W.LY / Nj, W.LZ / Nk
+    coef = o.poisson_stencil().reshape(o.mz, o.my, o.mx, 19)
+    c = coef[2, 3, 4]
+    EP, WP, NP, SP, TP, BP = c[1], c[2], c[3], c[4], c[5], c[6]
+    assert rel_err([EP, WP], [dy * dz / dx] * 2) < 1e-13
+    assert rel_err([NP, SP], [dz * dx / dy] * 2) < 1e-13
+    assert rel_err([TP, BP], [dx * dy / dz] * 2) < 1e-13
+    assert np.abs(c[7:]).max() < 1e-12
+    assert abs(c[0] + c[1:7].sum()) < 1e-12 * abs(c[0])
+    # wall row: the wall neighbour is missing (Neumann)
+    w = coef[2, 1, 4]
+    assert w[4] == 0 and abs(w[0] + w[1:7].sum()) < 1e-12 * abs(w[0])
+
+
+def _csr_dense(o):
+    rp, col, val = o.poisson_lhs()
+    n = o.n
+    A = np.zeros((n, n))
+    for r in range(n):
+        for q in range(rp[r], rp[r + 1]):
+            A[r, col[q]] += val[q]
+    return A
+
+
+def test_poisson_conservation_and_nullspace_curvilinear():
+    """Flux form on the curvilinear grid with the sphere: A 1 = 0 (constants
+    are the null space) and 1^T A = 0 (face fluxes cancel pairwise)."""
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (12, 10, 8))
+    o.setup_gid()
+    A = _csr_dense(o)
+    scale = np.abs(A).max()
+    assert np.abs(A.sum(axis=1)).max() < 1e-13 * scale
+    assert np.abs(A.sum(axis=0)).max() < 1e-13 * scale
+
+
+def test_poisson_uniform_operator_symmetric():
+    cfg, coor, o = uniform_case((8, 6, 5))
+    o.setup_gid()
+    A = _csr_dense(o)
+    assert np.abs(A - A.T).max() < 1e-13 * np.abs(A).max()
+
+
+def test_poisson_linear_field_on_sheared_grid():
+    """On an affine (sheared) grid the metrics are constant and exact, so the
+    19-point operator applied to any linear pressure vanishes on rows whose
+    stencil stays clear of walls and periodic seams: this exercises every
+    cross-derivative coefficient and its sign."""
+    Ni, Nj, Nk = 10, 10, 10
+    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=1,
+               bctype=[7, 7, 1, 1, 7, 7], immersed=0, les=0, second_order=0, dpdz_set=1, ren=100.0, dt=1e-3,
+               st=1.0, mean_pressure_gradient=0.0)
+    coor = np.zeros(3 * (Ni + 2) * (Nj + 2) * (Nk + 2))
+    X = coor.reshape(Nk + 2, Nj + 2, Ni + 2, 3)
+    kk, jj, ii = np.meshgrid(np.arange(Nk + 1), np.arange(Nj + 1), np.arange(Ni + 1), indexing="ij")
+    xi, et, ze = ii * 0.1, jj * 0.13, kk * 0.09
+    X[:Nk + 1, :Nj + 1, :Ni + 1, 0] = xi + 0.31 * et + 0.05 * ze
+    X[:Nk + 1, :Nj + 1, :Ni + 1, 1] = et + 0.17 * ze + 0.08 * xi
+    X[:Nk + 1, :Nj + 1, :Ni + 1, 2] = ze + 0.23 * xi - 0.11 * et
+    o = Oracle(cfg, coor)
+    cent = o.g["cent"].reshape(-1, 3)
+    phi = 0.7 * cent[:, 0] - 1.3 * cent[:, 1] + 0.4 * cent[:, 2]
+    o.setup_gid()
+    o.poisson_lhs()
+    y = o.sca()
+    o.csr_apply(phi.copy(), y)
+    Y = arr(o, y, 1)
+    inner = Y[3:-3, 3:-3, 3:-3]
+    coef = o.poisson_stencil().reshape(o.mz, o.my, o.mx, 19)
+    scale = np.abs(coef[3:-3, 3:-3, 3:-3]).max() * np.abs(phi).max()
+    assert np.abs(coef[5, 5, 5, 7:]).max() > 1e-6   # cross terms present
+    assert np.abs(inner).max() < 1e-12 * scale
+
+
+# ---------------------------------------------------------------------------
+# projection chain (poisson_hypre.c, poisson.c, main.c)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("name", ["C1", "C2"])
+def test_projection_removes_divergence(name):
+    cfg, spec, coor, o, ucat, ucont = oracle_case(name, (12, 10, 8))
+    uc = ucont.copy()
+    if spec["immersed"]:
+        o.volume_flux(uc)
+    b = o.sca()
+    o.poisson_rhs(uc, b)
+    d0, _ = o.divergence(uc)
+    phi = o.sca()
+    it, rel = o.poisson_solve(b, phi, rtol=1e-13)
+    assert rel < 1e-12
+    o.projection(phi, uc)
+    d1, _ = o.divergence(uc)
+    assert d1 < 1e-9 * d0
+
+
+def test_volume_flux_balances_ib_faces():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (24, 16, 12))
+    uc = ucont.copy()
+    flux, area = o.volume_flux(uc)
+    assert area > 0
+    assert abs(flux) < 1e-13 * area
+
+
+def test_poisson_rhs_has_zero_sum():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (12, 10, 8))
+    b = o.sca()
+    o.poisson_rhs(ucont.copy(), b)
+    assert abs(b.sum()) < 1e-10 * np.abs(b).sum()
+
+
+def test_update_pressure_rules():
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (24, 16, 12))
+    p = np.full(o.P, 2.0)
+    phi = np.arange(o.P, dtype=float) * 1e-3
+    p0 = p.copy()
+    o.update_pressure(p, phi)
+    nv = o.nvert
+    P = arr(o, p, 1)[1:-1, 1:-1, 1:-1]
+    N = arr(o, nv, 1)[1:-1, 1:-1, 1:-1]
+    PH = arr(o, phi, 1)[1:-1, 1:-1, 1:-1]
+    assert np.all(P[N > 1.1] == 0)
+    assert np.allclose(P[N < 0.1], 2.0 + PH[N < 0.1], rtol=0, atol=0)
+    assert np.all(P[(N > 0.1) & (N < 1.1)] == 2.0)

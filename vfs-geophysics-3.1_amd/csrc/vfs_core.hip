@@ -1,0 +1,214 @@
+// Context life cycle, device memory helpers, periodic ghost copies and the
+// second pass of the deterministic reductions.
+#include <cstring>
+#include <string>
+
+#include "vfs_internal.h"
+#include "vfs_reduce.h"
+
+static thread_local std::string g_last_error;
+
+void vfs_set_error(const std::string &msg) { g_last_error = msg; }
+
+extern "C" const char *vfs_last_error(void) { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// periodic ghost copies over the ghost shell only
+// ---------------------------------------------------------------------------
+__global__ void k_periodic_vec(Dims d, double *v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int k, j, i;
+    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
+    int a, b, c;
+    if (!per_src(d, i, j, k, a, b, c)) return;
+    long long s = IX(d, c, b, a), o = IX(d, k, j, i);
+    v[3 * o] = v[3 * s]; v[3 * o + 1] = v[3 * s + 1]; v[3 * o + 2] = v[3 * s + 2];
+}
+
+__global__ void k_periodic_sca(Dims d, double *v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int k, j, i;
+    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
+    int a, b, c;
+    if (!per_src(d, i, j, k, a, b, c)) return;
+    v[IX(d, k, j, i)] = v[IX(d, c, b, a)];
+}
+
+int vfs_periodic_vec(vfs_ctx *c, double *v)
+{
+    if (!(c->d.ip || c->d.jp || c->d.kp)) return 0;
+    long long n = shell_count(c->d);
+    hipLaunchKernelGGL(k_periodic_vec, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->d, v);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+int vfs_periodic_sca(vfs_ctx *c, double *v)
+{
+    if (!(c->d.ip || c->d.jp || c->d.kp)) return 0;
+    long long n = shell_count(c->d);
+    hipLaunchKernelGGL(k_periodic_sca, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->d, v);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// reduction finish: partials at red[v*VFS_RED_BLOCKS + b], b < nblocks
+// ---------------------------------------------------------------------------
+__global__ void k_reduce_finish(const double *part, int nblocks, int op, double *out)
+{
+    __shared__ double sh[4];
+    const int v = blockIdx.x;
+    double acc = op == 0 ? 0.0 : -1.0e308;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
+        double x = part[(long long)v * VFS_RED_BLOCKS + b];
+        acc = op == 0 ? acc + x : fmax(acc, x);
+    }
+    double r = op == 0 ? block_sum<256>(acc, sh) : block_max<256>(acc, sh);
+    if (threadIdx.x == 0) out[v] = r;
+}
+
+int vfs_reduce_finish(vfs_ctx *c, int nblocks, int nvals, int op, double *dev_out)
+{
+    hipLaunchKernelGGL(k_reduce_finish, dim3(nvals), dim3(256), 0, c->stream, c->red, nblocks, op, dev_out);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+int vfs_read_scalars(vfs_ctx *c, const double *dev, double *host, int n)
+{
+    VFS_CHECK_HIP(hipMemcpyAsync(host, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+static int check_config(const vfs_config *cfg)
+{
+    if (cfg->mx < 7 || cfg->my < 7 || cfg->mz < 7) {
+        vfs_set_error("grid too small: every direction needs at least 5 cells");
+        return -2;
+    }
+    const int per[3] = {cfg->i_periodic, cfg->j_periodic, cfg->k_periodic};
+    for (int f = 0; f < 6; f++) {
+        int bt = cfg->bctype[f];
+        if (per[f / 2]) {
+            if (bt != 7) { vfs_set_error("periodic faces must carry bctype 7"); return -3; }
+        } else if (bt != 1) {
+            vfs_set_error("unsupported bctype (supported: 1 no-slip, 7 periodic)");
+            return -3;
+        }
+    }
+    if (cfg->immersed != 0 && cfg->immersed != 1) { vfs_set_error("immersed must be 0 or 1"); return -4; }
+    if (cfg->k_periodic && !cfg->dpdz_set) {
+        vfs_set_error("k-periodic flux forcing (inlet_flux) is not supported; set dpdz_set");
+        return -5;
+    }
+    return 0;
+}
+
+extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
+{
+    if (!cfg || !out) { vfs_set_error("null argument"); return -2; }
+    int rc = check_config(cfg);
+    if (rc) return rc;
+    vfs_ctx *c = new vfs_ctx();
+    std::memset((void *)c, 0, sizeof(vfs_ctx));
+    c->cfg = *cfg;
+    Dims &d = c->d;
+    d.mx = cfg->mx; d.my = cfg->my; d.mz = cfg->mz;
+    d.ip = cfg->i_periodic; d.jp = cfg->j_periodic; d.kp = cfg->k_periodic;
+    for (int f = 0; f < 6; f++) d.bc[f] = cfg->bctype[f];
+    d.immersed = cfg->immersed; d.les = cfg->les; d.second_order = cfg->second_order;
+    d.dpdz_set = cfg->dpdz_set;
+    d.ren = cfg->ren; d.dt = cfg->dt; d.st = cfg->st; d.mpg = cfg->mean_pressure_gradient;
+    d.P = (long long)d.mx * d.my * d.mz;
+    if (hipGetDevice(&c->device) != hipSuccess) { delete c; vfs_set_error("no HIP device"); return -1; }
+    c->stream = 0;
+    const size_t P = (size_t)d.P;
+    // metrics: 13 vector arrays + 4 scalar arrays
+    size_t ng = (13 * 3 + 4) * P;
+    if (hipMalloc(&c->geom_block, ng * sizeof(double)) != hipSuccess) { delete c; vfs_set_error("hipMalloc geometry"); return -1; }
+    hipMemset(c->geom_block, 0, ng * sizeof(double));
+    double *p = c->geom_block;
+    double **vecs[13] = {&c->g.csi, &c->g.eta, &c->g.zet, &c->g.icsi, &c->g.ieta, &c->g.izet,
+                         &c->g.jcsi, &c->g.jeta, &c->g.jzet, &c->g.kcsi, &c->g.keta, &c->g.kzet, &c->g.cent};
+    for (int n = 0; n < 13; n++) { *vecs[n] = p; p += 3 * P; }
+    double **scas[4] = {&c->g.aj, &c->g.iaj, &c->g.jaj, &c->g.kaj};
+    for (int n = 0; n < 4; n++) { *scas[n] = p; p += P; }
+    bool ok = hipMalloc(&c->nvert, P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->face, 18 * P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->fp, 3 * P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->snap, 3 * P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->tmp3, 3 * P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->red, (VFS_RED_SLOTS * VFS_RED_BLOCKS + 256) * sizeof(double)) == hipSuccess;
+    if (!ok) { vfs_ctx_destroy(c); vfs_set_error("hipMalloc scratch"); return -1; }
+    hipMemset(c->nvert, 0, P * sizeof(double));
+    hipMemset(c->face, 0, 18 * P * sizeof(double));
+    hipMemset(c->fp, 0, 3 * P * sizeof(double));
+    hipMemset(c->snap, 0, 3 * P * sizeof(double));
+    hipMemset(c->tmp3, 0, 3 * P * sizeof(double));
+    c->red_blocks = VFS_RED_BLOCKS;
+    if (hipDeviceSynchronize() != hipSuccess) { vfs_ctx_destroy(c); vfs_set_error("device sync"); return -1; }
+    *out = c;
+    return 0;
+}
+
+extern "C" int vfs_ctx_destroy(vfs_ctx *c)
+{
+    if (!c) return 0;
+    hipDeviceSynchronize();
+    vfs_poisson_destroy(c);
+    hipFree(c->geom_block); hipFree(c->nvert); hipFree(c->face); hipFree(c->fp);
+    hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red);
+    delete c;
+    return 0;
+}
+
+extern "C" int vfs_set_stream(vfs_ctx *c, void *s)
+{
+    c->stream = (hipStream_t)s;
+    return 0;
+}
+
+extern "C" int vfs_synchronize(vfs_ctx *c)
+{
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int vfs_device_malloc(size_t bytes, void **ptr)
+{
+    VFS_CHECK_HIP(hipMalloc(ptr, bytes ? bytes : 8));
+    VFS_CHECK_HIP(hipMemset(*ptr, 0, bytes ? bytes : 8));
+    return 0;
+}
+
+extern "C" int vfs_device_free(void *ptr)
+{
+    VFS_CHECK_HIP(hipFree(ptr));
+    return 0;
+}
+
+extern "C" int vfs_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    VFS_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+}
+
+extern "C" int vfs_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    VFS_CHECK_HIP(hipDeviceSynchronize());
+    VFS_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int vfs_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+    VFS_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}

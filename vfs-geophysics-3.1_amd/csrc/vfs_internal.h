@@ -1,0 +1,148 @@
+// Internal declarations shared by the HIP translation units of libvfsgpu.
+//
+// Data layout in HBM: every field is a flat array on the reference's DMDA
+// single-rank layout (mx*my*mz points, i fastest), vectors as {x,y,z} AoS.
+// Kernels are launched on the context's stream; nothing here allocates per
+// call except the one-time scratch owned by the context.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string>
+
+#include "vfsgpu.h"
+
+#define VFS_HOST_DEVICE __host__ __device__ __forceinline__
+
+// Grid description passed by value to every kernel.
+struct Dims {
+    int mx, my, mz;
+    int ip, jp, kp;          // i/j/k_periodic
+    int bc[6];               // bctype
+    int immersed, les, second_order, dpdz_set;
+    double ren, dt, st, mpg;
+    long long P;             // mx*my*mz
+};
+
+VFS_HOST_DEVICE long long IX(const Dims &d, int k, int j, int i)
+{
+    return ((long long)k * d.my + j) * (long long)d.mx + i;
+}
+
+// Periodic ghost source of point (k,j,i): the reference's `flag/a/b/c` idiom
+// (e.g. momentum.c:642-669).  Sources are never flagged themselves.
+VFS_HOST_DEVICE bool per_src(const Dims &d, int i, int j, int k, int &a, int &b, int &c)
+{
+    bool flag = false;
+    a = i; b = j; c = k;
+    if (d.ip && i == 0) a = d.mx - 2, flag = true;
+    else if (d.ip && i == d.mx - 1) a = 1, flag = true;
+    if (d.jp && j == 0) b = d.my - 2, flag = true;
+    else if (d.jp && j == d.my - 1) b = 1, flag = true;
+    if (d.kp && k == 0) c = d.mz - 2, flag = true;
+    else if (d.kp && k == d.mz - 1) c = 1, flag = true;
+    return flag;
+}
+
+// Points of the ghost shell (any index on a boundary plane), enumerated as
+// the two k planes, then the two j planes without k ghosts, then the two i
+// planes without j/k ghosts.
+VFS_HOST_DEVICE long long shell_count(const Dims &d)
+{
+    long long A = (long long)d.mx * d.my, B = (long long)d.mx * (d.mz - 2),
+              C = (long long)(d.my - 2) * (d.mz - 2);
+    return 2 * (A + B + C);
+}
+
+VFS_HOST_DEVICE bool shell_point(const Dims &d, long long t, int &k, int &j, int &i)
+{
+    long long A = (long long)d.mx * d.my;
+    if (t < 2 * A) {
+        k = t < A ? 0 : d.mz - 1;
+        long long r = t % A;
+        j = (int)(r / d.mx); i = (int)(r % d.mx);
+        return true;
+    }
+    t -= 2 * A;
+    long long B = (long long)d.mx * (d.mz - 2);
+    if (t < 2 * B) {
+        j = t < B ? 0 : d.my - 1;
+        long long r = t % B;
+        k = 1 + (int)(r / d.mx); i = (int)(r % d.mx);
+        return true;
+    }
+    t -= 2 * B;
+    long long C = (long long)(d.my - 2) * (d.mz - 2);
+    if (t < 2 * C) {
+        i = t < C ? 0 : d.mx - 1;
+        long long r = t % C;
+        k = 1 + (int)(r / (d.my - 2)); j = 1 + (int)(r % (d.my - 2));
+        return true;
+    }
+    return false;
+}
+
+struct Geom {
+    double *csi, *eta, *zet, *aj;
+    double *icsi, *ieta, *izet, *iaj;
+    double *jcsi, *jeta, *jzet, *jaj;
+    double *kcsi, *keta, *kzet, *kaj;
+    double *cent;
+};
+
+struct Poisson;   // defined in vfs_solver.hip
+
+struct vfs_ctx {
+    vfs_config cfg;
+    Dims d;
+    hipStream_t stream;
+    int device;
+    Geom g;
+    double *geom_block;      // one allocation holding every metric array
+    double *nvert;
+    // scratch (device)
+    double *face;            // 18*P: div1..3 and visc1..3 (vector each)
+    double *fp;              // 3*P
+    double *snap;            // 3*P: Contra2Cart boundary snapshot / SNES copies
+    double *tmp3;            // 3*P general
+    double *red;             // reduction partials + result scalars
+    int red_blocks;
+    Poisson *pois;
+};
+
+// error handling
+void vfs_set_error(const std::string &msg);
+#define VFS_CHECK_HIP(expr)                                                      \
+    do {                                                                         \
+        hipError_t _e = (expr);                                                  \
+        if (_e != hipSuccess) {                                                  \
+            vfs_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+#define VFS_CHECK_LAUNCH()                                                       \
+    do {                                                                         \
+        hipError_t _e = hipGetLastError();                                       \
+        if (_e != hipSuccess) {                                                  \
+            vfs_set_error(std::string("kernel launch: ") + hipGetErrorString(_e)); \
+            return -1;                                                           \
+        }                                                                        \
+    } while (0)
+#define VFS_REQUIRE(cond, msg)                                                   \
+    do {                                                                         \
+        if (!(cond)) { vfs_set_error(msg); return -2; }                          \
+    } while (0)
+
+static inline unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
+
+// shared launch helpers (vfs_core.hip)
+int vfs_periodic_vec(vfs_ctx *c, double *v);      // periodic ghost copy, vector field
+int vfs_periodic_sca(vfs_ctx *c, double *v);      // periodic ghost copy, scalar field
+// deterministic reductions over n doubles produced by a per-block kernel
+int vfs_reduce_finish(vfs_ctx *c, int nblocks, int nvals, int op /*0 sum, 1 max*/, double *dev_out);
+int vfs_read_scalars(vfs_ctx *c, const double *dev, double *host, int n);
+int vfs_poisson_destroy(vfs_ctx *c);
+
+// fixed sizes of the reduction buffer
+#define VFS_RED_BLOCKS 1024
+#define VFS_RED_SLOTS 64

@@ -1,0 +1,828 @@
+// Pressure Poisson solve on gfx950: restarted GMRES, right-preconditioned by
+// one V(1,1)-cycle of an aggregation multigrid, in FP64 or FP32 inside an
+// FP64 defect-correction loop.  Replaces HYPRE GMRES(51) + BoomerAMG of
+// PoissonSolver_Hypre (poisson_hypre.c:26-89, 498-630); parity with the
+// reference is on the converged solution (true FP64 residual <= rtol, then
+// the mean-free normalisation of Remove_Nullspace, poisson_hypre.c:194-255).
+//
+// Multigrid: 2x2x2 piecewise-constant aggregation with Galerkin coarse
+// operators (27-point), damped j-line Jacobi smoothing (a tridiagonal Thomas
+// solve along every wall-normal grid line: the tanh-stretched direction is
+// where point smoothers stall), and a dense inverse of the regularised
+// coarsest operator.  The constant vector spans the null space and is the
+// left null vector of every level (the flux form makes column sums vanish),
+// so the regularised coarse solve returns the mean-free correction.
+#include <algorithm>
+#include <type_traits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vfs_internal.h"
+#include "vfs_poisson.h"
+#include "vfs_reduce.h"
+
+struct LV {
+    int nx, ny, nz;
+    long long n;
+    int pi, pj, pk;
+    int nslot;
+};
+
+static LV lv_of(const MGLevel &l) { return LV{l.nx, l.ny, l.nz, l.n, l.per_i, l.per_j, l.per_k, l.nslot}; }
+
+__constant__ int s_slot19[19][3] = {
+    {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+    {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}, {0, 1, 1}, {0, 1, -1}, {0, -1, 1},
+    {0, -1, -1}, {1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+
+__device__ __forceinline__ void slot_off(int nslot, int s, int &di, int &dj, int &dk)
+{
+    if (nslot == 19) { di = s_slot19[s][0]; dj = s_slot19[s][1]; dk = s_slot19[s][2]; }
+    else { dk = s / 9 - 1; dj = (s / 3) % 3 - 1; di = s % 3 - 1; }
+}
+
+// wrap (periodic) or reject (bounded) a neighbour coordinate
+__device__ __forceinline__ bool nb_ok(const LV &L, int &k, int &j, int &i)
+{
+    if (i < 0) { if (!L.pi) return false; i += L.nx; } else if (i >= L.nx) { if (!L.pi) return false; i -= L.nx; }
+    if (j < 0) { if (!L.pj) return false; j += L.ny; } else if (j >= L.ny) { if (!L.pj) return false; j -= L.ny; }
+    if (k < 0) { if (!L.pk) return false; k += L.nz; } else if (k >= L.nz) { if (!L.pk) return false; k -= L.nz; }
+    return true;
+}
+
+__device__ __forceinline__ long long cix(const LV &L, int k, int j, int i) { return ((long long)k * L.ny + j) * L.nx + i; }
+
+// ---------------------------------------------------------------------------
+// set-up kernels
+// ---------------------------------------------------------------------------
+__global__ void k_gather_level0(Dims d, LV L, const double *__restrict__ coef, double *A64, float *A32)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    long long g = IX(d, k + 1, j + 1, i + 1);
+    for (int s = 0; s < 19; s++) {
+        double v = coef[(long long)s * d.P + g];
+        A64[(long long)s * L.n + t] = v;
+        A32[(long long)s * L.n + t] = (float)v;
+    }
+}
+
+// Galerkin product R A P for piecewise-constant 2x2x2 aggregation
+__global__ void k_galerkin(LV F, LV C, const double *__restrict__ Af, double *Ac64, float *Ac32)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C.n) return;
+    int I = (int)(t % C.nx), J = (int)((t / C.nx) % C.ny), K = (int)(t / ((long long)C.nx * C.ny));
+    double acc[27];
+#pragma unroll
+    for (int s = 0; s < 27; s++) acc[s] = 0;
+    for (int ck = 0; ck < 2; ck++)
+        for (int cj = 0; cj < 2; cj++)
+            for (int ci = 0; ci < 2; ci++) {
+                int fk = (F.nz == 1 ? K : 2 * K + ck), fj = (F.ny == 1 ? J : 2 * J + cj), fi = (F.nx == 1 ? I : 2 * I + ci);
+                if ((F.nz == 1 && ck) || (F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
+                if (fk >= F.nz || fj >= F.ny || fi >= F.nx) continue;
+                long long f = cix(F, fk, fj, fi);
+                for (int s = 0; s < F.nslot; s++) {
+                    double a = Af[(long long)s * F.n + f];
+                    if (a == 0.0) continue;
+                    int di, dj, dk;
+                    slot_off(F.nslot, s, di, dj, dk);
+                    int nk = fk + dk, nj = fj + dj, ni = fi + di;
+                    if (!nb_ok(F, nk, nj, ni)) continue;
+                    int NK = F.nz == 1 ? nk : nk / 2, NJ = F.ny == 1 ? nj : nj / 2, NI = F.nx == 1 ? ni : ni / 2;
+                    int oK = NK - K, oJ = NJ - J, oI = NI - I;
+                    if (C.pk) { if (oK > 1) oK -= C.nz; if (oK < -1) oK += C.nz; }
+                    if (C.pj) { if (oJ > 1) oJ -= C.ny; if (oJ < -1) oJ += C.ny; }
+                    if (C.pi) { if (oI > 1) oI -= C.nx; if (oI < -1) oI += C.nx; }
+                    acc[(oK + 1) * 9 + (oJ + 1) * 3 + (oI + 1)] += a;
+                }
+            }
+    for (int s = 0; s < 27; s++) {
+        Ac64[(long long)s * C.n + t] = acc[s];
+        Ac32[(long long)s * C.n + t] = (float)acc[s];
+    }
+}
+
+// Thomas factors of the j-line tridiagonal part, one thread per (k,i) line
+__global__ void k_line_factors(LV L, const double *__restrict__ A, double *tl64, double *tu64, double *ti64,
+                               float *tl32, float *tu32, float *ti32)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)L.nx * L.nz) return;
+    int i = (int)(t % L.nx), k = (int)(t / L.nx);
+    const int sl = L.nslot == 19 ? 4 : 10, sc = L.nslot == 19 ? 0 : 13, su = L.nslot == 19 ? 3 : 16;
+    double up = 0;
+    for (int j = 0; j < L.ny; j++) {
+        long long id = cix(L, k, j, i);
+        double a = j > 0 ? A[(long long)sl * L.n + id] : 0.0;
+        double dgl = A[(long long)sc * L.n + id];
+        double c = j < L.ny - 1 ? A[(long long)su * L.n + id] : 0.0;
+        if (dgl == 0.0) { dgl = 1.0; a = 0.0; c = 0.0; }   // no unknown here: identity row
+        double den = dgl - a * up;
+        double inv = 1.0 / den;
+        up = c * inv;
+        tl64[id] = a; tu64[id] = up; ti64[id] = inv;
+        tl32[id] = (float)a; tu32[id] = (float)up; ti32[id] = (float)inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// multigrid kernels (precision T, accumulation in T except reductions)
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T apply_row(const LV &L, const T *__restrict__ A, const T *__restrict__ x, int k, int j, int i,
+                                       long long id, const T *__restrict__ e, LV C, T alpha)
+{
+    T s = 0;
+    for (int q = 0; q < L.nslot; q++) {
+        T a = A[(long long)q * L.n + id];
+        if (a == T(0)) continue;
+        int di, dj, dk;
+        slot_off(L.nslot, q, di, dj, dk);
+        int nk = k + dk, nj = j + dj, ni = i + di;
+        if (!nb_ok(L, nk, nj, ni)) continue;
+        long long nb = cix(L, nk, nj, ni);
+        T xv = x[nb];
+        if (e) {
+            int ck = L.nz == 1 ? nk : nk / 2, cj = L.ny == 1 ? nj : nj / 2, ci = L.nx == 1 ? ni : ni / 2;
+            xv += alpha * e[cix(C, ck, cj, ci)];
+        }
+        s += a * xv;
+    }
+    return s;
+}
+
+// y = A x
+template <typename T>
+__global__ void k_spmv(LV L, const T *__restrict__ A, const T *__restrict__ x, T *__restrict__ y)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    y[t] = apply_row<T>(L, A, x, k, j, i, t, nullptr, L, T(0));
+}
+
+// Damped j-line Jacobi: xn = xo + omega * T^{-1} (b - A xo), with xo = x + alpha*P e
+// (prolongation fused) or xo = 0 when x == nullptr.  One thread per line.
+template <typename T>
+__global__ void k_line_smooth(LV L, const T *__restrict__ A, const T *__restrict__ tl, const T *__restrict__ tu,
+                              const T *__restrict__ ti, const T *__restrict__ b, const T *__restrict__ x,
+                              const T *__restrict__ e, LV C, T alpha, T omega, T *__restrict__ dscr, T *__restrict__ xn)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)L.nx * L.nz) return;
+    int i = (int)(t % L.nx), k = (int)(t / L.nx);
+    T dp = 0;
+    for (int j = 0; j < L.ny; j++) {
+        long long id = cix(L, k, j, i);
+        T r = b[id];
+        if (x) r -= apply_row<T>(L, A, x, k, j, i, id, e, C, alpha);
+        dp = (r - tl[id] * dp) * ti[id];
+        dscr[id] = dp;
+    }
+    T xj = 0;
+    for (int j = L.ny - 1; j >= 0; j--) {
+        long long id = cix(L, k, j, i);
+        xj = dscr[id] - tu[id] * xj;
+        T xo = 0;
+        if (x) {
+            xo = x[id];
+            if (e) {
+                int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
+                xo += alpha * e[cix(C, ck, cj, ci)];
+            }
+        }
+        xn[id] = xo + omega * xj;
+    }
+}
+
+// coarse right-hand side: sum of the fine residuals of each aggregate
+template <typename T>
+__global__ void k_resid_restrict(LV F, LV C, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ b,
+                                 T *__restrict__ bc)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C.n) return;
+    int I = (int)(t % C.nx), J = (int)((t / C.nx) % C.ny), K = (int)(t / ((long long)C.nx * C.ny));
+    T s = 0;
+    for (int ck = 0; ck < 2; ck++)
+        for (int cj = 0; cj < 2; cj++)
+            for (int ci = 0; ci < 2; ci++) {
+                if ((F.nz == 1 && ck) || (F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
+                int fk = F.nz == 1 ? K : 2 * K + ck, fj = F.ny == 1 ? J : 2 * J + cj, fi = F.nx == 1 ? I : 2 * I + ci;
+                if (fk >= F.nz || fj >= F.ny || fi >= F.nx) continue;
+                long long f = cix(F, fk, fj, fi);
+                s += b[f] - apply_row<T>(F, A, x, fk, fj, fi, f, nullptr, F, T(0));
+            }
+    bc[t] = s;
+}
+
+// dense coarsest solve x = Cinv b over the listed unknowns; other cells 0
+template <typename T>
+__global__ void k_coarse_solve(long long n, int nc, const long long *__restrict__ cidx, const T *__restrict__ cinv,
+                               const T *__restrict__ b, T *__restrict__ x)
+{
+    for (long long q = threadIdx.x; q < n; q += blockDim.x) x[q] = 0;
+    __syncthreads();
+    for (int r = threadIdx.x; r < nc; r += blockDim.x) {
+        T s = 0;
+        for (int c = 0; c < nc; c++) s += cinv[(long long)r * nc + c] * b[cidx[c]];
+        x[cidx[r]] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels for the Krylov loop (reductions accumulate in FP64)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_multidot(long long n, const T *__restrict__ V, long long ldv, int nv, const T *__restrict__ w, double *part)
+{
+    __shared__ double sh[4];
+    for (int v = 0; v < nv; v++) {
+        double acc = 0;
+        const T *vv = V + (long long)v * ldv;
+        for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x)
+            acc += (double)vv[q] * (double)w[q];
+        double r = block_sum<256>(acc, sh);
+        if (threadIdx.x == 0) part[(long long)v * VFS_RED_BLOCKS + blockIdx.x] = r;
+    }
+}
+
+// w -= sum_v h[v] V_v  (h on the device)
+template <typename T>
+__global__ void k_multiaxpy(long long n, const T *__restrict__ V, long long ldv, int nv, const double *__restrict__ h,
+                            T *__restrict__ w)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    T s = w[q];
+    for (int v = 0; v < nv; v++) s -= (T)h[v] * V[(long long)v * ldv + q];
+    w[q] = s;
+}
+
+// u = sum_v y[v] V_v  (y on the device)
+template <typename T>
+__global__ void k_combine(long long n, const T *__restrict__ V, long long ldv, int nv, const double *__restrict__ y,
+                          T *__restrict__ u)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    T s = 0;
+    for (int v = 0; v < nv; v++) s += (T)y[v] * V[(long long)v * ldv + q];
+    u[q] = s;
+}
+
+// out = w * (1/sqrt(nrm2[0]))
+template <typename T>
+__global__ void k_normalize(long long n, const T *__restrict__ w, const double *__restrict__ nrm2, T *__restrict__ out)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double s = 1.0 / sqrt(nrm2[0]);
+    out[q] = (T)((double)w[q] * s);
+}
+
+// r = b - A x on level 0 (FP64 operator), also used by the defect correction
+__global__ void k_resid64(LV L, const double *__restrict__ A, const double *__restrict__ x, const double *__restrict__ b,
+                          double *__restrict__ r)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    r[t] = b[t] - apply_row<double>(L, A, x, k, j, i, t, nullptr, L, 0.0);
+}
+
+template <typename T>
+__global__ void k_cast_scale(long long n, const double *__restrict__ src, const double *__restrict__ nrm2, T *__restrict__ dst)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double s = nrm2 ? 1.0 / sqrt(nrm2[0]) : 1.0;
+    dst[q] = (T)(src[q] * s);
+}
+
+// x64 += scale * e  (scale = sqrt(nrm2[0]))
+template <typename T>
+__global__ void k_add_scaled(long long n, const T *__restrict__ e, const double *__restrict__ nrm2, double *__restrict__ x)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    double s = nrm2 ? sqrt(nrm2[0]) : 1.0;
+    x[q] += s * (double)e[q];
+}
+
+template <typename T>
+__global__ void k_axpy_dev(long long n, const T *__restrict__ z, T *__restrict__ x)
+{
+    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    x[q] += z[q];
+}
+
+// DA <-> level-0 compact
+__global__ void k_da_to_c(Dims d, LV L, const double *__restrict__ src, const unsigned char *__restrict__ rowflag, double *dst)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    long long g = IX(d, k + 1, j + 1, i + 1);
+    dst[t] = rowflag[g] ? src[g] : 0.0;
+}
+
+// phi on the DA layout: unknown cells from x, others 0 (Convert_Phi2_Phi,
+// poisson.c:92-130); nullspace mean removed over nvert < 0.1 cells.
+__global__ void k_c_to_da(Dims d, LV L, const double *__restrict__ x, const double *__restrict__ nvert,
+                          const double *__restrict__ mean, double *phi)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    double v = 0;
+    if (i >= 1 && j >= 1 && k >= 1 && i <= d.mx - 2 && j <= d.my - 2 && k <= d.mz - 2) {
+        if (!((int)nvert[t] > 0.1)) {
+            v = x[cix(L, k - 1, j - 1, i - 1)];
+            if (nvert[t] < 0.1) v += mean[0];
+        }
+    }
+    phi[t] = v;
+}
+
+// sum and count over nvert < 0.1 interior cells (Remove_Nullspace)
+__global__ void k_nullsum(Dims d, LV L, const double *__restrict__ x, const double *__restrict__ nvert, double *part)
+{
+    __shared__ double sh[4];
+    double s = 0, c = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < L.n; t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+        if (nvert[IX(d, k + 1, j + 1, i + 1)] < 0.1) { s += x[t]; c += 1.0; }
+    }
+    double r0 = block_sum<256>(s, sh), r1 = block_sum<256>(c, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = r0; part[VFS_RED_BLOCKS + blockIdx.x] = r1; }
+}
+
+__global__ void k_neg_mean(double *v) { v[0] = -v[0] / v[1]; }
+
+__global__ void k_set_scalar(double *p, double v) { p[0] = v; }
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T> static T *vec_of(MGLevel &l, int which);
+template <> double *vec_of<double>(MGLevel &l, int w) { return w == 0 ? l.b64 : w == 1 ? l.x64 : w == 2 ? l.y64 : l.d64; }
+template <> float *vec_of<float>(MGLevel &l, int w) { return w == 0 ? l.b32 : w == 1 ? l.x32 : w == 2 ? l.y32 : l.d32; }
+template <typename T> static const T *A_of(const MGLevel &l);
+template <> const double *A_of<double>(const MGLevel &l) { return l.A64; }
+template <> const float *A_of<float>(const MGLevel &l) { return l.A32; }
+template <typename T> static void fac_of(const MGLevel &l, const T *&a, const T *&u, const T *&i);
+template <> void fac_of<double>(const MGLevel &l, const double *&a, const double *&u, const double *&i) { a = l.tl64; u = l.tu64; i = l.ti64; }
+template <> void fac_of<float>(const MGLevel &l, const float *&a, const float *&u, const float *&i) { a = l.tl32; u = l.tu32; i = l.ti32; }
+
+static void free_level(MGLevel &l)
+{
+    hipFree(l.A64); hipFree(l.A32); hipFree(l.tl64); hipFree(l.tu64); hipFree(l.ti64);
+    hipFree(l.tl32); hipFree(l.tu32); hipFree(l.ti32);
+    hipFree(l.b64); hipFree(l.x64); hipFree(l.y64); hipFree(l.d64);
+    hipFree(l.b32); hipFree(l.x32); hipFree(l.y32); hipFree(l.d32);
+}
+
+int vfs_poisson_destroy(vfs_ctx *c)
+{
+    Poisson *ps = c->pois;
+    if (!ps) return 0;
+    hipFree(ps->gid); hipFree(ps->blocksum); hipFree(ps->total); hipFree(ps->coef); hipFree(ps->coef32);
+    hipFree(ps->rowflag);
+    for (auto &l : ps->lv) free_level(l);
+    hipFree(ps->cinv64); hipFree(ps->cinv32); hipFree(ps->cidx);
+    hipFree(ps->V64); hipFree(ps->V32); hipFree(ps->w64); hipFree(ps->z64); hipFree(ps->r64); hipFree(ps->x64);
+    hipFree(ps->bb64); hipFree(ps->e64); hipFree(ps->w32); hipFree(ps->z32); hipFree(ps->x32); hipFree(ps->bb32);
+    hipFree(ps->dots);
+    if (ps->hdots) hipHostFree(ps->hdots);
+    delete ps;
+    c->pois = nullptr;
+    return 0;
+}
+
+#define DMALLOC(ptr, bytes)                                                         \
+    do {                                                                            \
+        if (hipMalloc((void **)&(ptr), (bytes) ? (bytes) : 8) != hipSuccess) {      \
+            vfs_set_error("hipMalloc failed in the Poisson set-up");               \
+            return -1;                                                              \
+        }                                                                           \
+        hipMemsetAsync((ptr), 0, (bytes) ? (bytes) : 8, c->stream);                 \
+    } while (0)
+
+int vfs_poisson_alloc(vfs_ctx *c)
+{
+    if (c->pois) vfs_poisson_destroy(c);
+    Poisson *ps = new Poisson();
+    c->pois = ps;
+    const size_t P = (size_t)c->d.P;
+    DMALLOC(ps->gid, P * sizeof(double));
+    int nb = (int)grid_for((long long)P, 1024);
+    DMALLOC(ps->blocksum, nb * sizeof(int));
+    DMALLOC(ps->total, sizeof(int));
+    DMALLOC(ps->coef, 19 * P * sizeof(double));
+    DMALLOC(ps->coef32, 19 * P * sizeof(float));
+    DMALLOC(ps->rowflag, P);
+    ps->omega = 0.8;
+    ps->alpha = 1.0;
+    ps->m = 40;
+    return 0;
+}
+
+static int alloc_level(vfs_ctx *c, MGLevel &l)
+{
+    const size_t n = (size_t)l.n;
+    DMALLOC(l.A64, (size_t)l.nslot * n * sizeof(double));
+    DMALLOC(l.A32, (size_t)l.nslot * n * sizeof(float));
+    DMALLOC(l.tl64, n * sizeof(double)); DMALLOC(l.tu64, n * sizeof(double)); DMALLOC(l.ti64, n * sizeof(double));
+    DMALLOC(l.tl32, n * sizeof(float)); DMALLOC(l.tu32, n * sizeof(float)); DMALLOC(l.ti32, n * sizeof(float));
+    DMALLOC(l.b64, n * sizeof(double)); DMALLOC(l.x64, n * sizeof(double));
+    DMALLOC(l.y64, n * sizeof(double)); DMALLOC(l.d64, n * sizeof(double));
+    DMALLOC(l.b32, n * sizeof(float)); DMALLOC(l.x32, n * sizeof(float));
+    DMALLOC(l.y32, n * sizeof(float)); DMALLOC(l.d32, n * sizeof(float));
+    return 0;
+}
+
+// Gauss-Jordan inverse with partial pivoting (host, n <= a few hundred)
+static int invert_dense(std::vector<double> &a, int n)
+{
+    std::vector<double> inv((size_t)n * n, 0.0);
+    for (int q = 0; q < n; q++) inv[(size_t)q * n + q] = 1.0;
+    for (int col = 0; col < n; col++) {
+        int piv = col;
+        double best = fabs(a[(size_t)col * n + col]);
+        for (int r = col + 1; r < n; r++)
+            if (fabs(a[(size_t)r * n + col]) > best) best = fabs(a[(size_t)r * n + col]), piv = r;
+        if (best == 0.0) return -1;
+        if (piv != col)
+            for (int q = 0; q < n; q++) {
+                std::swap(a[(size_t)piv * n + q], a[(size_t)col * n + q]);
+                std::swap(inv[(size_t)piv * n + q], inv[(size_t)col * n + q]);
+            }
+        double s = 1.0 / a[(size_t)col * n + col];
+        for (int q = 0; q < n; q++) { a[(size_t)col * n + q] *= s; inv[(size_t)col * n + q] *= s; }
+        for (int r = 0; r < n; r++) {
+            if (r == col) continue;
+            double f = a[(size_t)r * n + col];
+            if (f == 0.0) continue;
+            for (int q = 0; q < n; q++) {
+                a[(size_t)r * n + q] -= f * a[(size_t)col * n + q];
+                inv[(size_t)r * n + q] -= f * inv[(size_t)col * n + q];
+            }
+        }
+    }
+    a.swap(inv);
+    return 0;
+}
+
+int vfs_mg_setup(vfs_ctx *c)
+{
+    Poisson *ps = c->pois;
+    const Dims d = c->d;
+    ps->lv.clear();
+    MGLevel l0;
+    std::memset(&l0, 0, sizeof(l0));
+    l0.nx = d.mx - 2; l0.ny = d.my - 2; l0.nz = d.mz - 2;
+    l0.n = (long long)l0.nx * l0.ny * l0.nz;
+    l0.per_i = d.ip; l0.per_j = d.jp; l0.per_k = d.kp;
+    l0.nslot = 19;
+    if (alloc_level(c, l0)) return -1;
+    hipLaunchKernelGGL(k_gather_level0, dim3(grid_for(l0.n, 256)), dim3(256), 0, c->stream, d, lv_of(l0), ps->coef, l0.A64, l0.A32);
+    VFS_CHECK_LAUNCH();
+    ps->lv.push_back(l0);
+    const long long NCOARSE = 600;
+    while (ps->lv.back().n > NCOARSE) {
+        MGLevel &f = ps->lv.back();
+        MGLevel cl;
+        std::memset(&cl, 0, sizeof(cl));
+        cl.nx = f.nx > 1 ? (f.nx + 1) / 2 : 1;
+        cl.ny = f.ny > 1 ? (f.ny + 1) / 2 : 1;
+        cl.nz = f.nz > 1 ? (f.nz + 1) / 2 : 1;
+        cl.n = (long long)cl.nx * cl.ny * cl.nz;
+        cl.per_i = f.per_i; cl.per_j = f.per_j; cl.per_k = f.per_k;
+        cl.nslot = 27;
+        if (cl.n == f.n) break;
+        if (alloc_level(c, cl)) return -1;
+        LV F = lv_of(f), C = lv_of(cl);
+        hipLaunchKernelGGL(k_galerkin, dim3(grid_for(cl.n, 128)), dim3(128), 0, c->stream, F, C, f.A64, cl.A64, cl.A32);
+        VFS_CHECK_LAUNCH();
+        ps->lv.push_back(cl);
+    }
+    for (auto &l : ps->lv) {
+        LV L = lv_of(l);
+        hipLaunchKernelGGL(k_line_factors, dim3(grid_for((long long)l.nx * l.nz, 128)), dim3(128), 0, c->stream, L,
+                           l.A64, l.tl64, l.tu64, l.ti64, l.tl32, l.tu32, l.ti32);
+        VFS_CHECK_LAUNCH();
+    }
+    // dense coarsest operator: regularised by 1 1^T / n so the singular
+    // Neumann/periodic system has the mean-free solution
+    MGLevel &cl = ps->lv.back();
+    std::vector<double> A((size_t)cl.nslot * cl.n);
+    VFS_CHECK_HIP(hipMemcpyAsync(A.data(), cl.A64, A.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    std::vector<long long> list;
+    std::vector<int> pos(cl.n, -1);
+    const int center = cl.nslot == 19 ? 0 : 13;
+    for (long long q = 0; q < cl.n; q++)
+        if (A[(size_t)center * cl.n + q] != 0.0) { pos[q] = (int)list.size(); list.push_back(q); }
+    int nc = (int)list.size();
+    VFS_REQUIRE(nc > 0, "no Poisson unknowns");
+    std::vector<double> M((size_t)nc * nc, 1.0 / nc);
+    LV L = lv_of(cl);
+    for (int r = 0; r < nc; r++) {
+        long long q = list[r];
+        int i = (int)(q % L.nx), j = (int)((q / L.nx) % L.ny), k = (int)(q / ((long long)L.nx * L.ny));
+        for (int s = 0; s < cl.nslot; s++) {
+            double a = A[(size_t)s * cl.n + q];
+            if (a == 0.0) continue;
+            int di, dj, dk;
+            if (cl.nslot == 19) {
+                static const int t19[19][3] = {{0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+                                               {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}, {0, 1, 1}, {0, 1, -1}, {0, -1, 1},
+                                               {0, -1, -1}, {1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+                di = t19[s][0]; dj = t19[s][1]; dk = t19[s][2];
+            } else { dk = s / 9 - 1; dj = (s / 3) % 3 - 1; di = s % 3 - 1; }
+            int ni = i + di, nj = j + dj, nk = k + dk;
+            auto wrap = [](int &v, int n, int per) -> bool {
+                if (v < 0) { if (!per) return false; v += n; } else if (v >= n) { if (!per) return false; v -= n; }
+                return true;
+            };
+            if (!wrap(ni, L.nx, L.pi) || !wrap(nj, L.ny, L.pj) || !wrap(nk, L.nz, L.pk)) continue;
+            long long nbq = ((long long)nk * L.ny + nj) * L.nx + ni;
+            if (pos[nbq] < 0) continue;
+            M[(size_t)r * nc + pos[nbq]] += a;
+        }
+    }
+    if (invert_dense(M, nc)) { vfs_set_error("singular coarsest Poisson operator"); return -1; }
+    ps->nc = nc;
+    DMALLOC(ps->cinv64, (size_t)nc * nc * sizeof(double));
+    DMALLOC(ps->cinv32, (size_t)nc * nc * sizeof(float));
+    DMALLOC(ps->cidx, (size_t)nc * sizeof(long long));
+    std::vector<float> M32(M.begin(), M.end());
+    VFS_CHECK_HIP(hipMemcpy(ps->cinv64, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+    VFS_CHECK_HIP(hipMemcpy(ps->cinv32, M32.data(), M32.size() * sizeof(float), hipMemcpyHostToDevice));
+    VFS_CHECK_HIP(hipMemcpy(ps->cidx, list.data(), list.size() * sizeof(long long), hipMemcpyHostToDevice));
+    // Krylov space
+    const size_t n0 = (size_t)ps->lv[0].n;
+    DMALLOC(ps->V64, (size_t)(ps->m + 1) * n0 * sizeof(double));
+    DMALLOC(ps->V32, (size_t)(ps->m + 1) * n0 * sizeof(float));
+    DMALLOC(ps->w64, n0 * sizeof(double)); DMALLOC(ps->z64, n0 * sizeof(double)); DMALLOC(ps->r64, n0 * sizeof(double));
+    DMALLOC(ps->x64, n0 * sizeof(double)); DMALLOC(ps->bb64, n0 * sizeof(double)); DMALLOC(ps->e64, n0 * sizeof(double));
+    DMALLOC(ps->w32, n0 * sizeof(float)); DMALLOC(ps->z32, n0 * sizeof(float)); DMALLOC(ps->x32, n0 * sizeof(float));
+    DMALLOC(ps->bb32, n0 * sizeof(float));
+    DMALLOC(ps->dots, 256 * sizeof(double));
+    VFS_CHECK_HIP(hipHostMalloc((void **)&ps->hdots, 256 * sizeof(double), hipHostMallocDefault));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// one V(1,1)-cycle: x_l = M_l^{-1} b_l (level vectors b, x of each level)
+template <typename T>
+static int vcycle(vfs_ctx *c, int l)
+{
+    Poisson *ps = c->pois;
+    MGLevel &L = ps->lv[l];
+    LV Lv = lv_of(L);
+    const T omega = (T)ps->omega, alpha = (T)ps->alpha;
+    T *b = vec_of<T>(L, 0), *x = vec_of<T>(L, 1), *y = vec_of<T>(L, 2), *dd = vec_of<T>(L, 3);
+    const T *A = A_of<T>(L);
+    if (l == (int)ps->lv.size() - 1) {
+        const T *cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
+        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), 0, c->stream, L.n, ps->nc, ps->cidx, cinv, b, x);
+        VFS_CHECK_LAUNCH();
+        return 0;
+    }
+    const T *tl, *tu, *ti;
+    fac_of<T>(L, tl, tu, ti);
+    unsigned gl = grid_for((long long)L.nx * L.nz, 64);
+    // pre-smooth from a zero guess
+    hipLaunchKernelGGL(k_line_smooth<T>, dim3(gl), dim3(64), 0, c->stream, Lv, A, tl, tu, ti, b, (const T *)nullptr,
+                       (const T *)nullptr, Lv, T(0), omega, dd, x);
+    VFS_CHECK_LAUNCH();
+    MGLevel &C = ps->lv[l + 1];
+    LV Cv = lv_of(C);
+    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(grid_for(C.n, 128)), dim3(128), 0, c->stream, Lv, Cv, A, x, b, vec_of<T>(C, 0));
+    VFS_CHECK_LAUNCH();
+    if (vcycle<T>(c, l + 1)) return -1;
+    // post-smooth on x + alpha*P e, result in y, then swap
+    hipLaunchKernelGGL(k_line_smooth<T>, dim3(gl), dim3(64), 0, c->stream, Lv, A, tl, tu, ti, b, (const T *)x,
+                       (const T *)vec_of<T>(C, 1), Cv, alpha, omega, dd, y);
+    VFS_CHECK_LAUNCH();
+    if (std::is_same<T, double>::value) std::swap(L.x64, L.y64);
+    else std::swap(L.x32, L.y32);
+    return 0;
+}
+
+template <typename T> static T *kv(Poisson *ps, int w);
+template <> double *kv<double>(Poisson *ps, int w) { return w == 0 ? ps->V64 : w == 1 ? ps->w64 : w == 2 ? ps->z64 : ps->x64; }
+template <> float *kv<float>(Poisson *ps, int w) { return w == 0 ? ps->V32 : w == 1 ? ps->w32 : w == 2 ? ps->z32 : ps->x32; }
+
+static int multidot_finish(vfs_ctx *c, int nv, int nb, double *dst)
+{
+    return vfs_reduce_finish(c, nb, nv, 0, dst);
+}
+
+// Right-preconditioned restarted GMRES in precision T on A_T e = r, started
+// from e = 0, where r (level-0 compact, precision T) is in V[0]'s slot
+// (unnormalised in ps->bb*).  Stops when the estimated residual drops below
+// tol * ||r|| or after maxit iterations.  Result accumulated into kv(...,3).
+template <typename T>
+static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, double *est)
+{
+    Poisson *ps = c->pois;
+    MGLevel &L0 = ps->lv[0];
+    LV Lv = lv_of(L0);
+    const long long n = L0.n;
+    const int m = ps->m;
+    T *V = kv<T>(ps, 0), *w = kv<T>(ps, 1), *z = kv<T>(ps, 2), *x = kv<T>(ps, 3);
+    const T *A = A_of<T>(L0);
+    unsigned gb = grid_for(n, 256);
+    int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
+    double *dres = c->red + VFS_RED_SLOTS * VFS_RED_BLOCKS + 16;   // device results
+    VFS_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), c->stream));
+    // r0 = rhs (x = 0)
+    hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, rhs, n, 1, rhs, c->red);
+    VFS_CHECK_LAUNCH();
+    if (multidot_finish(c, 1, nbr, dres)) return -1;
+    VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    double bnorm = sqrt(ps->hdots[0]);
+    int it = 0;
+    if (bnorm == 0.0) { *iters = 0; *est = 0; return 0; }
+    std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m);
+    const T *cur_rhs = rhs;
+    double res = 1.0;
+    bool first = true;
+    while (it < maxit) {
+        // v0 = r / ||r||, r = rhs - A x
+        if (first) {
+            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, cur_rhs, dres, V);
+            VFS_CHECK_LAUNCH();
+            s.assign(m + 1, 0.0);
+            s[0] = bnorm;
+        } else {
+            hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->stream, Lv, A, x, w);
+            VFS_CHECK_LAUNCH();
+            // w = rhs - A x  via w = -(A x - rhs)
+            hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, dres + 8, 1.0);
+            VFS_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_multiaxpy<T>, dim3(gb), dim3(256), 0, c->stream, n, rhs, n, 1, dres + 8, w);
+            VFS_CHECK_LAUNCH();
+            // now w = A x - rhs; residual norm
+            hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
+            VFS_CHECK_LAUNCH();
+            if (multidot_finish(c, 1, nbr, dres)) return -1;
+            VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+            double beta = sqrt(ps->hdots[0]);
+            res = beta / bnorm;
+            if (res <= tol) break;
+            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, w, dres, V);
+            VFS_CHECK_LAUNCH();
+            s.assign(m + 1, 0.0);
+            s[0] = -beta;   // v0 = (A x - rhs)/beta = -r/beta
+        }
+        first = false;
+        int kk = 0;
+        for (; kk < m && it < maxit; kk++, it++) {
+            // z = M^{-1} v_kk ; w = A z
+            T *vk = V + (size_t)kk * n;
+            VFS_CHECK_HIP(hipMemcpyAsync(vec_of<T>(L0, 0), vk, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+            if (vcycle<T>(c, 0)) return -1;
+            hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->stream, Lv, A, (const T *)vec_of<T>(L0, 1), w);
+            VFS_CHECK_LAUNCH();
+            // classical Gram-Schmidt, twice
+            for (int pass = 0; pass < 2; pass++) {
+                hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, V, n, kk + 1, w, c->red);
+                VFS_CHECK_LAUNCH();
+                if (multidot_finish(c, kk + 1, nbr, ps->dots + 64 * pass)) return -1;
+                hipLaunchKernelGGL(k_multiaxpy<T>, dim3(gb), dim3(256), 0, c->stream, n, V, n, kk + 1, ps->dots + 64 * pass, w);
+                VFS_CHECK_LAUNCH();
+            }
+            hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
+            VFS_CHECK_LAUNCH();
+            if (multidot_finish(c, 1, nbr, ps->dots + 128)) return -1;
+            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, w, ps->dots + 128, V + (size_t)(kk + 1) * n);
+            VFS_CHECK_LAUNCH();
+            VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, 129 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+            for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = ps->hdots[q] + ps->hdots[64 + q];
+            H[(size_t)(kk + 1) * m + kk] = sqrt(ps->hdots[128]);
+            for (int q = 0; q < kk; q++) {
+                double t0 = cs[q] * H[(size_t)q * m + kk] + sn[q] * H[(size_t)(q + 1) * m + kk];
+                H[(size_t)(q + 1) * m + kk] = -sn[q] * H[(size_t)q * m + kk] + cs[q] * H[(size_t)(q + 1) * m + kk];
+                H[(size_t)q * m + kk] = t0;
+            }
+            double a = H[(size_t)kk * m + kk], bb = H[(size_t)(kk + 1) * m + kk];
+            double den = sqrt(a * a + bb * bb);
+            cs[kk] = den > 0 ? a / den : 1.0;
+            sn[kk] = den > 0 ? bb / den : 0.0;
+            H[(size_t)kk * m + kk] = den;
+            H[(size_t)(kk + 1) * m + kk] = 0;
+            s[kk + 1] = -sn[kk] * s[kk];
+            s[kk] = cs[kk] * s[kk];
+            res = fabs(s[kk + 1]) / bnorm;
+            if (res <= tol) { kk++; it++; break; }
+        }
+        // x += M^{-1} V y
+        for (int q = kk - 1; q >= 0; q--) {
+            double t0 = s[q];
+            for (int p = q + 1; p < kk; p++) t0 -= H[(size_t)q * m + p] * y[p];
+            y[q] = t0 / H[(size_t)q * m + q];
+        }
+        for (int q = 0; q < kk; q++) ps->hdots[192 + q] = y[q];   // pinned staging
+        VFS_CHECK_HIP(hipMemcpyAsync(ps->dots + 192, ps->hdots + 192, kk * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        hipLaunchKernelGGL(k_combine<T>, dim3(gb), dim3(256), 0, c->stream, n, V, n, kk, ps->dots + 192, vec_of<T>(L0, 0));
+        VFS_CHECK_LAUNCH();
+        if (vcycle<T>(c, 0)) return -1;
+        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(gb), dim3(256), 0, c->stream, n, (const T *)vec_of<T>(L0, 1), x);
+        VFS_CHECK_LAUNCH();
+        if (res <= tol) break;
+    }
+    *iters = it;
+    *est = res;
+    return 0;
+}
+
+static int norm2_64(vfs_ctx *c, const double *v, long long n, double *dev_out, double *host_out)
+{
+    int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
+    hipLaunchKernelGGL(k_multidot<double>, dim3(nbr), dim3(256), 0, c->stream, n, v, n, 1, v, c->red);
+    VFS_CHECK_LAUNCH();
+    if (vfs_reduce_finish(c, nbr, 1, 0, dev_out)) return -1;
+    if (host_out) return vfs_read_scalars(c, dev_out, host_out, 1);
+    return 0;
+}
+
+extern "C" int vfs_poisson_solve(vfs_ctx *c, const double *b, double *phi, double rtol, int precision, int maxit,
+                                 int *iters, double *relres)
+{
+    VFS_REQUIRE(c->pois && c->pois->ready, "vfs_poisson_setup has not been called");
+    VFS_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 + FP64 refinement)");
+    Poisson *ps = c->pois;
+    const Dims d = c->d;
+    MGLevel &L0 = ps->lv[0];
+    LV Lv = lv_of(L0);
+    const long long n = L0.n;
+    unsigned gb = grid_for(n, 256);
+    double *dres = c->red + VFS_RED_SLOTS * VFS_RED_BLOCKS + 32;
+    // b and the initial guess on the compact layout
+    hipLaunchKernelGGL(k_da_to_c, dim3(gb), dim3(256), 0, c->stream, d, Lv, b, ps->rowflag, ps->bb64);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_da_to_c, dim3(gb), dim3(256), 0, c->stream, d, Lv, phi, ps->rowflag, ps->e64);
+    VFS_CHECK_LAUNCH();
+    double bn2;
+    if (norm2_64(c, ps->bb64, n, dres, &bn2)) return -1;
+    double bnorm = sqrt(bn2);
+    int total = 0;
+    double rel = 0;
+    if (bnorm == 0.0) bnorm = 1.0;
+    for (int outer = 0; outer < 50; outer++) {
+        // FP64 defect r = b - A x (FP64 operator)
+        hipLaunchKernelGGL(k_resid64, dim3(gb), dim3(256), 0, c->stream, Lv, L0.A64, ps->e64, ps->bb64, ps->r64);
+        VFS_CHECK_LAUNCH();
+        double rn2;
+        if (norm2_64(c, ps->r64, n, dres, &rn2)) return -1;
+        rel = sqrt(rn2) / bnorm;
+        if (rel <= rtol || total >= maxit || rn2 == 0.0) break;
+        int it = 0;
+        double est = 0;
+        if (precision == 0) {
+            // FP64 GMRES on the defect to the remaining reduction
+            double inner = std::max(rtol / rel, 1e-12) * 0.5;
+            if (gmres<double>(c, ps->r64, inner, maxit - total, &it, &est)) return -1;
+            hipLaunchKernelGGL(k_add_scaled<double>, dim3(gb), dim3(256), 0, c->stream, n, (const double *)ps->x64,
+                               (const double *)nullptr, ps->e64);
+            VFS_CHECK_LAUNCH();
+        } else {
+            // FP32 inner solve on the normalised defect, FP64 update
+            hipLaunchKernelGGL(k_cast_scale<float>, dim3(gb), dim3(256), 0, c->stream, n, (const double *)ps->r64,
+                               (const double *)dres, ps->bb32);
+            VFS_CHECK_LAUNCH();
+            double inner = std::max(rtol / rel * 0.5, 2e-5);
+            if (gmres<float>(c, ps->bb32, inner, maxit - total, &it, &est)) return -1;
+            hipLaunchKernelGGL(k_add_scaled<float>, dim3(gb), dim3(256), 0, c->stream, n, (const float *)ps->x32,
+                               (const double *)dres, ps->e64);
+            VFS_CHECK_LAUNCH();
+        }
+        total += it;
+        if (it == 0) break;
+    }
+    // Remove_Nullspace over nvert < 0.1 cells, then Convert_Phi2_Phi
+    int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
+    hipLaunchKernelGGL(k_nullsum, dim3(nbr), dim3(256), 0, c->stream, d, Lv, ps->e64, c->nvert, c->red);
+    VFS_CHECK_LAUNCH();
+    if (vfs_reduce_finish(c, nbr, 2, 0, dres + 4)) return -1;
+    hipLaunchKernelGGL(k_neg_mean, dim3(1), dim3(1), 0, c->stream, dres + 4);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_c_to_da, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, Lv, ps->e64, c->nvert, dres + 4, phi);
+    VFS_CHECK_LAUNCH();
+    if (vfs_periodic_sca(c, phi)) return -1;
+    if (iters) *iters = total;
+    if (relres) *relres = rel;
+    return 0;
+}

@@ -1,0 +1,307 @@
+// Per-point stencil bodies of the momentum right-hand side, shared by the
+// reference-structured passes and the fused kernel.  Each function keeps the
+// reference's expression and operation order (cited per function).
+#pragma once
+#include "vfs_internal.h"
+
+#define VFS_SOLID 0.1 // k-omega.c:11
+
+// face-centred velocity gradients (k-omega.c:48-278)
+template <typename Fetch>
+__device__ __forceinline__ void du_face(int dir, const Fetch &U, const double *__restrict__ nvert, const Dims &d,
+                                        int i, int j, int k, double dc[3], double de[3], double dz[3])
+{
+#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+    if (dir == 0) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) dc[q] = U(k, j, i + 1, q) - U(k, j, i, q);
+        if (NV(k, j + 1, i) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k, j, i + 1, q) + U(k, j, i, q) - U(k, j - 1, i + 1, q) - U(k, j - 1, i, q)) * 0.5;
+        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k, j - 1, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k, j + 1, i + 1, q) + U(k, j + 1, i, q) - U(k, j, i + 1, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k, j + 1, i + 1, q) + U(k, j + 1, i, q) - U(k, j - 1, i + 1, q) - U(k, j - 1, i, q)) * 0.25;
+        }
+        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k, j, i + 1, q) + U(k, j, i, q) - U(k - 1, j, i + 1, q) - U(k - 1, j, i, q)) * 0.5;
+        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j, i + 1, q) + U(k + 1, j, i, q) - U(k, j, i + 1, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j, i + 1, q) + U(k + 1, j, i, q) - U(k - 1, j, i + 1, q) - U(k - 1, j, i, q)) * 0.25;
+        }
+    } else if (dir == 1) {
+        if (NV(k, j, i + 1) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i, q) + U(k, j, i, q) - U(k, j + 1, i - 1, q) - U(k, j, i - 1, q)) * 0.5;
+        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k, j + 1, i - 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i + 1, q) + U(k, j, i + 1, q) - U(k, j + 1, i, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i + 1, q) + U(k, j, i + 1, q) - U(k, j + 1, i - 1, q) - U(k, j, i - 1, q)) * 0.25;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) de[q] = U(k, j + 1, i, q) - U(k, j, i, q);
+        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k, j + 1, i, q) + U(k, j, i, q) - U(k - 1, j + 1, i, q) - U(k - 1, j, i, q)) * 0.5;
+        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j + 1, i) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j + 1, i, q) + U(k + 1, j, i, q) - U(k, j + 1, i, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j + 1, i, q) + U(k + 1, j, i, q) - U(k - 1, j + 1, i, q) - U(k - 1, j, i, q)) * 0.25;
+        }
+    } else {
+        if (NV(k, j, i + 1) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i, q) + U(k, j, i, q) - U(k + 1, j, i - 1, q) - U(k, j, i - 1, q)) * 0.5;
+        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k + 1, j, i - 1) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i + 1, q) + U(k, j, i + 1, q) - U(k + 1, j, i, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i + 1, q) + U(k, j, i + 1, q) - U(k + 1, j, i - 1, q) - U(k, j, i - 1, q)) * 0.25;
+        }
+        if (NV(k, j + 1, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j, i, q) + U(k, j, i, q) - U(k + 1, j - 1, i, q) - U(k, j - 1, i, q)) * 0.5;
+        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k + 1, j - 1, i) > VFS_SOLID) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j + 1, i, q) + U(k, j + 1, i, q) - U(k + 1, j, i, q) - U(k, j, i, q)) * 0.5;
+        } else {
+#pragma unroll
+            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j + 1, i, q) + U(k, j + 1, i, q) - U(k + 1, j - 1, i, q) - U(k, j - 1, i, q)) * 0.25;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) dz[q] = U(k + 1, j, i, q) - U(k, j, i, q);
+    }
+#undef NV
+}
+
+// cell-centred gradient for the LES model (k-omega.c:305-423, k==0 quirk at 391)
+__device__ __forceinline__ void du_center(const Dims &d, int i, int j, int k, const double *__restrict__ ucat,
+                                          const double *__restrict__ nvert, double dc[3], double de[3], double dz[3])
+{
+#define U3(kk, jj, ii, q) ucat[3 * IX(d, kk, jj, ii) + (q)]
+#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+    if (NV(k, j, i + 1) > VFS_SOLID || (!d.ip && i == d.mx - 2)) {
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i, q) - U3(k, j, i - 1, q));
+    } else if (NV(k, j, i - 1) > VFS_SOLID || (!d.ip && i == 1)) {
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i + 1, q) - U3(k, j, i, q));
+    } else {
+        for (int q = 0; q < 3; q++) dc[q] = (U3(k, j, i + 1, q) - U3(k, j, i - 1, q)) * 0.5;
+    }
+    if (NV(k, j + 1, i) > VFS_SOLID || (!d.jp && j == d.my - 2)) {
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j, i, q) - U3(k, j - 1, i, q));
+    } else if (NV(k, j - 1, i) > VFS_SOLID || (!d.jp && j == 1)) {
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i, q) - U3(k, j, i, q));
+    } else {
+        for (int q = 0; q < 3; q++) de[q] = (U3(k, j + 1, i, q) - U3(k, j - 1, i, q)) * 0.5;
+    }
+    if (NV(k + 1, j, i) > VFS_SOLID || (!d.kp && k == d.mz - 2)) {
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k, j, i, q) - U3(k - 1, j, i, q));
+    } else if (NV(k - 1, j, i) > VFS_SOLID || (!d.kp && k == 0)) {
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i, q) - U3(k, j, i, q));
+    } else {
+        for (int q = 0; q < 3; q++) dz[q] = (U3(k + 1, j, i, q) - U3(k - 1, j, i, q)) * 0.5;
+    }
+#undef U3
+#undef NV
+}
+
+struct GlobalVec {
+    const double *__restrict__ p;
+    Dims d;
+    __device__ __forceinline__ double operator()(int k, int j, int i, int q) const { return p[3 * IX(d, k, j, i) + q]; }
+};
+
+// Convective (div) and viscous (visc) flux through face `dir` of point
+// (k,j,i): momentum.c:672-931 (i), 934-1166 (j), 1169-1409 (k).
+template <int DIR>
+__device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const double *__restrict__ nvert,
+                                          const double *__restrict__ ucont, const double *__restrict__ ucat,
+                                          const double *__restrict__ nu_t, int i, int j, int k, double D[3], double V[3])
+{
+    const long long id = IX(d, k, j, i);
+    const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
+    const double *fe = DIR == 0 ? g.ieta : DIR == 1 ? g.jeta : g.keta;
+    const double *fz = DIR == 0 ? g.izet : DIR == 1 ? g.jzet : g.kzet;
+    const double *faj = DIR == 0 ? g.iaj : DIR == 1 ? g.jaj : g.kaj;
+    const double ajc = faj[id];
+    const double csi0 = fc[3 * id], csi1 = fc[3 * id + 1], csi2 = fc[3 * id + 2];
+    const double eta0 = fe[3 * id], eta1 = fe[3 * id + 1], eta2 = fe[3 * id + 2];
+    const double zet0 = fz[3 * id], zet1 = fz[3 * id + 1], zet2 = fz[3 * id + 2];
+    GlobalVec U{ucat, d};
+    double dc[3], de[3], dz[3];
+    du_face(DIR, U, nvert, d, i, j, k, dc, de, dz);
+    double g11, g21, g31, n0, n1, n2;
+    if (DIR == 0) {
+        g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2;
+        g21 = eta0 * csi0 + eta1 * csi1 + eta2 * csi2;
+        g31 = zet0 * csi0 + zet1 * csi1 + zet2 * csi2;
+        n0 = csi0, n1 = csi1, n2 = csi2;
+    } else if (DIR == 1) {
+        g11 = csi0 * eta0 + csi1 * eta1 + csi2 * eta2;
+        g21 = eta0 * eta0 + eta1 * eta1 + eta2 * eta2;
+        g31 = zet0 * eta0 + zet1 * eta1 + zet2 * eta2;
+        n0 = eta0, n1 = eta1, n2 = eta2;
+    } else {
+        g11 = csi0 * zet0 + csi1 * zet1 + csi2 * zet2;
+        g21 = eta0 * zet0 + eta1 * zet1 + eta2 * zet2;
+        g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
+        n0 = zet0, n1 = zet1, n2 = zet2;
+    }
+    const double r11 = dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0;
+    const double r21 = dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0;
+    const double r31 = dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0;
+    const double r12 = dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1;
+    const double r22 = dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1;
+    const double r32 = dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1;
+    const double r13 = dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2;
+    const double r23 = dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2;
+    const double r33 = dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2;
+
+    const int n = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int m = DIR == 0 ? d.mx : DIR == 1 ? d.my : d.mz;
+    const int per = DIR == 0 ? d.ip : DIR == 1 ? d.jp : d.kp;
+    const long long st = DIR == 0 ? 1 : DIR == 1 ? (long long)d.mx : (long long)d.mx * d.my;
+    const long long base = id - (long long)n * st;   // point with index 0 along DIR
+#define NB(tt) nvert[base + (long long)(tt) * st]
+#define UB(tt, q) ucat[3 * (base + (long long)(tt) * st) + (q)]
+#define UC(tt) ucont[3 * (base + (long long)(tt) * st) + DIR]
+    int L = n - 1, R = n + 2;
+    if (n == 0 || n == m - 2) {
+        if (per) L = m - 3, R = 2;
+        else L = n, R = n + 1;
+    } else if (NB(L) + NB(R) > 0.1) {
+        L = n, R = n + 1;
+    }
+    if (d.second_order) L = n, R = n + 1;
+    const double ucon_f = UC(n);
+    double ucon = ucon_f;
+    if (per && (n == 0 || n == m - 2)) ucon = UC(m - 2);
+    const double up = -0.5 * (ucon + fabs(ucon));
+    const double um = -0.5 * (ucon - fabs(ucon));
+    if (d.immersed && n != m - 2 && NB(n) > 0.1) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            D[q] = um * (0.125 * (-UB(n + 2, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
+                   up * (0.125 * (-UB(n, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+    } else if (d.immersed && n != 0 && NB(n + 1) > 0.1) {
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+            D[q] = um * (0.125 * (-UB(n + 1, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
+                   up * (0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+    } else if (d.second_order) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
+    } else {
+#pragma unroll
+        for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
+    }
+    if (NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = 0;
+
+    const double nu = 1. / d.ren;
+    double v0, v1, v2;
+    if (d.les) {
+        double nut;
+        if ((n == 0 && !per) || NB(n) > 0.1) nut = nu_t[base + (long long)(n + 1) * st];
+        else if ((n == m - 2 && !per) || NB(n + 1) > 0.1) nut = nu_t[id];
+        else nut = 0.5 * (nu_t[id] + nu_t[base + (long long)(n + 1) * st]);
+        v0 = (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nut);
+        v1 = (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nut);
+        v2 = (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nut);
+    } else {
+        v0 = v1 = v2 = 0;
+    }
+    v0 += (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nu);
+    v1 += (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nu);
+    v2 += (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
+    V[0] = v0; V[1] = v1; V[2] = v2;
+#undef NB
+#undef UB
+#undef UC
+}
+
+// index of a face along a periodic direction as stored: face 0 is a copy of
+// face m-2 (momentum.c:1463-1504)
+__device__ __forceinline__ int wrap_face(int f, int m, int per) { return (per && f == 0) ? m - 2 : f; }
+
+// fp of cell (k,j,i) from the face arrays (momentum.c:1506-1636).
+// face: slot-major SoA, slot s = 3*kind + comp with kind 0..2 div1..3, 3..5 visc1..3.
+__device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict__ nvert,
+                                        const double *__restrict__ face, int i, int j, int k, double f[3])
+{
+    const long long P = d.P;
+    const int ip = d.ip, jp = d.jp, kp = d.kp, mx = d.mx, my = d.my, mz = d.mz;
+#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+#define FI(kind, q, kk, jj, ii) face[(long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii)]
+    const int im = wrap_face(i - 1, mx, ip), jm = wrap_face(j - 1, my, jp), km = wrap_face(k - 1, mz, kp);
+    int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
+    double den_i = 3., den_j = 3., den_k = 3.;
+    if (i == 1) { if (ip) iL = mx - 3; else iR = i, iL = i - 1, den_i = 1.; }
+    else if (i == 2 || i == mx - 3) { if (!ip) iR = i, iL = i - 1, den_i = 1.; }
+    else if (i == mx - 2) { if (ip) iR = 1; else iR = i, iL = i - 1, den_i = 1.; }
+    if (j == 1) { if (jp) jL = my - 3; else jR = j, jL = j - 1, den_j = 1.; }
+    else if (j == 2 || j == my - 3) { if (!jp) jR = j, jL = j - 1, den_j = 1.; }
+    else if (j == my - 2) { if (jp) jR = 1; else jR = j, jL = j - 1, den_j = 1.; }
+    if (k == 1) { if (kp) kL = mz - 3; else kR = k, kL = k - 1, den_k = 1.; }
+    else if (k == 2 || k == mz - 3) { if (!kp) kR = k, kL = k - 1, den_k = 1.; }
+    else if (k == mz - 2) { if (kp) kR = 1; else kR = k, kL = k - 1, den_k = 1.; }
+    if (NV(k, j, i - 1) + NV(k, j, i) + NV(k, j, i + 1) > 0.1) iR = i, iL = i - 1, den_i = 1.;
+    if (NV(k, j - 1, i) + NV(k, j, i) + NV(k, j + 1, i) > 0.1) jR = j, jL = j - 1, den_j = 1.;
+    if (NV(k - 1, j, i) + NV(k, j, i) + NV(k + 1, j, i) > 0.1) kR = k, kL = k - 1, den_k = 1.;
+    iL = wrap_face(iL, mx, ip); jL = wrap_face(jL, my, jp); kL = wrap_face(kL, mz, kp);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
+                     FI(2, q, k, j, i) - FI(2, q, km, j, i));
+        double r;
+        if (d.second_order) {
+            r = dv;
+        } else {
+            double d4 = 0;
+            d4 += (FI(0, q, k, j, iR) - FI(0, q, k, j, iL)) * (1. / den_i);
+            d4 += (FI(1, q, k, jR, i) - FI(1, q, k, jL, i)) * (1. / den_j);
+            d4 += (FI(2, q, kR, j, i) - FI(2, q, kL, j, i)) * (1. / den_k);
+            r = 9. / 8. * dv + -1. / 8. * d4;
+        }
+        r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) +
+              FI(5, q, k, j, i) - FI(5, q, km, j, i)) / 1.0;
+        f[q] = r;
+    }
+#undef NV
+#undef FI
+}
+
+// cell index of a periodic ghost read: cell m-1 is a copy of cell 1
+__device__ __forceinline__ int wrap_cell(int c, int m, int per) { return (per && c == m - 1) ? 1 : c; }
+
+// rhs += scale * (contravariant projection of fp) with the face masks
+// (momentum.c:1673-1693, 1795-1803)
+__device__ __forceinline__ void project_rhs(const Dims &d, const Geom &g, const double *__restrict__ nvert,
+                                            const double *__restrict__ fp, int i, int j, int k, long long id,
+                                            double scale, double r[3])
+{
+    const long long a = IX(d, k, j, wrap_cell(i + 1, d.mx, d.ip)), b = IX(d, k, wrap_cell(j + 1, d.my, d.jp), i),
+                    c = IX(d, wrap_cell(k + 1, d.mz, d.kp), j, i);
+    const long long ga = id + 1, gb = id + d.mx, gc = id + (long long)d.mx * d.my;  // metric ghosts are stored
+    const double *cs = g.csi, *et = g.eta, *ze = g.zet;
+    r[0] += scale * (0.5 * (cs[3 * id] * fp[3 * id] + cs[3 * id + 1] * fp[3 * id + 1] + cs[3 * id + 2] * fp[3 * id + 2]) +
+                     0.5 * (cs[3 * ga] * fp[3 * a] + cs[3 * ga + 1] * fp[3 * a + 1] + cs[3 * ga + 2] * fp[3 * a + 2])) * g.iaj[id];
+    r[1] += scale * (0.5 * (et[3 * id] * fp[3 * id] + et[3 * id + 1] * fp[3 * id + 1] + et[3 * id + 2] * fp[3 * id + 2]) +
+                     0.5 * (et[3 * gb] * fp[3 * b] + et[3 * gb + 1] * fp[3 * b + 1] + et[3 * gb + 2] * fp[3 * b + 2])) * g.jaj[id];
+    r[2] += scale * (0.5 * (ze[3 * id] * fp[3 * id] + ze[3 * id + 1] * fp[3 * id + 1] + ze[3 * id + 2] * fp[3 * id + 2]) +
+                     0.5 * (ze[3 * gc] * fp[3 * c] + ze[3 * gc + 1] * fp[3 * c + 1] + ze[3 * gc + 2] * fp[3 * c + 2])) * g.kaj[id];
+    const double n0 = nvert[id];
+    if (n0 + nvert[id + 1] > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
+    if (n0 + nvert[id + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
+    if (n0 + nvert[id + (long long)d.mx * d.my] > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
+}

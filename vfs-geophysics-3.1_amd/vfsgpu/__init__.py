@@ -1,0 +1,308 @@
+"""Host-side binding of libvfsgpu.so, the MI355X implementation of the
+VFS-Geophysics fractional-step hot path (momentum RHS + pressure projection).
+
+Mirrors the reference's per-step call sequence (solvers.c:562-1520):
+``Ctx`` stands for UserCtx; its methods are the C-ABI entry points declared in
+include/vfsgpu.h.  Device buffers are plain device pointers owned by
+``DeviceArray``.  There is no CPU fallback: if the HIP library is missing or no
+GPU is present, the calls fail with an exception.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(os.path.dirname(_PKG), "csrc")
+LIBPATH = os.path.join(CSRC, "libvfsgpu.so")
+
+_lib = None
+
+
+def lib():
+    """Load libvfsgpu.so (built in-tree by __graft_entry__.build())."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"{LIBPATH} is missing: build it with __graft_entry__.build()")
+        _lib = C.CDLL(LIBPATH)
+        _lib.vfs_last_error.restype = C.c_char_p
+        _lib.vfs_wl_uniform.restype = C.c_double
+        _lib.vfs_wl_uniform.argtypes = [C.c_uint64, C.c_uint64]
+    return _lib
+
+
+class VfsError(RuntimeError):
+    pass
+
+
+def _ck(rc, what):
+    if rc != 0:
+        raise VfsError(f"{what} failed ({rc}): {lib().vfs_last_error().decode()}")
+
+
+class Config(C.Structure):
+    _fields_ = [("mx", C.c_int), ("my", C.c_int), ("mz", C.c_int),
+                ("i_periodic", C.c_int), ("j_periodic", C.c_int), ("k_periodic", C.c_int),
+                ("bctype", C.c_int * 6), ("immersed", C.c_int), ("les", C.c_int),
+                ("second_order", C.c_int), ("dpdz_set", C.c_int),
+                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+                ("mean_pressure_gradient", C.c_double)]
+
+    @classmethod
+    def from_dict(cls, d):
+        c = cls()
+        for k, v in d.items():
+            if k == "bctype":
+                for n in range(6):
+                    c.bctype[n] = int(v[n])
+            elif k in dict(cls._fields_):
+                setattr(c, k, v)
+        return c
+
+
+class DeviceArray:
+    """A zero-initialised device allocation of n float64 (or float32) values."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        _ck(lib().vfs_device_malloc(C.c_size_t(self.nbytes), C.byref(p)), "vfs_device_malloc")
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _ck(lib().vfs_memcpy_htod(self.ptr, a.ctypes.data_as(C.c_void_p), C.c_size_t(self.nbytes)), "htod")
+
+    def host(self):
+        out = np.empty(self.n, self.dtype)
+        _ck(lib().vfs_memcpy_dtoh(out.ctypes.data_as(C.c_void_p), self.ptr, C.c_size_t(self.nbytes)), "dtoh")
+        return out
+
+    def copy_from(self, other):
+        _ck(lib().vfs_memcpy_dtod(self.ptr, other.ptr, C.c_size_t(self.nbytes)), "dtod")
+
+    def free(self):
+        if self.ptr:
+            lib().vfs_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(x):
+    if x is None:
+        return None
+    return x.ptr
+
+
+METRIC_NAMES = ["csi", "eta", "zet", "aj", "icsi", "ieta", "izet", "iaj",
+                "jcsi", "jeta", "jzet", "jaj", "kcsi", "keta", "kzet", "kaj", "cent"]
+SCALAR_METRICS = {"aj", "iaj", "jaj", "kaj"}
+
+
+class Ctx:
+    """One grid block: the reference's UserCtx with metrics and nvert on the GPU."""
+
+    def __init__(self, cfg: dict):
+        self.cfgd = dict(cfg)
+        self.cfg = Config.from_dict(cfg)
+        self.mx, self.my, self.mz = cfg["mx"], cfg["my"], cfg["mz"]
+        self.P = self.mx * self.my * self.mz
+        h = C.c_void_p()
+        _ck(lib().vfs_ctx_create(C.byref(self.cfg), C.byref(h)), "vfs_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().vfs_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def vec(self):
+        return DeviceArray(3 * self.P)
+
+    def sca(self):
+        return DeviceArray(self.P)
+
+    def sync(self):
+        _ck(lib().vfs_synchronize(self.h), "vfs_synchronize")
+
+    # geometry
+    def build_metrics(self, coor):
+        _ck(lib().vfs_build_metrics(self.h, coor.ptr), "vfs_build_metrics")
+
+    def metric(self, name):
+        w = METRIC_NAMES.index(name)
+        out = DeviceArray(self.P * (1 if name in SCALAR_METRICS else 3))
+        _ck(lib().vfs_get_metric(self.h, C.c_int(w), out.ptr), "vfs_get_metric")
+        return out
+
+    def set_nvert(self, nvert):
+        _ck(lib().vfs_set_nvert(self.h, nvert.ptr), "vfs_set_nvert")
+
+    def nvert(self):
+        out = self.sca()
+        _ck(lib().vfs_get_nvert(self.h, out.ptr), "vfs_get_nvert")
+        return out
+
+    def nvert_sphere(self, cx, cy, cz, r):
+        out = self.sca()
+        _ck(lib().vfs_nvert_sphere(self.h, C.c_double(cx), C.c_double(cy), C.c_double(cz), C.c_double(r), out.ptr),
+            "vfs_nvert_sphere")
+        return out
+
+    # momentum
+    def contra2cart(self, ucont, ucat):
+        _ck(lib().vfs_contra2cart(self.h, ucont.ptr, ucat.ptr), "vfs_contra2cart")
+
+    def eddy_viscosity(self, ucat, cs, nu_t):
+        _ck(lib().vfs_eddy_viscosity(self.h, ucat.ptr, C.c_double(cs), nu_t.ptr), "vfs_eddy_viscosity")
+
+    def pressure_gradient(self, p, dp):
+        _ck(lib().vfs_pressure_gradient(self.h, p.ptr, dp.ptr), "vfs_pressure_gradient")
+
+    def ib_bc(self, ucont, ucat):
+        _ck(lib().vfs_ib_bc(self.h, ucont.ptr, ucat.ptr), "vfs_ib_bc")
+
+    def formfunction2(self, ucont, ucat, nu_t, rhs, scale):
+        _ck(lib().vfs_formfunction2(self.h, ucont.ptr, ucat.ptr, _p(nu_t), rhs.ptr, C.c_double(scale)),
+            "vfs_formfunction2")
+
+    def formfunction_snes(self, U, U_o, rhs_o, dP, nu_t, lucont, ucat, rhs):
+        _ck(lib().vfs_formfunction_snes(self.h, U.ptr, U_o.ptr, rhs_o.ptr, dP.ptr, _p(nu_t), lucont.ptr, ucat.ptr,
+                                        rhs.ptr), "vfs_formfunction_snes")
+
+    def rk_stage(self, u0, ucont, ucat, nu_t, dP, alfa, rhs):
+        _ck(lib().vfs_rk_stage(self.h, u0.ptr, ucont.ptr, ucat.ptr, _p(nu_t), dP.ptr, C.c_double(alfa), rhs.ptr),
+            "vfs_rk_stage")
+
+    def min_dt(self, ucont):
+        v = C.c_double()
+        _ck(lib().vfs_min_dt(self.h, ucont.ptr, C.byref(v)), "vfs_min_dt")
+        return v.value
+
+    def rhs_jvp(self, U, ucat, nu_t, w, jw):
+        _ck(lib().vfs_rhs_jvp(self.h, U.ptr, ucat.ptr, _p(nu_t), w.ptr, jw.ptr), "vfs_rhs_jvp")
+
+    def rhs_vjp(self, U, ucat, nu_t, v, jtv):
+        _ck(lib().vfs_rhs_vjp(self.h, U.ptr, ucat.ptr, _p(nu_t), v.ptr, jtv.ptr), "vfs_rhs_vjp")
+
+    # pressure
+    def poisson_setup(self):
+        n = C.c_int()
+        _ck(lib().vfs_poisson_setup(self.h, C.byref(n)), "vfs_poisson_setup")
+        return n.value
+
+    def poisson_gid(self):
+        out = self.sca()
+        _ck(lib().vfs_poisson_gid(self.h, out.ptr), "vfs_poisson_gid")
+        return out
+
+    def poisson_coefficients(self):
+        out = DeviceArray(19 * self.P)
+        _ck(lib().vfs_poisson_coefficients(self.h, out.ptr), "vfs_poisson_coefficients")
+        return out
+
+    def poisson_apply(self, x, y):
+        _ck(lib().vfs_poisson_apply(self.h, x.ptr, y.ptr), "vfs_poisson_apply")
+
+    def volume_flux(self, ucont):
+        f, a = C.c_double(), C.c_double()
+        _ck(lib().vfs_volume_flux(self.h, ucont.ptr, C.byref(f), C.byref(a)), "vfs_volume_flux")
+        return f.value, a.value
+
+    def poisson_rhs(self, ucont, b):
+        _ck(lib().vfs_poisson_rhs(self.h, ucont.ptr, b.ptr), "vfs_poisson_rhs")
+
+    def poisson_solve(self, b, phi, rtol=1e-8, precision=0, maxit=500):
+        it, rel = C.c_int(), C.c_double()
+        _ck(lib().vfs_poisson_solve(self.h, b.ptr, phi.ptr, C.c_double(rtol), C.c_int(precision), C.c_int(maxit),
+                                    C.byref(it), C.byref(rel)), "vfs_poisson_solve")
+        return it.value, rel.value
+
+    def update_pressure(self, p, phi):
+        _ck(lib().vfs_update_pressure(self.h, p.ptr, phi.ptr), "vfs_update_pressure")
+
+    def projection(self, phi, ucont):
+        _ck(lib().vfs_projection(self.h, phi.ptr, ucont.ptr), "vfs_projection")
+
+    def divergence(self, ucont, div=None):
+        m = C.c_double()
+        _ck(lib().vfs_divergence(self.h, ucont.ptr, _p(div), C.byref(m)), "vfs_divergence")
+        return m.value
+
+
+# ---------------------------------------------------------------------------
+# synthetic workloads (host generators compiled into the library)
+# ---------------------------------------------------------------------------
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def channel_grid(IM, JM, KM, Lx, Ly, Lz, beta=0.0, amp=0.0):
+    mx, my, mz = IM + 1, JM + 1, KM + 1
+    coor = np.zeros(3 * mx * my * mz)
+    _ck(lib().vfs_wl_channel_grid(IM, JM, KM, C.c_double(Lx), C.c_double(Ly), C.c_double(Lz), C.c_double(beta),
+                                  C.c_double(amp), _dp(coor)), "vfs_wl_channel_grid")
+    return coor
+
+
+def poiseuille(cfg, Ly, noise, seed, cent, nvert, icsi, jeta, kzet):
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    per = (C.c_int * 3)(cfg["i_periodic"], cfg["j_periodic"], cfg["k_periodic"])
+    ucat = np.zeros(3 * mx * my * mz)
+    ucont = np.zeros(3 * mx * my * mz)
+    _ck(lib().vfs_wl_poiseuille(mx, my, mz, per, C.c_double(Ly), C.c_double(noise), C.c_uint64(seed), _dp(cent),
+                                _dp(nvert), _dp(icsi), _dp(jeta), _dp(kzet), _dp(ucat), _dp(ucont)), "poiseuille")
+    return ucat, ucont
+
+
+def turbulent(cfg, Lx, Ly, Lz, nmodes, nmax, urms, seed, cent, nvert, icsi, jeta, kzet):
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    per = (C.c_int * 3)(cfg["i_periodic"], cfg["j_periodic"], cfg["k_periodic"])
+    ucat = np.zeros(3 * mx * my * mz)
+    ucont = np.zeros(3 * mx * my * mz)
+    _ck(lib().vfs_wl_turbulent(mx, my, mz, per, C.c_double(Lx), C.c_double(Ly), C.c_double(Lz), nmodes, nmax,
+                               C.c_double(urms), C.c_uint64(seed), _dp(cent), _dp(nvert), _dp(icsi), _dp(jeta),
+                               _dp(kzet), _dp(ucat), _dp(ucont)), "turbulent")
+    return ucat, ucont
+
+
+def random_scalar(cfg, amp, seed, nvert=None, aj=None):
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    out = np.zeros(mx * my * mz)
+    _ck(lib().vfs_wl_random_scalar(mx, my, mz, C.c_double(amp), C.c_uint64(seed),
+                                   _dp(nvert) if nvert is not None else None,
+                                   _dp(aj) if aj is not None else None, _dp(out)), "random_scalar")
+    return out
+
+
+def random_vector(cfg, amp, seed):
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    out = np.zeros(3 * mx * my * mz)
+    _ck(lib().vfs_wl_random_vector(mx, my, mz, C.c_double(amp), C.c_uint64(seed), _dp(out)), "random_vector")
+    return out
+
+
+def uniform(seed, counter):
+    return lib().vfs_wl_uniform(seed, counter)

@@ -1,0 +1,69 @@
+"""The five benchmark configurations of the fractional-step hot path as
+concrete, seeded workloads (grid, mask, velocity, pressure RHS).
+
+C1  laminar channel, uniform 64^3, Re=180, periodic x/z, no-slip y walls,
+    Poiseuille + U[-1,1)*0.05 noise (seed 1001), nvert = 0.
+C2  256x128x128 tanh-stretched (beta=2) channel with a sinusoidal bottom wall
+    (amp 0.1, fading linearly to a flat top), Re=5000, Smagorinsky LES
+    (Cs^2 = 0.01), immersed sphere r=0.25 at (Lx/2, 1, Lz/2); solenoidal
+    random-Fourier k^-5/3 field (seed 1002) on a Poiseuille mean.
+C3  the C2 generator at 512x256x256 with a random Poisson RHS (seed 1003).
+C4  C2 fields with unit-norm random tangent/cotangent (seeds 14001/24001).
+C5  768x384x384 turbine wake: needs inflow/outflow and wall-function
+    boundaries (bctypes 5/4/-1), which this build does not implement yet.
+Box: Lx = 2 pi, Ly = 2, Lz = pi in every configuration.
+"""
+import math
+
+import numpy as np
+
+from . import channel_grid, poiseuille, random_scalar, random_vector, turbulent
+
+LX, LY, LZ = 2 * math.pi, 2.0, math.pi
+
+BASE = {
+    "C1": dict(N=(64, 64, 64), beta=0.0, amp=0.0, ren=180.0, les=0, immersed=0, field="poiseuille", seed=1001),
+    "C2": dict(N=(256, 128, 128), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1002),
+    "C3": dict(N=(512, 256, 256), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1003),
+    "C4": dict(N=(256, 128, 128), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1002),
+}
+CS_LES = 0.01            # Compute_Smagorinsky_Constant_1, les.c:74-77 (Cs = 0.1^2)
+SPHERE = (LX / 2, 1.0, LZ / 2, 0.25)
+
+
+def make_config(name, N=None, dt=1e-3):
+    """Return (cfg dict, spec dict) for a configuration, optionally resized to
+    N = (Ni, Nj, Nk) cells (tests use reduced sizes of the same generator)."""
+    spec = dict(BASE[name])
+    if N is not None:
+        spec["N"] = tuple(N)
+    Ni, Nj, Nk = spec["N"]
+    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=1,
+               bctype=[7, 7, 1, 1, 7, 7], immersed=spec["immersed"], les=spec["les"], second_order=0,
+               dpdz_set=1, ren=spec["ren"], dt=dt, st=1.0, mean_pressure_gradient=0.0)
+    spec["name"] = name
+    return cfg, spec
+
+
+def grid(spec):
+    Ni, Nj, Nk = spec["N"]
+    return channel_grid(Ni + 1, Nj + 1, Nk + 1, LX, LY, LZ, spec["beta"], spec["amp"])
+
+
+def fields(cfg, spec, metrics, nvert):
+    """Velocity (ucat, ucont) for the configuration from host metric arrays
+    (dict with cent, icsi, jeta, kzet)."""
+    if spec["field"] == "poiseuille":
+        return poiseuille(cfg, LY, 0.05, spec["seed"], metrics["cent"], nvert, metrics["icsi"], metrics["jeta"],
+                          metrics["kzet"])
+    return turbulent(cfg, LX, LY, LZ, 24, 16, 0.1, spec["seed"], metrics["cent"], nvert, metrics["icsi"],
+                     metrics["jeta"], metrics["kzet"])
+
+
+def poisson_rhs_random(cfg, seed, nvert, aj):
+    return random_scalar(cfg, 1.0, seed, nvert, aj)
+
+
+def unit_vector(cfg, seed):
+    v = random_vector(cfg, 1.0, seed)
+    return v / np.linalg.norm(v)
