@@ -72,7 +72,13 @@ __global__ void k_reduce_finish(const double *part, int nblocks, int op, double 
 
 int vfs_reduce_finish(vfs_ctx *c, int nblocks, int nvals, int op, double *dev_out)
 {
-    hipLaunchKernelGGL(k_reduce_finish, dim3(nvals), dim3(256), 0, c->stream, c->red, nblocks, op, dev_out);
+    return vfs_reduce_finish_slots(c, 0, nblocks, nvals, op, dev_out);
+}
+
+int vfs_reduce_finish_slots(vfs_ctx *c, int first_slot, int nblocks, int nvals, int op, double *dev_out)
+{
+    hipLaunchKernelGGL(k_reduce_finish, dim3(nvals), dim3(256), 0, c->stream,
+                       c->red + (long long)first_slot * VFS_RED_BLOCKS, nblocks, op, dev_out);
     VFS_CHECK_LAUNCH();
     return 0;
 }

@@ -140,6 +140,7 @@ int vfs_periodic_vec(vfs_ctx *c, double *v);      // periodic ghost copy, vector
 int vfs_periodic_sca(vfs_ctx *c, double *v);      // periodic ghost copy, scalar field
 // deterministic reductions over n doubles produced by a per-block kernel
 int vfs_reduce_finish(vfs_ctx *c, int nblocks, int nvals, int op /*0 sum, 1 max*/, double *dev_out);
+int vfs_reduce_finish_slots(vfs_ctx *c, int first_slot, int nblocks, int nvals, int op, double *dev_out);
 int vfs_read_scalars(vfs_ctx *c, const double *dev, double *host, int n);
 int vfs_poisson_destroy(vfs_ctx *c);
 

@@ -124,8 +124,9 @@ __global__ void k_line_factors(LV L, const double *__restrict__ A, double *tl64,
         double den = dgl - a * up;
         double inv = 1.0 / den;
         up = c * inv;
-        tl64[id] = a; tu64[id] = up; ti64[id] = inv;
-        tl32[id] = (float)a; tu32[id] = (float)up; ti32[id] = (float)inv;
+        // forward sweep coefficients of dp_j = r_j*inv_j - (a_j*inv_j)*dp_{j-1}
+        tl64[id] = a * inv; tu64[id] = up; ti64[id] = inv;
+        tl32[id] = (float)(a * inv); tu32[id] = (float)up; ti32[id] = (float)inv;
     }
 }
 
@@ -155,54 +156,63 @@ __device__ __forceinline__ T apply_row(const LV &L, const T *__restrict__ A, con
     return s;
 }
 
+// Row of A x with constant neighbour offsets: only for rows whose stencil
+// stays inside the level (no wrap, no boundary), i.e. the bulk of the grid.
+template <typename T, int NS>
+__device__ __forceinline__ T apply_row_fast(const LV &L, const T *__restrict__ A, const T *__restrict__ x, long long id)
+{
+    const long long sx = 1, sy = L.nx, sz = (long long)L.nx * L.ny;
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        int di, dj, dk;
+        slot_off(NS, q, di, dj, dk);
+        s += A[(long long)q * L.n + id] * x[id + di * sx + dj * sy + dk * sz];
+    }
+    return s;
+}
+
+template <typename T, int NS>
+__device__ __forceinline__ T apply_row_any(const LV &L, const T *__restrict__ A, const T *__restrict__ x, long long id)
+{
+    int i = (int)(id % L.nx), j = (int)((id / L.nx) % L.ny), k = (int)(id / ((long long)L.nx * L.ny));
+    if (i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 && k <= L.nz - 2)
+        return apply_row_fast<T, NS>(L, A, x, id);
+    return apply_row<T>(L, A, x, k, j, i, id, (const T *)nullptr, L, T(0));
+}
+
 // y = A x
-template <typename T>
+template <typename T, int NS>
 __global__ void k_spmv(LV L, const T *__restrict__ A, const T *__restrict__ x, T *__restrict__ y)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= L.n) return;
-    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
-    y[t] = apply_row<T>(L, A, x, k, j, i, t, nullptr, L, T(0));
+    y[t] = apply_row_any<T, NS>(L, A, x, t);
 }
 
-// Damped j-line Jacobi: xn = xo + omega * T^{-1} (b - A xo), with xo = x + alpha*P e
-// (prolongation fused) or xo = 0 when x == nullptr.  One thread per line.
-template <typename T>
-__global__ void k_line_smooth(LV L, const T *__restrict__ A, const T *__restrict__ tl, const T *__restrict__ tu,
-                              const T *__restrict__ ti, const T *__restrict__ b, const T *__restrict__ x,
-                              const T *__restrict__ e, LV C, T alpha, T omega, T *__restrict__ dscr, T *__restrict__ xn)
+// r = b - A x
+template <typename T, int NS>
+__global__ void k_resid(LV L, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)L.nx * L.nz) return;
-    int i = (int)(t % L.nx), k = (int)(t / L.nx);
-    T dp = 0;
-    for (int j = 0; j < L.ny; j++) {
-        long long id = cix(L, k, j, i);
-        T r = b[id];
-        if (x) r -= apply_row<T>(L, A, x, k, j, i, id, e, C, alpha);
-        dp = (r - tl[id] * dp) * ti[id];
-        dscr[id] = dp;
-    }
-    T xj = 0;
-    for (int j = L.ny - 1; j >= 0; j--) {
-        long long id = cix(L, k, j, i);
-        xj = dscr[id] - tu[id] * xj;
-        T xo = 0;
-        if (x) {
-            xo = x[id];
-            if (e) {
-                int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
-                xo += alpha * e[cix(C, ck, cj, ci)];
-            }
-        }
-        xn[id] = xo + omega * xj;
-    }
+    if (t >= L.n) return;
+    r[t] = b[t] - apply_row_any<T, NS>(L, A, x, t);
+}
+
+// x += alpha * P e  (piecewise-constant prolongation)
+template <typename T>
+__global__ void k_prolong(LV L, LV C, const T *__restrict__ e, T alpha, T *__restrict__ x)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= L.n) return;
+    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
+    x[t] += alpha * e[cix(C, ck, cj, ci)];
 }
 
 // coarse right-hand side: sum of the fine residuals of each aggregate
 template <typename T>
-__global__ void k_resid_restrict(LV F, LV C, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ b,
-                                 T *__restrict__ bc)
+__global__ void k_restrict(LV F, LV C, const T *__restrict__ r, T *__restrict__ bc)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= C.n) return;
@@ -214,10 +224,63 @@ __global__ void k_resid_restrict(LV F, LV C, const T *__restrict__ A, const T *_
                 if ((F.nz == 1 && ck) || (F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
                 int fk = F.nz == 1 ? K : 2 * K + ck, fj = F.ny == 1 ? J : 2 * J + cj, fi = F.nx == 1 ? I : 2 * I + ci;
                 if (fk >= F.nz || fj >= F.ny || fi >= F.nx) continue;
-                long long f = cix(F, fk, fj, fi);
-                s += b[f] - apply_row<T>(F, A, x, fk, fj, fi, f, nullptr, F, T(0));
+                s += r[cix(F, fk, fj, fi)];
             }
     bc[t] = s;
+}
+
+// Damped j-line Jacobi update x = xo + omega * T^{-1} r along every (k,i)
+// grid line (Thomas sweeps with the factors of k_line_factors).  One lane per
+// line; the loads of a batch of B steps are issued before its dependent
+// recurrence so that each wavefront keeps B x 3 loads in flight.
+template <typename T>
+__global__ void __launch_bounds__(64) k_thomas(LV L, const T *__restrict__ r, const T *__restrict__ ti,
+                                               const T *__restrict__ tm, const T *__restrict__ tu,
+                                               const T *xo, T omega, T *__restrict__ dscr, T *xn)
+{
+    constexpr int B = 8;
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)L.nx * L.nz) return;
+    const int i = (int)(t % L.nx), k = (int)(t / L.nx);
+    const long long base = (long long)k * L.ny * L.nx + i, sy = L.nx;
+    T dp = 0;
+    for (int j0 = 0; j0 < L.ny; j0 += B) {
+        T rr[B], ii[B], mm[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            if (j0 + q < L.ny) {
+                long long id = base + (j0 + q) * sy;
+                rr[q] = r[id]; ii[q] = ti[id]; mm[q] = tm[id];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            if (j0 + q < L.ny) {
+                dp = rr[q] * ii[q] - mm[q] * dp;
+                dscr[base + (j0 + q) * sy] = dp;
+            }
+        }
+    }
+    T xj = 0;
+    for (int j1 = L.ny - 1; j1 >= 0; j1 -= B) {
+        T dd[B], uu[B], oo[B];
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            int j = j1 - q;
+            if (j >= 0) {
+                long long id = base + j * sy;
+                dd[q] = dscr[id]; uu[q] = tu[id]; oo[q] = xo ? xo[id] : T(0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < B; q++) {
+            int j = j1 - q;
+            if (j >= 0) {
+                xj = dd[q] - uu[q] * xj;
+                xn[base + j * sy] = oo[q] + omega * xj;
+            }
+        }
+    }
 }
 
 // dense coarsest solve x = Cinv b over the listed unknowns; other cells 0
@@ -248,6 +311,33 @@ __global__ void k_multidot(long long n, const T *__restrict__ V, long long ldv, 
             acc += (double)vv[q] * (double)w[q];
         double r = block_sum<256>(acc, sh);
         if (threadIdx.x == 0) part[(long long)v * VFS_RED_BLOCKS + blockIdx.x] = r;
+    }
+}
+
+// dots of w with up to 8 basis vectors V_{v0..v0+nv-1} in one pass over w,
+// plus |w|^2 when self != 0; partials per block at part[slot*RB + block]
+template <typename T>
+__global__ void k_multidot8(long long n, const T *__restrict__ V, long long ldv, int v0, int nv, int self,
+                            const T *__restrict__ w, double *part)
+{
+    __shared__ double sh[4];
+    double acc[8], accs = 0;
+#pragma unroll
+    for (int v = 0; v < 8; v++) acc[v] = 0;
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
+        double wq = (double)w[q];
+#pragma unroll
+        for (int v = 0; v < 8; v++)
+            if (v < nv) acc[v] += (double)V[(long long)(v0 + v) * ldv + q] * wq;
+        accs += wq * wq;
+    }
+    for (int v = 0; v < nv; v++) {
+        double r = block_sum<256>(acc[v], sh);
+        if (threadIdx.x == 0) part[(long long)(v0 + v) * VFS_RED_BLOCKS + blockIdx.x] = r;
+    }
+    if (self) {
+        double r = block_sum<256>(accs, sh);
+        if (threadIdx.x == 0) part[(long long)(VFS_RED_SLOTS - 1) * VFS_RED_BLOCKS + blockIdx.x] = r;
     }
 }
 
@@ -283,16 +373,6 @@ __global__ void k_normalize(long long n, const T *__restrict__ w, const double *
     if (q >= n) return;
     double s = 1.0 / sqrt(nrm2[0]);
     out[q] = (T)((double)w[q] * s);
-}
-
-// r = b - A x on level 0 (FP64 operator), also used by the defect correction
-__global__ void k_resid64(LV L, const double *__restrict__ A, const double *__restrict__ x, const double *__restrict__ b,
-                          double *__restrict__ r)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
-    r[t] = b[t] - apply_row<double>(L, A, x, k, j, i, t, nullptr, L, 0.0);
 }
 
 template <typename T>
@@ -580,7 +660,31 @@ int vfs_mg_setup(vfs_ctx *c)
     return 0;
 }
 
-// one V(1,1)-cycle: x_l = M_l^{-1} b_l (level vectors b, x of each level)
+// launch y = A x / r = b - A x with the level's slot count
+template <typename T>
+static int launch_spmv(vfs_ctx *c, const MGLevel &L, const T *A, const T *x, T *y)
+{
+    LV Lv = lv_of(L);
+    unsigned gb = grid_for(L.n, 256);
+    if (L.nslot == 19) hipLaunchKernelGGL((k_spmv<T, 19>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, y);
+    else hipLaunchKernelGGL((k_spmv<T, 27>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, y);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+template <typename T>
+static int launch_resid(vfs_ctx *c, const MGLevel &L, const T *A, const T *x, const T *b, T *r)
+{
+    LV Lv = lv_of(L);
+    unsigned gb = grid_for(L.n, 256);
+    if (L.nslot == 19) hipLaunchKernelGGL((k_resid<T, 19>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, b, r);
+    else hipLaunchKernelGGL((k_resid<T, 27>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, b, r);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+// one V(1,1)-cycle: x_l = M_l^{-1} b_l (level vectors b, x of each level;
+// y holds residuals, d the Thomas forward sweep)
 template <typename T>
 static int vcycle(vfs_ctx *c, int l)
 {
@@ -596,24 +700,27 @@ static int vcycle(vfs_ctx *c, int l)
         VFS_CHECK_LAUNCH();
         return 0;
     }
-    const T *tl, *tu, *ti;
-    fac_of<T>(L, tl, tu, ti);
+    const T *tm, *tu, *ti;
+    fac_of<T>(L, tm, tu, ti);
     unsigned gl = grid_for((long long)L.nx * L.nz, 64);
-    // pre-smooth from a zero guess
-    hipLaunchKernelGGL(k_line_smooth<T>, dim3(gl), dim3(64), 0, c->stream, Lv, A, tl, tu, ti, b, (const T *)nullptr,
-                       (const T *)nullptr, Lv, T(0), omega, dd, x);
+    // pre-smooth from a zero guess: x = omega T^{-1} b
+    hipLaunchKernelGGL(k_thomas<T>, dim3(gl), dim3(64), 0, c->stream, Lv, (const T *)b, ti, tm, tu, (const T *)nullptr,
+                       omega, dd, x);
     VFS_CHECK_LAUNCH();
+    if (launch_resid<T>(c, L, A, x, b, y)) return -1;
     MGLevel &C = ps->lv[l + 1];
     LV Cv = lv_of(C);
-    hipLaunchKernelGGL(k_resid_restrict<T>, dim3(grid_for(C.n, 128)), dim3(128), 0, c->stream, Lv, Cv, A, x, b, vec_of<T>(C, 0));
+    hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, Lv, Cv, (const T *)y, vec_of<T>(C, 0));
     VFS_CHECK_LAUNCH();
     if (vcycle<T>(c, l + 1)) return -1;
-    // post-smooth on x + alpha*P e, result in y, then swap
-    hipLaunchKernelGGL(k_line_smooth<T>, dim3(gl), dim3(64), 0, c->stream, Lv, A, tl, tu, ti, b, (const T *)x,
-                       (const T *)vec_of<T>(C, 1), Cv, alpha, omega, dd, y);
+    hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(L.n, 256)), dim3(256), 0, c->stream, Lv, Cv,
+                       (const T *)vec_of<T>(C, 1), alpha, x);
     VFS_CHECK_LAUNCH();
-    if (std::is_same<T, double>::value) std::swap(L.x64, L.y64);
-    else std::swap(L.x32, L.y32);
+    // post-smooth: x += omega T^{-1} (b - A x), in place
+    if (launch_resid<T>(c, L, A, x, b, y)) return -1;
+    hipLaunchKernelGGL(k_thomas<T>, dim3(gl), dim3(64), 0, c->stream, Lv, (const T *)y, ti, tm, tu, (const T *)x, omega,
+                       dd, x);
+    VFS_CHECK_LAUNCH();
     return 0;
 }
 
@@ -665,8 +772,7 @@ static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, do
             s.assign(m + 1, 0.0);
             s[0] = bnorm;
         } else {
-            hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->stream, Lv, A, x, w);
-            VFS_CHECK_LAUNCH();
+            if (launch_spmv<T>(c, L0, A, x, w)) return -1;
             // w = rhs - A x  via w = -(A x - rhs)
             hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, dres + 8, 1.0);
             VFS_CHECK_LAUNCH();
@@ -693,24 +799,36 @@ static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, do
             T *vk = V + (size_t)kk * n;
             VFS_CHECK_HIP(hipMemcpyAsync(vec_of<T>(L0, 0), vk, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
             if (vcycle<T>(c, 0)) return -1;
-            hipLaunchKernelGGL(k_spmv<T>, dim3(gb), dim3(256), 0, c->stream, Lv, A, (const T *)vec_of<T>(L0, 1), w);
-            VFS_CHECK_LAUNCH();
-            // classical Gram-Schmidt, twice
+            if (launch_spmv<T>(c, L0, A, (const T *)vec_of<T>(L0, 1), w)) return -1;
+            // classical Gram-Schmidt; a second pass only when the first
+            // cancelled more than half of |w| (Daniel-Gragg-Kaufman-Stewart)
+            double h1[64], h2[64];
             for (int pass = 0; pass < 2; pass++) {
-                hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, V, n, kk + 1, w, c->red);
-                VFS_CHECK_LAUNCH();
+                for (int v0 = 0; v0 <= kk; v0 += 8) {
+                    int nv = std::min(8, kk + 1 - v0);
+                    hipLaunchKernelGGL(k_multidot8<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, v0, nv,
+                                       (int)(pass == 0 && v0 == 0), (const T *)w, c->red);
+                    VFS_CHECK_LAUNCH();
+                }
                 if (multidot_finish(c, kk + 1, nbr, ps->dots + 64 * pass)) return -1;
+                if (pass == 0 && vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, ps->dots + 130)) return -1;
                 hipLaunchKernelGGL(k_multiaxpy<T>, dim3(gb), dim3(256), 0, c->stream, n, V, n, kk + 1, ps->dots + 64 * pass, w);
                 VFS_CHECK_LAUNCH();
+                hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
+                VFS_CHECK_LAUNCH();
+                if (multidot_finish(c, 1, nbr, ps->dots + 128)) return -1;
+                VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, 136 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+                for (int q = 0; q <= kk; q++) (pass == 0 ? h1 : h2)[q] = ps->hdots[64 * pass + q];
+                if (pass == 0) {
+                    for (int q = 0; q <= kk; q++) h2[q] = 0;
+                    double before = ps->hdots[130], after = ps->hdots[128];
+                    if (after > 0.25 * before) break;
+                }
             }
-            hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
-            VFS_CHECK_LAUNCH();
-            if (multidot_finish(c, 1, nbr, ps->dots + 128)) return -1;
             hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, w, ps->dots + 128, V + (size_t)(kk + 1) * n);
             VFS_CHECK_LAUNCH();
-            VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, 129 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
-            for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = ps->hdots[q] + ps->hdots[64 + q];
+            for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = h1[q] + h2[q];
             H[(size_t)(kk + 1) * m + kk] = sqrt(ps->hdots[128]);
             for (int q = 0; q < kk; q++) {
                 double t0 = cs[q] * H[(size_t)q * m + kk] + sn[q] * H[(size_t)(q + 1) * m + kk];
@@ -783,8 +901,7 @@ extern "C" int vfs_poisson_solve(vfs_ctx *c, const double *b, double *phi, doubl
     if (bnorm == 0.0) bnorm = 1.0;
     for (int outer = 0; outer < 50; outer++) {
         // FP64 defect r = b - A x (FP64 operator)
-        hipLaunchKernelGGL(k_resid64, dim3(gb), dim3(256), 0, c->stream, Lv, L0.A64, ps->e64, ps->bb64, ps->r64);
-        VFS_CHECK_LAUNCH();
+        if (launch_resid<double>(c, L0, L0.A64, ps->e64, ps->bb64, ps->r64)) return -1;
         double rn2;
         if (norm2_64(c, ps->r64, n, dres, &rn2)) return -1;
         rel = sqrt(rn2) / bnorm;
