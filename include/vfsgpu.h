@@ -105,6 +105,33 @@ int vfs_rk_stage(vfs_ctx *ctx, const double *u0, double *ucont, double *ucat,
 /* Calc_Minimum_dt (momentum.c:104-195): CFL-1 time step. */
 int vfs_min_dt(vfs_ctx *ctx, const double *ucont, double *dt_out);
 
+/* ---- derivatives of the implicit residual ----------------------------------
+ * F(U) is the FormFunction_SNES residual (momentum.c:2197-2301) with U_o,
+ * rhs_o, dP and nu_t held fixed, and the ucat of cells with nvert >= 0.1
+ * taken from `ucat` (frozen, as inside the reference's SNES solve).
+ * J = dF/dU replaces the finite-difference MatMFFD product of
+ * Implicit_MatrixFree (implicitsolver.c:4222-4236).  No argument is modified
+ * except the output; v and jtv must not alias. */
+/* jw = J w */
+int vfs_rhs_jvp(vfs_ctx *ctx, const double *U, const double *ucat, const double *nu_t,
+                const double *w, double *jw);
+/* jtv = J^T v */
+int vfs_rhs_vjp(vfs_ctx *ctx, const double *U, const double *ucat, const double *nu_t,
+                const double *v, double *jtv);
+/* Implicit_MatrixFree (implicitsolver.c:4196-4441): solves F(U) = 0 by
+ * Newton's method to ||F|| <= rtol ||F(U_in)|| (rtol = imp_free_tol, 1e-4 by
+ * default in main.c:212; at most max_newton steps, 50 in the reference),
+ * each step an unpreconditioned restarted GMRES(restart) on J dU = -F with
+ * the exact JVP and Eisenstat-Walker forcing terms (at most max_krylov
+ * iterations, 1000 in the reference), then a backtracking line search on
+ * ||F||.  U (in/out) returns with zero no-slip boundary fluxes; ucat (in/out)
+ * returns Contra2Cart of the solution with the frozen cells unchanged.
+ * Reports the Newton and total Krylov iteration counts and the final ||F||. */
+int vfs_implicit_matrixfree(vfs_ctx *ctx, double *U, const double *U_o, const double *rhs_o,
+                            const double *dP, const double *nu_t, double *ucat, double rtol,
+                            int max_newton, int restart, int max_krylov, int *newton_its,
+                            int *krylov_its, double *fnorm);
+
 /* ---- pressure projection ------------------------------------------------ */
 /* setup_lidx2 + PoissonLHSNew (poisson.c:182-329, 971-1682): numbers the
  * unknowns (returned in *n_unknowns) and builds the 19-point coefficients

@@ -11,8 +11,8 @@ SMALL = {
 }
 
 
-def oracle_case(name, N=None):
-    cfg, spec = W.make_config(name, N or SMALL.get(name))
+def oracle_case(name, N=None, dt=1e-3):
+    cfg, spec = W.make_config(name, N or SMALL.get(name), dt=dt)
     coor = W.grid(spec)
     o = Oracle(cfg, coor)
     if spec["immersed"]:

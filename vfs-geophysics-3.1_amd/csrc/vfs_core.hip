@@ -159,6 +159,7 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     hipMemset(c->snap, 0, 3 * P * sizeof(double));
     hipMemset(c->tmp3, 0, 3 * P * sizeof(double));
     c->red_blocks = VFS_RED_BLOCKS;
+    c->nvert_f32 = 1;
     if (hipDeviceSynchronize() != hipSuccess) { vfs_ctx_destroy(c); vfs_set_error("device sync"); return -1; }
     *out = c;
     return 0;
@@ -169,8 +170,10 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     if (!c) return 0;
     hipDeviceSynchronize();
     vfs_poisson_destroy(c);
+    vfs_implicit_destroy(c);
     hipFree(c->geom_block); hipFree(c->nvert); hipFree(c->face); hipFree(c->fp);
     hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red);
+    if (c->ad) hipFree(c->ad);
     delete c;
     return 0;
 }

@@ -1,10 +1,216 @@
-// Fused Formfunction_2 (momentum.c:456-1979): placeholder that routes to the
-// reference-structured passes until the k-marching kernel lands.
+// Formfunction_2 (momentum.c:456-1979) with the face fluxes computed from
+// LDS-staged planes.
+//
+// The face fluxes dominate the right-hand side: each face reads about 54 ucat
+// values and 10 nvert values on a 2x4x3 neighbourhood.  Read from global
+// memory as AoS doubles, those loads are bound by the L1/TA rate long before
+// HBM.  Here a workgroup owns a 64(i) x 4(j) column of cells and marches in
+// k.  It keeps four k-planes of ucat (SoA, FP64) and nvert (FP32, exact for
+// the small-integer masks) with a one-cell i/j halo (two on the high i/j
+// side) in LDS: planes k-1..k+2 are what the i-, j- and k-faces of plane k
+// read.  The next plane is prefetched into registers while the current one
+// is computed.  Reads that fall outside the staged box (the periodic
+// wrap-around of the 4th-order stencil at the first and last faces) go to
+// global memory.
+//
+// Every expression is the one in vfs_stencil.h, so results are bit-identical
+// to the reference-structured pass k_ff2_faces.  Face points on the
+// low boundary planes (i = 0, j = 0, k = 0) are computed by a small separate
+// kernel.  The cell divergence and the projection then run as in the
+// reference-structured path.
 #include "vfs_internal.h"
+#include "vfs_stencil.h"
 
 int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale);
+int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
+
+namespace {
+
+constexpr int TI = 64;          // cells per workgroup row (one wavefront)
+constexpr int TJ = 4;           // rows per workgroup
+constexpr int BI = TI + 3;      // staged columns: i0-1 .. i0+TI+1
+constexpr int BJ = TJ + 3;      // staged rows:    j0-1 .. j0+TJ+1
+constexpr int BP = BI * BJ;     // points per staged plane
+constexpr int NT = TI * TJ;     // threads
+constexpr int NU = (3 * BP + NT - 1) / NT;   // ucat doubles per thread per plane
+constexpr int NN = (BP + NT - 1) / NT;       // nvert values per thread per plane
+
+// staged planes (one set per workgroup): ucat [4][3][BJ][BI], nvert [4][BJ][BI]
+__shared__ double sU[4 * 3 * BP];
+__shared__ float sN[4 * BP];
+
+// Readers of the staged box.  Only faces whose whole stencil lies in the box
+// use them (every interior face except the periodic wrap-around faces).
+struct LdsU {
+    int i0, j0;
+    __device__ __forceinline__ double operator()(int k, int j, int i, int q) const
+    {
+        return sU[(((k & 3) * 3 + q) * BJ + (j - j0 + 1)) * BI + (i - i0 + 1)];
+    }
+};
+
+struct LdsN {
+    int i0, j0;
+    __device__ __forceinline__ double operator()(int k, int j, int i) const
+    {
+        return (double)sN[((k & 3) * BJ + (j - j0 + 1)) * BI + (i - i0 + 1)];
+    }
+};
+
+// global loads of plane k of the staged box into registers
+__device__ __forceinline__ void fetch_plane(const Dims &d, const double *__restrict__ ucat,
+                                            const double *__restrict__ nvert, int i0, int j0, int k, double (&ru)[NU],
+                                            float (&rn)[NN])
+{
+    const bool kin = k >= 0 && k < d.mz;
+#pragma unroll
+    for (int n = 0; n < NU; n++) {
+        const int e = threadIdx.x + n * NT;
+        const int r = e / (3 * BI), c = e - r * (3 * BI);
+        const int gi = i0 - 1 + c / 3, gj = j0 - 1 + r;
+        double v = 0;
+        if (kin && e < 3 * BP && gi < d.mx && gj < d.my) v = ucat[3 * IX(d, k, gj, i0 - 1) + c];
+        ru[n] = v;
+    }
+#pragma unroll
+    for (int n = 0; n < NN; n++) {
+        const int e = threadIdx.x + n * NT;
+        const int r = e / BI, c = e - r * BI;
+        const int gi = i0 - 1 + c, gj = j0 - 1 + r;
+        float v = 0;
+        if (kin && e < BP && gi < d.mx && gj < d.my) v = (float)nvert[IX(d, k, gj, gi)];
+        rn[n] = v;
+    }
+}
+
+__device__ __forceinline__ void store_plane(int k, const double (&ru)[NU], const float (&rn)[NN])
+{
+    const int slot = k & 3;
+#pragma unroll
+    for (int n = 0; n < NU; n++) {
+        const int e = threadIdx.x + n * NT;
+        if (e < 3 * BP) {
+            const int r = e / (3 * BI), c = e - r * (3 * BI);
+            const int p = c / 3, q = c - 3 * p;
+            sU[((slot * 3 + q) * BJ + r) * BI + p] = ru[n];
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NN; n++) {
+        const int e = threadIdx.x + n * NT;
+        if (e < BP) sN[slot * BP + e] = rn[n];
+    }
+}
+
+}  // namespace
+
+// Face fluxes of all three directions at every interior point
+// (1..mx-2, 1..my-2, k in [kb, ke)).  grid: (ceil((mx-2)/64), ceil((my-2)/4), k chunks)
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert,
+                                                      const double *__restrict__ ucont,
+                                                      const double *__restrict__ ucat,
+                                                      const double *__restrict__ nu_t, double *__restrict__ face,
+                                                      int kchunk)
+{
+    const int tx = threadIdx.x % TI, ty = threadIdx.x / TI;
+    const int i0 = 1 + blockIdx.x * TI, j0 = 1 + blockIdx.y * TJ;
+    const int kb = 1 + blockIdx.z * kchunk;
+    const int ke = min(kb + kchunk, d.mz - 1);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = i <= d.mx - 2 && j <= d.my - 2;
+    const long long P = d.P;
+
+    double ru[NU];
+    float rn[NN];
+    for (int k = kb - 1; k <= kb + 2; k++) {
+        fetch_plane(d, ucat, nvert, i0, j0, k, ru, rn);
+        store_plane(k, ru, rn);
+    }
+    __syncthreads();
+    for (int k = kb; k < ke; k++) {
+        // prefetch plane k+3 while plane k is computed
+        fetch_plane(d, ucat, nvert, i0, j0, k + 3, ru, rn);
+        if (active) {
+            const LdsU U{i0, j0};
+            const LdsN N{i0, j0};
+            const long long t = IX(d, k, j, i);
+            double D[3], V[3];
+            if (!(d.ip && i == d.mx - 2)) {
+                face_flux_t<0, double>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+                for (int q = 0; q < 3; q++) { face[(0 * 3 + q) * P + t] = D[q]; face[(3 * 3 + q) * P + t] = V[q]; }
+            }
+            // one face at a time: keeps the live registers of a single face
+            __builtin_amdgcn_sched_barrier(0);
+            if (!(d.jp && j == d.my - 2)) {
+                face_flux_t<1, double>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+                for (int q = 0; q < 3; q++) { face[(1 * 3 + q) * P + t] = D[q]; face[(4 * 3 + q) * P + t] = V[q]; }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (!(d.kp && k == d.mz - 2)) {
+                face_flux_t<2, double>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+                for (int q = 0; q < 3; q++) { face[(2 * 3 + q) * P + t] = D[q]; face[(5 * 3 + q) * P + t] = V[q]; }
+            }
+        }
+        __syncthreads();   // every read of plane k-1 is done
+        store_plane(k + 3, ru, rn);
+        __syncthreads();
+    }
+}
+
+// Face points the staged kernel does not compute: the low boundary planes
+// (i-faces at i = 0, j-faces at j = 0, k-faces at k = 0) and, in periodic
+// directions, the last interior faces (i = mx-2, j = my-2, k = mz-2), whose
+// 4th-order stencil wraps around to cell 2 (momentum.c:699-705).
+// Segment s of the launch covers face direction s % 3 on plane 0 (s < 3) or
+// on plane m-2 (s >= 3).
+__global__ void k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
+                                const double *__restrict__ ucat, const double *__restrict__ nu_t,
+                                double *__restrict__ face)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long n3[3] = {(long long)(d.my - 2) * (d.mz - 2), (long long)(d.mx - 2) * (d.mz - 2),
+                             (long long)(d.mx - 2) * (d.my - 2)};
+    const int per[3] = {d.ip, d.jp, d.kp};
+    const long long P = d.P;
+    for (int s = 0; s < 6; s++) {
+        const int dir = s % 3;
+        if (s >= 3 && !per[dir]) continue;
+        if (t >= n3[dir]) { t -= n3[dir]; continue; }
+        int i, j, k;
+        if (dir == 0) {
+            i = s < 3 ? 0 : d.mx - 2; j = 1 + (int)(t % (d.my - 2)); k = 1 + (int)(t / (d.my - 2));
+        } else if (dir == 1) {
+            j = s < 3 ? 0 : d.my - 2; i = 1 + (int)(t % (d.mx - 2)); k = 1 + (int)(t / (d.mx - 2));
+        } else {
+            k = s < 3 ? 0 : d.mz - 2; i = 1 + (int)(t % (d.mx - 2)); j = 1 + (int)(t / (d.mx - 2));
+        }
+        const long long o = IX(d, k, j, i);
+        double D[3], V[3];
+        if (dir == 0) face_flux<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        else if (dir == 1) face_flux<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        else face_flux<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        for (int q = 0; q < 3; q++) { face[(dir * 3 + q) * P + o] = D[q]; face[((3 + dir) * 3 + q) * P + o] = V[q]; }
+        return;
+    }
+}
 
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale)
 {
-    return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
+    if (!c->nvert_f32) return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
+    const Dims d = c->d;
+    const int kchunk = 16;
+    const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);
+    hipLaunchKernelGGL(k_ff2_faces_lds, grid, dim3(NT), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t, c->face,
+                       kchunk);
+    VFS_CHECK_LAUNCH();
+    const long long n3[3] = {(long long)(d.my - 2) * (d.mz - 2), (long long)(d.mx - 2) * (d.mz - 2),
+                             (long long)(d.mx - 2) * (d.my - 2)};
+    const long long nb = n3[0] * (1 + d.ip) + n3[1] * (1 + d.jp) + n3[2] * (1 + d.kp);
+    hipLaunchKernelGGL(k_ff2_faces_bnd, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont,
+                       ucat, nu_t, c->face);
+    VFS_CHECK_LAUNCH();
+    return vfs_ff2_from_faces(c, rhs, scale);
 }

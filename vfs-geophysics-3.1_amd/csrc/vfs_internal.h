@@ -100,12 +100,15 @@ struct vfs_ctx {
     Geom g;
     double *geom_block;      // one allocation holding every metric array
     double *nvert;
+    int nvert_f32;           // every nvert value is exactly representable in FP32
     // scratch (device)
     double *face;            // 18*P: div1..3 and visc1..3 (vector each)
     double *fp;              // 3*P
     double *snap;            // 3*P: Contra2Cart boundary snapshot / SNES copies
     double *tmp3;            // 3*P general
     double *red;             // reduction partials + result scalars
+    double *ad;              // 12*P: primal and adjoint state of the JVP/VJP (allocated on first use)
+    void *nk;                // Newton-Krylov workspace (allocated on first use)
     int red_blocks;
     Poisson *pois;
 };
@@ -143,6 +146,7 @@ int vfs_reduce_finish(vfs_ctx *c, int nblocks, int nvals, int op /*0 sum, 1 max*
 int vfs_reduce_finish_slots(vfs_ctx *c, int first_slot, int nblocks, int nvals, int op, double *dev_out);
 int vfs_read_scalars(vfs_ctx *c, const double *dev, double *host, int n);
 int vfs_poisson_destroy(vfs_ctx *c);
+void vfs_implicit_destroy(vfs_ctx *c);
 
 // fixed sizes of the reduction buffer
 #define VFS_RED_BLOCKS 1024

@@ -7,6 +7,7 @@
 // earlier ones (boundary mirror, then periodic copies, then face averages,
 // then the final periodic copy).
 #include "vfs_internal.h"
+#include "vfs_reduce.h"
 
 #define CO(k, j, i, q) coor[3 * IX(d, k, j, i) + (q)]
 
@@ -177,9 +178,27 @@ extern "C" int vfs_get_metric(vfs_ctx *c, int which, double *dst)
     return 0;
 }
 
+// count of nvert values that FP32 does not hold exactly (per-block partials)
+__global__ void k_nvert_f32_check(long long n, const double *__restrict__ nvert, double *part)
+{
+    __shared__ double sh[4];
+    double acc = 0;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        acc += (double)(float)nvert[t] != nvert[t] ? 1.0 : 0.0;
+    double r = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 extern "C" int vfs_set_nvert(vfs_ctx *c, const double *nvert)
 {
     VFS_CHECK_HIP(hipMemcpyAsync(c->nvert, nvert, (size_t)c->d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_nvert_f32_check, dim3(VFS_RED_BLOCKS), dim3(256), 0, c->stream, c->d.P, c->nvert, c->red);
+    VFS_CHECK_LAUNCH();
+    double *res = c->red + (long long)VFS_RED_SLOTS * VFS_RED_BLOCKS;
+    if (vfs_reduce_finish(c, VFS_RED_BLOCKS, 1, 0, res)) return -1;
+    double bad = 0;
+    if (vfs_read_scalars(c, res, &bad, 1)) return -1;
+    c->nvert_f32 = bad == 0.0;
     return 0;
 }
 
@@ -253,6 +272,7 @@ extern "C" int vfs_nvert_sphere(vfs_ctx *c, double cx, double cy, double cz, dou
     VFS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_sphere_relabel, dim3(gb), dim3(256), 0, c->stream, d, c->nvert);
     VFS_CHECK_LAUNCH();
+    c->nvert_f32 = 1;   // values 0, 1, 3
     if (nvert_out)
         VFS_CHECK_HIP(hipMemcpyAsync(nvert_out, c->nvert, (size_t)d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     return 0;

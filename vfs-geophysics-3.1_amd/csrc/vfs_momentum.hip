@@ -376,6 +376,18 @@ int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const do
     return 0;
 }
 
+// cell divergence and projection of face fluxes already in c->face
+int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale)
+{
+    const Dims d = c->d;
+    unsigned gb = grid_for(d.P, 256);
+    hipLaunchKernelGGL(k_ff2_cells, dim3(gb), dim3(256), 0, c->stream, d, c->nvert, c->face, c->fp);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_ff2_rhs, dim3(gb), dim3(256), 0, c->stream, d, c->g, c->nvert, c->fp, rhs, scale);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale);
 
 extern "C" int vfs_formfunction2(vfs_ctx *c, double *ucont, const double *ucat, const double *nu_t,

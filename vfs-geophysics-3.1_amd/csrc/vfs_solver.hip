@@ -31,14 +31,18 @@ struct LV {
 
 static LV lv_of(const MGLevel &l) { return LV{l.nx, l.ny, l.nz, l.n, l.per_i, l.per_j, l.per_k, l.nslot}; }
 
-__constant__ int s_slot19[19][3] = {
-    {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
-    {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}, {0, 1, 1}, {0, 1, -1}, {0, -1, 1},
-    {0, -1, -1}, {1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+// neighbour offsets (di, dj, dk) of the 19 slots of poisson.c:951-969; a
+// constexpr table so unrolled loops fold the offsets into immediates
+struct Slot19 {
+    static constexpr int t[19][3] = {
+        {0, 0, 0}, {1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1},
+        {1, 1, 0}, {1, -1, 0}, {-1, 1, 0}, {-1, -1, 0}, {0, 1, 1}, {0, 1, -1}, {0, -1, 1},
+        {0, -1, -1}, {1, 0, 1}, {1, 0, -1}, {-1, 0, 1}, {-1, 0, -1}};
+};
 
 __device__ __forceinline__ void slot_off(int nslot, int s, int &di, int &dj, int &dk)
 {
-    if (nslot == 19) { di = s_slot19[s][0]; dj = s_slot19[s][1]; dk = s_slot19[s][2]; }
+    if (nslot == 19) { di = Slot19::t[s][0]; dj = Slot19::t[s][1]; dk = Slot19::t[s][2]; }
     else { dk = s / 9 - 1; dj = (s / 3) % 3 - 1; di = s % 3 - 1; }
 }
 
@@ -172,40 +176,58 @@ __device__ __forceinline__ T apply_row_fast(const LV &L, const T *__restrict__ A
     return s;
 }
 
+// Branch-free row of A x for any cell: periodic neighbours wrap, bounded
+// neighbours are clamped (their coefficients are zero by construction: the
+// drop rule removes couplings to non-unknowns on level 0 and Galerkin
+// products inherit that), so every lane of a wavefront runs the same code.
 template <typename T, int NS>
-__device__ __forceinline__ T apply_row_any(const LV &L, const T *__restrict__ A, const T *__restrict__ x, long long id)
+__device__ __forceinline__ T apply_row_any(const LV &L, const T *__restrict__ A, const T *__restrict__ x, long long id,
+                                           int i, int j, int k)
 {
-    int i = (int)(id % L.nx), j = (int)((id / L.nx) % L.ny), k = (int)(id / ((long long)L.nx * L.ny));
-    if (i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 && k <= L.nz - 2)
-        return apply_row_fast<T, NS>(L, A, x, id);
-    return apply_row<T>(L, A, x, k, j, i, id, (const T *)nullptr, L, T(0));
+    const int im = i > 0 ? i - 1 : (L.pi ? L.nx - 1 : 0), ip = i < L.nx - 1 ? i + 1 : (L.pi ? 0 : L.nx - 1);
+    const int jm = j > 0 ? j - 1 : (L.pj ? L.ny - 1 : 0), jp = j < L.ny - 1 ? j + 1 : (L.pj ? 0 : L.ny - 1);
+    const int km = k > 0 ? k - 1 : (L.pk ? L.nz - 1 : 0), kp = k < L.nz - 1 ? k + 1 : (L.pk ? 0 : L.nz - 1);
+    const long long sy = L.nx, sz = (long long)L.nx * L.ny;
+    const long long I[3] = {im, i, ip}, J[3] = {jm * sy, j * sy, jp * sy}, K[3] = {km * sz, k * sz, kp * sz};
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        int di, dj, dk;
+        slot_off(NS, q, di, dj, dk);
+        s += A[(long long)q * L.n + id] * x[K[dk + 1] + J[dj + 1] + I[di + 1]];
+    }
+    return s;
 }
+
+// Row kernels launch one thread per i and one block row per (j, k)
+// (blockIdx.y = j, blockIdx.z = k), so all index arithmetic but i is scalar.
+#define ROW_IDX(L)                                                             \
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;                       \
+    if (i >= (L).nx) return;                                                   \
+    const int j = blockIdx.y, k = blockIdx.z;                                  \
+    const long long t = ((long long)k * (L).ny + j) * (L).nx + i;
 
 // y = A x
 template <typename T, int NS>
 __global__ void k_spmv(LV L, const T *__restrict__ A, const T *__restrict__ x, T *__restrict__ y)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    y[t] = apply_row_any<T, NS>(L, A, x, t);
+    ROW_IDX(L)
+    y[t] = apply_row_any<T, NS>(L, A, x, t, i, j, k);
 }
 
 // r = b - A x
 template <typename T, int NS>
 __global__ void k_resid(LV L, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ b, T *__restrict__ r)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    r[t] = b[t] - apply_row_any<T, NS>(L, A, x, t);
+    ROW_IDX(L)
+    r[t] = b[t] - apply_row_any<T, NS>(L, A, x, t, i, j, k);
 }
 
 // x += alpha * P e  (piecewise-constant prolongation)
 template <typename T>
 __global__ void k_prolong(LV L, LV C, const T *__restrict__ e, T alpha, T *__restrict__ x)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= L.n) return;
-    int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+    ROW_IDX(L)
     int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
     x[t] += alpha * e[cix(C, ck, cj, ci)];
 }
@@ -239,9 +261,9 @@ __global__ void __launch_bounds__(64) k_thomas(LV L, const T *__restrict__ r, co
                                                const T *xo, T omega, T *__restrict__ dscr, T *xn)
 {
     constexpr int B = 8;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)L.nx * L.nz) return;
-    const int i = (int)(t % L.nx), k = (int)(t / L.nx);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= L.nx) return;
+    const int k = blockIdx.y;
     const long long base = (long long)k * L.ny * L.nx + i, sy = L.nx;
     T dp = 0;
     for (int j0 = 0; j0 < L.ny; j0 += B) {
@@ -661,13 +683,16 @@ int vfs_mg_setup(vfs_ctx *c)
 }
 
 // launch y = A x / r = b - A x with the level's slot count
+static inline int row_block(int nx) { return nx >= 256 ? 256 : ((nx + 63) / 64) * 64; }
+static inline dim3 row_grid(const MGLevel &L) { int b = row_block(L.nx); return dim3((L.nx + b - 1) / b, L.ny, L.nz); }
+
 template <typename T>
 static int launch_spmv(vfs_ctx *c, const MGLevel &L, const T *A, const T *x, T *y)
 {
     LV Lv = lv_of(L);
-    unsigned gb = grid_for(L.n, 256);
-    if (L.nslot == 19) hipLaunchKernelGGL((k_spmv<T, 19>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, y);
-    else hipLaunchKernelGGL((k_spmv<T, 27>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, y);
+    dim3 gb = row_grid(L), bb(row_block(L.nx));
+    if (L.nslot == 19) hipLaunchKernelGGL((k_spmv<T, 19>), gb, bb, 0, c->stream, Lv, A, x, y);
+    else hipLaunchKernelGGL((k_spmv<T, 27>), gb, bb, 0, c->stream, Lv, A, x, y);
     VFS_CHECK_LAUNCH();
     return 0;
 }
@@ -676,9 +701,9 @@ template <typename T>
 static int launch_resid(vfs_ctx *c, const MGLevel &L, const T *A, const T *x, const T *b, T *r)
 {
     LV Lv = lv_of(L);
-    unsigned gb = grid_for(L.n, 256);
-    if (L.nslot == 19) hipLaunchKernelGGL((k_resid<T, 19>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, b, r);
-    else hipLaunchKernelGGL((k_resid<T, 27>), dim3(gb), dim3(256), 0, c->stream, Lv, A, x, b, r);
+    dim3 gb = row_grid(L), bb(row_block(L.nx));
+    if (L.nslot == 19) hipLaunchKernelGGL((k_resid<T, 19>), gb, bb, 0, c->stream, Lv, A, x, b, r);
+    else hipLaunchKernelGGL((k_resid<T, 27>), gb, bb, 0, c->stream, Lv, A, x, b, r);
     VFS_CHECK_LAUNCH();
     return 0;
 }
@@ -702,7 +727,7 @@ static int vcycle(vfs_ctx *c, int l)
     }
     const T *tm, *tu, *ti;
     fac_of<T>(L, tm, tu, ti);
-    unsigned gl = grid_for((long long)L.nx * L.nz, 64);
+    dim3 gl((L.nx + 63) / 64, L.nz);
     // pre-smooth from a zero guess: x = omega T^{-1} b
     hipLaunchKernelGGL(k_thomas<T>, dim3(gl), dim3(64), 0, c->stream, Lv, (const T *)b, ti, tm, tu, (const T *)nullptr,
                        omega, dd, x);
@@ -713,7 +738,7 @@ static int vcycle(vfs_ctx *c, int l)
     hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, Lv, Cv, (const T *)y, vec_of<T>(C, 0));
     VFS_CHECK_LAUNCH();
     if (vcycle<T>(c, l + 1)) return -1;
-    hipLaunchKernelGGL(k_prolong<T>, dim3(grid_for(L.n, 256)), dim3(256), 0, c->stream, Lv, Cv,
+    hipLaunchKernelGGL(k_prolong<T>, row_grid(L), dim3(row_block(L.nx)), 0, c->stream, Lv, Cv,
                        (const T *)vec_of<T>(C, 1), alpha, x);
     VFS_CHECK_LAUNCH();
     // post-smooth: x += omega T^{-1} (b - A x), in place

@@ -7,11 +7,11 @@
 #define VFS_SOLID 0.1 // k-omega.c:11
 
 // face-centred velocity gradients (k-omega.c:48-278)
-template <typename Fetch>
-__device__ __forceinline__ void du_face(int dir, const Fetch &U, const double *__restrict__ nvert, const Dims &d,
-                                        int i, int j, int k, double dc[3], double de[3], double dz[3])
+template <typename S, typename Fetch, typename NFetch>
+__device__ __forceinline__ void du_face(int dir, const Fetch &U, const NFetch &N, const Dims &d,
+                                        int i, int j, int k, S dc[3], S de[3], S dz[3])
 {
-#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+#define NV(kk, jj, ii) N(kk, jj, ii)
     if (dir == 0) {
 #pragma unroll
         for (int q = 0; q < 3; q++) dc[q] = U(k, j, i + 1, q) - U(k, j, i, q);
@@ -116,18 +116,52 @@ __device__ __forceinline__ void du_center(const Dims &d, int i, int j, int k, co
 #undef NV
 }
 
+// Dual number (value, tangent) for the tangent-linear (JVP) instantiation of
+// the face fluxes; |x| differentiates to sign(x) (0 at x = 0).
+struct Dual {
+    double v, d;
+    __device__ __forceinline__ Dual() : v(0), d(0) {}
+    __device__ __forceinline__ Dual(double a) : v(a), d(0) {}
+    __device__ __forceinline__ Dual(double a, double b) : v(a), d(b) {}
+};
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual(a.v + b.v, a.d + b.d); }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual(a.v - b.v, a.d - b.d); }
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual(-a.v, -a.d); }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual(a.v * b.v, a.d * b.v + a.v * b.d); }
+__device__ __forceinline__ Dual operator*(double a, Dual b) { return Dual(a * b.v, a * b.d); }
+__device__ __forceinline__ Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
+__device__ __forceinline__ Dual &operator+=(Dual &a, Dual b) { a = a + b; return a; }
+__device__ __forceinline__ Dual fabs(Dual a) { return Dual(fabs(a.v), a.v > 0 ? a.d : (a.v < 0 ? -a.d : 0.0)); }
+
+struct GlobalSca {
+    const double *__restrict__ p;
+    Dims d;
+    __device__ __forceinline__ double operator()(int k, int j, int i) const { return p[IX(d, k, j, i)]; }
+};
+
 struct GlobalVec {
     const double *__restrict__ p;
     Dims d;
     __device__ __forceinline__ double operator()(int k, int j, int i, int q) const { return p[3 * IX(d, k, j, i) + q]; }
 };
+struct DualVec {
+    const double *__restrict__ p, *__restrict__ t;
+    Dims d;
+    __device__ __forceinline__ Dual operator()(int k, int j, int i, int q) const
+    {
+        long long o = 3 * IX(d, k, j, i) + q;
+        return Dual(p[o], t[o]);
+    }
+};
 
 // Convective (div) and viscous (visc) flux through face `dir` of point
 // (k,j,i): momentum.c:672-931 (i), 934-1166 (j), 1169-1409 (k).
-template <int DIR>
-__device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const double *__restrict__ nvert,
-                                          const double *__restrict__ ucont, const double *__restrict__ ucat,
-                                          const double *__restrict__ nu_t, int i, int j, int k, double D[3], double V[3])
+// S = double for the residual, Dual for its tangent; U fetches ucat, UCon
+// fetches the DIR component of ucont at a DA index.
+template <int DIR, typename S, typename FU, typename FC, typename FN>
+__device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
+                                            const FC &UCon, const FU &U, const double *__restrict__ nu_t,
+                                            int i, int j, int k, S D[3], S V[3])
 {
     const long long id = IX(d, k, j, i);
     const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
@@ -138,9 +172,8 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
     const double csi0 = fc[3 * id], csi1 = fc[3 * id + 1], csi2 = fc[3 * id + 2];
     const double eta0 = fe[3 * id], eta1 = fe[3 * id + 1], eta2 = fe[3 * id + 2];
     const double zet0 = fz[3 * id], zet1 = fz[3 * id + 1], zet2 = fz[3 * id + 2];
-    GlobalVec U{ucat, d};
-    double dc[3], de[3], dz[3];
-    du_face(DIR, U, nvert, d, i, j, k, dc, de, dz);
+    S dc[3], de[3], dz[3];
+    du_face<S>(DIR, U, N, d, i, j, k, dc, de, dz);
     double g11, g21, g31, n0, n1, n2;
     if (DIR == 0) {
         g11 = csi0 * csi0 + csi1 * csi1 + csi2 * csi2;
@@ -158,24 +191,23 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
         g31 = zet0 * zet0 + zet1 * zet1 + zet2 * zet2;
         n0 = zet0, n1 = zet1, n2 = zet2;
     }
-    const double r11 = dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0;
-    const double r21 = dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0;
-    const double r31 = dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0;
-    const double r12 = dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1;
-    const double r22 = dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1;
-    const double r32 = dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1;
-    const double r13 = dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2;
-    const double r23 = dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2;
-    const double r33 = dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2;
+    const S r11 = dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0;
+    const S r21 = dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0;
+    const S r31 = dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0;
+    const S r12 = dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1;
+    const S r22 = dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1;
+    const S r32 = dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1;
+    const S r13 = dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2;
+    const S r23 = dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2;
+    const S r33 = dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2;
 
     const int n = DIR == 0 ? i : DIR == 1 ? j : k;
     const int m = DIR == 0 ? d.mx : DIR == 1 ? d.my : d.mz;
     const int per = DIR == 0 ? d.ip : DIR == 1 ? d.jp : d.kp;
     const long long st = DIR == 0 ? 1 : DIR == 1 ? (long long)d.mx : (long long)d.mx * d.my;
     const long long base = id - (long long)n * st;   // point with index 0 along DIR
-#define NB(tt) nvert[base + (long long)(tt) * st]
-#define UB(tt, q) ucat[3 * (base + (long long)(tt) * st) + (q)]
-#define UC(tt) ucont[3 * (base + (long long)(tt) * st) + DIR]
+#define NB(tt) N(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i)
+#define UB(tt, q) U(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i, q)
     int L = n - 1, R = n + 2;
     if (n == 0 || n == m - 2) {
         if (per) L = m - 3, R = 2;
@@ -184,11 +216,11 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
         L = n, R = n + 1;
     }
     if (d.second_order) L = n, R = n + 1;
-    const double ucon_f = UC(n);
-    double ucon = ucon_f;
-    if (per && (n == 0 || n == m - 2)) ucon = UC(m - 2);
-    const double up = -0.5 * (ucon + fabs(ucon));
-    const double um = -0.5 * (ucon - fabs(ucon));
+    const S ucon_f = UCon(base + (long long)n * st);
+    S ucon = ucon_f;
+    if (per && (n == 0 || n == m - 2)) ucon = UCon(base + (long long)(m - 2) * st);
+    const S up = -0.5 * (ucon + fabs(ucon));
+    const S um = -0.5 * (ucon - fabs(ucon));
     if (d.immersed && n != m - 2 && NB(n) > 0.1) {
 #pragma unroll
         for (int q = 0; q < 3; q++)
@@ -206,10 +238,10 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
     }
-    if (NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = 0;
+    if (NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = S(0.0);
 
     const double nu = 1. / d.ren;
-    double v0, v1, v2;
+    S v0, v1, v2;
     if (d.les) {
         double nut;
         if ((n == 0 && !per) || NB(n) > 0.1) nut = nu_t[base + (long long)(n + 1) * st];
@@ -219,7 +251,7 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
         v1 = (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nut);
         v2 = (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nut);
     } else {
-        v0 = v1 = v2 = 0;
+        v0 = v1 = v2 = S(0.0);
     }
     v0 += (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nu);
     v1 += (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nu);
@@ -227,23 +259,45 @@ __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const do
     V[0] = v0; V[1] = v1; V[2] = v2;
 #undef NB
 #undef UB
-#undef UC
+}
+
+struct UConPlain {
+    const double *__restrict__ p;
+    int dir;
+    __device__ __forceinline__ double operator()(long long o) const { return p[3 * o + dir]; }
+};
+struct UConDual {
+    const double *__restrict__ p, *__restrict__ t;
+    int dir;
+    __device__ __forceinline__ Dual operator()(long long o) const { return Dual(p[3 * o + dir], t[3 * o + dir]); }
+};
+
+template <int DIR>
+__device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const double *__restrict__ nvert,
+                                          const double *__restrict__ ucont, const double *__restrict__ ucat,
+                                          const double *__restrict__ nu_t, int i, int j, int k, double D[3], double V[3])
+{
+    face_flux_t<DIR, double>(d, g, GlobalSca{nvert, d}, UConPlain{ucont, DIR}, GlobalVec{ucat, d}, nu_t, i, j, k, D, V);
 }
 
 // index of a face along a periodic direction as stored: face 0 is a copy of
 // face m-2 (momentum.c:1463-1504)
 __device__ __forceinline__ int wrap_face(int f, int m, int per) { return (per && f == 0) ? m - 2 : f; }
 
-// fp of cell (k,j,i) from the face arrays (momentum.c:1506-1636).
-// face: slot-major SoA, slot s = 3*kind + comp with kind 0..2 div1..3, 3..5 visc1..3.
-__device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict__ nvert,
-                                        const double *__restrict__ face, int i, int j, int k, double f[3])
+// Face indices read by the cell divergence of cell (k,j,i): the upwind face
+// of the second-order difference (im/jm/km) and the far faces and divisor of
+// the 4th-order correction (iR/iL/den_i ...), momentum.c:1506-1600.
+struct CellFaces {
+    int im, jm, km, iR, iL, jR, jL, kR, kL;
+    double den_i, den_j, den_k;
+};
+
+__device__ __forceinline__ CellFaces cell_faces(const Dims &d, const double *__restrict__ nvert, int i, int j, int k)
 {
-    const long long P = d.P;
     const int ip = d.ip, jp = d.jp, kp = d.kp, mx = d.mx, my = d.my, mz = d.mz;
 #define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
-#define FI(kind, q, kk, jj, ii) face[(long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii)]
-    const int im = wrap_face(i - 1, mx, ip), jm = wrap_face(j - 1, my, jp), km = wrap_face(k - 1, mz, kp);
+    CellFaces c;
+    c.im = wrap_face(i - 1, mx, ip); c.jm = wrap_face(j - 1, my, jp); c.km = wrap_face(k - 1, mz, kp);
     int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
     double den_i = 3., den_j = 3., den_k = 3.;
     if (i == 1) { if (ip) iL = mx - 3; else iR = i, iL = i - 1, den_i = 1.; }
@@ -258,7 +312,22 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
     if (NV(k, j, i - 1) + NV(k, j, i) + NV(k, j, i + 1) > 0.1) iR = i, iL = i - 1, den_i = 1.;
     if (NV(k, j - 1, i) + NV(k, j, i) + NV(k, j + 1, i) > 0.1) jR = j, jL = j - 1, den_j = 1.;
     if (NV(k - 1, j, i) + NV(k, j, i) + NV(k + 1, j, i) > 0.1) kR = k, kL = k - 1, den_k = 1.;
-    iL = wrap_face(iL, mx, ip); jL = wrap_face(jL, my, jp); kL = wrap_face(kL, mz, kp);
+    c.iL = wrap_face(iL, mx, ip); c.jL = wrap_face(jL, my, jp); c.kL = wrap_face(kL, mz, kp);
+    c.iR = iR; c.jR = jR; c.kR = kR;
+    c.den_i = den_i; c.den_j = den_j; c.den_k = den_k;
+#undef NV
+    return c;
+}
+
+// fp of cell (k,j,i) from the face arrays (momentum.c:1506-1636).
+// face: slot-major SoA, slot s = 3*kind + comp with kind 0..2 div1..3, 3..5 visc1..3.
+__device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict__ nvert,
+                                        const double *__restrict__ face, int i, int j, int k, double f[3])
+{
+    const long long P = d.P;
+#define FI(kind, q, kk, jj, ii) face[(long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii)]
+    const CellFaces c = cell_faces(d, nvert, i, j, k);
+    const int im = c.im, jm = c.jm, km = c.km;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
@@ -268,16 +337,15 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
             r = dv;
         } else {
             double d4 = 0;
-            d4 += (FI(0, q, k, j, iR) - FI(0, q, k, j, iL)) * (1. / den_i);
-            d4 += (FI(1, q, k, jR, i) - FI(1, q, k, jL, i)) * (1. / den_j);
-            d4 += (FI(2, q, kR, j, i) - FI(2, q, kL, j, i)) * (1. / den_k);
+            d4 += (FI(0, q, k, j, c.iR) - FI(0, q, k, j, c.iL)) * (1. / c.den_i);
+            d4 += (FI(1, q, k, c.jR, i) - FI(1, q, k, c.jL, i)) * (1. / c.den_j);
+            d4 += (FI(2, q, c.kR, j, i) - FI(2, q, c.kL, j, i)) * (1. / c.den_k);
             r = 9. / 8. * dv + -1. / 8. * d4;
         }
         r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) +
               FI(5, q, k, j, i) - FI(5, q, km, j, i)) / 1.0;
         f[q] = r;
     }
-#undef NV
 #undef FI
 }
 

@@ -207,6 +207,16 @@ class Ctx:
     def rhs_vjp(self, U, ucat, nu_t, v, jtv):
         _ck(lib().vfs_rhs_vjp(self.h, U.ptr, ucat.ptr, _p(nu_t), v.ptr, jtv.ptr), "vfs_rhs_vjp")
 
+    def implicit_matrixfree(self, U, U_o, rhs_o, dP, nu_t, ucat, rtol=1e-4, max_newton=50, restart=30,
+                            max_krylov=1000):
+        """Implicit_MatrixFree: returns (newton_its, krylov_its, final ||F||)."""
+        ni, ki, f = C.c_int(), C.c_int(), C.c_double()
+        _ck(lib().vfs_implicit_matrixfree(self.h, U.ptr, U_o.ptr, rhs_o.ptr, dP.ptr, _p(nu_t), ucat.ptr,
+                                          C.c_double(rtol), C.c_int(max_newton), C.c_int(restart),
+                                          C.c_int(max_krylov), C.byref(ni), C.byref(ki), C.byref(f)),
+            "vfs_implicit_matrixfree")
+        return ni.value, ki.value, f.value
+
     # pressure
     def poisson_setup(self):
         n = C.c_int()
