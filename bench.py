@@ -57,7 +57,7 @@ class HipEvents:
         return ms.value
 
 
-def setup(name, N=None):
+def setup(name, N=None, poisson=True):
     import vfsgpu as V
     from vfsgpu import workloads as W
     cfg, spec = W.make_config(name, N)
@@ -69,7 +69,7 @@ def setup(name, N=None):
     nvert = ctx.nvert().host()
     metrics = {n: ctx.metric(n).host() for n in ("cent", "icsi", "jeta", "kzet")}
     ucat, ucont = W.fields(cfg, spec, metrics, nvert)
-    n_unknowns = ctx.poisson_setup()
+    n_unknowns = ctx.poisson_setup() if poisson else 0
     return cfg, spec, ctx, ucat, ucont, n_unknowns
 
 
@@ -127,6 +127,37 @@ def time_rhs(ctx, cfg, s, reps):
     return ev.elapsed_ms(a, b) / reps
 
 
+def traffic_from_profile(name):
+    """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
+    scaled by the same counter on a device copy of known size), or None."""
+    import re
+    path = os.path.join(ROOT, "profiles", f"formfunction2_{name}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_call\s*=\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def rhs_only(ctx, cfg, s, reps, calib_reps):
+    """Short run for the counter passes: Formfunction_2 `reps` times, then a
+    device-to-device copy of 3P doubles `calib_reps` times (the calibration
+    kernel of known traffic)."""
+    from vfsgpu import workloads as W
+    import vfsgpu as V
+    ctx.contra2cart(s.uc, s.u)
+    if cfg["les"]:
+        ctx.eddy_viscosity(s.u, W.CS_LES, s.nut)
+    for _ in range(reps):
+        ctx.formfunction2(s.uc, s.u, s.nut, s.rhs, 1.0)
+    ctx.sync()
+    nbytes = 3 * ctx.P * 8
+    for _ in range(calib_reps):
+        V._ck(V.lib().vfs_memcpy_dtod(s.rhs.ptr, s.u.ptr, C.c_size_t(nbytes)), "vfs_memcpy_dtod")
+    ctx.sync()
+    log(f"[bench] rhs-only: {reps} Formfunction_2 calls, {calib_reps} copies of {nbytes} bytes")
+
+
 def cpu_baseline(name, budget_s):
     """The CPU oracle's Formfunction_2 on a bounded sample of the same
     generator (single thread)."""
@@ -166,17 +197,23 @@ def main():
     ap.add_argument("--rhs-reps", type=int, default=20)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--rhs-only", action="store_true",
+                    help="only Formfunction_2 calls and a calibration copy (for counter runs)")
+    ap.add_argument("--calib-reps", type=int, default=5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the multi-GPU k-slab decomposition is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
-    cfg, spec, ctx, ucat, ucont, nunk = setup(args.config)
+    cfg, spec, ctx, ucat, ucont, nunk = setup(args.config, poisson=not args.rhs_only)
     Ni, Nj, Nk = spec["N"]
     cells = Ni * Nj * Nk
     log(f"[bench] {args.config} {Ni}x{Nj}x{Nk} cells={cells} unknowns={nunk}")
     s = State(ctx, ucat, ucont)
+    if args.rhs_only:
+        rhs_only(ctx, cfg, s, args.rhs_reps, args.calib_reps)
+        return
 
     res = {}
     for mode, prec in (("fp64", 0), ("mixed", 1)):
@@ -239,7 +276,7 @@ def main():
         "step_ms": {"fp64": round(res["fp64"]["step_ms"], 3), "mixed": round(res["mixed"]["step_ms"], 3),
                     "speedup_mixed_vs_fp64": round(res["fp64"]["step_ms"] / res["mixed"]["step_ms"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.config),
                      "kernel": "Formfunction_2", "bytes_per_cell": bytes_per_cell},
         "cpu_baseline": cpu,
     }

@@ -24,16 +24,38 @@
 int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale);
 int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
 
+// Build-time shape parameters (tuned on MI355X; see the Makefile).
+#ifndef VFS_FACES_KCHUNK
+#define VFS_FACES_KCHUNK 16      // k-planes per workgroup of the face kernel
+#endif
+#ifndef VFS_FACES_SPLIT
+#define VFS_FACES_SPLIT 0        // 1: convective and viscous fluxes in separate waves
+#endif
+#ifndef VFS_FACES_WAVES
+#define VFS_FACES_WAVES 2        // target waves per SIMD of the face kernel
+#endif
+#ifndef VFS_CELLRHS
+#define VFS_CELLRHS 0            // 1: fused divergence+projection, 0: two passes
+#endif
+#ifndef VFS_CELLRHS_KCHUNK
+#define VFS_CELLRHS_KCHUNK 16
+#endif
+
 namespace {
 
 constexpr int TI = 64;          // cells per workgroup row (one wavefront)
-constexpr int TJ = 4;           // rows per workgroup
+#ifndef VFS_FACES_TJ
+#define VFS_FACES_TJ 4
+#endif
+constexpr int TJ = VFS_FACES_TJ; // rows per workgroup
 constexpr int BI = TI + 3;      // staged columns: i0-1 .. i0+TI+1
 constexpr int BJ = TJ + 3;      // staged rows:    j0-1 .. j0+TJ+1
 constexpr int BP = BI * BJ;     // points per staged plane
-constexpr int NT = TI * TJ;     // threads
-constexpr int NU = (3 * BP + NT - 1) / NT;   // ucat doubles per thread per plane
-constexpr int NN = (BP + NT - 1) / NT;       // nvert values per thread per plane
+constexpr int NT = TI * TJ;     // threads per face group
+constexpr int NG = VFS_FACES_SPLIT ? 2 : 1;  // groups: (convective, viscous) or both in one
+constexpr int NTB = NT * NG;    // threads per workgroup
+constexpr int NU = (3 * BP + NTB - 1) / NTB;   // ucat doubles per thread per plane
+constexpr int NN = (BP + NTB - 1) / NTB;       // nvert values per thread per plane
 
 // staged planes (one set per workgroup): ucat [4][3][BJ][BI], nvert [4][BJ][BI]
 __shared__ double sU[4 * 3 * BP];
@@ -65,7 +87,7 @@ __device__ __forceinline__ void fetch_plane(const Dims &d, const double *__restr
     const bool kin = k >= 0 && k < d.mz;
 #pragma unroll
     for (int n = 0; n < NU; n++) {
-        const int e = threadIdx.x + n * NT;
+        const int e = threadIdx.x + n * NTB;
         const int r = e / (3 * BI), c = e - r * (3 * BI);
         const int gi = i0 - 1 + c / 3, gj = j0 - 1 + r;
         double v = 0;
@@ -74,7 +96,7 @@ __device__ __forceinline__ void fetch_plane(const Dims &d, const double *__restr
     }
 #pragma unroll
     for (int n = 0; n < NN; n++) {
-        const int e = threadIdx.x + n * NT;
+        const int e = threadIdx.x + n * NTB;
         const int r = e / BI, c = e - r * BI;
         const int gi = i0 - 1 + c, gj = j0 - 1 + r;
         float v = 0;
@@ -88,7 +110,7 @@ __device__ __forceinline__ void store_plane(int k, const double (&ru)[NU], const
     const int slot = k & 3;
 #pragma unroll
     for (int n = 0; n < NU; n++) {
-        const int e = threadIdx.x + n * NT;
+        const int e = threadIdx.x + n * NTB;
         if (e < 3 * BP) {
             const int r = e / (3 * BI), c = e - r * (3 * BI);
             const int p = c / 3, q = c - 3 * p;
@@ -97,22 +119,66 @@ __device__ __forceinline__ void store_plane(int k, const double (&ru)[NU], const
     }
 #pragma unroll
     for (int n = 0; n < NN; n++) {
-        const int e = threadIdx.x + n * NT;
+        const int e = threadIdx.x + n * NTB;
         if (e < BP) sN[slot * BP + e] = rn[n];
     }
 }
 
 }  // namespace
 
+// The three face fluxes of point (k,j,i) from the staged planes, written to
+// the face array.  PARTS: 1 convective only, 2 viscous only, 3 both (the
+// unused part is removed at compile time).  The periodic wrap-around faces
+// are left to k_ff2_faces_bnd.
+template <int PARTS>
+__device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const double *__restrict__ ucont,
+                                         const double *__restrict__ nu_t, double *__restrict__ face, int i0, int j0,
+                                         int i, int j, int k)
+{
+    const LdsU U{i0, j0};
+    const LdsN N{i0, j0};
+    const long long P = d.P;
+    const long long t = IX(d, k, j, i);
+    double D[3], V[3];
+    if (!(d.ip && i == d.mx - 2)) {
+        face_flux_t<0, double>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (PARTS & 1) face[(0 * 3 + q) * P + t] = D[q];
+            if (PARTS & 2) face[(3 * 3 + q) * P + t] = V[q];
+        }
+    }
+    // one face at a time: keeps the live registers of a single face
+    __builtin_amdgcn_sched_barrier(0);
+    if (!(d.jp && j == d.my - 2)) {
+        face_flux_t<1, double>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (PARTS & 1) face[(1 * 3 + q) * P + t] = D[q];
+            if (PARTS & 2) face[(4 * 3 + q) * P + t] = V[q];
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (!(d.kp && k == d.mz - 2)) {
+        face_flux_t<2, double>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (PARTS & 1) face[(2 * 3 + q) * P + t] = D[q];
+            if (PARTS & 2) face[(5 * 3 + q) * P + t] = V[q];
+        }
+    }
+}
+
 // Face fluxes of all three directions at every interior point
 // (1..mx-2, 1..my-2, k in [kb, ke)).  grid: (ceil((mx-2)/64), ceil((my-2)/4), k chunks)
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert,
+__global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FACES_WAVES, VFS_FACES_WAVES))) k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert,
                                                       const double *__restrict__ ucont,
                                                       const double *__restrict__ ucat,
                                                       const double *__restrict__ nu_t, double *__restrict__ face,
                                                       int kchunk)
 {
-    const int tx = threadIdx.x % TI, ty = threadIdx.x / TI;
+    const int tx = threadIdx.x % TI, ty = (threadIdx.x / TI) % TJ;
+    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);   // 0: convective (or both), 1: viscous
     const int i0 = 1 + blockIdx.x * TI, j0 = 1 + blockIdx.y * TJ;
     const int kb = 1 + blockIdx.z * kchunk;
     const int ke = min(kb + kchunk, d.mz - 1);
@@ -131,28 +197,9 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(2, 2)))
         // prefetch plane k+3 while plane k is computed
         fetch_plane(d, ucat, nvert, i0, j0, k + 3, ru, rn);
         if (active) {
-            const LdsU U{i0, j0};
-            const LdsN N{i0, j0};
-            const long long t = IX(d, k, j, i);
-            double D[3], V[3];
-            if (!(d.ip && i == d.mx - 2)) {
-                face_flux_t<0, double>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
-#pragma unroll
-                for (int q = 0; q < 3; q++) { face[(0 * 3 + q) * P + t] = D[q]; face[(3 * 3 + q) * P + t] = V[q]; }
-            }
-            // one face at a time: keeps the live registers of a single face
-            __builtin_amdgcn_sched_barrier(0);
-            if (!(d.jp && j == d.my - 2)) {
-                face_flux_t<1, double>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
-#pragma unroll
-                for (int q = 0; q < 3; q++) { face[(1 * 3 + q) * P + t] = D[q]; face[(4 * 3 + q) * P + t] = V[q]; }
-            }
-            __builtin_amdgcn_sched_barrier(0);
-            if (!(d.kp && k == d.mz - 2)) {
-                face_flux_t<2, double>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
-#pragma unroll
-                for (int q = 0; q < 3; q++) { face[(2 * 3 + q) * P + t] = D[q]; face[(5 * 3 + q) * P + t] = V[q]; }
-            }
+            if (!VFS_FACES_SPLIT) faces_at<3>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            else if (grp == 0) faces_at<1>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            else faces_at<2>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
         }
         __syncthreads();   // every read of plane k-1 is done
         store_plane(k + 3, ru, rn);
@@ -197,13 +244,89 @@ __global__ void k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert
     }
 }
 
+// Cell divergence (momentum.c:1506-1636) and projection onto the faces
+// (momentum.c:1673-1900) in one k-marching pass over a 64 x 4 column of
+// cells.  Each thread keeps fp of its own cell at planes k and k+1; the fp of
+// the i+1 column and the j+1 row of the tile (periodic-wrapped) are computed
+// by the first 68 threads; neighbours are exchanged through LDS.
+constexpr int CI = 64, CJ = 4, CT = CI * CJ;
+
+__global__ void __launch_bounds__(CT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_ff2_cellrhs(Dims d, Geom g, const double *__restrict__ nvert,
+                                                   const double *__restrict__ face, double *__restrict__ rhs,
+                                                   double scale, int kchunk)
+{
+    __shared__ double sfp[3][CJ + 1][CI + 1];
+    const int tx = threadIdx.x % CI, ty = threadIdx.x / CI;
+    const int i0 = 1 + blockIdx.x * CI, j0 = 1 + blockIdx.y * CJ;
+    const int kb = 1 + blockIdx.z * kchunk;
+    const int ke = min(kb + kchunk, d.mz - 1);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool active = i <= d.mx - 2 && j <= d.my - 2;
+    // last tile column / row that holds an interior cell
+    const int ilast = min(i0 + CI - 1, d.mx - 2), jlast = min(j0 + CJ - 1, d.my - 2);
+    // halo work: threads 0..CI-1 the j+1 row, CI..CI+CJ-1 the i+1 column
+    int hi = -1, hj = -1, hslot_i = 0, hslot_j = 0;
+    if (threadIdx.x < CI) {
+        hi = i0 + threadIdx.x; hj = wrap_cell(jlast + 1, d.my, d.jp);
+        hslot_i = threadIdx.x; hslot_j = CJ;
+        if (hi > d.mx - 2) hi = -1;
+    } else if (threadIdx.x < CI + CJ) {
+        hj = j0 + threadIdx.x - CI; hi = wrap_cell(ilast + 1, d.mx, d.ip);
+        hslot_i = CI; hslot_j = threadIdx.x - CI;
+        if (hj > d.my - 2) hi = -1;
+    }
+    // a wrapped neighbour that is a ghost (non-periodic) only feeds masked faces
+    const bool hvalid = hi >= 1 && hi <= d.mx - 2 && hj >= 1 && hj <= d.my - 2;
+
+    double f0[3] = {0, 0, 0}, f1[3] = {0, 0, 0};
+    if (active) cell_fp(d, nvert, face, i, j, kb, f0);
+    for (int k = kb; k < ke; k++) {
+        const int kn = wrap_cell(k + 1, d.mz, d.kp);
+        if (active) {
+            if (kn <= d.mz - 2) cell_fp(d, nvert, face, i, j, kn, f1);
+            else f1[0] = f1[1] = f1[2] = 0;
+            sfp[0][ty][tx] = f0[0]; sfp[1][ty][tx] = f0[1]; sfp[2][ty][tx] = f0[2];
+        }
+        if (hvalid) {
+            double h[3];
+            cell_fp(d, nvert, face, hi, hj, k, h);
+            sfp[0][hslot_j][hslot_i] = h[0]; sfp[1][hslot_j][hslot_i] = h[1]; sfp[2][hslot_j][hslot_i] = h[2];
+        } else if (threadIdx.x < CI + CJ) {
+            sfp[0][hslot_j][hslot_i] = 0; sfp[1][hslot_j][hslot_i] = 0; sfp[2][hslot_j][hslot_i] = 0;
+        }
+        __syncthreads();
+        if (active) {
+            const long long id = IX(d, k, j, i);
+            const int ai = i < ilast ? tx + 1 : CI, bj = j < jlast ? ty + 1 : CJ;
+            const int ay = i < ilast ? ty : ty, bx = j < jlast ? tx : tx;
+            double fa[3], fb[3];
+            for (int q = 0; q < 3; q++) { fa[q] = sfp[q][ay][ai]; fb[q] = sfp[q][bj][bx]; }
+            double r[3] = {rhs[3 * id], rhs[3 * id + 1], rhs[3 * id + 2]};
+            project_rhs_v(d, g, nvert, i, j, k, id, scale, f0, fa, fb, f1, r);
+            rhs[3 * id] = r[0]; rhs[3 * id + 1] = r[1]; rhs[3 * id + 2] = r[2];
+        }
+        __syncthreads();
+        f0[0] = f1[0]; f0[1] = f1[1]; f0[2] = f1[2];
+    }
+}
+
+// rhs = 0 on the ghost shell (the boundary planes of k_ff2_rhs)
+__global__ void k_zero_shell_vec(Dims d, double *v)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int k, j, i;
+    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
+    long long o = IX(d, k, j, i);
+    v[3 * o] = v[3 * o + 1] = v[3 * o + 2] = 0;
+}
+
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale)
 {
     if (!c->nvert_f32) return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
     const Dims d = c->d;
-    const int kchunk = 16;
+    const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL(k_ff2_faces_lds, grid, dim3(NT), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t, c->face,
+    hipLaunchKernelGGL(k_ff2_faces_lds, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t, c->face,
                        kchunk);
     VFS_CHECK_LAUNCH();
     const long long n3[3] = {(long long)(d.my - 2) * (d.mz - 2), (long long)(d.mx - 2) * (d.mz - 2),
@@ -212,5 +335,12 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     hipLaunchKernelGGL(k_ff2_faces_bnd, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont,
                        ucat, nu_t, c->face);
     VFS_CHECK_LAUNCH();
-    return vfs_ff2_from_faces(c, rhs, scale);
+    if (!VFS_CELLRHS) return vfs_ff2_from_faces(c, rhs, scale);
+    const int kc2 = VFS_CELLRHS_KCHUNK;
+    const dim3 g2((d.mx - 2 + CI - 1) / CI, (d.my - 2 + CJ - 1) / CJ, (d.mz - 2 + kc2 - 1) / kc2);
+    hipLaunchKernelGGL(k_ff2_cellrhs, g2, dim3(CT), 0, c->stream, d, c->g, c->nvert, c->face, rhs, scale, kc2);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_zero_shell_vec, dim3(grid_for(shell_count(d), 256)), dim3(256), 0, c->stream, d, rhs);
+    VFS_CHECK_LAUNCH();
+    return 0;
 }

@@ -18,8 +18,8 @@ struct MGLevel {
     double *tl64, *tu64, *ti64;
     float *tl32, *tu32, *ti32;
     // work vectors (both precisions)
-    double *b64, *x64, *y64, *d64;
-    float *b32, *x32, *y32, *d32;
+    double *b64, *x64, *y64, *d64, *e64;
+    float *b32, *x32, *y32, *d32, *e32;
 };
 
 struct Poisson {
@@ -41,4 +41,5 @@ struct Poisson {
     double *dots;                       // device scalars for multi-dot results
     double *hdots;                      // pinned host mirror
     double omega, alpha;
+    int wdepth;                         // levels l < wdepth visit level l+1 twice (W-cycle)
 };

@@ -473,8 +473,14 @@ __global__ void k_set_scalar(double *p, double v) { p[0] = v; }
 // host side
 // ---------------------------------------------------------------------------
 template <typename T> static T *vec_of(MGLevel &l, int which);
-template <> double *vec_of<double>(MGLevel &l, int w) { return w == 0 ? l.b64 : w == 1 ? l.x64 : w == 2 ? l.y64 : l.d64; }
-template <> float *vec_of<float>(MGLevel &l, int w) { return w == 0 ? l.b32 : w == 1 ? l.x32 : w == 2 ? l.y32 : l.d32; }
+template <> double *vec_of<double>(MGLevel &l, int w)
+{
+    return w == 0 ? l.b64 : w == 1 ? l.x64 : w == 2 ? l.y64 : w == 3 ? l.d64 : l.e64;
+}
+template <> float *vec_of<float>(MGLevel &l, int w)
+{
+    return w == 0 ? l.b32 : w == 1 ? l.x32 : w == 2 ? l.y32 : w == 3 ? l.d32 : l.e32;
+}
 template <typename T> static const T *A_of(const MGLevel &l);
 template <> const double *A_of<double>(const MGLevel &l) { return l.A64; }
 template <> const float *A_of<float>(const MGLevel &l) { return l.A32; }
@@ -486,8 +492,8 @@ static void free_level(MGLevel &l)
 {
     hipFree(l.A64); hipFree(l.A32); hipFree(l.tl64); hipFree(l.tu64); hipFree(l.ti64);
     hipFree(l.tl32); hipFree(l.tu32); hipFree(l.ti32);
-    hipFree(l.b64); hipFree(l.x64); hipFree(l.y64); hipFree(l.d64);
-    hipFree(l.b32); hipFree(l.x32); hipFree(l.y32); hipFree(l.d32);
+    hipFree(l.b64); hipFree(l.x64); hipFree(l.y64); hipFree(l.d64); hipFree(l.e64);
+    hipFree(l.b32); hipFree(l.x32); hipFree(l.y32); hipFree(l.d32); hipFree(l.e32);
 }
 
 int vfs_poisson_destroy(vfs_ctx *c)
@@ -531,6 +537,7 @@ int vfs_poisson_alloc(vfs_ctx *c)
     DMALLOC(ps->rowflag, P);
     ps->omega = 0.8;
     ps->alpha = 1.0;
+    ps->wdepth = 2;
     ps->m = 40;
     return 0;
 }
@@ -543,9 +550,9 @@ static int alloc_level(vfs_ctx *c, MGLevel &l)
     DMALLOC(l.tl64, n * sizeof(double)); DMALLOC(l.tu64, n * sizeof(double)); DMALLOC(l.ti64, n * sizeof(double));
     DMALLOC(l.tl32, n * sizeof(float)); DMALLOC(l.tu32, n * sizeof(float)); DMALLOC(l.ti32, n * sizeof(float));
     DMALLOC(l.b64, n * sizeof(double)); DMALLOC(l.x64, n * sizeof(double));
-    DMALLOC(l.y64, n * sizeof(double)); DMALLOC(l.d64, n * sizeof(double));
+    DMALLOC(l.y64, n * sizeof(double)); DMALLOC(l.d64, n * sizeof(double)); DMALLOC(l.e64, n * sizeof(double));
     DMALLOC(l.b32, n * sizeof(float)); DMALLOC(l.x32, n * sizeof(float));
-    DMALLOC(l.y32, n * sizeof(float)); DMALLOC(l.d32, n * sizeof(float));
+    DMALLOC(l.y32, n * sizeof(float)); DMALLOC(l.d32, n * sizeof(float)); DMALLOC(l.e32, n * sizeof(float));
     return 0;
 }
 
@@ -738,6 +745,19 @@ static int vcycle(vfs_ctx *c, int l)
     hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, Lv, Cv, (const T *)y, vec_of<T>(C, 0));
     VFS_CHECK_LAUNCH();
     if (vcycle<T>(c, l + 1)) return -1;
+    if (l < ps->wdepth && l + 2 < (int)ps->lv.size()) {
+        // W-cycle: a second coarse correction on the residual of the first
+        // (a V-cycle alone loses about one digit per level added on the
+        // piecewise-constant aggregation hierarchy)
+        const T *Ac = A_of<T>(C);
+        T *cb = vec_of<T>(C, 0), *cx = vec_of<T>(C, 1), *cy = vec_of<T>(C, 2), *ce = vec_of<T>(C, 4);
+        if (launch_resid<T>(c, C, Ac, cx, cb, cy)) return -1;
+        VFS_CHECK_HIP(hipMemcpyAsync(ce, cx, (size_t)C.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        VFS_CHECK_HIP(hipMemcpyAsync(cb, cy, (size_t)C.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+        if (vcycle<T>(c, l + 1)) return -1;
+        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, C.n, (const T *)ce, cx);
+        VFS_CHECK_LAUNCH();
+    }
     hipLaunchKernelGGL(k_prolong<T>, row_grid(L), dim3(row_block(L.nx)), 0, c->stream, Lv, Cv,
                        (const T *)vec_of<T>(C, 1), alpha, x);
     VFS_CHECK_LAUNCH();

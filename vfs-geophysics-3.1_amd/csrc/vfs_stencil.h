@@ -6,83 +6,45 @@
 
 #define VFS_SOLID 0.1 // k-omega.c:11
 
-// face-centred velocity gradients (k-omega.c:48-278)
+// face-centred velocity gradients (k-omega.c:48-278).
+// The face joins cell c0 = (k,j,i) and c1 = c0 + e_dir.  The normal
+// derivative is U(c1) - U(c0).  Each transverse derivative (direction t) is
+//   (U(c1 + p e_t) + U(c0 + p e_t) - U(c1 + m e_t) - U(c0 + m e_t)) * s
+// with (p, m, s) = (0, -1, 0.5) when a cell on the + side is solid,
+// (1, 0, 0.5) when one on the - side is, and (1, -1, 0.25) otherwise
+// (k-omega.c:74-104 and the analogous j/k branches).  Selecting (p, m, s)
+// first and reading once gives the same expression as each branch of the
+// reference, in the same order.
 template <typename S, typename Fetch, typename NFetch>
 __device__ __forceinline__ void du_face(int dir, const Fetch &U, const NFetch &N, const Dims &d,
                                         int i, int j, int k, S dc[3], S de[3], S dz[3])
 {
-#define NV(kk, jj, ii) N(kk, jj, ii)
-    if (dir == 0) {
+    (void)d;
+    // unit offsets of the face normal and the two transverse directions
+    const int ni = dir == 0, nj = dir == 1, nk = dir == 2;
+    const int i1 = i + ni, j1 = j + nj, k1 = k + nk;   // c1
+    S *dn = dir == 0 ? dc : dir == 1 ? de : dz;
 #pragma unroll
-        for (int q = 0; q < 3; q++) dc[q] = U(k, j, i + 1, q) - U(k, j, i, q);
-        if (NV(k, j + 1, i) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
+    for (int q = 0; q < 3; q++) dn[q] = U(k1, j1, i1, q) - U(k, j, i, q);
 #pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k, j, i + 1, q) + U(k, j, i, q) - U(k, j - 1, i + 1, q) - U(k, j - 1, i, q)) * 0.5;
-        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k, j - 1, i + 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k, j + 1, i + 1, q) + U(k, j + 1, i, q) - U(k, j, i + 1, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k, j + 1, i + 1, q) + U(k, j + 1, i, q) - U(k, j - 1, i + 1, q) - U(k, j - 1, i, q)) * 0.25;
+    for (int tt = 0; tt < 2; tt++) {
+        // transverse direction: the two of {i, j, k} other than dir, in order
+        const int t = dir == 0 ? 1 + tt : dir == 1 ? 2 * tt : tt;
+        const int ti = t == 0, tj = t == 1, tk = t == 2;
+        int p = 1, m = -1;
+        double sc = 0.25;
+        if (N(k + tk, j + tj, i + ti) > VFS_SOLID || N(k1 + tk, j1 + tj, i1 + ti) > VFS_SOLID) {
+            p = 0; m = -1; sc = 0.5;
+        } else if (N(k - tk, j - tj, i - ti) > VFS_SOLID || N(k1 - tk, j1 - tj, i1 - ti) > VFS_SOLID) {
+            p = 1; m = 0; sc = 0.5;
         }
-        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
+        S *dt = t == 0 ? dc : t == 1 ? de : dz;
 #pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k, j, i + 1, q) + U(k, j, i, q) - U(k - 1, j, i + 1, q) - U(k - 1, j, i, q)) * 0.5;
-        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j, i + 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j, i + 1, q) + U(k + 1, j, i, q) - U(k, j, i + 1, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j, i + 1, q) + U(k + 1, j, i, q) - U(k - 1, j, i + 1, q) - U(k - 1, j, i, q)) * 0.25;
-        }
-    } else if (dir == 1) {
-        if (NV(k, j, i + 1) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i, q) + U(k, j, i, q) - U(k, j + 1, i - 1, q) - U(k, j, i - 1, q)) * 0.5;
-        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k, j + 1, i - 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i + 1, q) + U(k, j, i + 1, q) - U(k, j + 1, i, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k, j + 1, i + 1, q) + U(k, j, i + 1, q) - U(k, j + 1, i - 1, q) - U(k, j, i - 1, q)) * 0.25;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) de[q] = U(k, j + 1, i, q) - U(k, j, i, q);
-        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k, j + 1, i, q) + U(k, j, i, q) - U(k - 1, j + 1, i, q) - U(k - 1, j, i, q)) * 0.5;
-        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j + 1, i) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j + 1, i, q) + U(k + 1, j, i, q) - U(k, j + 1, i, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dz[q] = (U(k + 1, j + 1, i, q) + U(k + 1, j, i, q) - U(k - 1, j + 1, i, q) - U(k - 1, j, i, q)) * 0.25;
-        }
-    } else {
-        if (NV(k, j, i + 1) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i, q) + U(k, j, i, q) - U(k + 1, j, i - 1, q) - U(k, j, i - 1, q)) * 0.5;
-        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k + 1, j, i - 1) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i + 1, q) + U(k, j, i + 1, q) - U(k + 1, j, i, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) dc[q] = (U(k + 1, j, i + 1, q) + U(k, j, i + 1, q) - U(k + 1, j, i - 1, q) - U(k, j, i - 1, q)) * 0.25;
-        }
-        if (NV(k, j + 1, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j, i, q) + U(k, j, i, q) - U(k + 1, j - 1, i, q) - U(k, j - 1, i, q)) * 0.5;
-        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k + 1, j - 1, i) > VFS_SOLID) {
-#pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j + 1, i, q) + U(k, j + 1, i, q) - U(k + 1, j, i, q) - U(k, j, i, q)) * 0.5;
-        } else {
-#pragma unroll
-            for (int q = 0; q < 3; q++) de[q] = (U(k + 1, j + 1, i, q) + U(k, j + 1, i, q) - U(k + 1, j - 1, i, q) - U(k, j - 1, i, q)) * 0.25;
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) dz[q] = U(k + 1, j, i, q) - U(k, j, i, q);
+        for (int q = 0; q < 3; q++)
+            dt[q] = (U(k1 + p * tk, j1 + p * tj, i1 + p * ti, q) + U(k + p * tk, j + p * tj, i + p * ti, q) -
+                     U(k1 + m * tk, j1 + m * tj, i1 + m * ti, q) - U(k + m * tk, j + m * tj, i + m * ti, q)) *
+                    sc;
     }
-#undef NV
 }
 
 // cell-centred gradient for the LES model (k-omega.c:305-423, k==0 quirk at 391)
@@ -158,20 +120,33 @@ struct DualVec {
 // (k,j,i): momentum.c:672-931 (i), 934-1166 (j), 1169-1409 (k).
 // S = double for the residual, Dual for its tangent; U fetches ucat, UCon
 // fetches the DIR component of ucont at a DA index.
-template <int DIR, typename S, typename FU, typename FC, typename FN>
+// Face metrics of the DIR face at DA index id, read from the stored arrays.
+struct StoredFaceMetric {
+    template <int DIR>
+    __device__ __forceinline__ void get(const Dims &, const Geom &g, long long id, double cs[3], double et[3],
+                                        double ze[3], double &aj) const
+    {
+        const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
+        const double *fe = DIR == 0 ? g.ieta : DIR == 1 ? g.jeta : g.keta;
+        const double *fz = DIR == 0 ? g.izet : DIR == 1 ? g.jzet : g.kzet;
+        const double *faj = DIR == 0 ? g.iaj : DIR == 1 ? g.jaj : g.kaj;
+        aj = faj[id];
+#pragma unroll
+        for (int q = 0; q < 3; q++) { cs[q] = fc[3 * id + q]; et[q] = fe[3 * id + q]; ze[q] = fz[3 * id + q]; }
+    }
+};
+
+template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric>
 __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
                                             const FC &UCon, const FU &U, const double *__restrict__ nu_t,
-                                            int i, int j, int k, S D[3], S V[3])
+                                            int i, int j, int k, S D[3], S V[3], const FM &fm = FM())
 {
     const long long id = IX(d, k, j, i);
-    const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
-    const double *fe = DIR == 0 ? g.ieta : DIR == 1 ? g.jeta : g.keta;
-    const double *fz = DIR == 0 ? g.izet : DIR == 1 ? g.jzet : g.kzet;
-    const double *faj = DIR == 0 ? g.iaj : DIR == 1 ? g.jaj : g.kaj;
-    const double ajc = faj[id];
-    const double csi0 = fc[3 * id], csi1 = fc[3 * id + 1], csi2 = fc[3 * id + 2];
-    const double eta0 = fe[3 * id], eta1 = fe[3 * id + 1], eta2 = fe[3 * id + 2];
-    const double zet0 = fz[3 * id], zet1 = fz[3 * id + 1], zet2 = fz[3 * id + 2];
+    double mcs[3], met[3], mze[3], ajc;
+    fm.template get<DIR>(d, g, id, mcs, met, mze, ajc);
+    const double csi0 = mcs[0], csi1 = mcs[1], csi2 = mcs[2];
+    const double eta0 = met[0], eta1 = met[1], eta2 = met[2];
+    const double zet0 = mze[0], zet1 = mze[1], zet2 = mze[2];
     S dc[3], de[3], dz[3];
     du_face<S>(DIR, U, N, d, i, j, k, dc, de, dz);
     double g11, g21, g31, n0, n1, n2;
@@ -351,6 +326,27 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
 
 // cell index of a periodic ghost read: cell m-1 is a copy of cell 1
 __device__ __forceinline__ int wrap_cell(int c, int m, int per) { return (per && c == m - 1) ? 1 : c; }
+
+// rhs += scale * (contravariant projection of fp) with the face masks
+// (momentum.c:1673-1693, 1795-1803), from fp values already at hand: fc at
+// the cell, fa/fb/fcc at its i+1 / j+1 / k+1 neighbours (periodic-wrapped).
+__device__ __forceinline__ void project_rhs_v(const Dims &d, const Geom &g, const double *__restrict__ nvert, int i,
+                                              int j, int k, long long id, double scale, const double fc[3],
+                                              const double fa[3], const double fb[3], const double fcc[3], double r[3])
+{
+    const long long ga = id + 1, gb = id + d.mx, gc = id + (long long)d.mx * d.my;  // metric ghosts are stored
+    const double *cs = g.csi, *et = g.eta, *ze = g.zet;
+    r[0] += scale * (0.5 * (cs[3 * id] * fc[0] + cs[3 * id + 1] * fc[1] + cs[3 * id + 2] * fc[2]) +
+                     0.5 * (cs[3 * ga] * fa[0] + cs[3 * ga + 1] * fa[1] + cs[3 * ga + 2] * fa[2])) * g.iaj[id];
+    r[1] += scale * (0.5 * (et[3 * id] * fc[0] + et[3 * id + 1] * fc[1] + et[3 * id + 2] * fc[2]) +
+                     0.5 * (et[3 * gb] * fb[0] + et[3 * gb + 1] * fb[1] + et[3 * gb + 2] * fb[2])) * g.jaj[id];
+    r[2] += scale * (0.5 * (ze[3 * id] * fc[0] + ze[3 * id + 1] * fc[1] + ze[3 * id + 2] * fc[2]) +
+                     0.5 * (ze[3 * gc] * fcc[0] + ze[3 * gc + 1] * fcc[1] + ze[3 * gc + 2] * fcc[2])) * g.kaj[id];
+    const double n0 = nvert[id];
+    if (n0 + nvert[id + 1] > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
+    if (n0 + nvert[id + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
+    if (n0 + nvert[id + (long long)d.mx * d.my] > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
+}
 
 // rhs += scale * (contravariant projection of fp) with the face masks
 // (momentum.c:1673-1693, 1795-1803)
