@@ -23,6 +23,7 @@
 
 int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale);
 int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
+int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale);
 
 // Build-time shape parameters (tuned on MI355X; see the Makefile).
 #ifndef VFS_FACES_KCHUNK
@@ -33,6 +34,12 @@ int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
 #endif
 #ifndef VFS_FACES_WAVES
 #define VFS_FACES_WAVES 2        // target waves per SIMD of the face kernel
+#endif
+#ifndef VFS_CELLS_KMARCH
+#define VFS_CELLS_KMARCH 1       // k-marching cell divergence
+#endif
+#ifndef VFS_CELLS_KCHUNK
+#define VFS_CELLS_KCHUNK 16
 #endif
 #ifndef VFS_CELLRHS
 #define VFS_CELLRHS 0            // 1: fused divergence+projection, 0: two passes
@@ -45,7 +52,7 @@ namespace {
 
 constexpr int TI = 64;          // cells per workgroup row (one wavefront)
 #ifndef VFS_FACES_TJ
-#define VFS_FACES_TJ 4
+#define VFS_FACES_TJ 8
 #endif
 constexpr int TJ = VFS_FACES_TJ; // rows per workgroup
 constexpr int BI = TI + 3;      // staged columns: i0-1 .. i0+TI+1
@@ -179,12 +186,15 @@ __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FA
 {
     const int tx = threadIdx.x % TI, ty = (threadIdx.x / TI) % TJ;
     const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);   // 0: convective (or both), 1: viscous
-    const int i0 = 1 + blockIdx.x * TI, j0 = 1 + blockIdx.y * TJ;
-    const int kb = 1 + blockIdx.z * kchunk;
+    // logical tile of this workgroup: XCD x takes a contiguous range (x fastest)
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const int bx = lb % gridDim.x, by = (lb / gridDim.x) % gridDim.y, bz = lb / (gridDim.x * gridDim.y);
+    const int i0 = 1 + bx * TI, j0 = 1 + by * TJ;
+    const int kb = 1 + bz * kchunk;
     const int ke = min(kb + kchunk, d.mz - 1);
     const int i = i0 + tx, j = j0 + ty;
     const bool active = i <= d.mx - 2 && j <= d.my - 2;
-    const long long P = d.P;
 
     double ru[NU];
     float rn[NN];
@@ -242,6 +252,85 @@ __global__ void k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert
         for (int q = 0; q < 3; q++) { face[(dir * 3 + q) * P + o] = D[q]; face[((3 + dir) * 3 + q) * P + o] = V[q]; }
         return;
     }
+}
+
+// Cell divergence (momentum.c:1506-1636), k-marching: each thread owns a
+// cell column and keeps the k-direction face fluxes it reuses (convective at
+// planes k-2..k+1, viscous at k-1..k) in registers, so every k-face value is
+// read once per column instead of up to four times.  Planes outside that
+// window (the periodic wrap-around of the first and last cells) are read
+// from memory.  Same expressions and order as cell_fp.
+constexpr int DI = 64, DJ = 4, DT = DI * DJ;
+
+__global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__restrict__ nvert,
+                                                    const double *__restrict__ face, double *__restrict__ fp,
+                                                    int kchunk)
+{
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const int bx = lb % gridDim.x, by = (lb / gridDim.x) % gridDim.y, bz = lb / (gridDim.x * gridDim.y);
+    const int i = 1 + bx * DI + (int)(threadIdx.x % DI), j = 1 + by * DJ + (int)(threadIdx.x / DI);
+    if (i > d.mx - 2 || j > d.my - 2) return;
+    const int kb = 1 + bz * kchunk, ke = min(kb + kchunk, d.mz - 1);
+    const long long P = d.P, sk = (long long)d.mx * d.my;
+    const long long col = IX(d, 0, j, i);
+#define FI(kind, q, kk, jj, ii) face[(long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii)]
+    // ring: convective k-flux at planes k-2..k+1, viscous at k-1..k
+    double c0[3], c1[3], c2[3], c3[3], v0[3], v1[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        const double *D2 = face + (long long)(6 + q) * P + col, *V2 = face + (long long)(15 + q) * P + col;
+        c0[q] = kb - 2 >= 0 ? D2[(kb - 2) * sk] : 0.0;
+        c1[q] = D2[(kb - 1) * sk];
+        c2[q] = D2[kb * sk];
+        c3[q] = kb + 1 <= d.mz - 2 ? D2[(kb + 1) * sk] : 0.0;
+        v0[q] = V2[(kb - 1) * sk];
+        v1[q] = V2[kb * sk];
+    }
+    for (int k = kb; k < ke; k++) {
+        // prefetch the planes the next step brings into the window
+        double nc[3], nv[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            nc[q] = k + 2 <= d.mz - 2 ? face[(long long)(6 + q) * P + col + (k + 2) * sk] : 0.0;
+            nv[q] = face[(long long)(15 + q) * P + col + (k + 1) * sk];
+        }
+        const CellFaces cf = cell_faces(d, nvert, i, j, k);
+        const int im = cf.im, jm = cf.jm, km = cf.km;
+        double f[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            // k-direction fluxes from the window, or memory outside it
+            auto K2 = [&](int kk) -> double {
+                return kk == k - 2 ? c0[q] : kk == k - 1 ? c1[q] : kk == k ? c2[q] : kk == k + 1 ? c3[q]
+                                                                                     : FI(2, q, kk, j, i);
+            };
+            auto K5 = [&](int kk) -> double { return kk == k - 1 ? v0[q] : kk == k ? v1[q] : FI(5, q, kk, j, i); };
+            double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
+                         c2[q] - K2(km));
+            double r;
+            if (d.second_order) {
+                r = dv;
+            } else {
+                double d4 = 0;
+                d4 += (FI(0, q, k, j, cf.iR) - FI(0, q, k, j, cf.iL)) * (1. / cf.den_i);
+                d4 += (FI(1, q, k, cf.jR, i) - FI(1, q, k, cf.jL, i)) * (1. / cf.den_j);
+                d4 += (K2(cf.kR) - K2(cf.kL)) * (1. / cf.den_k);
+                r = 9. / 8. * dv + -1. / 8. * d4;
+            }
+            r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) + v1[q] -
+                  K5(km)) / 1.0;
+            f[q] = r;
+        }
+        const long long t = IX(d, k, j, i);
+        fp[3 * t] = f[0]; fp[3 * t + 1] = f[1]; fp[3 * t + 2] = f[2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            c0[q] = c1[q]; c1[q] = c2[q]; c2[q] = c3[q]; c3[q] = nc[q];
+            v0[q] = v1[q]; v1[q] = nv[q];
+        }
+    }
+#undef FI
 }
 
 // Cell divergence (momentum.c:1506-1636) and projection onto the faces
@@ -335,7 +424,17 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     hipLaunchKernelGGL(k_ff2_faces_bnd, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont,
                        ucat, nu_t, c->face);
     VFS_CHECK_LAUNCH();
-    if (!VFS_CELLRHS) return vfs_ff2_from_faces(c, rhs, scale);
+    if (!VFS_CELLRHS) {
+#if VFS_CELLS_KMARCH
+        const int kc3 = VFS_CELLS_KCHUNK;
+        const dim3 g3((d.mx - 2 + DI - 1) / DI, (d.my - 2 + DJ - 1) / DJ, (d.mz - 2 + kc3 - 1) / kc3);
+        hipLaunchKernelGGL(k_ff2_cells_km, g3, dim3(DT), 0, c->stream, d, c->nvert, c->face, c->fp, kc3);
+        VFS_CHECK_LAUNCH();
+        return vfs_ff2_rhs_only(c, rhs, scale);
+#else
+        return vfs_ff2_from_faces(c, rhs, scale);
+#endif
+    }
     const int kc2 = VFS_CELLRHS_KCHUNK;
     const dim3 g2((d.mx - 2 + CI - 1) / CI, (d.my - 2 + CJ - 1) / CJ, (d.mz - 2 + kc2 - 1) / kc2);
     hipLaunchKernelGGL(k_ff2_cellrhs, g2, dim3(CT), 0, c->stream, d, c->g, c->nvert, c->face, rhs, scale, kc2);

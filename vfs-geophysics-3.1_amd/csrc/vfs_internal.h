@@ -138,6 +138,24 @@ void vfs_set_error(const std::string &msg);
 
 static inline unsigned grid_for(long long n, int block) { return (unsigned)((n + block - 1) / block); }
 
+// Workgroups are dispatched to the 8 XCDs of the MI355X round-robin by
+// linear workgroup id, and each XCD has its own L2.  xcd_block maps the
+// hardware id to a logical id so that XCD x runs one contiguous range of
+// logical blocks (a bijection for any nb): neighbouring rows and planes of a
+// stencil then meet in the same L2.
+#define VFS_NUM_XCD 8
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+#ifdef VFS_NO_XCD_REMAP
+    (void)nb;
+    return b;
+#else
+    const unsigned q = nb / VFS_NUM_XCD, rem = nb % VFS_NUM_XCD;
+    const unsigned x = b % VFS_NUM_XCD, r = b / VFS_NUM_XCD;
+    return x * q + (x < rem ? x : rem) + r;
+#endif
+}
+
 // shared launch helpers (vfs_core.hip)
 int vfs_periodic_vec(vfs_ctx *c, double *v);      // periodic ghost copy, vector field
 int vfs_periodic_sca(vfs_ctx *c, double *v);      // periodic ghost copy, scalar field

@@ -338,7 +338,7 @@ __global__ void k_ff2_faces(Dims d, Geom g, const double *__restrict__ nvert, co
 // cell divergence with the 4th-order deferred correction (momentum.c:1506-1636)
 __global__ void k_ff2_cells(Dims d, const double *__restrict__ nvert, const double *__restrict__ face, double *__restrict__ fp)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
     if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
@@ -351,7 +351,7 @@ __global__ void k_ff2_cells(Dims d, const double *__restrict__ nvert, const doub
 __global__ void k_ff2_rhs(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ fp,
                           double *__restrict__ rhs, double scale)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
     if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) {
@@ -384,6 +384,15 @@ int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale)
     hipLaunchKernelGGL(k_ff2_cells, dim3(gb), dim3(256), 0, c->stream, d, c->nvert, c->face, c->fp);
     VFS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_ff2_rhs, dim3(gb), dim3(256), 0, c->stream, d, c->g, c->nvert, c->fp, rhs, scale);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+// projection of fp already in c->fp
+int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale)
+{
+    const Dims d = c->d;
+    hipLaunchKernelGGL(k_ff2_rhs, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, c->fp, rhs, scale);
     VFS_CHECK_LAUNCH();
     return 0;
 }

@@ -42,4 +42,5 @@ struct Poisson {
     double *hdots;                      // pinned host mirror
     double omega, alpha;
     int wdepth;                         // levels l < wdepth visit level l+1 twice (W-cycle)
+    int lsmall;                         // first level cycled by the one-workgroup kernel
 };

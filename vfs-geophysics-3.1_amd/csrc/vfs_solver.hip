@@ -201,10 +201,14 @@ __device__ __forceinline__ T apply_row_any(const LV &L, const T *__restrict__ A,
 
 // Row kernels launch one thread per i and one block row per (j, k)
 // (blockIdx.y = j, blockIdx.z = k), so all index arithmetic but i is scalar.
+// (i, j, k) of this thread on a (ceil(nx/b), ny, nz) grid, with the
+// workgroups of each XCD on a contiguous range of planes
 #define ROW_IDX(L)                                                             \
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;                       \
+    const unsigned _lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), \
+                                   gridDim.x * gridDim.y * gridDim.z);         \
+    const int i = (int)(_lb % gridDim.x) * blockDim.x + threadIdx.x;           \
     if (i >= (L).nx) return;                                                   \
-    const int j = blockIdx.y, k = blockIdx.z;                                  \
+    const int j = (int)((_lb / gridDim.x) % gridDim.y), k = (int)(_lb / (gridDim.x * gridDim.y)); \
     const long long t = ((long long)k * (L).ny + j) * (L).nx + i;
 
 // y = A x
@@ -306,16 +310,150 @@ __global__ void __launch_bounds__(64) k_thomas(LV L, const T *__restrict__ r, co
 }
 
 // dense coarsest solve x = Cinv b over the listed unknowns; other cells 0
+// (cinv column-major; nc <= VFS_MAX_DENSE; one workgroup)
+#define VFS_MAX_DENSE 1024
 template <typename T>
-__global__ void k_coarse_solve(long long n, int nc, const long long *__restrict__ cidx, const T *__restrict__ cinv,
-                               const T *__restrict__ b, T *__restrict__ x)
+__device__ __forceinline__ void dense_solve(long long n, int nc, const long long *__restrict__ cidx,
+                                            const T *__restrict__ cinv, const T *__restrict__ b, T *__restrict__ x,
+                                            T *sb)
 {
+    for (int q = threadIdx.x; q < nc; q += blockDim.x) sb[q] = b[cidx[q]];
+    __syncthreads();
     for (long long q = threadIdx.x; q < n; q += blockDim.x) x[q] = 0;
     __syncthreads();
     for (int r = threadIdx.x; r < nc; r += blockDim.x) {
         T s = 0;
-        for (int c = 0; c < nc; c++) s += cinv[(long long)r * nc + c] * b[cidx[c]];
+        for (int c = 0; c < nc; c++) s += cinv[(long long)c * nc + r] * sb[c];
         x[cidx[r]] = s;
+    }
+}
+
+template <typename T>
+__global__ void k_coarse_solve(long long n, int nc, const long long *__restrict__ cidx, const T *__restrict__ cinv,
+                               const T *__restrict__ b, T *__restrict__ x)
+{
+    __shared__ T sb[VFS_MAX_DENSE];
+    dense_solve<T>(n, nc, cidx, cinv, b, x, sb);
+}
+
+// ---------------------------------------------------------------------------
+// The small levels in one workgroup.  Below ~16K unknowns every multigrid
+// kernel is a few microseconds of launch latency around nanoseconds of work,
+// so the whole V-cycle of levels first..last (the last one dense) runs as a
+// single 1024-thread workgroup with barriers between its phases.  The phases
+// are the ones of vcycle(): zero-guess line smoothing, residual, restriction,
+// dense solve, prolongation, residual, line smoothing.
+// ---------------------------------------------------------------------------
+#define MG_MAXL 16
+template <typename T>
+struct SmallLevels {
+    int first, last;
+    LV lv[MG_MAXL];
+    const T *A[MG_MAXL], *tl[MG_MAXL], *tu[MG_MAXL], *ti[MG_MAXL];
+    T *b[MG_MAXL], *x[MG_MAXL], *y[MG_MAXL], *d[MG_MAXL];
+    int nc;
+    const long long *cidx;
+    const T *cinv;
+    T omega, alpha;
+};
+
+// xn = xo + omega T^{-1} r along every j-line (xo may be null: zero guess)
+template <typename T>
+__device__ __forceinline__ void wg_thomas(const LV &L, const T *r, const T *ti, const T *tm, const T *tu, const T *xo,
+                                          T omega, T *dscr, T *xn)
+{
+    constexpr int MJ = 8;    // lines up to this length are held in registers
+    const long long nl = (long long)L.nx * L.nz, sy = L.nx;
+    for (long long ln = threadIdx.x; ln < nl; ln += blockDim.x) {
+        const int i = (int)(ln % L.nx), k = (int)(ln / L.nx);
+        const long long base = (long long)k * L.ny * L.nx + i;
+        if (L.ny <= MJ) {
+            // all loads of the line first, then the two recurrences
+            T rr[MJ], ii[MJ], mm[MJ], uu[MJ], oo[MJ];
+#pragma unroll
+            for (int j = 0; j < MJ; j++)
+                if (j < L.ny) {
+                    const long long id = base + j * sy;
+                    rr[j] = r[id]; ii[j] = ti[id]; mm[j] = tm[id]; uu[j] = tu[id]; oo[j] = xo ? xo[id] : T(0);
+                }
+            T dp = 0;
+#pragma unroll
+            for (int j = 0; j < MJ; j++)
+                if (j < L.ny) { dp = rr[j] * ii[j] - mm[j] * dp; rr[j] = dp; }
+            T xj = 0;
+#pragma unroll
+            for (int j = MJ - 1; j >= 0; j--)
+                if (j < L.ny) { xj = rr[j] - uu[j] * xj; xn[base + j * sy] = oo[j] + omega * xj; }
+            continue;
+        }
+        T dp = 0;
+        for (int j = 0; j < L.ny; j++) {
+            const long long id = base + j * sy;
+            dp = r[id] * ti[id] - tm[id] * dp;
+            dscr[id] = dp;
+        }
+        T xj = 0;
+        for (int j = L.ny - 1; j >= 0; j--) {
+            const long long id = base + j * sy;
+            xj = dscr[id] - tu[id] * xj;
+            xn[id] = (xo ? xo[id] : T(0)) + omega * xj;
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void wg_resid(const LV &L, const T *A, const T *x, const T *b, T *r)
+{
+    for (long long t = threadIdx.x; t < L.n; t += blockDim.x) {
+        const int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+        r[t] = b[t] - apply_row_any<T, 27>(L, A, x, t, i, j, k);
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(1024) k_small_cycle(SmallLevels<T> S)
+{
+    for (int l = S.first; l < S.last; l++) {
+        const LV L = S.lv[l], C = S.lv[l + 1];
+        wg_thomas<T>(L, S.b[l], S.ti[l], S.tl[l], S.tu[l], nullptr, S.omega, S.d[l], S.x[l]);
+        __syncthreads();
+        wg_resid<T>(L, S.A[l], S.x[l], S.b[l], S.y[l]);
+        __syncthreads();
+        for (long long t = threadIdx.x; t < C.n; t += blockDim.x) {
+            const int I = (int)(t % C.nx), J = (int)((t / C.nx) % C.ny), K = (int)(t / ((long long)C.nx * C.ny));
+            T s = 0;
+            for (int ck = 0; ck < 2; ck++)
+                for (int cj = 0; cj < 2; cj++)
+                    for (int ci = 0; ci < 2; ci++) {
+                        if ((L.nz == 1 && ck) || (L.ny == 1 && cj) || (L.nx == 1 && ci)) continue;
+                        const int fk = L.nz == 1 ? K : 2 * K + ck, fj = L.ny == 1 ? J : 2 * J + cj,
+                                  fi = L.nx == 1 ? I : 2 * I + ci;
+                        if (fk >= L.nz || fj >= L.ny || fi >= L.nx) continue;
+                        s += S.y[l][cix(L, fk, fj, fi)];
+                    }
+            S.b[l + 1][t] = s;
+        }
+        __syncthreads();
+    }
+    {   // dense solve on the last level
+        __shared__ T sb[VFS_MAX_DENSE];
+        dense_solve<T>(S.lv[S.last].n, S.nc, S.cidx, S.cinv, S.b[S.last], S.x[S.last], sb);
+        __syncthreads();
+    }
+    for (int l = S.last - 1; l >= S.first; l--) {
+        const LV L = S.lv[l], C = S.lv[l + 1];
+        T *x = S.x[l];
+        const T *e = S.x[l + 1];
+        for (long long t = threadIdx.x; t < L.n; t += blockDim.x) {
+            const int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
+            const int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
+            x[t] += S.alpha * e[cix(C, ck, cj, ci)];
+        }
+        __syncthreads();
+        wg_resid<T>(L, S.A[l], x, S.b[l], S.y[l]);
+        __syncthreads();
+        wg_thomas<T>(L, S.y[l], S.ti[l], S.tl[l], S.tu[l], x, S.omega, S.d[l], x);
+        __syncthreads();
     }
 }
 
@@ -668,9 +806,21 @@ int vfs_mg_setup(vfs_ctx *c)
     }
     if (invert_dense(M, nc)) { vfs_set_error("singular coarsest Poisson operator"); return -1; }
     ps->nc = nc;
+    // the first level (below the finest) small enough for the one-workgroup cycle
+    ps->lsmall = (int)ps->lv.size();
+    for (int q = 1; q < (int)ps->lv.size(); q++)
+        if (ps->lv[q].n <= 2048 && (int)ps->lv.size() - q <= MG_MAXL) { ps->lsmall = q; break; }
     DMALLOC(ps->cinv64, (size_t)nc * nc * sizeof(double));
     DMALLOC(ps->cinv32, (size_t)nc * nc * sizeof(float));
     DMALLOC(ps->cidx, (size_t)nc * sizeof(long long));
+    // column-major (cinv[c * nc + r]) so that the rows of a solve, one per
+    // thread, read consecutive addresses for each column
+    {
+        std::vector<double> T((size_t)nc * nc);
+        for (int r = 0; r < nc; r++)
+            for (int q = 0; q < nc; q++) T[(size_t)q * nc + r] = M[(size_t)r * nc + q];
+        M.swap(T);
+    }
     std::vector<float> M32(M.begin(), M.end());
     VFS_CHECK_HIP(hipMemcpy(ps->cinv64, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
     VFS_CHECK_HIP(hipMemcpy(ps->cinv32, M32.data(), M32.size() * sizeof(float), hipMemcpyHostToDevice));
@@ -729,6 +879,26 @@ static int vcycle(vfs_ctx *c, int l)
     if (l == (int)ps->lv.size() - 1) {
         const T *cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
         hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), 0, c->stream, L.n, ps->nc, ps->cidx, cinv, b, x);
+        VFS_CHECK_LAUNCH();
+        return 0;
+    }
+    if (l == ps->lsmall) {
+        SmallLevels<T> S;
+        S.first = l;
+        S.last = (int)ps->lv.size() - 1;
+        for (int q = l; q <= S.last; q++) {
+            MGLevel &Q = ps->lv[q];
+            S.lv[q] = lv_of(Q);
+            S.A[q] = A_of<T>(Q);
+            fac_of<T>(Q, S.tl[q], S.tu[q], S.ti[q]);
+            S.b[q] = vec_of<T>(Q, 0); S.x[q] = vec_of<T>(Q, 1); S.y[q] = vec_of<T>(Q, 2); S.d[q] = vec_of<T>(Q, 3);
+        }
+        S.nc = ps->nc;
+        S.cidx = ps->cidx;
+        S.cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
+        S.omega = omega;
+        S.alpha = alpha;
+        hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), 0, c->stream, S);
         VFS_CHECK_LAUNCH();
         return 0;
     }
