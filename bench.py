@@ -234,6 +234,17 @@ def main():
                          maxdiv=max(x["maxdiv"] for x in timers))
         log(f"[bench] {mode}: {res[mode]}")
 
+    # mixed vs FP64 solution of one right-hand side (both to 1e-8)
+    ctx.volume_flux(s.uc)
+    ctx.poisson_rhs(s.uc, s.b)
+    phis = []
+    for prec in (0, 1):
+        phi = ctx.sca()
+        ctx.poisson_solve(s.b, phi, rtol=1e-8, precision=prec, maxit=1000)
+        phis.append(phi.host())
+    phi_diff = float(np.linalg.norm(phis[1] - phis[0]) / np.linalg.norm(phis[0]))
+    log(f"[bench] |phi_mixed - phi_fp64| / |phi_fp64| = {phi_diff:.2e}")
+
     ctx.contra2cart(s.uc, s.u)
     if cfg["les"]:
         from vfsgpu import workloads as W
@@ -272,7 +283,8 @@ def main():
                     "mixed_s_to_1e-8": round(res["mixed"]["poisson_s"], 5),
                     "fp64_iters": res["fp64"]["iters"], "mixed_iters": res["mixed"]["iters"],
                     "max_relres": max(res["fp64"]["relres"], res["mixed"]["relres"]),
-                    "max_div_after_projection": max(res["fp64"]["maxdiv"], res["mixed"]["maxdiv"])},
+                    "max_div_after_projection": max(res["fp64"]["maxdiv"], res["mixed"]["maxdiv"]),
+                    "phi_rel_diff_mixed_vs_fp64": phi_diff},
         "step_ms": {"fp64": round(res["fp64"]["step_ms"], 3), "mixed": round(res["mixed"]["step_ms"], 3),
                     "speedup_mixed_vs_fp64": round(res["fp64"]["step_ms"] / res["mixed"]["step_ms"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

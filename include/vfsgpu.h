@@ -40,6 +40,8 @@ typedef struct {
     int les;             /* -les 1: constant-coefficient Smagorinsky    */
     int second_order;    /* -second_order                              */
     int dpdz_set;        /* k-periodic forcing from mean_pressure_gradient */
+    int weno;            /* -weno 1: WENO3 convective face values (level.c:1734) */
+    int reserved;        /* keeps the doubles 8-byte aligned; set to 0 */
     double ren, dt, st;  /* Reynolds number, time step, Strouhal factor */
     double mean_pressure_gradient;
 } vfs_config;

@@ -76,7 +76,35 @@ static int check_cfg(const vo_cfg *c)
         else if (bt != 1) return -3;
     }
     if (c->immersed != 0 && c->immersed != 1) return -4;
+    if (c->weno != 0 && c->weno != 1) return -4;
     return 0;
+}
+
+/* level.c:1734-1771 weno3.  pow(x, 2.) of the reference is written x * x:
+ * the reference is built with -O3 (makefile:19), where GCC folds pow(x, 2.)
+ * into that multiplication. */
+static double sq(double x) { return x * x; }
+
+double vo_weno3(double f0, double f1, double f2, double f3, double wavespeed)
+{
+    double fL, fC, fR;
+    if (wavespeed > 0) {
+        fL = f0; fC = f1; fR = f2;
+    } else {
+        fL = f3; fC = f2; fR = f1;   /* mirror */
+    }
+    const double d0 = 2. / 3., d1 = 1. / 3.;
+    const double eps = 1.e-6;
+    double beta0 = sq(fC - fR);
+    double beta1 = sq(fL - fC);
+    double alpha0 = d0 / sq(eps + beta0);
+    double alpha1 = d1 / sq(eps + beta1);
+    double sumalpha = alpha0 + alpha1;
+    double w0 = alpha0 / sumalpha;
+    double w1 = alpha1 / sumalpha;
+    double u0 = (fC * 0.5 + fR * 0.5);
+    double u1 = (-fL * 0.5 + fC * 1.5);
+    return w0 * u0 + w1 * u1;
 }
 
 static void copy3(double *d, size_t di, const double *s, size_t si)
@@ -694,6 +722,9 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                                      up * (0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
                     } else if (c->second_order) {
                         for (int q = 0; q < 3; q++) d3v[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
+                    } else if (c->weno) {   /* momentum.c:782-786, 1044-1048, 1283-1287 */
+                        for (int q = 0; q < 3; q++)
+                            d3v[q] = -ucon_f * vo_weno3(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f);
                     } else {
                         for (int q = 0; q < 3; q++)
                             d3v[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));

@@ -29,6 +29,8 @@ typedef struct {
     int les;               /* 0/1: constant-Cs Smagorinsky                          */
     int second_order;      /* momentum.c -second_order                              */
     int dpdz_set;          /* 1: k-periodic forcing uses mean_pressure_gradient     */
+    int weno;              /* 1: WENO3 convective face values (momentum.c:782-786)  */
+    int reserved;
     double ren, dt, st;
     double mean_pressure_gradient;
 } vo_cfg;
@@ -68,6 +70,9 @@ int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert,
 
 /* momentum.c:456-1979 Formfunction_2: rhs += scale * F(ucont, ucat).
  * ucont: in/out (periodic ghosts are rewritten, momentum.c:642-669). */
+/* level.c:1734-1771 weno3 (exposed for its known-answer tests) */
+double vo_weno3(double f0, double f1, double f2, double f3, double wavespeed);
+
 int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                      double *ucont, const double *ucat, const double *nu_t,
                      double *rhs, double scale);

@@ -33,4 +33,4 @@ def test_workload_rng_is_deterministic_and_uniform():
 
 def test_config_layout_matches_header():
     # vfs_config: 3 ints + 3 ints + 6 ints + 4 ints, then 4 doubles
-    assert C.sizeof(V.Config) == 16 * 4 + 4 * 8
+    assert C.sizeof(V.Config) == 18 * 4 + 4 * 8

@@ -98,6 +98,29 @@ def test_formfunction2_reference_passes(case, monkeypatch):
     test_formfunction2(case)
 
 
+def test_formfunction2_weno(case):
+    """-weno 1: WENO3 convective face values (momentum.c:782-786 and the j/k
+    counterparts) on the same fields."""
+    cfg = dict(case["cfg"], weno=1)
+    _, spec, coor, o, ucat, ucont = oracle_case(case["name"])
+    o = __import__("oracle_py").Oracle(cfg, coor)
+    if spec["immersed"]:
+        o.nvert_sphere(*W.SPHERE)
+    uc, u = ucont.copy(), ucat.copy()
+    o.contra2cart(uc, u)
+    nut = o.sca()
+    if cfg["les"]:
+        o.eddy_viscosity(u, W.CS_LES, nut)
+    ctx = gpu_case(cfg, spec, coor)
+    duc, du, dn = (V.DeviceArray.from_host(x) for x in (uc, u, nut))
+    rhs = o.vec()
+    o.formfunction2(uc, u, nut, rhs, 1.0)
+    drhs = ctx.vec()
+    ctx.formfunction2(duc, du, dn, drhs, 1.0)
+    assert rel_err(drhs.host(), rhs) < TOL
+    ctx.close()
+
+
 def test_pressure_gradient(case):
     o, ctx = case["o"], case["ctx"]
     p = W.random_scalar(case["cfg"], 1.0, 555)

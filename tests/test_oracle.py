@@ -7,6 +7,7 @@ plausible restatement mistake (a dropped term, a wrong sign or index, a
 transposed operand) fails at least one of them.
 """
 import math
+import os
 
 import numpy as np
 import pytest
@@ -215,6 +216,87 @@ def test_ff2_accumulates_and_scales():
     m = R1 != 0
     # rhs += scale*F on unmasked interior faces
     assert rel_err(R2[m], 1.0 + 0.5 * R1[m]) < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# WENO3 (level.c:1734-1771, -weno 1)
+# ---------------------------------------------------------------------------
+def weno3(f0, f1, f2, f3, ws):
+    from oracle_py import lib as olib
+    import ctypes as C
+    fn = olib().vo_weno3
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_double] * 5
+    return fn(f0, f1, f2, f3, ws)
+
+
+def test_weno3_known_answers():
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        a, b = rng.uniform(-2, 2, 2)
+        f = [a + b * t for t in (-1.0, 0.0, 1.0, 2.0)]
+        for ws in (1.0, -1.0):
+            # linear data: both 2-point candidates equal the face value a + b/2
+            assert abs(weno3(*f, ws) - (a + 0.5 * b)) <= 1e-14 * (abs(a) + abs(b) + 1)
+        c = rng.uniform(-1, 1)
+        assert abs(weno3(c, c, c, c, 1.0) - c) <= 1e-15 * (1 + abs(c))
+    # a step: the stencil that does not cross it wins (upwind side by wave speed)
+    assert abs(weno3(0.0, 0.0, 1.0, 1.0, 1.0) - 0.0) < 1e-10
+    assert abs(weno3(0.0, 0.0, 1.0, 1.0, -1.0) - 1.0) < 1e-10
+    # wave speed > 0 uses (f0, f1, f2); < 0 mirrors to (f3, f2, f1)
+    assert weno3(0.3, -0.2, 0.9, 5.0, -1.0) == weno3(123.0, -0.2, 0.9, 5.0, -1.0)
+    assert weno3(0.3, -0.2, 0.9, 5.0, 1.0) == weno3(0.3, -0.2, 0.9, -42.0, 1.0)
+
+
+def test_weno3_square_is_pow():
+    """The reference squares with pow(x, 2.) and is built with -O3
+    (makefile:19), where GCC folds pow(x, 2.) into x * x.  Compile that
+    expression the same way and check the restatement's x * x against it."""
+    import ctypes as C
+    import subprocess
+    import tempfile
+    d = tempfile.mkdtemp()
+    with open(os.path.join(d, "p.c"), "w") as f:
+        f.write("#include <math.h>\ndouble sqpow(double x) { return pow(x, 2.); }\n")
+    subprocess.run(["gcc", "-O3", "-shared", "-fPIC", "-o", os.path.join(d, "p.so"), os.path.join(d, "p.c"), "-lm"],
+                   check=True)
+    fn = C.CDLL(os.path.join(d, "p.so")).sqpow
+    fn.restype = C.c_double
+    fn.argtypes = [C.c_double]
+    rng = np.random.default_rng(6)
+    for x in np.concatenate([rng.uniform(-3, 3, 2000), rng.uniform(-1e-3, 1e-3, 2000) + 1e-6]):
+        assert fn(float(x)) == float(x) * float(x)
+
+
+def test_ff2_weno_equals_central_on_linear_fields():
+    """On a uniform grid a field linear in x, y, z is reproduced exactly by
+    both 2-point WENO candidates and by the 4th-order central formula, so the
+    two schemes agree away from the boundary faces (where they differ)."""
+    N = (14, 12, 12)
+    Ni, Nj, Nk = N
+    base = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=0, j_periodic=0, k_periodic=0,
+                bctype=[1, 1, 1, 1, 1, 1], immersed=0, les=0, second_order=0, dpdz_set=1, ren=100.0,
+                dt=1e-3, st=1.0, mean_pressure_gradient=0.0)
+    coor = V.channel_grid(Ni + 1, Nj + 1, Nk + 1, 1.4, 1.2, 1.2, 0.0, 0.0)
+    out = {}
+    for weno in (0, 1):
+        cfg = dict(base, weno=weno)
+        o = Oracle(cfg, coor)
+        # linear in the cell index = linear in space on a uniform grid, ghosts included
+        kk, jj, ii = np.meshgrid(np.arange(o.mz), np.arange(o.my), np.arange(o.mx), indexing="ij")
+        U = np.empty((o.mz, o.my, o.mx, 3))
+        A = np.array([[0.3, 0.05, -0.02, 0.01], [-0.4, 0.02, 0.03, -0.05], [0.2, -0.01, 0.04, 0.06]])
+        for q in range(3):
+            U[..., q] = A[q, 0] + A[q, 1] * ii + A[q, 2] * jj + A[q, 3] * kk
+        ucont = W.random_vector(cfg, 1.0, 77)
+        rhs = o.vec()
+        o.formfunction2(ucont.copy(), U.reshape(-1).copy(), o.sca(), rhs, 1.0)
+        out[weno] = arr(o, rhs)
+    inner = (slice(4, -4), slice(4, -4), slice(4, -4))
+    a, b = out[1][inner], out[0][inner]
+    assert rel_err(a, b) < 1e-12
+    # and the WENO branch is really taken: near the walls the schemes differ
+    assert np.abs(out[1] - out[0]).max() > 1e-8 * np.abs(out[0]).max()
 
 
 # ---------------------------------------------------------------------------

@@ -94,6 +94,33 @@ __device__ __forceinline__ void du_face_adj(int dir, const Acc &A, const double 
 #undef NV
 }
 
+// weno3 (level.c:1734-1771) and its gradient with respect to f0..f3 (the
+// wave speed only selects the upwind side).  With fL, fC, fR the upwind
+// triple, w0 = alpha0 / (alpha0 + alpha1), W = w0 u0 + (1 - w0) u1:
+//   dW = (u0 - u1) dw0 + w0 du0 + w1 du1,
+//   dw0 = (alpha1 dalpha0 - alpha0 dalpha1) / (alpha0 + alpha1)^2,
+//   dalpha_m = -2 d_m / (eps + beta_m)^3 dbeta_m.
+__device__ __forceinline__ double weno3_grad(double f0, double f1, double f2, double f3, double ws, double g[4])
+{
+    const bool mir = !(ws > 0);
+    const double fL = mir ? f3 : f0, fC = mir ? f2 : f1, fR = mir ? f1 : f2;
+    const double d0 = 2. / 3., d1 = 1. / 3., eps = 1.e-6;
+    const double b0 = fC - fR, b1 = fL - fC;
+    const double e0 = eps + b0 * b0, e1 = eps + b1 * b1;
+    const double a0 = d0 / (e0 * e0), a1 = d1 / (e1 * e1);
+    const double sa = a0 + a1, w0 = a0 / sa, w1 = a1 / sa;
+    const double u0 = fC * 0.5 + fR * 0.5, u1 = -fL * 0.5 + fC * 1.5;
+    const double da0 = -2. * d0 / (e0 * e0 * e0), da1 = -2. * d1 / (e1 * e1 * e1);   // d alpha / d beta
+    const double A = (u0 - u1) / (sa * sa);
+    // d beta0 = 2 b0 (dfC - dfR), d beta1 = 2 b1 (dfL - dfC)
+    const double gL = A * (-a0 * da1 * 2. * b1) - 0.5 * w1;
+    const double gC = A * (a1 * da0 * 2. * b0 + a0 * da1 * 2. * b1) + 0.5 * w0 + 1.5 * w1;
+    const double gR = A * (-a1 * da0 * 2. * b0) + 0.5 * w0;
+    if (!mir) { g[0] = gL; g[1] = gC; g[2] = gR; g[3] = 0; }
+    else { g[3] = gL; g[2] = gC; g[1] = gR; g[0] = 0; }
+    return w0 * u0 + w1 * u1;
+}
+
 // Transpose of face_flux_t<DIR, double> with respect to ucont (component DIR)
 // and ucat, at the linearisation point (ucont, ucat); nu_t is frozen.
 // Db/Vb are the adjoints of the convective and viscous face fluxes.
@@ -172,6 +199,17 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
                 ucon_fb += Db[q] * (-0.5 * (UB(n, q) + UB(n + 1, q)));
                 AU(n, q, Db[q] * -ucon_f * 0.5);
                 AU(n + 1, q, Db[q] * -ucon_f * 0.5);
+            }
+        } else if (d.weno) {
+            for (int q = 0; q < 3; q++) {
+                double gw[4];
+                const double W = weno3_grad(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f, gw);
+                ucon_fb += Db[q] * -W;
+                const double c = Db[q] * -ucon_f;
+                AU(L, q, c * gw[0]);
+                AU(n, q, c * gw[1]);
+                AU(n + 1, q, c * gw[2]);
+                AU(R, q, c * gw[3]);
             }
         } else {
             const double c = -ucon_f * 0.0625;

@@ -137,7 +137,7 @@ __device__ __forceinline__ void store_plane(int k, const double (&ru)[NU], const
 // the face array.  PARTS: 1 convective only, 2 viscous only, 3 both (the
 // unused part is removed at compile time).  The periodic wrap-around faces
 // are left to k_ff2_faces_bnd.
-template <int PARTS>
+template <int PARTS, bool WENO>
 __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const double *__restrict__ ucont,
                                          const double *__restrict__ nu_t, double *__restrict__ face, int i0, int j0,
                                          int i, int j, int k)
@@ -148,7 +148,7 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
     const long long t = IX(d, k, j, i);
     double D[3], V[3];
     if (!(d.ip && i == d.mx - 2)) {
-        face_flux_t<0, double>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
+        face_flux_t<0, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             if (PARTS & 1) face[(0 * 3 + q) * P + t] = D[q];
@@ -158,7 +158,7 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
     // one face at a time: keeps the live registers of a single face
     __builtin_amdgcn_sched_barrier(0);
     if (!(d.jp && j == d.my - 2)) {
-        face_flux_t<1, double>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
+        face_flux_t<1, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             if (PARTS & 1) face[(1 * 3 + q) * P + t] = D[q];
@@ -167,7 +167,7 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
     }
     __builtin_amdgcn_sched_barrier(0);
     if (!(d.kp && k == d.mz - 2)) {
-        face_flux_t<2, double>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
+        face_flux_t<2, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             if (PARTS & 1) face[(2 * 3 + q) * P + t] = D[q];
@@ -178,6 +178,7 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
 
 // Face fluxes of all three directions at every interior point
 // (1..mx-2, 1..my-2, k in [kb, ke)).  grid: (ceil((mx-2)/64), ceil((my-2)/4), k chunks)
+template <bool WENO>
 __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FACES_WAVES, VFS_FACES_WAVES))) k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert,
                                                       const double *__restrict__ ucont,
                                                       const double *__restrict__ ucat,
@@ -207,9 +208,9 @@ __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FA
         // prefetch plane k+3 while plane k is computed
         fetch_plane(d, ucat, nvert, i0, j0, k + 3, ru, rn);
         if (active) {
-            if (!VFS_FACES_SPLIT) faces_at<3>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
-            else if (grp == 0) faces_at<1>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
-            else faces_at<2>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            if (!VFS_FACES_SPLIT) faces_at<3, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            else if (grp == 0) faces_at<1, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            else faces_at<2, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
         }
         __syncthreads();   // every read of plane k-1 is done
         store_plane(k + 3, ru, rn);
@@ -415,8 +416,12 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     const Dims d = c->d;
     const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);
-    hipLaunchKernelGGL(k_ff2_faces_lds, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t, c->face,
-                       kchunk);
+    if (d.weno)
+        hipLaunchKernelGGL(k_ff2_faces_lds<true>, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t,
+                           c->face, kchunk);
+    else
+        hipLaunchKernelGGL(k_ff2_faces_lds<false>, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t,
+                           c->face, kchunk);
     VFS_CHECK_LAUNCH();
     const long long n3[3] = {(long long)(d.my - 2) * (d.mz - 2), (long long)(d.mx - 2) * (d.mz - 2),
                              (long long)(d.mx - 2) * (d.my - 2)};

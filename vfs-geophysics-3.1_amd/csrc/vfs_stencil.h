@@ -94,6 +94,35 @@ __device__ __forceinline__ Dual operator*(double a, Dual b) { return Dual(a * b.
 __device__ __forceinline__ Dual operator*(Dual a, double b) { return Dual(a.v * b, a.d * b); }
 __device__ __forceinline__ Dual &operator+=(Dual &a, Dual b) { a = a + b; return a; }
 __device__ __forceinline__ Dual fabs(Dual a) { return Dual(fabs(a.v), a.v > 0 ? a.d : (a.v < 0 ? -a.d : 0.0)); }
+__device__ __forceinline__ Dual operator+(double a, Dual b) { return Dual(a + b.v, b.d); }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b) { return Dual(a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)); }
+__device__ __forceinline__ Dual operator/(double a, Dual b) { return Dual(a / b.v, -a * b.d / (b.v * b.v)); }
+__device__ __forceinline__ double value_of(double a) { return a; }
+__device__ __forceinline__ double value_of(Dual a) { return a.v; }
+
+// level.c:1734-1771 weno3; the reference's pow(x, 2.) is x * x (its -O3
+// build folds it so).  The upwind side follows the sign of the wave speed.
+template <typename S>
+__device__ __forceinline__ S weno3(S f0, S f1, S f2, S f3, S wavespeed)
+{
+    S fL, fC, fR;
+    if (value_of(wavespeed) > 0) {
+        fL = f0; fC = f1; fR = f2;
+    } else {
+        fL = f3; fC = f2; fR = f1;
+    }
+    const double d0 = 2. / 3., d1 = 1. / 3.;
+    const double eps = 1.e-6;
+    const S b0 = fC - fR, b1 = fL - fC;
+    const S beta0 = b0 * b0, beta1 = b1 * b1;
+    const S e0 = eps + beta0, e1 = eps + beta1;
+    const S alpha0 = d0 / (e0 * e0), alpha1 = d1 / (e1 * e1);
+    const S sumalpha = alpha0 + alpha1;
+    const S w0 = alpha0 / sumalpha, w1 = alpha1 / sumalpha;
+    const S u0 = (fC * 0.5 + fR * 0.5);
+    const S u1 = (-fL * 0.5 + fC * 1.5);
+    return w0 * u0 + w1 * u1;
+}
 
 struct GlobalSca {
     const double *__restrict__ p;
@@ -136,7 +165,9 @@ struct StoredFaceMetric {
     }
 };
 
-template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric>
+// WENO: false compiles the WENO3 branch out (for kernels launched only when
+// the weno flag is off).
+template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric, bool WENO = true>
 __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
                                             const FC &UCon, const FU &U, const double *__restrict__ nu_t,
                                             int i, int j, int k, S D[3], S V[3], const FM &fm = FM())
@@ -209,6 +240,9 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     } else if (d.second_order) {
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
+    } else if (WENO && d.weno) {   // momentum.c:782-786, 1044-1048, 1283-1287
+#pragma unroll
+        for (int q = 0; q < 3; q++) D[q] = -ucon_f * weno3<S>(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f);
     } else {
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));

@@ -40,7 +40,7 @@ def make_config(name, N=None, dt=1e-3):
     Ni, Nj, Nk = spec["N"]
     cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=1,
                bctype=[7, 7, 1, 1, 7, 7], immersed=spec["immersed"], les=spec["les"], second_order=0,
-               dpdz_set=1, ren=spec["ren"], dt=dt, st=1.0, mean_pressure_gradient=0.0)
+               dpdz_set=1, weno=0, ren=spec["ren"], dt=dt, st=1.0, mean_pressure_gradient=0.0)
     spec["name"] = name
     return cfg, spec
 
