@@ -1294,6 +1294,37 @@ int vo_volume_flux(const vo_cfg *c, const vo_geom *g, const double *nvert,
     const double eps = 1.e-8, ibmval = 1.1;
     double flux = 0, area = 0;
 #define AREA(v, id) sqrt(VX(v, id) * VX(v, id) + VY(v, id) * VY(v, id) + VZ(v, id) * VZ(v, id))
+    if (!c->immersed) {
+        /* poisson.c:3306-3500: every fluid / non-fluid face, then the
+         * correction and, for immersed == 0, zero flux on those faces */
+        for (int pass = 0; pass < 3; pass++) {
+            double corr = 0;
+            if (pass == 1) corr = area > 1.e-15 ? flux / area : 0;
+            if (pass != 1) flux = 0, area = 0;
+            for (int k = 1; k < mz - 1; k++)
+                for (int j = 1; j < my - 1; j++)
+                    for (int i = 1; i < mx - 1; i++) {
+                        size_t id = IX(k, j, i);
+                        const double *ar[3] = {g->csi, g->eta, g->zet};
+                        double nb[3] = {NV(k, j, i + 1), NV(k, j + 1, i), NV(k + 1, j, i)};
+                        for (int q = 0; q < 3; q++)
+                            if (NV(k, j, i) < 0.1 && nb[q] > 0.1) {
+                                double *u = &ucont[3 * id + q];
+                                if (pass == 1) { *u -= AREA(ar[q], id) * corr; *u = 0; }
+                                else { flux += *u; area += AREA(ar[q], id); }
+                            }
+                        for (int q = 0; q < 3; q++)
+                            if (NV(k, j, i) > 0.1 && nb[q] < 0.1) {
+                                double *u = &ucont[3 * id + q];
+                                if (pass == 1) { *u += AREA(ar[q], id) * corr; *u = 0; }
+                                else { flux -= *u; area += AREA(ar[q], id); }
+                            }
+                    }
+        }
+        if (flux_out) *flux_out = flux;
+        if (area_out) *area_out = area;
+        return 0;
+    }
     for (int pass = 0; pass < 3; pass++) {
         double corr = 0;
         if (pass == 1) corr = area > 1.e-15 ? flux / area : 0;

@@ -253,6 +253,26 @@ def test_volume_flux_and_rhs(case, poisson):
     assert rel_err(db.host(), b) < TOL
 
 
+def test_volume_flux_plain_variant(case):
+    """VolumeFlux with immersed == 0 and a marked region (poisson.c:3306-3500)."""
+    cfg = dict(case["cfg"], immersed=0)
+    _, spec, coor, _, _, _ = oracle_case(case["name"])
+    o = __import__("oracle_py").Oracle(cfg, coor)
+    nv = np.zeros((o.mz, o.my, o.mx))
+    nv[3:6, 2:5, 4:7] = 1.0
+    o.nvert[:] = nv.ravel()
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(nv.ravel()))
+    uc = case["ucont"].copy()
+    f_o, a_o = o.volume_flux(uc)
+    duc = V.DeviceArray.from_host(case["ucont"])
+    f_g, a_g = ctx.volume_flux(duc)
+    assert np.array_equal(duc.host(), uc)
+    assert f_g == 0.0 and f_o == 0.0 and rel_err(a_g, a_o) < TOL
+    ctx.close()
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_poisson_solve_converged_solution(case, poisson, precision):
     """Parity on the converged solution: both sides reach their residual and

@@ -447,6 +447,41 @@ def test_volume_flux_balances_ib_faces():
     assert abs(flux) < 1e-13 * area
 
 
+def test_volume_flux_without_immersed_zeroes_mask_faces():
+    """poisson.c:3306-3500 (immersed == 0): every face between a fluid cell
+    (nvert < 0.1) and a marked one (> 0.1) ends with zero flux; the other
+    faces are untouched; the returned flux is 0 and the area the sum of the
+    cell-metric norms on those faces (brute force)."""
+    cfg, coor, o = uniform_case((10, 8, 8))
+    nv = np.zeros((o.mz, o.my, o.mx))
+    nv[3:6, 2:5, 4:7] = 1.0
+    nv[4, 3, 5] = 3.0
+    o.nvert[:] = nv.ravel()
+    uc = W.random_vector(cfg, 1.0, 91)
+    u = uc.copy()
+    f, a = o.volume_flux(u)
+    U, U0 = arr(o, u), arr(o, uc)
+    met = [arr(o, o.g[n]) for n in ("csi", "eta", "zet")]
+    area = 0.0
+    changed = np.zeros(U.shape, bool)
+    for k in range(1, o.mz - 1):
+        for j in range(1, o.my - 1):
+            for i in range(1, o.mx - 1):
+                nbs = [nv[k, j, i + 1], nv[k, j + 1, i], nv[k + 1, j, i]]
+                for q in range(3):
+                    if (nv[k, j, i] < 0.1) != (nbs[q] < 0.1):
+                        changed[k, j, i, q] = True
+                        area += np.linalg.norm(met[q][k, j, i])
+    assert np.all(U[changed] == 0)
+    assert np.array_equal(U[~changed], U0[~changed])
+    assert abs(f) == 0.0 and rel_err(a, area) < 1e-13
+    # nvert == 0 everywhere: nothing to do
+    o.nvert[:] = 0
+    u = uc.copy()
+    f, a = o.volume_flux(u)
+    assert np.array_equal(u, uc) and f == 0 and a == 0
+
+
 def test_poisson_rhs_has_zero_sum():
     cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (12, 10, 8))
     b = o.sca()

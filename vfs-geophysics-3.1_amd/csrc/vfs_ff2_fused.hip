@@ -38,6 +38,9 @@ int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale);
 #ifndef VFS_CELLS_KMARCH
 #define VFS_CELLS_KMARCH 1       // k-marching cell divergence
 #endif
+#ifndef VFS_RHS_KMARCH
+#define VFS_RHS_KMARCH 0         // 1: k-marching projection (measured no faster)
+#endif
 #ifndef VFS_CELLS_KCHUNK
 #define VFS_CELLS_KCHUNK 16
 #endif
@@ -334,6 +337,57 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
 #undef FI
 }
 
+// Projection onto the faces with the masks (momentum.c:1673-1900),
+// k-marching: the fp, zeta metric and nvert of the k+1 neighbour read at
+// plane k are the cell's own values at plane k+1, so they stay in registers.
+// Same expressions and order as project_rhs; interior cells only (the ghost
+// shell is zeroed by k_zero_shell_vec).
+__global__ void __launch_bounds__(DT) k_ff2_rhs_km(Dims d, Geom g, const double *__restrict__ nvert,
+                                                  const double *__restrict__ fp, double *__restrict__ rhs,
+                                                  double scale, int kchunk)
+{
+    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
+    const int bx = lb % gridDim.x, by = (lb / gridDim.x) % gridDim.y, bz = lb / (gridDim.x * gridDim.y);
+    const int i = 1 + bx * DI + (int)(threadIdx.x % DI), j = 1 + by * DJ + (int)(threadIdx.x / DI);
+    if (i > d.mx - 2 || j > d.my - 2) return;
+    const int kb = 1 + bz * kchunk, ke = min(kb + kchunk, d.mz - 1);
+    const long long sk = (long long)d.mx * d.my;
+    const int ia = wrap_cell(i + 1, d.mx, d.ip), jb = wrap_cell(j + 1, d.my, d.jp);
+    const double *cs = g.csi, *et = g.eta, *ze = g.zet;
+    long long id = IX(d, kb, j, i);
+    double fc[3] = {fp[3 * id], fp[3 * id + 1], fp[3 * id + 2]};
+    double zc[3] = {ze[3 * id], ze[3 * id + 1], ze[3 * id + 2]};
+    double n0 = nvert[id];
+    for (int k = kb; k < ke; k++, id += sk) {
+        const int kc = wrap_cell(k + 1, d.mz, d.kp);
+        const long long a = IX(d, k, j, ia), b = IX(d, k, jb, i), c = IX(d, kc, j, i);
+        const long long ga = id + 1, gb = id + d.mx, gc = id + sk;   // metric ghosts are stored
+        // next plane's own values (the k+1 neighbour of this plane)
+        const double fn[3] = {fp[3 * c], fp[3 * c + 1], fp[3 * c + 2]};
+        const double zn[3] = {ze[3 * gc], ze[3 * gc + 1], ze[3 * gc + 2]};
+        const double nn = nvert[gc];
+        double r[3] = {rhs[3 * id], rhs[3 * id + 1], rhs[3 * id + 2]};
+        r[0] += scale * (0.5 * (cs[3 * id] * fc[0] + cs[3 * id + 1] * fc[1] + cs[3 * id + 2] * fc[2]) +
+                         0.5 * (cs[3 * ga] * fp[3 * a] + cs[3 * ga + 1] * fp[3 * a + 1] + cs[3 * ga + 2] * fp[3 * a + 2])) *
+                g.iaj[id];
+        r[1] += scale * (0.5 * (et[3 * id] * fc[0] + et[3 * id + 1] * fc[1] + et[3 * id + 2] * fc[2]) +
+                         0.5 * (et[3 * gb] * fp[3 * b] + et[3 * gb + 1] * fp[3 * b + 1] + et[3 * gb + 2] * fp[3 * b + 2])) *
+                g.jaj[id];
+        r[2] += scale * (0.5 * (zc[0] * fc[0] + zc[1] * fc[1] + zc[2] * fc[2]) +
+                         0.5 * (zn[0] * fn[0] + zn[1] * fn[1] + zn[2] * fn[2])) * g.kaj[id];
+        if (n0 + nvert[id + 1] > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
+        if (n0 + nvert[id + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
+        if (n0 + nn > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
+        rhs[3 * id] = r[0]; rhs[3 * id + 1] = r[1]; rhs[3 * id + 2] = r[2];
+        // the k+1 neighbour is the cell of the next step (a periodic wrap to
+        // plane 1 only happens at the last plane, where the loop ends)
+        fc[0] = fn[0]; fc[1] = fn[1]; fc[2] = fn[2];
+        zc[0] = zn[0]; zc[1] = zn[1]; zc[2] = zn[2];
+        n0 = nn;
+    }
+}
+
 // Cell divergence (momentum.c:1506-1636) and projection onto the faces
 // (momentum.c:1673-1900) in one k-marching pass over a 64 x 4 column of
 // cells.  Each thread keeps fp of its own cell at planes k and k+1; the fp of
@@ -435,7 +489,15 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
         const dim3 g3((d.mx - 2 + DI - 1) / DI, (d.my - 2 + DJ - 1) / DJ, (d.mz - 2 + kc3 - 1) / kc3);
         hipLaunchKernelGGL(k_ff2_cells_km, g3, dim3(DT), 0, c->stream, d, c->nvert, c->face, c->fp, kc3);
         VFS_CHECK_LAUNCH();
+#if VFS_RHS_KMARCH
+        hipLaunchKernelGGL(k_ff2_rhs_km, g3, dim3(DT), 0, c->stream, d, c->g, c->nvert, c->fp, rhs, scale, kc3);
+        VFS_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_zero_shell_vec, dim3(grid_for(shell_count(d), 256)), dim3(256), 0, c->stream, d, rhs);
+        VFS_CHECK_LAUNCH();
+        return 0;
+#else
         return vfs_ff2_rhs_only(c, rhs, scale);
+#endif
 #else
         return vfs_ff2_from_faces(c, rhs, scale);
 #endif

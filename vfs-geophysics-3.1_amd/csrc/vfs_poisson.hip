@@ -307,8 +307,12 @@ extern "C" int vfs_poisson_rhs(vfs_ctx *c, const double *ucont, double *b)
 // VolumeFlux (immersed==1, flg=0): pass 0 zero small IB fluxes + sum,
 // pass 1 correct, pass 2 sum again
 // ---------------------------------------------------------------------------
+// VolumeFlux, immersed == 1 (poisson.c:3570-3910) or, with plain != 0, the
+// variant for immersed == 0 (poisson.c:3306-3500): every fluid / non-fluid
+// face (no upper nvert bound, last face included, no small-flux zeroing),
+// whose fluxes the correction pass sets to zero.
 __global__ void k_vflux(Dims d, Geom g, const double *__restrict__ nvert, double *__restrict__ ucont, int pass,
-                        const double *__restrict__ sums, double *part)
+                        const double *__restrict__ sums, double *part, int plain)
 {
     __shared__ double sh[4];
     const double eps = 1.e-8, ibmval = 1.1;
@@ -324,13 +328,20 @@ __global__ void k_vflux(Dims d, Geom g, const double *__restrict__ nvert, double
         const bool inb[3] = {i < d.mx - 2, j < d.my - 2, k < d.mz - 2};
         for (int side = 0; side < 2; side++) {
             for (int q = 0; q < 3; q++) {
-                bool hit = side == 0 ? (nv < 0.1 && nb[q] > 0.1 && nb[q] < ibmval && inb[q])
+                bool hit;
+                if (plain) hit = side == 0 ? (nv < 0.1 && nb[q] > 0.1) : (nv > 0.1 && nb[q] < 0.1);
+                else hit = side == 0 ? (nv < 0.1 && nb[q] > 0.1 && nb[q] < ibmval && inb[q])
                                      : (nv > 0.1 && nv < ibmval && nb[q] < 0.1 && inb[q]);
                 if (!hit) continue;
                 const double *av = arr[q] + 3 * t;
                 double a = sqrt(av[0] * av[0] + av[1] * av[1] + av[2] * av[2]);
                 double u = ucont[3 * t + q];
                 double sgn = side == 0 ? 1.0 : -1.0;
+                if (plain) {
+                    if (pass == 1) ucont[3 * t + q] = 0.;   // corrected, then zeroed (poisson.c:3400-3430)
+                    else { fl = side == 0 ? fl + u : fl - u; ar += a; }
+                    continue;
+                }
                 if (pass == 0) {
                     if (fabs(u) > eps) { fl = side == 0 ? fl + u : fl - u; ar += a; }
                     else ucont[3 * t + q] = 0.;
@@ -355,21 +366,15 @@ extern "C" int vfs_volume_flux(vfs_ctx *c, double *ucont, double *flux, double *
 {
     const Dims d = c->d;
     double *res = c->red + VFS_RED_SLOTS * VFS_RED_BLOCKS;
-    if (!d.immersed) {
-        // with immersed == 0 only the wall-function nvert marking can create
-        // IB-adjacent faces (poisson.c:3306-3500); no supported bctype does
-        if (flux) *flux = 0;
-        if (area) *area = 0;
-        return 0;
-    }
+    const int plain = !d.immersed;
     int nb = (int)grid_for(d.P, 256);
     if (nb > VFS_RED_BLOCKS) nb = VFS_RED_BLOCKS;
-    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 0, res, c->red);
+    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 0, res, c->red, plain);
     VFS_CHECK_LAUNCH();
     if (vfs_reduce_finish(c, nb, 2, 0, res)) return -1;
-    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 1, res, c->red);
+    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 1, res, c->red, plain);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 2, res, c->red);
+    hipLaunchKernelGGL(k_vflux, dim3(nb), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont, 2, res, c->red, plain);
     VFS_CHECK_LAUNCH();
     if (vfs_reduce_finish(c, nb, 2, 0, res + 2)) return -1;
     if (flux || area) {
