@@ -84,6 +84,7 @@ class State:
         self.dp = ctx.vec()
         self.rhs = ctx.vec()
         self.phi = ctx.sca()
+        self.zero = ctx.sca()
         self.b = ctx.sca()
 
 
@@ -99,7 +100,7 @@ def fractional_step(ctx, cfg, s, precision, rtol=1e-8, timers=None):
     ctx.poisson_rhs(s.uc, s.b)
     ctx.sync()
     t0 = time.perf_counter()
-    s.phi.upload(np.zeros(s.phi.n))
+    s.phi.copy_from(s.zero)
     it, rel = ctx.poisson_solve(s.b, s.phi, rtol=rtol, precision=precision, maxit=1000)
     ctx.sync()
     t1 = time.perf_counter()

@@ -227,85 +227,131 @@ __global__ void k_resid(LV L, const T *__restrict__ A, const T *__restrict__ x, 
     r[t] = b[t] - apply_row_any<T, NS>(L, A, x, t, i, j, k);
 }
 
+// Restricted residual bc = P^T (b - A x) in one pass: a thread takes one fine
+// i and the (up to) 2 x 2 fine cells of its aggregate in j and k, and the
+// even lane adds its odd neighbour's sum (the aggregate's other i).  Grid
+// (ceil(F.nx / bs), C.ny, C.nz) with an even block size, and no lane leaves
+// before the exchange.
+template <typename T, int NS>
+__global__ void k_resid_restrict(LV F, LV C, const T *__restrict__ A, const T *__restrict__ x,
+                                 const T *__restrict__ b, T *__restrict__ bc)
+{
+    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z),
+                                  gridDim.x * gridDim.y * gridDim.z);
+    const int i = (int)(lb % gridDim.x) * blockDim.x + threadIdx.x;
+    const int J = (int)((lb / gridDim.x) % gridDim.y), K = (int)(lb / (gridDim.x * gridDim.y));
+    const int nj = F.ny > 1 ? 2 : 1, nk = F.nz > 1 ? 2 : 1;
+    T s = 0;
+    if (i < F.nx) {
+        for (int ck = 0; ck < nk; ck++) {
+            const int k = nk == 2 ? 2 * K + ck : K;
+            if (k >= F.nz) break;
+            for (int cj = 0; cj < nj; cj++) {
+                const int j = nj == 2 ? 2 * J + cj : J;
+                if (j >= F.ny) break;
+                const long long t = cix(F, k, j, i);
+                s += b[t] - apply_row_any<T, NS>(F, A, x, t, i, j, k);
+            }
+        }
+    }
+    if (F.nx > 1) {
+        s += __shfl_xor(s, 1, 64);
+        if (i < F.nx && !(i & 1)) bc[cix(C, K, J, i >> 1)] = s;
+    } else if (i == 0) {
+        bc[cix(C, K, J, 0)] = s;
+    }
+}
+
+// Row of A (x + alpha P e): x on level L, the coarse correction e on level C
+template <typename T, int NS>
+__device__ __forceinline__ T apply_row_pe(const LV &L, const LV &C, const T *__restrict__ A, const T *__restrict__ x,
+                                          const T *__restrict__ e, T alpha, long long id, int i, int j, int k)
+{
+    const int im = i > 0 ? i - 1 : (L.pi ? L.nx - 1 : 0), ip = i < L.nx - 1 ? i + 1 : (L.pi ? 0 : L.nx - 1);
+    const int jm = j > 0 ? j - 1 : (L.pj ? L.ny - 1 : 0), jp = j < L.ny - 1 ? j + 1 : (L.pj ? 0 : L.ny - 1);
+    const int km = k > 0 ? k - 1 : (L.pk ? L.nz - 1 : 0), kp = k < L.nz - 1 ? k + 1 : (L.pk ? 0 : L.nz - 1);
+    const long long sy = L.nx, sz = (long long)L.nx * L.ny, cy = C.nx, cz = (long long)C.nx * C.ny;
+    const long long I[3] = {im, i, ip}, J[3] = {jm * sy, j * sy, jp * sy}, K[3] = {km * sz, k * sz, kp * sz};
+    const long long CI[3] = {im >> 1, i >> 1, ip >> 1}, CJ[3] = {(jm >> 1) * cy, (j >> 1) * cy, (jp >> 1) * cy},
+                    CK[3] = {(km >> 1) * cz, (k >> 1) * cz, (kp >> 1) * cz};
+    T s = 0;
+#pragma unroll
+    for (int q = 0; q < NS; q++) {
+        int di, dj, dk;
+        slot_off(NS, q, di, dj, dk);
+        const T xv = x[K[dk + 1] + J[dj + 1] + I[di + 1]] + alpha * e[CK[dk + 1] + CJ[dj + 1] + CI[di + 1]];
+        s += A[(long long)q * L.n + id] * xv;
+    }
+    return s;
+}
+
+// xe = x + alpha P e and r = b - A xe in one pass (x itself is not written,
+// since the neighbours' old values are still being read)
+template <typename T, int NS>
+__global__ void k_resid_pe(LV L, LV C, const T *__restrict__ A, const T *__restrict__ x, const T *__restrict__ e,
+                           T alpha, const T *__restrict__ b, T *__restrict__ r, T *__restrict__ xe)
+{
+    ROW_IDX(L)
+    r[t] = b[t] - apply_row_pe<T, NS>(L, C, A, x, e, alpha, t, i, j, k);
+    xe[t] = x[t] + alpha * e[cix(C, k >> 1, j >> 1, i >> 1)];
+}
+
 // x += alpha * P e  (piecewise-constant prolongation)
 template <typename T>
 __global__ void k_prolong(LV L, LV C, const T *__restrict__ e, T alpha, T *__restrict__ x)
 {
     ROW_IDX(L)
-    int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
-    x[t] += alpha * e[cix(C, ck, cj, ci)];
+    x[t] += alpha * e[cix(C, k >> 1, j >> 1, i >> 1)];
 }
 
-// coarse right-hand side: sum of the fine residuals of each aggregate
-template <typename T>
-__global__ void k_restrict(LV F, LV C, const T *__restrict__ r, T *__restrict__ bc)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= C.n) return;
-    int I = (int)(t % C.nx), J = (int)((t / C.nx) % C.ny), K = (int)(t / ((long long)C.nx * C.ny));
-    T s = 0;
-    for (int ck = 0; ck < 2; ck++)
-        for (int cj = 0; cj < 2; cj++)
-            for (int ci = 0; ci < 2; ci++) {
-                if ((F.nz == 1 && ck) || (F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
-                int fk = F.nz == 1 ? K : 2 * K + ck, fj = F.ny == 1 ? J : 2 * J + cj, fi = F.nx == 1 ? I : 2 * I + ci;
-                if (fk >= F.nz || fj >= F.ny || fi >= F.nx) continue;
-                s += r[cix(F, fk, fj, fi)];
-            }
-    bc[t] = s;
-}
-
-// Damped j-line Jacobi update x = xo + omega * T^{-1} r along every (k,i)
-// grid line (Thomas sweeps with the factors of k_line_factors).  One lane per
-// line; the loads of a batch of B steps are issued before its dependent
-// recurrence so that each wavefront keeps B x 3 loads in flight.
-template <typename T>
+// Damped j-line Jacobi update xn = xo + omega * T^{-1} r along every (k,i)
+// grid line (Thomas sweeps with the factors of k_line_factors); ZG: zero
+// guess (xo unused).  xo may alias xn: each lane touches its own line only.
+// One lane per line; the loads of a batch of B steps are issued before its
+// dependent recurrence, so a wavefront keeps B x 3 loads in flight: the
+// fine-level lines are few (nx * nz lanes) and long, so the sweeps are
+// latency-bound.
+template <typename T, int B, bool ZG>
 __global__ void __launch_bounds__(64) k_thomas(LV L, const T *__restrict__ r, const T *__restrict__ ti,
-                                               const T *__restrict__ tm, const T *__restrict__ tu,
-                                               const T *xo, T omega, T *__restrict__ dscr, T *xn)
+                                               const T *__restrict__ tm, const T *__restrict__ tu, const T *xo,
+                                               T omega, T *__restrict__ dscr, T *xn)
 {
-    constexpr int B = 8;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= L.nx) return;
-    const int k = blockIdx.y;
-    const long long base = (long long)k * L.ny * L.nx + i, sy = L.nx;
+    // rows are addressed as (uniform row pointer)[lane], so the loads take a
+    // scalar base and a 32-bit lane offset
+    const unsigned i = blockIdx.x * 64u + threadIdx.x;
+    if (i >= (unsigned)L.nx) return;
+    const long long plane = (long long)blockIdx.y * L.ny * L.nx, sy = L.nx;
     T dp = 0;
     for (int j0 = 0; j0 < L.ny; j0 += B) {
         T rr[B], ii[B], mm[B];
 #pragma unroll
-        for (int q = 0; q < B; q++) {
+        for (int q = 0; q < B; q++)
             if (j0 + q < L.ny) {
-                long long id = base + (j0 + q) * sy;
-                rr[q] = r[id]; ii[q] = ti[id]; mm[q] = tm[id];
+                const long long o = plane + (j0 + q) * sy;
+                rr[q] = (r + o)[i]; ii[q] = (ti + o)[i]; mm[q] = (tm + o)[i];
             }
-        }
 #pragma unroll
-        for (int q = 0; q < B; q++) {
+        for (int q = 0; q < B; q++)
             if (j0 + q < L.ny) {
                 dp = rr[q] * ii[q] - mm[q] * dp;
-                dscr[base + (j0 + q) * sy] = dp;
+                (dscr + plane + (j0 + q) * sy)[i] = dp;
             }
-        }
     }
     T xj = 0;
     for (int j1 = L.ny - 1; j1 >= 0; j1 -= B) {
         T dd[B], uu[B], oo[B];
 #pragma unroll
-        for (int q = 0; q < B; q++) {
-            int j = j1 - q;
-            if (j >= 0) {
-                long long id = base + j * sy;
-                dd[q] = dscr[id]; uu[q] = tu[id]; oo[q] = xo ? xo[id] : T(0);
+        for (int q = 0; q < B; q++)
+            if (j1 - q >= 0) {
+                const long long o = plane + (j1 - q) * sy;
+                dd[q] = (dscr + o)[i]; uu[q] = (tu + o)[i]; oo[q] = ZG ? T(0) : (xo + o)[i];
             }
-        }
 #pragma unroll
-        for (int q = 0; q < B; q++) {
-            int j = j1 - q;
-            if (j >= 0) {
+        for (int q = 0; q < B; q++)
+            if (j1 - q >= 0) {
                 xj = dd[q] - uu[q] * xj;
-                xn[base + j * sy] = oo[q] + omega * xj;
+                (xn + plane + (j1 - q) * sy)[i] = oo[q] + omega * xj;
             }
-        }
     }
 }
 
@@ -326,6 +372,29 @@ __device__ __forceinline__ void dense_solve(long long n, int nc, const long long
         for (int c = 0; c < nc; c++) s += cinv[(long long)c * nc + r] * sb[c];
         x[cidx[r]] = s;
     }
+}
+
+// The same solve by wavefronts: one per coarsest cell, which takes its row
+// of the row-major inverse (coalesced across the lanes) or 0 when it is not
+// an unknown.  Grid ceil(n / 4) x 256.
+template <typename T>
+__global__ void __launch_bounds__(256) k_dense_mv(long long n, int nc, const int *__restrict__ rowof,
+                                                  const long long *__restrict__ cidx, const T *__restrict__ cinvr,
+                                                  const T *__restrict__ b, T *__restrict__ x)
+{
+    const long long q = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= n) return;   // uniform per wavefront
+    const int r = rowof[q];
+    if (r < 0) {
+        if (lane == 0) x[q] = T(0);
+        return;
+    }
+    const T *row = cinvr + (long long)r * nc;
+    double s = 0;
+    for (int c = lane; c < nc; c += 64) s += (double)row[c] * (double)b[cidx[c]];
+    s = wave_sum(s);
+    if (lane == 0) x[q] = (T)s;
 }
 
 template <typename T>
@@ -355,6 +424,7 @@ struct SmallLevels {
     const long long *cidx;
     const T *cinv;
     T omega, alpha;
+    int zero;    // the first level starts from x = 0 (else from the given x)
 };
 
 // xn = xo + omega T^{-1} r along every j-line (xo may be null: zero guess)
@@ -415,7 +485,13 @@ __global__ void __launch_bounds__(1024) k_small_cycle(SmallLevels<T> S)
 {
     for (int l = S.first; l < S.last; l++) {
         const LV L = S.lv[l], C = S.lv[l + 1];
-        wg_thomas<T>(L, S.b[l], S.ti[l], S.tl[l], S.tu[l], nullptr, S.omega, S.d[l], S.x[l]);
+        if (l == S.first && !S.zero) {
+            wg_resid<T>(L, S.A[l], S.x[l], S.b[l], S.y[l]);
+            __syncthreads();
+            wg_thomas<T>(L, S.y[l], S.ti[l], S.tl[l], S.tu[l], S.x[l], S.omega, S.d[l], S.x[l]);
+        } else {
+            wg_thomas<T>(L, S.b[l], S.ti[l], S.tl[l], S.tu[l], nullptr, S.omega, S.d[l], S.x[l]);
+        }
         __syncthreads();
         wg_resid<T>(L, S.A[l], S.x[l], S.b[l], S.y[l]);
         __syncthreads();
@@ -475,21 +551,38 @@ __global__ void k_multidot(long long n, const T *__restrict__ V, long long ldv, 
 }
 
 // dots of w with up to 8 basis vectors V_{v0..v0+nv-1} in one pass over w,
-// plus |w|^2 when self != 0; partials per block at part[slot*RB + block]
+// plus |w|^2 when self != 0; partials per block at part[v*RB + block] (|w|^2
+// in the last slot).  Each trip of the grid-stride
+// loop issues the loads of U blocks of elements before using them, and the
+// element blocks are addressed as (uniform pointer)[thread], so the loads
+// take a scalar base and a 32-bit offset.  Block size 256.
 template <typename T>
-__global__ void k_multidot8(long long n, const T *__restrict__ V, long long ldv, int v0, int nv, int self,
-                            const T *__restrict__ w, double *part)
+__global__ void __launch_bounds__(256) k_multidot8(long long n, const T *__restrict__ V, long long ldv, int v0, int nv,
+                                                   int self, const T *__restrict__ w, double *part)
 {
+    constexpr int U = 4;
     __shared__ double sh[4];
     double acc[8], accs = 0;
 #pragma unroll
     for (int v = 0; v < 8; v++) acc[v] = 0;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (long long)gridDim.x * blockDim.x) {
-        double wq = (double)w[q];
+    const unsigned t = threadIdx.x;
+    const long long S = (long long)gridDim.x * 256;
+    for (long long b0 = (long long)blockIdx.x * 256; b0 < n; b0 += U * S) {
+        T wv[U], vv[U][8];
 #pragma unroll
-        for (int v = 0; v < 8; v++)
-            if (v < nv) acc[v] += (double)V[(long long)(v0 + v) * ldv + q] * wq;
-        accs += wq * wq;
+        for (int u = 0; u < U; u++) {
+            const long long qb = b0 + u * S;
+            const bool ok = qb + t < n;
+            wv[u] = ok ? (w + qb)[t] : T(0);
+#pragma unroll
+            for (int v = 0; v < 8; v++) vv[u][v] = (v < nv && ok) ? (V + (long long)(v0 + v) * ldv + qb)[t] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int v = 0; v < 8; v++) acc[v] += (double)vv[u][v] * (double)wv[u];
+            accs += (double)wv[u] * (double)wv[u];
+        }
     }
     for (int v = 0; v < nv; v++) {
         double r = block_sum<256>(acc[v], sh);
@@ -501,16 +594,48 @@ __global__ void k_multidot8(long long n, const T *__restrict__ V, long long ldv,
     }
 }
 
-// w -= sum_v h[v] V_v  (h on the device)
+// wout = win - sum_v (h[v] / nrm2[v]) V_v  (nrm2 null: 1), h and nrm2 on the
+// device; win may alias wout.  With part, also the block partials of
+// |wout|^2 at part[(VFS_RED_SLOTS-1)*RB + block].  nv <= 64, block size 256.
 template <typename T>
-__global__ void k_multiaxpy(long long n, const T *__restrict__ V, long long ldv, int nv, const double *__restrict__ h,
-                            T *__restrict__ w)
+__global__ void __launch_bounds__(256) k_multiaxpy(long long n, const T *__restrict__ V, long long ldv, int nv,
+                                                   const double *__restrict__ h, const double *__restrict__ nrm2,
+                                                   const T *win, T *wout, double *part)
 {
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    T s = w[q];
-    for (int v = 0; v < nv; v++) s -= (T)h[v] * V[(long long)v * ldv + q];
-    w[q] = s;
+    constexpr int U = 4;   // elements per thread and trip, loaded together
+    __shared__ double sc[64];
+    __shared__ double sh[4];
+    if (threadIdx.x < nv) sc[threadIdx.x] = h[threadIdx.x] / (nrm2 ? nrm2[threadIdx.x] : 1.0);
+    __syncthreads();
+    const unsigned t = threadIdx.x;
+    const long long S = (long long)gridDim.x * 256;
+    double acc = 0;
+    for (long long b0 = (long long)blockIdx.x * 256; b0 < n; b0 += U * S) {
+        T s[U];
+        bool ok[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            ok[u] = b0 + u * S + t < n;
+            s[u] = ok[u] ? (win + b0 + u * S)[t] : T(0);
+        }
+        for (int v = 0; v < nv; v++) {
+            const T cv = (T)sc[v];
+            const T *vp = V + (long long)v * ldv + b0;
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (ok[u]) s[u] -= cv * (vp + u * S)[t];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (ok[u]) {
+                (wout + b0 + u * S)[t] = s[u];
+                acc += (double)s[u] * (double)s[u];
+            }
+    }
+    if (part) {
+        double r = block_sum<256>(acc, sh);
+        if (threadIdx.x == 0) part[(long long)(VFS_RED_SLOTS - 1) * VFS_RED_BLOCKS + blockIdx.x] = r;
+    }
 }
 
 // u = sum_v y[v] V_v  (y on the device)
@@ -523,16 +648,6 @@ __global__ void k_combine(long long n, const T *__restrict__ V, long long ldv, i
     T s = 0;
     for (int v = 0; v < nv; v++) s += (T)y[v] * V[(long long)v * ldv + q];
     u[q] = s;
-}
-
-// out = w * (1/sqrt(nrm2[0]))
-template <typename T>
-__global__ void k_normalize(long long n, const T *__restrict__ w, const double *__restrict__ nrm2, T *__restrict__ out)
-{
-    long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    double s = 1.0 / sqrt(nrm2[0]);
-    out[q] = (T)((double)w[q] * s);
 }
 
 template <typename T>
@@ -642,6 +757,7 @@ int vfs_poisson_destroy(vfs_ctx *c)
     hipFree(ps->rowflag);
     for (auto &l : ps->lv) free_level(l);
     hipFree(ps->cinv64); hipFree(ps->cinv32); hipFree(ps->cidx);
+    hipFree(ps->cinvr64); hipFree(ps->cinvr32); hipFree(ps->rowof);
     hipFree(ps->V64); hipFree(ps->V32); hipFree(ps->w64); hipFree(ps->z64); hipFree(ps->r64); hipFree(ps->x64);
     hipFree(ps->bb64); hipFree(ps->e64); hipFree(ps->w32); hipFree(ps->z32); hipFree(ps->x32); hipFree(ps->bb32);
     hipFree(ps->dots);
@@ -711,15 +827,14 @@ static int invert_dense(std::vector<double> &a, int n)
                 std::swap(inv[(size_t)piv * n + q], inv[(size_t)col * n + q]);
             }
         double s = 1.0 / a[(size_t)col * n + col];
-        for (int q = 0; q < n; q++) { a[(size_t)col * n + q] *= s; inv[(size_t)col * n + q] *= s; }
+        for (int q = col; q < n; q++) a[(size_t)col * n + q] *= s;
+        for (int q = 0; q < n; q++) inv[(size_t)col * n + q] *= s;
         for (int r = 0; r < n; r++) {
             if (r == col) continue;
             double f = a[(size_t)r * n + col];
             if (f == 0.0) continue;
-            for (int q = 0; q < n; q++) {
-                a[(size_t)r * n + q] -= f * a[(size_t)col * n + q];
-                inv[(size_t)r * n + q] -= f * inv[(size_t)col * n + q];
-            }
+            for (int q = col; q < n; q++) a[(size_t)r * n + q] -= f * a[(size_t)col * n + q];
+            for (int q = 0; q < n; q++) inv[(size_t)r * n + q] -= f * inv[(size_t)col * n + q];
         }
     }
     a.swap(inv);
@@ -741,7 +856,9 @@ int vfs_mg_setup(vfs_ctx *c)
     hipLaunchKernelGGL(k_gather_level0, dim3(grid_for(l0.n, 256)), dim3(256), 0, c->stream, d, lv_of(l0), ps->coef, l0.A64, l0.A32);
     VFS_CHECK_LAUNCH();
     ps->lv.push_back(l0);
-    const long long NCOARSE = 600;
+    // coarsen until the level fits the dense solve (the one-workgroup solve
+    // of the small-level kernel holds at most VFS_MAX_DENSE unknowns)
+    const long long NCOARSE = VFS_MAX_DENSE;
     while (ps->lv.back().n > NCOARSE) {
         MGLevel &f = ps->lv.back();
         MGLevel cl;
@@ -813,6 +930,16 @@ int vfs_mg_setup(vfs_ctx *c)
     DMALLOC(ps->cinv64, (size_t)nc * nc * sizeof(double));
     DMALLOC(ps->cinv32, (size_t)nc * nc * sizeof(float));
     DMALLOC(ps->cidx, (size_t)nc * sizeof(long long));
+    DMALLOC(ps->cinvr64, (size_t)nc * nc * sizeof(double));
+    DMALLOC(ps->cinvr32, (size_t)nc * nc * sizeof(float));
+    DMALLOC(ps->rowof, (size_t)cl.n * sizeof(int));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));   // the zero fills before the uploads
+    {
+        std::vector<float> R32(M.begin(), M.end());
+        VFS_CHECK_HIP(hipMemcpy(ps->cinvr64, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
+        VFS_CHECK_HIP(hipMemcpy(ps->cinvr32, R32.data(), R32.size() * sizeof(float), hipMemcpyHostToDevice));
+        VFS_CHECK_HIP(hipMemcpy(ps->rowof, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+    }
     // column-major (cinv[c * nc + r]) so that the rows of a solve, one per
     // thread, read consecutive addresses for each column
     {
@@ -833,8 +960,8 @@ int vfs_mg_setup(vfs_ctx *c)
     DMALLOC(ps->x64, n0 * sizeof(double)); DMALLOC(ps->bb64, n0 * sizeof(double)); DMALLOC(ps->e64, n0 * sizeof(double));
     DMALLOC(ps->w32, n0 * sizeof(float)); DMALLOC(ps->z32, n0 * sizeof(float)); DMALLOC(ps->x32, n0 * sizeof(float));
     DMALLOC(ps->bb32, n0 * sizeof(float));
-    DMALLOC(ps->dots, 256 * sizeof(double));
-    VFS_CHECK_HIP(hipHostMalloc((void **)&ps->hdots, 256 * sizeof(double), hipHostMallocDefault));
+    DMALLOC(ps->dots, 512 * sizeof(double));
+    VFS_CHECK_HIP(hipHostMalloc((void **)&ps->hdots, 512 * sizeof(double), hipHostMallocDefault));
     VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -865,20 +992,49 @@ static int launch_resid(vfs_ctx *c, const MGLevel &L, const T *A, const T *x, co
     return 0;
 }
 
-// one V(1,1)-cycle: x_l = M_l^{-1} b_l (level vectors b, x of each level;
-// y holds residuals, d the Thomas forward sweep)
+template <typename T, int B>
+static void launch_thomas_b(vfs_ctx *c, dim3 gl, LV Lv, const T *r, const T *ti, const T *tm, const T *tu, const T *xo,
+                            T *dd, T *xn)
+{
+    const T omega = (T)c->pois->omega;
+    if (xo) hipLaunchKernelGGL((k_thomas<T, B, false>), gl, dim3(64), 0, c->stream, Lv, r, ti, tm, tu, xo, omega, dd, xn);
+    else hipLaunchKernelGGL((k_thomas<T, B, true>), gl, dim3(64), 0, c->stream, Lv, r, ti, tm, tu, xo, omega, dd, xn);
+}
+
+// xn = xo + omega T^{-1} r on level L (xo null: zero guess)
 template <typename T>
-static int vcycle(vfs_ctx *c, int l)
+static int launch_thomas(vfs_ctx *c, const MGLevel &L, const T *r, const T *xo, T *xn)
+{
+    const T *tm, *tu, *ti;
+    fac_of<T>(L, tm, tu, ti);
+    T *dd = vec_of<T>(const_cast<MGLevel &>(L), 3);
+    dim3 gl((L.nx + 63) / 64, L.nz);
+    launch_thomas_b<T, 8>(c, gl, lv_of(L), r, ti, tm, tu, xo, dd, xn);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+// One multigrid cycle on level l: x = M_l^{-1} b from x = 0 (zero_guess), or
+// one cycle started from the given x.  Per level: line smoothing, the
+// restricted residual, the coarse cycle (twice on levels l < wdepth: a
+// W-cycle there, since a V-cycle loses about one digit per added level on
+// the piecewise-constant aggregation hierarchy), the residual of the
+// corrected iterate, and line smoothing with the prolongation folded in.
+// Scratch: the level's y and d vectors and the coarser levels' b and x.
+template <typename T>
+static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
 {
     Poisson *ps = c->pois;
     MGLevel &L = ps->lv[l];
     LV Lv = lv_of(L);
-    const T omega = (T)ps->omega, alpha = (T)ps->alpha;
-    T *b = vec_of<T>(L, 0), *x = vec_of<T>(L, 1), *y = vec_of<T>(L, 2), *dd = vec_of<T>(L, 3);
+    const T alpha = (T)ps->alpha;
+    T *y = vec_of<T>(L, 2);
     const T *A = A_of<T>(L);
     if (l == (int)ps->lv.size() - 1) {
-        const T *cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
-        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), 0, c->stream, L.n, ps->nc, ps->cidx, cinv, b, x);
+        // dense solve: exact, whatever the starting guess
+        const T *cinvr = std::is_same<T, double>::value ? (const T *)ps->cinvr64 : (const T *)ps->cinvr32;
+        hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((L.n + 3) / 4)), dim3(256), 0, c->stream, L.n, ps->nc,
+                           (const int *)ps->rowof, (const long long *)ps->cidx, cinvr, b, x);
         VFS_CHECK_LAUNCH();
         return 0;
     }
@@ -893,50 +1049,56 @@ static int vcycle(vfs_ctx *c, int l)
             fac_of<T>(Q, S.tl[q], S.tu[q], S.ti[q]);
             S.b[q] = vec_of<T>(Q, 0); S.x[q] = vec_of<T>(Q, 1); S.y[q] = vec_of<T>(Q, 2); S.d[q] = vec_of<T>(Q, 3);
         }
+        S.b[l] = const_cast<T *>(b);
+        S.x[l] = x;
         S.nc = ps->nc;
         S.cidx = ps->cidx;
         S.cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
-        S.omega = omega;
+        S.omega = (T)ps->omega;
         S.alpha = alpha;
+        S.zero = zero_guess ? 1 : 0;
         hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), 0, c->stream, S);
         VFS_CHECK_LAUNCH();
         return 0;
     }
-    const T *tm, *tu, *ti;
-    fac_of<T>(L, tm, tu, ti);
-    dim3 gl((L.nx + 63) / 64, L.nz);
-    // pre-smooth from a zero guess: x = omega T^{-1} b
-    hipLaunchKernelGGL(k_thomas<T>, dim3(gl), dim3(64), 0, c->stream, Lv, (const T *)b, ti, tm, tu, (const T *)nullptr,
-                       omega, dd, x);
-    VFS_CHECK_LAUNCH();
-    if (launch_resid<T>(c, L, A, x, b, y)) return -1;
+    // pre-smoothing
+    if (zero_guess) {
+        if (launch_thomas<T>(c, L, b, nullptr, x)) return -1;
+    } else {
+        if (launch_resid<T>(c, L, A, x, b, y)) return -1;
+        if (launch_thomas<T>(c, L, y, x, x)) return -1;
+    }
     MGLevel &C = ps->lv[l + 1];
     LV Cv = lv_of(C);
-    hipLaunchKernelGGL(k_restrict<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, Lv, Cv, (const T *)y, vec_of<T>(C, 0));
-    VFS_CHECK_LAUNCH();
-    if (vcycle<T>(c, l + 1)) return -1;
-    if (l < ps->wdepth && l + 2 < (int)ps->lv.size()) {
-        // W-cycle: a second coarse correction on the residual of the first
-        // (a V-cycle alone loses about one digit per level added on the
-        // piecewise-constant aggregation hierarchy)
-        const T *Ac = A_of<T>(C);
-        T *cb = vec_of<T>(C, 0), *cx = vec_of<T>(C, 1), *cy = vec_of<T>(C, 2), *ce = vec_of<T>(C, 4);
-        if (launch_resid<T>(c, C, Ac, cx, cb, cy)) return -1;
-        VFS_CHECK_HIP(hipMemcpyAsync(ce, cx, (size_t)C.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        VFS_CHECK_HIP(hipMemcpyAsync(cb, cy, (size_t)C.n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-        if (vcycle<T>(c, l + 1)) return -1;
-        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(grid_for(C.n, 256)), dim3(256), 0, c->stream, C.n, (const T *)ce, cx);
+    T *cb = vec_of<T>(C, 0), *cx = vec_of<T>(C, 1);
+    {
+        const int bs = row_block(L.nx);
+        dim3 g((L.nx + bs - 1) / bs, C.ny, C.nz);
+        if (L.nslot == 19)
+            hipLaunchKernelGGL((k_resid_restrict<T, 19>), g, dim3(bs), 0, c->stream, Lv, Cv, A, (const T *)x, b, cb);
+        else
+            hipLaunchKernelGGL((k_resid_restrict<T, 27>), g, dim3(bs), 0, c->stream, Lv, Cv, A, (const T *)x, b, cb);
         VFS_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(k_prolong<T>, row_grid(L), dim3(row_block(L.nx)), 0, c->stream, Lv, Cv,
-                       (const T *)vec_of<T>(C, 1), alpha, x);
+    if (vcycle<T>(c, l + 1, cb, cx, true)) return -1;
+    if (l < ps->wdepth && l + 2 < (int)ps->lv.size())
+        if (vcycle<T>(c, l + 1, cb, cx, false)) return -1;
+    // post-smoothing of x + alpha P e.  On the 19-point fine level a
+    // prolongation pass and the plain residual (the coarse reads at every
+    // stencil point cost more there than the pass); on the 27-point levels,
+    // where launches dominate, one kernel forms both x + alpha P e and its
+    // residual.
+    if (L.nslot == 19) {
+        hipLaunchKernelGGL(k_prolong<T>, row_grid(L), dim3(row_block(L.nx)), 0, c->stream, Lv, Cv, (const T *)cx, alpha, x);
+        VFS_CHECK_LAUNCH();
+        if (launch_resid<T>(c, L, A, x, b, y)) return -1;
+        return launch_thomas<T>(c, L, y, x, x);
+    }
+    T *xe = vec_of<T>(L, 4);
+    hipLaunchKernelGGL((k_resid_pe<T, 27>), row_grid(L), dim3(row_block(L.nx)), 0, c->stream, Lv, Cv, A, (const T *)x,
+                       (const T *)cx, alpha, b, y, xe);
     VFS_CHECK_LAUNCH();
-    // post-smooth: x += omega T^{-1} (b - A x), in place
-    if (launch_resid<T>(c, L, A, x, b, y)) return -1;
-    hipLaunchKernelGGL(k_thomas<T>, dim3(gl), dim3(64), 0, c->stream, Lv, (const T *)y, ti, tm, tu, (const T *)x, omega,
-                       dd, x);
-    VFS_CHECK_LAUNCH();
-    return 0;
+    return launch_thomas<T>(c, L, y, xe, x);
 }
 
 template <typename T> static T *kv(Poisson *ps, int w);
@@ -948,103 +1110,106 @@ static int multidot_finish(vfs_ctx *c, int nv, int nb, double *dst)
     return vfs_reduce_finish(c, nb, nv, 0, dst);
 }
 
-// Right-preconditioned restarted GMRES in precision T on A_T e = r, started
-// from e = 0, where r (level-0 compact, precision T) is in V[0]'s slot
-// (unnormalised in ps->bb*).  Stops when the estimated residual drops below
-// tol * ||r|| or after maxit iterations.  Result accumulated into kv(...,3).
+// Right-preconditioned restarted GMRES in precision T on A_T e = rhs from
+// e = 0 (rhs: level-0 compact, precision T), with classical Gram-Schmidt
+// (a second pass only on heavy cancellation, see below).
+// The basis vectors U_q are kept unnormalised, |U_q|^2 in nrm (device) and
+// hnrm (host), and the Hessenberg entries are scaled on the host:
+//   H[q][k] = (d1_q + d2_q) / (|U_q| |U_k|),  H[k+1][k] = |U_{k+1}| / |U_k|,
+// where d1, d2 are the passes' dots of U_q with w = A M^{-1} U_k.  The
+// orthogonalisation writes U_{k+1} and returns its norm in the same pass, so
+// a new basis vector costs no normalisation pass.  One host synchronisation
+// per pass.  Stops when the estimated residual drops below tol * |rhs| or
+// after maxit iterations; the result is in kv(..., 3).
 template <typename T>
 static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, double *est)
 {
     Poisson *ps = c->pois;
     MGLevel &L0 = ps->lv[0];
-    LV Lv = lv_of(L0);
     const long long n = L0.n;
     const int m = ps->m;
+    VFS_REQUIRE(m + 1 <= 64, "GMRES restart length above 63");
     T *V = kv<T>(ps, 0), *w = kv<T>(ps, 1), *z = kv<T>(ps, 2), *x = kv<T>(ps, 3);
+    T *xl = vec_of<T>(L0, 1);   // preconditioner output
     const T *A = A_of<T>(L0);
     unsigned gb = grid_for(n, 256);
-    int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
-    double *dres = c->red + VFS_RED_SLOTS * VFS_RED_BLOCKS + 16;   // device results
+    const int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
+    double *one = c->red + VFS_RED_SLOTS * VFS_RED_BLOCKS + 24;   // device constant 1.0
+    double *nrm = ps->dots + 256, *hnrm = ps->hdots + 256;
     VFS_CHECK_HIP(hipMemsetAsync(x, 0, n * sizeof(T), c->stream));
-    // r0 = rhs (x = 0)
-    hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, rhs, n, 1, rhs, c->red);
+    // U_0 = rhs (x = 0)
+    VFS_CHECK_HIP(hipMemcpyAsync(V, rhs, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, 1, (const T *)V, c->red);
     VFS_CHECK_LAUNCH();
-    if (multidot_finish(c, 1, nbr, dres)) return -1;
-    VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (multidot_finish(c, 1, nbr, nrm)) return -1;
+    VFS_CHECK_HIP(hipMemcpyAsync(hnrm, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
-    double bnorm = sqrt(ps->hdots[0]);
+    const double bnorm = sqrt(hnrm[0]);
     int it = 0;
     if (bnorm == 0.0) { *iters = 0; *est = 0; return 0; }
+    hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, one, 1.0);
+    VFS_CHECK_LAUNCH();
     std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), s(m + 1), y(m);
-    const T *cur_rhs = rhs;
     double res = 1.0;
     bool first = true;
     while (it < maxit) {
-        // v0 = r / ||r||, r = rhs - A x
+        s.assign(m + 1, 0.0);
         if (first) {
-            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, cur_rhs, dres, V);
-            VFS_CHECK_LAUNCH();
-            s.assign(m + 1, 0.0);
             s[0] = bnorm;
         } else {
-            if (launch_spmv<T>(c, L0, A, x, w)) return -1;
-            // w = rhs - A x  via w = -(A x - rhs)
-            hipLaunchKernelGGL(k_set_scalar, dim3(1), dim3(1), 0, c->stream, dres + 8, 1.0);
+            // U_0 = A x - rhs = -r, and |r|
+            if (launch_spmv<T>(c, L0, A, (const T *)x, V)) return -1;
+            hipLaunchKernelGGL(k_multiaxpy<T>, dim3(nbr), dim3(256), 0, c->stream, n, rhs, n, 1, (const double *)one,
+                               (const double *)nullptr, (const T *)V, V, c->red);
             VFS_CHECK_LAUNCH();
-            hipLaunchKernelGGL(k_multiaxpy<T>, dim3(gb), dim3(256), 0, c->stream, n, rhs, n, 1, dres + 8, w);
-            VFS_CHECK_LAUNCH();
-            // now w = A x - rhs; residual norm
-            hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
-            VFS_CHECK_LAUNCH();
-            if (multidot_finish(c, 1, nbr, dres)) return -1;
-            VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, dres, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+            if (vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, nrm)) return -1;
+            VFS_CHECK_HIP(hipMemcpyAsync(hnrm, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
             VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
-            double beta = sqrt(ps->hdots[0]);
+            const double beta = sqrt(hnrm[0]);
             res = beta / bnorm;
             if (res <= tol) break;
-            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, w, dres, V);
-            VFS_CHECK_LAUNCH();
-            s.assign(m + 1, 0.0);
-            s[0] = -beta;   // v0 = (A x - rhs)/beta = -r/beta
+            s[0] = -beta;
         }
         first = false;
         int kk = 0;
         for (; kk < m && it < maxit; kk++, it++) {
-            // z = M^{-1} v_kk ; w = A z
-            T *vk = V + (size_t)kk * n;
-            VFS_CHECK_HIP(hipMemcpyAsync(vec_of<T>(L0, 0), vk, n * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-            if (vcycle<T>(c, 0)) return -1;
-            if (launch_spmv<T>(c, L0, A, (const T *)vec_of<T>(L0, 1), w)) return -1;
-            // classical Gram-Schmidt; a second pass only when the first
-            // cancelled more than half of |w| (Daniel-Gragg-Kaufman-Stewart)
+            // w = A M^{-1} U_kk
+            T *uk = V + (size_t)kk * n, *un = V + (size_t)(kk + 1) * n;
+            if (vcycle<T>(c, 0, (const T *)uk, xl, true)) return -1;
+            if (launch_spmv<T>(c, L0, A, (const T *)xl, w)) return -1;
+            // one classical Gram-Schmidt pass; a second one only when the first
+            // cancelled so much of w that its orthogonality error, about
+            // u |w| / |w_perp|, would exceed u / sqrt(tau) (tau 1e-12 in FP64,
+            // 1e-4 in FP32: errors below 1e-10 and 1e-5, far under the
+            // tolerances the two precisions are asked for)
+            const double tau = std::is_same<T, double>::value ? 1e-12 : 1e-4;
             double h1[64], h2[64];
             for (int pass = 0; pass < 2; pass++) {
+                const T *src = pass == 0 ? w : un;
                 for (int v0 = 0; v0 <= kk; v0 += 8) {
-                    int nv = std::min(8, kk + 1 - v0);
+                    const int nv = std::min(8, kk + 1 - v0);
                     hipLaunchKernelGGL(k_multidot8<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, v0, nv,
-                                       (int)(pass == 0 && v0 == 0), (const T *)w, c->red);
+                                       (int)(pass == 0 && v0 == 0), src, c->red);
                     VFS_CHECK_LAUNCH();
                 }
                 if (multidot_finish(c, kk + 1, nbr, ps->dots + 64 * pass)) return -1;
                 if (pass == 0 && vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, ps->dots + 130)) return -1;
-                hipLaunchKernelGGL(k_multiaxpy<T>, dim3(gb), dim3(256), 0, c->stream, n, V, n, kk + 1, ps->dots + 64 * pass, w);
+                hipLaunchKernelGGL(k_multiaxpy<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, kk + 1,
+                                   (const double *)(ps->dots + 64 * pass), (const double *)nrm, src, un, c->red);
                 VFS_CHECK_LAUNCH();
-                hipLaunchKernelGGL(k_multidot<T>, dim3(nbr), dim3(256), 0, c->stream, n, w, n, 1, w, c->red);
-                VFS_CHECK_LAUNCH();
-                if (multidot_finish(c, 1, nbr, ps->dots + 128)) return -1;
-                VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, 136 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+                if (vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, nrm + kk + 1)) return -1;
+                VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, (256 + kk + 2) * sizeof(double),
+                                             hipMemcpyDeviceToHost, c->stream));
                 VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
                 for (int q = 0; q <= kk; q++) (pass == 0 ? h1 : h2)[q] = ps->hdots[64 * pass + q];
                 if (pass == 0) {
                     for (int q = 0; q <= kk; q++) h2[q] = 0;
-                    double before = ps->hdots[130], after = ps->hdots[128];
-                    if (after > 0.25 * before) break;
+                    if (hnrm[kk + 1] >= tau * ps->hdots[130]) break;
                 }
             }
-            hipLaunchKernelGGL(k_normalize<T>, dim3(gb), dim3(256), 0, c->stream, n, w, ps->dots + 128, V + (size_t)(kk + 1) * n);
-            VFS_CHECK_LAUNCH();
-            for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = h1[q] + h2[q];
-            H[(size_t)(kk + 1) * m + kk] = sqrt(ps->hdots[128]);
+            const double nk = sqrt(hnrm[kk]);
+            for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = (h1[q] + h2[q]) / (sqrt(hnrm[q]) * nk);
+            H[(size_t)(kk + 1) * m + kk] = sqrt(hnrm[kk + 1]) / nk;
             for (int q = 0; q < kk; q++) {
                 double t0 = cs[q] * H[(size_t)q * m + kk] + sn[q] * H[(size_t)(q + 1) * m + kk];
                 H[(size_t)(q + 1) * m + kk] = -sn[q] * H[(size_t)q * m + kk] + cs[q] * H[(size_t)(q + 1) * m + kk];
@@ -1059,20 +1224,21 @@ static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, do
             s[kk + 1] = -sn[kk] * s[kk];
             s[kk] = cs[kk] * s[kk];
             res = fabs(s[kk + 1]) / bnorm;
-            if (res <= tol) { kk++; it++; break; }
+            if (res <= tol || hnrm[kk + 1] == 0.0) { kk++; it++; break; }
         }
-        // x += M^{-1} V y
+        // x += M^{-1} sum_q U_q y_q / |U_q|
         for (int q = kk - 1; q >= 0; q--) {
             double t0 = s[q];
             for (int p = q + 1; p < kk; p++) t0 -= H[(size_t)q * m + p] * y[p];
             y[q] = t0 / H[(size_t)q * m + q];
         }
-        for (int q = 0; q < kk; q++) ps->hdots[192 + q] = y[q];   // pinned staging
+        for (int q = 0; q < kk; q++) ps->hdots[192 + q] = y[q] / sqrt(hnrm[q]);   // pinned staging
         VFS_CHECK_HIP(hipMemcpyAsync(ps->dots + 192, ps->hdots + 192, kk * sizeof(double), hipMemcpyHostToDevice, c->stream));
-        hipLaunchKernelGGL(k_combine<T>, dim3(gb), dim3(256), 0, c->stream, n, V, n, kk, ps->dots + 192, vec_of<T>(L0, 0));
+        hipLaunchKernelGGL(k_combine<T>, dim3(gb), dim3(256), 0, c->stream, n, (const T *)V, n, kk,
+                           (const double *)(ps->dots + 192), z);
         VFS_CHECK_LAUNCH();
-        if (vcycle<T>(c, 0)) return -1;
-        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(gb), dim3(256), 0, c->stream, n, (const T *)vec_of<T>(L0, 1), x);
+        if (vcycle<T>(c, 0, (const T *)z, xl, true)) return -1;
+        hipLaunchKernelGGL(k_axpy_dev<T>, dim3(gb), dim3(256), 0, c->stream, n, (const T *)xl, x);
         VFS_CHECK_LAUNCH();
         if (res <= tol) break;
     }
