@@ -32,9 +32,8 @@ struct Poisson {
     std::vector<MGLevel> lv;
     int nc;                             // unknowns of the dense coarsest level
     double *cinv64; float *cinv32;      // (A_c + 1 1^T / nc)^{-1}, nc*nc, column-major
-    double *cinvr64; float *cinvr32;    // the same, row-major
+    double *cinvr64; float *cinvr32;    // the same scattered to all n cells of the level, row-major (n*n)
     long long *cidx;                    // coarsest compact index of each dense unknown
-    int *rowof;                         // dense row of each coarsest cell, -1 if none
     // Krylov space (level-0 compact vectors)
     int m;
     double *V64; float *V32;            // (m+1) vectors

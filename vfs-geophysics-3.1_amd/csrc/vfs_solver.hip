@@ -374,25 +374,20 @@ __device__ __forceinline__ void dense_solve(long long n, int nc, const long long
     }
 }
 
-// The same solve by wavefronts: one per coarsest cell, which takes its row
-// of the row-major inverse (coalesced across the lanes) or 0 when it is not
-// an unknown.  Grid ceil(n / 4) x 256.
+// The same solve by wavefronts, x = F b with F the inverse scattered to all
+// n cells of the coarsest level (zero rows and columns for the cells that
+// are no unknowns), row-major: one wavefront per row, coalesced across the
+// lanes, with no gather.  Grid ceil(n / 4) x 256.
 template <typename T>
-__global__ void __launch_bounds__(256) k_dense_mv(long long n, int nc, const int *__restrict__ rowof,
-                                                  const long long *__restrict__ cidx, const T *__restrict__ cinvr,
-                                                  const T *__restrict__ b, T *__restrict__ x)
+__global__ void __launch_bounds__(256) k_dense_mv(long long n, const T *__restrict__ F, const T *__restrict__ b,
+                                                  T *__restrict__ x)
 {
     const long long q = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (q >= n) return;   // uniform per wavefront
-    const int r = rowof[q];
-    if (r < 0) {
-        if (lane == 0) x[q] = T(0);
-        return;
-    }
-    const T *row = cinvr + (long long)r * nc;
+    const T *row = F + q * n;
     double s = 0;
-    for (int c = lane; c < nc; c += 64) s += (double)row[c] * (double)b[cidx[c]];
+    for (long long c = lane; c < n; c += 64) s += (double)row[c] * (double)b[c];
     s = wave_sum(s);
     if (lane == 0) x[q] = (T)s;
 }
@@ -550,47 +545,67 @@ __global__ void k_multidot(long long n, const T *__restrict__ V, long long ldv, 
     }
 }
 
-// dots of w with up to 8 basis vectors V_{v0..v0+nv-1} in one pass over w,
-// plus |w|^2 when self != 0; partials per block at part[v*RB + block] (|w|^2
-// in the last slot).  Each trip of the grid-stride
-// loop issues the loads of U blocks of elements before using them, and the
-// element blocks are addressed as (uniform pointer)[thread], so the loads
-// take a scalar base and a 32-bit offset.  Block size 256.
-template <typename T>
-__global__ void __launch_bounds__(256) k_multidot8(long long n, const T *__restrict__ V, long long ldv, int v0, int nv,
-                                                   int self, const T *__restrict__ w, double *part)
+// dots of w with NV basis vectors V_{v0..v0+NV-1} in one pass over w, plus
+// |w|^2 when self != 0; partials per block at part[v*RB + block] (|w|^2 in
+// the last slot).  The full trips of the grid-stride loop issue the loads of
+// U element blocks with no predicate, so none waits for another; elements
+// are addressed as (uniform pointer)[thread] (a scalar base and a 32-bit
+// offset).  The partial blocks at the end go one at a time.  Block size 256.
+template <typename T, int NV>
+__global__ void __launch_bounds__(256) k_multidot8(long long n, const T *__restrict__ V, long long ldv, int v0, int self,
+                                                   const T *__restrict__ w, double *part)
 {
     constexpr int U = 4;
     __shared__ double sh[4];
-    double acc[8], accs = 0;
+    double acc[NV], accs = 0;
 #pragma unroll
-    for (int v = 0; v < 8; v++) acc[v] = 0;
+    for (int v = 0; v < NV; v++) acc[v] = 0;
     const unsigned t = threadIdx.x;
     const long long S = (long long)gridDim.x * 256;
-    for (long long b0 = (long long)blockIdx.x * 256; b0 < n; b0 += U * S) {
-        T wv[U], vv[U][8];
+    const T *Vb = V + (long long)v0 * ldv;
+    long long b0 = (long long)blockIdx.x * 256;
+    for (; b0 + (U - 1) * S + 256 <= n; b0 += U * S) {
+        T wv[U], vv[U][NV];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const long long qb = b0 + u * S;
-            const bool ok = qb + t < n;
-            wv[u] = ok ? (w + qb)[t] : T(0);
+            wv[u] = (w + b0 + u * S)[t];
 #pragma unroll
-            for (int v = 0; v < 8; v++) vv[u][v] = (v < nv && ok) ? (V + (long long)(v0 + v) * ldv + qb)[t] : T(0);
+            for (int v = 0; v < NV; v++) vv[u][v] = (Vb + v * ldv + b0 + u * S)[t];
         }
 #pragma unroll
         for (int u = 0; u < U; u++) {
 #pragma unroll
-            for (int v = 0; v < 8; v++) acc[v] += (double)vv[u][v] * (double)wv[u];
+            for (int v = 0; v < NV; v++) acc[v] += (double)vv[u][v] * (double)wv[u];
             accs += (double)wv[u] * (double)wv[u];
         }
     }
-    for (int v = 0; v < nv; v++) {
+    for (; b0 < n; b0 += S)
+        if (b0 + t < n) {
+            const double wq = (double)(w + b0)[t];
+#pragma unroll
+            for (int v = 0; v < NV; v++) acc[v] += (double)(Vb + v * ldv + b0)[t] * wq;
+            accs += wq * wq;
+        }
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
         double r = block_sum<256>(acc[v], sh);
         if (threadIdx.x == 0) part[(long long)(v0 + v) * VFS_RED_BLOCKS + blockIdx.x] = r;
     }
     if (self) {
         double r = block_sum<256>(accs, sh);
         if (threadIdx.x == 0) part[(long long)(VFS_RED_SLOTS - 1) * VFS_RED_BLOCKS + blockIdx.x] = r;
+    }
+}
+
+template <typename T>
+static void launch_multidot8(hipStream_t st, int nbr, long long n, const T *V, long long ldv, int v0, int nv, int self,
+                             const T *w, double *part)
+{
+    switch (nv) {
+#define MD8(K) case K: hipLaunchKernelGGL((k_multidot8<T, K>), dim3(nbr), dim3(256), 0, st, n, V, ldv, v0, self, w, part); break;
+        MD8(1) MD8(2) MD8(3) MD8(4) MD8(5) MD8(6) MD8(7) MD8(8)
+#undef MD8
+        default: break;
     }
 }
 
@@ -602,7 +617,7 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long long n, const T *__restr
                                                    const double *__restrict__ h, const double *__restrict__ nrm2,
                                                    const T *win, T *wout, double *part)
 {
-    constexpr int U = 4;   // elements per thread and trip, loaded together
+    constexpr int U = 4;   // element blocks per thread and trip, loaded together
     __shared__ double sc[64];
     __shared__ double sh[4];
     if (threadIdx.x < nv) sc[threadIdx.x] = h[threadIdx.x] / (nrm2 ? nrm2[threadIdx.x] : 1.0);
@@ -610,28 +625,30 @@ __global__ void __launch_bounds__(256) k_multiaxpy(long long n, const T *__restr
     const unsigned t = threadIdx.x;
     const long long S = (long long)gridDim.x * 256;
     double acc = 0;
-    for (long long b0 = (long long)blockIdx.x * 256; b0 < n; b0 += U * S) {
+    long long b0 = (long long)blockIdx.x * 256;
+    for (; b0 + (U - 1) * S + 256 <= n; b0 += U * S) {
         T s[U];
-        bool ok[U];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            ok[u] = b0 + u * S + t < n;
-            s[u] = ok[u] ? (win + b0 + u * S)[t] : T(0);
-        }
+        for (int u = 0; u < U; u++) s[u] = (win + b0 + u * S)[t];
         for (int v = 0; v < nv; v++) {
             const T cv = (T)sc[v];
             const T *vp = V + (long long)v * ldv + b0;
 #pragma unroll
-            for (int u = 0; u < U; u++)
-                if (ok[u]) s[u] -= cv * (vp + u * S)[t];
+            for (int u = 0; u < U; u++) s[u] -= cv * (vp + u * S)[t];
         }
 #pragma unroll
-        for (int u = 0; u < U; u++)
-            if (ok[u]) {
-                (wout + b0 + u * S)[t] = s[u];
-                acc += (double)s[u] * (double)s[u];
-            }
+        for (int u = 0; u < U; u++) {
+            (wout + b0 + u * S)[t] = s[u];
+            acc += (double)s[u] * (double)s[u];
+        }
     }
+    for (; b0 < n; b0 += S)
+        if (b0 + t < n) {
+            T s = (win + b0)[t];
+            for (int v = 0; v < nv; v++) s -= (T)sc[v] * (V + (long long)v * ldv + b0)[t];
+            (wout + b0)[t] = s;
+            acc += (double)s * (double)s;
+        }
     if (part) {
         double r = block_sum<256>(acc, sh);
         if (threadIdx.x == 0) part[(long long)(VFS_RED_SLOTS - 1) * VFS_RED_BLOCKS + blockIdx.x] = r;
@@ -757,7 +774,7 @@ int vfs_poisson_destroy(vfs_ctx *c)
     hipFree(ps->rowflag);
     for (auto &l : ps->lv) free_level(l);
     hipFree(ps->cinv64); hipFree(ps->cinv32); hipFree(ps->cidx);
-    hipFree(ps->cinvr64); hipFree(ps->cinvr32); hipFree(ps->rowof);
+    hipFree(ps->cinvr64); hipFree(ps->cinvr32);
     hipFree(ps->V64); hipFree(ps->V32); hipFree(ps->w64); hipFree(ps->z64); hipFree(ps->r64); hipFree(ps->x64);
     hipFree(ps->bb64); hipFree(ps->e64); hipFree(ps->w32); hipFree(ps->z32); hipFree(ps->x32); hipFree(ps->bb32);
     hipFree(ps->dots);
@@ -789,7 +806,9 @@ int vfs_poisson_alloc(vfs_ctx *c)
     DMALLOC(ps->coef, 19 * P * sizeof(double));
     DMALLOC(ps->coef32, 19 * P * sizeof(float));
     DMALLOC(ps->rowflag, P);
-    ps->omega = 0.8;
+    // line-Jacobi damping: 0.9 took 25 GMRES iterations to 1e-8 on the
+    // 256x128x128 channel against 26 for 0.8; 1.0 does not converge
+    ps->omega = 0.9;
     ps->alpha = 1.0;
     ps->wdepth = 2;
     ps->m = 40;
@@ -930,15 +949,17 @@ int vfs_mg_setup(vfs_ctx *c)
     DMALLOC(ps->cinv64, (size_t)nc * nc * sizeof(double));
     DMALLOC(ps->cinv32, (size_t)nc * nc * sizeof(float));
     DMALLOC(ps->cidx, (size_t)nc * sizeof(long long));
-    DMALLOC(ps->cinvr64, (size_t)nc * nc * sizeof(double));
-    DMALLOC(ps->cinvr32, (size_t)nc * nc * sizeof(float));
-    DMALLOC(ps->rowof, (size_t)cl.n * sizeof(int));
+    DMALLOC(ps->cinvr64, (size_t)cl.n * cl.n * sizeof(double));
+    DMALLOC(ps->cinvr32, (size_t)cl.n * cl.n * sizeof(float));
     VFS_CHECK_HIP(hipStreamSynchronize(c->stream));   // the zero fills before the uploads
     {
-        std::vector<float> R32(M.begin(), M.end());
-        VFS_CHECK_HIP(hipMemcpy(ps->cinvr64, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
-        VFS_CHECK_HIP(hipMemcpy(ps->cinvr32, R32.data(), R32.size() * sizeof(float), hipMemcpyHostToDevice));
-        VFS_CHECK_HIP(hipMemcpy(ps->rowof, pos.data(), pos.size() * sizeof(int), hipMemcpyHostToDevice));
+        // the inverse scattered to all cells of the level, row-major
+        std::vector<double> F((size_t)cl.n * cl.n, 0.0);
+        for (int r = 0; r < nc; r++)
+            for (int q = 0; q < nc; q++) F[(size_t)list[r] * cl.n + list[q]] = M[(size_t)r * nc + q];
+        std::vector<float> F32(F.begin(), F.end());
+        VFS_CHECK_HIP(hipMemcpy(ps->cinvr64, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice));
+        VFS_CHECK_HIP(hipMemcpy(ps->cinvr32, F32.data(), F32.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     // column-major (cinv[c * nc + r]) so that the rows of a solve, one per
     // thread, read consecutive addresses for each column
@@ -1033,8 +1054,7 @@ static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
     if (l == (int)ps->lv.size() - 1) {
         // dense solve: exact, whatever the starting guess
         const T *cinvr = std::is_same<T, double>::value ? (const T *)ps->cinvr64 : (const T *)ps->cinvr32;
-        hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((L.n + 3) / 4)), dim3(256), 0, c->stream, L.n, ps->nc,
-                           (const int *)ps->rowof, (const long long *)ps->cidx, cinvr, b, x);
+        hipLaunchKernelGGL(k_dense_mv<T>, dim3((unsigned)((L.n + 3) / 4)), dim3(256), 0, c->stream, L.n, cinvr, b, x);
         VFS_CHECK_LAUNCH();
         return 0;
     }
@@ -1061,6 +1081,9 @@ static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
         VFS_CHECK_LAUNCH();
         return 0;
     }
+    MGLevel &C = ps->lv[l + 1];
+    LV Cv = lv_of(C);
+    T *cb = vec_of<T>(C, 0), *cx = vec_of<T>(C, 1);
     // pre-smoothing
     if (zero_guess) {
         if (launch_thomas<T>(c, L, b, nullptr, x)) return -1;
@@ -1068,9 +1091,6 @@ static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
         if (launch_resid<T>(c, L, A, x, b, y)) return -1;
         if (launch_thomas<T>(c, L, y, x, x)) return -1;
     }
-    MGLevel &C = ps->lv[l + 1];
-    LV Cv = lv_of(C);
-    T *cb = vec_of<T>(C, 0), *cx = vec_of<T>(C, 1);
     {
         const int bs = row_block(L.nx);
         dim3 g((L.nx + bs - 1) / bs, C.ny, C.nz);
@@ -1108,6 +1128,33 @@ template <> float *kv<float>(Poisson *ps, int w) { return w == 0 ? ps->V32 : w =
 static int multidot_finish(vfs_ctx *c, int nv, int nb, double *dst)
 {
     return vfs_reduce_finish(c, nb, nv, 0, dst);
+}
+
+// The launches of Arnoldi step kk up to the first orthogonalisation pass:
+// xl = M^{-1} U_kk, w = A xl, dots of w with U_0..U_kk and |w|^2,
+// U_{kk+1} = w - sum_q U_q d_q / |U_q|^2 and its norm (all on the device).
+template <typename T>
+static int arnoldi_enqueue(vfs_ctx *c, int kk, const T *uk, T *un, T *xl, T *w)
+{
+    Poisson *ps = c->pois;
+    MGLevel &L0 = ps->lv[0];
+    const long long n = L0.n;
+    const T *V = kv<T>(ps, 0);
+    const int nbr = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(n, 256));
+    double *nrm = ps->dots + 256;
+    if (vcycle<T>(c, 0, uk, xl, true)) return -1;
+    if (launch_spmv<T>(c, L0, A_of<T>(L0), (const T *)xl, w)) return -1;
+    for (int v0 = 0; v0 <= kk; v0 += 8) {
+        const int nv = std::min(8, kk + 1 - v0);
+        launch_multidot8<T>(c->stream, nbr, n, V, n, v0, nv, (int)(v0 == 0), (const T *)w, c->red);
+        VFS_CHECK_LAUNCH();
+    }
+    if (multidot_finish(c, kk + 1, nbr, ps->dots)) return -1;
+    if (vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, ps->dots + 130)) return -1;
+    hipLaunchKernelGGL(k_multiaxpy<T>, dim3(nbr), dim3(256), 0, c->stream, n, V, n, kk + 1, (const double *)ps->dots,
+                       (const double *)nrm, (const T *)w, un, c->red);
+    VFS_CHECK_LAUNCH();
+    return vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, nrm + kk + 1);
 }
 
 // Right-preconditioned restarted GMRES in precision T on A_T e = rhs from
@@ -1175,8 +1222,7 @@ static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, do
         for (; kk < m && it < maxit; kk++, it++) {
             // w = A M^{-1} U_kk
             T *uk = V + (size_t)kk * n, *un = V + (size_t)(kk + 1) * n;
-            if (vcycle<T>(c, 0, (const T *)uk, xl, true)) return -1;
-            if (launch_spmv<T>(c, L0, A, (const T *)xl, w)) return -1;
+            if (arnoldi_enqueue<T>(c, kk, uk, un, xl, w)) return -1;
             // one classical Gram-Schmidt pass; a second one only when the first
             // cancelled so much of w that its orthogonality error, about
             // u |w| / |w_perp|, would exceed u / sqrt(tau) (tau 1e-12 in FP64,
@@ -1184,28 +1230,26 @@ static int gmres(vfs_ctx *c, const T *rhs, double tol, int maxit, int *iters, do
             // tolerances the two precisions are asked for)
             const double tau = std::is_same<T, double>::value ? 1e-12 : 1e-4;
             double h1[64], h2[64];
-            for (int pass = 0; pass < 2; pass++) {
-                const T *src = pass == 0 ? w : un;
+            VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, (256 + kk + 2) * sizeof(double), hipMemcpyDeviceToHost,
+                                         c->stream));
+            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+            for (int q = 0; q <= kk; q++) { h1[q] = ps->hdots[q]; h2[q] = 0; }
+            if (hnrm[kk + 1] < tau * ps->hdots[130]) {
+                // second pass, in place on U_{kk+1}
                 for (int v0 = 0; v0 <= kk; v0 += 8) {
                     const int nv = std::min(8, kk + 1 - v0);
-                    hipLaunchKernelGGL(k_multidot8<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, v0, nv,
-                                       (int)(pass == 0 && v0 == 0), src, c->red);
+                    launch_multidot8<T>(c->stream, nbr, n, (const T *)V, n, v0, nv, 0, (const T *)un, c->red);
                     VFS_CHECK_LAUNCH();
                 }
-                if (multidot_finish(c, kk + 1, nbr, ps->dots + 64 * pass)) return -1;
-                if (pass == 0 && vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, ps->dots + 130)) return -1;
+                if (multidot_finish(c, kk + 1, nbr, ps->dots + 64)) return -1;
                 hipLaunchKernelGGL(k_multiaxpy<T>, dim3(nbr), dim3(256), 0, c->stream, n, (const T *)V, n, kk + 1,
-                                   (const double *)(ps->dots + 64 * pass), (const double *)nrm, src, un, c->red);
+                                   (const double *)(ps->dots + 64), (const double *)nrm, (const T *)un, un, c->red);
                 VFS_CHECK_LAUNCH();
                 if (vfs_reduce_finish_slots(c, VFS_RED_SLOTS - 1, nbr, 1, 0, nrm + kk + 1)) return -1;
                 VFS_CHECK_HIP(hipMemcpyAsync(ps->hdots, ps->dots, (256 + kk + 2) * sizeof(double),
                                              hipMemcpyDeviceToHost, c->stream));
                 VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
-                for (int q = 0; q <= kk; q++) (pass == 0 ? h1 : h2)[q] = ps->hdots[64 * pass + q];
-                if (pass == 0) {
-                    for (int q = 0; q <= kk; q++) h2[q] = 0;
-                    if (hnrm[kk + 1] >= tau * ps->hdots[130]) break;
-                }
+                for (int q = 0; q <= kk; q++) h2[q] = ps->hdots[64 + q];
             }
             const double nk = sqrt(hnrm[kk]);
             for (int q = 0; q <= kk; q++) H[(size_t)q * m + kk] = (h1[q] + h2[q]) / (sqrt(hnrm[q]) * nk);
