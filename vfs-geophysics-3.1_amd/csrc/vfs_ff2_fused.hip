@@ -89,44 +89,55 @@ struct LdsN {
     }
 };
 
-// global loads of plane k of the staged box into registers
-__device__ __forceinline__ void fetch_plane(const Dims &d, const double *__restrict__ ucat,
-                                            const double *__restrict__ nvert, int i0, int j0, int k, double (&ru)[NU],
-                                            float (&rn)[NN])
+// Per-thread offsets of the staged box, fixed over the k march: for each
+// ucat value a thread stages, its offset within a global plane (-1: outside
+// the grid) and within an LDS plane; likewise for nvert.
+struct StageMap {
+    int gu[NU], lu[NU], gn[NN];
+};
+
+__device__ __forceinline__ StageMap stage_map(const Dims &d, int i0, int j0)
 {
-    const bool kin = k >= 0 && k < d.mz;
+    StageMap m;
 #pragma unroll
     for (int n = 0; n < NU; n++) {
         const int e = threadIdx.x + n * NTB;
         const int r = e / (3 * BI), c = e - r * (3 * BI);
-        const int gi = i0 - 1 + c / 3, gj = j0 - 1 + r;
-        double v = 0;
-        if (kin && e < 3 * BP && gi < d.mx && gj < d.my) v = ucat[3 * IX(d, k, gj, i0 - 1) + c];
-        ru[n] = v;
+        const int p = c / 3, q = c - 3 * p;
+        const int gi = i0 - 1 + p, gj = j0 - 1 + r;
+        const bool ok = e < 3 * BP && gi < d.mx && gj < d.my;
+        m.gu[n] = ok ? 3 * (gj * d.mx + i0 - 1) + c : -1;
+        m.lu[n] = e < 3 * BP ? (q * BJ + r) * BI + p : -1;
     }
 #pragma unroll
     for (int n = 0; n < NN; n++) {
         const int e = threadIdx.x + n * NTB;
         const int r = e / BI, c = e - r * BI;
         const int gi = i0 - 1 + c, gj = j0 - 1 + r;
-        float v = 0;
-        if (kin && e < BP && gi < d.mx && gj < d.my) v = (float)nvert[IX(d, k, gj, gi)];
-        rn[n] = v;
+        m.gn[n] = e < BP && gi < d.mx && gj < d.my ? gj * d.mx + gi : -1;
     }
+    return m;
 }
 
-__device__ __forceinline__ void store_plane(int k, const double (&ru)[NU], const float (&rn)[NN])
+// global loads of plane k of the staged box into registers
+__device__ __forceinline__ void fetch_plane(const Dims &d, const double *__restrict__ ucat,
+                                            const double *__restrict__ nvert, const StageMap &m, int k,
+                                            double (&ru)[NU], float (&rn)[NN])
+{
+    const bool kin = k >= 0 && k < d.mz;
+    const unsigned pl = (unsigned)k * (unsigned)d.mx * (unsigned)d.my;
+#pragma unroll
+    for (int n = 0; n < NU; n++) ru[n] = kin && m.gu[n] >= 0 ? ld32(ucat, 3 * pl + (unsigned)m.gu[n]) : 0.0;
+#pragma unroll
+    for (int n = 0; n < NN; n++) rn[n] = kin && m.gn[n] >= 0 ? (float)ld32(nvert, pl + (unsigned)m.gn[n]) : 0.0f;
+}
+
+__device__ __forceinline__ void store_plane(int k, const StageMap &m, const double (&ru)[NU], const float (&rn)[NN])
 {
     const int slot = k & 3;
 #pragma unroll
-    for (int n = 0; n < NU; n++) {
-        const int e = threadIdx.x + n * NTB;
-        if (e < 3 * BP) {
-            const int r = e / (3 * BI), c = e - r * (3 * BI);
-            const int p = c / 3, q = c - 3 * p;
-            sU[((slot * 3 + q) * BJ + r) * BI + p] = ru[n];
-        }
-    }
+    for (int n = 0; n < NU; n++)
+        if (m.lu[n] >= 0) sU[slot * 3 * BP + m.lu[n]] = ru[n];
 #pragma unroll
     for (int n = 0; n < NN; n++) {
         const int e = threadIdx.x + n * NTB;
@@ -148,33 +159,37 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
     const LdsU U{i0, j0};
     const LdsN N{i0, j0};
     const long long P = d.P;
-    const long long t = IX(d, k, j, i);
+    const unsigned t = IX32(d, k, j, i);
     double D[3], V[3];
+    // slot s of the face array starts at face + s * P (a uniform pointer)
     if (!(d.ip && i == d.mx - 2)) {
-        face_flux_t<0, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 0}, U, nu_t, i, j, k, D, V);
+        face_flux_t<0, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 0}, U,
+                                                                                        nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) face[(0 * 3 + q) * P + t] = D[q];
-            if (PARTS & 2) face[(3 * 3 + q) * P + t] = V[q];
+            if (PARTS & 1) st32(face + (0 * 3 + q) * P, t, D[q]);
+            if (PARTS & 2) st32(face + (3 * 3 + q) * P, t, V[q]);
         }
     }
     // one face at a time: keeps the live registers of a single face
     __builtin_amdgcn_sched_barrier(0);
     if (!(d.jp && j == d.my - 2)) {
-        face_flux_t<1, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 1}, U, nu_t, i, j, k, D, V);
+        face_flux_t<1, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 1}, U,
+                                                                                        nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) face[(1 * 3 + q) * P + t] = D[q];
-            if (PARTS & 2) face[(4 * 3 + q) * P + t] = V[q];
+            if (PARTS & 1) st32(face + (1 * 3 + q) * P, t, D[q]);
+            if (PARTS & 2) st32(face + (4 * 3 + q) * P, t, V[q]);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
     if (!(d.kp && k == d.mz - 2)) {
-        face_flux_t<2, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO>(d, g, N, UConPlain{ucont, 2}, U, nu_t, i, j, k, D, V);
+        face_flux_t<2, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 2}, U,
+                                                                                        nu_t, i, j, k, D, V);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) face[(2 * 3 + q) * P + t] = D[q];
-            if (PARTS & 2) face[(5 * 3 + q) * P + t] = V[q];
+            if (PARTS & 1) st32(face + (2 * 3 + q) * P, t, D[q]);
+            if (PARTS & 2) st32(face + (5 * 3 + q) * P, t, V[q]);
         }
     }
 }
@@ -202,21 +217,22 @@ __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FA
 
     double ru[NU];
     float rn[NN];
+    const StageMap sm = stage_map(d, i0, j0);
     for (int k = kb - 1; k <= kb + 2; k++) {
-        fetch_plane(d, ucat, nvert, i0, j0, k, ru, rn);
-        store_plane(k, ru, rn);
+        fetch_plane(d, ucat, nvert, sm, k, ru, rn);
+        store_plane(k, sm, ru, rn);
     }
     __syncthreads();
     for (int k = kb; k < ke; k++) {
         // prefetch plane k+3 while plane k is computed
-        fetch_plane(d, ucat, nvert, i0, j0, k + 3, ru, rn);
+        fetch_plane(d, ucat, nvert, sm, k + 3, ru, rn);
         if (active) {
             if (!VFS_FACES_SPLIT) faces_at<3, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
             else if (grp == 0) faces_at<1, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
             else faces_at<2, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
         }
         __syncthreads();   // every read of plane k-1 is done
-        store_plane(k + 3, ru, rn);
+        store_plane(k + 3, sm, ru, rn);
         __syncthreads();
     }
 }
@@ -276,30 +292,31 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
     const int i = 1 + bx * DI + (int)(threadIdx.x % DI), j = 1 + by * DJ + (int)(threadIdx.x / DI);
     if (i > d.mx - 2 || j > d.my - 2) return;
     const int kb = 1 + bz * kchunk, ke = min(kb + kchunk, d.mz - 1);
-    const long long P = d.P, sk = (long long)d.mx * d.my;
-    const long long col = IX(d, 0, j, i);
-#define FI(kind, q, kk, jj, ii) face[(long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii)]
+    const long long P = d.P;
+    const unsigned sk = (unsigned)d.mx * (unsigned)d.my, col = IX32(d, 0, j, i);
+    // slot s of the face array starts at face + s * P (a uniform pointer)
+#define FI(kind, q, kk, jj, ii) ld32(face + (3 * (kind) + (q)) * P, IX32(d, kk, jj, ii))
     // ring: convective k-flux at planes k-2..k+1, viscous at k-1..k
     double c0[3], c1[3], c2[3], c3[3], v0[3], v1[3];
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        const double *D2 = face + (long long)(6 + q) * P + col, *V2 = face + (long long)(15 + q) * P + col;
-        c0[q] = kb - 2 >= 0 ? D2[(kb - 2) * sk] : 0.0;
-        c1[q] = D2[(kb - 1) * sk];
-        c2[q] = D2[kb * sk];
-        c3[q] = kb + 1 <= d.mz - 2 ? D2[(kb + 1) * sk] : 0.0;
-        v0[q] = V2[(kb - 1) * sk];
-        v1[q] = V2[kb * sk];
+        const double *D2 = face + (6 + q) * P, *V2 = face + (15 + q) * P;
+        c0[q] = kb - 2 >= 0 ? ld32(D2, col + (kb - 2) * sk) : 0.0;
+        c1[q] = ld32(D2, col + (kb - 1) * sk);
+        c2[q] = ld32(D2, col + kb * sk);
+        c3[q] = kb + 1 <= d.mz - 2 ? ld32(D2, col + (kb + 1) * sk) : 0.0;
+        v0[q] = ld32(V2, col + (kb - 1) * sk);
+        v1[q] = ld32(V2, col + kb * sk);
     }
     for (int k = kb; k < ke; k++) {
         // prefetch the planes the next step brings into the window
         double nc[3], nv[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            nc[q] = k + 2 <= d.mz - 2 ? face[(long long)(6 + q) * P + col + (k + 2) * sk] : 0.0;
-            nv[q] = face[(long long)(15 + q) * P + col + (k + 1) * sk];
+            nc[q] = k + 2 <= d.mz - 2 ? ld32(face + (6 + q) * P, col + (k + 2) * sk) : 0.0;
+            nv[q] = ld32(face + (15 + q) * P, col + (k + 1) * sk);
         }
-        const CellFaces cf = cell_faces(d, nvert, i, j, k);
+        const CellFaces cf = cell_faces<unsigned>(d, nvert, i, j, k);
         const int im = cf.im, jm = cf.jm, km = cf.km;
         double f[3];
 #pragma unroll
@@ -326,8 +343,8 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
                   K5(km)) / 1.0;
             f[q] = r;
         }
-        const long long t = IX(d, k, j, i);
-        fp[3 * t] = f[0]; fp[3 * t + 1] = f[1]; fp[3 * t + 2] = f[2];
+        const unsigned t = col + k * sk;
+        st32(fp, 3 * t, f[0]); st32(fp, 3 * t + 1, f[1]); st32(fp, 3 * t + 2, f[2]);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             c0[q] = c1[q]; c1[q] = c2[q]; c2[q] = c3[q]; c3[q] = nc[q];
@@ -466,7 +483,9 @@ __global__ void k_zero_shell_vec(Dims d, double *v)
 
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale)
 {
-    if (!c->nvert_f32) return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
+    // the staged kernels hold nvert as FP32 (exact for the integer masks) and
+    // address the arrays with 32-bit byte offsets
+    if (!c->nvert_f32 || !VFS_IX32_OK(c->d.P)) return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
     const Dims d = c->d;
     const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);

@@ -29,6 +29,33 @@ VFS_HOST_DEVICE long long IX(const Dims &d, int k, int j, int i)
     return ((long long)k * d.my + j) * (long long)d.mx + i;
 }
 
+// 32-bit indexing for the hot kernels: the point index in 32 bits, and array
+// elements read by (uniform base pointer) + 32-bit byte offset, which the
+// loads take as a scalar base and a vector offset with no 64-bit address
+// arithmetic.  Valid while every byte offset fits 32 bits (24 * P < 2^32,
+// checked on the host before such a kernel is launched).
+VFS_HOST_DEVICE unsigned IX32(const Dims &d, int k, int j, int i)
+{
+    return ((unsigned)k * (unsigned)d.my + (unsigned)j) * (unsigned)d.mx + (unsigned)i;
+}
+template <typename T>
+__device__ __forceinline__ T ld32(const T *p, unsigned e)
+{
+    return *(const T *)((const char *)p + e * (unsigned)sizeof(T));
+}
+template <typename T>
+__device__ __forceinline__ void st32(T *p, unsigned e, T v)
+{
+    *(T *)((char *)p + e * (unsigned)sizeof(T)) = v;
+}
+// element e of p with either index type
+template <typename T> __device__ __forceinline__ T ldx(const T *p, long long e) { return p[e]; }
+template <typename T> __device__ __forceinline__ T ldx(const T *p, unsigned e) { return ld32(p, e); }
+template <typename Idx> __device__ __forceinline__ Idx ixt(const Dims &d, int k, int j, int i);
+template <> __device__ __forceinline__ long long ixt<long long>(const Dims &d, int k, int j, int i) { return IX(d, k, j, i); }
+template <> __device__ __forceinline__ unsigned ixt<unsigned>(const Dims &d, int k, int j, int i) { return IX32(d, k, j, i); }
+#define VFS_IX32_OK(P) (24ull * (unsigned long long)(P) < (1ull << 32))
+
 // Periodic ghost source of point (k,j,i): the reference's `flag/a/b/c` idiom
 // (e.g. momentum.c:642-669).  Sources are never flagged themselves.
 VFS_HOST_DEVICE bool per_src(const Dims &d, int i, int j, int k, int &a, int &b, int &c)

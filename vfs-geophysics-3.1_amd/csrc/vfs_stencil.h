@@ -151,28 +151,32 @@ struct DualVec {
 // fetches the DIR component of ucont at a DA index.
 // Face metrics of the DIR face at DA index id, read from the stored arrays.
 struct StoredFaceMetric {
-    template <int DIR>
-    __device__ __forceinline__ void get(const Dims &, const Geom &g, long long id, double cs[3], double et[3],
+    template <int DIR, typename Idx>
+    __device__ __forceinline__ void get(const Dims &, const Geom &g, Idx id, double cs[3], double et[3],
                                         double ze[3], double &aj) const
     {
         const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
         const double *fe = DIR == 0 ? g.ieta : DIR == 1 ? g.jeta : g.keta;
         const double *fz = DIR == 0 ? g.izet : DIR == 1 ? g.jzet : g.kzet;
         const double *faj = DIR == 0 ? g.iaj : DIR == 1 ? g.jaj : g.kaj;
-        aj = faj[id];
+        aj = ldx(faj, id);
 #pragma unroll
-        for (int q = 0; q < 3; q++) { cs[q] = fc[3 * id + q]; et[q] = fe[3 * id + q]; ze[q] = fz[3 * id + q]; }
+        for (int q = 0; q < 3; q++) {
+            cs[q] = ldx(fc, (Idx)(3 * id + q)); et[q] = ldx(fe, (Idx)(3 * id + q)); ze[q] = ldx(fz, (Idx)(3 * id + q));
+        }
     }
 };
 
 // WENO: false compiles the WENO3 branch out (for kernels launched only when
 // the weno flag is off).
-template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric, bool WENO = true>
+// Idx: index type of the global arrays (unsigned: 32-bit offsets).
+template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric, bool WENO = true,
+          typename Idx = long long>
 __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
                                             const FC &UCon, const FU &U, const double *__restrict__ nu_t,
                                             int i, int j, int k, S D[3], S V[3], const FM &fm = FM())
 {
-    const long long id = IX(d, k, j, i);
+    const Idx id = ixt<Idx>(d, k, j, i);
     double mcs[3], met[3], mze[3], ajc;
     fm.template get<DIR>(d, g, id, mcs, met, mze, ajc);
     const double csi0 = mcs[0], csi1 = mcs[1], csi2 = mcs[2];
@@ -210,8 +214,8 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     const int n = DIR == 0 ? i : DIR == 1 ? j : k;
     const int m = DIR == 0 ? d.mx : DIR == 1 ? d.my : d.mz;
     const int per = DIR == 0 ? d.ip : DIR == 1 ? d.jp : d.kp;
-    const long long st = DIR == 0 ? 1 : DIR == 1 ? (long long)d.mx : (long long)d.mx * d.my;
-    const long long base = id - (long long)n * st;   // point with index 0 along DIR
+    const Idx st = DIR == 0 ? (Idx)1 : DIR == 1 ? (Idx)d.mx : (Idx)d.mx * (Idx)d.my;
+    const Idx base = id - (Idx)n * st;   // point with index 0 along DIR
 #define NB(tt) N(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i)
 #define UB(tt, q) U(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i, q)
     int L = n - 1, R = n + 2;
@@ -222,9 +226,9 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
         L = n, R = n + 1;
     }
     if (d.second_order) L = n, R = n + 1;
-    const S ucon_f = UCon(base + (long long)n * st);
+    const S ucon_f = UCon(id);
     S ucon = ucon_f;
-    if (per && (n == 0 || n == m - 2)) ucon = UCon(base + (long long)(m - 2) * st);
+    if (per && (n == 0 || n == m - 2)) ucon = UCon((Idx)(base + (Idx)(m - 2) * st));
     const S up = -0.5 * (ucon + fabs(ucon));
     const S um = -0.5 * (ucon - fabs(ucon));
     if (d.immersed && n != m - 2 && NB(n) > 0.1) {
@@ -253,9 +257,10 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     S v0, v1, v2;
     if (d.les) {
         double nut;
-        if ((n == 0 && !per) || NB(n) > 0.1) nut = nu_t[base + (long long)(n + 1) * st];
-        else if ((n == m - 2 && !per) || NB(n + 1) > 0.1) nut = nu_t[id];
-        else nut = 0.5 * (nu_t[id] + nu_t[base + (long long)(n + 1) * st]);
+        const Idx id1 = id + st;   // the point across the face
+        if ((n == 0 && !per) || NB(n) > 0.1) nut = ldx(nu_t, id1);
+        else if ((n == m - 2 && !per) || NB(n + 1) > 0.1) nut = ldx(nu_t, id);
+        else nut = 0.5 * (ldx(nu_t, id) + ldx(nu_t, id1));
         v0 = (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nut);
         v1 = (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nut);
         v2 = (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nut);
@@ -274,6 +279,7 @@ struct UConPlain {
     const double *__restrict__ p;
     int dir;
     __device__ __forceinline__ double operator()(long long o) const { return p[3 * o + dir]; }
+    __device__ __forceinline__ double operator()(unsigned o) const { return ld32(p, 3 * o + dir); }
 };
 struct UConDual {
     const double *__restrict__ p, *__restrict__ t;
@@ -301,10 +307,11 @@ struct CellFaces {
     double den_i, den_j, den_k;
 };
 
+template <typename Idx = long long>
 __device__ __forceinline__ CellFaces cell_faces(const Dims &d, const double *__restrict__ nvert, int i, int j, int k)
 {
     const int ip = d.ip, jp = d.jp, kp = d.kp, mx = d.mx, my = d.my, mz = d.mz;
-#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+#define NV(kk, jj, ii) ldx(nvert, ixt<Idx>(d, kk, jj, ii))
     CellFaces c;
     c.im = wrap_face(i - 1, mx, ip); c.jm = wrap_face(j - 1, my, jp); c.km = wrap_face(k - 1, mz, kp);
     int iR = i + 1, iL = i - 2, jR = j + 1, jL = j - 2, kR = k + 1, kL = k - 2;
@@ -384,22 +391,29 @@ __device__ __forceinline__ void project_rhs_v(const Dims &d, const Geom &g, cons
 
 // rhs += scale * (contravariant projection of fp) with the face masks
 // (momentum.c:1673-1693, 1795-1803)
+template <typename Idx = long long>
 __device__ __forceinline__ void project_rhs(const Dims &d, const Geom &g, const double *__restrict__ nvert,
-                                            const double *__restrict__ fp, int i, int j, int k, long long id,
+                                            const double *__restrict__ fp, int i, int j, int k, Idx id,
                                             double scale, double r[3])
 {
-    const long long a = IX(d, k, j, wrap_cell(i + 1, d.mx, d.ip)), b = IX(d, k, wrap_cell(j + 1, d.my, d.jp), i),
-                    c = IX(d, wrap_cell(k + 1, d.mz, d.kp), j, i);
-    const long long ga = id + 1, gb = id + d.mx, gc = id + (long long)d.mx * d.my;  // metric ghosts are stored
+    const Idx a = ixt<Idx>(d, k, j, wrap_cell(i + 1, d.mx, d.ip)), b = ixt<Idx>(d, k, wrap_cell(j + 1, d.my, d.jp), i),
+              c = ixt<Idx>(d, wrap_cell(k + 1, d.mz, d.kp), j, i);
+    const Idx sy = (Idx)d.mx, sz = (Idx)d.mx * (Idx)d.my;
+    const Idx ga = id + 1, gb = id + sy, gc = id + sz;  // metric ghosts are stored
     const double *cs = g.csi, *et = g.eta, *ze = g.zet;
-    r[0] += scale * (0.5 * (cs[3 * id] * fp[3 * id] + cs[3 * id + 1] * fp[3 * id + 1] + cs[3 * id + 2] * fp[3 * id + 2]) +
-                     0.5 * (cs[3 * ga] * fp[3 * a] + cs[3 * ga + 1] * fp[3 * a + 1] + cs[3 * ga + 2] * fp[3 * a + 2])) * g.iaj[id];
-    r[1] += scale * (0.5 * (et[3 * id] * fp[3 * id] + et[3 * id + 1] * fp[3 * id + 1] + et[3 * id + 2] * fp[3 * id + 2]) +
-                     0.5 * (et[3 * gb] * fp[3 * b] + et[3 * gb + 1] * fp[3 * b + 1] + et[3 * gb + 2] * fp[3 * b + 2])) * g.jaj[id];
-    r[2] += scale * (0.5 * (ze[3 * id] * fp[3 * id] + ze[3 * id + 1] * fp[3 * id + 1] + ze[3 * id + 2] * fp[3 * id + 2]) +
-                     0.5 * (ze[3 * gc] * fp[3 * c] + ze[3 * gc + 1] * fp[3 * c + 1] + ze[3 * gc + 2] * fp[3 * c + 2])) * g.kaj[id];
-    const double n0 = nvert[id];
-    if (n0 + nvert[id + 1] > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
-    if (n0 + nvert[id + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
-    if (n0 + nvert[id + (long long)d.mx * d.my] > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
+#define L3(p, o, q) ldx(p, (Idx)(3 * (o) + (q)))
+    r[0] += scale * (0.5 * (L3(cs, id, 0) * L3(fp, id, 0) + L3(cs, id, 1) * L3(fp, id, 1) + L3(cs, id, 2) * L3(fp, id, 2)) +
+                     0.5 * (L3(cs, ga, 0) * L3(fp, a, 0) + L3(cs, ga, 1) * L3(fp, a, 1) + L3(cs, ga, 2) * L3(fp, a, 2))) *
+            ldx(g.iaj, id);
+    r[1] += scale * (0.5 * (L3(et, id, 0) * L3(fp, id, 0) + L3(et, id, 1) * L3(fp, id, 1) + L3(et, id, 2) * L3(fp, id, 2)) +
+                     0.5 * (L3(et, gb, 0) * L3(fp, b, 0) + L3(et, gb, 1) * L3(fp, b, 1) + L3(et, gb, 2) * L3(fp, b, 2))) *
+            ldx(g.jaj, id);
+    r[2] += scale * (0.5 * (L3(ze, id, 0) * L3(fp, id, 0) + L3(ze, id, 1) * L3(fp, id, 1) + L3(ze, id, 2) * L3(fp, id, 2)) +
+                     0.5 * (L3(ze, gc, 0) * L3(fp, c, 0) + L3(ze, gc, 1) * L3(fp, c, 1) + L3(ze, gc, 2) * L3(fp, c, 2))) *
+            ldx(g.kaj, id);
+#undef L3
+    const double n0 = ldx(nvert, id);
+    if (n0 + ldx(nvert, (Idx)(id + 1)) > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
+    if (n0 + ldx(nvert, (Idx)(id + sy)) > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
+    if (n0 + ldx(nvert, (Idx)(id + sz)) > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
 }
