@@ -169,6 +169,31 @@ int vfs_projection(vfs_ctx *ctx, const double *phi, double *ucont);
  * div (may be NULL) receives the per-cell field. */
 int vfs_divergence(vfs_ctx *ctx, const double *ucont, double *div, double *maxdiv);
 
+/* ---- reference file formats (host I/O) ------------------------------------
+ * Replaces the grid reading of MG_Initial (init.c:316-436): format 0 text
+ * grid.dat, 1 binary grid.dat (-binary 1), 2 tensor-product xyz.dat (-xyz 1).
+ * Single-block files only.  Node coordinates value / cl * L_dim
+ * (cl = -chact_leng, init.c:289-291) are written to coor in the layout
+ * vfs_build_metrics takes: (IM+1)*(JM+1)*(KM+1) Cmpnts, node (k,j,i) at
+ * ((k*(JM+1) + j)*(IM+1) + i); the other points are left untouched.
+ * coor == NULL returns only dims = {IM, JM, KM}.  Host memory. */
+int vfs_read_grid(const char *path, int format, double cl, double L_dim, int dims[3], double *coor);
+/* PETSc binary vector files (big-endian class id 1211214, int32 length,
+ * float64 values): what VecView / VecLoadIntoVector write and read for the
+ * restart files (main.c:394-420, 799-847).  Host memory; reading requires the
+ * file's length to equal n. */
+int vfs_vec_write_petsc(const char *path, const double *v, long long n);
+int vfs_vec_read_petsc(const char *path, double *v, long long n);
+/* The restart set "<dir>/{vfield,ufield,pfield,nvfield}%06d_%1d.dat" of one
+ * block at step ti: Ucont_P_Binary_Output (main.c:799-847, without the
+ * optional averaging and Q fields) and Ucont_P_Binary_Input
+ * (main.c:394-420, which loads nvfield into Nvert_o).  Device arrays. */
+int vfs_restart_write(vfs_ctx *ctx, const char *dir, int ti, int block, const double *ucont,
+                      const double *ucat, const double *p, const double *nvert);
+int vfs_restart_read(vfs_ctx *ctx, const char *dir, int ti, int block, double *ucont, double *ucat,
+                     double *p, double *nvert_o);
+int vfs_restart_path(char *out, size_t cap, const char *dir, const char *stem, int ti, int block);
+
 /* ---- synthetic workloads (host-side generators) -------------------------- */
 double vfs_wl_uniform(uint64_t seed, uint64_t counter);
 int vfs_wl_channel_grid(int IM, int JM, int KM, double Lx, double Ly, double Lz,

@@ -2,6 +2,7 @@
 // second pass of the deterministic reductions.
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "vfs_internal.h"
 #include "vfs_reduce.h"
@@ -221,5 +222,57 @@ extern "C" int vfs_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 extern "C" int vfs_memcpy_dtod(void *dst, const void *src, size_t bytes)
 {
     VFS_CHECK_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// restart files (host I/O in vfs_io.c)
+// ---------------------------------------------------------------------------
+extern "C" void vfs_io_set_error(const char *msg) { vfs_set_error(msg); }
+extern "C" int vfs_vec_write_petsc(const char *path, const double *v, long long n);
+extern "C" int vfs_vec_read_petsc(const char *path, double *v, long long n);
+extern "C" int vfs_restart_path(char *out, size_t cap, const char *dir, const char *stem, int ti, int block);
+
+// Ucont_P_Binary_Output (main.c:799-847) without the optional averaging and
+// Q-criterion fields: vfield (ucont), ufield (ucat), pfield (p) and nvfield
+// (nvert) of one block at step ti, from device arrays.
+extern "C" int vfs_restart_write(vfs_ctx *c, const char *dir, int ti, int block, const double *ucont,
+                                 const double *ucat, const double *p, const double *nvert)
+{
+    VFS_REQUIRE(c && dir && ucont && ucat && p && nvert, "null argument");
+    const long long P = c->d.P;
+    std::vector<double> h((size_t)(3 * P));
+    char path[1024];
+    const struct { const char *stem; const double *src; int dof; } f[4] = {
+        {"vfield", ucont, 3}, {"ufield", ucat, 3}, {"pfield", p, 1}, {"nvfield", nvert, 1}};
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 4; q++) {
+        const long long n = f[q].dof * P;
+        VFS_CHECK_HIP(hipMemcpy(h.data(), f[q].src, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        if (vfs_restart_path(path, sizeof path, dir, f[q].stem, ti, block)) return -2;
+        if (vfs_vec_write_petsc(path, h.data(), n)) return -3;
+    }
+    return 0;
+}
+
+// Ucont_P_Binary_Input (main.c:394-420): vfield -> ucont, pfield -> p,
+// nvfield -> nvert_o (the reference loads it into Nvert_o), ufield -> ucat,
+// into device arrays.
+extern "C" int vfs_restart_read(vfs_ctx *c, const char *dir, int ti, int block, double *ucont, double *ucat,
+                                double *p, double *nvert_o)
+{
+    VFS_REQUIRE(c && dir && ucont && ucat && p && nvert_o, "null argument");
+    const long long P = c->d.P;
+    std::vector<double> h((size_t)(3 * P));
+    char path[1024];
+    const struct { const char *stem; double *dst; int dof; } f[4] = {
+        {"vfield", ucont, 3}, {"pfield", p, 1}, {"nvfield", nvert_o, 1}, {"ufield", ucat, 3}};
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    for (int q = 0; q < 4; q++) {
+        const long long n = f[q].dof * P;
+        if (vfs_restart_path(path, sizeof path, dir, f[q].stem, ti, block)) return -2;
+        if (vfs_vec_read_petsc(path, h.data(), n)) return -3;
+        VFS_CHECK_HIP(hipMemcpy(f[q].dst, h.data(), (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+    }
     return 0;
 }

@@ -261,12 +261,61 @@ class Ctx:
         _ck(lib().vfs_divergence(self.h, ucont.ptr, _p(div), C.byref(m)), "vfs_divergence")
         return m.value
 
+    def restart_write(self, directory, ti, block, ucont, ucat, p, nvert):
+        """Ucont_P_Binary_Output (main.c:799-847): the vfield/ufield/pfield/
+        nvfield restart files of block `block` at step `ti`."""
+        _ck(lib().vfs_restart_write(self.h, str(directory).encode(), int(ti), int(block), ucont.ptr, ucat.ptr, p.ptr,
+                                    nvert.ptr), "vfs_restart_write")
+
+    def restart_read(self, directory, ti, block, ucont, ucat, p, nvert_o):
+        """Ucont_P_Binary_Input (main.c:394-420); nvfield goes to nvert_o."""
+        _ck(lib().vfs_restart_read(self.h, str(directory).encode(), int(ti), int(block), ucont.ptr, ucat.ptr, p.ptr,
+                                   nvert_o.ptr), "vfs_restart_read")
+
 
 # ---------------------------------------------------------------------------
 # synthetic workloads (host generators compiled into the library)
 # ---------------------------------------------------------------------------
 def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+# ---------------------------------------------------------------------------
+# the reference's file formats (host I/O in the library)
+# ---------------------------------------------------------------------------
+GRID_TEXT, GRID_BINARY, GRID_XYZ = 0, 1, 2
+
+
+def read_grid(path, fmt=GRID_TEXT, cl=1.0, L_dim=1.0):
+    """Grid file of MG_Initial (init.c:316-436) -> ((IM, JM, KM), coor) with
+    coor in the layout Ctx.build_metrics takes ((IM+1)(JM+1)(KM+1) Cmpnts)."""
+    dims = (C.c_int * 3)()
+    args = (str(path).encode(), int(fmt), C.c_double(cl), C.c_double(L_dim), dims)
+    _ck(lib().vfs_read_grid(*args, None), "vfs_read_grid")
+    IM, JM, KM = dims[0], dims[1], dims[2]
+    coor = np.zeros(3 * (IM + 1) * (JM + 1) * (KM + 1))
+    _ck(lib().vfs_read_grid(*args, _dp(coor)), "vfs_read_grid")
+    return (IM, JM, KM), coor
+
+
+def write_petsc_vec(path, a):
+    """A host array as a PETSc binary vector file."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    _ck(lib().vfs_vec_write_petsc(str(path).encode(), _dp(a), C.c_longlong(a.size)), "vfs_vec_write_petsc")
+
+
+def read_petsc_vec(path, n):
+    """A PETSc binary vector file of exactly n values."""
+    a = np.zeros(int(n))
+    _ck(lib().vfs_vec_read_petsc(str(path).encode(), _dp(a), C.c_longlong(a.size)), "vfs_vec_read_petsc")
+    return a
+
+
+def restart_path(directory, stem, ti, block):
+    buf = C.create_string_buffer(1024)
+    _ck(lib().vfs_restart_path(buf, C.c_size_t(1024), str(directory).encode(), stem.encode(), int(ti), int(block)),
+        "vfs_restart_path")
+    return buf.value.decode()
 
 
 def channel_grid(IM, JM, KM, Lx, Ly, Lz, beta=0.0, amp=0.0):
