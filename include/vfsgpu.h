@@ -31,7 +31,9 @@ typedef struct vfs_ctx vfs_ctx;
 
 /* Run-time flags of the reference that the hot path reads (main.c:1602-2110,
  * init.c:52-69, bcs.dat via init.c:561-566).  bctype: 1 = no-slip wall,
- * 7 = periodic (the matching *_periodic flag must be set). */
+ * 7 = periodic (the matching *_periodic flag must be set), -1 = wall
+ * function (Cabot wall model, wallfunction.c) on an i or j wall: without an
+ * immersed body its first cells become IB nodes (momentum.c:2014-2039). */
 typedef struct {
     int mx, my, mz;
     int i_periodic, j_periodic, k_periodic;
@@ -80,6 +82,10 @@ int vfs_nvert_sphere(vfs_ctx *ctx, double cx, double cy, double cz, double r, do
  * ucont: in/out (periodic ghosts refreshed), ucat: in/out (values at cells
  * with nvert >= 0.1 are inputs and are kept, solid cells are zeroed). */
 int vfs_contra2cart(vfs_ctx *ctx, double *ucont, double *ucat);
+/* The friction velocity user->lUstar that Contra2Cart_2 writes on the first
+ * cells of wall-function walls (rhs.c:303-401; zero elsewhere).  dst: P
+ * doubles on the device.  Only for contexts with a bctype -1 wall. */
+int vfs_get_ustar(vfs_ctx *ctx, double *dst);
 /* Compute_eddy_viscosity_LES (les.c:1135-1252) with constant Cs
  * (Compute_Smagorinsky_Constant_1, les.c:67-77: Cs = 0.01 for -les 1). */
 int vfs_eddy_viscosity(vfs_ctx *ctx, const double *ucat, double cs, double *nu_t);

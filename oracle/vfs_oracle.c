@@ -73,7 +73,7 @@ static int check_cfg(const vo_cfg *c)
     for (int f = 0; f < 6; f++) {
         int bt = c->bctype[f];
         if (per[f / 2]) { if (bt != 7) return -3; }
-        else if (bt != 1) return -3;
+        else if (bt != 1 && !(bt == -1 && f < 4)) return -3;   /* -1: wall function (i/j walls) */
     }
     if (c->immersed != 0 && c->immersed != 1) return -4;
     if (c->weno != 0 && c->weno != 1) return -4;
@@ -278,8 +278,189 @@ int vo_nvert_sphere(const vo_cfg *c, const vo_geom *g, double cx, double cy,
 /* ------------------------------------------------------------------------ */
 /* rhs.c:57-741 Contra2Cart_2 (bctype 1 / periodic subset)                  */
 /* ------------------------------------------------------------------------ */
+/* ------------------------------------------------------------------------ */
+/* wallfunction.c: the Cabot wall model behind bctype -1                      */
+/* ------------------------------------------------------------------------ */
+#define WF_KAPPA 0.41          /* wallfunction.c:15 */
+#define WF_INTERVAL 2          /* interval_yp, wallfunction.c:288 */
+#define WF_MAX_YP 1000000      /* max_yp */
+#define WF_NYP (WF_MAX_YP / WF_INTERVAL)
+
+/* wallfunction.c:271-274: nu_t / nu; pow(x, 2.0) is x * x at -O3 */
+static double wf_nut(double yplus) { return WF_KAPPA * yplus * sq(1. - exp(-yplus / 19.)); }
+
+/* wallfunction.c:291-323 pre_integrate: cumulative integral of
+ * 1 / (1 + nu_t/nu) over y+ in steps of 2, each by 24 Simpson 3/8 panels */
+static double *wf_table = NULL;
+const double *vo_wf_table(void)
+{
+    if (wf_table) return wf_table;
+    double *b = (double *)malloc(sizeof(double) * (WF_NYP + 1));
+    if (!b) return NULL;
+    b[0] = 0.;
+    double E[25];
+    for (int i = 1; i <= WF_NYP; i++) {
+        const int N = 24;
+        double ya_p = (double)(i - 1) * WF_INTERVAL;
+        double yb_p = (double)(i) * WF_INTERVAL;
+        double ydiff = yb_p - ya_p, dy_p = ydiff / (double)N;
+        double val = 0, ybegin_p = ya_p;
+        for (int k = 0; k <= N; k++) E[k] = 1. / (1. + wf_nut(ybegin_p + dy_p * k));
+        for (int k = 0; k < N; k++) {
+            double F[4];
+            F[0] = E[k];
+            F[1] = 1. / (1. + wf_nut(ybegin_p + dy_p * 1. / 3.));
+            F[2] = 1. / (1. + wf_nut(ybegin_p + dy_p * 2. / 3.));
+            F[3] = E[k + 1];
+            val += dy_p / 3. * (3 * F[0] + 9 * F[1] + 9 * F[2] + 3 * F[3]) / 8.;
+            ybegin_p += dy_p;
+        }
+        b[i] = b[i - 1] + val;
+    }
+    wf_table = b;
+    return b;
+}
+
+/* wallfunction.c:325-371 integrate_F: the table interpolated linearly up to
+ * y+ = 1e6, four Simpson 3/8 panels beyond.  The table index is clamped to
+ * the table (the reference reads outside it for y+ < 0). */
+double vo_wf_integrate_F(double nu, double utau, double yb)
+{
+    const double *buf = vo_wf_table();
+    double val = 0;
+    double ya_plus = 0 * utau / nu;
+    double yb_plus = yb * utau / nu;
+    int ib = (int)(yb_plus / (double)WF_INTERVAL);
+    const int N = 4;
+    if (yb_plus <= (double)WF_MAX_YP) {
+        if (ib < 0) ib = 0;
+        if (ib > WF_NYP - 1) ib = WF_NYP - 1;
+        double int_a = 0;
+        double int_b = (buf[ib + 1] - buf[ib]) / (double)(WF_INTERVAL) * (yb_plus - (double)(ib)*WF_INTERVAL) + buf[ib];
+        return (int_b - int_a) / utau;
+    }
+    val = buf[WF_NYP];
+    ya_plus = WF_MAX_YP;
+    double ydiff = yb_plus - ya_plus, dy = ydiff / (double)N;
+    double E[5];
+    double ybegin = ya_plus;
+    for (int i = 0; i <= N; i++) E[i] = 1. / (1. + wf_nut(ybegin + dy * i));
+    for (int i = 0; i < N; i++) {
+        double F[4];
+        F[0] = E[i];
+        F[1] = 1. / (1. + wf_nut(ybegin + dy * 1. / 3.));
+        F[2] = 1. / (1. + wf_nut(ybegin + dy * 2. / 3.));
+        F[3] = E[i + 1];
+        val += dy / 3. * (3 * F[0] + 9 * F[1] + 9 * F[2] + 3 * F[3]) / 8.;
+        ybegin += dy;
+    }
+    val /= utau;
+    return val;
+}
+
+/* wallfunction.c:211-214, 234-237, 252-256, 389-404 (dpdn = 0) */
+static double wf_u_cabot(double nu, double y, double utau) { return utau * utau * vo_wf_integrate_F(nu, utau, y); }
+static double wf_f_cabot(double nu, double u, double y, double utau)
+{
+    return utau * utau * vo_wf_integrate_F(nu, utau, y) - u;
+}
+static double wf_df_cabot(double nu, double u, double y, double utau)
+{
+    double eps = 1.e-7;
+    return (wf_f_cabot(nu, u, y, utau + eps) - wf_f_cabot(nu, u, y, utau - eps)) / (2 * eps);
+}
+double vo_wf_find_utau(double nu, double u, double y, double guess)
+{
+    double x = guess, x0 = guess;
+    for (int i = 0; i < 30; i++) {
+        x = x0 - wf_f_cabot(nu, u, y, x0) / wf_df_cabot(nu, u, y, x0);
+        if (fabs(x0 - x) < 1.e-10) break;
+        x0 = x;
+    }
+    return x;
+}
+
+/* wallfunction.c:18-43 wall_function with Ua = 0: the tangential velocity of
+ * the first cell from the wall model, the normal part scaled by sb / sc */
+void vo_wall_function(double nu, double sc, double sb, const double Uc[3], double Ub[3], double *ustar,
+                      double nx, double ny, double nz)
+{
+    const double Ua[3] = {0, 0, 0};
+    double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
+    double un = u_c * nx + v_c * ny + w_c * nz;
+    double ut = u_c - un * nx, vt = v_c - un * ny, wt = w_c - un * nz;
+    double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+    *ustar = vo_wf_find_utau(nu, ut_mag, sc, 0.01);
+    double ut_mag_modeled = wf_u_cabot(nu, sb, *ustar);
+    if (ut_mag > 1.e-10) {
+        ut *= ut_mag_modeled / ut_mag;
+        vt *= ut_mag_modeled / ut_mag;
+        wt *= ut_mag_modeled / ut_mag;
+    } else ut = vt = wt = 0;
+    Ub[0] = ut + sb / sc * un * nx;
+    Ub[1] = vt + sb / sc * un * ny;
+    Ub[2] = wt + sb / sc * un * nz;
+    Ub[0] += Ua[0];
+    Ub[1] += Ua[1];
+    Ub[2] += Ua[2];
+}
+
+/* rhs2.c:539-583 Calculate_Covariant_metrics + Calculate_normal: the unit
+ * covariant base vectors (columns of the inverse metric matrix) */
+static void calc_normal(const double *csi, const double *eta, const double *zet, double ni[3], double nj[3],
+                        double nk[3])
+{
+    const double a11 = csi[0], a12 = csi[1], a13 = csi[2];
+    const double a21 = eta[0], a22 = eta[1], a23 = eta[2];
+    const double a31 = zet[0], a32 = zet[1], a33 = zet[2];
+    double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    double G00 = (a33 * a22 - a32 * a23) / det, G01 = -(a33 * a12 - a32 * a13) / det, G02 = (a23 * a12 - a22 * a13) / det;
+    double G10 = -(a33 * a21 - a31 * a23) / det, G11 = (a33 * a11 - a31 * a13) / det, G12 = -(a23 * a11 - a21 * a13) / det;
+    double G20 = (a32 * a21 - a31 * a22) / det, G21 = -(a32 * a11 - a31 * a12) / det, G22 = (a22 * a11 - a21 * a12) / det;
+    double nx_i = G00, ny_i = G10, nz_i = G20;
+    double nx_j = G01, ny_j = G11, nz_j = G21;
+    double nx_k = G02, ny_k = G12, nz_k = G22;
+    double sum_i = sqrt(nx_i * nx_i + ny_i * ny_i + nz_i * nz_i);
+    double sum_j = sqrt(nx_j * nx_j + ny_j * ny_j + nz_j * nz_j);
+    double sum_k = sqrt(nx_k * nx_k + ny_k * ny_k + nz_k * nz_k);
+    nx_i /= sum_i, ny_i /= sum_i, nz_i /= sum_i;
+    nx_j /= sum_j, ny_j /= sum_j, nz_j /= sum_j;
+    nx_k /= sum_k, ny_k /= sum_k, nz_k /= sum_k;
+    ni[0] = nx_i, ni[1] = ny_i, ni[2] = nz_i;
+    nj[0] = nx_j, nj[1] = ny_j, nj[2] = nz_j;
+    nk[0] = nx_k, nk[1] = ny_k, nk[2] = nz_k;
+}
+
+/* momentum.c:2014-2039: with no immersed body, the first cells of the
+ * wall-function walls become IB nodes (nvert = 1) */
+int vo_wall_mask(const vo_cfg *c, double *nvert)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    if (c->immersed) return 0;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (c->bctype[0] == -1 && i == 1) nvert[id] = 1;
+                if (c->bctype[1] == -1 && i == mx - 2) nvert[id] = 1;
+                if (c->bctype[2] == -1 && j == 1) nvert[id] = 1;
+                if (c->bctype[3] == -1 && j == my - 2) nvert[id] = 1;
+                if (c->bctype[4] == -1 && k == 1) nvert[id] = 1;
+                if (c->bctype[5] == -1 && k == mz - 2) nvert[id] = 1;
+            }
+    return 0;
+}
+
 int vo_contra2cart(const vo_cfg *c, const vo_geom *g, const double *nvert,
                    double *ucont, double *ucat)
+{
+    return vo_contra2cart_ustar(c, g, nvert, ucont, ucat, NULL);
+}
+
+int vo_contra2cart_ustar(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         double *ucont, double *ucat, double *ustar)
 {
     DIMS;
     int rc = check_cfg(c);
@@ -319,6 +500,28 @@ int vo_contra2cart(const vo_cfg *c, const vo_geom *g, const double *nvert,
             for (int i = 0; i < mx; i++) {
                 size_t id = IX(k, j, i);
                 if ((int)(nvert[id] + 0.1) == 3) { VX(ucat, id) = VY(ucat, id) = VZ(ucat, id) = 0; continue; }
+                /* wall function for boundary (rhs.c:303-333, 369-401) */
+                for (int dir = 0; dir < 2; dir++) {
+                    const int n = dir == 0 ? i : j, m = dir == 0 ? mx : my;
+                    const int t1 = dir == 0 ? j : i, m1 = dir == 0 ? my : mx;
+                    if (!(t1 != 0 && t1 != m1 - 1 && k != 0 && k != mz - 1 &&
+                          ((c->bctype[2 * dir] == -1 && n == 1) || (c->bctype[2 * dir + 1] == -1 && n == m - 2))))
+                        continue;
+                    const double *a = (dir == 0 ? g->csi : g->eta) + 3 * id;
+                    double area = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+                    double sb, sc, ni[3], nj[3], nk[3], Ub[3], us;
+                    sb = 0.5 / g->aj[id] / area;
+                    const size_t nb = dir == 0 ? (n == 1 ? IX(k, j, i + 1) : IX(k, j, i - 1))
+                                               : (n == 1 ? IX(k, j + 1, i) : IX(k, j - 1, i));
+                    sc = 2 * sb + 0.5 / g->aj[nb] / area;
+                    const double Uc[3] = {VX(L, nb), VY(L, nb), VZ(L, nb)};
+                    calc_normal(g->csi + 3 * id, g->eta + 3 * id, g->zet + 3 * id, ni, nj, nk);
+                    double *nv = dir == 0 ? ni : nj;
+                    if (n == m - 2) nv[0] *= -1, nv[1] *= -1, nv[2] *= -1;
+                    vo_wall_function(1. / c->ren, sc, sb, Uc, Ub, &us, nv[0], nv[1], nv[2]);
+                    if (ustar) ustar[id] = us;
+                    VX(ucat, id) = Ub[0]; VY(ucat, id) = Ub[1]; VZ(ucat, id) = Ub[2];
+                }
                 size_t s = (size_t)-1;
                 if (i == 0 && c->bctype[0] == 1 && (j != 0 && k != 0)) s = IX(k, j, i + 1);
                 if (i == mx - 1 && c->bctype[1] == 1 && (j != 0 && k != 0)) s = IX(k, j, i - 1);
@@ -613,6 +816,19 @@ int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert, double *uco
                     }
                 }
     }
+    /* wall-function walls: the wall face flux is zero (momentum.c:2135-2153) */
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                if ((c->bctype[0] == -1 && i == 1) || (c->bctype[1] == -1 && i == mx - 2)) {
+                    if (i == 1) VX(ucont, IX(k, j, i - 1)) = 0;
+                    else VX(ucont, IX(k, j, i)) = 0;
+                }
+                if ((c->bctype[2] == -1 && j == 1) || (c->bctype[3] == -1 && j == my - 2)) {
+                    if (j == 1) VY(ucont, IX(k, j - 1, i)) = 0;
+                    else VY(ucont, IX(k, j, i)) = 0;
+                }
+            }
     /* periodic face copies, momentum.c:2167-2181 */
     for (int k = 0; k < mz; k++)
         for (int j = 0; j < my; j++)

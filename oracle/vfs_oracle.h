@@ -56,6 +56,24 @@ int vo_nvert_sphere(const vo_cfg *c, const vo_geom *g, double cx, double cy,
 int vo_contra2cart(const vo_cfg *c, const vo_geom *g, const double *nvert,
                    double *ucont, double *ucat);
 
+/* The same, also writing the friction velocity of the wall-function cells
+ * (bctype -1, rhs.c:303-401) into ustar (user->lUstar); other points untouched. */
+int vo_contra2cart_ustar(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         double *ucont, double *ucat, double *ustar);
+
+/* wallfunction.c: the Cabot wall model of bctype -1.  The y+ integral table
+ * of pre_integrate (291-323), integrate_F (325-371), the Newton solve of
+ * find_utau_Cabot (389-404) and wall_function (18-43) with Ua = 0. */
+const double *vo_wf_table(void);
+double vo_wf_integrate_F(double nu, double utau, double yb);
+double vo_wf_find_utau(double nu, double u, double y, double guess);
+void vo_wall_function(double nu, double sc, double sb, const double Uc[3], double Ub[3], double *ustar,
+                      double nx, double ny, double nz);
+
+/* momentum.c:2014-2039: without an immersed body, the first cells of the
+ * wall-function walls become IB nodes (nvert = 1). */
+int vo_wall_mask(const vo_cfg *c, double *nvert);
+
 /* les.c:1135-1252 Compute_eddy_viscosity_LES with a constant Cs (les.c:67-77). */
 int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
                       const double *ucat, double cs, double *nu_t);

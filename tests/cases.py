@@ -8,6 +8,8 @@ from vfsgpu import workloads as W
 SMALL = {
     "C1": (16, 12, 14),
     "C2": (32, 24, 20),
+    "C1w": (16, 12, 14),
+    "C2w": (32, 24, 20),
 }
 
 
@@ -17,6 +19,7 @@ def oracle_case(name, N=None, dt=1e-3):
     o = Oracle(cfg, coor)
     if spec["immersed"]:
         o.nvert_sphere(*W.SPHERE)
+    o.wall_mask()   # IB nodes on wall-function walls (no-op unless bctype -1 and no body)
     ucat, ucont = W.fields(cfg, spec, o.g, o.nvert)
     return cfg, spec, coor, o, ucat, ucont
 

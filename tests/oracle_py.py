@@ -48,8 +48,11 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).restype = d
+        for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
+            getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
+        _lib.vo_wf_table.restype = C.POINTER(d)
     return _lib
 
 
@@ -110,6 +113,13 @@ class Oracle:
 
     def contra2cart(self, ucont, ucat):
         self._call("vo_contra2cart", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat))
+
+    def contra2cart_ustar(self, ucont, ucat, ustar):
+        self._call("vo_contra2cart_ustar", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat), ptr(ustar))
+
+    def wall_mask(self):
+        self._call("vo_wall_mask", ptr(self.nvert))
+        return self.nvert
 
     def eddy_viscosity(self, ucat, cs, nu_t):
         self._call("vo_eddy_viscosity", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.c_double(cs), ptr(nu_t))
@@ -189,3 +199,29 @@ class Oracle:
         div = self.sca()
         m = lib().vo_divergence(C.byref(self.c), C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(div))
         return m, div
+
+
+# the Cabot wall model of the wall-function walls (wallfunction.c)
+WF_NYP = 500000
+
+
+def wf_table():
+    return np.ctypeslib.as_array(lib().vo_wf_table(), shape=(WF_NYP + 1,)).copy()
+
+
+def wf_integrate_F(nu, utau, yb):
+    return lib().vo_wf_integrate_F(nu, utau, yb)
+
+
+def wf_find_utau(nu, u, y, guess=0.01):
+    return lib().vo_wf_find_utau(nu, u, y, guess)
+
+
+def wall_function(nu, sc, sb, Uc, n):
+    """-> (Ub, ustar) of wall_function with Ua = 0."""
+    Uc = np.ascontiguousarray(Uc, dtype=np.float64)
+    Ub = np.zeros(3)
+    us = C.c_double()
+    d = C.c_double
+    lib().vo_wall_function(d(nu), d(sc), d(sb), ptr(Uc), ptr(Ub), C.byref(us), d(n[0]), d(n[1]), d(n[2]))
+    return Ub, us.value

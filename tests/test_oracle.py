@@ -166,6 +166,98 @@ def test_contra2cart_noslip_ghosts_and_linearity():
 
 
 # ---------------------------------------------------------------------------
+# wall-function walls (bctype -1): the Cabot wall model of wallfunction.c
+# ---------------------------------------------------------------------------
+def _wf_integrand(yp):
+    """1 / (1 + nu_t/nu), nu_t/nu = kappa y+ (1 - exp(-y+/19))^2, kappa 0.41."""
+    return 1.0 / (1.0 + 0.41 * yp * (1.0 - math.exp(-yp / 19.0)) ** 2)
+
+
+def test_wall_table_is_the_mixing_length_integral():
+    """table[n] = integral of 1/(1 + nu_t/nu) over y+ in [0, 2n], against
+    adaptive quadrature; in the log region the integral has the closed form
+    ln((1 + kappa b) / (1 + kappa a)) / kappa."""
+    from scipy.integrate import quad
+    import oracle_py as OP
+    tab = OP.wf_table()
+    assert tab[0] == 0.0 and tab.size == 500001
+    for n in (1, 3, 10, 50, 500):
+        want = quad(_wf_integrand, 0.0, 2.0 * n, epsabs=0, epsrel=1e-13, limit=400)[0]
+        assert abs(tab[n] - want) <= 1e-10 * want, n
+    for a, b in ((2000, 20000), (20000, 1000000)):
+        want = math.log((1 + 0.41 * b) / (1 + 0.41 * a)) / 0.41
+        assert abs((tab[b // 2] - tab[a // 2]) - want) <= 1e-10 * want
+
+
+def test_wall_model_utau_and_profile():
+    """find_utau inverts u = utau^2 integrate_F(utau, y): in the viscous
+    sublayer utau = sqrt(u nu / y) up to the table's linear first panel, and
+    on table nodes integrate_F is exactly table / utau."""
+    import oracle_py as OP
+    tab = OP.wf_table()
+    nu, y = 1e-3, 1e-4
+    u = 0.25 * y / nu                                   # utau = 0.5 gives y+ = 0.05
+    ut = OP.wf_find_utau(nu, u, y)
+    slope = tab[1] / 2.0                                # linear interpolation on [0, 2]
+    assert abs(ut - math.sqrt(u * nu / (y * slope))) <= 1e-9 * ut
+    assert abs(ut - 0.5) <= 2.5e-3
+    # a node: y+ = 2000 exactly
+    utau, yy = 0.05, 2000 * nu / 0.05
+    assert OP.wf_integrate_F(nu, utau, yy) == tab[1000] / utau
+    # Newton recovers utau from the velocity it implies (log region)
+    U = utau * utau * OP.wf_integrate_F(nu, utau, yy)
+    assert abs(OP.wf_find_utau(nu, U, yy) - utau) <= 1e-9
+
+
+def test_wall_function_splits_tangent_and_normal():
+    """Ub: tangential part along Uc's with |Ub_t| = u_Cabot(sb, utau), utau
+    from |Uc_t| at sc; normal part sb/sc times Uc's."""
+    import oracle_py as OP
+    rng = np.random.default_rng(3)
+    n = rng.standard_normal(3); n /= np.linalg.norm(n)
+    Uc = rng.standard_normal(3)
+    nu, sb, sc = 1 / 5000.0, 0.004, 0.012
+    Ub, us = OP.wall_function(nu, sc, sb, Uc, n)
+    un = Uc @ n
+    ut_c = Uc - un * n
+    assert abs(us - OP.wf_find_utau(nu, np.linalg.norm(ut_c), sc)) <= 1e-15 * us
+    ub_n = Ub @ n
+    ub_t = Ub - ub_n * n
+    assert abs(ub_n - sb / sc * un) <= 1e-13 * abs(un)
+    assert abs(np.linalg.norm(ub_t) - us * us * OP.wf_integrate_F(nu, us, sb)) <= 1e-13
+    assert np.linalg.norm(np.cross(ub_t, ut_c)) <= 1e-13 * np.linalg.norm(ub_t) * np.linalg.norm(ut_c)
+
+
+def test_wall_cells_on_a_uniform_channel():
+    """Uniform Cartesian channel with a wall-function bottom: the first
+    cells are IB nodes, their velocity is the wall model of the second cell's
+    with sb = dy/2, sc = 3dy/2 and the normal +y, the top wall keeps the
+    no-slip mirror, and IB_BC zeroes the bottom wall flux."""
+    import oracle_py as OP
+    cfg, coor, o0 = uniform_case()
+    cfg["bctype"] = [7, 7, -1, 1, 7, 7]
+    cfg["ren"] = 5000.0
+    o = Oracle(cfg, coor)
+    o.wall_mask()
+    nv = arr(o, o.nvert, 1)
+    assert np.all(nv[1:-1, 1, 1:-1] == 1) and nv[1:-1, 2:-1, 1:-1].sum() == 0 and nv[:, 0].sum() == 0
+    ucat = W.random_vector(cfg, 0.5, 71)
+    ucont = W.random_vector(cfg, 0.5, 72)
+    us = o.sca()
+    o.contra2cart_ustar(ucont.copy(), ucat, us)
+    U, S = arr(o, ucat), arr(o, us, 1)
+    dy = W.LY / (o.my - 2)
+    for (k, i) in [(1, 1), (3, 5), (o.mz - 2, o.mx - 2)]:
+        Ub, ust = OP.wall_function(1 / 5000.0, 1.5 * dy, 0.5 * dy, U[k, 2, i], np.array([0.0, 1.0, 0.0]))
+        assert rel_err(U[k, 1, i], Ub) < 1e-12 and abs(S[k, 1, i] - ust) <= 1e-12 * ust
+    assert np.array_equal(U[3, o.my - 1, 4], -U[3, o.my - 2, 4])
+    assert S[:, 2:].sum() == 0
+    uc = W.random_vector(cfg, 0.5, 73)
+    o.ib_bc(uc, ucat)
+    assert not arr(o, uc)[1:-1, 0, 1:-1, 1].any()
+
+
+# ---------------------------------------------------------------------------
 # Formfunction_2 (momentum.c)
 # ---------------------------------------------------------------------------
 def test_ff2_poiseuille_closed_form():

@@ -105,8 +105,8 @@ static int check_config(const vfs_config *cfg)
         int bt = cfg->bctype[f];
         if (per[f / 2]) {
             if (bt != 7) { vfs_set_error("periodic faces must carry bctype 7"); return -3; }
-        } else if (bt != 1) {
-            vfs_set_error("unsupported bctype (supported: 1 no-slip, 7 periodic)");
+        } else if (bt != 1 && !(bt == -1 && f < 4)) {
+            vfs_set_error("unsupported bctype (supported: 1 no-slip, 7 periodic, -1 wall function on i/j walls)");
             return -3;
         }
     }
@@ -164,6 +164,10 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     c->red_blocks = VFS_RED_BLOCKS;
     c->nvert_f32 = 1;
     if (hipDeviceSynchronize() != hipSuccess) { vfs_ctx_destroy(c); vfs_set_error("device sync"); return -1; }
+    if (vfs_wall_setup(c) || vfs_wall_mask(c) || hipDeviceSynchronize() != hipSuccess) {
+        vfs_ctx_destroy(c);
+        return -1;
+    }
     *out = c;
     return 0;
 }
@@ -177,6 +181,8 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     hipFree(c->geom_block); hipFree(c->nvert); hipFree(c->face); hipFree(c->fp);
     hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red);
     if (c->ad) hipFree(c->ad);
+    if (c->wf_table) hipFree(c->wf_table);
+    if (c->ustar) hipFree(c->ustar);
     delete c;
     return 0;
 }

@@ -136,9 +136,18 @@ struct vfs_ctx {
     double *red;             // reduction partials + result scalars
     double *ad;              // 12*P: primal and adjoint state of the JVP/VJP (allocated on first use)
     void *nk;                // Newton-Krylov workspace (allocated on first use)
+    double *wf_table;        // wall-function y+ integral table (bctype -1 only)
+    double *ustar;           // P: friction velocity of the wall-function cells
     int red_blocks;
     Poisson *pois;
 };
+
+// wall-function walls (vfs_wall.hip)
+bool vfs_has_wallfn(const Dims &d);
+int vfs_wall_setup(vfs_ctx *c);
+int vfs_wall_mask(vfs_ctx *c);
+int vfs_c2c_wall(vfs_ctx *c, const double *L, double *ucat);
+int vfs_ibbc_wall(vfs_ctx *c, double *ucont);
 
 // error handling
 void vfs_set_error(const std::string &msg);

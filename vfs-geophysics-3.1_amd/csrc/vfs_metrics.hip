@@ -192,6 +192,7 @@ __global__ void k_nvert_f32_check(long long n, const double *__restrict__ nvert,
 extern "C" int vfs_set_nvert(vfs_ctx *c, const double *nvert)
 {
     VFS_CHECK_HIP(hipMemcpyAsync(c->nvert, nvert, (size_t)c->d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (vfs_wall_mask(c)) return -1;   // the wall-function cells stay IB nodes
     hipLaunchKernelGGL(k_nvert_f32_check, dim3(VFS_RED_BLOCKS), dim3(256), 0, c->stream, c->d.P, c->nvert, c->red);
     VFS_CHECK_LAUNCH();
     double *res = c->red + (long long)VFS_RED_SLOTS * VFS_RED_BLOCKS;

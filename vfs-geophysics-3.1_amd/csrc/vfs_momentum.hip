@@ -90,13 +90,14 @@ extern "C" int vfs_contra2cart(vfs_ctx *c, double *ucont, double *ucat)
     if (vfs_periodic_vec(c, ucat)) return -1;
     const double *L = ucat;
     int nonper = !d.ip + !d.jp + !d.kp;
-    if (nonper >= 2) {
+    if (nonper >= 2 || vfs_has_wallfn(d)) {
         VFS_CHECK_HIP(hipMemcpyAsync(c->snap, ucat, 3 * (size_t)d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
         L = c->snap;
     }
     long long ns = shell_count(d);
     hipLaunchKernelGGL(k_c2c_bc_ghost, dim3(grid_for(ns, 256)), dim3(256), 0, c->stream, d, c->nvert, L, ucat);
     VFS_CHECK_LAUNCH();
+    if (vfs_c2c_wall(c, L, ucat)) return -1;   // wall-function cells (rhs.c:303-401)
     hipLaunchKernelGGL(k_c2c_bc_interior, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->nvert, ucat);
     VFS_CHECK_LAUNCH();
     return vfs_periodic_vec(c, ucat);
@@ -298,6 +299,7 @@ extern "C" int vfs_ib_bc(vfs_ctx *c, double *ucont, double *ucat)
         hipLaunchKernelGGL(k_ibbc, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, ucont);
         VFS_CHECK_LAUNCH();
     }
+    if (vfs_ibbc_wall(c, ucont)) return -1;
     if (d.ip || d.jp || d.kp) {
         hipLaunchKernelGGL(k_periodic_faces, dim3(grid_for(shell_count(d), 256)), dim3(256), 0, c->stream, d, ucont);
         VFS_CHECK_LAUNCH();

@@ -261,6 +261,12 @@ class Ctx:
         _ck(lib().vfs_divergence(self.h, ucont.ptr, _p(div), C.byref(m)), "vfs_divergence")
         return m.value
 
+    def ustar(self):
+        """Friction velocity of the wall-function cells (user->lUstar)."""
+        out = self.sca()
+        _ck(lib().vfs_get_ustar(self.h, out.ptr), "vfs_get_ustar")
+        return out
+
     def restart_write(self, directory, ti, block, ucont, ucat, p, nvert):
         """Ucont_P_Binary_Output (main.c:799-847): the vfield/ufield/pfield/
         nvfield restart files of block `block` at step `ti`."""
