@@ -119,7 +119,9 @@ int vfs_min_dt(vfs_ctx *ctx, const double *ucont, double *dt_out);
  * taken from `ucat` (frozen, as inside the reference's SNES solve).
  * J = dF/dU replaces the finite-difference MatMFFD product of
  * Implicit_MatrixFree (implicitsolver.c:4222-4236).  No argument is modified
- * except the output; v and jtv must not alias. */
+ * except the output; v and jtv must not alias.  Contexts with a
+ * wall-function wall (bctype -1) are rejected: the wall model is not
+ * differentiated. */
 /* jw = J w */
 int vfs_rhs_jvp(vfs_ctx *ctx, const double *U, const double *ucat, const double *nu_t,
                 const double *w, double *jw);

@@ -227,8 +227,8 @@ extern "C" int vfs_implicit_matrixfree(vfs_ctx *c, double *U, const double *U_o,
     const Dims d = c->d;
     VFS_REQUIRE(U && U_o && rhs_o && dP && ucat, "null argument");
     VFS_REQUIRE(nu_t || !d.les, "nu_t required when les is on");
-    VFS_REQUIRE(!(d.immersed && vfs_has_wallfn(d)),
-                "wall-function walls with an immersed body: the wall cells are not frozen IB nodes");
+    VFS_REQUIRE(!vfs_has_wallfn(d),
+                "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(restart >= 1 && restart <= VFS_RED_SLOTS - 4, "restart must be in [1, 60]");
     VFS_REQUIRE(rtol > 0 && max_newton >= 0 && max_krylov >= 1, "bad tolerance or iteration limits");
     if (nk_alloc(c, restart)) return -1;

@@ -144,8 +144,8 @@ extern "C" int vfs_rhs_jvp(vfs_ctx *c, const double *U, const double *ucat, cons
                            double *jw)
 {
     VFS_REQUIRE(nu_t || !c->d.les, "nu_t required when les is on");
-    VFS_REQUIRE(!(c->d.immersed && vfs_has_wallfn(c->d)),
-                "wall-function walls with an immersed body: the wall cells are not frozen IB nodes");
+    VFS_REQUIRE(!vfs_has_wallfn(c->d),
+                "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(U && ucat && w && jw, "null argument");
     VFS_REQUIRE(w != jw, "w and jw must not alias");
     if (vfs_rhs_prepare(c, U, ucat)) return -1;
@@ -419,8 +419,8 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
 {
     const Dims d = c->d;
     VFS_REQUIRE(nu_t || !d.les, "nu_t required when les is on");
-    VFS_REQUIRE(!(d.immersed && vfs_has_wallfn(d)),
-                "wall-function walls with an immersed body: the wall cells are not frozen IB nodes");
+    VFS_REQUIRE(!vfs_has_wallfn(d),
+                "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(U && ucat && v && jtv, "null argument");
     VFS_REQUIRE(v != jtv, "v and jtv must not alias");
     if (ensure_ad(c)) return -1;
