@@ -12,8 +12,9 @@ initial guess, UpdatePressure, Projection, Contra2Cart, Divergence.
 Prints one JSON line.  value = RHS throughput in Gcell/s of Formfunction_2
 (FP64, interior cells / mean kernel time from HIP events on the library's
 stream); the Poisson time-to-1e-8 in FP64 and in mixed FP32/FP64, the step
-times of both modes and the roofline of the dominant kernel are reported in
-the same line.
+times of both modes, the roofline of the dominant kernel and the implicit
+momentum solve at CFL 2 (Newton-Krylov with the exact JVP, plus the JVP and
+VJP times) are reported in the same line.
 """
 import argparse
 import ctypes as C
@@ -128,6 +129,60 @@ def time_rhs(ctx, cfg, s, reps):
     return ev.elapsed_ms(a, b) / reps
 
 
+def implicit_leg(ctx, cfg, spec, s):
+    """Implicit_MatrixFree (implicitsolver.c:4196-4441) at CFL 2 on the full
+    grid: dt = 2 x Calc_Minimum_dt, Newton-Krylov with the exact JVP of the
+    FormFunction_SNES residual, rtol 1e-4 (imp_free_tol); plus the time of one
+    JVP and one VJP against one Formfunction_2."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    Uo = V.DeviceArray.from_host(s.uc.host())
+    ctx.contra2cart(Uo, s.u)
+    if cfg["les"]:
+        ctx.eddy_viscosity(s.u, W.CS_LES, s.nut)
+    dt = 2.0 * ctx.min_dt(Uo)
+    cfg2 = dict(cfg, dt=dt)
+    c2 = V.Ctx(cfg2)
+    c2.build_metrics(V.DeviceArray.from_host(W.grid(spec)))
+    if spec["immersed"]:
+        c2.nvert_sphere(*W.SPHERE)
+    rhs_o = c2.vec()
+    c2.formfunction2(V.DeviceArray.from_host(Uo.host()), s.u, s.nut, rhs_o, 1.0)
+    cent = c2.metric("cent").host().reshape(-1, 3)
+    p = V.DeviceArray.from_host(0.01 * np.sin(cent[:, 0]) * np.cos(np.pi * cent[:, 1] / 2))
+    dP = c2.vec()
+    c2.pressure_gradient(p, dP)
+    U = V.DeviceArray.from_host(Uo.host())
+    ucat = V.DeviceArray.from_host(s.u.host())
+    lu, uu, r0 = c2.vec(), V.DeviceArray.from_host(s.u.host()), c2.vec()
+    c2.formfunction_snes(U, Uo, rhs_o, dP, s.nut, lu, uu, r0)
+    f0 = float(np.linalg.norm(r0.host()))
+    ev = HipEvents()
+    w, jw = V.DeviceArray.from_host(W.unit_vector(cfg, 14001)), c2.vec()
+    for _ in range(2):
+        c2.rhs_jvp(U, ucat, s.nut, w, jw)
+        c2.rhs_vjp(U, ucat, s.nut, w, jw)
+    c2.sync()
+    a, b, e = ev.event(), ev.event(), ev.event()
+    ev.record(a)
+    for _ in range(5):
+        c2.rhs_jvp(U, ucat, s.nut, w, jw)
+    ev.record(b)
+    for _ in range(5):
+        c2.rhs_vjp(U, ucat, s.nut, w, jw)
+    ev.record(e)
+    jvp_ms, vjp_ms = ev.elapsed_ms(a, b) / 5, ev.elapsed_ms(b, e) / 5
+    c2.sync()
+    t0 = time.perf_counter()
+    ni, ki, fn = c2.implicit_matrixfree(U, Uo, rhs_o, dP, s.nut, ucat, rtol=1e-4, max_newton=50, restart=30,
+                                        max_krylov=1000)
+    c2.sync()
+    t1 = time.perf_counter()
+    c2.close()
+    return {"dt_cfl2": dt, "newton_its": ni, "krylov_its": ki, "rel_residual": fn / f0,
+            "solve_ms": round((t1 - t0) * 1e3, 2), "jvp_ms": round(jvp_ms, 3), "vjp_ms": round(vjp_ms, 3)}
+
+
 def traffic_from_profile(name):
     """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
@@ -201,6 +256,7 @@ def main():
     ap.add_argument("--rhs-only", action="store_true",
                     help="only Formfunction_2 calls and a calibration copy (for counter runs)")
     ap.add_argument("--calib-reps", type=int, default=5)
+    ap.add_argument("--no-implicit", action="store_true", help="skip the Newton-Krylov leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -256,6 +312,11 @@ def main():
     achieved = bytes_per_cell * cells / (rhs_ms * 1e-3) / 1e9
     log(f"[bench] Formfunction_2 {rhs_ms:.3f} ms -> {gcells:.2f} Gcell/s, {achieved:.0f} GB/s")
 
+    implicit = None
+    if not args.no_implicit:
+        implicit = implicit_leg(ctx, cfg, spec, s)
+        log(f"[bench] implicit (CFL 2, exact JVP): {implicit}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -291,6 +352,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.config),
                      "kernel": "Formfunction_2", "bytes_per_cell": bytes_per_cell},
+        "implicit_cfl2": implicit,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
