@@ -14,7 +14,8 @@ Prints one JSON line.  value = RHS throughput in Gcell/s of Formfunction_2
 stream); the Poisson time-to-1e-8 in FP64 and in mixed FP32/FP64, the step
 times of both modes, the roofline of the dominant kernel and the implicit
 momentum solve at CFL 2 (Newton-Krylov with the exact JVP, plus the JVP and
-VJP times) are reported in the same line.
+VJP times) and the dynamic Smagorinsky procedure (-les 2) with its eddy
+viscosity are reported in the same line.
 """
 import argparse
 import ctypes as C
@@ -183,6 +184,63 @@ def implicit_leg(ctx, cfg, spec, s):
             "solve_ms": round((t1 - t0) * 1e3, 2), "jvp_ms": round(jvp_ms, 3), "vjp_ms": round(vjp_ms, 3)}
 
 
+# the dynamic procedure's compulsory traffic a point, by kernel: weight
+# (aj, nvert -> w) 24, gradient pass 192, Germano pass 208, Cs pass 40,
+# smoothing 32; the eddy viscosity from the Cs field 128
+DSM_BYTES_PER_CELL = 496
+NUT_BYTES_PER_CELL = 128
+
+
+def dynamic_les_leg(ctx, cfg, s, reps, cpu_budget):
+    """Compute_Smagorinsky_Constant_1 with -les 2 (les.c:67-1133) and the eddy
+    viscosity from its Cs field on the full grid: HIP-event time per call,
+    and the oracle on a 128x64x64 sample of the same generator."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    cs = ctx.sca()
+    for _ in range(2):
+        ctx.smagorinsky_constant(s.u, cs)
+        ctx.eddy_viscosity_cs(s.u, cs, s.nut)
+    ctx.sync()
+    ev = HipEvents()
+    a, b, e = ev.event(), ev.event(), ev.event()
+    ev.record(a)
+    for _ in range(reps):
+        ctx.smagorinsky_constant(s.u, cs)
+    ev.record(b)
+    for _ in range(reps):
+        ctx.eddy_viscosity_cs(s.u, cs, s.nut)
+    ev.record(e)
+    dsm_ms, nut_ms = ev.elapsed_ms(a, b) / reps, ev.elapsed_ms(b, e) / reps
+    Ni, Nj, Nk = cfg["mx"] - 2, cfg["my"] - 2, cfg["mz"] - 2
+    P = cfg["mx"] * cfg["my"] * cfg["mz"]
+    h = cs.host()
+    out = {"procedure_ms": round(dsm_ms, 4), "eddy_viscosity_ms": round(nut_ms, 4),
+           "gcells": round(Ni * Nj * Nk / (dsm_ms * 1e-3) / 1e9, 3),
+           "bytes_per_point": DSM_BYTES_PER_CELL,
+           "achieved_gbs": round(DSM_BYTES_PER_CELL * P / (dsm_ms * 1e-3) / 1e9, 1),
+           "frac_hbm": round(DSM_BYTES_PER_CELL * P / (dsm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "cs_max": round(float(h.max()), 4), "cs_nonzero_frac": round(float((h > 0).mean()), 3)}
+    if cpu_budget > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_py import Oracle
+        N = (128, 64, 64)
+        c2, spec = W.make_config("C2", N)
+        o = Oracle(c2, W.grid(spec))
+        o.nvert_sphere(*W.SPHERE)
+        ucat, ucont = W.fields(c2, spec, o.g, o.nvert)
+        o.contra2cart(ucont, ucat)
+        reps_c, t = 0, 0.0
+        while t < cpu_budget and reps_c < 100:
+            t0 = time.perf_counter()
+            o.smagorinsky_constant(ucat)
+            t += time.perf_counter() - t0
+            reps_c += 1
+        out["cpu_oracle_gcells"] = round(N[0] * N[1] * N[2] * reps_c / t / 1e9, 5)
+        out["cpu_sample"] = f"{reps_c} calls of the oracle at 128x64x64 cells, one thread ({t:.1f} s)"
+    return out
+
+
 def traffic_from_profile(name):
     """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
@@ -257,6 +315,7 @@ def main():
                     help="only Formfunction_2 calls and a calibration copy (for counter runs)")
     ap.add_argument("--calib-reps", type=int, default=5)
     ap.add_argument("--no-implicit", action="store_true", help="skip the Newton-Krylov leg")
+    ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic Smagorinsky leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -317,6 +376,11 @@ def main():
         implicit = implicit_leg(ctx, cfg, spec, s)
         log(f"[bench] implicit (CFL 2, exact JVP): {implicit}")
 
+    dyn = None
+    if not args.no_dynamic:
+        dyn = dynamic_les_leg(ctx, cfg, s, 5, 0.0 if args.no_cpu else 5.0)
+        log(f"[bench] dynamic Smagorinsky: {dyn}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -353,6 +417,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(args.config),
                      "kernel": "Formfunction_2", "bytes_per_cell": bytes_per_cell},
         "implicit_cfl2": implicit,
+        "dynamic_les": dyn,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

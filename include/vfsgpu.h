@@ -89,6 +89,19 @@ int vfs_get_ustar(vfs_ctx *ctx, double *dst);
 /* Compute_eddy_viscosity_LES (les.c:1135-1252) with constant Cs
  * (Compute_Smagorinsky_Constant_1, les.c:67-77: Cs = 0.01 for -les 1). */
 int vfs_eddy_viscosity(vfs_ctx *ctx, const double *ucat, double cs, double *nu_t);
+/* The same with a per-cell Cs field (user->lCs, les.c:1171-1203): cs, P
+ * doubles on the device, as vfs_smagorinsky_constant writes it. */
+int vfs_eddy_viscosity_cs(vfs_ctx *ctx, const double *ucat, const double *cs, double *nu_t);
+/* Compute_Smagorinsky_Constant_1 (les.c:67-1133) into cs (P doubles on the
+ * device, every point written).  les 1: Cs = 0.01; les 2: the dynamic
+ * procedure with the default filter options (3-D box test filter,
+ * -testfilter_ik 0; no -i/j/k_homo_filter averaging; no -clark term),
+ * clipped to [0, max_cs] (-max_cs, default 0.5) with at least 0.001 on IB
+ * nodes.  Cs = 0 while ti < 2 on a fresh start (tistart 0, rstart 0).
+ * ucat must carry its ghost values (after vfs_contra2cart).  Scratch of 12
+ * doubles a point is allocated on the first dynamic call and kept. */
+int vfs_smagorinsky_constant(vfs_ctx *ctx, const double *ucat, int les, int ti, int tistart, int rstart,
+                             double max_cs, double *cs);
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

@@ -639,8 +639,30 @@ static void du_center(const vo_cfg *c, int i, int j, int k, const double *ucat, 
 /* ------------------------------------------------------------------------ */
 /* les.c:1135-1252 Compute_eddy_viscosity_LES                               */
 /* ------------------------------------------------------------------------ */
-int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
-                      const double *ucat, double cs, double *nu_t)
+/* Compute_du_dxyz (k-omega.c:597-610) at cell id from du_center's
+ * computational-space differences: D[a][b] = d u_a / d x_b */
+static void du_dxyz(const vo_geom *g, size_t id, const double dc[3], const double de[3], const double dz[3],
+                    double D[3][3])
+{
+    const double ajc = g->aj[id];
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++)
+            D[a][b] = (dc[a] * g->csi[3 * id + b] + de[a] * g->eta[3 * id + b] + dz[a] * g->zet[3 * id + b]) * ajc;
+}
+
+/* |S| = sqrt(2 S:S) in the reference's term order (les.c:199-203) */
+static double strain_abs(const double D[3][3], double S[3][3])
+{
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) S[a][b] = 0.5 * (D[a][b] + D[b][a]);
+    double s = S[0][0] * S[0][0];
+    for (int n = 1; n < 9; n++) s += S[n / 3][n % 3] * S[n / 3][n % 3];
+    return sqrt(2.0 * s);
+}
+
+/* nu_t = Cs |S| (1/aj)^(2/3) with Cs either the constant cs or csf[id] */
+static int eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                          const double *ucat, double cs, const double *csf, double *nu_t)
 {
     DIMS;
     int rc = check_cfg(c);
@@ -671,10 +693,327 @@ int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
                 double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
                                           Szx * Szx + Szy * Szy + Szz * Szz));
                 double filter = pow(1. / g->aj[id], 1. / 3.);
-                nu_t[id] = cs * pow(filter, 2.0) * Sabs;
+                nu_t[id] = (csf ? csf[id] : cs) * pow(filter, 2.0) * Sabs;
             }
     per_copy_sca(c, nu_t); /* les.c:1215-1243 */
     return 0;
+}
+
+int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                      const double *ucat, double cs, double *nu_t)
+{
+    return eddy_viscosity(c, g, nvert, ucat, cs, NULL, nu_t);
+}
+
+int vo_eddy_viscosity_cs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         const double *ucat, const double *cs, double *nu_t)
+{
+    return eddy_viscosity(c, g, nvert, ucat, 0, cs, nu_t);
+}
+
+/* ------------------------------------------------------------------------ */
+/* les.c:67-1133 Compute_Smagorinsky_Constant_1, les == 2 with the default    */
+/* options: 3-D box test filter (testfilter_ik 0), no homogeneous averaging,  */
+/* no Clark term                                                              */
+/* ------------------------------------------------------------------------ */
+#define LES_EPS 1.e-4  /* les.c:19 */
+#define WALL_CS 0.001  /* les.c:20 */
+#define PMAX(a, b) ((a) < (b) ? (b) : (a))   /* PetscMax */
+#define PMIN(a, b) ((a) < (b) ? (a) : (b))   /* PetscMin */
+
+/* rhs2.c:412-475 integrate_testfilter: the weighted mean over the eight
+ * 2x2x2 octants of the 3x3x3 box, in the reference's summation order.
+ * val[R][Q][P] with R, Q, P the k, j, i offsets + 1. */
+double vo_testfilter(const double val[3][3][3], const double w[3][3][3])
+{
+    double vs = 0, ws = 0;
+    for (int o = 0; o < 8; o++) {
+        const int oa = o & 1, oc = (o >> 1) & 1, ob = (o >> 2) & 1;   /* v1..v8 */
+        double v = 0, ww = 0;
+        for (int t = 0; t < 8; t++) {
+            const int a = oa + (t & 1), cc = oc + ((t >> 1) & 1), b = ob + ((t >> 2) & 1);
+            v = t ? v + val[a][b][cc] * w[a][b][cc] : val[a][b][cc] * w[a][b][cc];
+            ww = t ? ww + w[a][b][cc] : w[a][b][cc];
+        }
+        vs = o ? vs + v : v;
+        ws = o ? ws + ww : ww;
+    }
+    return vs / ws;
+}
+
+/* les.c:23-64 get_weight: 1/aj, 0 in solid cells (the boundary rules are
+ * commented out there) */
+static void get_weight(const vo_cfg *c, const vo_geom *g, const double *nvert, int i, int j, int k,
+                       double w[3][3][3])
+{
+    DIMS;
+    for (int p = -1; p <= 1; p++)
+        for (int q = -1; q <= 1; q++)
+            for (int r = -1; r <= 1; r++) {
+                const size_t id = IX(k + r, j + q, i + p);
+                w[r + 1][q + 1][p + 1] = nvert[id] > 1.1 ? 0 : 1. / g->aj[id];
+            }
+}
+
+/* les.c:717-758 / 1066-1107: weights of the Cs-averaging filters (1/aj, 0 in
+ * solids and on non-periodic ghost planes; the j rule tests the centre's j,
+ * so the j = my-1 ghost plane keeps its weight) and the periodic sources */
+static void cs_weight(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *f, int i, int j, int k,
+                      double w[3][3][3], double v[3][3][3])
+{
+    DIMS;
+    for (int a = -1; a <= 1; a++)
+        for (int b = -1; b <= 1; b++)
+            for (int cc = -1; cc <= 1; cc++) {
+                const int R = cc + 1, Q = b + 1, P = a + 1;
+                int K = k + cc, J = j + b, I = i + a;
+                double wt = 1. / g->aj[IX(K, J, I)];
+                if (nvert[IX(K, J, I)] > 1.1) wt = 0;
+                if (c->i_periodic) { if (I == 0) I = mx - 2; else if (I == mx - 1) I = 1; }
+                else if (I == 0 || I == mx - 1) wt = 0;
+                if (c->j_periodic) { if (J == 0) J = my - 2; else if (J == my - 1) J = 1; }
+                else if (J == 0 || j == my - 1) wt = 0;
+                if (c->k_periodic) { if (K == 0) K = mz - 2; else if (K == mz - 1) K = 1; }
+                else if (K == 0 || K == mz - 1) wt = 0;
+                w[R][Q][P] = wt;
+                v[R][Q][P] = f[IX(K, J, I)];
+            }
+}
+
+/* les.c:188-257, first pass: velocity gradient grad[9] (du/dx, du/dy, du/dz,
+ * dv/dx, ...: the reference's Ax, Ay, Az), |S| and the test-filtered velocity
+ * at the non-solid interior cells, 0 elsewhere, then periodic copies. */
+int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                 double *grad, double *sabs, double *ucat_f)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    memset(grad, 0, 9 * P * sizeof(double));
+    memset(sabs, 0, P * sizeof(double));
+    memset(ucat_f, 0, 3 * P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i);
+                if (nvert[id] > 1.1) continue;
+                double dc[3], de[3], dz[3], D[3][3], S[3][3], w[3][3][3], u[3][3][3];
+                du_center(c, i, j, k, ucat, nvert, dc, de, dz);
+                du_dxyz(g, id, dc, de, dz, D);
+                sabs[id] = strain_abs(D, S);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) grad[9 * id + 3 * a + b] = D[a][b];
+                get_weight(c, g, nvert, i, j, k, w);
+                for (int q = 0; q < 3; q++) {
+                    for (int r = 0; r < 27; r++)
+                        u[r / 9][(r / 3) % 3][r % 3] = ucat[3 * IX(k + r / 9 - 1, j + (r / 3) % 3 - 1, i + r % 3 - 1) + q];
+                    ucat_f[3 * id + q] = vo_testfilter(u, w);
+                }
+            }
+    for (int k = 0; k < mz; k++)   /* les.c:271-304 */
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int a, b, cc;
+                if (!per_src(c, i, j, k, &a, &b, &cc)) continue;
+                const size_t d = IX(k, j, i), s = IX(cc, b, a);
+                sabs[d] = sabs[s];
+                for (int q = 0; q < 9; q++) grad[9 * d + q] = grad[9 * s + q];
+                for (int q = 0; q < 3; q++) ucat_f[3 * d + q] = ucat_f[3 * s + q];
+            }
+    return 0;
+}
+
+/* rhs2.c:538-552 Calculate_Covariant_metrics: the inverse of [csi; eta; zet] */
+static void covariant_metrics(const double gg[3][3], double G[3][3])
+{
+    const double a11 = gg[0][0], a12 = gg[0][1], a13 = gg[0][2];
+    const double a21 = gg[1][0], a22 = gg[1][1], a23 = gg[1][2];
+    const double a31 = gg[2][0], a32 = gg[2][1], a33 = gg[2][2];
+    const double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    G[0][0] = (a33 * a22 - a32 * a23) / det, G[0][1] = -(a33 * a12 - a32 * a13) / det, G[0][2] = (a23 * a12 - a22 * a13) / det;
+    G[1][0] = -(a33 * a21 - a31 * a23) / det, G[1][1] = (a33 * a11 - a31 * a13) / det, G[1][2] = -(a23 * a11 - a21 * a13) / det;
+    G[2][0] = (a32 * a21 - a31 * a22) / det, G[2][1] = -(a32 * a11 - a31 * a12) / det, G[2][2] = (a22 * a11 - a21 * a12) / det;
+}
+
+/* les.c:306-687, second pass: the Germano identity contracted in the
+ * contravariant form, LM = L_ba M_aq G_bq and MM = M_nm M_nl G_ml, at the
+ * non-solid interior cells (0 elsewhere), then periodic copies. */
+int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                 const double *grad, const double *sabs, const double *ucat_f, double *LM, double *MM)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    memset(LM, 0, P * sizeof(double));
+    memset(MM, 0, P * sizeof(double));
+    const double coef[3][3][3] = {{{0.125, 0.250, 0.125}, {0.250, 0.500, 0.250}, {0.125, 0.250, 0.125}},
+                                  {{0.250, 0.500, 0.250}, {0.500, 1.000, 0.500}, {0.250, 0.500, 0.250}},
+                                  {{0.125, 0.250, 0.125}, {0.250, 0.500, 0.250}, {0.125, 0.250, 0.125}}};
+    const double *met[3] = {g->csi, g->eta, g->zet};
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i);
+                if (nvert[id] > 1.1) continue;
+                double w[3][3][3], UU[3][3][3][3][3], SS[3][3][3][3][3];
+                get_weight(c, g, nvert, i, j, k, w);
+                for (int r = 0; r < 27; r++) {
+                    const int R = r / 9, Q = (r / 3) % 3, Pp = r % 3;
+                    const size_t n = IX(k + R - 1, j + Q - 1, i + Pp - 1);
+                    const double u[3] = {ucat[3 * n], ucat[3 * n + 1], ucat[3 * n + 2]};
+                    for (int b = 0; b < 3; b++) {
+                        const double *m = met[b] + 3 * n;
+                        const double U = u[0] * m[0] + u[1] * m[1] + u[2] * m[2];
+                        for (int a = 0; a < 3; a++) UU[b][a][R][Q][Pp] = U * u[a];
+                    }
+                    double D[3][3], S[3][3];
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) D[a][b] = grad[9 * n + 3 * a + b];
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) S[a][b] = 0.5 * (D[a][b] + D[b][a]);
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) SS[a][b][R][Q][Pp] = S[a][b] * sabs[n];
+                }
+                double sum_weight = 0;
+                for (int p = -1; p <= 1; p++)
+                    for (int q = -1; q <= 1; q++)
+                        for (int r = -1; r <= 1; r++) sum_weight += w[r + 1][q + 1][p + 1] * coef[r + 1][q + 1][p + 1];
+                const double filter = pow(1. / g->aj[id], 1. / 3.);
+                const double test_filter = pow(sum_weight, 1. / 3.);
+                const double uf[3] = {ucat_f[3 * id], ucat_f[3 * id + 1], ucat_f[3 * id + 2]};
+                double Uf[3];
+                for (int b = 0; b < 3; b++) {
+                    const double *m = met[b] + 3 * id;
+                    Uf[b] = uf[0] * m[0] + uf[1] * m[1] + uf[2] * m[2];
+                }
+                double dc[3], de[3], dz[3], D[3][3], Sh[3][3];
+                du_center(c, i, j, k, ucat_f, nvert, dc, de, dz);
+                du_dxyz(g, id, dc, de, dz, D);
+                const double S_hat = strain_abs(D, Sh);
+                double L[3][3], SSh[3][3], Mc[3][3], M[3][3], gg[3][3], gc[3][3], G[3][3];
+                for (int b = 0; b < 3; b++)
+                    for (int a = 0; a < 3; a++) L[b][a] = vo_testfilter(UU[b][a], w) - Uf[b] * uf[a];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) SSh[a][b] = vo_testfilter(SS[a][b], w);
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++) gg[a][b] = met[a][3 * id + b];
+                covariant_metrics(gg, gc);
+                /* G = (x_xi)^T (x_xi): dot products of the covariant base vectors */
+                for (int a = 0; a < 3; a++)
+                    G[a][a] = gc[0][a] * gc[0][a] + gc[1][a] * gc[1][a] + gc[2][a] * gc[2][a];
+                G[0][1] = G[1][0] = gc[0][1] * gc[0][0] + gc[1][1] * gc[1][0] + gc[2][1] * gc[2][0];
+                G[0][2] = G[2][0] = gc[0][2] * gc[0][0] + gc[1][2] * gc[1][0] + gc[2][2] * gc[2][0];
+                G[1][2] = G[2][1] = gc[0][1] * gc[0][2] + gc[1][1] * gc[1][2] + gc[2][1] * gc[2][2];
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        Mc[a][b] = -sq(test_filter) * S_hat * Sh[a][b] + sq(filter) * SSh[a][b];
+                for (int a = 0; a < 3; a++)
+                    for (int q = 0; q < 3; q++)
+                        M[a][q] = Mc[a][0] * gg[q][0] + Mc[a][1] * gg[q][1] + Mc[a][2] * gg[q][2];
+                double num = 0, denom = 0;
+                for (int q = 0; q < 3; q++)
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
+                for (int m = 0; m < 3; m++)
+                    for (int n = 0; n < 3; n++)
+                        for (int l = 0; l < 3; l++) denom += M[n][m] * M[n][l] * G[m][l];
+                LM[id] = num;
+                MM[id] = denom;
+            }
+    per_copy_sca(c, LM);   /* les.c:678-706 */
+    per_copy_sca(c, MM);
+    return 0;
+}
+
+/* les.c:708-789 and 943-994, third pass: Cs = max(0.5 <LM> / (<MM> + eps), 0)
+ * with <.> the 1/aj-weighted box filter, then the clipping: 0 in solids and
+ * on every ghost point, at least wall_cs at IB nodes, at most max_cs; then
+ * periodic copies (les.c:1018-1047). */
+int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *LM, const double *MM,
+                 double max_cs, double *Cs)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i);
+                if (nvert[id] > 1.1) { Cs[id] = 0; continue; }
+                double w[3][3][3], L0[3][3][3], M0[3][3][3];
+                cs_weight(c, g, nvert, LM, i, j, k, w, L0);
+                cs_weight(c, g, nvert, MM, i, j, k, w, M0);
+                const double C = 0.5 * vo_testfilter(L0, w) / (vo_testfilter(M0, w) + LES_EPS);
+                Cs[id] = PMAX(C, 0);
+            }
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                const size_t id = IX(k, j, i);
+                if (nvert[id] > 1.1 || k == 0 || k == mz - 1 || j == 0 || j == my - 1 || i == 0 || i == mx - 1) {
+                    Cs[id] = 0;
+                } else {
+                    if (nvert[id] > 0.1 && nvert[id] < 1.1) Cs[id] = PMAX(WALL_CS, Cs[id]);
+                    Cs[id] = PMIN(PMAX(Cs[id], 0), max_cs);
+                }
+            }
+    per_copy_sca(c, Cs);
+    return 0;
+}
+
+/* les.c:1049-1107: the fluid cells' Cs replaced by its 1/aj-weighted box
+ * filter (of the clipped field); IB nodes, solids and ghosts keep theirs. */
+int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, double *Cs)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    double *tmp = malloc(P * sizeof(double));
+    if (!tmp) return -9;
+    memcpy(tmp, Cs, P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i);
+                if (!(nvert[id] < 0.1)) continue;
+                double w[3][3][3], C[3][3][3];
+                cs_weight(c, g, nvert, tmp, i, j, k, w, C);
+                Cs[id] = vo_testfilter(C, w);
+            }
+    free(tmp);
+    return 0;
+}
+
+/* les.c:67-1133 with the fresh-start rule (Cs = 0 while ti < 2 on a run
+ * that is not a restart, 69-72) and les == 1 (Cs = 0.01, 74-77). */
+int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                            int les, int ti, int tistart, int rstart, double max_cs, double *Cs)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    if (les != 1 && les != 2) return -4;
+    if (ti < 2 && tistart == 0 && !rstart) {
+        for (size_t n = 0; n < P; n++) Cs[n] = 0.0;
+        return 0;
+    }
+    if (les == 1) {
+        for (size_t n = 0; n < P; n++) Cs[n] = 0.01;
+        return 0;
+    }
+    double *grad = malloc(9 * P * sizeof(double)), *sabs = malloc(P * sizeof(double));
+    double *uf = malloc(3 * P * sizeof(double)), *LM = malloc(P * sizeof(double)), *MM = malloc(P * sizeof(double));
+    if (!grad || !sabs || !uf || !LM || !MM) rc = -9;
+    if (!rc) rc = vo_dsm_pass1(c, g, nvert, ucat, grad, sabs, uf);
+    if (!rc) rc = vo_dsm_pass2(c, g, nvert, ucat, grad, sabs, uf, LM, MM);
+    if (!rc) rc = vo_dsm_pass3(c, g, nvert, LM, MM, max_cs, Cs);
+    if (!rc) rc = vo_dsm_smooth(c, g, nvert, Cs);
+    free(grad); free(sabs); free(uf); free(LM); free(MM);
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */

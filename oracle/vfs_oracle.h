@@ -78,6 +78,31 @@ int vo_wall_mask(const vo_cfg *c, double *nvert);
 int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
                       const double *ucat, double cs, double *nu_t);
 
+/* The same with a per-cell Cs field (the dynamic procedure's lCs). */
+int vo_eddy_viscosity_cs(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                         const double *ucat, const double *cs, double *nu_t);
+
+/* les.c:67-1133 Compute_Smagorinsky_Constant_1: les 1 (Cs = 0.01) or 2 (the
+ * dynamic procedure with the default options: 3-D box test filter, no
+ * homogeneous averaging, no Clark term).  Cs = 0 while ti < 2 on a fresh
+ * start (tistart 0, not a restart).  Cs: every point written. */
+int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                            int les, int ti, int tistart, int rstart, double max_cs, double *Cs);
+
+/* Its stages, exposed for their own tests.  rhs2.c:412-475 integrate_testfilter
+ * (val, w indexed [k][j][i] offset + 1); pass 1 (les.c:188-304): grad (9
+ * doubles a point: du/dx, du/dy, du/dz, dv/dx, ...), |S| and the filtered
+ * velocity; pass 2 (306-706): LM and MM; pass 3 and the clipping (708-1047);
+ * the final Cs smoothing (1049-1107, in place). */
+double vo_testfilter(const double val[3][3][3], const double w[3][3][3]);
+int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                 double *grad, double *sabs, double *ucat_f);
+int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+                 const double *grad, const double *sabs, const double *ucat_f, double *LM, double *MM);
+int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *LM, const double *MM,
+                 double max_cs, double *Cs);
+int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, double *Cs);
+
 /* momentum.c:197-440 Pressure_Gradient. p: in/out (periodic ghosts). */
 int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          double *p, double *dp);

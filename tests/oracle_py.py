@@ -48,7 +48,7 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter"]:
             getattr(_lib, name).restype = d
         for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
@@ -123,6 +123,35 @@ class Oracle:
 
     def eddy_viscosity(self, ucat, cs, nu_t):
         self._call("vo_eddy_viscosity", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.c_double(cs), ptr(nu_t))
+
+    def eddy_viscosity_cs(self, ucat, cs, nu_t):
+        self._call("vo_eddy_viscosity_cs", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(cs), ptr(nu_t))
+
+    def smagorinsky_constant(self, ucat, les=2, ti=5, tistart=0, rstart=0, max_cs=0.5):
+        cs = self.sca()
+        self._call("vo_smagorinsky_constant", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.c_int(les),
+                   C.c_int(ti), C.c_int(tistart), C.c_int(rstart), C.c_double(max_cs), ptr(cs))
+        return cs
+
+    def dsm_pass1(self, ucat):
+        grad, sabs, uf = np.zeros(9 * self.P), self.sca(), self.vec()
+        self._call("vo_dsm_pass1", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(grad), ptr(sabs), ptr(uf))
+        return grad, sabs, uf
+
+    def dsm_pass2(self, ucat, grad, sabs, uf):
+        LM, MM = self.sca(), self.sca()
+        self._call("vo_dsm_pass2", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(grad), ptr(sabs), ptr(uf),
+                   ptr(LM), ptr(MM))
+        return LM, MM
+
+    def dsm_pass3(self, LM, MM, max_cs=0.5):
+        cs = self.sca()
+        self._call("vo_dsm_pass3", C.byref(self.geom), ptr(self.nvert), ptr(LM), ptr(MM), C.c_double(max_cs), ptr(cs))
+        return cs
+
+    def dsm_smooth(self, cs):
+        self._call("vo_dsm_smooth", C.byref(self.geom), ptr(self.nvert), ptr(cs))
+        return cs
 
     def pressure_gradient(self, p, dp):
         self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
@@ -203,6 +232,14 @@ class Oracle:
 
 # the Cabot wall model of the wall-function walls (wallfunction.c)
 WF_NYP = 500000
+
+
+def testfilter(val, w):
+    """integrate_testfilter of 3x3x3 arrays indexed [k][j][i] offset + 1."""
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    assert val.shape == w.shape == (3, 3, 3)
+    return lib().vo_testfilter(ptr(val), ptr(w))
 
 
 def wf_table():

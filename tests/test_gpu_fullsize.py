@@ -145,3 +145,22 @@ def test_derivatives_full_size(big):
     j2 = ctx.vec()
     ctx.rhs_jvp(U, du, dn, V.DeviceArray.from_host(2.0 * w), j2)
     assert rel_err(j2.host(), 2.0 * Jw) < 1e-14
+
+
+def test_dynamic_smagorinsky_full_size(big):
+    """The dynamic procedure (-les 2) and its eddy viscosity on the full grid
+    against the oracle (about 20 s of oracle time at C2)."""
+    o, ctx = big["o"], big["ctx"]
+    du = V.DeviceArray.from_host(big["u"])
+    cs, nut = ctx.sca(), ctx.sca()
+    ctx.smagorinsky_constant(du, cs)
+    ctx.eddy_viscosity_cs(du, cs, nut)
+    say("  dynamic Cs on the GPU done; oracle ...")
+    want = o.smagorinsky_constant(big["u"])
+    nut_o = o.sca()
+    o.eddy_viscosity_cs(big["u"], want, nut_o)
+    got = cs.host()
+    say(f"  Cs rel err {rel_err(got, want):.2e}, max Cs {want.max():.3f}")
+    assert rel_err(got, want) < TOL
+    assert np.array_equal(got == 0, want == 0)
+    assert rel_err(nut.host(), nut_o) < TOL

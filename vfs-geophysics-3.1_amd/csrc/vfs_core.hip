@@ -183,6 +183,7 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     if (c->ad) hipFree(c->ad);
     if (c->wf_table) hipFree(c->wf_table);
     if (c->ustar) hipFree(c->ustar);
+    if (c->dsm) hipFree(c->dsm);
     delete c;
     return 0;
 }

@@ -1,0 +1,408 @@
+// The dynamic Smagorinsky procedure: Compute_Smagorinsky_Constant_1 with
+// -les 2 (les.c:67-1133) and the default filter options (3-D box test
+// filter, testfilter_ik 0; no homogeneous-direction averaging; no Clark
+// term), and -les 1 / the fresh-start rule of the same routine.
+//
+// Four kernels over the padded grid, one thread a point, each followed by
+// the periodic ghost copies the reference makes at that point:
+//   1. k_dsm_grad   velocity gradient at the non-solid interior cells,
+//                   stored as the six products S_ab |S| the test filter of
+//                   the second pass reads (the reference keeps the nine
+//                   gradients and |S| and forms these products there), and
+//                   the test-filtered velocity;
+//   2. k_dsm_lm     the Germano identity in the reference's contravariant
+//                   form: LM = L_ba M_aq G_bq, MM = M_nm M_nl G_ml;
+//   3. k_dsm_cs     Cs = max(0.5 <LM> / (<MM> + 1e-4), 0) and the clipping;
+//   4. k_dsm_smooth the box filter of the clipped Cs at the fluid cells.
+// Scratch: 13 doubles a point, allocated on first use and kept by the
+// context.  Built without FMA contraction; every sum keeps the reference's
+// order, so results differ from the oracle only through pow(x, 1/3).
+#include "vfs_internal.h"
+#include "vfs_stencil.h"
+
+#define VFS_LES_EPS 1.e-4   // les.c:19
+#define VFS_WALL_CS 0.001   // les.c:20
+#define PMAX(a, b) ((a) < (b) ? (b) : (a))   // PetscMax
+#define PMIN(a, b) ((a) < (b) ? (a) : (b))   // PetscMin
+
+// rhs2.c:412-475 integrate_testfilter: the weighted mean over the eight
+// 2x2x2 octants of the 3x3x3 box, sum(v1..v8) / sum(w1..w8), each octant
+// sum v_o = sum of its eight v * w products in the reference's order.
+// Within an octant that order is lexicographic in (j, i, k) offsets, so the
+// 27 points are streamed once in that order (OCT_FOR) and each product is
+// added to the accumulators of the octants holding it: the same sums,
+// rounding for rounding, with 8 accumulators a quantity instead of 27
+// stored values.  Octant o holds the k offsets (o & 1) + {0,1}, the i offsets
+// ((o >> 1) & 1) + {0,1} and the j offsets ((o >> 2) & 1) + {0,1}.
+struct Oct {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+// add product x of the point with k, j, i offsets + 1 = (R, Q, P); with
+// run-time offsets the octant tests become selects
+__device__ __forceinline__ void oct_add(Oct &a, int R, int Q, int P, double x)
+{
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+        const int oa = o & 1, oc = (o >> 1) & 1, ob = (o >> 2) & 1;
+        const bool in = (unsigned)(R - oa) <= 1u && (unsigned)(Q - ob) <= 1u && (unsigned)(P - oc) <= 1u;
+        const bool first = R == oa && Q == ob && P == oc;
+        a.s[o] = in ? (first ? x : a.s[o] + x) : a.s[o];
+    }
+}
+__device__ __forceinline__ double oct_total(const Oct &a)
+{
+    double t = a.s[0];
+#pragma unroll
+    for (int o = 1; o < 8; o++) t += a.s[o];
+    return t;
+}
+// the 27 points in (j, i, k) lexicographic order; R, Q, P are the k, j, i
+// offsets + 1.  OCT_FOR unrolls the box; OCT_FOR_RT keeps the nine-point j
+// rows as a run-time loop, which bounds the loads the compiler issues ahead.
+#define OCT_FOR                                                                                       \
+    _Pragma("unroll") for (int Q = 0; Q < 3; Q++) _Pragma("unroll") for (int P = 0; P < 3; P++)       \
+        _Pragma("unroll") for (int R = 0; R < 3; R++)
+#define OCT_FOR_RT                                                                                    \
+    _Pragma("unroll 1") for (int Q = 0; Q < 3; Q++) _Pragma("unroll") for (int P = 0; P < 3; P++)     \
+        _Pragma("unroll") for (int R = 0; R < 3; R++)
+__device__ __forceinline__ int slot_off(const Dims &d, int R, int Q, int P)
+{
+    return ((R - 1) * d.my + (Q - 1)) * d.mx + (P - 1);
+}
+
+// les.c:23-64 get_weight: 1/aj, 0 in solid cells (a function of the
+// neighbour alone, kept as a field)
+__global__ void k_dsm_weight(Dims d, const double *__restrict__ aj, const double *__restrict__ nvert,
+                             double *__restrict__ wt)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < d.P) wt[t] = nvert[t] > 1.1 ? 0 : 1. / aj[t];
+}
+
+// les.c:717-758 / 1066-1107: the Cs-averaging weight of neighbour (K,J,I)
+// of centre row j (the get_weight value, 0 on non-periodic ghost planes,
+// where the j test reads the centre's j, so a j = my-1 neighbour keeps its
+// weight) and the point its value is read from (periodic wrap)
+__device__ __forceinline__ double cs_weight(const Dims &d, const double *__restrict__ wt, int j, int K, int J, int I,
+                                            unsigned &src)
+{
+    double w = ld32(wt, IX32(d, K, J, I));
+    if (d.ip) { if (I == 0) I = d.mx - 2; else if (I == d.mx - 1) I = 1; }
+    else if (I == 0 || I == d.mx - 1) w = 0;
+    if (d.jp) { if (J == 0) J = d.my - 2; else if (J == d.my - 1) J = 1; }
+    else if (J == 0 || j == d.my - 1) w = 0;
+    if (d.kp) { if (K == 0) K = d.mz - 2; else if (K == d.mz - 1) K = 1; }
+    else if (K == 0 || K == d.mz - 1) w = 0;
+    src = IX32(d, K, J, I);
+    return w;
+}
+
+// k-omega.c:597-610 Compute_du_dxyz: D[a][b] = d u_a / d x_b
+__device__ __forceinline__ void du_dxyz(const Geom &g, long long t, const double dc[3], const double de[3],
+                                        const double dz[3], double D[3][3])
+{
+    const double ajc = g.aj[t];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+            D[a][b] = (dc[a] * g.csi[3 * t + b] + de[a] * g.eta[3 * t + b] + dz[a] * g.zet[3 * t + b]) * ajc;
+}
+
+// S_ab = (D_ab + D_ba) / 2 and |S| = sqrt(2 S:S), terms in the order xx, xy,
+// xz, yx, ... (les.c:199-203)
+__device__ __forceinline__ double strain(const double D[3][3], double S[3][3])
+{
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S[a][b] = 0.5 * (D[a][b] + D[b][a]);
+    double s = S[0][0] * S[0][0];
+#pragma unroll
+    for (int n = 1; n < 9; n++) s += S[n / 3][n % 3] * S[n / 3][n % 3];
+    return sqrt(2.0 * s);
+}
+
+__device__ __forceinline__ bool interior(const Dims &d, int i, int j, int k)
+{
+    return i >= 1 && j >= 1 && k >= 1 && i <= d.mx - 2 && j <= d.my - 2 && k <= d.mz - 2;
+}
+
+// one thread a point; 32-bit point and element indices (loads as uniform
+// base + 32-bit offset), valid while 24 P < 2^32 (checked on the host)
+#define DSM_PREAMBLE                                                                                 \
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;                                        \
+    if (t >= (unsigned)d.P) return;                                                                  \
+    const int i = (int)(t % (unsigned)d.mx), j = (int)((t / (unsigned)d.mx) % (unsigned)d.my),       \
+              k = (int)(t / ((unsigned)d.mx * (unsigned)d.my))
+
+// the six S_ab |S| planes: (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+__device__ __forceinline__ int ss_plane(int a, int b)
+{
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
+}
+
+// les.c:188-257 (first pass)
+__global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucat,
+                           const double *__restrict__ wt, double *__restrict__ ss, double *__restrict__ uf)
+{
+    DSM_PREAMBLE;
+    double *pl[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) pl[q] = ss + (size_t)q * d.P;
+    if (!interior(d, i, j, k) || ld32(nvert, t) > 1.1) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) st32(pl[q], t, 0.0);
+#pragma unroll
+        for (int q = 0; q < 3; q++) st32(uf, 3 * t + q, 0.0);
+        return;
+    }
+    double dc[3], de[3], dz[3], D[3][3], S[3][3];
+    du_center(d, i, j, k, ucat, nvert, dc, de, dz);
+    du_dxyz(g, t, dc, de, dz, D);
+    const double sabs = strain(D, S);
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = a; b < 3; b++) st32(pl[ss_plane(a, b)], t, S[a][b] * sabs);
+    Oct v[3], w;
+    OCT_FOR
+    {
+        const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+        const double wn = ld32(wt, n);
+        oct_add(w, R, Q, P, wn);
+#pragma unroll
+        for (int q = 0; q < 3; q++) oct_add(v[q], R, Q, P, ld32(ucat, 3 * n + q) * wn);
+    }
+    const double ws = oct_total(w);
+#pragma unroll
+    for (int q = 0; q < 3; q++) st32(uf, 3 * t + q, oct_total(v[q]) / ws);
+}
+
+// rhs2.c:538-552 Calculate_Covariant_metrics: the inverse of [csi; eta; zet]
+__device__ __forceinline__ void covariant_metrics(const double gg[3][3], double G[3][3])
+{
+    const double a11 = gg[0][0], a12 = gg[0][1], a13 = gg[0][2];
+    const double a21 = gg[1][0], a22 = gg[1][1], a23 = gg[1][2];
+    const double a31 = gg[2][0], a32 = gg[2][1], a33 = gg[2][2];
+    const double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    G[0][0] = (a33 * a22 - a32 * a23) / det, G[0][1] = -(a33 * a12 - a32 * a13) / det;
+    G[0][2] = (a23 * a12 - a22 * a13) / det;
+    G[1][0] = -(a33 * a21 - a31 * a23) / det, G[1][1] = (a33 * a11 - a31 * a13) / det;
+    G[1][2] = -(a23 * a11 - a21 * a13) / det;
+    G[2][0] = (a32 * a21 - a31 * a22) / det, G[2][1] = -(a32 * a11 - a31 * a12) / det;
+    G[2][2] = (a22 * a11 - a21 * a12) / det;
+}
+
+// les.c:306-687 (second pass)
+__global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__restrict__ nvert,
+                                                const double *__restrict__ ucat, const double *__restrict__ wt,
+                                                const double *__restrict__ ss, const double *__restrict__ uf,
+                                                double *__restrict__ LM, double *__restrict__ MM)
+{
+    DSM_PREAMBLE;
+    if (!interior(d, i, j, k) || ld32(nvert, t) > 1.1) {
+        st32(LM, t, 0.0);
+        st32(MM, t, 0.0);
+        return;
+    }
+    const double *met[3] = {g.csi, g.eta, g.zet};
+    // les.c:380-393: sum of weight * coef over p (i), q (j), r (k)
+    double sum_weight = 0;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const double cf = 0.125 * (double)((1 + (r == 1)) * (1 + (q == 1)) * (1 + (p == 1)));
+                sum_weight += ld32(wt, t + (unsigned)slot_off(d, r, q, p)) * cf;
+            }
+    const double filter = pow(1. / ld32(g.aj, t), 1. / 3.);
+    const double test_filter = pow(sum_weight, 1. / 3.);
+    const double ufc[3] = {ld32(uf, 3 * t), ld32(uf, 3 * t + 1), ld32(uf, 3 * t + 2)};
+    // L_ba = <U_b u_a> - U_b(<u>) <u>_a with U_b = u . (csi, eta, zet)_b, then
+    // <S_ab |S|>: five streams over the box, three quantities each
+    double L[3][3], ws = 0;
+#pragma unroll
+    for (int b = 0; b < 3; b++) {
+        Oct v[3], w;
+        OCT_FOR_RT
+        {
+            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+            const double wn = ld32(wt, n), u0 = ld32(ucat, 3 * n), u1 = ld32(ucat, 3 * n + 1), u2 = ld32(ucat, 3 * n + 2);
+            const double U = u0 * ld32(met[b], 3 * n) + u1 * ld32(met[b], 3 * n + 1) + u2 * ld32(met[b], 3 * n + 2);
+            if (b == 0) oct_add(w, R, Q, P, wn);
+            oct_add(v[0], R, Q, P, U * u0 * wn);
+            oct_add(v[1], R, Q, P, U * u1 * wn);
+            oct_add(v[2], R, Q, P, U * u2 * wn);
+        }
+        if (b == 0) ws = oct_total(w);
+        const double Uf = ufc[0] * ld32(met[b], 3 * t) + ufc[1] * ld32(met[b], 3 * t + 1) + ufc[2] * ld32(met[b], 3 * t + 2);
+#pragma unroll
+        for (int a = 0; a < 3; a++) L[b][a] = oct_total(v[a]) / ws - Uf * ufc[a];
+    }
+    double SSh[3][3];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {   // planes 0-2, then 3-5
+        Oct v[3];
+        const double *pl[3] = {ss + (size_t)(3 * h) * d.P, ss + (size_t)(3 * h + 1) * d.P, ss + (size_t)(3 * h + 2) * d.P};
+        OCT_FOR_RT
+        {
+            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+            const double wn = ld32(wt, n);
+#pragma unroll
+            for (int q = 0; q < 3; q++) oct_add(v[q], R, Q, P, ld32(pl[q], n) * wn);
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const int pn = 3 * h + q, a = pn < 3 ? 0 : (pn < 5 ? 1 : 2), b = pn < 3 ? pn : (pn < 5 ? pn - 2 : 2);
+            SSh[a][b] = SSh[b][a] = oct_total(v[q]) / ws;
+        }
+    }
+    double dc[3], de[3], dz[3], D[3][3], Sh[3][3];
+    du_center(d, i, j, k, uf, nvert, dc, de, dz);
+    du_dxyz(g, t, dc, de, dz, D);
+    const double S_hat = strain(D, Sh);
+    double gg[3][3], gc[3][3], G[3][3], M[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) gg[a][b] = ld32(met[a], 3 * t + b);
+    covariant_metrics(gg, gc);
+#pragma unroll
+    for (int a = 0; a < 3; a++) G[a][a] = gc[0][a] * gc[0][a] + gc[1][a] * gc[1][a] + gc[2][a] * gc[2][a];
+    G[0][1] = G[1][0] = gc[0][1] * gc[0][0] + gc[1][1] * gc[1][0] + gc[2][1] * gc[2][0];
+    G[0][2] = G[2][0] = gc[0][2] * gc[0][0] + gc[1][2] * gc[1][0] + gc[2][2] * gc[2][0];
+    G[1][2] = G[2][1] = gc[0][1] * gc[0][2] + gc[1][1] * gc[1][2] + gc[2][1] * gc[2][2];
+    const double tf2 = test_filter * test_filter, f2 = filter * filter;   // pow(x, 2.)
+    double Mc[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) Mc[a][b] = -tf2 * S_hat * Sh[a][b] + f2 * SSh[a][b];
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) M[a][q] = Mc[a][0] * gg[q][0] + Mc[a][1] * gg[q][1] + Mc[a][2] * gg[q][2];
+    double num = 0, denom = 0;
+#pragma unroll
+    for (int q = 0; q < 3; q++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+        for (int n = 0; n < 3; n++)
+#pragma unroll
+            for (int l = 0; l < 3; l++) denom += M[n][m] * M[n][l] * G[m][l];
+    st32(LM, t, num);
+    st32(MM, t, denom);
+}
+
+// les.c:708-789 (third pass) and the clipping of les.c:943-994
+__global__ void __launch_bounds__(256) k_dsm_cs(Dims d, const double *__restrict__ nvert, const double *__restrict__ wt,
+                         const double *__restrict__ LM, const double *__restrict__ MM, double max_cs,
+                         double *__restrict__ cs)
+{
+    DSM_PREAMBLE;
+    const double nv = ld32(nvert, t);
+    if (!interior(d, i, j, k) || nv > 1.1) {
+        st32(cs, t, 0.0);
+        return;
+    }
+    Oct l, m, w;
+    OCT_FOR
+    {
+        unsigned src;
+        const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
+        oct_add(w, R, Q, P, wn);
+        oct_add(l, R, Q, P, ld32(LM, src) * wn);
+        oct_add(m, R, Q, P, ld32(MM, src) * wn);
+    }
+    const double ws = oct_total(w);
+    const double C = 0.5 * (oct_total(l) / ws) / (oct_total(m) / ws + VFS_LES_EPS);
+    double c = PMAX(C, 0);
+    if (nv > 0.1 && nv < 1.1) c = PMAX(VFS_WALL_CS, c);
+    st32(cs, t, PMIN(PMAX(c, 0), max_cs));
+}
+
+// les.c:1049-1107: the box filter of the clipped Cs at the fluid cells
+__global__ void __launch_bounds__(256) k_dsm_smooth(Dims d, const double *__restrict__ nvert, const double *__restrict__ wt,
+                             const double *__restrict__ cst, double *__restrict__ cs)
+{
+    DSM_PREAMBLE;
+    if (!interior(d, i, j, k) || !(ld32(nvert, t) < 0.1)) {
+        st32(cs, t, ld32(cst, t));
+        return;
+    }
+    Oct v, w;
+    OCT_FOR
+    {
+        unsigned src;
+        const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
+        oct_add(w, R, Q, P, wn);
+        oct_add(v, R, Q, P, ld32(cst, src) * wn);
+    }
+    st32(cs, t, oct_total(v) / oct_total(w));
+}
+
+__global__ void k_fill(long long n, double v, double *__restrict__ x)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] = v;
+}
+
+// periodic ghost copies of np consecutive scalar planes of P points
+__global__ void k_periodic_planes(Dims d, int np, double *v)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    int k, j, i, a, b, c;
+    if (t >= shell_count(d) || !shell_point(d, t, k, j, i) || !per_src(d, i, j, k, a, b, c)) return;
+    const long long o = IX(d, k, j, i), s = IX(d, c, b, a);
+    for (int p = 0; p < np; p++) v[p * d.P + o] = v[p * d.P + s];
+}
+
+static int periodic_planes(vfs_ctx *c, double *v, int np)
+{
+    if (!(c->d.ip || c->d.jp || c->d.kp)) return 0;
+    const long long n = shell_count(c->d);
+    hipLaunchKernelGGL(k_periodic_planes, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->d, np, v);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
+extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, int les, int ti, int tistart, int rstart,
+                                        double max_cs, double *cs)
+{
+    VFS_REQUIRE(les == 1 || les == 2, "les must be 1 (constant Cs) or 2 (dynamic procedure)");
+    VFS_REQUIRE(ucat && cs, "ucat and cs are required");
+    const Dims d = c->d;
+    VFS_REQUIRE(VFS_IX32_OK(d.P), "grid too large for the dynamic procedure's 32-bit indexing (24 P >= 2^32)");
+    if ((ti < 2 && tistart == 0 && !rstart) || les == 1) {   // les.c:69-77
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d.P,
+                           les == 1 && !(ti < 2 && tistart == 0 && !rstart) ? 0.01 : 0.0, cs);
+        VFS_CHECK_LAUNCH();
+        return 0;
+    }
+    if (!c->dsm) VFS_CHECK_HIP(hipMalloc(&c->dsm, (size_t)13 * d.P * sizeof(double)));
+    double *ss = c->dsm, *uf = ss + 6 * d.P, *LM = uf + 3 * d.P, *MM = LM + d.P, *cst = MM + d.P, *wt = cst + d.P;
+    const dim3 blk(256), grd(grid_for(d.P, 256));
+    hipLaunchKernelGGL(k_dsm_weight, grd, blk, 0, c->stream, d, c->g.aj, c->nvert, wt);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_dsm_grad, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, ss, uf);
+    VFS_CHECK_LAUNCH();
+    if (periodic_planes(c, ss, 6) || vfs_periodic_vec(c, uf)) return -1;
+    hipLaunchKernelGGL(k_dsm_lm, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, ss, uf, LM, MM);
+    VFS_CHECK_LAUNCH();
+    if (periodic_planes(c, LM, 2)) return -1;   // LM and MM are adjacent planes
+    hipLaunchKernelGGL(k_dsm_cs, grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, max_cs, cst);
+    VFS_CHECK_LAUNCH();
+    if (vfs_periodic_sca(c, cst)) return -1;
+    hipLaunchKernelGGL(k_dsm_smooth, grd, blk, 0, c->stream, d, c->nvert, wt, cst, cs);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}

@@ -107,7 +107,7 @@ extern "C" int vfs_contra2cart(vfs_ctx *c, double *ucont, double *ucat)
 // Compute_eddy_viscosity_LES
 // ---------------------------------------------------------------------------
 __global__ void k_les(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucat,
-                      double cs, double *__restrict__ nu_t)
+                      double cs, const double *__restrict__ csf, double *__restrict__ nu_t)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
@@ -135,13 +135,25 @@ __global__ void k_les(Dims d, Geom g, const double *__restrict__ nvert, const do
     double Sabs = sqrt(2.0 * (Sxx * Sxx + Sxy * Sxy + Sxz * Sxz + Syx * Syx + Syy * Syy + Syz * Syz +
                               Szx * Szx + Szy * Szy + Szz * Szz));
     double filter = pow(1. / g.aj[t], 1. / 3.);
-    nu_t[t] = cs * pow(filter, 2.0) * Sabs;
+    nu_t[t] = (csf ? csf[t] : cs) * pow(filter, 2.0) * Sabs;
 }
 
 extern "C" int vfs_eddy_viscosity(vfs_ctx *c, const double *ucat, double cs, double *nu_t)
 {
     const Dims d = c->d;
-    hipLaunchKernelGGL(k_les, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, cs, nu_t);
+    hipLaunchKernelGGL(k_les, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, cs,
+                       (const double *)nullptr, nu_t);
+    VFS_CHECK_LAUNCH();
+    return vfs_periodic_sca(c, nu_t);
+}
+
+// the same with the per-cell Cs of the dynamic procedure (user->lCs)
+extern "C" int vfs_eddy_viscosity_cs(vfs_ctx *c, const double *ucat, const double *cs, double *nu_t)
+{
+    VFS_REQUIRE(cs, "cs field required");
+    const Dims d = c->d;
+    hipLaunchKernelGGL(k_les, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, 0.0, cs,
+                       nu_t);
     VFS_CHECK_LAUNCH();
     return vfs_periodic_sca(c, nu_t);
 }

@@ -178,6 +178,16 @@ class Ctx:
     def eddy_viscosity(self, ucat, cs, nu_t):
         _ck(lib().vfs_eddy_viscosity(self.h, ucat.ptr, C.c_double(cs), nu_t.ptr), "vfs_eddy_viscosity")
 
+    def eddy_viscosity_cs(self, ucat, cs, nu_t):
+        """nu_t from a per-cell Cs field (DeviceArray of P doubles)."""
+        _ck(lib().vfs_eddy_viscosity_cs(self.h, ucat.ptr, cs.ptr, nu_t.ptr), "vfs_eddy_viscosity_cs")
+
+    def smagorinsky_constant(self, ucat, cs, les=2, ti=2, tistart=0, rstart=0, max_cs=0.5):
+        """Compute_Smagorinsky_Constant_1 (les.c:67-1133) into cs: -les 1
+        (0.01) or 2 (dynamic procedure); 0 while ti < 2 on a fresh start."""
+        _ck(lib().vfs_smagorinsky_constant(self.h, ucat.ptr, int(les), int(ti), int(tistart), int(rstart),
+                                           C.c_double(max_cs), cs.ptr), "vfs_smagorinsky_constant")
+
     def pressure_gradient(self, p, dp):
         _ck(lib().vfs_pressure_gradient(self.h, p.ptr, dp.ptr), "vfs_pressure_gradient")
 
