@@ -98,7 +98,7 @@ int vfs_eddy_viscosity_cs(vfs_ctx *ctx, const double *ucat, const double *cs, do
  * -testfilter_ik 0; no -i/j/k_homo_filter averaging; no -clark term),
  * clipped to [0, max_cs] (-max_cs, default 0.5) with at least 0.001 on IB
  * nodes.  Cs = 0 while ti < 2 on a fresh start (tistart 0, rstart 0).
- * ucat must carry its ghost values (after vfs_contra2cart).  Scratch of 12
+ * ucat must carry its ghost values (after vfs_contra2cart).  Scratch of 19
  * doubles a point is allocated on the first dynamic call and kept. */
 int vfs_smagorinsky_constant(vfs_ctx *ctx, const double *ucat, int les, int ti, int tistart, int rstart,
                              double max_cs, double *cs);

@@ -3,8 +3,9 @@
 // filter, testfilter_ik 0; no homogeneous-direction averaging; no Clark
 // term), and -les 1 / the fresh-start rule of the same routine.
 //
-// Four kernels over the padded grid, one thread a point, each followed by
-// the periodic ghost copies the reference makes at that point:
+// Five kernels over the padded grid, each followed by the periodic ghost
+// copies the reference makes at that point:
+//   0. k_dsm_point  point-wise inputs of the filters as unit-stride planes;
 //   1. k_dsm_grad   velocity gradient at the non-solid interior cells,
 //                   stored as the six products S_ab |S| the test filter of
 //                   the second pass reads (the reference keeps the nine
@@ -14,7 +15,7 @@
 //                   form: LM = L_ba M_aq G_bq, MM = M_nm M_nl G_ml;
 //   3. k_dsm_cs     Cs = max(0.5 <LM> / (<MM> + 1e-4), 0) and the clipping;
 //   4. k_dsm_smooth the box filter of the clipped Cs at the fluid cells.
-// Scratch: 13 doubles a point, allocated on first use and kept by the
+// Scratch: 19 doubles a point, allocated on first use and kept by the
 // context.  Built without FMA contraction; every sum keeps the reference's
 // order, so results differ from the oracle only through pow(x, 1/3).
 #include "vfs_internal.h"
@@ -56,6 +57,25 @@ __device__ __forceinline__ double oct_total(const Oct &a)
     for (int o = 1; o < 8; o++) t += a.s[o];
     return t;
 }
+// sum(v1..v8) of 27 stored products p[9 R + 3 Q + P], in the reference's
+// order (the octant's eight terms with the k offset fastest, then i, then j)
+__device__ __forceinline__ double octsum27(const double p[27])
+{
+    double vs = 0;
+#pragma unroll
+    for (int o = 0; o < 8; o++) {
+        const int oa = o & 1, oc = (o >> 1) & 1, ob = (o >> 2) & 1;
+        double v = 0;
+#pragma unroll
+        for (int tt = 0; tt < 8; tt++) {
+            const int r = 9 * (oa + (tt & 1)) + 3 * (ob + ((tt >> 2) & 1)) + oc + ((tt >> 1) & 1);
+            v = tt ? v + p[r] : p[r];
+        }
+        vs = o ? vs + v : v;
+    }
+    return vs;
+}
+
 // the 27 points in (j, i, k) lexicographic order; R, Q, P are the k, j, i
 // offsets + 1.  OCT_FOR unrolls the box; OCT_FOR_RT keeps the nine-point j
 // rows as a run-time loop, which bounds the loads the compiler issues ahead.
@@ -70,13 +90,32 @@ __device__ __forceinline__ int slot_off(const Dims &d, int R, int Q, int P)
     return ((R - 1) * d.my + (Q - 1)) * d.mx + (P - 1);
 }
 
-// les.c:23-64 get_weight: 1/aj, 0 in solid cells (a function of the
-// neighbour alone, kept as a field)
-__global__ void k_dsm_weight(Dims d, const double *__restrict__ aj, const double *__restrict__ nvert,
-                             double *__restrict__ wt)
+// Point-wise inputs of the box filters, as planes of P doubles so that the
+// stencil passes read them with unit stride: the get_weight value (les.c:
+// 23-64: 1/aj, 0 in solid cells; a function of the neighbour alone), the
+// velocity components and U_b = u . (csi, eta, zet)_b (les.c:348-354).
+__global__ void k_dsm_point(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucat,
+                            double *__restrict__ wt, double *__restrict__ uq, double *__restrict__ Ub)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < d.P) wt[t] = nvert[t] > 1.1 ? 0 : 1. / aj[t];
+    const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)d.P) return;
+    const size_t P = (size_t)d.P;
+    st32(wt, t, ld32(nvert, t) > 1.1 ? 0 : 1. / ld32(g.aj, t));
+    const double u0 = ld32(ucat, 3 * t), u1 = ld32(ucat, 3 * t + 1), u2 = ld32(ucat, 3 * t + 2);
+    st32(uq, t, u0);
+    st32(uq + P, t, u1);
+    st32(uq + 2 * P, t, u2);
+    const double *met[3] = {g.csi, g.eta, g.zet};
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+        st32(Ub + b * P, t, u0 * ld32(met[b], 3 * t) + u1 * ld32(met[b], 3 * t + 1) + u2 * ld32(met[b], 3 * t + 2));
+}
+
+// a cell whose box holds no ghost point: the Cs-averaging weights are the
+// get_weight values and no index wraps
+__device__ __forceinline__ bool away_from_ghosts(const Dims &d, int i, int j, int k)
+{
+    return i >= 2 && j >= 2 && k >= 2 && i <= d.mx - 3 && j <= d.my - 3 && k <= d.mz - 3;
 }
 
 // les.c:717-758 / 1066-1107: the Cs-averaging weight of neighbour (K,J,I)
@@ -144,8 +183,10 @@ __device__ __forceinline__ int ss_plane(int a, int b)
 }
 
 // les.c:188-257 (first pass)
-__global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucat,
-                           const double *__restrict__ wt, double *__restrict__ ss, double *__restrict__ uf)
+__global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *__restrict__ nvert,
+                                                  const double *__restrict__ ucat, const double *__restrict__ uq,
+                                                  const double *__restrict__ wt, double *__restrict__ ss,
+                                                  double *__restrict__ uf)
 {
     DSM_PREAMBLE;
     double *pl[6];
@@ -173,7 +214,7 @@ __global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *
         const double wn = ld32(wt, n);
         oct_add(w, R, Q, P, wn);
 #pragma unroll
-        for (int q = 0; q < 3; q++) oct_add(v[q], R, Q, P, ld32(ucat, 3 * n + q) * wn);
+        for (int q = 0; q < 3; q++) oct_add(v[q], R, Q, P, ld32(uq + (size_t)q * d.P, n) * wn);
     }
     const double ws = oct_total(w);
 #pragma unroll
@@ -195,72 +236,96 @@ __device__ __forceinline__ void covariant_metrics(const double gg[3][3], double 
     G[2][2] = (a22 * a11 - a21 * a12) / det;
 }
 
-// les.c:306-687 (second pass)
+// les.c:306-687 (second pass).  A block of four waves covers 64 consecutive
+// points; each wave streams a share of the sixteen box filters for all 64
+// (wave b < 3: <U_b u_a> for a = 0..2, U_b = u . (csi, eta, zet)_b, and
+// <S|S|> plane b; wave 3: planes 3-5 and the weight sums), so a wave's role
+// is uniform and its loads run along i.  The raw octant totals meet in LDS,
+// and wave 0 finishes each point: the divisions by the weight total, the
+// Germano terms and the contractions.
+#define DSM_LM_PTS 64
 __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__restrict__ nvert,
-                                                const double *__restrict__ ucat, const double *__restrict__ wt,
-                                                const double *__restrict__ ss, const double *__restrict__ uf,
-                                                double *__restrict__ LM, double *__restrict__ MM)
+                                                const double *__restrict__ uq, const double *__restrict__ Ub,
+                                                const double *__restrict__ wt, const double *__restrict__ ss,
+                                                const double *__restrict__ uf, double *__restrict__ LM,
+                                                double *__restrict__ MM)
 {
-    DSM_PREAMBLE;
-    if (!interior(d, i, j, k) || ld32(nvert, t) > 1.1) {
+    __shared__ double sh[17][DSM_LM_PTS];   // 0-8 <U_b u_a> (3b+a), 9-14 <S|S|> planes, 15 w total, 16 sum_weight
+    const unsigned lane = threadIdx.x & 63, role = threadIdx.x >> 6;
+    const unsigned t = blockIdx.x * DSM_LM_PTS + lane;
+    const unsigned mx = (unsigned)d.mx, my = (unsigned)d.my;
+    const int i = (int)(t % mx), j = (int)((t / mx) % my), k = (int)(t / (mx * my));
+    const bool act = t < (unsigned)d.P && interior(d, i, j, k) && ld32(nvert, t) <= 1.1;
+    if (act) {
+        const size_t Pz = (size_t)d.P;
+        if (role < 3) {   // <U_b u_a>, a = 0..2, for b = role, and the S|S| plane b
+            const double *Upl = Ub + role * Pz, *u0p = uq, *u1p = uq + Pz, *u2p = uq + 2 * Pz, *sp = ss + role * Pz;
+            Oct v[4];
+            OCT_FOR
+            {
+                const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+                const double wn = ld32(wt, n), U = ld32(Upl, n);
+                const double u0 = ld32(u0p, n), u1 = ld32(u1p, n), u2 = ld32(u2p, n);
+                oct_add(v[0], R, Q, P, U * u0 * wn);
+                oct_add(v[1], R, Q, P, U * u1 * wn);
+                oct_add(v[2], R, Q, P, U * u2 * wn);
+                oct_add(v[3], R, Q, P, ld32(sp, n) * wn);
+            }
+#pragma unroll
+            for (int a = 0; a < 3; a++) sh[3 * role + a][lane] = oct_total(v[a]);
+            sh[9 + role][lane] = oct_total(v[3]);
+        } else {          // S|S| planes 3-5 and the weight sums
+            const double *p3 = ss + 3 * Pz, *p4 = p3 + Pz, *p5 = p4 + Pz;
+            Oct v[3], w;
+            OCT_FOR
+            {
+                const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+                const double wn = ld32(wt, n);
+                oct_add(v[0], R, Q, P, ld32(p3, n) * wn);
+                oct_add(v[1], R, Q, P, ld32(p4, n) * wn);
+                oct_add(v[2], R, Q, P, ld32(p5, n) * wn);
+                oct_add(w, R, Q, P, wn);
+            }
+#pragma unroll
+            for (int q = 0; q < 3; q++) sh[12 + q][lane] = oct_total(v[q]);
+            sh[15][lane] = oct_total(w);
+            // les.c:380-393: sum of weight * coef over p (i), q (j), r (k)
+            double sum_weight = 0;
+#pragma unroll
+            for (int pp = 0; pp < 3; pp++)
+#pragma unroll
+                for (int q = 0; q < 3; q++)
+#pragma unroll
+                    for (int r = 0; r < 3; r++) {
+                        const double cf = 0.125 * (double)((1 + (r == 1)) * (1 + (q == 1)) * (1 + (pp == 1)));
+                        sum_weight += ld32(wt, t + (unsigned)slot_off(d, r, q, pp)) * cf;
+                    }
+            sh[16][lane] = sum_weight;
+        }
+    }
+    __syncthreads();
+    if (role != 0 || t >= (unsigned)d.P) return;
+    if (!act) {
         st32(LM, t, 0.0);
         st32(MM, t, 0.0);
         return;
     }
     const double *met[3] = {g.csi, g.eta, g.zet};
-    // les.c:380-393: sum of weight * coef over p (i), q (j), r (k)
-    double sum_weight = 0;
-#pragma unroll
-    for (int p = 0; p < 3; p++)
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-#pragma unroll
-            for (int r = 0; r < 3; r++) {
-                const double cf = 0.125 * (double)((1 + (r == 1)) * (1 + (q == 1)) * (1 + (p == 1)));
-                sum_weight += ld32(wt, t + (unsigned)slot_off(d, r, q, p)) * cf;
-            }
+    const double ws = sh[15][lane];
     const double filter = pow(1. / ld32(g.aj, t), 1. / 3.);
-    const double test_filter = pow(sum_weight, 1. / 3.);
+    const double test_filter = pow(sh[16][lane], 1. / 3.);
     const double ufc[3] = {ld32(uf, 3 * t), ld32(uf, 3 * t + 1), ld32(uf, 3 * t + 2)};
-    // L_ba = <U_b u_a> - U_b(<u>) <u>_a with U_b = u . (csi, eta, zet)_b, then
-    // <S_ab |S|>: five streams over the box, three quantities each
-    double L[3][3], ws = 0;
+    double L[3][3], SSh[3][3];
 #pragma unroll
     for (int b = 0; b < 3; b++) {
-        Oct v[3], w;
-        OCT_FOR_RT
-        {
-            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
-            const double wn = ld32(wt, n), u0 = ld32(ucat, 3 * n), u1 = ld32(ucat, 3 * n + 1), u2 = ld32(ucat, 3 * n + 2);
-            const double U = u0 * ld32(met[b], 3 * n) + u1 * ld32(met[b], 3 * n + 1) + u2 * ld32(met[b], 3 * n + 2);
-            if (b == 0) oct_add(w, R, Q, P, wn);
-            oct_add(v[0], R, Q, P, U * u0 * wn);
-            oct_add(v[1], R, Q, P, U * u1 * wn);
-            oct_add(v[2], R, Q, P, U * u2 * wn);
-        }
-        if (b == 0) ws = oct_total(w);
         const double Uf = ufc[0] * ld32(met[b], 3 * t) + ufc[1] * ld32(met[b], 3 * t + 1) + ufc[2] * ld32(met[b], 3 * t + 2);
 #pragma unroll
-        for (int a = 0; a < 3; a++) L[b][a] = oct_total(v[a]) / ws - Uf * ufc[a];
+        for (int a = 0; a < 3; a++) L[b][a] = sh[3 * b + a][lane] / ws - Uf * ufc[a];
     }
-    double SSh[3][3];
 #pragma unroll
-    for (int h = 0; h < 2; h++) {   // planes 0-2, then 3-5
-        Oct v[3];
-        const double *pl[3] = {ss + (size_t)(3 * h) * d.P, ss + (size_t)(3 * h + 1) * d.P, ss + (size_t)(3 * h + 2) * d.P};
-        OCT_FOR_RT
-        {
-            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
-            const double wn = ld32(wt, n);
+    for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int q = 0; q < 3; q++) oct_add(v[q], R, Q, P, ld32(pl[q], n) * wn);
-        }
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int pn = 3 * h + q, a = pn < 3 ? 0 : (pn < 5 ? 1 : 2), b = pn < 3 ? pn : (pn < 5 ? pn - 2 : 2);
-            SSh[a][b] = SSh[b][a] = oct_total(v[q]) / ws;
-        }
-    }
+        for (int b = a; b < 3; b++) SSh[a][b] = SSh[b][a] = sh[9 + ss_plane(a, b)][lane] / ws;
     double dc[3], de[3], dz[3], D[3][3], Sh[3][3];
     du_center(d, i, j, k, uf, nvert, dc, de, dz);
     du_dxyz(g, t, dc, de, dz, D);
@@ -315,13 +380,24 @@ __global__ void __launch_bounds__(256) k_dsm_cs(Dims d, const double *__restrict
         return;
     }
     Oct l, m, w;
-    OCT_FOR
-    {
-        unsigned src;
-        const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
-        oct_add(w, R, Q, P, wn);
-        oct_add(l, R, Q, P, ld32(LM, src) * wn);
-        oct_add(m, R, Q, P, ld32(MM, src) * wn);
+    if (away_from_ghosts(d, i, j, k)) {
+        OCT_FOR
+        {
+            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+            const double wn = ld32(wt, n);
+            oct_add(w, R, Q, P, wn);
+            oct_add(l, R, Q, P, ld32(LM, n) * wn);
+            oct_add(m, R, Q, P, ld32(MM, n) * wn);
+        }
+    } else {
+        OCT_FOR_RT
+        {
+            unsigned src;
+            const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
+            oct_add(w, R, Q, P, wn);
+            oct_add(l, R, Q, P, ld32(LM, src) * wn);
+            oct_add(m, R, Q, P, ld32(MM, src) * wn);
+        }
     }
     const double ws = oct_total(w);
     const double C = 0.5 * (oct_total(l) / ws) / (oct_total(m) / ws + VFS_LES_EPS);
@@ -388,15 +464,18 @@ extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, int les,
         VFS_CHECK_LAUNCH();
         return 0;
     }
-    if (!c->dsm) VFS_CHECK_HIP(hipMalloc(&c->dsm, (size_t)13 * d.P * sizeof(double)));
-    double *ss = c->dsm, *uf = ss + 6 * d.P, *LM = uf + 3 * d.P, *MM = LM + d.P, *cst = MM + d.P, *wt = cst + d.P;
+    if (!c->dsm) VFS_CHECK_HIP(hipMalloc(&c->dsm, (size_t)19 * d.P * sizeof(double)));
+    const size_t P = (size_t)d.P;
+    double *ss = c->dsm, *uf = ss + 6 * P, *LM = uf + 3 * P, *MM = LM + P, *cst = MM + P, *wt = cst + P;
+    double *uq = wt + P, *Ub = uq + 3 * P;
     const dim3 blk(256), grd(grid_for(d.P, 256));
-    hipLaunchKernelGGL(k_dsm_weight, grd, blk, 0, c->stream, d, c->g.aj, c->nvert, wt);
+    hipLaunchKernelGGL(k_dsm_point, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, uq, Ub);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_dsm_grad, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, ss, uf);
+    hipLaunchKernelGGL(k_dsm_grad, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, ss, 6) || vfs_periodic_vec(c, uf)) return -1;
-    hipLaunchKernelGGL(k_dsm_lm, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, ss, uf, LM, MM);
+    hipLaunchKernelGGL(k_dsm_lm, dim3(grid_for(d.P, DSM_LM_PTS)), dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub,
+                       wt, ss, uf, LM, MM);
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, LM, 2)) return -1;   // LM and MM are adjacent planes
     hipLaunchKernelGGL(k_dsm_cs, grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, max_cs, cst);
