@@ -102,6 +102,36 @@ int vfs_eddy_viscosity_cs(vfs_ctx *ctx, const double *ucat, const double *cs, do
  * doubles a point is allocated on the first dynamic call and kept. */
 int vfs_smagorinsky_constant(vfs_ctx *ctx, const double *ucat, int les, int ti, int tistart, int rstart,
                              double max_cs, double *cs);
+/* ---- actuator models (rotor_model.c) ------------------------------------
+ * Element arrays are device arrays of n values in blocks: cent x|y|z (3n),
+ * f_lagr x|y|z (3n), dA (n), box i_min|i_max|j_min|j_max|k_min|k_max (6n,
+ * upper bounds exclusive, the reference's IBMNodes fields), dh x|y|z (3n).
+ * coor: the node coordinates given to vfs_build_metrics. */
+/* The element search of the rotor pre-processing (rotor_model.c:2590-2737):
+ * the closest grid node of each element centre (every 20th node first, then
+ * a +-21 window; the first minimum in i, j, k loop order), the grid widths
+ * there, and the index box closest node +- itpwidth (the reference's
+ * Itpwidth, (int)halfwidth_dfunc + 2 = 6) clamped to the interior. */
+int vfs_rotor_elements(vfs_ctx *ctx, const double *coor, int n, const double *cent, int itpwidth, int *box,
+                       double *dh);
+/* Calc_F_eul (rotor_model.c:4792-5137): f_eul (3P, every point written) =
+ * sum over elements, in order, of F_lagr dA times the discrete delta
+ * function of each face centre's offset (covariant directions, scaled by the
+ * widths), projected on csi / eta / zet; zero where the 3x3x3 nvert sum
+ * exceeds 2.9.  deltafunc: 0 dfunc_2h, 6 dfunc_nh(halfwidth), 7
+ * dfunc_exp(halfwidth), 8 dfunc_s4h, otherwise dfunc_sc4h (the reference's
+ * default 10).  width_mode 0: widths from the cell metrics (default); 1:
+ * dh_fixed (3 host doubles, -forcewidthfixed); 2: dA, 3: sqrt(dA)
+ * (-forcedistrwidth_surfacescale).  Boxes must lie in the interior, as
+ * vfs_rotor_elements makes them.  Synchronizes the context's stream. */
+int vfs_calc_f_eul(vfs_ctx *ctx, const double *coor, int n, const double *cent, const double *f_lagr,
+                   const double *dA, const int *box, int deltafunc, double halfwidth, int width_mode,
+                   const double *dh_fixed, double *f_eul);
+/* Attach F_eul (3P doubles on the device, kept by the caller) to
+ * vfs_formfunction_snes and the Newton-Krylov residual, which add it last
+ * (momentum.c:2288, nacelle_model / rotor_model); NULL detaches. */
+int vfs_set_body_force(vfs_ctx *ctx, const double *f_eul);
+
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

@@ -1419,6 +1419,16 @@ int vo_formfunction_snes(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          const double *dP, const double *nu_t, double *lucont,
                          double *ucat, double *rhs)
 {
+    return vo_formfunction_snes_f(c, g, nvert, U, U_o, rhs_o, dP, nu_t, NULL, lucont, ucat, rhs);
+}
+
+/* the same with the body force of the rotor/nacelle models added last
+ * (momentum.c:2288); f_eul NULL: none */
+int vo_formfunction_snes_f(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                           const double *U, const double *U_o, const double *rhs_o,
+                           const double *dP, const double *nu_t, const double *f_eul, double *lucont,
+                           double *ucat, double *rhs)
+{
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
@@ -1452,9 +1462,227 @@ int vo_formfunction_snes(const vo_cfg *c, const vo_geom *g, const double *nvert,
     for (size_t n = 0; n < 3 * P; n++) {
         rhs[n] += 0.5 * rhs_o[n];
         rhs[n] += -1 * dP[n];
+        if (f_eul) rhs[n] += 1. * f_eul[n];
     }
     free(Ug);
     return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Actuator models: the element search of Pre_process (rotor_model.c:       */
+/* 2540-2740) and the Eulerian force spreading Calc_F_eul (4792-5137) with  */
+/* its discrete delta functions (6042-6200)                                 */
+/* ------------------------------------------------------------------------ */
+#define VO_PI 3.14159265   /* the truncated pi of dfunc_s4h (rotor_model.c:6104) */
+
+static double dfunc_2h(double r) { return fabs(r) < 1.0 ? 1.0 - fabs(r) : 0.0; }   /* 6085-6095 */
+
+static double dfunc_s4h(double r)   /* 6097-6115 */
+{
+    if (fabs(r) <= 0.5) return 3.0 / 8.0 + VO_PI / 32.0 - pow(r, 2) / 4.0;
+    if (fabs(r) >= 0.5 && fabs(r) <= 1.5)
+        return 1.0 / 4.0 + (1.0 - fabs(r)) * sqrt(-2.0 + 8.0 * fabs(r) - 4.0 * pow(r, 2)) / 8.0 -
+               asin(sqrt(2.0) * (fabs(r) - 1.0)) / 8.0;
+    if (fabs(r) >= 1.5 && fabs(r) <= 2.5)
+        return 17.0 / 16.0 - VO_PI / 64.0 - 3.0 * fabs(r) / 4.0 + pow(r, 2) / 8.0 +
+               (fabs(r) - 2.0) * sqrt(-14.0 + 16.0 * fabs(r) - 4.0 * pow(r, 2)) / 16.0 +
+               asin(sqrt(2.0) * (fabs(r) - 2.0)) / 16.0;
+    return 0.0;
+}
+
+static double dfunc_sc4h(double r)   /* 6117-6133 (M_PI) */
+{
+    const double pi = 3.14159265358979323846;
+    if (fabs(r) <= 1.5)
+        return 0.25 * (pi + 2 * sin(0.25 * pi * (2 * fabs(r) + 1)) - 2 * sin(0.25 * pi * (2 * fabs(r) - 1))) / pi;
+    if (fabs(r) >= 1.5 && fabs(r) <= 2.5)
+        return -0.125 * (-5 * pi + 2 * pi * fabs(r) + 4 * sin(0.25 * pi * (2 * fabs(r) - 1))) / pi;
+    return 0.0;
+}
+
+static double dfunc_nh(double r, double n) { return fabs(r) < n ? (n - fabs(r)) / pow(n, 2) : 0.0; }   /* 6183-6191 */
+
+static double dfunc_exp(double r, double n)   /* 6193-6197 */
+{
+    return exp(-(r / n) * (r / n)) / (pow(n, 1) * pow(3.1415926, 0.5));
+}
+
+double vo_dfunc(int df, double r, double n)
+{
+    switch (df) {
+    case 0: return dfunc_2h(r);
+    case 6: return dfunc_nh(r, n);
+    case 7: return dfunc_exp(r, n);
+    case 8: return dfunc_s4h(r);
+    default: return dfunc_sc4h(r);
+    }
+}
+
+/* rotor_model.c:2590-2737 for one block on one rank: the closest grid node
+ * of each element centre (every 20th node first, then a +-21 window; the
+ * first minimum in i, j, k loop order), the grid widths there, and the
+ * element's index box, closest node +- itpwidth clamped to the interior
+ * (upper bounds exclusive).  cent: x[n], y[n], z[n]; box: i_min[n], i_max[n],
+ * j_min[n], j_max[n], k_min[n], k_max[n]; dh: dhx[n], dhy[n], dhz[n]. */
+int vo_rotor_elements(const vo_cfg *c, const vo_geom *g, const double *coor, int n, const double *cent, int itpwidth,
+                      int *box, double *dh)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const int lxs = 1, lxe = mx - 1, lys = 1, lye = my - 1, lzs = 1, lze = mz - 1;
+    for (int l = 0; l < n; l++) {
+        const double cx = cent[l], cy = cent[n + l], cz = cent[2 * n + l];
+        double dmin = 1.e6;
+        int icc = lxs, jcc = lys, kcc = lzs;
+        for (int i = lxs; i < lxe; i = i + 20)
+            for (int j = lys; j < lye; j = j + 20)
+                for (int k = lzs; k < lze; k = k + 20) {
+                    const size_t id = IX(k, j, i);
+                    double r1 = cx - VX(coor, id), r2 = cy - VY(coor, id), r3 = cz - VZ(coor, id);
+                    double d1 = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+                    if (d1 < dmin) { dmin = d1; icc = i; jcc = j; kcc = k; }
+                }
+        double dm = 1.e13;
+        int ic = 0, jc = 0, kc = 0;
+        for (int i = (icc - 21 > lxs ? icc - 21 : lxs); i < (icc + 21 < lxe ? icc + 21 : lxe); i++)
+            for (int j = (jcc - 21 > lys ? jcc - 21 : lys); j < (jcc + 21 < lye ? jcc + 21 : lye); j++)
+                for (int k = (kcc - 21 > lzs ? kcc - 21 : lzs); k < (kcc + 21 < lze ? kcc + 21 : lze); k++) {
+                    const size_t id = IX(k, j, i);
+                    double r1 = cx - VX(coor, id), r2 = cy - VY(coor, id), r3 = cz - VZ(coor, id);
+                    double d1 = sqrt(r1 * r1 + r2 * r2 + r3 * r3);
+                    if (d1 < dm) { dm = d1; ic = i; jc = j; kc = k; }
+                }
+        const size_t id = IX(kc, jc, ic);
+        const double *a = g->csi + 3 * id, *b = g->eta + 3 * id, *e = g->zet + 3 * id;
+        double area = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+        dh[l] = 1.0 / g->aj[id] / area;
+        area = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+        dh[n + l] = 1.0 / g->aj[id] / area;
+        area = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+        dh[2 * n + l] = 1.0 / g->aj[id] / area;
+        const int cc[3] = {ic, jc, kc}, lo[3] = {lxs, lys, lzs}, hi[3] = {lxe, lye, lze};
+        for (int q = 0; q < 3; q++) {
+            const int c1 = cc[q] - itpwidth, c2 = cc[q] + itpwidth;
+            if (c1 > hi[q] || c2 < lo[q]) {
+                box[2 * q * n + l] = lo[q];
+                box[(2 * q + 1) * n + l] = lo[q];
+            } else {
+                box[2 * q * n + l] = c1 > lo[q] ? c1 : lo[q];
+                box[(2 * q + 1) * n + l] = c2 < hi[q] ? c2 : hi[q];
+            }
+        }
+    }
+    return 0;
+}
+
+/* rotor_model.c:4792-5137 Calc_F_eul for one block on one rank:
+ * f_eul (zeroed first) += F_lagr dA times the delta function of each face
+ * centre's offset from the element centre, in the covariant directions and
+ * scaled by the widths, projected on csi / eta / zet; elements in order,
+ * each over its index box.  Then f_eul = 0 where the 3x3x3 nvert sum
+ * (clamped to the interior) exceeds 2.9, and the ghost-to-first-cell folds,
+ * which add the (zero) ghost values.  width_mode 0: widths from the cell
+ * metrics; 1: dh_fixed[3] (-forcewidthfixed); 2: dA; 3: sqrt(dA)
+ * (-forcedistrwidth_surfacescale, rotor_model 5). */
+int vo_calc_f_eul(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor, int n,
+                  const double *cent, const double *F, const double *dA, const int *box, int df, double halfwidth,
+                  int width_mode, const double *dh_fixed, double *f_eul)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    memset(f_eul, 0, 3 * P * sizeof(double));
+    for (int l = 0; l < n; l++) {
+        const double cx = cent[l], cy = cent[n + l], cz = cent[2 * n + l];
+        const double Fx = F[l], Fy = F[n + l], Fz = F[2 * n + l];
+        for (int k = box[4 * n + l]; k < box[5 * n + l]; k++)
+            for (int j = box[2 * n + l]; j < box[3 * n + l]; j++)
+                for (int i = box[l]; i < box[n + l]; i++) {
+#define CO(kk, jj, ii, q) coor[3 * IX(kk, jj, ii) + (q)]
+                    double xi[3], xj[3], xk[3];
+                    for (int q = 0; q < 3; q++) {
+                        xi[q] = (CO(k, j, i, q) + CO(k - 1, j, i, q) + CO(k, j - 1, i, q) + CO(k - 1, j - 1, i, q)) * 0.25;
+                        xj[q] = (CO(k, j, i, q) + CO(k - 1, j, i, q) + CO(k, j, i - 1, q) + CO(k - 1, j, i - 1, q)) * 0.25;
+                        xk[q] = (CO(k, j, i, q) + CO(k, j - 1, i, q) + CO(k, j, i - 1, q) + CO(k, j - 1, i - 1, q)) * 0.25;
+                    }
+#undef CO
+                    const size_t id = IX(k, j, i);
+                    const double *cs = g->csi + 3 * id, *et = g->eta + 3 * id, *ze = g->zet + 3 * id;
+                    double ni[3], nj[3], nk[3];
+                    calc_normal(cs, et, ze, ni, nj, nk);
+                    double dhx, dhy, dhz;
+                    if (width_mode == 1) {
+                        dhx = dh_fixed[0]; dhy = dh_fixed[1]; dhz = dh_fixed[2];
+                    } else if (width_mode == 2 || width_mode == 3) {
+                        dhx = width_mode == 3 ? sqrt(dA[l]) : dA[l];
+                        dhy = dhx; dhz = dhx;
+                    } else {
+                        double area = sqrt(cs[0] * cs[0] + cs[1] * cs[1] + cs[2] * cs[2]);
+                        dhx = 1.0 / g->aj[id] / area;
+                        area = sqrt(et[0] * et[0] + et[1] * et[1] + et[2] * et[2]);
+                        dhy = 1.0 / g->aj[id] / area;
+                        area = sqrt(ze[0] * ze[0] + ze[1] * ze[1] + ze[2] * ze[2]);
+                        dhz = 1.0 / g->aj[id] / area;
+                    }
+                    const double rx[3] = {xi[0] - cx, xj[0] - cx, xk[0] - cx};
+                    const double ry[3] = {xi[1] - cy, xj[1] - cy, xk[1] - cy};
+                    const double rz[3] = {xi[2] - cz, xj[2] - cz, xk[2] - cz};
+                    const double vol_eul = 1.0 / (dhx * dhy * dhz);
+                    double dfn[3];
+                    for (int f = 0; f < 3; f++) {
+                        const double r1 = fabs(rx[f] * ni[0] + ry[f] * ni[1] + rz[f] * ni[2]) / dhx;
+                        const double r2 = fabs(rx[f] * nj[0] + ry[f] * nj[1] + rz[f] * nj[2]) / dhy;
+                        const double r3 = fabs(rx[f] * nk[0] + ry[f] * nk[1] + rz[f] * nk[2]) / dhz;
+                        dfn[f] = vol_eul * vo_dfunc(df, r1, halfwidth) * vo_dfunc(df, r2, halfwidth) *
+                                 vo_dfunc(df, r3, halfwidth);
+                    }
+                    const double *m[3] = {cs, et, ze};
+                    for (int f = 0; f < 3; f++)
+                        f_eul[3 * id + f] += Fx * dfn[f] * dA[l] * m[f][0] + Fy * dfn[f] * dA[l] * m[f][1] +
+                                             Fz * dfn[f] * dA[l] * m[f][2];
+                }
+    }
+    /* 5006-5020: no force where the 3x3x3 box (within the interior) is
+     * mostly solid */
+    double *tmp = malloc(3 * P * sizeof(double));
+    if (!tmp) return -9;
+    memcpy(tmp, f_eul, 3 * P * sizeof(double));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                double nv = 0.0;
+                for (int kk = (k - 1 > 1 ? k - 1 : 1); kk < (k + 2 < mz - 1 ? k + 2 : mz - 1); kk++)
+                    for (int jj = (j - 1 > 1 ? j - 1 : 1); jj < (j + 2 < my - 1 ? j + 2 : my - 1); jj++)
+                        for (int ii = (i - 1 > 1 ? i - 1 : 1); ii < (i + 2 < mx - 1 ? i + 2 : mx - 1); ii++)
+                            nv += nvert[IX(kk, jj, ii)];
+                if (nv > 2.9) tmp[3 * IX(k, j, i)] = tmp[3 * IX(k, j, i) + 1] = tmp[3 * IX(k, j, i) + 2] = 0.0;
+            }
+    memcpy(f_eul, tmp, 3 * P * sizeof(double));
+    free(tmp);
+    /* 5021-5118: the ghost-to-first-cell folds, in the reference's order */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                int a = i, b = j, cc = k;
+                const int tests[6] = {i == 1, i == mx - 2, j == 1, j == my - 2, k == 1, k == mz - 2};
+                for (int t = 0; t < 6; t++) {
+                    if (!tests[t]) continue;
+                    if (t == 0) a = 0;
+                    else if (t == 1) a = mx - 1;
+                    else if (t == 2) b = 0;
+                    else if (t == 3) b = my - 1;
+                    else if (t == 4) cc = 0;
+                    else cc = mz - 1;
+                    const size_t d0 = IX(k, j, i), s0 = IX(cc, b, a);
+                    for (int q = 0; q < 3; q++) {
+                        f_eul[3 * d0 + q] += f_eul[3 * s0 + q];
+                        f_eul[3 * s0 + q] = 0.0;
+                    }
+                }
+            }
+    return 0;
 }
 
 /* momentum.c:104-195 Calc_Minimum_dt */

@@ -128,6 +128,26 @@ int vo_formfunction_snes(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          const double *dP, const double *nu_t, double *lucont,
                          double *ucat, double *rhs);
 
+/* The same with the rotor/nacelle body force added last (momentum.c:2288);
+ * f_eul NULL: none. */
+int vo_formfunction_snes_f(const vo_cfg *c, const vo_geom *g, const double *nvert,
+                           const double *U, const double *U_o, const double *rhs_o,
+                           const double *dP, const double *nu_t, const double *f_eul, double *lucont,
+                           double *ucat, double *rhs);
+
+/* Actuator models.  rotor_model.c:6042-6200 discrete delta functions
+ * (df 0: dfunc_2h, 6: dfunc_nh(n), 7: dfunc_exp(n), 8: dfunc_s4h, other:
+ * dfunc_sc4h); rotor_model.c:2590-2737 element search (closest node, widths,
+ * index boxes); rotor_model.c:4792-5137 Calc_F_eul.  Element arrays are
+ * blocks of n: cent x|y|z, F_lagr x|y|z, box i_min|i_max|j_min|j_max|k_min|k_max,
+ * dh x|y|z. */
+double vo_dfunc(int df, double r, double n);
+int vo_rotor_elements(const vo_cfg *c, const vo_geom *g, const double *coor, int n, const double *cent, int itpwidth,
+                      int *box, double *dh);
+int vo_calc_f_eul(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor, int n,
+                  const double *cent, const double *F, const double *dA, const int *box, int df, double halfwidth,
+                  int width_mode, const double *dh_fixed, double *f_eul);
+
 /* momentum.c:104-195 Calc_Minimum_dt: CFL-1 time step. */
 double vo_min_dt(const vo_cfg *c, const vo_geom *g, const double *ucont);
 

@@ -138,7 +138,8 @@ struct vfs_ctx {
     void *nk;                // Newton-Krylov workspace (allocated on first use)
     double *wf_table;        // wall-function y+ integral table (bctype -1 only)
     double *ustar;           // P: friction velocity of the wall-function cells
-    double *dsm;             // 12*P: dynamic Smagorinsky scratch (allocated on first use)
+    double *dsm;             // 19*P: dynamic Smagorinsky scratch (allocated on first use)
+    const double *body_force;   // F_eul added to the implicit residual (caller's array, may be null)
     int red_blocks;
     Poisson *pois;
 };

@@ -449,13 +449,17 @@ __global__ void k_snes_init(Dims d, const double *__restrict__ U, const double *
     }
 }
 
-__global__ void k_snes_final(long long n, const double *__restrict__ rhs_o, const double *__restrict__ dP, double *rhs)
+// rhs += 0.5 rhs_o; rhs -= dP; rhs += F_eul when a body force is attached
+// (momentum.c:2277-2288)
+__global__ void k_snes_final(long long n, const double *__restrict__ rhs_o, const double *__restrict__ dP,
+                             const double *__restrict__ f_eul, double *rhs)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     double r = rhs[t];
     r += 0.5 * rhs_o[t];
     r += -1 * dP[t];
+    if (f_eul) r += 1. * f_eul[t];
     rhs[t] = r;
 }
 
@@ -468,7 +472,8 @@ extern "C" int vfs_formfunction_snes(vfs_ctx *c, const double *U, const double *
     if (vfs_contra2cart(c, lucont, ucat)) return -1;
     if (vfs_ib_bc(c, lucont, ucat)) return -1;
     if (vfs_formfunction2(c, lucont, ucat, nu_t, rhs, 0.5)) return -1;
-    hipLaunchKernelGGL(k_snes_final, dim3(grid_for(3 * d.P, 256)), dim3(256), 0, c->stream, 3 * d.P, rhs_o, dP, rhs);
+    hipLaunchKernelGGL(k_snes_final, dim3(grid_for(3 * d.P, 256)), dim3(256), 0, c->stream, 3 * d.P, rhs_o, dP,
+                       c->body_force, rhs);
     VFS_CHECK_LAUNCH();
     return 0;
 }

@@ -182,6 +182,26 @@ class Ctx:
         """nu_t from a per-cell Cs field (DeviceArray of P doubles)."""
         _ck(lib().vfs_eddy_viscosity_cs(self.h, ucat.ptr, cs.ptr, nu_t.ptr), "vfs_eddy_viscosity_cs")
 
+    def rotor_elements(self, coor, cent, itpwidth=6):
+        """cent: DeviceArray of 3n doubles (x|y|z blocks) -> (box, dh) as
+        DeviceArrays (box holds 6n int32 values)."""
+        n = cent.n // 3
+        box = DeviceArray(6 * n, np.int32)
+        dh = DeviceArray(3 * n)
+        _ck(lib().vfs_rotor_elements(self.h, coor.ptr, int(n), cent.ptr, int(itpwidth), box.ptr, dh.ptr),
+            "vfs_rotor_elements")
+        return box, dh
+
+    def calc_f_eul(self, coor, cent, f_lagr, dA, box, f_eul, deltafunc=10, halfwidth=4.0, width_mode=0,
+                   dh_fixed=None):
+        fx = None if dh_fixed is None else (C.c_double * 3)(*dh_fixed)
+        _ck(lib().vfs_calc_f_eul(self.h, coor.ptr, int(cent.n // 3), cent.ptr, f_lagr.ptr, dA.ptr, box.ptr,
+                                 int(deltafunc), C.c_double(halfwidth), int(width_mode), fx, f_eul.ptr),
+            "vfs_calc_f_eul")
+
+    def set_body_force(self, f_eul):
+        _ck(lib().vfs_set_body_force(self.h, None if f_eul is None else f_eul.ptr), "vfs_set_body_force")
+
     def smagorinsky_constant(self, ucat, cs, les=2, ti=2, tistart=0, rstart=0, max_cs=0.5):
         """Compute_Smagorinsky_Constant_1 (les.c:67-1133) into cs: -les 1
         (0.01) or 2 (dynamic procedure); 0 while ti < 2 on a fresh start."""
