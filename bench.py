@@ -14,8 +14,9 @@ Prints one JSON line.  value = RHS throughput in Gcell/s of Formfunction_2
 stream); the Poisson time-to-1e-8 in FP64 and in mixed FP32/FP64, the step
 times of both modes, the roofline of the dominant kernel and the implicit
 momentum solve at CFL 2 (Newton-Krylov with the exact JVP, plus the JVP and
-VJP times) and the dynamic Smagorinsky procedure (-les 2) with its eddy
-viscosity are reported in the same line.
+VJP times), the dynamic Smagorinsky procedure (-les 2) with its eddy
+viscosity and the actuator-disk force spreading are reported in the same
+line.
 """
 import argparse
 import ctypes as C
@@ -184,11 +185,12 @@ def implicit_leg(ctx, cfg, spec, s):
             "solve_ms": round((t1 - t0) * 1e3, 2), "jvp_ms": round(jvp_ms, 3), "vjp_ms": round(vjp_ms, 3)}
 
 
-# the dynamic procedure's compulsory traffic a point, by kernel: weight
-# (aj, nvert -> w) 24, gradient pass 192, Germano pass 208, Cs pass 40,
-# smoothing 32; the eddy viscosity from the Cs field 128
-DSM_BYTES_PER_CELL = 496
-NUT_BYTES_PER_CELL = 128
+# the dynamic procedure's algorithmic bytes a point: ucat, the cell metrics,
+# aj and nvert read once, Cs written once (every intermediate kept on chip)
+DSM_BYTES_PER_CELL = 120
+# what the five kernels of this build move a point (each array once): staging
+# 168, gradient pass 216, Germano pass 232, Cs pass 40, smoothing 32
+DSM_STAGED_BYTES_PER_CELL = 688
 
 
 def dynamic_les_leg(ctx, cfg, s, reps, cpu_budget):
@@ -220,6 +222,8 @@ def dynamic_les_leg(ctx, cfg, s, reps, cpu_budget):
            "bytes_per_point": DSM_BYTES_PER_CELL,
            "achieved_gbs": round(DSM_BYTES_PER_CELL * P / (dsm_ms * 1e-3) / 1e9, 1),
            "frac_hbm": round(DSM_BYTES_PER_CELL * P / (dsm_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "staged_bytes_per_point": DSM_STAGED_BYTES_PER_CELL,
+           "staged_gbs": round(DSM_STAGED_BYTES_PER_CELL * P / (dsm_ms * 1e-3) / 1e9, 1),
            "cs_max": round(float(h.max()), 4), "cs_nonzero_frac": round(float((h > 0).mean()), 3)}
     if cpu_budget > 0:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -238,6 +242,51 @@ def dynamic_les_leg(ctx, cfg, s, reps, cpu_budget):
             reps_c += 1
         out["cpu_oracle_gcells"] = round(N[0] * N[1] * N[2] * reps_c / t / 1e9, 5)
         out["cpu_sample"] = f"{reps_c} calls of the oracle at 128x64x64 cells, one thread ({t:.1f} s)"
+    return out
+
+
+def actuator_leg(ctx, cfg, spec, reps, with_cpu):
+    """The actuator-disk force on the full grid: a ring of 512 elements
+    (radius 0.5 about (Lx/4, 1, Lz/2), normal to x, thrust -1 per unit area)
+    spread with the Gaussian delta function (deltafunc 7, halfwidth 4, the
+    reference's default width): element search and Calc_F_eul times."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    n = 512
+    th = 2 * np.pi * np.arange(n) / n
+    cent = np.stack([np.full(n, W.LX / 4), 1.0 + 0.5 * np.cos(th), W.LZ / 2 + 0.5 * np.sin(th)])
+    F = np.stack([-np.ones(n), np.zeros(n), np.zeros(n)])
+    dA = np.full(n, 2 * np.pi * 0.5 / n * 0.1)
+    coor = W.grid(spec)
+    dcoor, dcent = V.DeviceArray.from_host(coor), V.DeviceArray.from_host(cent.reshape(-1))
+    dF, ddA, f = V.DeviceArray.from_host(F.reshape(-1)), V.DeviceArray.from_host(dA), ctx.vec()
+    box, dh = ctx.rotor_elements(dcoor, dcent)
+    ctx.calc_f_eul(dcoor, dcent, dF, ddA, box, f, deltafunc=7, halfwidth=4.0)
+    ctx.sync()
+    ev = HipEvents()
+    a, b = ev.event(), ev.event()
+    ev.record(a)
+    for _ in range(reps):
+        ctx.rotor_elements(dcoor, dcent)
+    ev.record(b)
+    search_ms = ev.elapsed_ms(a, b) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.calc_f_eul(dcoor, dcent, dF, ddA, box, f, deltafunc=7, halfwidth=4.0)
+    ctx.sync()
+    spread_ms = (time.perf_counter() - t0) / reps * 1e3
+    h = f.host()
+    out = {"elements": n, "search_ms": round(search_ms, 4), "spread_ms": round(spread_ms, 4),
+           "nonzero_points": int(np.count_nonzero(h.reshape(-1, 3).any(axis=1)))}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_py import Oracle
+        o = Oracle(cfg, coor)
+        o.nvert_sphere(*W.SPHERE)
+        t0 = time.perf_counter()
+        obox, _ = o.rotor_elements(coor, cent)
+        o.calc_f_eul(coor, cent, F, dA, obox, df=7, halfwidth=4.0)
+        out["cpu_oracle_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
     return out
 
 
@@ -316,6 +365,7 @@ def main():
     ap.add_argument("--calib-reps", type=int, default=5)
     ap.add_argument("--no-implicit", action="store_true", help="skip the Newton-Krylov leg")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic Smagorinsky leg")
+    ap.add_argument("--no-actuator", action="store_true", help="skip the actuator-disk leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -381,6 +431,11 @@ def main():
         dyn = dynamic_les_leg(ctx, cfg, s, 5, 0.0 if args.no_cpu else 5.0)
         log(f"[bench] dynamic Smagorinsky: {dyn}")
 
+    act = None
+    if not args.no_actuator:
+        act = actuator_leg(ctx, cfg, spec, 5, not args.no_cpu)
+        log(f"[bench] actuator disk: {act}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -418,6 +473,7 @@ def main():
                      "kernel": "Formfunction_2", "bytes_per_cell": bytes_per_cell},
         "implicit_cfl2": implicit,
         "dynamic_les": dyn,
+        "actuator_disk": act,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
