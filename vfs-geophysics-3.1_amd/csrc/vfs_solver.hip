@@ -317,64 +317,41 @@ __global__ void __launch_bounds__(64) k_thomas(LV L, const T *__restrict__ r, co
                                                T omega, T *__restrict__ dscr, T *xn)
 {
     // rows are addressed as (uniform row pointer)[lane], so the loads take a
-    // scalar base and a 32-bit lane offset.  Both sweeps run in batches of B
-    // rows, and the loads of the next batch are issued before the current
-    // batch is computed, so the recurrence waits for memory once per sweep
-    // rather than once per batch.
+    // scalar base and a 32-bit lane offset
     const unsigned i = blockIdx.x * 64u + threadIdx.x;
     if (i >= (unsigned)L.nx) return;
     const long long plane = (long long)blockIdx.y * L.ny * L.nx, sy = L.nx;
-    T rr[B], ii[B], mm[B];
-    auto load_fwd = [&](int j0, T *a, T *b, T *c) {
+    T dp = 0;
+    for (int j0 = 0; j0 < L.ny; j0 += B) {
+        T rr[B], ii[B], mm[B];
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (j0 + q < L.ny) {
                 const long long o = plane + (j0 + q) * sy;
-                a[q] = (r + o)[i]; b[q] = (ti + o)[i]; c[q] = (tm + o)[i];
+                rr[q] = (r + o)[i]; ii[q] = (ti + o)[i]; mm[q] = (tm + o)[i];
             }
-    };
-    load_fwd(0, rr, ii, mm);
-    T dp = 0;
-    for (int j0 = 0; j0 < L.ny; j0 += B) {
-        T nr[B], ni[B], nm[B];
-        const bool more = j0 + B < L.ny;
-        if (more) load_fwd(j0 + B, nr, ni, nm);
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (j0 + q < L.ny) {
                 dp = rr[q] * ii[q] - mm[q] * dp;
                 (dscr + plane + (j0 + q) * sy)[i] = dp;
             }
-        if (more) {
-#pragma unroll
-            for (int q = 0; q < B; q++) { rr[q] = nr[q]; ii[q] = ni[q]; mm[q] = nm[q]; }
-        }
     }
-    T dd[B], uu[B], oo[B];
-    auto load_bwd = [&](int j1, T *a, T *b, T *c) {
+    T xj = 0;
+    for (int j1 = L.ny - 1; j1 >= 0; j1 -= B) {
+        T dd[B], uu[B], oo[B];
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (j1 - q >= 0) {
                 const long long o = plane + (j1 - q) * sy;
-                a[q] = (dscr + o)[i]; b[q] = (tu + o)[i]; c[q] = ZG ? T(0) : (xo + o)[i];
+                dd[q] = (dscr + o)[i]; uu[q] = (tu + o)[i]; oo[q] = ZG ? T(0) : (xo + o)[i];
             }
-    };
-    load_bwd(L.ny - 1, dd, uu, oo);
-    T xj = 0;
-    for (int j1 = L.ny - 1; j1 >= 0; j1 -= B) {
-        T nd[B], nu[B], no[B];
-        const bool more = j1 - B >= 0;
-        if (more) load_bwd(j1 - B, nd, nu, no);
 #pragma unroll
         for (int q = 0; q < B; q++)
             if (j1 - q >= 0) {
                 xj = dd[q] - uu[q] * xj;
                 (xn + plane + (j1 - q) * sy)[i] = oo[q] + omega * xj;
             }
-        if (more) {
-#pragma unroll
-            for (int q = 0; q < B; q++) { dd[q] = nd[q]; uu[q] = nu[q]; oo[q] = no[q]; }
-        }
     }
 }
 
