@@ -92,16 +92,27 @@ int vfs_eddy_viscosity(vfs_ctx *ctx, const double *ucat, double cs, double *nu_t
 /* The same with a per-cell Cs field (user->lCs, les.c:1171-1203): cs, P
  * doubles on the device, as vfs_smagorinsky_constant writes it. */
 int vfs_eddy_viscosity_cs(vfs_ctx *ctx, const double *ucat, const double *cs, double *nu_t);
+/* The options of Compute_Smagorinsky_Constant_1 (main.c:202-255, 1721-1772,
+ * 2066). */
+typedef struct {
+    int les;                /* 1: Cs = 0.01; 2: dynamic procedure; 3: dynamic with
+                               temporal relaxation (les.c:777-782) */
+    int i_homo_filter;      /* -i_homo_filter: average LM, MM along i ... */
+    int j_homo_filter;      /* -j_homo_filter */
+    int k_homo_filter;      /* -k_homo_filter (i and k together: i-k planes) */
+    int testfilter_ik;      /* -testfilter_ik: Simpson filter in i-k planes */
+    int reserved;           /* set to 0 */
+    double max_cs;          /* -max_cs (the reference's default 0.5) */
+} vfs_les_options;
 /* Compute_Smagorinsky_Constant_1 (les.c:67-1133) into cs (P doubles on the
- * device, every point written).  les 1: Cs = 0.01; les 2: the dynamic
- * procedure with the default filter options (3-D box test filter,
- * -testfilter_ik 0; no -i/j/k_homo_filter averaging; no -clark term),
- * clipped to [0, max_cs] (-max_cs, default 0.5) with at least 0.001 on IB
- * nodes.  Cs = 0 while ti < 2 on a fresh start (tistart 0, rstart 0).
- * ucat must carry its ghost values (after vfs_contra2cart).  Scratch of 19
- * doubles a point is allocated on the first dynamic call and kept. */
-int vfs_smagorinsky_constant(vfs_ctx *ctx, const double *ucat, int les, int ti, int tistart, int rstart,
-                             double max_cs, double *cs);
+ * device, every point written; with -les 3 it is also read: the previous
+ * step's Cs).  The dynamic procedure (les 2, 3) is the reference's without
+ * the Clark term (-clark).  Cs = 0 while ti < 2 on a fresh start (tistart 0,
+ * rstart 0).  ucat must carry its ghost values (after vfs_contra2cart).
+ * Scratch of 19 doubles a point is allocated on the first dynamic call and
+ * kept. */
+int vfs_smagorinsky_constant(vfs_ctx *ctx, const double *ucat, const vfs_les_options *opt, int ti, int tistart,
+                             int rstart, double *cs);
 /* ---- actuator models (rotor_model.c) ------------------------------------
  * Element arrays are device arrays of n values in blocks: cent x|y|z (3n),
  * f_lagr x|y|z (3n), dA (n), box i_min|i_max|j_min|j_max|k_min|k_max (6n,

@@ -741,6 +741,23 @@ double vo_testfilter(const double val[3][3][3], const double w[3][3][3])
     return vs / ws;
 }
 
+/* rhs2.c:388-410 integrate_testfilter_ik: Simpson's rule over the i-k plane
+ * of the box (weights 1, 4, 16 over 36); the volumes are not used */
+double vo_testfilter_ik(const double val[3][3][3])
+{
+    return ((val[0][1][0] + val[2][1][0] + val[0][1][2] + val[2][1][2]) +
+            4. * (val[0][1][1] + val[1][1][0] + val[2][1][1] + val[1][1][2]) + 16. * val[1][1][1]) / 36.;
+}
+
+/* integrate_testfilter with the -testfilter_ik switch (rhs2.c:417-419) */
+static double tfilter(const double val[3][3][3], const double w[3][3][3], int ik)
+{
+    return ik ? vo_testfilter_ik(val) : vo_testfilter(val, w);
+}
+
+static const vo_les_opt les_default = {2, 0, 0, 0, 0, 0.5, 2, 0, 0};
+#define OPT(o) ((o) ? (o) : &les_default)
+
 /* les.c:23-64 get_weight: 1/aj, 0 in solid cells (the boundary rules are
  * commented out there) */
 static void get_weight(const vo_cfg *c, const vo_geom *g, const double *nvert, int i, int j, int k,
@@ -783,9 +800,10 @@ static void cs_weight(const vo_cfg *c, const vo_geom *g, const double *nvert, co
 /* les.c:188-257, first pass: velocity gradient grad[9] (du/dx, du/dy, du/dz,
  * dv/dx, ...: the reference's Ax, Ay, Az), |S| and the test-filtered velocity
  * at the non-solid interior cells, 0 elsewhere, then periodic copies. */
-int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat, const vo_les_opt *o,
                  double *grad, double *sabs, double *ucat_f)
 {
+    o = OPT(o);
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
@@ -808,7 +826,7 @@ int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                 for (int q = 0; q < 3; q++) {
                     for (int r = 0; r < 27; r++)
                         u[r / 9][(r / 3) % 3][r % 3] = ucat[3 * IX(k + r / 9 - 1, j + (r / 3) % 3 - 1, i + r % 3 - 1) + q];
-                    ucat_f[3 * id + q] = vo_testfilter(u, w);
+                    ucat_f[3 * id + q] = tfilter(u, w, o->testfilter_ik);
                 }
             }
     for (int k = 0; k < mz; k++)   /* les.c:271-304 */
@@ -839,9 +857,10 @@ static void covariant_metrics(const double gg[3][3], double G[3][3])
 /* les.c:306-687, second pass: the Germano identity contracted in the
  * contravariant form, LM = L_ba M_aq G_bq and MM = M_nm M_nl G_ml, at the
  * non-solid interior cells (0 elsewhere), then periodic copies. */
-int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat, const vo_les_opt *o,
                  const double *grad, const double *sabs, const double *ucat_f, double *LM, double *MM)
 {
+    o = OPT(o);
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
@@ -881,7 +900,7 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                     for (int q = -1; q <= 1; q++)
                         for (int r = -1; r <= 1; r++) sum_weight += w[r + 1][q + 1][p + 1] * coef[r + 1][q + 1][p + 1];
                 const double filter = pow(1. / g->aj[id], 1. / 3.);
-                const double test_filter = pow(sum_weight, 1. / 3.);
+                const double test_filter = o->testfilter_ik ? pow(5.0, 1. / 3.) * filter : pow(sum_weight, 1. / 3.);
                 const double uf[3] = {ucat_f[3 * id], ucat_f[3 * id + 1], ucat_f[3 * id + 2]};
                 double Uf[3];
                 for (int b = 0; b < 3; b++) {
@@ -894,9 +913,9 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                 const double S_hat = strain_abs(D, Sh);
                 double L[3][3], SSh[3][3], Mc[3][3], M[3][3], gg[3][3], gc[3][3], G[3][3];
                 for (int b = 0; b < 3; b++)
-                    for (int a = 0; a < 3; a++) L[b][a] = vo_testfilter(UU[b][a], w) - Uf[b] * uf[a];
+                    for (int a = 0; a < 3; a++) L[b][a] = tfilter(UU[b][a], w, o->testfilter_ik) - Uf[b] * uf[a];
                 for (int a = 0; a < 3; a++)
-                    for (int b = 0; b < 3; b++) SSh[a][b] = vo_testfilter(SS[a][b], w);
+                    for (int b = 0; b < 3; b++) SSh[a][b] = tfilter(SS[a][b], w, o->testfilter_ik);
                 for (int a = 0; a < 3; a++)
                     for (int b = 0; b < 3; b++) gg[a][b] = met[a][3 * id + b];
                 covariant_metrics(gg, gc);
@@ -927,27 +946,98 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
     return 0;
 }
 
-/* les.c:708-789 and 943-994, third pass: Cs = max(0.5 <LM> / (<MM> + eps), 0)
- * with <.> the 1/aj-weighted box filter, then the clipping: 0 in solids and
- * on every ghost point, at least wall_cs at IB nodes, at most max_cs; then
- * periodic copies (les.c:1018-1047). */
+/* les.c:708-789, third pass: Cs = max(0.5 <LM> / (<MM> + eps), 0) with <.>
+ * the 1/aj-weighted box filter (LM, MM themselves with a homogeneous
+ * direction or -les 3; -les 3 relaxes towards the previous Cs once ti >= 100
+ * or on a restart); the homogeneous-direction averages (les.c:791-942); the
+ * clipping (943-994): 0 in solids and on every ghost point, at least wall_cs
+ * at IB nodes, at most max_cs; then periodic copies (les.c:1018-1047).
+ * Cs: in (the previous Cs, read by -les 3) and out. */
 int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *LM, const double *MM,
-                 double max_cs, double *Cs)
+                 const vo_les_opt *o, double *Cs)
 {
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
+    o = OPT(o);
+    const int homo = o->i_homo_filter || o->j_homo_filter || o->k_homo_filter;
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
                 const size_t id = IX(k, j, i);
                 if (nvert[id] > 1.1) { Cs[id] = 0; continue; }
-                double w[3][3][3], L0[3][3][3], M0[3][3][3];
-                cs_weight(c, g, nvert, LM, i, j, k, w, L0);
-                cs_weight(c, g, nvert, MM, i, j, k, w, M0);
-                const double C = 0.5 * vo_testfilter(L0, w) / (vo_testfilter(M0, w) + LES_EPS);
+                double LM_avg, MM_avg;
+                if (homo || o->les == 3) {
+                    LM_avg = LM[id];
+                    MM_avg = MM[id];
+                } else {
+                    double w[3][3][3], L0[3][3][3], M0[3][3][3];
+                    cs_weight(c, g, nvert, LM, i, j, k, w, L0);
+                    cs_weight(c, g, nvert, MM, i, j, k, w, M0);
+                    LM_avg = tfilter(L0, w, o->testfilter_ik);
+                    MM_avg = tfilter(M0, w, o->testfilter_ik);
+                }
+                double C = 0.5 * LM_avg / (MM_avg + LES_EPS);
+                if (o->les == 3 && !(o->ti < 100 && o->tistart == 0 && !o->rstart))
+                    C = (1.0 - 0.001) * Cs[id] + 0.001 * C;
                 Cs[id] = PMAX(C, 0);
             }
+    if (o->les != 3 && o->i_homo_filter && o->k_homo_filter) {
+        /* les.c:791-829: averages over the i-k planes of the fluid cells */
+        double *sl = calloc(2 * (size_t)my, sizeof(double));
+        int *cnt = calloc((size_t)my, sizeof(int));
+        if (!sl || !cnt) { free(sl); free(cnt); return -9; }
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++)
+                    if (nvert[IX(k, j, i)] < 0.1) {
+                        sl[j] += LM[IX(k, j, i)];
+                        sl[my + j] += MM[IX(k, j, i)];
+                        cnt[j]++;
+                    }
+        for (int j = 0; j < my; j++)
+            if (cnt[j] > 0) {
+                sl[j] /= (double)(cnt[j]);
+                sl[my + j] /= (double)(cnt[j]);
+            }
+        for (int j = 1; j < my - 1; j++)
+            for (int k = 1; k < mz - 1; k++)
+                for (int i = 1; i < mx - 1; i++) Cs[IX(k, j, i)] = 0.5 * sl[j] / (sl[my + j] + LES_EPS);
+        free(sl); free(cnt);
+    } else if (o->les != 3 && homo) {
+        /* les.c:830-942: averages along the homogeneous direction, for the
+         * fluid cells; lines indexed (k,j), (k,i) or (j,i) */
+        const int dir = o->i_homo_filter ? 0 : (o->j_homo_filter ? 1 : 2);
+        const int n1 = dir == 2 ? my : mz, n2 = dir == 0 ? my : mx;   /* line (p1, p2) */
+        const size_t nl = (size_t)n1 * n2;
+        double *sl = calloc(2 * nl, sizeof(double));
+        int *cnt = calloc(nl, sizeof(int));
+        if (!sl || !cnt) { free(sl); free(cnt); return -9; }
+        const int len = dir == 0 ? mx : (dir == 1 ? my : mz);
+        for (int p1 = 1; p1 < n1 - 1; p1++)
+            for (int p2 = 1; p2 < n2 - 1; p2++)
+                for (int q = 1; q < len - 1; q++) {
+                    const int k = dir == 2 ? q : p1, j = dir == 0 ? p2 : (dir == 1 ? q : p1);
+                    const int i = dir == 0 ? q : p2;
+                    const size_t id = IX(k, j, i), pos = (size_t)p1 * n2 + p2;
+                    if (nvert[id] < 0.1) { sl[pos] += LM[id]; sl[nl + pos] += MM[id]; cnt[pos]++; }
+                }
+        for (size_t pos = 0; pos < nl; pos++)
+            if (cnt[pos] > 0) {
+                const double N = (double)(cnt[pos]);
+                sl[pos] /= N;
+                sl[nl + pos] /= N;
+            }
+        for (int p1 = 1; p1 < n1 - 1; p1++)
+            for (int p2 = 1; p2 < n2 - 1; p2++)
+                for (int q = 1; q < len - 1; q++) {
+                    const int k = dir == 2 ? q : p1, j = dir == 0 ? p2 : (dir == 1 ? q : p1);
+                    const int i = dir == 0 ? q : p2;
+                    const size_t id = IX(k, j, i), pos = (size_t)p1 * n2 + p2;
+                    if (nvert[id] < 0.1) Cs[id] = 0.5 * sl[pos] / (sl[nl + pos] + LES_EPS);
+                }
+        free(sl); free(cnt);
+    }
     for (int k = 0; k < mz; k++)
         for (int j = 0; j < my; j++)
             for (int i = 0; i < mx; i++) {
@@ -956,7 +1046,7 @@ int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                     Cs[id] = 0;
                 } else {
                     if (nvert[id] > 0.1 && nvert[id] < 1.1) Cs[id] = PMAX(WALL_CS, Cs[id]);
-                    Cs[id] = PMIN(PMAX(Cs[id], 0), max_cs);
+                    Cs[id] = PMIN(PMAX(Cs[id], 0), o->max_cs);
                 }
             }
     per_copy_sca(c, Cs);
@@ -965,11 +1055,12 @@ int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
 
 /* les.c:1049-1107: the fluid cells' Cs replaced by its 1/aj-weighted box
  * filter (of the clipped field); IB nodes, solids and ghosts keep theirs. */
-int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, double *Cs)
+int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_les_opt *o, double *Cs)
 {
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
+    o = OPT(o);
     const size_t P = (size_t)mx * my * mz;
     double *tmp = malloc(P * sizeof(double));
     if (!tmp) return -9;
@@ -981,37 +1072,40 @@ int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, double
                 if (!(nvert[id] < 0.1)) continue;
                 double w[3][3][3], C[3][3][3];
                 cs_weight(c, g, nvert, tmp, i, j, k, w, C);
-                Cs[id] = vo_testfilter(C, w);
+                Cs[id] = tfilter(C, w, o->testfilter_ik);
             }
     free(tmp);
     return 0;
 }
 
 /* les.c:67-1133 with the fresh-start rule (Cs = 0 while ti < 2 on a run
- * that is not a restart, 69-72) and les == 1 (Cs = 0.01, 74-77). */
+ * that is not a restart, 69-72) and les == 1 (Cs = 0.01, 74-77).  Cs: in
+ * (the previous call's Cs, read by -les 3) and out, every point. */
 int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
-                            int les, int ti, int tistart, int rstart, double max_cs, double *Cs)
+                            const vo_les_opt *o, double *Cs)
 {
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
+    o = OPT(o);
     const size_t P = (size_t)mx * my * mz;
-    if (les != 1 && les != 2) return -4;
-    if (ti < 2 && tistart == 0 && !rstart) {
+    if (o->les < 1 || o->les > 3) return -4;
+    if (o->ti < 2 && o->tistart == 0 && !o->rstart) {
         for (size_t n = 0; n < P; n++) Cs[n] = 0.0;
         return 0;
     }
-    if (les == 1) {
+    if (o->les == 1) {
         for (size_t n = 0; n < P; n++) Cs[n] = 0.01;
         return 0;
     }
     double *grad = malloc(9 * P * sizeof(double)), *sabs = malloc(P * sizeof(double));
     double *uf = malloc(3 * P * sizeof(double)), *LM = malloc(P * sizeof(double)), *MM = malloc(P * sizeof(double));
     if (!grad || !sabs || !uf || !LM || !MM) rc = -9;
-    if (!rc) rc = vo_dsm_pass1(c, g, nvert, ucat, grad, sabs, uf);
-    if (!rc) rc = vo_dsm_pass2(c, g, nvert, ucat, grad, sabs, uf, LM, MM);
-    if (!rc) rc = vo_dsm_pass3(c, g, nvert, LM, MM, max_cs, Cs);
-    if (!rc) rc = vo_dsm_smooth(c, g, nvert, Cs);
+    if (!rc) rc = vo_dsm_pass1(c, g, nvert, ucat, o, grad, sabs, uf);
+    if (!rc) rc = vo_dsm_pass2(c, g, nvert, ucat, o, grad, sabs, uf, LM, MM);
+    if (!rc) rc = vo_dsm_pass3(c, g, nvert, LM, MM, o, Cs);
+    if (!rc && !(o->i_homo_filter || o->j_homo_filter || o->k_homo_filter) && o->les != 3)
+        rc = vo_dsm_smooth(c, g, nvert, o, Cs);
     free(grad); free(sabs); free(uf); free(LM); free(MM);
     return rc;
 }

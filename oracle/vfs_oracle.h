@@ -82,26 +82,38 @@ int vo_eddy_viscosity(const vo_cfg *c, const vo_geom *g, const double *nvert,
 int vo_eddy_viscosity_cs(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          const double *ucat, const double *cs, double *nu_t);
 
-/* les.c:67-1133 Compute_Smagorinsky_Constant_1: les 1 (Cs = 0.01) or 2 (the
- * dynamic procedure with the default options: 3-D box test filter, no
- * homogeneous averaging, no Clark term).  Cs = 0 while ti < 2 on a fresh
- * start (tistart 0, not a restart).  Cs: every point written. */
+/* The options of Compute_Smagorinsky_Constant_1 (main.c:202-255, 1721-1772,
+ * 2066): -les, -i/j/k_homo_filter, -testfilter_ik, -max_cs, and the step
+ * counters of the fresh-start and -les 3 rules. */
+typedef struct {
+    int les;                 /* 1 constant, 2 dynamic, 3 dynamic with relaxation */
+    int i_homo_filter, j_homo_filter, k_homo_filter;
+    int testfilter_ik;
+    double max_cs;
+    int ti, tistart, rstart;
+} vo_les_opt;
+
+/* les.c:67-1133 Compute_Smagorinsky_Constant_1 without the Clark term
+ * (o NULL: -les 2 with the default options at ti = 2).  Cs = 0 while ti < 2
+ * on a fresh start.  Cs: in (the previous Cs, read by -les 3) and out. */
 int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
-                            int les, int ti, int tistart, int rstart, double max_cs, double *Cs);
+                            const vo_les_opt *o, double *Cs);
 
 /* Its stages, exposed for their own tests.  rhs2.c:412-475 integrate_testfilter
- * (val, w indexed [k][j][i] offset + 1); pass 1 (les.c:188-304): grad (9
- * doubles a point: du/dx, du/dy, du/dz, dv/dx, ...), |S| and the filtered
- * velocity; pass 2 (306-706): LM and MM; pass 3 and the clipping (708-1047);
- * the final Cs smoothing (1049-1107, in place). */
+ * and rhs2.c:388-410 integrate_testfilter_ik (val, w indexed [k][j][i] offset
+ * + 1); pass 1 (les.c:188-304): grad (9 doubles a point: du/dx, du/dy, du/dz,
+ * dv/dx, ...), |S| and the filtered velocity; pass 2 (306-706): LM and MM;
+ * pass 3 with the homogeneous averages and the clipping (708-1047); the
+ * final Cs smoothing (1049-1107, in place). */
 double vo_testfilter(const double val[3][3][3], const double w[3][3][3]);
-int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+double vo_testfilter_ik(const double val[3][3][3]);
+int vo_dsm_pass1(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat, const vo_les_opt *o,
                  double *grad, double *sabs, double *ucat_f);
-int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
+int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat, const vo_les_opt *o,
                  const double *grad, const double *sabs, const double *ucat_f, double *LM, double *MM);
 int vo_dsm_pass3(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *LM, const double *MM,
-                 double max_cs, double *Cs);
-int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, double *Cs);
+                 const vo_les_opt *o, double *Cs);
+int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_les_opt *o, double *Cs);
 
 /* momentum.c:197-440 Pressure_Gradient. p: in/out (periodic ghosts). */
 int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,

@@ -29,6 +29,16 @@ VECTOR_GEOM = {"csi", "eta", "zet", "icsi", "ieta", "izet", "jcsi", "jeta", "jze
                "kcsi", "keta", "kzet", "cent"}
 
 
+class VoLesOpt(C.Structure):
+    _fields_ = [("les", C.c_int), ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int), ("k_homo_filter", C.c_int),
+                ("testfilter_ik", C.c_int), ("max_cs", C.c_double), ("ti", C.c_int), ("tistart", C.c_int),
+                ("rstart", C.c_int)]
+
+
+def les_opt(les=2, ti=5, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0, k_homo=0, ik=0):
+    return VoLesOpt(les, i_homo, j_homo, k_homo, ik, max_cs, ti, tistart, rstart)
+
+
 class VoGeom(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in GEOM_FIELDS]
 
@@ -48,7 +58,7 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_dfunc"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc"]:
             getattr(_lib, name).restype = d
         for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
@@ -127,57 +137,37 @@ class Oracle:
     def eddy_viscosity_cs(self, ucat, cs, nu_t):
         self._call("vo_eddy_viscosity_cs", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(cs), ptr(nu_t))
 
-    def smagorinsky_constant(self, ucat, les=2, ti=5, tistart=0, rstart=0, max_cs=0.5):
-        cs = self.sca()
-        self._call("vo_smagorinsky_constant", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.c_int(les),
-                   C.c_int(ti), C.c_int(tistart), C.c_int(rstart), C.c_double(max_cs), ptr(cs))
+    def smagorinsky_constant(self, ucat, cs_prev=None, **opt):
+        """opt: les, ti, tistart, rstart, max_cs, i_homo, j_homo, k_homo, ik"""
+        cs = self.sca() if cs_prev is None else np.array(cs_prev, dtype=np.float64)
+        o = les_opt(**opt)
+        self._call("vo_smagorinsky_constant", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.byref(o), ptr(cs))
         return cs
 
-    def dsm_pass1(self, ucat):
+    def dsm_pass1(self, ucat, **opt):
         grad, sabs, uf = np.zeros(9 * self.P), self.sca(), self.vec()
-        self._call("vo_dsm_pass1", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(grad), ptr(sabs), ptr(uf))
+        o = les_opt(**opt)
+        self._call("vo_dsm_pass1", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.byref(o), ptr(grad), ptr(sabs),
+                   ptr(uf))
         return grad, sabs, uf
 
-    def dsm_pass2(self, ucat, grad, sabs, uf):
+    def dsm_pass2(self, ucat, grad, sabs, uf, **opt):
         LM, MM = self.sca(), self.sca()
-        self._call("vo_dsm_pass2", C.byref(self.geom), ptr(self.nvert), ptr(ucat), ptr(grad), ptr(sabs), ptr(uf),
-                   ptr(LM), ptr(MM))
+        o = les_opt(**opt)
+        self._call("vo_dsm_pass2", C.byref(self.geom), ptr(self.nvert), ptr(ucat), C.byref(o), ptr(grad), ptr(sabs),
+                   ptr(uf), ptr(LM), ptr(MM))
         return LM, MM
 
-    def dsm_pass3(self, LM, MM, max_cs=0.5):
-        cs = self.sca()
-        self._call("vo_dsm_pass3", C.byref(self.geom), ptr(self.nvert), ptr(LM), ptr(MM), C.c_double(max_cs), ptr(cs))
+    def dsm_pass3(self, LM, MM, cs_prev=None, **opt):
+        cs = self.sca() if cs_prev is None else np.array(cs_prev, dtype=np.float64)
+        o = les_opt(**opt)
+        self._call("vo_dsm_pass3", C.byref(self.geom), ptr(self.nvert), ptr(LM), ptr(MM), C.byref(o), ptr(cs))
         return cs
 
-    def dsm_smooth(self, cs):
-        self._call("vo_dsm_smooth", C.byref(self.geom), ptr(self.nvert), ptr(cs))
+    def dsm_smooth(self, cs, **opt):
+        o = les_opt(**opt)
+        self._call("vo_dsm_smooth", C.byref(self.geom), ptr(self.nvert), C.byref(o), ptr(cs))
         return cs
-
-    def formfunction_snes_f(self, U, U_o, rhs_o, dP, nu_t, f_eul, lucont, ucat, rhs):
-        self._call("vo_formfunction_snes_f", C.byref(self.geom), ptr(self.nvert), ptr(U), ptr(U_o), ptr(rhs_o),
-                   ptr(dP), ptr(nu_t), None if f_eul is None else ptr(f_eul), ptr(lucont), ptr(ucat), ptr(rhs))
-
-    def rotor_elements(self, coor, cent, itpwidth=6):
-        """cent: (3, n) -> box (6, n) int32, dh (3, n)"""
-        cent = np.ascontiguousarray(cent, dtype=np.float64)
-        n = cent.shape[1]
-        box = np.zeros((6, n), dtype=np.int32)
-        dh = np.zeros((3, n))
-        self._call("vo_rotor_elements", C.byref(self.geom), ptr(coor), C.c_int(n), ptr(cent), C.c_int(itpwidth),
-                   ptr(box), ptr(dh))
-        return box, dh
-
-    def calc_f_eul(self, coor, cent, F, dA, box, df=10, halfwidth=4.0, width_mode=0, dh_fixed=None):
-        cent = np.ascontiguousarray(cent, dtype=np.float64)
-        F = np.ascontiguousarray(F, dtype=np.float64)
-        dA = np.ascontiguousarray(dA, dtype=np.float64)
-        box = np.ascontiguousarray(box, dtype=np.int32)
-        fx = np.ascontiguousarray(dh_fixed if dh_fixed is not None else np.zeros(3), dtype=np.float64)
-        f = self.vec()
-        self._call("vo_calc_f_eul", C.byref(self.geom), ptr(self.nvert), ptr(coor), C.c_int(cent.shape[1]), ptr(cent),
-                   ptr(F), ptr(dA), ptr(box), C.c_int(df), C.c_double(halfwidth), C.c_int(width_mode), ptr(fx),
-                   ptr(f))
-        return f
 
     def pressure_gradient(self, p, dp):
         self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
@@ -262,6 +252,12 @@ WF_NYP = 500000
 
 def dfunc(df, r, n=4.0):
     return lib().vo_dfunc(C.c_int(df), C.c_double(r), C.c_double(n))
+
+
+def testfilter_ik(val):
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    assert val.shape == (3, 3, 3)
+    return lib().vo_testfilter_ik(ptr(val))
 
 
 def testfilter(val, w):

@@ -304,5 +304,143 @@ def test_gpu_fresh_start_and_constant_rules():
     ctx.smagorinsky_constant(du, cs, ti=1, rstart=1)
     assert rel_err(cs.host(), o.smagorinsky_constant(u, ti=1, rstart=1)) < TOL
     with pytest.raises(V.VfsError, match="les must be"):
-        ctx.smagorinsky_constant(du, cs, les=3)
+        ctx.smagorinsky_constant(du, cs, les=4)
+    with pytest.raises(V.VfsError, match="homogeneous"):
+        ctx.smagorinsky_constant(du, cs, i_homo=1, j_homo=1)
     ctx.close()
+
+
+OPTION_CASES = [dict(ik=1), dict(i_homo=1, k_homo=1), dict(i_homo=1), dict(j_homo=1), dict(k_homo=1),
+                dict(les=3, ti=50), dict(les=3, ti=150), dict(les=3, ti=5, rstart=1, ik=1)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["C2", "C1w"])
+@pytest.mark.parametrize("opt", OPTION_CASES, ids=lambda o: "-".join(f"{k}{v}" for k, v in o.items()))
+def test_gpu_dynamic_options(name, opt):
+    print(f"[dsm options] {name} {opt}", flush=True)
+    cfg, spec, coor, o, u = _ucat_after_contra2cart(name)
+    ctx = gpu_case(cfg, spec, coor)
+    du = V.DeviceArray.from_host(u)
+    prev = np.abs(np.random.default_rng(3).standard_normal(o.P)) * 0.05
+    want = o.smagorinsky_constant(u, cs_prev=prev, max_cs=0.3, **opt)
+    cs = V.DeviceArray.from_host(prev)
+    ctx.smagorinsky_constant(du, cs, max_cs=0.3, **opt)
+    got = cs.host()
+    assert np.count_nonzero(want) > want.size // 4
+    assert rel_err(got, want) < TOL
+    assert np.array_equal(got == 0, want == 0)
+    ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# the other options of the routine: -testfilter_ik, -i/j/k_homo_filter, -les 3
+# ---------------------------------------------------------------------------
+from oracle_py import testfilter_ik as simpson_ik  # noqa: E402
+
+
+def test_testfilter_ik_is_simpson_in_the_i_k_plane():
+    """x^2 (i offset squared) filters to 1/3, k^2 likewise, x k to 0; the
+    j = -1 and +1 planes and the weights are ignored."""
+    R, Q, P = np.meshgrid(np.arange(3) - 1.0, np.arange(3) - 1.0, np.arange(3) - 1.0, indexing="ij")
+    assert simpson_ik(P ** 2) == pytest.approx(1 / 3, rel=1e-15)
+    assert simpson_ik(R ** 2) == pytest.approx(1 / 3, rel=1e-15)
+    assert simpson_ik(R * P) == 0.0
+    v = np.random.default_rng(2).standard_normal((3, 3, 3))
+    v2 = v.copy()
+    v2[:, 0] += 5.0
+    v2[:, 2] -= 3.0
+    assert simpson_ik(v2) == simpson_ik(v)
+
+
+def simpson_xz(f):
+    for ax in (-1, -3):
+        f = (np.roll(f, 1, ax) + 4 * f + np.roll(f, -1, ax)) / 6
+    return f
+
+
+def test_testfilter_ik_reduces_to_the_textbook_2d_filter():
+    """Triply periodic uniform box: with -testfilter_ik the model is Germano-
+    Lilly with the 2-D Simpson filter in x-z and test width 5^(1/3) Delta."""
+    N = (10, 8, 12)
+    cfg, o = uniform_box(N)
+    h = (W.LX / N[0], W.LY / N[1], W.LZ / N[2])
+    rng = np.random.default_rng(17)
+    u = rng.standard_normal((3, N[2], N[1], N[0]))
+    ucat = np.zeros(3 * o.P)
+    ucat.reshape(o.mz, o.my, o.mx, 3)[1:-1, 1:-1, 1:-1] = np.moveaxis(u, 0, -1)
+    periodic_ghosts(o, ucat, 3)
+    axes = (-1, -2, -3)
+
+    def strain(v):
+        D = np.array([[(np.roll(v[a], -1, axes[b]) - np.roll(v[a], 1, axes[b])) / (2 * h[b]) for b in range(3)]
+                      for a in range(3)])
+        S = 0.5 * (D + D.transpose(1, 0, 2, 3, 4))
+        return S, np.sqrt(2 * (S * S).sum(axis=(0, 1)))
+    vol = h[0] * h[1] * h[2]
+    S, Sa = strain(u)
+    uf = np.array([simpson_xz(u[a]) for a in range(3)])
+    L = np.array([[simpson_xz(u[a] * u[b]) - uf[a] * uf[b] for b in range(3)] for a in range(3)])
+    Sh, Sha = strain(uf)
+    M = np.array([[-5 ** (2 / 3) * vol ** (2 / 3) * Sha * Sh[a, b] + vol ** (2 / 3) * simpson_xz(Sa * S[a, b])
+                   for b in range(3)] for a in range(3)])
+    LM_t, MM_t = (L * M).sum(axis=(0, 1)), (M * M).sum(axis=(0, 1))
+    grad, sabs, ufo = o.dsm_pass1(ucat, ik=1)
+    LM, MM = o.dsm_pass2(ucat, grad, sabs, ufo, ik=1)
+    inner = (slice(1, -1),) * 3
+    assert rel_err(LM.reshape(o.mz, o.my, o.mx)[inner], LM_t) < 1e-11
+    assert rel_err(MM.reshape(o.mz, o.my, o.mx)[inner], MM_t) < 1e-11
+    C = np.minimum(np.maximum(0.5 * simpson_xz(LM_t) / (simpson_xz(MM_t) + LES_EPS), 0), 0.5)
+    cs = o.smagorinsky_constant(ucat, ik=1)
+    assert rel_err(cs.reshape(o.mz, o.my, o.mx)[inner], simpson_xz(C)) < 1e-11
+
+
+@pytest.mark.parametrize("mode", ["ik", "i", "j", "k"])
+def test_homogeneous_averages(mode):
+    """Plane (i and k) or line averages of LM and MM over the fluid cells,
+    Cs = 0.5 <LM> / (<MM> + eps) then clipped; no smoothing afterwards."""
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", (24, 18, 16))
+    uc, u = ucont.copy(), ucat.copy()
+    o.contra2cart(uc, u)
+    flags = dict(i_homo=int("i" in mode), j_homo=int(mode == "j"), k_homo=int("k" in mode))
+    grad, sabs, uf = o.dsm_pass1(u, **flags)
+    LM, MM = o.dsm_pass2(u, grad, sabs, uf, **flags)
+    cs = o.smagorinsky_constant(u, **flags)
+    nv = o.nvert.reshape(o.mz, o.my, o.mx)
+    L3, M3 = LM.reshape(nv.shape), MM.reshape(nv.shape)
+    fl = (nv < 0.1)
+    fl[0] = fl[-1] = False; fl[:, 0] = fl[:, -1] = False; fl[:, :, 0] = fl[:, :, -1] = False
+    axes = {"ik": (0, 2), "i": (2,), "j": (1,), "k": (0,)}[mode]
+    cnt = fl.sum(axis=axes, keepdims=True)
+    mL = np.where(cnt > 0, (L3 * fl).sum(axis=axes, keepdims=True) / np.maximum(cnt, 1), 0)
+    mM = np.where(cnt > 0, (M3 * fl).sum(axis=axes, keepdims=True) / np.maximum(cnt, 1), 0)
+    want = np.broadcast_to(0.5 * mL / (mM + LES_EPS), nv.shape)
+    got = cs.reshape(nv.shape)
+    inner = np.zeros(nv.shape, bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    target = fl if mode != "ik" else inner
+    clipped = np.minimum(np.maximum(want, 0), 0.5)
+    ibn = (nv > 0.1) & (nv < 1.1) & inner
+    clipped = np.where(ibn, np.minimum(np.maximum(np.maximum(0.001, want), 0), 0.5), clipped)
+    sel = target & (nv < 1.1)
+    assert rel_err(got[sel], clipped[sel]) < 1e-12
+    assert not got[nv > 1.1].any()
+    if mode != "ik":   # IB nodes keep the pointwise value
+        C = np.maximum(0.5 * L3 / (M3 + LES_EPS), 0)
+        assert rel_err(got[ibn], np.minimum(np.maximum(np.maximum(0.001, C), 0), 0.5)[ibn]) < 1e-12
+
+
+def test_les3_relaxation():
+    """-les 3: no test filtering of LM, MM and no smoothing; from ti = 100 on
+    (or on a restart) Cs = 0.999 Cs_prev + 0.001 C."""
+    cfg, o = uniform_box((8, 9, 7), all_periodic=False)
+    lm, mm, prev = 0.02, 0.3, 0.2
+    LM, MM = o.sca() + lm, o.sca() + mm
+    C = 0.5 * lm / (mm + LES_EPS)
+    inner = (slice(1, -1),) * 3
+    fresh = o.dsm_pass3(LM, MM, cs_prev=o.sca() + prev, les=3, ti=50).reshape(o.mz, o.my, o.mx)
+    assert np.all(fresh[inner] == C)        # no filter: the top row is not 3/4 of it
+    relax = o.dsm_pass3(LM, MM, cs_prev=o.sca() + prev, les=3, ti=100).reshape(o.mz, o.my, o.mx)
+    assert np.all(relax[inner] == (1.0 - 0.001) * prev + 0.001 * C)
+    restart = o.dsm_pass3(LM, MM, cs_prev=o.sca() + prev, les=3, ti=5, rstart=1).reshape(o.mz, o.my, o.mx)
+    assert np.array_equal(restart, relax)

@@ -57,6 +57,16 @@ __device__ __forceinline__ double oct_total(const Oct &a)
     for (int o = 1; o < 8; o++) t += a.s[o];
     return t;
 }
+// rhs2.c:388-410 integrate_testfilter_ik: Simpson's rule over the i-k plane
+// through the centre (weights 1, 4, 16 over 36, no volumes); f(R, P) is the
+// value at k, i offsets + 1 on the centre's j
+template <typename F>
+__device__ __forceinline__ double simpson_ik(F f)
+{
+    return ((f(0, 0) + f(2, 0) + f(0, 2) + f(2, 2)) + 4. * (f(0, 1) + f(1, 0) + f(2, 1) + f(1, 2)) + 16. * f(1, 1)) /
+           36.;
+}
+
 // sum(v1..v8) of 27 stored products p[9 R + 3 Q + P], in the reference's
 // order (the octant's eight terms with the k offset fastest, then i, then j)
 __device__ __forceinline__ double octsum27(const double p[27])
@@ -183,6 +193,7 @@ __device__ __forceinline__ int ss_plane(int a, int b)
 }
 
 // les.c:188-257 (first pass)
+template <bool IK>
 __global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *__restrict__ nvert,
                                                   const double *__restrict__ ucat, const double *__restrict__ uq,
                                                   const double *__restrict__ wt, double *__restrict__ ss,
@@ -207,6 +218,14 @@ __global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int b = a; b < 3; b++) st32(pl[ss_plane(a, b)], t, S[a][b] * sabs);
+    if (IK) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double *u = uq + (size_t)q * d.P;
+            st32(uf, 3 * t + q, simpson_ik([&](int R, int P) { return ld32(u, t + (unsigned)slot_off(d, R, 1, P)); }));
+        }
+        return;
+    }
     Oct v[3], w;
     OCT_FOR
     {
@@ -244,6 +263,7 @@ __device__ __forceinline__ void covariant_metrics(const double gg[3][3], double 
 // and wave 0 finishes each point: the divisions by the weight total, the
 // Germano terms and the contractions.
 #define DSM_LM_PTS 64
+template <bool IK>
 __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__restrict__ nvert,
                                                 const double *__restrict__ uq, const double *__restrict__ Ub,
                                                 const double *__restrict__ wt, const double *__restrict__ ss,
@@ -256,7 +276,28 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
     const unsigned mx = (unsigned)d.mx, my = (unsigned)d.my;
     const int i = (int)(t % mx), j = (int)((t / mx) % my), k = (int)(t / (mx * my));
     const bool act = t < (unsigned)d.P && interior(d, i, j, k) && ld32(nvert, t) <= 1.1;
-    if (act) {
+    if (act && IK) {   // -testfilter_ik: the plane filters, no weights (totals of 1)
+        const size_t Pz = (size_t)d.P;
+        auto at = [&](int R, int P) { return t + (unsigned)slot_off(d, R, 1, P); };
+        if (role < 3) {
+            const double *Upl = Ub + role * Pz, *sp = ss + role * Pz;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                const double *ua = uq + a * Pz;
+                sh[3 * role + a][lane] =
+                    simpson_ik([&](int R, int P) { return ld32(Upl, at(R, P)) * ld32(ua, at(R, P)); });
+            }
+            sh[9 + role][lane] = simpson_ik([&](int R, int P) { return ld32(sp, at(R, P)); });
+        } else {
+#pragma unroll
+            for (int q = 3; q < 6; q++) {
+                const double *pq = ss + q * Pz;
+                sh[9 + q][lane] = simpson_ik([&](int R, int P) { return ld32(pq, at(R, P)); });
+            }
+            sh[15][lane] = 1.0;
+            sh[16][lane] = 0.0;
+        }
+    } else if (act) {
         const size_t Pz = (size_t)d.P;
         if (role < 3) {   // <U_b u_a>, a = 0..2, for b = role, and the S|S| plane b
             const double *Upl = Ub + role * Pz, *u0p = uq, *u1p = uq + Pz, *u2p = uq + 2 * Pz, *sp = ss + role * Pz;
@@ -313,7 +354,7 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
     const double *met[3] = {g.csi, g.eta, g.zet};
     const double ws = sh[15][lane];
     const double filter = pow(1. / ld32(g.aj, t), 1. / 3.);
-    const double test_filter = pow(sh[16][lane], 1. / 3.);
+    const double test_filter = IK ? pow(5.0, 1. / 3.) * filter : pow(sh[16][lane], 1. / 3.);
     const double ufc[3] = {ld32(uf, 3 * t), ld32(uf, 3 * t + 1), ld32(uf, 3 * t + 2)};
     double L[3][3], SSh[3][3];
 #pragma unroll
@@ -368,10 +409,23 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
     st32(MM, t, denom);
 }
 
-// les.c:708-789 (third pass) and the clipping of les.c:943-994
-__global__ void __launch_bounds__(256) k_dsm_cs(Dims d, const double *__restrict__ nvert, const double *__restrict__ wt,
-                         const double *__restrict__ LM, const double *__restrict__ MM, double max_cs,
-                         double *__restrict__ cs)
+// the clipping of les.c:943-994 at point (k,j,i) with mask value nv
+__device__ __forceinline__ double clip_cs(const Dims &d, int i, int j, int k, double nv, double c, double max_cs)
+{
+    if (!interior(d, i, j, k) || nv > 1.1) return 0;
+    if (nv > 0.1 && nv < 1.1) c = PMAX(VFS_WALL_CS, c);
+    return PMIN(PMAX(c, 0), max_cs);
+}
+
+// les.c:708-789 (third pass), with the clipping of les.c:943-994 when CLIP.
+// FILTER: the test filter of LM and MM (not with a homogeneous direction or
+// -les 3); IK: its i-k plane form; relax: -les 3 from ti = 100 on, with the
+// previous Cs read from cs (in place).
+template <bool FILTER, bool IK>
+__global__ void __launch_bounds__(256) k_dsm_cs(Dims d, const double *__restrict__ nvert,
+                                                const double *__restrict__ wt, const double *__restrict__ LM,
+                                                const double *__restrict__ MM, int relax, int clip, double max_cs,
+                                                double *cs)
 {
     DSM_PREAMBLE;
     const double nv = ld32(nvert, t);
@@ -379,40 +433,122 @@ __global__ void __launch_bounds__(256) k_dsm_cs(Dims d, const double *__restrict
         st32(cs, t, 0.0);
         return;
     }
-    Oct l, m, w;
-    if (away_from_ghosts(d, i, j, k)) {
-        OCT_FOR
-        {
-            const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
-            const double wn = ld32(wt, n);
-            oct_add(w, R, Q, P, wn);
-            oct_add(l, R, Q, P, ld32(LM, n) * wn);
-            oct_add(m, R, Q, P, ld32(MM, n) * wn);
-        }
+    double lm, mm;
+    if (!FILTER) {
+        lm = ld32(LM, t);
+        mm = ld32(MM, t);
+    } else if (IK) {
+        auto src = [&](int R, int P) { unsigned s; cs_weight(d, wt, j, k + R - 1, j, i + P - 1, s); return s; };
+        lm = simpson_ik([&](int R, int P) { return ld32(LM, src(R, P)); });
+        mm = simpson_ik([&](int R, int P) { return ld32(MM, src(R, P)); });
     } else {
-        OCT_FOR_RT
-        {
-            unsigned src;
-            const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
-            oct_add(w, R, Q, P, wn);
-            oct_add(l, R, Q, P, ld32(LM, src) * wn);
-            oct_add(m, R, Q, P, ld32(MM, src) * wn);
+        Oct l, m, w;
+        if (away_from_ghosts(d, i, j, k)) {
+            OCT_FOR
+            {
+                const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+                const double wn = ld32(wt, n);
+                oct_add(w, R, Q, P, wn);
+                oct_add(l, R, Q, P, ld32(LM, n) * wn);
+                oct_add(m, R, Q, P, ld32(MM, n) * wn);
+            }
+        } else {
+            OCT_FOR_RT
+            {
+                unsigned src;
+                const double wn = cs_weight(d, wt, j, k + R - 1, j + Q - 1, i + P - 1, src);
+                oct_add(w, R, Q, P, wn);
+                oct_add(l, R, Q, P, ld32(LM, src) * wn);
+                oct_add(m, R, Q, P, ld32(MM, src) * wn);
+            }
+        }
+        const double ws = oct_total(w);
+        lm = oct_total(l) / ws;
+        mm = oct_total(m) / ws;
+    }
+    double C = 0.5 * lm / (mm + VFS_LES_EPS);
+    if (relax) C = (1.0 - 0.001) * ld32(cs, t) + 0.001 * C;   // les.c:777-782
+    const double c = PMAX(C, 0);
+    st32(cs, t, clip ? clip_cs(d, i, j, k, nv, c, max_cs) : c);
+}
+
+// les.c:791-942: sums of LM and MM over the fluid cells of each i-k plane
+// (dir 3; one thread a j) or each line along direction dir (one thread a
+// line (p1, p2): (k,j) for i, (k,i) for j, (j,i) for k), in the reference's
+// order; sums[3 n + pos]: LM, MM and the count of line pos of n
+__global__ void k_homo_sums(Dims d, int dir, const double *__restrict__ nvert, const double *__restrict__ LM,
+                            const double *__restrict__ MM, double *__restrict__ sums)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double sl = 0, sm = 0, cnt = 0;
+    if (dir == 3) {
+        if (t >= d.my) return;
+        const int j = (int)t;
+        if (j >= 1 && j <= d.my - 2)
+            for (int k = 1; k < d.mz - 1; k++)
+                for (int i = 1; i < d.mx - 1; i++) {
+                    const long long id = IX(d, k, j, i);
+                    if (nvert[id] < 0.1) { sl += LM[id]; sm += MM[id]; cnt += 1; }
+                }
+        sums[t] = sl; sums[d.my + t] = sm; sums[2 * d.my + t] = cnt;
+        return;
+    }
+    const int n1 = dir == 2 ? d.my : d.mz, n2 = dir == 0 ? d.my : d.mx, len = dir == 0 ? d.mx : (dir == 1 ? d.my : d.mz);
+    const long long nl = (long long)n1 * n2;
+    if (t >= nl) return;
+    const int p1 = (int)(t / n2), p2 = (int)(t % n2);
+    if (p1 >= 1 && p1 <= n1 - 2 && p2 >= 1 && p2 <= n2 - 2)
+        for (int q = 1; q < len - 1; q++) {
+            const int k = dir == 2 ? q : p1, j = dir == 0 ? p2 : (dir == 1 ? q : p1), i = dir == 0 ? q : p2;
+            const long long id = IX(d, k, j, i);
+            if (nvert[id] < 0.1) { sl += LM[id]; sm += MM[id]; cnt += 1; }
+        }
+    sums[t] = sl; sums[nl + t] = sm; sums[2 * nl + t] = cnt;
+}
+
+// the averages (sums / count where count > 0) into Cs: every interior cell
+// for the i-k planes, the fluid cells for lines; then the clipping of every
+// point (les.c:943-994)
+__global__ void k_homo_assign(Dims d, int dir, const double *__restrict__ nvert, const double *__restrict__ sums,
+                              double max_cs, double *__restrict__ cs)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    const double nv = nvert[t];
+    double c = cs[t];
+    if (interior(d, i, j, k)) {
+        long long pos, nl;
+        if (dir == 3) { pos = j; nl = d.my; }
+        else {
+            const int n2 = dir == 0 ? d.my : d.mx;
+            const int p1 = dir == 2 ? j : k, p2 = dir == 0 ? j : i;
+            pos = (long long)p1 * n2 + p2;
+            nl = (long long)(dir == 2 ? d.my : d.mz) * n2;
+        }
+        if (dir == 3 || nv < 0.1) {
+            double a = sums[pos], b = sums[nl + pos];
+            const double n = sums[2 * nl + pos];
+            if (n > 0) { a /= n; b /= n; }
+            c = 0.5 * a / (b + VFS_LES_EPS);
         }
     }
-    const double ws = oct_total(w);
-    const double C = 0.5 * (oct_total(l) / ws) / (oct_total(m) / ws + VFS_LES_EPS);
-    double c = PMAX(C, 0);
-    if (nv > 0.1 && nv < 1.1) c = PMAX(VFS_WALL_CS, c);
-    st32(cs, t, PMIN(PMAX(c, 0), max_cs));
+    cs[t] = clip_cs(d, i, j, k, nv, c, max_cs);
 }
 
 // les.c:1049-1107: the box filter of the clipped Cs at the fluid cells
+template <bool IK>
 __global__ void __launch_bounds__(256) k_dsm_smooth(Dims d, const double *__restrict__ nvert, const double *__restrict__ wt,
                              const double *__restrict__ cst, double *__restrict__ cs)
 {
     DSM_PREAMBLE;
     if (!interior(d, i, j, k) || !(ld32(nvert, t) < 0.1)) {
         st32(cs, t, ld32(cst, t));
+        return;
+    }
+    if (IK) {
+        auto src = [&](int R, int P) { unsigned s; cs_weight(d, wt, j, k + R - 1, j, i + P - 1, s); return s; };
+        st32(cs, t, simpson_ik([&](int R, int P) { return ld32(cst, src(R, P)); }));
         return;
     }
     Oct v, w;
@@ -451,16 +587,20 @@ static int periodic_planes(vfs_ctx *c, double *v, int np)
     return 0;
 }
 
-extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, int les, int ti, int tistart, int rstart,
-                                        double max_cs, double *cs)
+extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, const vfs_les_options *o, int ti, int tistart,
+                                        int rstart, double *cs)
 {
-    VFS_REQUIRE(les == 1 || les == 2, "les must be 1 (constant Cs) or 2 (dynamic procedure)");
+    VFS_REQUIRE(o, "options required");
+    VFS_REQUIRE(o->les >= 1 && o->les <= 3, "les must be 1 (constant Cs), 2 (dynamic) or 3 (dynamic, relaxed)");
     VFS_REQUIRE(ucat && cs, "ucat and cs are required");
+    const int nh = (o->i_homo_filter != 0) + (o->j_homo_filter != 0) + (o->k_homo_filter != 0);
+    VFS_REQUIRE(nh <= 1 || (nh == 2 && o->i_homo_filter && o->k_homo_filter),
+                "homogeneous filters: one direction, or i and k together");
     const Dims d = c->d;
     VFS_REQUIRE(VFS_IX32_OK(d.P), "grid too large for the dynamic procedure's 32-bit indexing (24 P >= 2^32)");
-    if ((ti < 2 && tistart == 0 && !rstart) || les == 1) {   // les.c:69-77
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d.P,
-                           les == 1 && !(ti < 2 && tistart == 0 && !rstart) ? 0.01 : 0.0, cs);
+    const bool fresh = ti < 2 && tistart == 0 && !rstart;
+    if (fresh || o->les == 1) {   // les.c:69-77
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d.P, fresh ? 0.0 : 0.01, cs);
         VFS_CHECK_LAUNCH();
         return 0;
     }
@@ -468,20 +608,45 @@ extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, int les,
     const size_t P = (size_t)d.P;
     double *ss = c->dsm, *uf = ss + 6 * P, *LM = uf + 3 * P, *MM = LM + P, *cst = MM + P, *wt = cst + P;
     double *uq = wt + P, *Ub = uq + 3 * P;
+    const bool ik = o->testfilter_ik != 0, les3 = o->les == 3, homo = nh > 0;
     const dim3 blk(256), grd(grid_for(d.P, 256));
     hipLaunchKernelGGL(k_dsm_point, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, uq, Ub);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_dsm_grad, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
+    if (ik) hipLaunchKernelGGL(k_dsm_grad<true>, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
+    else hipLaunchKernelGGL(k_dsm_grad<false>, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, ss, 6) || vfs_periodic_vec(c, uf)) return -1;
-    hipLaunchKernelGGL(k_dsm_lm, dim3(grid_for(d.P, DSM_LM_PTS)), dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub,
-                       wt, ss, uf, LM, MM);
+    const dim3 glm(grid_for(d.P, DSM_LM_PTS));
+    if (ik) hipLaunchKernelGGL(k_dsm_lm<true>, glm, dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub, wt, ss, uf, LM, MM);
+    else hipLaunchKernelGGL(k_dsm_lm<false>, glm, dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub, wt, ss, uf, LM, MM);
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, LM, 2)) return -1;   // LM and MM are adjacent planes
-    hipLaunchKernelGGL(k_dsm_cs, grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, max_cs, cst);
+    const double max_cs = o->max_cs;
+    if (homo || les3) {
+        // LM and MM unfiltered; -les 3 relaxes in place and skips the averages
+        const int relax = les3 && !(ti < 100 && tistart == 0 && !rstart);
+        hipLaunchKernelGGL((k_dsm_cs<false, false>), grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, relax,
+                           les3 ? 1 : 0, max_cs, cs);
+        VFS_CHECK_LAUNCH();
+        if (!les3) {
+            const int dir = nh == 2 ? 3 : (o->i_homo_filter ? 0 : (o->j_homo_filter ? 1 : 2));
+            const long long nl = dir == 3 ? d.my : (long long)(dir == 2 ? d.my : d.mz) * (dir == 0 ? d.my : d.mx);
+            double *sums = cst;   // 3 nl doubles: cst and wt (2 P) have room
+            VFS_REQUIRE(3 * nl <= 2 * (long long)P, "grid too small for the line sums");
+            hipLaunchKernelGGL(k_homo_sums, dim3(grid_for(nl, 64)), dim3(64), 0, c->stream, d, dir, c->nvert, LM, MM,
+                               sums);
+            VFS_CHECK_LAUNCH();
+            hipLaunchKernelGGL(k_homo_assign, grd, blk, 0, c->stream, d, dir, c->nvert, sums, max_cs, cs);
+            VFS_CHECK_LAUNCH();
+        }
+        return vfs_periodic_sca(c, cs);
+    }
+    if (ik) hipLaunchKernelGGL((k_dsm_cs<true, true>), grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, 0, 1, max_cs, cst);
+    else hipLaunchKernelGGL((k_dsm_cs<true, false>), grd, blk, 0, c->stream, d, c->nvert, wt, LM, MM, 0, 1, max_cs, cst);
     VFS_CHECK_LAUNCH();
     if (vfs_periodic_sca(c, cst)) return -1;
-    hipLaunchKernelGGL(k_dsm_smooth, grd, blk, 0, c->stream, d, c->nvert, wt, cst, cs);
+    if (ik) hipLaunchKernelGGL(k_dsm_smooth<true>, grd, blk, 0, c->stream, d, c->nvert, wt, cst, cs);
+    else hipLaunchKernelGGL(k_dsm_smooth<false>, grd, blk, 0, c->stream, d, c->nvert, wt, cst, cs);
     VFS_CHECK_LAUNCH();
     return 0;
 }

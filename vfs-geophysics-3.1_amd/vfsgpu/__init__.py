@@ -61,6 +61,11 @@ class Config(C.Structure):
         return c
 
 
+class LesOptions(C.Structure):
+    _fields_ = [("les", C.c_int), ("i_homo_filter", C.c_int), ("j_homo_filter", C.c_int),
+                ("k_homo_filter", C.c_int), ("testfilter_ik", C.c_int), ("reserved", C.c_int), ("max_cs", C.c_double)]
+
+
 class DeviceArray:
     """A zero-initialised device allocation of n float64 (or float32) values."""
 
@@ -202,11 +207,14 @@ class Ctx:
     def set_body_force(self, f_eul):
         _ck(lib().vfs_set_body_force(self.h, None if f_eul is None else f_eul.ptr), "vfs_set_body_force")
 
-    def smagorinsky_constant(self, ucat, cs, les=2, ti=2, tistart=0, rstart=0, max_cs=0.5):
+    def smagorinsky_constant(self, ucat, cs, les=2, ti=2, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0,
+                             k_homo=0, ik=0):
         """Compute_Smagorinsky_Constant_1 (les.c:67-1133) into cs: -les 1
-        (0.01) or 2 (dynamic procedure); 0 while ti < 2 on a fresh start."""
-        _ck(lib().vfs_smagorinsky_constant(self.h, ucat.ptr, int(les), int(ti), int(tistart), int(rstart),
-                                           C.c_double(max_cs), cs.ptr), "vfs_smagorinsky_constant")
+        (0.01), 2 (dynamic procedure) or 3 (relaxed; cs also holds the
+        previous Cs); 0 while ti < 2 on a fresh start."""
+        o = LesOptions(int(les), int(i_homo), int(j_homo), int(k_homo), int(ik), 0, float(max_cs))
+        _ck(lib().vfs_smagorinsky_constant(self.h, ucat.ptr, C.byref(o), int(ti), int(tistart), int(rstart), cs.ptr),
+            "vfs_smagorinsky_constant")
 
     def pressure_gradient(self, p, dp):
         _ck(lib().vfs_pressure_gradient(self.h, p.ptr, dp.ptr), "vfs_pressure_gradient")
