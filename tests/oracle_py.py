@@ -169,6 +169,32 @@ class Oracle:
         self._call("vo_dsm_smooth", C.byref(self.geom), ptr(self.nvert), C.byref(o), ptr(cs))
         return cs
 
+    def formfunction_snes_f(self, U, U_o, rhs_o, dP, nu_t, f_eul, lucont, ucat, rhs):
+        self._call("vo_formfunction_snes_f", C.byref(self.geom), ptr(self.nvert), ptr(U), ptr(U_o), ptr(rhs_o),
+                   ptr(dP), ptr(nu_t), None if f_eul is None else ptr(f_eul), ptr(lucont), ptr(ucat), ptr(rhs))
+
+    def rotor_elements(self, coor, cent, itpwidth=6):
+        """cent: (3, n) -> box (6, n) int32, dh (3, n)"""
+        cent = np.ascontiguousarray(cent, dtype=np.float64)
+        n = cent.shape[1]
+        box = np.zeros((6, n), dtype=np.int32)
+        dh = np.zeros((3, n))
+        self._call("vo_rotor_elements", C.byref(self.geom), ptr(coor), C.c_int(n), ptr(cent), C.c_int(itpwidth),
+                   ptr(box), ptr(dh))
+        return box, dh
+
+    def calc_f_eul(self, coor, cent, F, dA, box, df=10, halfwidth=4.0, width_mode=0, dh_fixed=None):
+        cent = np.ascontiguousarray(cent, dtype=np.float64)
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        dA = np.ascontiguousarray(dA, dtype=np.float64)
+        box = np.ascontiguousarray(box, dtype=np.int32)
+        fx = np.ascontiguousarray(dh_fixed if dh_fixed is not None else np.zeros(3), dtype=np.float64)
+        f = self.vec()
+        self._call("vo_calc_f_eul", C.byref(self.geom), ptr(self.nvert), ptr(coor), C.c_int(cent.shape[1]), ptr(cent),
+                   ptr(F), ptr(dA), ptr(box), C.c_int(df), C.c_double(halfwidth), C.c_int(width_mode), ptr(fx),
+                   ptr(f))
+        return f
+
     def pressure_gradient(self, p, dp):
         self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
 
