@@ -16,6 +16,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #define DIMS  const int mx = c->mx, my = c->my, mz = c->mz; (void)mx; (void)my; (void)mz
 #define IX(k, j, i) ((((size_t)(k)) * (size_t)my + (size_t)(j)) * (size_t)mx + (size_t)(i))
@@ -78,6 +81,17 @@ static int check_cfg(const vo_cfg *c)
     if (c->immersed != 0 && c->immersed != 1) return -4;
     if (c->weno != 0 && c->weno != 1) return -4;
     return 0;
+}
+
+/* threads the oracle's parallel loops use (1 without OpenMP).  Every such
+ * loop writes only its own points, so results do not depend on it. */
+int vo_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* level.c:1734-1771 weno3.  pow(x, 2.) of the reference is written x * x:
@@ -1303,6 +1317,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
         double *dv = dir == 0 ? div1 : dir == 1 ? div2 : div3;
         double *vs = dir == 0 ? visc1 : dir == 1 ? visc2 : visc3;
         /* face loops: i-faces momentum.c:672-931, j 934-1166, k 1169-1409 */
+#pragma omp parallel for schedule(static)
         for (int k = (dir == 2 ? 0 : 1); k < mz - 1; k++)
             for (int j = (dir == 1 ? 0 : 1); j < my - 1; j++)
                 for (int i = (dir == 0 ? 0 : 1); i < mx - 1; i++) {
@@ -1409,6 +1424,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
     per_copy_vec(c, visc1); per_copy_vec(c, visc2); per_copy_vec(c, visc3);
 
     /* cell divergence with 4th-order deferred correction, momentum.c:1506-1636 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {
@@ -1452,6 +1468,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
     per_copy_vec(c, fp); /* momentum.c:1643-1671 */
 
     /* contravariant projection back onto faces, momentum.c:1673-1693 */
+#pragma omp parallel for schedule(static)
     for (int k = 1; k < mz - 1; k++)
         for (int j = 1; j < my - 1; j++)
             for (int i = 1; i < mx - 1; i++) {

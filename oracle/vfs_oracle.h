@@ -43,6 +43,10 @@ typedef struct {
     double *cent;
 } vo_geom;
 
+/* threads of the parallel loops of vo_formfunction2 (OpenMP over k; the
+ * results are the same for any count) */
+int vo_threads(void);
+
 /* metrics.c:4-1346 (NEWMETRIC). coor: node coordinates on the DA layout
  * (nodes i<mx-1, j<my-1, k<mz-1). All geom arrays must be zero-initialised. */
 int vo_form_metrics(const vo_cfg *c, const double *coor, vo_geom *g);
