@@ -43,7 +43,8 @@ typedef struct {
     int second_order;    /* -second_order                              */
     int dpdz_set;        /* k-periodic forcing from mean_pressure_gradient */
     int weno;            /* -weno 1: WENO3 convective face values (level.c:1734) */
-    int reserved;        /* keeps the doubles 8-byte aligned; set to 0 */
+    int inviscid;        /* -inv: WENO3 convection, 2nd-order divergence, no
+                            viscous term (momentum.c:757, 1582, 1621) */
     double ren, dt, st;  /* Reynolds number, time step, Strouhal factor */
     double mean_pressure_gradient;
 } vfs_config;

@@ -5,9 +5,9 @@
  * Every routine follows the reference's loop order and operation order
  * (file:line cited per function), so that it reproduces the reference's
  * floating-point results on the same inputs.  Branches for options that no
- * supported configuration exercises (levelset, rans, skew, clark, weno,
- * wall functions, inflow/outflow bctypes, ii/jj/kk_periodic) are rejected
- * with a negative return code instead of being restated.
+ * supported configuration exercises (levelset, rans, skew, clark,
+ * inflow/outflow bctypes, ii/jj/kk_periodic) are rejected with a negative
+ * return code instead of being restated.
  *
  * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  No FMA
  * contraction, so every a*b+c rounds twice exactly as the reference does.
@@ -80,6 +80,7 @@ static int check_cfg(const vo_cfg *c)
     }
     if (c->immersed != 0 && c->immersed != 1) return -4;
     if (c->weno != 0 && c->weno != 1) return -4;
+    if (c->inviscid != 0 && c->inviscid != 1) return -4;
     return 0;
 }
 
@@ -1384,6 +1385,9 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                         for (int q = 0; q < 3; q++)
                             d3v[q] = um * (0.125 * (-UB(n + 1, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
                                      up * (0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+                    } else if (c->inviscid) {   /* momentum.c:757-773: WENO3 whatever else is set */
+                        for (int q = 0; q < 3; q++)
+                            d3v[q] = -ucon_f * vo_weno3(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f);
                     } else if (c->second_order) {
                         for (int q = 0; q < 3; q++) d3v[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
                     } else if (c->weno) {   /* momentum.c:782-786, 1044-1048, 1283-1287 */
@@ -1448,7 +1452,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                 if (NV(k, j, i - 1) + NV(k, j, i) + NV(k, j, i + 1) > 0.1) iR = i, iL = i - 1, den_i = 1.;
                 if (NV(k, j - 1, i) + NV(k, j, i) + NV(k, j + 1, i) > 0.1) jR = j, jL = j - 1, den_j = 1.;
                 if (NV(k - 1, j, i) + NV(k, j, i) + NV(k + 1, j, i) > 0.1) kR = k, kL = k - 1, den_k = 1.;
-                if (c->second_order) {
+                if (c->second_order || c->inviscid) {   /* momentum.c:1582-1588 */
                     for (int q = 0; q < 3; q++) f[q] = dv[q];
                 } else {
                     size_t a1 = IX(k, j, iR), b1 = IX(k, j, iL), a2 = IX(k, jR, i), b2 = IX(k, jL, i);
@@ -1460,9 +1464,10 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                         f[q] = 9. / 8. * dv[q] + -1. / 8. * d4[q];
                     }
                 }
-                for (int q = 0; q < 3; q++)
-                    f[q] += (visc1[3 * id + q] - visc1[3 * im + q] + visc2[3 * id + q] - visc2[3 * jm + q] +
-                             visc3[3 * id + q] - visc3[3 * km + q]) / 1.0;
+                if (!c->inviscid)   /* momentum.c:1621-1627 */
+                    for (int q = 0; q < 3; q++)
+                        f[q] += (visc1[3 * id + q] - visc1[3 * im + q] + visc2[3 * id + q] - visc2[3 * jm + q] +
+                                 visc3[3 * id + q] - visc3[3 * km + q]) / 1.0;
                 fp[3 * id] = f[0]; fp[3 * id + 1] = f[1]; fp[3 * id + 2] = f[2];
             }
     per_copy_vec(c, fp); /* momentum.c:1643-1671 */

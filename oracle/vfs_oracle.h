@@ -30,7 +30,7 @@ typedef struct {
     int second_order;      /* momentum.c -second_order                              */
     int dpdz_set;          /* 1: k-periodic forcing uses mean_pressure_gradient     */
     int weno;              /* 1: WENO3 convective face values (momentum.c:782-786)  */
-    int reserved;
+    int inviscid;          /* -inv: WENO3 convection, 2nd-order divergence, no viscous term */
     double ren, dt, st;
     double mean_pressure_gradient;
 } vo_cfg;

@@ -98,10 +98,13 @@ def test_formfunction2_reference_passes(case, monkeypatch):
     test_formfunction2(case)
 
 
-def test_formfunction2_weno(case):
+@pytest.mark.parametrize("scheme", [dict(weno=1), dict(inviscid=1), dict(inviscid=1, second_order=1)],
+                         ids=["weno", "inviscid", "inviscid-2nd"])
+def test_formfunction2_weno(case, scheme):
     """-weno 1: WENO3 convective face values (momentum.c:782-786 and the j/k
-    counterparts) on the same fields."""
-    cfg = dict(case["cfg"], weno=1)
+    counterparts); -inv: WENO3 whatever else is set, the 2nd-order divergence
+    and no viscous term (momentum.c:757-773, 1582, 1621) on the same fields."""
+    cfg = dict(case["cfg"], **scheme)
     _, spec, coor, o, ucat, ucont = oracle_case(case["name"])
     o = __import__("oracle_py").Oracle(cfg, coor)
     if spec["immersed"]:

@@ -279,6 +279,23 @@ def test_ff2_poiseuille_closed_form():
     assert np.abs(R[..., 1]).max() < 1e-15 and np.abs(R[..., 2]).max() < 1e-15
 
 
+def test_ff2_inviscid_poiseuille_has_zero_rhs():
+    """-inv drops the viscous term (momentum.c:1621): the same Poiseuille field
+    whose viscous rhs.x is -2 nu dy dz above gives zero, since its WENO3 face
+    values are i-invariant and the transverse fluxes vanish."""
+    Ni, Nj, Nk = 16, 16, 16
+    cfg, coor, o = uniform_case((Ni, Nj, Nk))
+    cfg = dict(cfg, inviscid=1)
+    o = Oracle(cfg, coor)
+    ucat, ucont = V.poiseuille(cfg, W.LY, 0.0, 1, o.g["cent"], o.nvert, o.g["icsi"], o.g["jeta"], o.g["kzet"])
+    u = ucat.copy(); uc = ucont.copy()
+    o.contra2cart(uc, u)
+    rhs = o.vec()
+    o.formfunction2(uc, u, o.sca(), rhs, 1.0)
+    nu, dy, dz = 1.0 / cfg["ren"], W.LY / Nj, W.LZ / Nk
+    assert np.abs(rhs).max() < 1e-12 * 2 * nu * dy * dz   # against the viscous value: roundoff only
+
+
 def test_ff2_uniform_flow_has_zero_rhs():
     """Triply periodic uniform grid, constant velocity: every face flux is the
     same, so divergence, 4th-order correction and viscous terms vanish."""

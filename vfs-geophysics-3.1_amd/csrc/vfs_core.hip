@@ -112,6 +112,7 @@ static int check_config(const vfs_config *cfg)
     }
     if (cfg->immersed != 0 && cfg->immersed != 1) { vfs_set_error("immersed must be 0 or 1"); return -4; }
     if (cfg->weno != 0 && cfg->weno != 1) { vfs_set_error("weno must be 0 or 1"); return -4; }
+    if (cfg->inviscid != 0 && cfg->inviscid != 1) { vfs_set_error("inviscid must be 0 or 1"); return -4; }
     if (cfg->k_periodic && !cfg->dpdz_set) {
         vfs_set_error("k-periodic flux forcing (inlet_flux) is not supported; set dpdz_set");
         return -5;
@@ -134,6 +135,7 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     d.immersed = cfg->immersed; d.les = cfg->les; d.second_order = cfg->second_order;
     d.dpdz_set = cfg->dpdz_set;
     d.weno = cfg->weno;
+    d.inviscid = cfg->inviscid;
     d.ren = cfg->ren; d.dt = cfg->dt; d.st = cfg->st; d.mpg = cfg->mean_pressure_gradient;
     d.P = (long long)d.mx * d.my * d.mz;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; vfs_set_error("no HIP device"); return -1; }

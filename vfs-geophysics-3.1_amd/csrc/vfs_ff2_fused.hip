@@ -330,7 +330,7 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
             double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
                          c2[q] - K2(km));
             double r;
-            if (d.second_order) {
+            if (d.second_order || d.inviscid) {
                 r = dv;
             } else {
                 double d4 = 0;
@@ -339,8 +339,9 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
                 d4 += (K2(cf.kR) - K2(cf.kL)) * (1. / cf.den_k);
                 r = 9. / 8. * dv + -1. / 8. * d4;
             }
-            r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) + v1[q] -
-                  K5(km)) / 1.0;
+            if (!d.inviscid)
+                r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) + v1[q] -
+                      K5(km)) / 1.0;
             f[q] = r;
         }
         const unsigned t = col + k * sk;
@@ -489,7 +490,7 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     const Dims d = c->d;
     const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);
-    if (d.weno)
+    if (d.weno || d.inviscid)
         hipLaunchKernelGGL(k_ff2_faces_lds<true>, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t,
                            c->face, kchunk);
     else

@@ -19,7 +19,7 @@ struct Dims {
     int mx, my, mz;
     int ip, jp, kp;          // i/j/k_periodic
     int bc[6];               // bctype
-    int immersed, les, second_order, dpdz_set, weno;
+    int immersed, les, second_order, dpdz_set, weno, inviscid;
     double ren, dt, st, mpg;
     long long P;             // mx*my*mz
 };

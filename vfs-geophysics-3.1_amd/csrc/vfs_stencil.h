@@ -241,6 +241,9 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
         for (int q = 0; q < 3; q++)
             D[q] = um * (0.125 * (-UB(n + 1, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
                    up * (0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
+    } else if (WENO && d.inviscid) {   // -inv: WENO3 whatever else is set (momentum.c:757-773)
+#pragma unroll
+        for (int q = 0; q < 3; q++) D[q] = -ucon_f * weno3<S>(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f);
     } else if (d.second_order) {
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.5 * (UB(n, q) + UB(n + 1, q));
@@ -349,7 +352,7 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
         double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
                      FI(2, q, k, j, i) - FI(2, q, km, j, i));
         double r;
-        if (d.second_order) {
+        if (d.second_order || d.inviscid) {   // momentum.c:1582-1588
             r = dv;
         } else {
             double d4 = 0;
@@ -358,8 +361,9 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
             d4 += (FI(2, q, c.kR, j, i) - FI(2, q, c.kL, j, i)) * (1. / c.den_k);
             r = 9. / 8. * dv + -1. / 8. * d4;
         }
-        r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) +
-              FI(5, q, k, j, i) - FI(5, q, km, j, i)) / 1.0;
+        if (!d.inviscid)   // momentum.c:1621-1627
+            r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) +
+                  FI(5, q, k, j, i) - FI(5, q, km, j, i)) / 1.0;
         f[q] = r;
     }
 #undef FI

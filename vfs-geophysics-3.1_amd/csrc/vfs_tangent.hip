@@ -146,6 +146,7 @@ extern "C" int vfs_rhs_jvp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_REQUIRE(nu_t || !c->d.les, "nu_t required when les is on");
     VFS_REQUIRE(!vfs_has_wallfn(c->d),
                 "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
+    VFS_REQUIRE(!c->d.inviscid, "the derivatives do not cover the inviscid scheme (-inv)");
     VFS_REQUIRE(U && ucat && w && jw, "null argument");
     VFS_REQUIRE(w != jw, "w and jw must not alias");
     if (vfs_rhs_prepare(c, U, ucat)) return -1;
@@ -421,6 +422,7 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_REQUIRE(nu_t || !d.les, "nu_t required when les is on");
     VFS_REQUIRE(!vfs_has_wallfn(d),
                 "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
+    VFS_REQUIRE(!d.inviscid, "the derivatives do not cover the inviscid scheme (-inv)");
     VFS_REQUIRE(U && ucat && v && jtv, "null argument");
     VFS_REQUIRE(v != jtv, "v and jtv must not alias");
     if (ensure_ad(c)) return -1;
