@@ -290,6 +290,51 @@ def actuator_leg(ctx, cfg, spec, reps, with_cpu):
     return out
 
 
+def ib_leg(ctx, cfg, spec, s, reps, with_cpu):
+    """CURVIB reconstruction at the immersed sphere: the node search against
+    the sphere as 5120 triangles (once per mask, or every step for a moving
+    body) and the per-step ibm_interpolation_advanced pass (node values,
+    element means, IB-face fluxes, Contra2Cart), on copies of the state."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    x, e, n = W.icosphere(*W.SPHERE, level=4)
+    surf = V.IbSurface(x, e, n)
+    nodes, nn = ctx.ib_search(surf)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nodes, nn = ctx.ib_search(surf)
+    search_ms = (time.perf_counter() - t0) / reps * 1e3
+    uc, u, p, ustar = ctx.vec(), ctx.vec(), ctx.sca(), ctx.sca()
+    uc.copy_from(s.uc)
+    u.copy_from(s.u)
+    p.copy_from(s.p)
+    ctx.ib_interpolate(surf, nodes, nn, uc, u, p, ustar)
+    ctx.sync()
+    ev = HipEvents()
+    a, b = ev.event(), ev.event()
+    ev.record(a)
+    for _ in range(reps):
+        ctx.ib_interpolate(surf, nodes, nn, uc, u, p, ustar)
+    ev.record(b)
+    interp_ms = ev.elapsed_ms(a, b) / reps
+    out = {"ib_nodes": nn, "elements": len(e), "search_ms": round(search_ms, 4), "interpolate_ms": round(interp_ms, 4)}
+    if with_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_py import IbBody, Oracle
+        o = Oracle(cfg, W.grid(spec))
+        o.nvert_sphere(*W.SPHERE)
+        body = IbBody(x, e, n)
+        t0 = time.perf_counter()
+        onodes = o.ib_search(body)
+        out["cpu_oracle_search_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+        ucont, ucat = s.uc.host(), s.u.host()
+        t0 = time.perf_counter()
+        o.ib_interpolate(body, onodes, ucont, ucat, s.p.host(), np.zeros(o.P))
+        out["cpu_oracle_interpolate_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
+    return out
+
+
 def traffic_from_profile(name):
     """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
@@ -382,6 +427,7 @@ def main():
     ap.add_argument("--no-implicit", action="store_true", help="skip the Newton-Krylov leg")
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic Smagorinsky leg")
     ap.add_argument("--no-actuator", action="store_true", help="skip the actuator-disk leg")
+    ap.add_argument("--no-ib", action="store_true", help="skip the CURVIB reconstruction leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -452,6 +498,11 @@ def main():
         act = actuator_leg(ctx, cfg, spec, 5, not args.no_cpu)
         log(f"[bench] actuator disk: {act}")
 
+    ib = None
+    if not args.no_ib and cfg["immersed"]:
+        ib = ib_leg(ctx, cfg, spec, s, 5, not args.no_cpu)
+        log(f"[bench] CURVIB reconstruction: {ib}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -490,6 +541,7 @@ def main():
         "implicit_cfl2": implicit,
         "dynamic_les": dyn,
         "actuator_disk": act,
+        "ib_reconstruction": ib,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -144,6 +144,54 @@ int vfs_calc_f_eul(vfs_ctx *ctx, const double *coor, int n, const double *cent, 
  * (momentum.c:2288, nacelle_model / rotor_model); NULL detaches. */
 int vfs_set_body_force(vfs_ctx *ctx, const double *f_eul);
 
+/* ---- CURVIB velocity reconstruction (ibm.c) -----------------------------
+ * A triangulated body as the reference's IBMNodes holds it: n_v vertices,
+ * n_elmt elements with unit normals pointing into the fluid, the vertex
+ * velocities (3*n_v, Cmpnts AoS; NULL = a body at rest).  Device arrays. */
+typedef struct {
+    int n_v, n_elmt;
+    const double *x_bp, *y_bp, *z_bp;
+    const int *nv1, *nv2, *nv3;
+    const double *nf_x, *nf_y, *nf_z;
+    const double *u;
+} vfs_ib_surface;
+/* One IB node and the geometry of its reconstruction, the fields of IBMInfo
+ * (variables.h:126-172) that ibm_search_advanced fills. */
+typedef struct {
+    int i, j, k;                    /* the node (ni, nj, nk) */
+    int cell;                       /* the nearest element */
+    int imode;                      /* the patch triangle hit (0..7); -100: the boundary fallback */
+    int i1, j1, k1, i2, j2, k2, i3, j3, k3;   /* the three interpolation cells */
+    double cr1, cr2, cr3;           /* their weights */
+    double cs1, cs2, cs3;           /* the nearest point's weights on the element */
+    double d_s;                     /* node to surface */
+    double d_i;                     /* node to the interception point */
+    double pmin[3];                 /* the nearest point on the surface */
+} vfs_ib_node;
+/* The node list of ibm_search_advanced (ibm.c:366-497) for a mask already
+ * set (vfs_set_nvert / vfs_nvert_sphere): every interior cell with
+ * (int)(nvert+0.5) == 1, except the planes of wall-function walls, gets the
+ * nearest element and point (nearestcell1, ibm.c:1366-1511, with the
+ * bounding spheres of ibm.c:1276-1364) and the interception of the normal
+ * ray with the cell-centre patches (InterceptionPoint, ibm.c:1759-2250), or
+ * the boundary fallback (ibm.c:413-493).  nodes: device array of cap
+ * entries, written in the reference's list order (AddIBMNode prepends: the
+ * last cell in k, j, i order comes first); *n_nodes (host) the count.  -3 if
+ * a node has no element or no interception (the reference prints "IBM
+ * Searching Fail!"), -4 if cap is too small.  Synchronizes the stream. */
+int vfs_ib_search(vfs_ctx *ctx, const vfs_ib_surface *surf, vfs_ib_node *nodes, int cap, int *n_nodes);
+/* ibm_interpolation_advanced (ibm.c:3294-3698) without averaging, levelset,
+ * FSI or an IB wall model: ucat at each node from the interpolated velocity
+ * along the normal (noslip, or freeslip with -inv, wallfunction.c:134-166;
+ * zero when an interpolation cell is not fluid or the node is solid), p from
+ * the interpolated pressure, ustar (a P field) from the noslip model; then
+ * the element means count / ustar^2 / p (n_elmt each, may all be NULL), the
+ * fluxes of the faces of every nvert == 1 cell from the new ucat, and
+ * Contra2Cart.  ucont, ucat, p, ustar: in/out device fields.  Reads see the
+ * fields as they were before the call. */
+int vfs_ib_interpolate(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n, double *ucont,
+                       double *ucat, double *p, double *ustar, int *count, double *shear, double *pressure);
+
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

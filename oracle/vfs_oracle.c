@@ -2592,3 +2592,413 @@ int vo_rk_stage(const vo_cfg *c, const vo_geom *g, const double *nvert,
     }
     return vo_contra2cart(c, g, nvert, ucont, ucat);
 }
+
+/* ------------------------------------------------------------------------ */
+/* CURVIB reconstruction: ibm_search_advanced's node geometry (ibm.c:366-497, */
+/* BoundingSphere 1276-1364, nearestcell1 1366-1511, ICP 1659-1757,          */
+/* InterceptionPoint 1759-2250; compgeom.c:56-230) and                        */
+/* ibm_interpolation_advanced (ibm.c:3294-3698)                               */
+/* ------------------------------------------------------------------------ */
+
+/* ibm.c:28 Dist */
+static double ib_dist(const double a[3], const double b[3])
+{
+    return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+static void ib_vertex(const vo_ib_surface *s, int v, double p[3])
+{
+    p[0] = s->x_bp[v]; p[1] = s->y_bp[v]; p[2] = s->z_bp[v];
+}
+
+/* ibm.c:30-33 Cross */
+static void ib_cross(double r[3], const double a[3], const double b[3])
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+/* ibm.c:1276-1364 BoundingSphere of element e: centre q, radius */
+void vo_ib_bounding_sphere(const vo_ib_surface *s, int e, double q[3], double *rad)
+{
+    double p1[3], p2[3], p3[3], pa[3], pb[3], pc[3], pf[3], pu[3], pv[3], pt[3], pd[3];
+    const double p0[3] = {0, 0, 0};
+    ib_vertex(s, s->nv1[e], p1); ib_vertex(s, s->nv2[e], p2); ib_vertex(s, s->nv3[e], p3);
+    const double l12 = ib_dist(p1, p2), l23 = ib_dist(p2, p3), l31 = ib_dist(p3, p1);
+    const double *a, *b, *cc;   /* the longest edge is a-b */
+    if (l12 > l23) {
+        if (l12 > l31) a = p1, b = p2, cc = p3;
+        else a = p3, b = p1, cc = p2;
+    } else {
+        if (l31 < l23) a = p2, b = p3, cc = p1;
+        else a = p3, b = p1, cc = p2;
+    }
+    for (int q0 = 0; q0 < 3; q0++) pa[q0] = a[q0], pb[q0] = b[q0], pc[q0] = cc[q0];
+    for (int q0 = 0; q0 < 3; q0++) pf[q0] = 0.5 * (pa[q0] + pb[q0]);
+    for (int q0 = 0; q0 < 3; q0++) pu[q0] = pa[q0] - pf[q0], pv[q0] = pc[q0] - pf[q0];
+    ib_cross(pt, pu, pv);
+    ib_cross(pd, pt, pu);
+    double gama = -(ib_dist(pu, p0) * ib_dist(pu, p0) - ib_dist(pv, p0) * ib_dist(pv, p0));
+    for (int q0 = 0; q0 < 3; q0++) pt[q0] = pv[q0] - pu[q0];
+    double lamda = 2 * (pd[0] * pt[0] + pd[1] * pt[1] + pd[2] * pt[2]);
+    gama /= lamda;
+    lamda = gama < 0 ? 0 : gama;
+    for (int q0 = 0; q0 < 3; q0++) q[q0] = pf[q0] + lamda * pd[q0];
+    *rad = ib_dist(q, pa);
+}
+
+/* compgeom.c:56-110 ISSameSide2D, as it runs: the t1/t2 of the "Change"
+ * block decide, with |t1|/|p1p2| < 1e-10 counted as on the line */
+static int ib_same_side(const double p[2], const double p1[2], const double p2[2], const double p3[2])
+{
+    const double t1 = (p[0] - p1[0]) * (p2[1] - p1[1]) - (p[1] - p1[1]) * (p2[0] - p1[0]);
+    const double t2 = (p3[0] - p1[0]) * (p2[1] - p1[1]) - (p3[1] - p1[1]) * (p2[0] - p1[0]);
+    const double lt = sqrt((p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]));
+    if (fabs(t1 / lt) < 1.e-10) return 1;
+    if (t1 > 0) return t2 > 0 ? 1 : -1;
+    return t2 < 0 ? 1 : -1;
+}
+
+/* compgeom.c:112-158 ISInsideTriangle2D on the plane ISPointInTriangle
+ * chooses from the normal: 1 inside (or on an edge), -1 outside */
+static int ib_point_in_triangle(const double p[3], const double p1[3], const double p2[3], const double p3[3],
+                                double nx, double ny, double nz)
+{
+    int u, v;
+    if (fabs(nz) >= fabs(nx) && fabs(nz) >= fabs(ny)) u = 0, v = 1;
+    else if (fabs(nx) >= fabs(ny) && fabs(nx) >= fabs(nz)) u = 2, v = 1;
+    else u = 0, v = 2;
+    const double a[2] = {p[u], p[v]}, b1[2] = {p1[u], p1[v]}, b2[2] = {p2[u], p2[v]}, b3[2] = {p3[u], p3[v]};
+    int ls = ib_same_side(a, b1, b2, b3);
+    if (ls < 0) return ls;
+    ls = ib_same_side(a, b2, b3, b1);
+    if (ls < 0) return ls;
+    return ib_same_side(a, b3, b1, b2);
+}
+
+/* compgeom.c:160-186 Dis_P_Line: closest point po on segment p1p2, distance d */
+static void ib_dis_p_line(const double p[3], const double p1[3], const double p2[3], double po[3], double *d)
+{
+    const double dx21 = p2[0] - p1[0], dy21 = p2[1] - p1[1], dz21 = p2[2] - p1[2];
+    const double dx31 = p[0] - p1[0], dy31 = p[1] - p1[1], dz31 = p[2] - p1[2];
+    const double t = (dx31 * dx21 + dy31 * dy21 + dz31 * dz21) / (dx21 * dx21 + dy21 * dy21 + dz21 * dz21);
+    if (t < 0) {
+        po[0] = p1[0], po[1] = p1[1], po[2] = p1[2];
+        *d = sqrt(dx31 * dx31 + dy31 * dy31 + dz31 * dz31);
+        return;
+    }
+    if (t > 1) po[0] = p2[0], po[1] = p2[1], po[2] = p2[2];
+    else po[0] = p1[0] + t * dx21, po[1] = p1[1] + t * dy21, po[2] = p1[2] + t * dz21;
+    *d = sqrt((p[0] - po[0]) * (p[0] - po[0]) + (p[1] - po[1]) * (p[1] - po[1]) + (p[2] - po[2]) * (p[2] - po[2]));
+}
+
+/* compgeom.c:188-230 triangle_intp2 / triangle_intp: barycentric weights of
+ * p in the plane (u, v); the projection is chosen as nearestcell1 and ICP do
+ * (x-dominant: (y, z); y-dominant: (x, z); z-dominant: (y, x)) */
+static void ib_triangle_intp(const double p[3], const double p1[3], const double p2[3], const double p3[3],
+                             double nx, double ny, double nz, double w[3])
+{
+    int u, v;
+    if (fabs(nx) >= fabs(ny) && fabs(nx) >= fabs(nz)) u = 1, v = 2;
+    else if (fabs(ny) >= fabs(nx) && fabs(ny) >= fabs(nz)) u = 0, v = 2;
+    else if (fabs(nz) >= fabs(ny) && fabs(nz) >= fabs(nx)) u = 1, v = 0;
+    else return;   /* NaN normal: the weights are left as they were */
+    const double x13 = p1[u] - p3[u], y13 = p1[v] - p3[v];
+    const double x23 = p2[u] - p3[u], y23 = p2[v] - p3[v];
+    const double xp3 = p[u] - p3[u], yp3 = p[v] - p3[v];
+    const double a = x13 * y23 - x23 * y13;
+    w[0] = (y23 * xp3 - x23 * yp3) / a;
+    w[1] = (-y13 * xp3 + x13 * yp3) / a;
+    w[2] = 1. - w[0] - w[1];
+}
+
+/* ibm.c:1366-1511 nearestcell1: the closest surface element to p among the
+ * elements whose plane has p on its positive side, scanned in element order
+ * with the bounding-sphere cut.  Returns -1 if no element qualifies. */
+static int ib_nearest(const vo_ib_surface *s, const double *qv, const double *rad, const double p[3],
+                      vo_ib_node *nd)
+{
+    double dmin = 1.e20, pmin[3] = {0, 0, 0};
+    int cell_min = -100;
+    for (int e = 0; e < s->n_elmt; e++) {
+        const double q[3] = {qv[3 * e], qv[3 * e + 1], qv[3 * e + 2]};
+        const double d_center = ib_dist(p, q);
+        if (!(d_center - rad[e] < dmin)) continue;
+        const int n1 = s->nv1[e];
+        const double nfx = s->nf_x[e], nfy = s->nf_y[e], nfz = s->nf_z[e];
+        double p1[3], p2[3], p3[3], po[3], d;
+        ib_vertex(s, n1, p1); ib_vertex(s, s->nv2[e], p2); ib_vertex(s, s->nv3[e], p3);
+        double tf = ((p[0] - s->x_bp[n1]) * nfx + (p[1] - s->y_bp[n1]) * nfy + (p[2] - s->z_bp[n1]) * nfz);
+        if (fabs(tf) < 1.e-10) tf = 1.e-10;
+        if (!(tf >= 0)) continue;
+        const double pj[3] = {p[0] - tf * nfx, p[1] - tf * nfy, p[2] - tf * nfz};
+        if (ib_point_in_triangle(pj, p1, p2, p3, nfx, nfy, nfz) == 1) {
+            if (tf < dmin) dmin = tf, pmin[0] = pj[0], pmin[1] = pj[1], pmin[2] = pj[2], cell_min = e;
+        } else {
+            ib_dis_p_line(p, p1, p2, po, &d);
+            if (d < dmin) dmin = d, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+            ib_dis_p_line(p, p2, p3, po, &d);
+            if (d < dmin) dmin = d, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+            ib_dis_p_line(p, p3, p1, po, &d);
+            if (d < dmin) dmin = d, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+        }
+    }
+    if (cell_min == -100) return -1;
+    nd->cell = cell_min;
+    nd->pmin[0] = pmin[0], nd->pmin[1] = pmin[1], nd->pmin[2] = pmin[2];
+    nd->d_s = dmin;
+    double p1[3], p2[3], p3[3], w[3] = {nd->cs1, nd->cs2, nd->cs3};
+    ib_vertex(s, s->nv1[cell_min], p1); ib_vertex(s, s->nv2[cell_min], p2); ib_vertex(s, s->nv3[cell_min], p3);
+    ib_triangle_intp(pmin, p1, p2, p3, s->nf_x[cell_min], s->nf_y[cell_min], s->nf_z[cell_min], w);
+    nd->cs1 = w[0], nd->cs2 = w[1], nd->cs3 = w[2];
+    return 0;
+}
+
+/* ibm.c:1670-1677: the eight triangles of a 3x3 patch of cell centres; the
+ * interception point's three cells are the vertices of the triangle hit
+ * (ibm.c:1800-1847 and the five like switches) */
+static const int IB_TRI[8][3] = {{0, 1, 4}, {1, 2, 4}, {2, 4, 5}, {4, 5, 8}, {4, 7, 8}, {4, 6, 7}, {3, 4, 6}, {3, 4, 0}};
+
+/* patch point n (0..8) of candidate plane pl (0..5: i-1, i+1, j-1, j+1,
+ * k-1, k+1) around cell (i, j, k), ibm.c:1785-1797 and the like blocks */
+void vo_ib_patch_point(int pl, int n, int i, int j, int k, int *ii, int *jj, int *kk)
+{
+    const int a = n % 3 - 1, b = n / 3 - 1;   /* fastest, slowest offsets of the patch */
+    const int s = (pl & 1) ? 1 : -1;
+    if (pl < 2) *ii = i + s, *jj = j + a, *kk = k + b;
+    else if (pl < 4) *ii = i + a, *jj = j + s, *kk = k + b;
+    else *ii = i + a, *jj = j + b, *kk = k + s;
+}
+
+/* ibm.c:1659-1757 ICP on one patch: the first of the eight triangles that
+ * the ray p + d n (d > 0) hits */
+static int ib_icp(const double p[3], const double pc[9][3], const double n[3], vo_ib_node *nd)
+{
+    const double nfx = n[0], nfy = n[1], nfz = n[2];
+    for (int t = 0; t < 8; t++) {
+        const double *p1 = pc[IB_TRI[t][0]], *p2 = pc[IB_TRI[t][1]], *p3 = pc[IB_TRI[t][2]];
+        const double dx1 = p[0] - p1[0], dy1 = p[1] - p1[1], dz1 = p[2] - p1[2];
+        const double dx2 = p2[0] - p1[0], dy2 = p2[1] - p1[1], dz2 = p2[2] - p1[2];
+        const double dx3 = p3[0] - p1[0], dy3 = p3[1] - p1[1], dz3 = p3[2] - p1[2];
+        double d = (nfx * (dy2 * dz3 - dz2 * dy3) - nfy * (dx2 * dz3 - dz2 * dx3) + nfz * (dx2 * dy3 - dy2 * dx3));
+        if (!(fabs(d) > 1.e-10)) continue;
+        d = -(dx1 * (dy2 * dz3 - dz2 * dy3) - dy1 * (dx2 * dz3 - dz2 * dx3) + dz1 * (dx2 * dy3 - dy2 * dx3)) / d;
+        if (!(d > 0)) continue;
+        const double pint[3] = {p[0] + d * nfx, p[1] + d * nfy, p[2] + d * nfz};
+        const double rx1 = p2[0] - p1[0], ry1 = p2[1] - p1[1], rz1 = p2[2] - p1[2];
+        const double rx2 = p3[0] - p1[0], ry2 = p3[1] - p1[1], rz2 = p3[2] - p1[2];
+        const double nfxt = ry1 * rz2 - rz1 * ry2, nfyt = -rx1 * rz2 + rz1 * rx2, nfzt = rx1 * ry2 - ry1 * rx2;
+        if (ib_point_in_triangle(pint, p1, p2, p3, nfxt, nfyt, nfzt) < 0) continue;
+        double w[3] = {nd->cr1, nd->cr2, nd->cr3};
+        ib_triangle_intp(pint, p1, p2, p3, nfxt, nfyt, nfzt, w);
+        nd->cr1 = w[0], nd->cr2 = w[1], nd->cr3 = w[2];
+        nd->d_i = sqrt((pint[0] - p[0]) * (pint[0] - p[0]) + (pint[1] - p[1]) * (pint[1] - p[1]) +
+                       (pint[2] - p[2]) * (pint[2] - p[2]));
+        return t;
+    }
+    return -100;
+}
+
+/* ibm.c:1759-2250 InterceptionPoint: the six patches in order; then the
+ * boundary fallback of ibm_search_advanced (ibm.c:413-493).  Returns -1 if
+ * the node has neither (the reference prints "IBM Searching Fail!" and keeps
+ * the previous node's values). */
+static int ib_intercept(const vo_cfg *c, const vo_geom *g, const vo_ib_surface *s, vo_ib_node *nd)
+{
+    DIMS;
+    const int i = nd->i, j = nd->j, k = nd->k;
+    const double *p = &g->cent[3 * IX(k, j, i)];
+    double n[3] = {p[0] - nd->pmin[0], p[1] - nd->pmin[1], p[2] - nd->pmin[2]};
+    const double dr = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    n[0] /= dr, n[1] /= dr, n[2] /= dr;
+    nd->imode = -100;
+    for (int pl = 0; pl < 6; pl++) {
+        int ip[9], jp[9], kp[9];
+        double pc[9][3];
+        for (int q = 0; q < 9; q++) {
+            vo_ib_patch_point(pl, q, i, j, k, &ip[q], &jp[q], &kp[q]);
+            const double *x = &g->cent[3 * IX(kp[q], jp[q], ip[q])];
+            pc[q][0] = x[0], pc[q][1] = x[1], pc[q][2] = x[2];
+        }
+        const int t = ib_icp(p, pc, n, nd);
+        nd->imode = t;
+        if (t >= 0) {
+            nd->i1 = ip[IB_TRI[t][0]], nd->j1 = jp[IB_TRI[t][0]], nd->k1 = kp[IB_TRI[t][0]];
+            nd->i2 = ip[IB_TRI[t][1]], nd->j2 = jp[IB_TRI[t][1]], nd->k2 = kp[IB_TRI[t][1]];
+            nd->i3 = ip[IB_TRI[t][2]], nd->j3 = jp[IB_TRI[t][2]], nd->k3 = kp[IB_TRI[t][2]];
+            return 0;
+        }
+    }
+    int ti, tj, tk;
+    if (i == 1 || i == mx - 2 || j == 1 || j == my - 2) ti = i, tj = j, tk = s->nf_z[nd->cell] > 0 ? k + 1 : k - 1;
+    else if (k == 1 || k == mz - 2) ti = i, tj = j + 1, tk = k;
+    else return -1;
+    const double *pt = &g->cent[3 * IX(tk, tj, ti)];
+    nd->d_i = sqrt((p[0] - pt[0]) * (p[0] - pt[0]) + (p[1] - pt[1]) * (p[1] - pt[1]) + (p[2] - pt[2]) * (p[2] - pt[2]));
+    nd->cr1 = 1., nd->cr2 = 0., nd->cr3 = 0.;
+    nd->i1 = nd->i2 = nd->i3 = ti;
+    nd->j1 = nd->j2 = nd->j3 = tj;
+    nd->k1 = nd->k2 = nd->k3 = tk;
+    return 0;
+}
+
+/* the body's IB nodes: interior cells with (int)(nvert+0.5) == 1, except
+ * the planes of wall-function walls, which ibm.c:518-533 sets to 1 only
+ * after the list is made */
+static int ib_is_node(const vo_cfg *c, const double *nvert, int i, int j, int k)
+{
+    DIMS;
+    if ((int)(nvert[IX(k, j, i)] + 0.5) != 1) return 0;
+    const int *bt = c->bctype;
+    if (((bt[0] == -1 || bt[0] == -2) && i == 1) || ((bt[1] == -1 || bt[1] == -2) && i == mx - 2)) return 0;
+    if (((bt[2] == -1 || bt[2] == -2) && j == 1) || ((bt[3] == -1 || bt[3] == -2) && j == my - 2)) return 0;
+    if (((bt[4] == -1 || bt[4] == -2) && k == 1) || ((bt[5] == -1 || bt[5] == -2) && k == mz - 2)) return 0;
+    return 1;
+}
+
+int vo_ib_search(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s, vo_ib_node *nodes,
+                 int cap, int *n_nodes)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    if (s->n_elmt <= 0 || s->n_v <= 0) return -2;
+    double *qv = malloc(3 * sizeof(double) * s->n_elmt), *rad = malloc(sizeof(double) * s->n_elmt);
+    for (int e = 0; e < s->n_elmt; e++) vo_ib_bounding_sphere(s, e, &qv[3 * e], &rad[e]);
+    int n = 0;
+    rc = 0;
+    for (int k = 1; k < mz - 1 && !rc; k++)
+        for (int j = 1; j < my - 1 && !rc; j++)
+            for (int i = 1; i < mx - 1 && !rc; i++) {
+                if (!ib_is_node(c, nvert, i, j, k)) continue;
+                if (n >= cap) { rc = -4; break; }
+                vo_ib_node *nd = &nodes[n++];
+                memset(nd, 0, sizeof *nd);
+                nd->i = i, nd->j = j, nd->k = k;
+                if (ib_nearest(s, qv, rad, &g->cent[3 * IX(k, j, i)], nd) || ib_intercept(c, g, s, nd)) rc = -3;
+            }
+    free(qv);
+    free(rad);
+    if (rc) return rc;
+    /* AddIBMNode (variables.c:43-51) prepends: the list runs from the last
+     * node found to the first */
+    for (int a = 0, b = n - 1; a < b; a++, b--) {
+        vo_ib_node t = nodes[a];
+        nodes[a] = nodes[b];
+        nodes[b] = t;
+    }
+    *n_nodes = n;
+    return 0;
+}
+
+int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
+                      const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
+                      int *count, double *shear, double *pressure)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    const size_t P = (size_t)mx * my * mz;
+    /* reads see the fields as they were before the pass (lUcat, lP) */
+    double *lucat = malloc(3 * P * sizeof(double)), *lp = malloc(P * sizeof(double));
+    memcpy(lucat, ucat, 3 * P * sizeof(double));
+    memcpy(lp, p, P * sizeof(double));
+    for (int e = 0; e < s->n_elmt; e++) count[e] = 0, shear[e] = 0, pressure[e] = 0;
+    for (int l = 0; l < n; l++) {
+        const vo_ib_node *nd = &nodes[l];
+        const int ni = nd->cell;
+        const size_t id = IX(nd->k, nd->j, nd->i);
+        const size_t q1 = IX(nd->k1, nd->j1, nd->i1), q2 = IX(nd->k2, nd->j2, nd->i2), q3 = IX(nd->k3, nd->j3, nd->i3);
+        const double sb = nd->d_s, sc = sb + nd->d_i;
+        const double sk1 = nd->cr1, sk2 = nd->cr2, sk3 = nd->cr3;
+        const double cs1 = nd->cs1, cs2 = nd->cs2, cs3 = nd->cs3;
+        const double nfx = s->nf_x[ni], nfy = s->nf_y[ni], nfz = s->nf_z[ni];
+        double Ua[3] = {0, 0, 0}, Uc[3], Ub[3];
+        if (ni >= 0 && s->u) {
+            const double *u1 = &s->u[3 * s->nv1[ni]], *u2 = &s->u[3 * s->nv2[ni]], *u3 = &s->u[3 * s->nv3[ni]];
+            for (int q = 0; q < 3; q++) Ua[q] = u1[q] * cs1 + u2[q] * cs2 + u3[q] * cs3;
+        }
+        for (int q = 0; q < 3; q++) Uc[q] = (lucat[3 * q1 + q] * sk1 + lucat[3 * q2 + q] * sk2 + lucat[3 * q3 + q] * sk3);
+        if (!c->inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) {
+            Ub[0] = Ub[1] = Ub[2] = 0;
+            ustar[id] = 0;
+        } else if (nvert[id] > 2.9) {
+            Ub[0] = Ub[1] = Ub[2] = 0;
+            ustar[id] = 0;
+        } else {
+            const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
+            const double un = u_c * nfx + v_c * nfy + w_c * nfz;
+            const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+            if (c->inviscid) {   /* freeslip, wallfunction.c:152-166 */
+                Ub[0] = ut + sb / sc * un * nfx;
+                Ub[1] = vt + sb / sc * un * nfy;
+                Ub[2] = wt + sb / sc * un * nfz;
+            } else {             /* noslip, wallfunction.c:134-150 */
+                const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+                ustar[id] = sqrt(ut_mag / sc / c->ren);
+                Ub[0] = sb / sc * u_c;
+                Ub[1] = sb / sc * v_c;
+                Ub[2] = sb / sc * w_c;
+            }
+            Ub[0] += Ua[0], Ub[1] += Ua[1], Ub[2] += Ua[2];
+        }
+        for (int q = 0; q < 3; q++) ucat[3 * id + q] = Ub[q];
+        p[id] = (lp[q1] * sk1 + lp[q2] * sk2 + lp[q3] * sk3);
+        count[ni]++;
+        shear[ni] += ustar[id] * ustar[id];
+        pressure[ni] += p[id];
+    }
+    for (int e = 0; e < s->n_elmt; e++)
+        if (count[e]) shear[e] /= (double)count[e], pressure[e] /= (double)count[e];
+    free(lucat);
+    free(lp);
+    /* fluxes of the faces of IB nodes from the new velocity, ibm.c:3619-3679 */
+    const int *bt = c->bctype;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i), ii = IX(k, j, i + 1), jj = IX(k, j + 1, i), kk = IX(k + 1, j, i);
+                const double *u0 = &ucat[3 * id];
+                double ucx, ucy, ucz;
+                if ((int)(nvert[id] + 0.5) == 1) {
+                    ucx = (u0[0] + ucat[3 * ii]) * 0.5;
+                    ucy = (u0[1] + ucat[3 * ii + 1]) * 0.5;
+                    ucz = (u0[2] + ucat[3 * ii + 2]) * 0.5;
+                    VX(ucont, id) = (ucx * VX(g->icsi, id) + ucy * VY(g->icsi, id) + ucz * VZ(g->icsi, id)) * 1.0;
+                    ucx = (ucat[3 * jj] + u0[0]) * 0.5;
+                    ucy = (ucat[3 * jj + 1] + u0[1]) * 0.5;
+                    ucz = (ucat[3 * jj + 2] + u0[2]) * 0.5;
+                    VY(ucont, id) = (ucx * VX(g->jeta, id) + ucy * VY(g->jeta, id) + ucz * VZ(g->jeta, id)) * 1.0;
+                    ucx = (ucat[3 * kk] + u0[0]) * 0.5;
+                    ucy = (ucat[3 * kk + 1] + u0[1]) * 0.5;
+                    ucz = (ucat[3 * kk + 2] + u0[2]) * 0.5;
+                    VZ(ucont, id) = (ucx * VX(g->kzet, id) + ucy * VY(g->kzet, id) + ucz * VZ(g->kzet, id)) * 1.0;
+                    if ((bt[0] == -1 || bt[0] == -2) && i == 1) VX(ucont, id) = 0;
+                    if ((bt[2] == -1 || bt[2] == -2) && j == 1) VY(ucont, id) = 0;
+                    if ((bt[4] == -1 || bt[4] == -2) && k == 1) VZ(ucont, id) = 0;
+                }
+                if ((int)(nvert[ii] + 0.5) == 1) {
+                    ucx = (u0[0] + ucat[3 * ii]) * 0.5;
+                    ucy = (u0[1] + ucat[3 * ii + 1]) * 0.5;
+                    ucz = (u0[2] + ucat[3 * ii + 2]) * 0.5;
+                    VX(ucont, id) = (ucx * VX(g->icsi, id) + ucy * VY(g->icsi, id) + ucz * VZ(g->icsi, id)) * 1.0;
+                    if ((bt[1] == -1 || bt[1] == -2) && i == mx - 3) VX(ucont, id) = 0;
+                }
+                if ((int)(nvert[jj] + 0.5) == 1) {
+                    ucx = (ucat[3 * jj] + u0[0]) * 0.5;
+                    ucy = (ucat[3 * jj + 1] + u0[1]) * 0.5;
+                    ucz = (ucat[3 * jj + 2] + u0[2]) * 0.5;
+                    VY(ucont, id) = (ucx * VX(g->jeta, id) + ucy * VY(g->jeta, id) + ucz * VZ(g->jeta, id)) * 1.0;
+                    if ((bt[3] == -1 || bt[3] == -2) && j == my - 3) VY(ucont, id) = 0;
+                }
+                if ((int)(nvert[kk] + 0.5) == 1) {
+                    ucx = (ucat[3 * kk] + u0[0]) * 0.5;
+                    ucy = (ucat[3 * kk + 1] + u0[1]) * 0.5;
+                    ucz = (ucat[3 * kk + 2] + u0[2]) * 0.5;
+                    VZ(ucont, id) = (ucx * VX(g->kzet, id) + ucy * VY(g->kzet, id) + ucz * VZ(g->kzet, id)) * 1.0;
+                    if ((bt[5] == -1 || bt[5] == -2) && k == mz - 3) VZ(ucont, id) = 0;
+                }
+            }
+    return vo_contra2cart(c, g, nvert, ucont, ucat);
+}

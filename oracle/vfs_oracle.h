@@ -220,6 +220,44 @@ int vo_rk_stage(const vo_cfg *c, const vo_geom *g, const double *nvert,
                 const double *u0, double *ucont, double *ucat, const double *nu_t,
                 const double *dP, double alfa, double *rhs_scratch);
 
+/* A triangulated body (IBMNodes): vertices x|y|z_bp[n_v], elements
+ * nv1..nv3[n_elmt] with unit normals nf pointing into the fluid, vertex
+ * velocities u (3*n_v, Cmpnts; NULL = at rest). */
+typedef struct {
+    int n_v, n_elmt;
+    const double *x_bp, *y_bp, *z_bp;
+    const int *nv1, *nv2, *nv3;
+    const double *nf_x, *nf_y, *nf_z;
+    const double *u;
+} vo_ib_surface;
+
+/* One IB node and its reconstruction geometry (IBMInfo, variables.h:126-172). */
+typedef struct {
+    int i, j, k, cell, imode;
+    int i1, j1, k1, i2, j2, k2, i3, j3, k3;
+    double cr1, cr2, cr3, cs1, cs2, cs3, d_s, d_i;
+    double pmin[3];
+} vo_ib_node;
+
+/* ibm.c:1276-1364 BoundingSphere of element e */
+void vo_ib_bounding_sphere(const vo_ib_surface *s, int e, double q[3], double *rad);
+/* cell of point n (0..8) of interception patch pl (0..5) around (i, j, k) */
+void vo_ib_patch_point(int pl, int n, int i, int j, int k, int *ii, int *jj, int *kk);
+
+/* ibm.c:366-497 ibm_search_advanced, the node list: nearestcell1 and
+ * InterceptionPoint for every IB node, in the list's order (last cell found
+ * first).  -3: a node with no element or no interception; -4: cap too small. */
+int vo_ib_search(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s, vo_ib_node *nodes,
+                 int cap, int *n_nodes);
+
+/* ibm.c:3294-3698 ibm_interpolation_advanced (no averaging, levelset, FSI or
+ * IB wall model): ucat and p of the nodes, the element means of ustar^2 and
+ * p, the fluxes of the IB faces, then Contra2Cart.  ucont, ucat, p, ustar
+ * (a field) in/out; count, shear, pressure: n_elmt each. */
+int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
+                      const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
+                      int *count, double *shear, double *pressure);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,28 @@ def les_opt(les=2, ti=5, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0, k_
     return VoLesOpt(les, i_homo, j_homo, k_homo, ik, max_cs, ti, tistart, rstart)
 
 
+class VoIbSurface(C.Structure):
+    _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
+                ("x_bp", C.POINTER(C.c_double)), ("y_bp", C.POINTER(C.c_double)), ("z_bp", C.POINTER(C.c_double)),
+                ("nv1", C.POINTER(C.c_int)), ("nv2", C.POINTER(C.c_int)), ("nv3", C.POINTER(C.c_int)),
+                ("nf_x", C.POINTER(C.c_double)), ("nf_y", C.POINTER(C.c_double)), ("nf_z", C.POINTER(C.c_double)),
+                ("u", C.POINTER(C.c_double))]
+
+
+class IbBody:
+    """Host arrays of a triangulated body plus the oracle's view of them."""
+
+    def __init__(self, verts, elems, normals, u=None):
+        self.cols = [np.ascontiguousarray(np.asarray(verts, np.float64)[:, q]) for q in range(3)]
+        self.cols += [np.ascontiguousarray(np.asarray(elems, np.int32)[:, q]) for q in range(3)]
+        self.cols += [np.ascontiguousarray(np.asarray(normals, np.float64)[:, q]) for q in range(3)]
+        self.verts, self.elems, self.normals = np.asarray(verts, float), np.asarray(elems), np.asarray(normals, float)
+        self.u = None if u is None else np.ascontiguousarray(np.asarray(u, np.float64).ravel())
+        self.n_v, self.n_elmt = len(verts), len(elems)
+        self.c = VoIbSurface(self.n_v, self.n_elmt, *[ptr(a) for a in self.cols],
+                             None if self.u is None else ptr(self.u))
+
+
 class VoGeom(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in GEOM_FIELDS]
 
@@ -194,6 +216,29 @@ class Oracle:
                    ptr(F), ptr(dA), ptr(box), C.c_int(df), C.c_double(halfwidth), C.c_int(width_mode), ptr(fx),
                    ptr(f))
         return f
+
+    def ib_search(self, body, cap=None):
+        """ibm_search_advanced's node list -> structured array (IB_NODE_DTYPE)."""
+        from vfsgpu import IB_NODE_DTYPE
+        cap = int(cap if cap is not None else max(1, self.P // 8))
+        nodes = np.zeros(cap, IB_NODE_DTYPE)
+        n = C.c_int(0)
+        rc = lib().vo_ib_search(C.byref(self.c), C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
+                                nodes.ctypes.data_as(C.c_void_p), C.c_int(cap), C.byref(n))
+        if rc:
+            raise RuntimeError(f"vo_ib_search rc={rc}")
+        return nodes[:n.value].copy()
+
+    def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar):
+        """ibm_interpolation_advanced; ucont, ucat, p, ustar in/out.
+        Returns the element (count, mean ustar^2, mean p)."""
+        nodes = np.ascontiguousarray(nodes)
+        cnt = np.zeros(body.n_elmt, np.int32)
+        sh, pr = np.zeros(body.n_elmt), np.zeros(body.n_elmt)
+        self._call("vo_ib_interpolate", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
+                   nodes.ctypes.data_as(C.c_void_p), C.c_int(len(nodes)), ptr(ucont), ptr(ucat), ptr(p), ptr(ustar),
+                   ptr(cnt), ptr(sh), ptr(pr))
+        return cnt, sh, pr
 
     def pressure_gradient(self, p, dp):
         self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))

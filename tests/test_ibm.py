@@ -1,0 +1,333 @@
+"""CURVIB velocity reconstruction: the node list of ibm_search_advanced
+(ibm.c:366-497: BoundingSphere 1276-1364, nearestcell1 1366-1511,
+InterceptionPoint 1759-2250 with ICP 1659-1757) and
+ibm_interpolation_advanced (ibm.c:3294-3698).
+
+CPU tests pin the oracle:
+  - the bounding spheres hold their elements, and the sphere of an acute
+    element is its circumsphere;
+  - d_s is the exact distance from each node to the triangulated body, from
+    a closest-point-on-triangle computation of its own (vertex / edge / face
+    regions); pmin lies on the element and cs1..3 rebuild it;
+  - the interception point lies on the node's normal ray at d_i, and
+    cr1..3 rebuild it from the three cell centres;
+  - a tilted plane body with a velocity and pressure linear in the distance
+    from the plane: noslip gives the field at the node, freeslip the
+    tangential part of the interception point and the scaled normal part,
+    ustar = sqrt(|shear| / Re), p the value at the interception point;
+  - a uniform stream past a sphere on a uniform grid (-inv) comes back
+    unchanged at every node, face flux and fluid cell;
+  - the list runs from the last cell in k, j, i order to the first.
+GPU tests compare the HIP path with the oracle.
+"""
+import numpy as np
+import pytest
+
+from cases import oracle_case
+from oracle_py import IbBody, Oracle, lib
+from vfsgpu import workloads as W
+import vfsgpu as V
+import ctypes as C
+
+
+def sphere_body(level=3, u=None):
+    x, e, n = W.icosphere(*W.SPHERE, level=level)
+    return x, e, n, u
+
+
+def closest_on_triangle(p, a, b, c):
+    """Closest point to p on triangle abc by its Voronoi regions (vertex,
+    edge, face), for many triangles at once (arrays of shape (m, 3))."""
+    ab, ac, ap = b - a, c - a, p - a
+    d1, d2 = np.einsum("ij,ij->i", ab, ap), np.einsum("ij,ij->i", ac, ap)
+    bp = p - b
+    d3, d4 = np.einsum("ij,ij->i", ab, bp), np.einsum("ij,ij->i", ac, bp)
+    cp = p - c
+    d5, d6 = np.einsum("ij,ij->i", ab, cp), np.einsum("ij,ij->i", ac, cp)
+    va, vb, vc = d3 * d6 - d5 * d4, d5 * d2 - d1 * d6, d1 * d4 - d3 * d2
+    out = np.empty_like(a)
+    done = np.zeros(len(a), bool)
+
+    def put(mask, val):
+        m = mask & ~done
+        out[m] = val[m]
+        done[:] |= m
+
+    put((d1 <= 0) & (d2 <= 0), a)
+    put((d3 >= 0) & (d4 <= d3), b)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        put((vc <= 0) & (d1 >= 0) & (d3 <= 0), a + (d1 / (d1 - d3))[:, None] * ab)
+        put((d6 >= 0) & (d5 <= d6), c)
+        put((vb <= 0) & (d2 >= 0) & (d6 <= 0), a + (d2 / (d2 - d6))[:, None] * ac)
+        put((va <= 0) & ((d4 - d3) >= 0) & ((d5 - d6) >= 0),
+            b + ((d4 - d3) / ((d4 - d3) + (d5 - d6)))[:, None] * (c - b))
+        den = 1.0 / (va + vb + vc)
+        put(np.ones(len(a), bool), a + (vb * den)[:, None] * ab + (vc * den)[:, None] * ac)
+    return out
+
+
+def test_bounding_spheres_hold_their_elements():
+    x, e, n, _ = sphere_body(2)
+    body = IbBody(x, e, n)
+    q, r = np.zeros(3), C.c_double(0)
+    qa = (C.c_double * 3)()
+    for el in range(0, len(e), 7):
+        lib().vo_ib_bounding_sphere(C.byref(body.c), C.c_int(el), qa, C.byref(r))
+        q = np.array(qa[:])
+        d = np.linalg.norm(x[e[el]] - q, axis=1)
+        assert np.all(d <= r.value * (1 + 1e-12))
+        a, b, c = x[e[el]]
+        angles = [np.dot(b - a, c - a), np.dot(a - b, c - b), np.dot(a - c, b - c)]
+        if min(angles) > 0:   # acute: the circumcentre, equidistant and in the plane
+            assert np.ptp(d) < 1e-12 * r.value
+            assert abs(np.dot(q - a, np.cross(b - a, c - a))) < 1e-12 * np.linalg.norm(np.cross(b - a, c - a))
+
+
+def test_patch_points_follow_the_reference_layout():
+    # ibm.c:1785-1797 (i-1 plane), 2058-2070 (j+1 plane), 2194-2206 (k+1 plane)
+    ii, jj, kk = C.c_int(), C.c_int(), C.c_int()
+    got = []
+    for pl in (0, 3, 5):
+        row = []
+        for q in range(9):
+            lib().vo_ib_patch_point(pl, q, 10, 20, 30, C.byref(ii), C.byref(jj), C.byref(kk))
+            row.append((ii.value, jj.value, kk.value))
+        got.append(row)
+    assert got[0] == [(9, 19, 29), (9, 20, 29), (9, 21, 29), (9, 19, 30), (9, 20, 30), (9, 21, 30),
+                      (9, 19, 31), (9, 20, 31), (9, 21, 31)]
+    assert got[1] == [(9, 21, 29), (10, 21, 29), (11, 21, 29), (9, 21, 30), (10, 21, 30), (11, 21, 30),
+                      (9, 21, 31), (10, 21, 31), (11, 21, 31)]
+    assert got[2] == [(9, 19, 31), (10, 19, 31), (11, 19, 31), (9, 20, 31), (10, 20, 31), (11, 20, 31),
+                      (9, 21, 31), (10, 21, 31), (11, 21, 31)]
+
+
+@pytest.mark.parametrize("N", [(32, 24, 20), (64, 48, 40)])
+def test_search_nearest_point_and_interception(N):
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", N)
+    x, e, n, _ = sphere_body(3)
+    nodes = o.ib_search(IbBody(x, e, n))
+    assert len(nodes) > 20
+    mx, my = cfg["mx"], cfg["my"]
+    lin = (nodes["k"] * my + nodes["j"]) * mx + nodes["i"]
+    assert np.all(np.diff(lin) < 0)                      # AddIBMNode prepends
+    nv = o.nvert
+    assert sorted(lin) == sorted(np.flatnonzero(np.rint(nv) == 1))
+    cent = o.g["cent"].reshape(-1, 3)
+    a, b, c = x[e[:, 0]], x[e[:, 1]], x[e[:, 2]]
+    for nd, li in zip(nodes, lin):
+        p = cent[li]
+        cp = closest_on_triangle(np.broadcast_to(p, a.shape), a, b, c)
+        dist = np.linalg.norm(cp - p, axis=1)
+        assert abs(nd["d_s"] - dist.min()) < 1e-12
+        el = nd["cell"]
+        assert abs(dist[el] - dist.min()) < 1e-12           # the element holds a nearest point
+        assert np.linalg.norm(nd["pmin"] - cp[el]) < 1e-10
+        rebuilt = nd["cs1"] * a[el] + nd["cs2"] * b[el] + nd["cs3"] * c[el]
+        assert np.linalg.norm(rebuilt - nd["pmin"]) < 1e-12
+        # the interception point: on the normal ray at d_i, rebuilt from the cells
+        nr = (p - nd["pmin"]) / np.linalg.norm(p - nd["pmin"])
+        pint = p + nd["d_i"] * nr
+        cells = [((nd[f"k{m}"] * my + nd[f"j{m}"]) * mx + nd[f"i{m}"]) for m in (1, 2, 3)]
+        w = [nd["cr1"], nd["cr2"], nd["cr3"]]
+        assert abs(sum(w) - 1) < 1e-14
+        assert np.linalg.norm(sum(wm * cent[cm] for wm, cm in zip(w, cells)) - pint) < 1e-12
+        assert 0 <= nd["imode"] < 8
+        for m in (1, 2, 3):
+            assert max(abs(nd[f"i{m}"] - nd["i"]), abs(nd[f"j{m}"] - nd["j"]), abs(nd[f"k{m}"] - nd["k"])) == 1
+
+
+# ---------------------------------------------------------------------------
+# a tilted plane body and fields linear in the distance from it
+# ---------------------------------------------------------------------------
+NORMAL = np.array([0.15, 1.0, -0.1]) / np.linalg.norm([0.15, 1.0, -0.1])
+TANGENT = np.cross(NORMAL, [0.0, 0.0, 1.0])
+TANGENT /= np.linalg.norm(TANGENT)
+PLANE_C = 0.62
+U0 = np.array([0.3, -0.2, 0.1])
+GAMMA, BETA = 1.7, -0.4
+P_A, P_B = 0.5, np.array([0.2, -0.7, 0.3])
+
+
+def plane_case(N=(24, 20, 18), inviscid=0):
+    Ni, Nj, Nk = N
+    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=1,
+               bctype=[7, 7, 1, 1, 7, 7], immersed=1, les=0, second_order=0, dpdz_set=1, inviscid=inviscid,
+               ren=400.0, dt=1e-3, st=1.0, mean_pressure_gradient=0.0)
+    coor = V.channel_grid(Ni + 1, Nj + 1, Nk + 1, W.LX, W.LY, W.LZ, 0.0, 0.0)
+    o = Oracle(cfg, coor)
+    cent = o.g["cent"].reshape(cfg["mz"], cfg["my"], cfg["mx"], 3)
+    inner = np.zeros(cent.shape[:3], bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    solid = inner & (cent @ NORMAL < PLANE_C)
+    near = np.zeros_like(solid)
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                near |= np.roll(solid, (dk, dj, di), axis=(0, 1, 2))
+    nv = np.where(solid, 3.0, np.where(inner & near, 1.0, 0.0))
+    o.nvert[:] = nv.ravel()
+    # the plane as 2 x 8 x 8 triangles over a rectangle wider than the box
+    t2 = np.cross(NORMAL, TANGENT)
+    base = NORMAL * PLANE_C + np.array([W.LX / 2, 0, W.LZ / 2]) - NORMAL * np.dot(NORMAL, [W.LX / 2, 0, W.LZ / 2])
+    s = np.linspace(-6, 6, 9)
+    verts = np.array([base + a * TANGENT + b * t2 for b in s for a in s])
+    elems = []
+    for b in range(8):
+        for a in range(8):
+            v0 = b * 9 + a
+            elems += [(v0, v0 + 1, v0 + 10), (v0, v0 + 10, v0 + 9)]
+    elems = np.array(elems, np.int32)
+    nrm = np.cross(verts[elems[:, 1]] - verts[elems[:, 0]], verts[elems[:, 2]] - verts[elems[:, 0]])
+    flip = nrm @ NORMAL < 0
+    elems[flip] = elems[flip][:, [0, 2, 1]]
+    normals = np.tile(NORMAL, (len(elems), 1))
+    uv = np.tile(U0, (len(verts), 1))
+    dist = cent @ NORMAL - PLANE_C
+    ucat = (U0 + (GAMMA * TANGENT + BETA * NORMAL) * dist[..., None]).ravel()
+    p = (P_A + cent @ P_B).ravel()
+    return cfg, coor, o, (verts, elems, normals, uv), ucat, p
+
+
+@pytest.mark.parametrize("inviscid", [0, 1])
+def test_plane_body_linear_fields_closed_form(inviscid):
+    cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case(inviscid=inviscid)
+    body = IbBody(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    cent = o.g["cent"].reshape(-1, 3)
+    ucat0, p0 = ucat.copy(), p.copy()
+    ucont = o.vec()
+    ustar = o.sca()
+    cnt, sh, pr = o.ib_interpolate(body, nodes, ucont, ucat, p, ustar)
+    assert cnt.sum() == len(nodes)
+    checked = nonzero = 0
+    for nd in nodes:
+        if not (2 <= nd["i"] <= mx - 3 and 2 <= nd["k"] <= mz - 3 and nd["j"] >= 2):
+            continue   # patches touching ghost cells
+        li = (nd["k"] * my + nd["j"]) * mx + nd["i"]
+        x = cent[li]
+        sb = x @ NORMAL - PLANE_C
+        assert abs(nd["d_s"] - sb) < 1e-12
+        sc = sb + nd["d_i"]
+        cells = [((nd[f"k{m}"] * my + nd[f"j{m}"]) * mx + nd[f"i{m}"]) for m in (1, 2, 3)]
+        got = ucat[3 * li:3 * li + 3]
+        assert abs(p[li] - (P_A + (x + nd["d_i"] * NORMAL) @ P_B)) < 1e-12
+        if not inviscid and sum(o.nvert[cm] for cm in cells) > 0.1:
+            assert np.all(got == 0) and ustar[li] == 0
+        elif inviscid:
+            want = U0 + GAMMA * sc * TANGENT + BETA * sb * NORMAL
+            assert np.abs(got - want).max() < 1e-12
+            nonzero += 1
+        else:
+            want = U0 + (GAMMA * TANGENT + BETA * NORMAL) * sb
+            assert np.abs(got - want).max() < 1e-12
+            assert abs(ustar[li] - np.sqrt(abs(GAMMA) / cfg["ren"])) < 1e-12
+            nonzero += 1
+        checked += 1
+    assert checked > 50 and nonzero > 10
+    # the element means are means over the element's nodes
+    lin = (nodes["k"] * my + nodes["j"]) * mx + nodes["i"]
+    np.testing.assert_allclose(pr * np.maximum(cnt, 1), np.bincount(nodes["cell"], p[lin], len(elems)), atol=1e-12)
+    np.testing.assert_allclose(sh * np.maximum(cnt, 1), np.bincount(nodes["cell"], ustar[lin] ** 2, len(elems)),
+                               atol=1e-12)
+
+
+def test_uniform_stream_survives_the_reconstruction():
+    # -inv (no fluid-support guard): every node gets Ua + 0, then the faces
+    # and Contra2Cart carry the stream unchanged (a uniform grid, where the
+    # cell metric is the mean of its face metrics)
+    cfg, spec = W.make_config("C2", (32, 24, 20))
+    cfg = dict(cfg, inviscid=1)
+    coor = V.channel_grid(33, 25, 21, W.LX, W.LY, W.LZ, 0.0, 0.0)
+    o = Oracle(cfg, coor)
+    o.nvert_sphere(*W.SPHERE)
+    x, e, n, _ = sphere_body(3)
+    body = IbBody(x, e, n, np.tile(U0, (len(x), 1)))
+    nodes = o.ib_search(body)
+    ucat = np.tile(U0, o.P)
+    ucont = o.vec()
+    for f, nm in enumerate(("icsi", "jeta", "kzet")):
+        ucont[f::3] = o.g[nm].reshape(-1, 3) @ U0
+    ref = ucont.copy()
+    p, ustar = o.sca(), o.sca()
+    o.ib_interpolate(body, nodes, ucont, ucat, p, ustar)
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    inner = np.zeros((mz, my, mx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    fluid = (inner & (o.nvert.reshape(mz, my, mx) < 0.1)).ravel()
+    assert np.abs(ucat.reshape(-1, 3)[fluid] - U0).max() < 1e-13
+    lin = (nodes["k"] * my + nodes["j"]) * mx + nodes["i"]
+    assert np.abs(ucat.reshape(-1, 3)[lin] - U0).max() < 1e-15
+    assert np.abs(ucont.reshape(-1, 3)[inner.ravel()] - ref.reshape(-1, 3)[inner.ravel()]).max() < 1e-15
+
+
+# ---------------------------------------------------------------------------
+# GPU parity
+# ---------------------------------------------------------------------------
+def gpu_surface(x, e, n, u=None):
+    return V.IbSurface(x, e, n, u)
+
+
+def nodes_equal(a, b):
+    for f in a.dtype.names:
+        assert np.array_equal(a[f], b[f]), f
+
+
+def run_pair(cfg, coor, o, body_arrays, ucat, ucont, p, nvert_gpu=None):
+    x, e, n, u = body_arrays
+    body = IbBody(x, e, n, u)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
+    surf = gpu_surface(x, e, n, u)
+    nodes_o = o.ib_search(body)
+    dn, cnt_n = ctx.ib_search(surf)
+    assert cnt_n == len(nodes_o)
+    nodes_g = dn.host()[:cnt_n * V.IB_NODE_DTYPE.itemsize].view(V.IB_NODE_DTYPE)
+    nodes_equal(nodes_g, nodes_o)
+    ustar = np.zeros(o.P)
+    d_ucat, d_ucont, d_p, d_ustar = (V.DeviceArray.from_host(a) for a in (ucat, ucont, p, ustar))
+    st = ctx.ib_interpolate(surf, dn, cnt_n, d_ucont, d_ucat, d_p, d_ustar)
+    ucat_o, ucont_o, p_o = ucat.copy(), ucont.copy(), p.copy()
+    cnt, sh, pr = o.ib_interpolate(body, nodes_o, ucont_o, ucat_o, p_o, ustar)
+    assert np.array_equal(d_ucat.host(), ucat_o)
+    assert np.array_equal(d_ucont.host(), ucont_o)
+    assert np.array_equal(d_p.host(), p_o)
+    assert np.array_equal(d_ustar.host(), ustar)
+    assert np.array_equal(st[0].host(), cnt)
+    assert np.array_equal(st[1].host(), sh)
+    assert np.array_equal(st[2].host(), pr)
+    return len(nodes_o)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,inviscid", [("C2", 0), ("C2", 1), ("C2w", 0)])
+def test_gpu_ib_reconstruction_sphere(name, inviscid):
+    cfg, spec, coor, o0, ucat, ucont = oracle_case(name)
+    cfg = dict(cfg, inviscid=inviscid)
+    o = Oracle(cfg, coor)
+    o.nvert[:] = o0.nvert
+    x, e, n, _ = sphere_body(3)
+    u = 0.05 * np.sin(np.arange(3 * len(x))).reshape(-1, 3)
+    p = np.sin(np.arange(o.P) * 0.37)
+    n_nodes = run_pair(cfg, coor, o, (x, e, n, u), ucat, ucont, p)
+    assert n_nodes > 20
+
+
+@pytest.mark.gpu
+def test_gpu_ib_reconstruction_full_size():
+    # 256 x 128 x 128 with the sphere as 5120 triangles
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", W.BASE["C2"]["N"])
+    x, e, n, _ = sphere_body(4)
+    p = np.cos(np.arange(o.P) * 0.013)
+    n_nodes = run_pair(cfg, coor, o, (x, e, n, None), ucat, ucont, p)
+    print("IB nodes", n_nodes, flush=True)
+    assert n_nodes > 1000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("inviscid", [0, 1])
+def test_gpu_ib_reconstruction_plane(inviscid):
+    cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case(inviscid=inviscid)
+    ucont = np.sin(np.arange(3 * o.P) * 0.11)
+    assert run_pair(cfg, coor, o, (verts, elems, normals, uv), ucat, ucont, p) > 100

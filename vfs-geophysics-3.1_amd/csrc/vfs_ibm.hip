@@ -1,0 +1,589 @@
+// CURVIB velocity reconstruction at immersed-boundary nodes.
+//
+// vfs_ib_search builds the node list of ibm_search_advanced (ibm.c:366-497)
+// for a given mask: for every IB node the nearest point of a triangulated
+// body (nearestcell1, ibm.c:1366-1511) and where the ray from that point
+// through the node meets the next patch of cell centres (InterceptionPoint,
+// ibm.c:1759-2250).  vfs_ib_interpolate then runs the per-step part,
+// ibm_interpolation_advanced (ibm.c:3294-3698): the velocity and pressure at
+// each node from the values interpolated at the interception point, the
+// fluxes of the IB faces, and Contra2Cart.
+//
+// One thread per node runs the reference's element scan in element order, so
+// the bounding-sphere cut and the strict-minimum updates see the same dmin
+// sequence as the reference and the nearest element is the same one.  The
+// per-step pass reads the old fields and writes the new node values through
+// a compact staging array, as the reference reads lUcat / lP while it writes
+// Ucat / P.
+#include <vector>
+
+#include "vfs_internal.h"
+
+namespace {
+
+struct IbSurf {
+    int n_v, n_elmt;
+    const double *x, *y, *z;
+    const int *nv1, *nv2, *nv3;
+    const double *nfx, *nfy, *nfz;
+    const double *u;
+};
+
+IbSurf to_surf(const vfs_ib_surface *s)
+{
+    return IbSurf{s->n_v, s->n_elmt, s->x_bp, s->y_bp, s->z_bp, s->nv1, s->nv2, s->nv3, s->nf_x, s->nf_y, s->nf_z, s->u};
+}
+
+// ibm.c:28 Dist
+__device__ __forceinline__ double dist3(const double a[3], const double b[3])
+{
+    return sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+}
+
+__device__ __forceinline__ void vertex(const IbSurf &s, int v, double p[3])
+{
+    p[0] = s.x[v];
+    p[1] = s.y[v];
+    p[2] = s.z[v];
+}
+
+// ibm.c:30-33 Cross
+__device__ __forceinline__ void cross3(double r[3], const double a[3], const double b[3])
+{
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// ibm.c:1276-1364 BoundingSphere: the circumcentre of an acute element, the
+// middle of the longest edge otherwise
+__global__ void k_ib_bound(IbSurf s, double *__restrict__ qv, double *__restrict__ rad)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= s.n_elmt) return;
+    double p1[3], p2[3], p3[3], pa[3], pc[3], pf[3], pu[3], pv[3], pt[3], pd[3];
+    const double p0[3] = {0, 0, 0};
+    vertex(s, s.nv1[e], p1);
+    vertex(s, s.nv2[e], p2);
+    vertex(s, s.nv3[e], p3);
+    const double l12 = dist3(p1, p2), l23 = dist3(p2, p3), l31 = dist3(p3, p1);
+    const double *a, *b, *c;
+    if (l12 > l23) {
+        if (l12 > l31) a = p1, b = p2, c = p3;
+        else a = p3, b = p1, c = p2;
+    } else {
+        if (l31 < l23) a = p2, b = p3, c = p1;
+        else a = p3, b = p1, c = p2;
+    }
+    for (int q = 0; q < 3; q++) {
+        pa[q] = a[q];
+        pc[q] = c[q];
+        pf[q] = 0.5 * (a[q] + b[q]);
+    }
+    for (int q = 0; q < 3; q++) pu[q] = pa[q] - pf[q], pv[q] = pc[q] - pf[q];
+    cross3(pt, pu, pv);
+    cross3(pd, pt, pu);
+    double gama = -(dist3(pu, p0) * dist3(pu, p0) - dist3(pv, p0) * dist3(pv, p0));
+    for (int q = 0; q < 3; q++) pt[q] = pv[q] - pu[q];
+    double lamda = 2 * (pd[0] * pt[0] + pd[1] * pt[1] + pd[2] * pt[2]);
+    gama /= lamda;
+    lamda = gama < 0 ? 0 : gama;
+    double qq[3];
+    for (int q = 0; q < 3; q++) qq[q] = pf[q] + lamda * pd[q];
+    qv[3 * e] = qq[0];
+    qv[3 * e + 1] = qq[1];
+    qv[3 * e + 2] = qq[2];
+    rad[e] = dist3(qq, pa);
+}
+
+// compgeom.c:56-110 ISSameSide2D as it runs: the t1 / t2 of its last block
+// decide, a point within |t1| / |p1p2| < 1e-10 of the line counts as inside
+__device__ __forceinline__ int same_side(const double p[2], const double p1[2], const double p2[2],
+                                         const double p3[2])
+{
+    const double t1 = (p[0] - p1[0]) * (p2[1] - p1[1]) - (p[1] - p1[1]) * (p2[0] - p1[0]);
+    const double t2 = (p3[0] - p1[0]) * (p2[1] - p1[1]) - (p3[1] - p1[1]) * (p2[0] - p1[0]);
+    const double lt = sqrt((p1[0] - p2[0]) * (p1[0] - p2[0]) + (p1[1] - p2[1]) * (p1[1] - p2[1]));
+    if (fabs(t1 / lt) < 1.e-10) return 1;
+    if (t1 > 0) return t2 > 0 ? 1 : -1;
+    return t2 < 0 ? 1 : -1;
+}
+
+// compgeom.c:112-158 ISPointInTriangle: 1 inside or on an edge, -1 outside,
+// on the coordinate plane the normal picks
+__device__ int point_in_triangle(const double p[3], const double p1[3], const double p2[3], const double p3[3],
+                                 double nx, double ny, double nz)
+{
+    int u, v;
+    if (fabs(nz) >= fabs(nx) && fabs(nz) >= fabs(ny)) u = 0, v = 1;
+    else if (fabs(nx) >= fabs(ny) && fabs(nx) >= fabs(nz)) u = 2, v = 1;
+    else u = 0, v = 2;
+    const double a[2] = {p[u], p[v]}, b1[2] = {p1[u], p1[v]}, b2[2] = {p2[u], p2[v]}, b3[2] = {p3[u], p3[v]};
+    int ls = same_side(a, b1, b2, b3);
+    if (ls < 0) return ls;
+    ls = same_side(a, b2, b3, b1);
+    if (ls < 0) return ls;
+    return same_side(a, b3, b1, b2);
+}
+
+// compgeom.c:160-186 Dis_P_Line
+__device__ void dis_p_line(const double p[3], const double p1[3], const double p2[3], double po[3], double &d)
+{
+    const double dx21 = p2[0] - p1[0], dy21 = p2[1] - p1[1], dz21 = p2[2] - p1[2];
+    const double dx31 = p[0] - p1[0], dy31 = p[1] - p1[1], dz31 = p[2] - p1[2];
+    const double t = (dx31 * dx21 + dy31 * dy21 + dz31 * dz21) / (dx21 * dx21 + dy21 * dy21 + dz21 * dz21);
+    if (t < 0) {
+        po[0] = p1[0], po[1] = p1[1], po[2] = p1[2];
+        d = sqrt(dx31 * dx31 + dy31 * dy31 + dz31 * dz31);
+        return;
+    }
+    if (t > 1) po[0] = p2[0], po[1] = p2[1], po[2] = p2[2];
+    else po[0] = p1[0] + t * dx21, po[1] = p1[1] + t * dy21, po[2] = p1[2] + t * dz21;
+    d = sqrt((p[0] - po[0]) * (p[0] - po[0]) + (p[1] - po[1]) * (p[1] - po[1]) + (p[2] - po[2]) * (p[2] - po[2]));
+}
+
+// compgeom.c:188-230 triangle_intp2 / triangle_intp: barycentric weights on
+// the plane the normal picks (x-dominant (y, z), y-dominant (x, z),
+// z-dominant (y, x)); a NaN normal leaves w as it was
+__device__ void triangle_intp(const double p[3], const double p1[3], const double p2[3], const double p3[3],
+                              double nx, double ny, double nz, double w[3])
+{
+    int u, v;
+    if (fabs(nx) >= fabs(ny) && fabs(nx) >= fabs(nz)) u = 1, v = 2;
+    else if (fabs(ny) >= fabs(nx) && fabs(ny) >= fabs(nz)) u = 0, v = 2;
+    else if (fabs(nz) >= fabs(ny) && fabs(nz) >= fabs(nx)) u = 1, v = 0;
+    else return;
+    const double x13 = p1[u] - p3[u], y13 = p1[v] - p3[v];
+    const double x23 = p2[u] - p3[u], y23 = p2[v] - p3[v];
+    const double xp3 = p[u] - p3[u], yp3 = p[v] - p3[v];
+    const double a = x13 * y23 - x23 * y13;
+    w[0] = (y23 * xp3 - x23 * yp3) / a;
+    w[1] = (-y13 * xp3 + x13 * yp3) / a;
+    w[2] = 1. - w[0] - w[1];
+}
+
+// ibm.c:1670-1677: the eight triangles of a 3x3 patch; the interpolation
+// cells are the vertices of the triangle hit (the switch blocks of
+// InterceptionPoint list the same triples)
+__constant__ int c_tri[8][3] = {{0, 1, 4}, {1, 2, 4}, {2, 4, 5}, {4, 5, 8}, {4, 7, 8}, {4, 6, 7}, {3, 4, 6}, {3, 4, 0}};
+
+// cell of point n of patch pl (i-1, i+1, j-1, j+1, k-1, k+1 planes in turn)
+__device__ __forceinline__ void patch_point(int pl, int n, int i, int j, int k, int &ii, int &jj, int &kk)
+{
+    const int a = n % 3 - 1, b = n / 3 - 1, s = (pl & 1) ? 1 : -1;
+    if (pl < 2) ii = i + s, jj = j + a, kk = k + b;
+    else if (pl < 4) ii = i + a, jj = j + s, kk = k + b;
+    else ii = i + a, jj = j + b, kk = k + s;
+}
+
+// the mask of the body's nodes: nvert == 1 off the wall-function planes,
+// which ibm.c:518-533 sets only after the list is made
+__device__ __forceinline__ int is_node(const Dims &d, const double *nvert, long long t)
+{
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    if (i < 1 || i > d.mx - 2 || j < 1 || j > d.my - 2 || k < 1 || k > d.mz - 2) return 0;
+    if ((int)(nvert[t] + 0.5) != 1) return 0;
+    const int *bt = d.bc;
+    if (((bt[0] == -1 || bt[0] == -2) && i == 1) || ((bt[1] == -1 || bt[1] == -2) && i == d.mx - 2)) return 0;
+    if (((bt[2] == -1 || bt[2] == -2) && j == 1) || ((bt[3] == -1 || bt[3] == -2) && j == d.my - 2)) return 0;
+    if (((bt[4] == -1 || bt[4] == -2) && k == 1) || ((bt[5] == -1 || bt[5] == -2) && k == d.mz - 2)) return 0;
+    return 1;
+}
+
+constexpr int LIST_BLOCK = 256;
+
+__global__ void __launch_bounds__(LIST_BLOCK) k_ib_count(Dims d, const double *__restrict__ nvert, int *__restrict__ cnt)
+{
+    const long long t = (long long)blockIdx.x * LIST_BLOCK + threadIdx.x;
+    const int f = t < d.P ? is_node(d, nvert, t) : 0;
+    const int c = __syncthreads_count(f);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+// node at linear position r (k, j, i order) goes to list entry n-1-r
+__global__ void __launch_bounds__(LIST_BLOCK) k_ib_place(Dims d, const double *__restrict__ nvert,
+                                                        const int *__restrict__ off, int n,
+                                                        vfs_ib_node *__restrict__ nodes)
+{
+    __shared__ int s[LIST_BLOCK];
+    const long long t = (long long)blockIdx.x * LIST_BLOCK + threadIdx.x;
+    const int f = t < d.P ? is_node(d, nvert, t) : 0;
+    s[threadIdx.x] = f;
+    __syncthreads();
+    for (int w = 1; w < LIST_BLOCK; w <<= 1) {   // inclusive scan
+        const int v = threadIdx.x >= (unsigned)w ? s[threadIdx.x - w] : 0;
+        __syncthreads();
+        s[threadIdx.x] += v;
+        __syncthreads();
+    }
+    if (!f) return;
+    const int r = off[blockIdx.x] + s[threadIdx.x] - 1;
+    vfs_ib_node nd = {};
+    nd.i = (int)(t % d.mx);
+    nd.j = (int)((t / d.mx) % d.my);
+    nd.k = (int)(t / ((long long)d.mx * d.my));
+    nodes[n - 1 - r] = nd;
+}
+
+// nearestcell1 (ibm.c:1366-1511), InterceptionPoint (ibm.c:1759-2250) with
+// ICP (ibm.c:1659-1757), and the boundary fallback (ibm.c:413-493)
+__global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, const double *__restrict__ qv,
+                                                  const double *__restrict__ rad, int n,
+                                                  vfs_ib_node *__restrict__ nodes, int *__restrict__ fail)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    vfs_ib_node nd = nodes[l];
+    const int i = nd.i, j = nd.j, k = nd.k;
+    const long long id = IX(d, k, j, i);
+    const double p[3] = {g.cent[3 * id], g.cent[3 * id + 1], g.cent[3 * id + 2]};
+
+    double dmin = 1.e20, pmin[3] = {0, 0, 0};
+    int cell_min = -100;
+    for (int e = 0; e < s.n_elmt; e++) {
+        const double q[3] = {qv[3 * e], qv[3 * e + 1], qv[3 * e + 2]};
+        if (!(dist3(p, q) - rad[e] < dmin)) continue;
+        const int n1 = s.nv1[e];
+        const double nfx = s.nfx[e], nfy = s.nfy[e], nfz = s.nfz[e];
+        double p1[3], p2[3], p3[3], po[3], dd;
+        vertex(s, n1, p1);
+        vertex(s, s.nv2[e], p2);
+        vertex(s, s.nv3[e], p3);
+        double tf = ((p[0] - p1[0]) * nfx + (p[1] - p1[1]) * nfy + (p[2] - p1[2]) * nfz);
+        if (fabs(tf) < 1.e-10) tf = 1.e-10;
+        if (!(tf >= 0)) continue;
+        const double pj[3] = {p[0] - tf * nfx, p[1] - tf * nfy, p[2] - tf * nfz};
+        if (point_in_triangle(pj, p1, p2, p3, nfx, nfy, nfz) == 1) {
+            if (tf < dmin) dmin = tf, pmin[0] = pj[0], pmin[1] = pj[1], pmin[2] = pj[2], cell_min = e;
+        } else {
+            dis_p_line(p, p1, p2, po, dd);
+            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+            dis_p_line(p, p2, p3, po, dd);
+            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+            dis_p_line(p, p3, p1, po, dd);
+            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+        }
+    }
+    if (cell_min == -100) {
+        atomicAdd(fail, 1);
+        nodes[l] = nd;
+        return;
+    }
+    nd.cell = cell_min;
+    nd.pmin[0] = pmin[0], nd.pmin[1] = pmin[1], nd.pmin[2] = pmin[2];
+    nd.d_s = dmin;
+    {
+        double p1[3], p2[3], p3[3], w[3] = {nd.cs1, nd.cs2, nd.cs3};
+        vertex(s, s.nv1[cell_min], p1);
+        vertex(s, s.nv2[cell_min], p2);
+        vertex(s, s.nv3[cell_min], p3);
+        triangle_intp(pmin, p1, p2, p3, s.nfx[cell_min], s.nfy[cell_min], s.nfz[cell_min], w);
+        nd.cs1 = w[0], nd.cs2 = w[1], nd.cs3 = w[2];
+    }
+
+    double nr[3] = {p[0] - pmin[0], p[1] - pmin[1], p[2] - pmin[2]};
+    const double dr = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+    nr[0] /= dr, nr[1] /= dr, nr[2] /= dr;
+    nd.imode = -100;
+    for (int pl = 0; pl < 6 && nd.imode < 0; pl++) {
+        double pc[9][3];
+        for (int q = 0; q < 9; q++) {
+            int ii, jj, kk;
+            patch_point(pl, q, i, j, k, ii, jj, kk);
+            const long long c = IX(d, kk, jj, ii);
+            pc[q][0] = g.cent[3 * c], pc[q][1] = g.cent[3 * c + 1], pc[q][2] = g.cent[3 * c + 2];
+        }
+        for (int t = 0; t < 8; t++) {
+            const double *p1 = pc[c_tri[t][0]], *p2 = pc[c_tri[t][1]], *p3 = pc[c_tri[t][2]];
+            const double dx1 = p[0] - p1[0], dy1 = p[1] - p1[1], dz1 = p[2] - p1[2];
+            const double dx2 = p2[0] - p1[0], dy2 = p2[1] - p1[1], dz2 = p2[2] - p1[2];
+            const double dx3 = p3[0] - p1[0], dy3 = p3[1] - p1[1], dz3 = p3[2] - p1[2];
+            double dd = (nr[0] * (dy2 * dz3 - dz2 * dy3) - nr[1] * (dx2 * dz3 - dz2 * dx3) +
+                         nr[2] * (dx2 * dy3 - dy2 * dx3));
+            if (!(fabs(dd) > 1.e-10)) continue;
+            dd = -(dx1 * (dy2 * dz3 - dz2 * dy3) - dy1 * (dx2 * dz3 - dz2 * dx3) + dz1 * (dx2 * dy3 - dy2 * dx3)) / dd;
+            if (!(dd > 0)) continue;
+            const double pint[3] = {p[0] + dd * nr[0], p[1] + dd * nr[1], p[2] + dd * nr[2]};
+            const double rx1 = p2[0] - p1[0], ry1 = p2[1] - p1[1], rz1 = p2[2] - p1[2];
+            const double rx2 = p3[0] - p1[0], ry2 = p3[1] - p1[1], rz2 = p3[2] - p1[2];
+            const double nxt = ry1 * rz2 - rz1 * ry2, nyt = -rx1 * rz2 + rz1 * rx2, nzt = rx1 * ry2 - ry1 * rx2;
+            if (point_in_triangle(pint, p1, p2, p3, nxt, nyt, nzt) < 0) continue;
+            double w[3] = {nd.cr1, nd.cr2, nd.cr3};
+            triangle_intp(pint, p1, p2, p3, nxt, nyt, nzt, w);
+            nd.cr1 = w[0], nd.cr2 = w[1], nd.cr3 = w[2];
+            nd.d_i = sqrt((pint[0] - p[0]) * (pint[0] - p[0]) + (pint[1] - p[1]) * (pint[1] - p[1]) +
+                          (pint[2] - p[2]) * (pint[2] - p[2]));
+            nd.imode = t;
+            int ii, jj, kk;
+            patch_point(pl, c_tri[t][0], i, j, k, ii, jj, kk);
+            nd.i1 = ii, nd.j1 = jj, nd.k1 = kk;
+            patch_point(pl, c_tri[t][1], i, j, k, ii, jj, kk);
+            nd.i2 = ii, nd.j2 = jj, nd.k2 = kk;
+            patch_point(pl, c_tri[t][2], i, j, k, ii, jj, kk);
+            nd.i3 = ii, nd.j3 = jj, nd.k3 = kk;
+            break;
+        }
+    }
+    if (nd.imode < 0) {
+        int ti, tj, tk;
+        if (i == 1 || i == d.mx - 2 || j == 1 || j == d.my - 2) ti = i, tj = j, tk = s.nfz[cell_min] > 0 ? k + 1 : k - 1;
+        else if (k == 1 || k == d.mz - 2) ti = i, tj = j + 1, tk = k;
+        else {
+            atomicAdd(fail, 1);
+            nodes[l] = nd;
+            return;
+        }
+        const long long c = IX(d, tk, tj, ti);
+        const double pt[3] = {g.cent[3 * c], g.cent[3 * c + 1], g.cent[3 * c + 2]};
+        nd.d_i = sqrt((p[0] - pt[0]) * (p[0] - pt[0]) + (p[1] - pt[1]) * (p[1] - pt[1]) + (p[2] - pt[2]) * (p[2] - pt[2]));
+        nd.cr1 = 1., nd.cr2 = 0., nd.cr3 = 0.;
+        nd.i1 = nd.i2 = nd.i3 = ti;
+        nd.j1 = nd.j2 = nd.j3 = tj;
+        nd.k1 = nd.k2 = nd.k3 = tk;
+    }
+    nodes[l] = nd;
+}
+
+// ibm.c:3390-3530: the node values from the old fields into the staging
+// array (u x|y|z, p in blocks of n); ustar of the node in the field
+__global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restrict__ nvert, IbSurf s,
+                                                  const vfs_ib_node *__restrict__ nodes, int n,
+                                                  const double *__restrict__ ucat, const double *__restrict__ pf,
+                                                  double *__restrict__ ustar, double *__restrict__ stage)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const vfs_ib_node nd = nodes[l];
+    const int ni = nd.cell;
+    const long long id = IX(d, nd.k, nd.j, nd.i);
+    const long long q1 = IX(d, nd.k1, nd.j1, nd.i1), q2 = IX(d, nd.k2, nd.j2, nd.i2), q3 = IX(d, nd.k3, nd.j3, nd.i3);
+    const double sb = nd.d_s, sc = sb + nd.d_i;
+    const double sk1 = nd.cr1, sk2 = nd.cr2, sk3 = nd.cr3;
+    const double nfx = s.nfx[ni], nfy = s.nfy[ni], nfz = s.nfz[ni];
+    double Ua[3] = {0, 0, 0}, Uc[3], Ub[3];
+    if (ni >= 0 && s.u) {
+        const double *u1 = &s.u[3 * (long long)s.nv1[ni]], *u2 = &s.u[3 * (long long)s.nv2[ni]],
+                     *u3 = &s.u[3 * (long long)s.nv3[ni]];
+        for (int q = 0; q < 3; q++) Ua[q] = u1[q] * nd.cs1 + u2[q] * nd.cs2 + u3[q] * nd.cs3;
+    }
+    for (int q = 0; q < 3; q++) Uc[q] = (ucat[3 * q1 + q] * sk1 + ucat[3 * q2 + q] * sk2 + ucat[3 * q3 + q] * sk3);
+    if ((!d.inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) || nvert[id] > 2.9) {
+        Ub[0] = Ub[1] = Ub[2] = 0;
+        ustar[id] = 0;
+    } else {
+        const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
+        const double un = u_c * nfx + v_c * nfy + w_c * nfz;
+        const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+        if (d.inviscid) {   // freeslip, wallfunction.c:152-166
+            Ub[0] = ut + sb / sc * un * nfx;
+            Ub[1] = vt + sb / sc * un * nfy;
+            Ub[2] = wt + sb / sc * un * nfz;
+        } else {            // noslip, wallfunction.c:134-150
+            const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+            ustar[id] = sqrt(ut_mag / sc / d.ren);
+            Ub[0] = sb / sc * u_c;
+            Ub[1] = sb / sc * v_c;
+            Ub[2] = sb / sc * w_c;
+        }
+        Ub[0] += Ua[0], Ub[1] += Ua[1], Ub[2] += Ua[2];
+    }
+    stage[l] = Ub[0];
+    stage[n + l] = Ub[1];
+    stage[2 * (long long)n + l] = Ub[2];
+    stage[3 * (long long)n + l] = (pf[q1] * sk1 + pf[q2] * sk2 + pf[q3] * sk3);
+}
+
+__global__ void __launch_bounds__(256) k_ib_scatter(Dims d, const vfs_ib_node *__restrict__ nodes, int n,
+                                                    const double *__restrict__ stage, double *__restrict__ ucat,
+                                                    double *__restrict__ pf)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const vfs_ib_node nd = nodes[l];
+    const long long id = IX(d, nd.k, nd.j, nd.i);
+    ucat[3 * id] = stage[l];
+    ucat[3 * id + 1] = stage[n + l];
+    ucat[3 * id + 2] = stage[2 * (long long)n + l];
+    pf[id] = stage[3 * (long long)n + l];
+}
+
+// ibm.c:3556-3576: per element, the count and the means of ustar^2 and p
+// over its nodes, added in list order (one thread an element, the nodes
+// staged through LDS)
+constexpr int STAT_BLOCK = 256;
+__global__ void __launch_bounds__(STAT_BLOCK) k_ib_stats(Dims d, int n_elmt, const vfs_ib_node *__restrict__ nodes,
+                                                         int n, const double *__restrict__ ustar,
+                                                         const double *__restrict__ stage, int *__restrict__ count,
+                                                         double *__restrict__ shear, double *__restrict__ pres)
+{
+    __shared__ int sc[STAT_BLOCK];
+    __shared__ double su[STAT_BLOCK], sp[STAT_BLOCK];
+    const int e = blockIdx.x * STAT_BLOCK + threadIdx.x;
+    int cnt = 0;
+    double sh = 0, pr = 0;
+    for (int l0 = 0; l0 < n; l0 += STAT_BLOCK) {
+        const int l = l0 + threadIdx.x;
+        if (l < n) {
+            const vfs_ib_node &nd = nodes[l];
+            sc[threadIdx.x] = nd.cell;
+            su[threadIdx.x] = ustar[IX(d, nd.k, nd.j, nd.i)];
+            sp[threadIdx.x] = stage[3 * (long long)n + l];
+        }
+        __syncthreads();
+        const int m = min(STAT_BLOCK, n - l0);
+        for (int q = 0; q < m; q++)
+            if (sc[q] == e) {
+                cnt++;
+                sh += su[q] * su[q];
+                pr += sp[q];
+            }
+        __syncthreads();
+    }
+    if (e >= n_elmt) return;
+    if (cnt) {
+        sh /= (double)cnt;
+        pr /= (double)cnt;
+    }
+    if (count) count[e] = cnt;
+    if (shear) shear[e] = sh;
+    if (pres) pres[e] = pr;
+}
+
+// ibm.c:3619-3679: the fluxes of the faces of every nvert == 1 cell from the
+// averaged new velocity (immersed != 3: factor 1)
+__global__ void __launch_bounds__(256) k_ib_faces(Dims d, Geom g, const double *__restrict__ nvert,
+                                                  const double *__restrict__ ucat, double *__restrict__ ucont)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    if (i < 1 || i > d.mx - 2 || j < 1 || j > d.my - 2 || k < 1 || k > d.mz - 2) return;
+    const long long ii = t + 1, jj = t + d.mx, kk = t + (long long)d.mx * d.my;
+    const int *bt = d.bc;
+    const double u0 = ucat[3 * t], v0 = ucat[3 * t + 1], w0 = ucat[3 * t + 2];
+    const bool self = (int)(nvert[t] + 0.5) == 1, ni = (int)(nvert[ii] + 0.5) == 1,
+               nj = (int)(nvert[jj] + 0.5) == 1, nk = (int)(nvert[kk] + 0.5) == 1;
+    double ux = ucont[3 * t], uy = ucont[3 * t + 1], uz = ucont[3 * t + 2];
+    // each flux is the same average whichever cell asks for it; the wall
+    // rule of the asking cell applies
+    double fx = 0, fy = 0, fz = 0;
+    if (self || ni) {
+        const double ucx = (u0 + ucat[3 * ii]) * 0.5, ucy = (v0 + ucat[3 * ii + 1]) * 0.5,
+                     ucz = (w0 + ucat[3 * ii + 2]) * 0.5;
+        fx = (ucx * g.icsi[3 * t] + ucy * g.icsi[3 * t + 1] + ucz * g.icsi[3 * t + 2]) * 1.0;
+    }
+    if (self || nj) {
+        const double ucx = (ucat[3 * jj] + u0) * 0.5, ucy = (ucat[3 * jj + 1] + v0) * 0.5,
+                     ucz = (ucat[3 * jj + 2] + w0) * 0.5;
+        fy = (ucx * g.jeta[3 * t] + ucy * g.jeta[3 * t + 1] + ucz * g.jeta[3 * t + 2]) * 1.0;
+    }
+    if (self || nk) {
+        const double ucx = (ucat[3 * kk] + u0) * 0.5, ucy = (ucat[3 * kk + 1] + v0) * 0.5,
+                     ucz = (ucat[3 * kk + 2] + w0) * 0.5;
+        fz = (ucx * g.kzet[3 * t] + ucy * g.kzet[3 * t + 1] + ucz * g.kzet[3 * t + 2]) * 1.0;
+    }
+    if (self) {
+        ux = ((bt[0] == -1 || bt[0] == -2) && i == 1) ? 0 : fx;
+        uy = ((bt[2] == -1 || bt[2] == -2) && j == 1) ? 0 : fy;
+        uz = ((bt[4] == -1 || bt[4] == -2) && k == 1) ? 0 : fz;
+    }
+    if (ni) ux = ((bt[1] == -1 || bt[1] == -2) && i == d.mx - 3) ? 0 : fx;
+    if (nj) uy = ((bt[3] == -1 || bt[3] == -2) && j == d.my - 3) ? 0 : fy;
+    if (nk) uz = ((bt[5] == -1 || bt[5] == -2) && k == d.mz - 3) ? 0 : fz;
+    ucont[3 * t] = ux;
+    ucont[3 * t + 1] = uy;
+    ucont[3 * t + 2] = uz;
+}
+
+int check_surface(const vfs_ib_surface *s)
+{
+    VFS_REQUIRE(s && s->n_v > 0 && s->n_elmt > 0, "the body needs vertices and elements");
+    VFS_REQUIRE(s->x_bp && s->y_bp && s->z_bp && s->nv1 && s->nv2 && s->nv3 && s->nf_x && s->nf_y && s->nf_z,
+                "bad surface arrays");
+    return 0;
+}
+
+}   // namespace
+
+extern "C" int vfs_ib_search(vfs_ctx *c, const vfs_ib_surface *surf, vfs_ib_node *nodes, int cap, int *n_nodes)
+{
+    const Dims d = c->d;
+    if (int rc = check_surface(surf)) return rc;
+    VFS_REQUIRE(n_nodes && cap >= 0 && (cap == 0 || nodes), "bad node list");
+    const IbSurf s = to_surf(surf);
+    const int nb = (int)grid_for(d.P, LIST_BLOCK);
+    // the element bounding spheres, the per-block node counts and the fail flag
+    const size_t bytes = (size_t)s.n_elmt * 4 * sizeof(double) + (size_t)nb * 2 * sizeof(int) + sizeof(int);
+    char *buf = nullptr;
+    VFS_CHECK_HIP(hipMalloc(&buf, bytes));
+    double *qv = (double *)buf, *rad = qv + 3 * (size_t)s.n_elmt;
+    int *cnt = (int *)(rad + s.n_elmt), *off = cnt + nb, *fail = off + nb;
+    std::vector<int> h(nb);
+    int total = 0, nfail = 0;
+    hipError_t e = hipMemsetAsync(fail, 0, sizeof(int), c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_ib_bound, dim3(grid_for(s.n_elmt, 256)), dim3(256), 0, c->stream, s, qv, rad);
+        hipLaunchKernelGGL(k_ib_count, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, cnt);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), cnt, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e == hipSuccess) {
+        for (int b = 0; b < nb; b++) {   // exclusive prefix of the block counts
+            const int v = h[b];
+            h[b] = total;
+            total += v;
+        }
+        if (total <= cap && total > 0) {
+            e = hipMemcpyAsync(off, h.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_ib_place, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, off, total, nodes);
+                hipLaunchKernelGGL(k_ib_search, dim3(grid_for(total, 64)), dim3(64), 0, c->stream, d, c->g, s, qv, rad,
+                                   total, nodes, fail);
+                e = hipGetLastError();
+            }
+            if (e == hipSuccess) e = hipMemcpyAsync(&nfail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        }
+    }
+    hipFree(buf);
+    VFS_CHECK_HIP(e);
+    *n_nodes = total;
+    if (total > cap) {
+        vfs_set_error("vfs_ib_search: " + std::to_string(total) + " IB nodes do not fit the list of " +
+                      std::to_string(cap));
+        return -4;
+    }
+    if (nfail) {
+        vfs_set_error("vfs_ib_search: " + std::to_string(nfail) +
+                      " IB nodes have no nearest element or no interception point");
+        return -3;
+    }
+    return 0;
+}
+
+extern "C" int vfs_ib_interpolate(vfs_ctx *c, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
+                                  double *ucont, double *ucat, double *p, double *ustar, int *count, double *shear,
+                                  double *pressure)
+{
+    const Dims d = c->d;
+    if (int rc = check_surface(surf)) return rc;
+    VFS_REQUIRE(n >= 0 && (n == 0 || nodes) && ucont && ucat && p && ustar, "bad arguments");
+    VFS_REQUIRE(4 * (long long)n <= 3 * d.P, "more IB nodes than cells");
+    const IbSurf s = to_surf(surf);
+    double *stage = c->tmp3;   // 4n doubles
+    if (n > 0) {
+        hipLaunchKernelGGL(k_ib_nodes, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->nvert, s, nodes, n, ucat, p,
+                           ustar, stage);
+        hipLaunchKernelGGL(k_ib_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, nodes, n, stage, ucat, p);
+        VFS_CHECK_LAUNCH();
+    }
+    if (count || shear || pressure) {
+        hipLaunchKernelGGL(k_ib_stats, dim3(grid_for(s.n_elmt, STAT_BLOCK)), dim3(STAT_BLOCK), 0, c->stream, d,
+                           s.n_elmt, nodes, n, ustar, stage, count, shear, pressure);
+        VFS_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ib_faces, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, ucont);
+    VFS_CHECK_LAUNCH();
+    return vfs_contra2cart(c, ucont, ucat);
+}

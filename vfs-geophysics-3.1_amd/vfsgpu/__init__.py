@@ -66,6 +66,23 @@ class LesOptions(C.Structure):
                 ("k_homo_filter", C.c_int), ("testfilter_ik", C.c_int), ("reserved", C.c_int), ("max_cs", C.c_double)]
 
 
+class IbSurfaceC(C.Structure):
+    """vfs_ib_surface: device pointers of a triangulated body."""
+    _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
+                ("x_bp", C.c_void_p), ("y_bp", C.c_void_p), ("z_bp", C.c_void_p),
+                ("nv1", C.c_void_p), ("nv2", C.c_void_p), ("nv3", C.c_void_p),
+                ("nf_x", C.c_void_p), ("nf_y", C.c_void_p), ("nf_z", C.c_void_p), ("u", C.c_void_p)]
+
+
+# vfs_ib_node as a numpy record (the C struct's layout: 14 ints, 11 doubles)
+IB_NODE_DTYPE = np.dtype([("i", "<i4"), ("j", "<i4"), ("k", "<i4"), ("cell", "<i4"), ("imode", "<i4"),
+                          ("i1", "<i4"), ("j1", "<i4"), ("k1", "<i4"), ("i2", "<i4"), ("j2", "<i4"), ("k2", "<i4"),
+                          ("i3", "<i4"), ("j3", "<i4"), ("k3", "<i4"),
+                          ("cr1", "<f8"), ("cr2", "<f8"), ("cr3", "<f8"), ("cs1", "<f8"), ("cs2", "<f8"),
+                          ("cs3", "<f8"), ("d_s", "<f8"), ("d_i", "<f8"), ("pmin", "<f8", (3,))])
+assert IB_NODE_DTYPE.itemsize == 144
+
+
 class DeviceArray:
     """A zero-initialised device allocation of n float64 (or float32) values."""
 
@@ -203,6 +220,26 @@ class Ctx:
         _ck(lib().vfs_calc_f_eul(self.h, coor.ptr, int(cent.n // 3), cent.ptr, f_lagr.ptr, dA.ptr, box.ptr,
                                  int(deltafunc), C.c_double(halfwidth), int(width_mode), fx, f_eul.ptr),
             "vfs_calc_f_eul")
+
+    def ib_search(self, surface, cap=None):
+        """ibm_search_advanced's node list (ibm.c:366-497) for the current
+        mask -> (nodes DeviceArray of 144-byte vfs_ib_node records, n)."""
+        cap = int(cap if cap is not None else max(1, self.P // 8))
+        nodes = DeviceArray(cap * IB_NODE_DTYPE.itemsize, np.uint8)
+        n = C.c_int(0)
+        _ck(lib().vfs_ib_search(self.h, C.byref(surface.c), nodes.ptr, C.c_int(cap), C.byref(n)), "vfs_ib_search")
+        return nodes, n.value
+
+    def ib_interpolate(self, surface, nodes, n, ucont, ucat, p, ustar, stats=True):
+        """ibm_interpolation_advanced (ibm.c:3294-3698): ucont, ucat, p, ustar
+        in/out.  Returns the element (count, mean ustar^2, mean p) as
+        DeviceArrays, or None with stats=False."""
+        ne = surface.n_elmt
+        st = (DeviceArray(ne, np.int32), DeviceArray(ne), DeviceArray(ne)) if stats else (None, None, None)
+        _ck(lib().vfs_ib_interpolate(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), ucont.ptr, ucat.ptr,
+                                     p.ptr, ustar.ptr, *[None if a is None else a.ptr for a in st]),
+            "vfs_ib_interpolate")
+        return st if stats else None
 
     def set_body_force(self, f_eul):
         _ck(lib().vfs_set_body_force(self.h, None if f_eul is None else f_eul.ptr), "vfs_set_body_force")
@@ -409,3 +446,20 @@ def random_vector(cfg, amp, seed):
 
 def uniform(seed, counter):
     return lib().vfs_wl_uniform(seed, counter)
+
+
+class IbSurface:
+    """A triangulated body on the device (the reference's IBMNodes): vertices
+    (n_v, 3), elements (n_elmt, 3) int, unit normals (n_elmt, 3) pointing
+    into the fluid, optional vertex velocities (n_v, 3)."""
+
+    def __init__(self, verts, elems, normals, u=None):
+        verts = np.asarray(verts, np.float64)
+        elems = np.asarray(elems, np.int32)
+        normals = np.asarray(normals, np.float64)
+        self.n_v, self.n_elmt = len(verts), len(elems)
+        self._a = [DeviceArray.from_host(np.ascontiguousarray(verts[:, q])) for q in range(3)]
+        self._a += [DeviceArray.from_host(np.ascontiguousarray(elems[:, q])) for q in range(3)]
+        self._a += [DeviceArray.from_host(np.ascontiguousarray(normals[:, q])) for q in range(3)]
+        self.u = None if u is None else DeviceArray.from_host(np.ascontiguousarray(u, np.float64).ravel())
+        self.c = IbSurfaceC(self.n_v, self.n_elmt, *[a.ptr for a in self._a], None if self.u is None else self.u.ptr)

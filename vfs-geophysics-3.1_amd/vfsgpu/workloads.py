@@ -73,3 +73,41 @@ def poisson_rhs_random(cfg, seed, nvert, aj):
 def unit_vector(cfg, seed):
     v = random_vector(cfg, 1.0, seed)
     return v / np.linalg.norm(v)
+
+
+def icosphere(cx, cy, cz, r, level):
+    """A triangulated sphere (the body of a CURVIB search): the icosahedron
+    with each triangle split in four `level` times, vertices projected onto
+    the sphere.  Returns (verts (n_v, 3), elems (n_elmt, 3) int32, outward
+    unit normals (n_elmt, 3)); 20 * 4**level elements."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(level):
+        mid = {}
+
+        def midpoint(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in mid:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                mid[key] = len(verts) - 1
+            return mid[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = midpoint(a, b), midpoint(b, c), midpoint(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    x = np.array(verts) * r + np.array([cx, cy, cz])
+    e = np.array(faces, np.int32)
+    n = np.cross(x[e[:, 1]] - x[e[:, 0]], x[e[:, 2]] - x[e[:, 0]])
+    n /= np.linalg.norm(n, axis=1)[:, None]
+    out = np.einsum("ij,ij->i", n, x[e[:, 0]] - np.array([cx, cy, cz])) < 0
+    e[out] = e[out][:, [0, 2, 1]]
+    n[out] = -n[out]
+    return x, e, n
