@@ -27,7 +27,7 @@ int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale);
 
 // Build-time shape parameters (tuned on MI355X; see the Makefile).
 #ifndef VFS_FACES_KCHUNK
-#define VFS_FACES_KCHUNK 16      // k-planes per workgroup of the face kernel
+#define VFS_FACES_KCHUNK 32      // k-planes per workgroup of the face kernel
 #endif
 #ifndef VFS_FACES_SPLIT
 #define VFS_FACES_SPLIT 0        // 1: convective and viscous fluxes in separate waves
