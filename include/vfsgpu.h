@@ -168,6 +168,22 @@ typedef struct {
     double d_i;                     /* node to the interception point */
     double pmin[3];                 /* the nearest point on the surface */
 } vfs_ib_node;
+/* The marking of ibm_search_advanced (ibm.c:110-360, 508-516) for one body:
+ * every interior cell whose centre lies inside the body's box (widened by
+ * 0.05) casts a ray towards a jittered point above the body
+ * (randomdirection, ibm.c:52-84; point_cell_advanced, ibm.c:1068-1183)
+ * against the elements binned on a 40^3 grid; an odd count of crossings
+ * marks it inside, two crossings closer than 1e-8 cast the next ray.  Then
+ * the 3x3x3 dilation and the relabelling: 3 solid, 1 IB node, as
+ * vfs_nvert_sphere.  reset: the first body (nvert < 5 cleared, ibm.c:247);
+ * later bodies add to the mask.  -3 if a cell finds no clear ray in 32
+ * tries.  The context's mask is set; nvert_out (P doubles) may be NULL.
+ * Synchronizes the stream. */
+int vfs_nvert_body(vfs_ctx *ctx, const vfs_ib_surface *surf, int reset, double *nvert_out);
+/* The ray jitter of randomdirection: the first rand() after srand(seed) of
+ * the C library, / (RAND_MAX + 1) - 0.5 (host only; the caller's rand()
+ * sequence is left alone). */
+int vfs_ib_ray_jitter(int seed, double *s);
 /* The node list of ibm_search_advanced (ibm.c:366-497) for a mask already
  * set (vfs_set_nvert / vfs_nvert_sphere): every interior cell with
  * (int)(nvert+0.5) == 1, except the planes of wall-function walls, gets the

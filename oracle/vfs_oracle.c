@@ -3002,3 +3002,190 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
             }
     return vo_contra2cart(c, g, nvert, ucont, ucat);
 }
+
+/* ------------------------------------------------------------------------ */
+/* ibm_search_advanced's marking (ibm.c:110-360): cells of a triangulated     */
+/* body by ray casting against a 40^3 bin grid of the elements, then the     */
+/* 3x3x3 dilation and the relabelling (ibm.c:319-341, 508-516)                */
+/* ------------------------------------------------------------------------ */
+#define IB_NC 40
+
+/* compgeom.c:17-51 intsect_triangle (Moller-Trumbore, EPSILON 1e-8) */
+static int ib_intsect(const double o[3], const double dir[3], const double v0[3], const double v1[3],
+                      const double v2[3], double *t, double *u, double *v)
+{
+    double e1[3], e2[3], tv[3], pv[3], qv[3];
+    for (int q = 0; q < 3; q++) e1[q] = v1[q] - v0[q], e2[q] = v2[q] - v0[q];
+    pv[0] = dir[1] * e2[2] - dir[2] * e2[1];
+    pv[1] = dir[2] * e2[0] - dir[0] * e2[2];
+    pv[2] = dir[0] * e2[1] - dir[1] * e2[0];
+    const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det > -1e-8 && det < 1e-8) return 0;
+    const double inv = 1.0 / det;
+    for (int q = 0; q < 3; q++) tv[q] = o[q] - v0[q];
+    *u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+    if (*u < 0.0 || *u > 1.0) return 0;
+    qv[0] = tv[1] * e1[2] - tv[2] * e1[1];
+    qv[1] = tv[2] * e1[0] - tv[0] * e1[2];
+    qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
+    *v = (dir[0] * qv[0] + dir[1] * qv[1] + dir[2] * qv[2]) * inv;
+    if (*v < 0.0 || *u + *v > 1.0) return 0;
+    *t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+    return 1;
+}
+
+/* the first rand() after srand(seed), scaled to [-0.5, 0.5) (ibm.c:66-68) */
+double vo_ib_ray_jitter(int seed)
+{
+    srand(seed);
+    return rand() / ((double)RAND_MAX + 1) - 0.5;
+}
+
+typedef struct {
+    double xmin, ymin, zmin, xmax, ymax, zmax, dcx, dcy, dcz;
+    int *head, *next, *node, n;   /* bin lists: head[bin] -> entry, next[entry], node[entry] */
+} ib_bins;
+
+/* ibm.c:135-209: the body's box widened by 0.05 and every element listed in
+ * the bins its own box touches (insertnode keeps one entry per element) */
+static void ib_make_bins(const vo_ib_surface *s, ib_bins *b)
+{
+    b->xmin = b->ymin = b->zmin = 1.e23;
+    b->xmax = b->ymax = b->zmax = -1.e23;
+    for (int v = 0; v < s->n_v; v++) {
+        b->xmin = fmin(b->xmin, s->x_bp[v]); b->xmax = fmax(b->xmax, s->x_bp[v]);
+        b->ymin = fmin(b->ymin, s->y_bp[v]); b->ymax = fmax(b->ymax, s->y_bp[v]);
+        b->zmin = fmin(b->zmin, s->z_bp[v]); b->zmax = fmax(b->zmax, s->z_bp[v]);
+    }
+    b->xmin -= 0.05; b->xmax += 0.05;
+    b->ymin -= 0.05; b->ymax += 0.05;
+    b->zmin -= 0.05; b->zmax += 0.05;
+    b->dcx = (b->xmax - b->xmin) / (IB_NC - 1.);
+    b->dcy = (b->ymax - b->ymin) / (IB_NC - 1.);
+    b->dcz = (b->zmax - b->zmin) / (IB_NC - 1.);
+    const int nb = IB_NC * IB_NC * IB_NC;
+    b->head = malloc(sizeof(int) * nb);
+    for (int q = 0; q < nb; q++) b->head[q] = -1;
+    int cap = 1024, n = 0;
+    b->next = malloc(sizeof(int) * cap);
+    b->node = malloc(sizeof(int) * cap);
+    for (int e = 0; e < s->n_elmt; e++) {
+        const int a = s->nv1[e], bb = s->nv2[e], c = s->nv3[e];
+        const double x0 = fmin(fmin(s->x_bp[a], s->x_bp[bb]), s->x_bp[c]), x1 = fmax(fmax(s->x_bp[a], s->x_bp[bb]), s->x_bp[c]);
+        const double y0 = fmin(fmin(s->y_bp[a], s->y_bp[bb]), s->y_bp[c]), y1 = fmax(fmax(s->y_bp[a], s->y_bp[bb]), s->y_bp[c]);
+        const double z0 = fmin(fmin(s->z_bp[a], s->z_bp[bb]), s->z_bp[c]), z1 = fmax(fmax(s->z_bp[a], s->z_bp[bb]), s->z_bp[c]);
+        int i0 = (int)floor((x0 - b->xmin) / b->dcx), i1 = (int)floor((x1 - b->xmin) / b->dcx) + 1;
+        int j0 = (int)floor((y0 - b->ymin) / b->dcy), j1 = (int)floor((y1 - b->ymin) / b->dcy) + 1;
+        int k0 = (int)floor((z0 - b->zmin) / b->dcz), k1 = (int)floor((z1 - b->zmin) / b->dcz) + 1;
+        i0 = i0 < 0 ? 0 : i0; i1 = i1 > IB_NC ? IB_NC : i1;
+        j0 = j0 < 0 ? 0 : j0; j1 = j1 > IB_NC ? IB_NC : j1;
+        k0 = k0 < 0 ? 0 : k0; k1 = k1 > IB_NC ? IB_NC : k1;
+        for (int k = k0; k < k1; k++)
+            for (int j = j0; j < j1; j++)
+                for (int i = i0; i < i1; i++) {
+                    const int bin = (k * IB_NC + j) * IB_NC + i;
+                    int dup = 0;
+                    for (int q = b->head[bin]; q >= 0 && !dup; q = b->next[q]) dup = b->node[q] == e;
+                    if (dup) continue;
+                    if (n == cap) {
+                        cap *= 2;
+                        b->next = realloc(b->next, sizeof(int) * cap);
+                        b->node = realloc(b->node, sizeof(int) * cap);
+                    }
+                    b->node[n] = e; b->next[n] = b->head[bin]; b->head[bin] = n; n++;
+                }
+    }
+    b->n = n;
+}
+
+/* ibm.c:1068-1183 point_cell_advanced: 4 if the ray from p towards a
+ * jittered point above the body crosses the surface an odd number of times,
+ * else 0; a ray with two crossings within 1e-8 of each other (normals not
+ * opposed) is cast again with the next seed */
+static int ib_point_cell(const vo_ib_surface *s, const ib_bins *b, const double p[3], int ip, int jp, int kp,
+                         char *searched)
+{
+    for (int seed = 1; seed < 100000; seed++) {
+        const double xpc = b->dcx * (ip + 0.5) + b->xmin, ypc = b->dcy * (jp + 0.5) + b->ymin;
+        const double sj = vo_ib_ray_jitter(seed);
+        double dir[3] = {xpc + sj * b->dcx - p[0], ypc + sj * b->dcy - p[1], b->zmax + 0.2 - p[2]};
+        const double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        dir[0] /= len, dir[1] /= len, dir[2] /= len;
+        memset(searched, 0, s->n_elmt);
+        double dt[1000], dn[1000][3];
+        int nintp = 0, sing = 0;
+        for (int k = kp; k < IB_NC && !sing; k++)
+            for (int q = b->head[(k * IB_NC + jp) * IB_NC + ip]; q >= 0 && !sing; q = b->next[q]) {
+                const int e = b->node[q];
+                if (searched[e]) continue;
+                searched[e] = 1;
+                double v0[3], v1[3], v2[3], t, u, v;
+                ib_vertex(s, s->nv1[e], v0); ib_vertex(s, s->nv2[e], v1); ib_vertex(s, s->nv3[e], v2);
+                if (!(ib_intsect(p, dir, v0, v1, v2, &t, &u, &v) > 0 && t > 0)) continue;
+                if (nintp == 1000) return -1;
+                dt[nintp] = t;
+                dn[nintp][0] = s->nf_x[e], dn[nintp][1] = s->nf_y[e], dn[nintp][2] = s->nf_z[e];
+                nintp++;
+                for (int m = 0; m < nintp - 1; m++) {
+                    const double ndotn = dn[m][0] * s->nf_x[e] + dn[m][1] * s->nf_y[e] + dn[m][2] * s->nf_z[e];
+                    if (fabs(t - dt[m]) < 1e-8 && ndotn > -0.97) sing = 1;
+                }
+            }
+        if (!sing) return nintp % 2 ? 4 : 0;
+    }
+    return -1;
+}
+
+int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    if (s->n_elmt <= 0 || s->n_v <= 0) return -2;
+    ib_bins b;
+    ib_make_bins(s, &b);
+    char *searched = malloc(s->n_elmt);
+    for (int k = 1; k < mz - 1 && !rc; k++)
+        for (int j = 1; j < my - 1 && !rc; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t id = IX(k, j, i);
+                const double *p = &g->cent[3 * id];
+                if (reset && nvert[id] < 5.) nvert[id] = 0;
+                if (!(p[0] > b.xmin && p[0] < b.xmax && p[1] > b.ymin && p[1] < b.ymax && p[2] > b.zmin && p[2] < b.zmax))
+                    continue;
+                const int ic = (int)floor((p[0] - b.xmin) / b.dcx), jc = (int)floor((p[1] - b.ymin) / b.dcy),
+                          kc = (int)floor((p[2] - b.zmin) / b.dcz);
+                const int val = ib_point_cell(s, &b, p, ic, jc, kc, searched);
+                if (val < 0) { rc = -3; break; }
+                nvert[id] = nvert[id] > val ? nvert[id] : val;
+                if (nvert[id] < 0) nvert[id] = 0;
+            }
+    free(searched);
+    free(b.head); free(b.next); free(b.node);
+    if (rc) return rc;
+    /* the dilation and the relabelling, as for the sphere */
+    for (int k = 0; k < mz; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++) {
+                size_t id = IX(k, j, i);
+                if (nvert[id] < 0) nvert[id] = 0;
+                int ip = (i < mx - 1 ? (i + 1) : (i)), im = (i > 0 ? (i - 1) : (i));
+                int jp = (j < my - 1 ? (j + 1) : (j)), jm = (j > 0 ? (j - 1) : (j));
+                int kp = (k < mz - 1 ? (k + 1) : (k)), km = (k > 0 ? (k - 1) : (k));
+                if ((int)(nvert[id] + 0.5) != 4) {
+                    for (int kk = km; kk < kp + 1; kk++)
+                        for (int jj = jm; jj < jp + 1; jj++)
+                            for (int ii = im; ii < ip + 1; ii++)
+                                if ((int)(nvert[IX(kk, jj, ii)] + 0.5) == 4)
+                                    nvert[id] = nvert[id] > 2 ? nvert[id] : 2;
+                }
+            }
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if ((int)(nvert[id] + 0.5) == 2) nvert[id] = 1;
+                if ((int)(nvert[id] + 0.5) == 4) nvert[id] = 3;
+            }
+    return 0;
+}

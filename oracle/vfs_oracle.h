@@ -258,6 +258,14 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
                       const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
                       int *count, double *shear, double *pressure);
 
+/* ibm_search_advanced's marking (ibm.c:110-360, 508-516): nvert of one
+ * triangulated body by ray casting (point_cell_advanced, ibm.c:1068-1183),
+ * the 3x3x3 dilation and the relabelling to 3 / 1.  reset: the first body
+ * of a search (nvert < 5 cleared first).  -3: a ray never came clear. */
+int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset);
+/* the first rand() after srand(seed), scaled to [-0.5, 0.5) */
+double vo_ib_ray_jitter(int seed);
+
 #ifdef __cplusplus
 }
 #endif

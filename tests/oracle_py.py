@@ -80,7 +80,7 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter"]:
             getattr(_lib, name).restype = d
         for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
@@ -228,6 +228,11 @@ class Oracle:
         if rc:
             raise RuntimeError(f"vo_ib_search rc={rc}")
         return nodes[:n.value].copy()
+
+    def nvert_body(self, body, reset=True):
+        """ibm_search_advanced's marking of one body into self.nvert."""
+        self._call("vo_nvert_body", C.byref(self.geom), ptr(self.nvert), C.byref(body.c), C.c_int(int(bool(reset))))
+        return self.nvert
 
     def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar):
         """ibm_interpolation_advanced; ucont, ucat, p, ustar in/out.

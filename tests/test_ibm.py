@@ -17,7 +17,11 @@ CPU tests pin the oracle:
     ustar = sqrt(|shear| / Re), p the value at the interception point;
   - a uniform stream past a sphere on a uniform grid (-inv) comes back
     unchanged at every node, face flux and fluid cell;
-  - the list runs from the last cell in k, j, i order to the first.
+  - the list runs from the last cell in k, j, i order to the first;
+  - the ray-casting marking of a triangulated body (point_cell_advanced)
+    against the exact inside test of a convex polyhedron (every face plane
+    has the point on its inner side), for one sphere and for two, with the
+    ray jitter equal to the C library's srand / rand.
 GPU tests compare the HIP path with the oracle.
 """
 import numpy as np
@@ -262,6 +266,74 @@ def test_uniform_stream_survives_the_reconstruction():
 
 
 # ---------------------------------------------------------------------------
+# the ray-casting marking
+# ---------------------------------------------------------------------------
+SPHERE_B = (W.LX / 2 + 0.6, 0.8, W.LZ / 2 - 0.3, 0.2)
+
+
+def two_spheres(level=3):
+    x1, e1, n1 = W.icosphere(*W.SPHERE, level=level)
+    x2, e2, n2 = W.icosphere(*SPHERE_B, level=level)
+    return np.vstack([x1, x2]), np.vstack([e1, e2 + len(x1)]), np.vstack([n1, n2])
+
+
+def convex_inside(cent, x, e, n):
+    """Strictly inside the convex polyhedron: every face plane has the point
+    on its inner side; None where a point lies within 1e-9 of a plane."""
+    d = np.einsum("pk,ek->pe", cent, n) - np.einsum("ek,ek->e", x[e[:, 0]], n)[None, :]
+    inside = (d < -1e-9).all(axis=1)
+    near = (np.abs(d) <= 1e-9).any(axis=1) & (d <= 1e-9).all(axis=1)
+    return inside, near
+
+
+def test_ray_jitter_is_the_c_library_rand():
+    import ctypes.util
+    libc = C.CDLL(ctypes.util.find_library("c"))
+    for seed in (1, 2, 3, 17, 31):
+        libc.srand(seed)
+        want = libc.rand() / (2147483647 + 1.0) - 0.5
+        assert V.ray_jitter(seed) == want
+        assert lib().vo_ib_ray_jitter(seed) == want
+
+
+@pytest.mark.parametrize("bodies", [1, 2])
+def test_ray_casting_marks_the_inside_of_convex_bodies(bodies):
+    cfg, spec, coor, o, _, _ = oracle_case("C2", (48, 40, 32))
+    o.nvert[:] = 0
+    if bodies == 1:
+        parts = [W.icosphere(*W.SPHERE, level=3)]
+    else:
+        parts = [W.icosphere(*W.SPHERE, level=3), W.icosphere(*SPHERE_B, level=3)]
+    for q, (x, e, n) in enumerate(parts):
+        o.nvert_body(IbBody(x, e, n), reset=(q == 0))
+    cent = o.g["cent"].reshape(-1, 3)
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    inner = np.zeros((mz, my, mx), bool)
+    inner[1:-1, 1:-1, 1:-1] = True
+    inner = inner.ravel()
+    want = np.zeros(o.P, bool)
+    for x, e, n in parts:
+        ins, near = convex_inside(cent, x, e, n)
+        assert not near[inner].any()
+        want |= ins
+    got = np.rint(o.nvert)
+    assert np.array_equal(got[inner] == 3, want[inner])
+    # the dilation: 1 exactly where a solid cell is in the 3x3x3 box
+    sol = (got == 3).reshape(mz, my, mx)
+    near = np.zeros_like(sol)
+    for dk in (-1, 0, 1):
+        for dj in (-1, 0, 1):
+            for di in (-1, 0, 1):
+                sh = np.zeros_like(sol)
+                sh[max(dk, 0):mz + min(dk, 0), max(dj, 0):my + min(dj, 0), max(di, 0):mx + min(di, 0)] = \
+                    sol[max(-dk, 0):mz - max(dk, 0), max(-dj, 0):my - max(dj, 0), max(-di, 0):mx - max(di, 0)]
+                near |= sh
+    ib = (near & ~sol).ravel() & inner
+    assert np.array_equal(got[inner] == 1, ib[inner])
+    assert (got == 3).sum() > 20
+
+
+# ---------------------------------------------------------------------------
 # GPU parity
 # ---------------------------------------------------------------------------
 def gpu_surface(x, e, n, u=None):
@@ -331,3 +403,34 @@ def test_gpu_ib_reconstruction_plane(inviscid):
     cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case(inviscid=inviscid)
     ucont = np.sin(np.arange(3 * o.P) * 0.11)
     assert run_pair(cfg, coor, o, (verts, elems, normals, uv), ucat, ucont, p) > 100
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bodies", [1, 2])
+def test_gpu_nvert_body(bodies):
+    cfg, spec, coor, o, _, _ = oracle_case("C2", (48, 40, 32))
+    o.nvert[:] = 0
+    parts = [W.icosphere(*W.SPHERE, level=3)] + ([W.icosphere(*SPHERE_B, level=3)] if bodies == 2 else [])
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    for q, (x, e, n) in enumerate(parts):
+        o.nvert_body(IbBody(x, e, n), reset=(q == 0))
+        got = ctx.nvert_body(V.IbSurface(x, e, n), reset=(q == 0))
+    assert np.array_equal(got.host(), o.nvert)
+    assert np.array_equal(ctx.nvert().host(), o.nvert)
+
+
+@pytest.mark.gpu
+def test_gpu_nvert_body_full_size_matches_search_chain():
+    # the triangulated sphere at 256 x 128 x 128: marking, node list and
+    # reconstruction all against the oracle
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2", W.BASE["C2"]["N"])
+    x, e, n = W.icosphere(*W.SPHERE, level=4)
+    o.nvert[:] = 0
+    o.nvert_body(IbBody(x, e, n))
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    got = ctx.nvert_body(V.IbSurface(x, e, n))
+    assert np.array_equal(got.host(), o.nvert)
+    n_nodes = run_pair(cfg, coor, o, (x, e, n, None), ucat, ucont, np.cos(np.arange(o.P) * 0.013))
+    print("IB nodes of the triangulated sphere", n_nodes, flush=True)

@@ -15,6 +15,8 @@
 // per-step pass reads the old fields and writes the new node values through
 // a compact staging array, as the reference reads lUcat / lP while it writes
 // Ucat / P.
+#include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "vfs_internal.h"
@@ -495,6 +497,101 @@ __global__ void __launch_bounds__(256) k_ib_faces(Dims d, Geom g, const double *
     ucont[3 * t + 2] = uz;
 }
 
+// ---- the marking of ibm_search_advanced (ibm.c:110-360) ----------------
+constexpr int NBIN = 40;        // ncx = ncy = ncz (ibm.c:104)
+constexpr int NTRY = 32;        // rays cast before a cell is reported undecided
+constexpr int MAXHIT = 32;      // crossings kept per ray for the coincidence test
+
+struct BodyBox {
+    double xmin, ymin, zmin, xmax, ymax, zmax, dcx, dcy, dcz;
+};
+
+// compgeom.c:17-51 intsect_triangle (Moller-Trumbore, EPSILON 1e-8)
+__device__ int intsect(const double o[3], const double dir[3], const double v0[3], const double v1[3],
+                       const double v2[3], double &t)
+{
+    double e1[3], e2[3], tv[3], pv[3], qv[3];
+    for (int q = 0; q < 3; q++) e1[q] = v1[q] - v0[q], e2[q] = v2[q] - v0[q];
+    pv[0] = dir[1] * e2[2] - dir[2] * e2[1];
+    pv[1] = dir[2] * e2[0] - dir[0] * e2[2];
+    pv[2] = dir[0] * e2[1] - dir[1] * e2[0];
+    const double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    if (det > -1e-8 && det < 1e-8) return 0;
+    const double inv = 1.0 / det;
+    for (int q = 0; q < 3; q++) tv[q] = o[q] - v0[q];
+    const double u = (tv[0] * pv[0] + tv[1] * pv[1] + tv[2] * pv[2]) * inv;
+    if (u < 0.0 || u > 1.0) return 0;
+    qv[0] = tv[1] * e1[2] - tv[2] * e1[1];
+    qv[1] = tv[2] * e1[0] - tv[0] * e1[2];
+    qv[2] = tv[0] * e1[1] - tv[1] * e1[0];
+    const double v = (dir[0] * qv[0] + dir[1] * qv[1] + dir[2] * qv[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    t = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) * inv;
+    return 1;
+}
+
+// point_cell_advanced (ibm.c:1068-1183) for every interior cell whose centre
+// lies strictly inside the widened box: the elements listed in the bins
+// (ic, jc, k >= kc) are those whose own bin range holds that column, tested
+// once each.  The result (the parity of the crossings, or a coincidence that
+// asks for the next ray) does not depend on the order the elements are
+// tested in.  reset clears nvert < 5 first (the first body of a search).
+__global__ void __launch_bounds__(256) k_body_inside(Dims d, const double *__restrict__ cent, IbSurf s,
+                                                     const int *__restrict__ bins, BodyBox b,
+                                                     const double *__restrict__ jitter, int reset,
+                                                     double *__restrict__ nvert, int *__restrict__ fail)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    if (i < 1 || i > d.mx - 2 || j < 1 || j > d.my - 2 || k < 1 || k > d.mz - 2) return;
+    double nv = nvert[t];
+    if (reset && nv < 5.) nv = 0;
+    const double p[3] = {cent[3 * t], cent[3 * t + 1], cent[3 * t + 2]};
+    if (p[0] > b.xmin && p[0] < b.xmax && p[1] > b.ymin && p[1] < b.ymax && p[2] > b.zmin && p[2] < b.zmax) {
+        const int ic = (int)floor((p[0] - b.xmin) / b.dcx), jc = (int)floor((p[1] - b.ymin) / b.dcy),
+                  kc = (int)floor((p[2] - b.zmin) / b.dcz);
+        int val = -1;
+        for (int seed = 1; seed <= NTRY && val < 0; seed++) {
+            const double xpc = b.dcx * (ic + 0.5) + b.xmin, ypc = b.dcy * (jc + 0.5) + b.ymin;
+            const double sj = jitter[seed - 1];
+            double dir[3] = {xpc + sj * b.dcx - p[0], ypc + sj * b.dcy - p[1], b.zmax + 0.2 - p[2]};
+            const double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+            dir[0] /= len, dir[1] /= len, dir[2] /= len;
+            double dt[MAXHIT];
+            int de[MAXHIT];
+            int nintp = 0, sing = 0;
+            for (int e = 0; e < s.n_elmt && !sing; e++) {
+                const int *r = &bins[6 * e];
+                if (!(r[0] <= ic && ic < r[1] && r[2] <= jc && jc < r[3] && (r[4] > kc ? r[4] : kc) < r[5])) continue;
+                double v0[3], v1[3], v2[3], tt;
+                vertex(s, s.nv1[e], v0);
+                vertex(s, s.nv2[e], v1);
+                vertex(s, s.nv3[e], v2);
+                if (!(intsect(p, dir, v0, v1, v2, tt) > 0 && tt > 0)) continue;
+                if (nintp == MAXHIT) {
+                    sing = 2;
+                    break;
+                }
+                for (int m = 0; m < nintp; m++) {
+                    const int f = de[m];
+                    const double ndotn = s.nfx[f] * s.nfx[e] + s.nfy[f] * s.nfy[e] + s.nfz[f] * s.nfz[e];
+                    if (fabs(tt - dt[m]) < 1e-8 && ndotn > -0.97) sing = 1;
+                }
+                dt[nintp] = tt;
+                de[nintp] = e;
+                nintp++;
+            }
+            if (sing == 2) break;
+            if (!sing) val = nintp % 2 ? 4 : 0;
+        }
+        if (val < 0) atomicAdd(fail, 1);
+        else nv = nv > val ? nv : (double)val;
+        if (nv < 0) nv = 0;
+    }
+    nvert[t] = nv;
+}
+
 int check_surface(const vfs_ib_surface *s)
 {
     VFS_REQUIRE(s && s->n_v > 0 && s->n_elmt > 0, "the body needs vertices and elements");
@@ -586,4 +683,99 @@ extern "C" int vfs_ib_interpolate(vfs_ctx *c, const vfs_ib_surface *surf, const 
     hipLaunchKernelGGL(k_ib_faces, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucat, ucont);
     VFS_CHECK_LAUNCH();
     return vfs_contra2cart(c, ucont, ucat);
+}
+
+// the first rand() after srand(seed) of glibc, scaled to [-0.5, 0.5) as
+// randomdirection does (ibm.c:52-84), from a private generator state so the
+// caller's rand() sequence is left alone (srand/rand are random()'s TYPE_3
+// generator with its 128-byte state)
+extern "C" int vfs_ib_ray_jitter(int seed, double *s)
+{
+    VFS_REQUIRE(s, "bad output");
+    struct random_data rd = {};
+    char state[128];
+    int32_t v = 0;
+    VFS_REQUIRE(initstate_r((unsigned)seed, state, sizeof state, &rd) == 0 && random_r(&rd, &v) == 0,
+                "random_r failed");
+    *s = v / ((double)RAND_MAX + 1) - 0.5;
+    return 0;
+}
+
+extern "C" int vfs_nvert_body(vfs_ctx *c, const vfs_ib_surface *surf, int reset, double *nvert_out)
+{
+    const Dims d = c->d;
+    if (int rc = check_surface(surf)) return rc;
+    const IbSurf s = to_surf(surf);
+    // the widened box and every element's bin range, on the host (ibm.c:135-209)
+    std::vector<double> x(s.n_v), y(s.n_v), z(s.n_v);
+    std::vector<int> n1(s.n_elmt), n2(s.n_elmt), n3(s.n_elmt);
+    VFS_CHECK_HIP(hipMemcpy(x.data(), s.x, s.n_v * sizeof(double), hipMemcpyDeviceToHost));
+    VFS_CHECK_HIP(hipMemcpy(y.data(), s.y, s.n_v * sizeof(double), hipMemcpyDeviceToHost));
+    VFS_CHECK_HIP(hipMemcpy(z.data(), s.z, s.n_v * sizeof(double), hipMemcpyDeviceToHost));
+    VFS_CHECK_HIP(hipMemcpy(n1.data(), s.nv1, s.n_elmt * sizeof(int), hipMemcpyDeviceToHost));
+    VFS_CHECK_HIP(hipMemcpy(n2.data(), s.nv2, s.n_elmt * sizeof(int), hipMemcpyDeviceToHost));
+    VFS_CHECK_HIP(hipMemcpy(n3.data(), s.nv3, s.n_elmt * sizeof(int), hipMemcpyDeviceToHost));
+    BodyBox b;
+    b.xmin = b.ymin = b.zmin = 1.e23;
+    b.xmax = b.ymax = b.zmax = -1.e23;
+    for (int v = 0; v < s.n_v; v++) {
+        VFS_REQUIRE(std::isfinite(x[v]) && std::isfinite(y[v]) && std::isfinite(z[v]), "vertex coordinates must be finite");
+        b.xmin = std::fmin(b.xmin, x[v]); b.xmax = std::fmax(b.xmax, x[v]);
+        b.ymin = std::fmin(b.ymin, y[v]); b.ymax = std::fmax(b.ymax, y[v]);
+        b.zmin = std::fmin(b.zmin, z[v]); b.zmax = std::fmax(b.zmax, z[v]);
+    }
+    b.xmin -= 0.05; b.xmax += 0.05;
+    b.ymin -= 0.05; b.ymax += 0.05;
+    b.zmin -= 0.05; b.zmax += 0.05;
+    b.dcx = (b.xmax - b.xmin) / (NBIN - 1.);
+    b.dcy = (b.ymax - b.ymin) / (NBIN - 1.);
+    b.dcz = (b.zmax - b.zmin) / (NBIN - 1.);
+    std::vector<int> bins(6 * (size_t)s.n_elmt);
+    for (int e = 0; e < s.n_elmt; e++) {
+        const int a = n1[e], bb = n2[e], cc = n3[e];
+        VFS_REQUIRE(a >= 0 && a < s.n_v && bb >= 0 && bb < s.n_v && cc >= 0 && cc < s.n_v, "element vertex out of range");
+        const double x0 = std::fmin(std::fmin(x[a], x[bb]), x[cc]), x1 = std::fmax(std::fmax(x[a], x[bb]), x[cc]);
+        const double y0 = std::fmin(std::fmin(y[a], y[bb]), y[cc]), y1 = std::fmax(std::fmax(y[a], y[bb]), y[cc]);
+        const double z0 = std::fmin(std::fmin(z[a], z[bb]), z[cc]), z1 = std::fmax(std::fmax(z[a], z[bb]), z[cc]);
+        int r[6] = {(int)std::floor((x0 - b.xmin) / b.dcx), (int)std::floor((x1 - b.xmin) / b.dcx) + 1,
+                    (int)std::floor((y0 - b.ymin) / b.dcy), (int)std::floor((y1 - b.ymin) / b.dcy) + 1,
+                    (int)std::floor((z0 - b.zmin) / b.dcz), (int)std::floor((z1 - b.zmin) / b.dcz) + 1};
+        for (int q = 0; q < 6; q += 2) {
+            r[q] = r[q] < 0 ? 0 : r[q];
+            r[q + 1] = r[q + 1] > NBIN ? NBIN : r[q + 1];
+        }
+        for (int q = 0; q < 6; q++) bins[6 * (size_t)e + q] = r[q];
+    }
+    double jit[NTRY];
+    for (int q = 0; q < NTRY; q++)
+        if (vfs_ib_ray_jitter(q + 1, &jit[q])) return -1;
+    const size_t bytes = bins.size() * sizeof(int) + sizeof jit + sizeof(int);
+    char *buf = nullptr;
+    VFS_CHECK_HIP(hipMalloc(&buf, bytes));
+    int *dbins = (int *)buf;
+    double *djit = (double *)(buf + bins.size() * sizeof(int));   // 24 bytes an element: 8-aligned
+    int *fail = (int *)(djit + NTRY);
+    int nfail = 0;
+    hipError_t e = hipMemcpyAsync(dbins, bins.data(), bins.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(djit, jit, sizeof jit, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(fail, 0, sizeof(int), c->stream);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_body_inside, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g.cent, s, dbins, b,
+                           djit, reset ? 1 : 0, c->nvert, fail);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&nfail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipFree(buf);
+    VFS_CHECK_HIP(e);
+    if (nfail) {
+        vfs_set_error("vfs_nvert_body: " + std::to_string(nfail) + " cells found no clear ray in " +
+                      std::to_string(NTRY) + " tries");
+        return -3;
+    }
+    if (vfs_mask_dilate_relabel(c)) return -1;
+    c->nvert_f32 = 1;   // small integers
+    if (nvert_out)
+        VFS_CHECK_HIP(hipMemcpyAsync(nvert_out, c->nvert, (size_t)d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return 0;
 }

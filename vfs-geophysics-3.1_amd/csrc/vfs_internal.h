@@ -144,6 +144,9 @@ struct vfs_ctx {
     Poisson *pois;
 };
 
+// the 3x3x3 dilation and 3 / 1 relabelling of an inside-marked mask (vfs_metrics.hip)
+int vfs_mask_dilate_relabel(vfs_ctx *c);
+
 // wall-function walls (vfs_wall.hip)
 bool vfs_has_wallfn(const Dims &d);
 int vfs_wall_setup(vfs_ctx *c);

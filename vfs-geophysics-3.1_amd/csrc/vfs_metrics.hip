@@ -263,6 +263,18 @@ __global__ void k_sphere_relabel(Dims d, double *nvert)
     nvert[t] = v;
 }
 
+// the dilation and the relabelling that follow the inside test of any body
+int vfs_mask_dilate_relabel(vfs_ctx *c)
+{
+    const Dims d = c->d;
+    unsigned gb = grid_for(d.P, 256);
+    hipLaunchKernelGGL(k_sphere_dilate, dim3(gb), dim3(256), 0, c->stream, d, c->nvert);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sphere_relabel, dim3(gb), dim3(256), 0, c->stream, d, c->nvert);
+    VFS_CHECK_LAUNCH();
+    return 0;
+}
+
 extern "C" int vfs_nvert_sphere(vfs_ctx *c, double cx, double cy, double cz, double r, double *nvert_out)
 {
     const Dims d = c->d;

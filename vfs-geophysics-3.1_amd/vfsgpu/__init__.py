@@ -221,6 +221,13 @@ class Ctx:
                                  int(deltafunc), C.c_double(halfwidth), int(width_mode), fx, f_eul.ptr),
             "vfs_calc_f_eul")
 
+    def nvert_body(self, surface, reset=True):
+        """ibm_search_advanced's marking (ibm.c:110-360) of one triangulated
+        body into the context's mask -> the mask as a DeviceArray."""
+        out = self.sca()
+        _ck(lib().vfs_nvert_body(self.h, C.byref(surface.c), int(bool(reset)), out.ptr), "vfs_nvert_body")
+        return out
+
     def ib_search(self, surface, cap=None):
         """ibm_search_advanced's node list (ibm.c:366-497) for the current
         mask -> (nodes DeviceArray of 144-byte vfs_ib_node records, n)."""
@@ -463,3 +470,10 @@ class IbSurface:
         self._a += [DeviceArray.from_host(np.ascontiguousarray(normals[:, q])) for q in range(3)]
         self.u = None if u is None else DeviceArray.from_host(np.ascontiguousarray(u, np.float64).ravel())
         self.c = IbSurfaceC(self.n_v, self.n_elmt, *[a.ptr for a in self._a], None if self.u is None else self.u.ptr)
+
+
+def ray_jitter(seed):
+    """randomdirection's jitter for a seed (the C library's first rand())."""
+    v = C.c_double()
+    _ck(lib().vfs_ib_ray_jitter(int(seed), C.byref(v)), "vfs_ib_ray_jitter")
+    return v.value
