@@ -291,14 +291,22 @@ def actuator_leg(ctx, cfg, spec, reps, with_cpu):
 
 
 def ib_leg(ctx, cfg, spec, s, reps, with_cpu):
-    """CURVIB reconstruction at the immersed sphere: the node search against
-    the sphere as 5120 triangles (once per mask, or every step for a moving
-    body) and the per-step ibm_interpolation_advanced pass (node values,
-    element means, IB-face fluxes, Contra2Cart), on copies of the state."""
+    """CURVIB reconstruction at the immersed sphere as 5120 triangles: the
+    ray-casting mask and the node search (once per body position) and the
+    per-step ibm_interpolation_advanced pass (node values, element means,
+    IB-face fluxes, Contra2Cart), on copies of the state."""
     import vfsgpu as V
     from vfsgpu import workloads as W
     x, e, n = W.icosphere(*W.SPHERE, level=4)
     surf = V.IbSurface(x, e, n)
+    # the mask of the triangulated sphere (ray casting); the analytic mask is
+    # restored at the end
+    ctx.nvert_body(surf)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.nvert_body(surf)
+    mark_ms = (time.perf_counter() - t0) / reps * 1e3
     nodes, nn = ctx.ib_search(surf)
     ctx.sync()
     t0 = time.perf_counter()
@@ -318,13 +326,17 @@ def ib_leg(ctx, cfg, spec, s, reps, with_cpu):
         ctx.ib_interpolate(surf, nodes, nn, uc, u, p, ustar)
     ev.record(b)
     interp_ms = ev.elapsed_ms(a, b) / reps
-    out = {"ib_nodes": nn, "elements": len(e), "search_ms": round(search_ms, 4), "interpolate_ms": round(interp_ms, 4)}
+    ctx.nvert_sphere(*W.SPHERE)
+    out = {"ib_nodes": nn, "elements": len(e), "marking_ms": round(mark_ms, 4), "search_ms": round(search_ms, 4),
+           "interpolate_ms": round(interp_ms, 4)}
     if with_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_py import IbBody, Oracle
         o = Oracle(cfg, W.grid(spec))
-        o.nvert_sphere(*W.SPHERE)
         body = IbBody(x, e, n)
+        t0 = time.perf_counter()
+        o.nvert_body(body)
+        out["cpu_oracle_marking_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
         t0 = time.perf_counter()
         onodes = o.ib_search(body)
         out["cpu_oracle_search_ms"] = round((time.perf_counter() - t0) * 1e3, 1)
