@@ -204,7 +204,9 @@ int vfs_ib_search(vfs_ctx *ctx, const vfs_ib_surface *surf, vfs_ib_node *nodes, 
  * the element means count / ustar^2 / p (n_elmt each, may all be NULL), the
  * fluxes of the faces of every nvert == 1 cell from the new ucat, and
  * Contra2Cart.  ucont, ucat, p, ustar: in/out device fields.  Reads see the
- * fields as they were before the call. */
+ * fields as they were before the call.  nodes: as vfs_ib_search writes them;
+ * an entry whose cells or element fall outside the grid or the body is
+ * skipped. */
 int vfs_ib_interpolate(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n, double *ucont,
                        double *ucat, double *p, double *ustar, int *count, double *shear, double *pressure);
 

@@ -434,3 +434,29 @@ def test_gpu_nvert_body_full_size_matches_search_chain():
     assert np.array_equal(got.host(), o.nvert)
     n_nodes = run_pair(cfg, coor, o, (x, e, n, None), ucat, ucont, np.cos(np.arange(o.P) * 0.013))
     print("IB nodes of the triangulated sphere", n_nodes, flush=True)
+
+
+@pytest.mark.gpu
+def test_gpu_ib_interpolate_skips_entries_outside_the_grid():
+    # one record with an element and an interpolation cell out of range is
+    # skipped; the others give what the list without it gives
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C2")
+    x, e, n, _ = sphere_body(3)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
+    surf = gpu_surface(x, e, n)
+    dn, nn = ctx.ib_search(surf)
+    nodes = dn.host()[:nn * V.IB_NODE_DTYPE.itemsize].view(V.IB_NODE_DTYPE).copy()
+    bad = nodes.copy()
+    bad[3]["cell"] = -5
+    bad[3]["i1"] = 10 ** 6
+    keep = np.delete(nodes, 3)
+    outs = []
+    for lst in (bad, keep):
+        d_ucat, d_ucont, d_p, d_ustar = (V.DeviceArray.from_host(a) for a in (ucat, ucont, np.zeros(o.P), np.zeros(o.P)))
+        dl = V.DeviceArray.from_host(lst.view(np.uint8))
+        ctx.ib_interpolate(surf, dl, len(lst), d_ucont, d_ucat, d_p, d_ustar, stats=False)
+        outs.append((d_ucat.host(), d_ucont.host(), d_p.host()))
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)

@@ -346,6 +346,17 @@ __global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, cons
     nodes[l] = nd;
 }
 
+// a node record that fits the grid and the body (entries that do not are
+// skipped, so no kernel reads outside its arrays)
+__device__ __forceinline__ bool node_ok(const Dims &d, int n_elmt, const vfs_ib_node &nd)
+{
+    auto in = [&](int i, int j, int k, int lo) {
+        return i >= lo && i <= d.mx - 1 - lo && j >= lo && j <= d.my - 1 - lo && k >= lo && k <= d.mz - 1 - lo;
+    };
+    return nd.cell >= 0 && nd.cell < n_elmt && in(nd.i, nd.j, nd.k, 1) && in(nd.i1, nd.j1, nd.k1, 0) &&
+           in(nd.i2, nd.j2, nd.k2, 0) && in(nd.i3, nd.j3, nd.k3, 0);
+}
+
 // ibm.c:3390-3530: the node values from the old fields into the staging
 // array (u x|y|z, p in blocks of n); ustar of the node in the field
 __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restrict__ nvert, IbSurf s,
@@ -356,6 +367,7 @@ __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restri
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const vfs_ib_node nd = nodes[l];
+    if (!node_ok(d, s.n_elmt, nd)) return;
     const int ni = nd.cell;
     const long long id = IX(d, nd.k, nd.j, nd.i);
     const long long q1 = IX(d, nd.k1, nd.j1, nd.i1), q2 = IX(d, nd.k2, nd.j2, nd.i2), q3 = IX(d, nd.k3, nd.j3, nd.i3);
@@ -395,13 +407,14 @@ __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restri
     stage[3 * (long long)n + l] = (pf[q1] * sk1 + pf[q2] * sk2 + pf[q3] * sk3);
 }
 
-__global__ void __launch_bounds__(256) k_ib_scatter(Dims d, const vfs_ib_node *__restrict__ nodes, int n,
+__global__ void __launch_bounds__(256) k_ib_scatter(Dims d, int n_elmt, const vfs_ib_node *__restrict__ nodes, int n,
                                                     const double *__restrict__ stage, double *__restrict__ ucat,
                                                     double *__restrict__ pf)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const vfs_ib_node nd = nodes[l];
+    if (!node_ok(d, n_elmt, nd)) return;
     const long long id = IX(d, nd.k, nd.j, nd.i);
     ucat[3 * id] = stage[l];
     ucat[3 * id + 1] = stage[n + l];
@@ -426,10 +439,11 @@ __global__ void __launch_bounds__(STAT_BLOCK) k_ib_stats(Dims d, int n_elmt, con
     for (int l0 = 0; l0 < n; l0 += STAT_BLOCK) {
         const int l = l0 + threadIdx.x;
         if (l < n) {
-            const vfs_ib_node &nd = nodes[l];
-            sc[threadIdx.x] = nd.cell;
-            su[threadIdx.x] = ustar[IX(d, nd.k, nd.j, nd.i)];
-            sp[threadIdx.x] = stage[3 * (long long)n + l];
+            const vfs_ib_node nd = nodes[l];
+            const bool ok = node_ok(d, n_elmt, nd);
+            sc[threadIdx.x] = ok ? nd.cell : -1;
+            su[threadIdx.x] = ok ? ustar[IX(d, nd.k, nd.j, nd.i)] : 0.0;
+            sp[threadIdx.x] = ok ? stage[3 * (long long)n + l] : 0.0;
         }
         __syncthreads();
         const int m = min(STAT_BLOCK, n - l0);
@@ -672,7 +686,8 @@ extern "C" int vfs_ib_interpolate(vfs_ctx *c, const vfs_ib_surface *surf, const 
     if (n > 0) {
         hipLaunchKernelGGL(k_ib_nodes, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->nvert, s, nodes, n, ucat, p,
                            ustar, stage);
-        hipLaunchKernelGGL(k_ib_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, nodes, n, stage, ucat, p);
+        hipLaunchKernelGGL(k_ib_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, s.n_elmt, nodes, n, stage,
+                           ucat, p);
         VFS_CHECK_LAUNCH();
     }
     if (count || shear || pressure) {
