@@ -210,6 +210,46 @@ int vfs_ib_search(vfs_ctx *ctx, const vfs_ib_surface *surf, vfs_ib_node *nodes, 
 int vfs_ib_interpolate(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n, double *ucont,
                        double *ucat, double *p, double *ustar, int *count, double *shear, double *pressure);
 
+/* ---- open k boundaries (bcs.c): an inlet on the k = 0 plane (bctype[4] =
+ * 5) and an outlet on the k = mz-1 plane (bctype[5] = 4) ------------------
+ * The routines need k_periodic = 0; they act on those planes whatever the
+ * context's k bctypes say, so the caller runs them where the reference does
+ * (InflowFlux / OutflowFlux / FormBCS before each solve, implicitsolver.c
+ * 4286-4396).  The other entry points keep treating the k ends as walls.
+ * Fluid faces: nvert < 0.1 (threshold, bcs.c:5).  Sums run in the
+ * reference's j, i order.  Each call synchronizes the stream. */
+typedef struct {
+    int inletprofile;       /* -inlet: 1 or 2 uniform (1, or inlet_flux / inlet_area
+                               when inlet_flux >= 0), 10 the power law of
+                               bcs.c:432-437, 21 1.5 y (2 - y); 0: the caller's
+                               uin_plane (a profile the reference reads from a file) */
+    int reserved;           /* set to 0 */
+    double inlet_flux;      /* -inlet_flux (the reference's default -1: not set) */
+    double inlet_area;      /* from vfs_inlet_area */
+} vfs_inflow_options;
+/* Calc_Inlet_Area (bcs.c:216-328): k_area (mz doubles on the device: the
+ * fluid |zet| of each k plane, k_area[0] = k_area[1]), its mean over
+ * 1..mz-2 and the fluid area of the inlet plane (host doubles). */
+int vfs_inlet_area(vfs_ctx *ctx, double *k_area, double *mean_k_area, double *inlet_area);
+/* InflowFlux (bcs.c:329-706) without levelset or fluctuations: at each face
+ * of the k = 0 plane whose cell above is fluid, U^zeta = uin |zet|, the
+ * other components 0, the ghost velocity 2 u - ucat(k = 1) with u =
+ * Contra2Cart_single of that flux, and ubcs = the ghost velocity; else the
+ * z components 0.  *flux_in (host): the fluid inflow.  coor: the node
+ * coordinates given to vfs_build_metrics; uin_plane: mx*my doubles
+ * (j*mx + i) for inletprofile 0, else NULL.  ucont, ucat, ubcs: device
+ * fields (3P). */
+int vfs_inflow_flux(vfs_ctx *ctx, const vfs_inflow_options *opt, const double *coor, const double *uin_plane,
+                    double *ucont, double *ucat, double *ubcs, double *flux_in);
+/* OutflowFlux (bcs.c:801-870): the fluid flux through the k = mz-2 faces. */
+int vfs_outflow_flux(vfs_ctx *ctx, const double *ucont, double *flux_out);
+/* FormBCS's outlet branch (bcs.c:2285-2418, no levelset): ubcs of the
+ * k = mz-1 plane from the last cells, the fluid flux and area of the
+ * k = mz-2 faces (host), and with scale (outflow_scale) the deficit
+ * flux_in - flux_out added to those faces in proportion to |zet|. */
+int vfs_outlet_correction(vfs_ctx *ctx, double flux_in, int scale, const double *ucat, double *ucont, double *ubcs,
+                          double *flux_out, double *area);
+
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

@@ -3189,3 +3189,152 @@ int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_
             }
     return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Open k boundaries: Calc_Inlet_Area (bcs.c:216-328), InflowFlux at a k = 0  */
+/* inlet (bcs.c:329-706), OutflowFlux at a k = mz-1 outlet (bcs.c:801-870)   */
+/* and FormBCS's outlet branch (bcs.c:2285-2418).  threshold = 0.1 (bcs.c:5) */
+/* ------------------------------------------------------------------------ */
+#define OB_THR 0.1
+
+static double ob_norm(const double *v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+/* rhs.c:34-55 Contra2Cart_single */
+void vo_contra2cart_single(const double *csi, const double *eta, const double *zet, const double *uc, double *u)
+{
+    const double det = csi[0] * (eta[1] * zet[2] - eta[2] * zet[1]) - csi[1] * (eta[0] * zet[2] - eta[2] * zet[0]) +
+                       csi[2] * (eta[0] * zet[1] - eta[1] * zet[0]);
+    const double det0 = uc[0] * (eta[1] * zet[2] - eta[2] * zet[1]) - uc[1] * (csi[1] * zet[2] - csi[2] * zet[1]) +
+                        uc[2] * (csi[1] * eta[2] - csi[2] * eta[1]);
+    const double det1 = -uc[0] * (eta[0] * zet[2] - eta[2] * zet[0]) + uc[1] * (csi[0] * zet[2] - csi[2] * zet[0]) -
+                        uc[2] * (csi[0] * eta[2] - csi[2] * eta[0]);
+    const double det2 = uc[0] * (eta[0] * zet[1] - eta[1] * zet[0]) - uc[1] * (csi[0] * zet[1] - csi[1] * zet[0]) +
+                        uc[2] * (csi[0] * eta[1] - csi[1] * eta[0]);
+    u[0] = det0 / det;
+    u[1] = det1 / det;
+    u[2] = det2 / det;
+}
+
+int vo_inlet_area(const vo_cfg *c, const vo_geom *g, const double *nvert, double *k_area, double *mean_k_area,
+                  double *inlet_area)
+{
+    DIMS;
+    if (c->k_periodic) return -2;
+    for (int k = 0; k < mz; k++) k_area[k] = 0;
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 0; j < my; j++)
+            for (int i = 0; i < mx; i++)
+                if (nvert[IX(k - 1, j, i)] + nvert[IX(k, j, i)] < 0.1 && j >= 1 && j <= my - 2 && i >= 1 && i <= mx - 2)
+                    k_area[k] += ob_norm(&g->zet[3 * IX(k, j, i)]);
+    k_area[0] = k_area[1];
+    double m = 0;
+    for (int k = 1; k <= mz - 2; k++) m += k_area[k];
+    *mean_k_area = m / (double)(mz - 2);
+    double a = 0;
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++)
+            if (nvert[IX(1, j, i)] < OB_THR) a += ob_norm(&g->zet[3 * IX(0, j, i)]);
+    *inlet_area = a;
+    return 0;
+}
+
+/* the inflow speed of InflowFlux's profiles (bcs.c:394-470) */
+static double ob_uin(const vo_inflow_opt *o, double yc, double given)
+{
+    switch (o->inletprofile) {
+    case 1:
+    case 2: return o->inlet_flux < 0 ? 1. : o->inlet_flux / o->inlet_area;
+    case 10: {
+        const double delta = 0.45263, a = 5.99;
+        if (yc >= delta) return 1.0;
+        if (yc <= 0) return 0.0;
+        return pow(yc / delta, 1. / a);
+    }
+    case 21: return 1.5 * 1.0 * yc * (2 - yc);
+    default: return given;
+    }
+}
+
+int vo_inflow_flux(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor, const vo_inflow_opt *o,
+                   const double *uin_plane, double *ucont, double *ucat, double *ubcs, double *flux_in)
+{
+    DIMS;
+    if (c->k_periodic) return -2;
+    const int k = 0;
+#define CO(kk, jj, ii, q) coor[3 * IX(kk, jj, ii) + (q)]
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+            const double yc = (CO(k + 1, j, i, 1) + CO(k + 1, j - 1, i, 1) + CO(k + 1, j, i - 1, 1) + CO(k + 1, j - 1, i - 1, 1)) * 0.25;
+            const size_t id = IX(k, j, i), up = IX(k + 1, j, i);
+            const double area = ob_norm(&g->zet[3 * id]);
+            const double uin = ob_uin(o, yc, uin_plane ? uin_plane[j * mx + i] : 0.);
+            if (nvert[up] < OB_THR) {
+                ucont[3 * id + 2] = uin * area;
+                ucont[3 * id + 0] = 0;
+                ucont[3 * id + 1] = 0;
+                double u[3];
+                vo_contra2cart_single(&g->csi[3 * id], &g->eta[3 * id], &g->zet[3 * id], &ucont[3 * id], u);
+                for (int q = 0; q < 3; q++) ucat[3 * id + q] = -ucat[3 * up + q] + 2 * u[q];
+                for (int q = 0; q < 3; q++) ubcs[3 * id + q] = ucat[3 * id + q];
+            } else {
+                ucat[3 * id + 2] = 0;
+                ubcs[3 * id + 2] = 0;
+                ucont[3 * id + 2] = 0;
+            }
+        }
+#undef CO
+    double f = 0;
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++)
+            if (nvert[IX(k + 1, j, i)] < OB_THR) f += ucont[3 * IX(k, j, i) + 2];
+    *flux_in = f;
+    return 0;
+}
+
+int vo_outflow_flux(const vo_cfg *c, const double *nvert, const double *ucont, double *flux_out)
+{
+    DIMS;
+    if (c->k_periodic) return -2;
+    const int k = mz - 2;
+    double f = 0;
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++)
+            if (nvert[IX(k, j, i)] < OB_THR) f += ucont[3 * IX(k, j, i) + 2];
+    *flux_out = f;
+    return 0;
+}
+
+int vo_outlet_correction(const vo_cfg *c, const vo_geom *g, const double *nvert, double flux_in, int scale,
+                         const double *ucat, double *ucont, double *ubcs, double *flux_out, double *area)
+{
+    DIMS;
+    if (c->k_periodic) return -2;
+    const int k = mz - 1;
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+            const size_t id = IX(k, j, i), in = IX(k - 1, j, i);
+            ubcs[3 * id] = ucat[3 * in];
+            ubcs[3 * id + 1] = ucat[3 * in + 1];
+            if (nvert[in] < OB_THR) ubcs[3 * id + 2] = ucat[3 * in + 2];
+            else ubcs[3 * id + 2] = ucat[3 * in + 2] * 0;
+        }
+    double fo = 0, a = 0;
+    for (int j = 1; j < my - 1; j++)
+        for (int i = 1; i < mx - 1; i++) {
+            const size_t in = IX(k - 1, j, i);
+            if (nvert[in] < OB_THR) {
+                fo += ucont[3 * in + 2];
+                a += ob_norm(&g->zet[3 * in]);
+            }
+        }
+    if (scale)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                const size_t in = IX(k - 1, j, i);
+                const double A = ob_norm(&g->zet[3 * in]);
+                if (nvert[in] < OB_THR) ucont[3 * in + 2] += (flux_in - fo) * A / a;
+            }
+    *flux_out = fo;
+    *area = a;
+    return 0;
+}

@@ -266,6 +266,32 @@ int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_
 /* the first rand() after srand(seed), scaled to [-0.5, 0.5) */
 double vo_ib_ray_jitter(int seed);
 
+/* Open k boundaries (an inlet at k = 0, an outlet at k = mz-1).
+ * inletprofile 1/2: uniform (1, or inlet_flux / inlet_area when inlet_flux
+ * > 0); 10: the power law of bcs.c:432-437; 21: 1.5 y (2 - y); anything
+ * else: the caller's uin_plane[j*mx + i]. */
+typedef struct {
+    int inletprofile;
+    int reserved;
+    double inlet_flux;
+    double inlet_area;
+} vo_inflow_opt;
+/* rhs.c:34-55 */
+void vo_contra2cart_single(const double *csi, const double *eta, const double *zet, const double *uc, double *u);
+/* bcs.c:216-328 Calc_Inlet_Area: k_area[mz], its mean over 1..mz-2, and the
+ * fluid area of the k = 0 plane */
+int vo_inlet_area(const vo_cfg *c, const vo_geom *g, const double *nvert, double *k_area, double *mean_k_area,
+                  double *inlet_area);
+/* bcs.c:329-706 InflowFlux (bctype[4] = 5, no levelset, no fluctuations) */
+int vo_inflow_flux(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor, const vo_inflow_opt *o,
+                   const double *uin_plane, double *ucont, double *ucat, double *ubcs, double *flux_in);
+/* bcs.c:801-870 OutflowFlux (bctype[5] = 4) */
+int vo_outflow_flux(const vo_cfg *c, const double *nvert, const double *ucont, double *flux_out);
+/* bcs.c:2285-2418 FormBCS, outlet (bctype[5] = 4, no levelset); scale is
+ * outflow_scale */
+int vo_outlet_correction(const vo_cfg *c, const vo_geom *g, const double *nvert, double flux_in, int scale,
+                         const double *ucat, double *ucont, double *ubcs, double *flux_out, double *area);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,11 @@ def les_opt(les=2, ti=5, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0, k_
     return VoLesOpt(les, i_homo, j_homo, k_homo, ik, max_cs, ti, tistart, rstart)
 
 
+class VoInflowOpt(C.Structure):
+    _fields_ = [("inletprofile", C.c_int), ("reserved", C.c_int), ("inlet_flux", C.c_double),
+                ("inlet_area", C.c_double)]
+
+
 class VoIbSurface(C.Structure):
     _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
                 ("x_bp", C.POINTER(C.c_double)), ("y_bp", C.POINTER(C.c_double)), ("z_bp", C.POINTER(C.c_double)),
@@ -228,6 +233,30 @@ class Oracle:
         if rc:
             raise RuntimeError(f"vo_ib_search rc={rc}")
         return nodes[:n.value].copy()
+
+    def inlet_area(self):
+        ka = np.zeros(self.mz)
+        m, a = C.c_double(), C.c_double()
+        self._call("vo_inlet_area", C.byref(self.geom), ptr(self.nvert), ptr(ka), C.byref(m), C.byref(a))
+        return ka, m.value, a.value
+
+    def inflow_flux(self, coor, ucont, ucat, ubcs, inletprofile=1, inlet_flux=-1.0, inlet_area=0.0, uin_plane=None):
+        o = VoInflowOpt(int(inletprofile), 0, float(inlet_flux), float(inlet_area))
+        f = C.c_double()
+        self._call("vo_inflow_flux", C.byref(self.geom), ptr(self.nvert), ptr(coor), C.byref(o),
+                   None if uin_plane is None else ptr(uin_plane), ptr(ucont), ptr(ucat), ptr(ubcs), C.byref(f))
+        return f.value
+
+    def outflow_flux(self, ucont):
+        f = C.c_double()
+        self._call("vo_outflow_flux", ptr(self.nvert), ptr(ucont), C.byref(f))
+        return f.value
+
+    def outlet_correction(self, flux_in, ucat, ucont, ubcs, scale=True):
+        f, a = C.c_double(), C.c_double()
+        self._call("vo_outlet_correction", C.byref(self.geom), ptr(self.nvert), C.c_double(flux_in), C.c_int(int(scale)),
+                   ptr(ucat), ptr(ucont), ptr(ubcs), C.byref(f), C.byref(a))
+        return f.value, a.value
 
     def nvert_body(self, body, reset=True):
         """ibm_search_advanced's marking of one body into self.nvert."""

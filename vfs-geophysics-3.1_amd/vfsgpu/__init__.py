@@ -66,6 +66,11 @@ class LesOptions(C.Structure):
                 ("k_homo_filter", C.c_int), ("testfilter_ik", C.c_int), ("reserved", C.c_int), ("max_cs", C.c_double)]
 
 
+class InflowOptions(C.Structure):
+    _fields_ = [("inletprofile", C.c_int), ("reserved", C.c_int), ("inlet_flux", C.c_double),
+                ("inlet_area", C.c_double)]
+
+
 class IbSurfaceC(C.Structure):
     """vfs_ib_surface: device pointers of a triangulated body."""
     _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
@@ -220,6 +225,34 @@ class Ctx:
         _ck(lib().vfs_calc_f_eul(self.h, coor.ptr, int(cent.n // 3), cent.ptr, f_lagr.ptr, dA.ptr, box.ptr,
                                  int(deltafunc), C.c_double(halfwidth), int(width_mode), fx, f_eul.ptr),
             "vfs_calc_f_eul")
+
+    def inlet_area(self):
+        """Calc_Inlet_Area (bcs.c:216-328) -> (k_area DeviceArray of mz,
+        mean_k_area, inlet_area)."""
+        ka = DeviceArray(self.mz)
+        m, a = C.c_double(), C.c_double()
+        _ck(lib().vfs_inlet_area(self.h, ka.ptr, C.byref(m), C.byref(a)), "vfs_inlet_area")
+        return ka, m.value, a.value
+
+    def inflow_flux(self, coor, ucont, ucat, ubcs, inletprofile=1, inlet_flux=-1.0, inlet_area=0.0, uin_plane=None):
+        """InflowFlux (bcs.c:329-706) on the k = 0 plane -> the inflow."""
+        o = InflowOptions(int(inletprofile), 0, float(inlet_flux), float(inlet_area))
+        f = C.c_double()
+        _ck(lib().vfs_inflow_flux(self.h, C.byref(o), coor.ptr, None if uin_plane is None else uin_plane.ptr,
+                                  ucont.ptr, ucat.ptr, ubcs.ptr, C.byref(f)), "vfs_inflow_flux")
+        return f.value
+
+    def outflow_flux(self, ucont):
+        f = C.c_double()
+        _ck(lib().vfs_outflow_flux(self.h, ucont.ptr, C.byref(f)), "vfs_outflow_flux")
+        return f.value
+
+    def outlet_correction(self, flux_in, ucat, ucont, ubcs, scale=True):
+        """FormBCS's outlet branch (bcs.c:2285-2418) -> (flux_out, area)."""
+        f, a = C.c_double(), C.c_double()
+        _ck(lib().vfs_outlet_correction(self.h, C.c_double(flux_in), int(bool(scale)), ucat.ptr, ucont.ptr, ubcs.ptr,
+                                        C.byref(f), C.byref(a)), "vfs_outlet_correction")
+        return f.value, a.value
 
     def nvert_body(self, surface, reset=True):
         """ibm_search_advanced's marking (ibm.c:110-360) of one triangulated
