@@ -76,7 +76,8 @@ static int check_cfg(const vo_cfg *c)
     for (int f = 0; f < 6; f++) {
         int bt = c->bctype[f];
         if (per[f / 2]) { if (bt != 7) return -3; }
-        else if (bt != 1 && !(bt == -1 && f < 4)) return -3;   /* -1: wall function (i/j walls) */
+        else if (bt != 1 && !(bt == -1 && f < 4) && !(bt == 5 && f == 4) && !(bt == 4 && f == 5))
+            return -3;   /* -1: wall function (i/j walls); 5 / 4: a k inlet / outlet */
     }
     if (c->immersed != 0 && c->immersed != 1) return -4;
     if (c->weno != 0 && c->weno != 1) return -4;
@@ -547,6 +548,11 @@ int vo_contra2cart_ustar(const vo_cfg *c, const vo_geom *g, const double *nvert,
                 if (s != (size_t)-1) {
                     VX(ucat, id) = -1 * VX(L, s); VY(ucat, id) = -1 * VY(L, s); VZ(ucat, id) = -1 * VZ(L, s);
                 }
+                /* outlet (rhs.c:646-663): zero behind a solid last cell; the
+                 * inlet ghost is InflowFlux's (rhs.c:636-644 is disabled) */
+                if (c->bctype[5] == 4 && k == mz - 1 && (i != 0 && i != mx - 1 && j != 0 && j != my - 1) &&
+                    nvert[IX(k - 1, j, i)] > 0.1)
+                    VX(ucat, id) = VY(ucat, id) = VZ(ucat, id) = 0;
                 /* 101220 corner zeroing (rhs.c:668-673) */
                 if (i != 0 && i != mx - 1 && j != 0 && j != my - 2 && k != 0 && k != mz - 2) {
                     int z = 0;
@@ -1374,6 +1380,8 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                     double ucon_f = UC(n);
                     double ucon = ucon_f;
                     if (per && (n == 0 || n == m - 2)) ucon = UC(m - 2);
+                    /* outlet: the last face convects with the flux before it (momentum.c:1227-1229) */
+                    if (dir == 2 && n == m - 2 && c->bctype[5] == 4 && (int)NV(k, j, i) == 0) ucon = UC(n - 1);
                     double up = -0.5 * (ucon + fabs(ucon));
                     double um = -0.5 * (ucon - fabs(ucon));
                     double d3v[3];

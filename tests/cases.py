@@ -10,6 +10,8 @@ SMALL = {
     "C2": (32, 24, 20),
     "C1w": (16, 12, 14),
     "C2w": (32, 24, 20),
+    "C1o": (16, 12, 14),
+    "C2o": (32, 24, 20),
 }
 
 

@@ -139,3 +139,84 @@ def test_gpu_open_boundaries(profile, sphere):
     assert g_out == o_out
     for g, h in ((d_uc.host(), uc), (d_b.host(), b)):
         assert np.abs(g - h).max() <= tol * np.abs(h).max()
+
+
+# ---------------------------------------------------------------------------
+# the outlet rules inside the momentum routines
+# ---------------------------------------------------------------------------
+def outlet_case(bct5, sphere_at_outlet=True):
+    N = (16, 12, 14)
+    cfg = dict(mx=N[0] + 2, my=N[1] + 2, mz=N[2] + 2, i_periodic=1, j_periodic=0, k_periodic=0,
+               bctype=[7, 7, 1, 1, 5, bct5], immersed=1, les=0, second_order=0, dpdz_set=1,
+               ren=1000.0, dt=1e-3, st=1.0, mean_pressure_gradient=0.0)
+    coor = V.channel_grid(N[0] + 1, N[1] + 1, N[2] + 1, W.LX, W.LY, W.LZ, 1.5, 0.1)
+    o = Oracle(cfg, coor)
+    if sphere_at_outlet:
+        o.nvert_sphere(W.LX / 2, 1.0, W.LZ - 0.2, 0.45)
+    rng = np.random.default_rng(11)
+    return cfg, coor, o, rng.uniform(-1, 1, 3 * o.P), rng.uniform(-1, 1, 3 * o.P)
+
+
+@pytest.mark.parametrize("bct5", [4, 1])
+def test_outlet_face_convects_with_the_flux_before_it(bct5):
+    # momentum.c:1227-1229: at an outlet the last k-face of a fluid cell takes
+    # its upwind direction from U^zeta of the face before it.  That direction
+    # is used where the cell beyond is masked (the upwind branch), so with
+    # ghost cells marked behind fluid last cells the rhs does not see U^zeta
+    # of those last faces at an outlet, and does behind a wall
+    cfg, coor, o, ucat, ucont = outlet_case(bct5, sphere_at_outlet=False)
+    mz, my, mx = cfg["mz"], cfg["my"], cfg["mx"]
+    nv = o.nvert.reshape(mz, my, mx)
+    nv[mz - 1, 3:7, 2:9] = 2.0     # masked ghosts behind fluid last cells
+    F = np.zeros((mz, my, mx), bool)
+    F[mz - 2, 3:7, 2:9] = True
+    assert np.all(nv[mz - 2][F[mz - 2]] == 0)
+    rhs0 = o.vec()
+    o.formfunction2(ucont.copy(), ucat.copy(), o.sca(), rhs0, 1.0)
+    uc2 = ucont.copy().reshape(mz, my, mx, 3)
+    uc2[..., 2][F] = -uc2[..., 2][F] + 0.37   # flips the upwind direction of most of them
+    rhs1 = o.vec()
+    o.formfunction2(uc2.ravel(), ucat.copy(), o.sca(), rhs1, 1.0)
+    if bct5 == 4:
+        assert np.array_equal(rhs0, rhs1)
+    else:
+        assert np.abs(rhs0 - rhs1).max() > 1e-6
+
+
+def test_outlet_ghost_zero_behind_solid_cells():
+    # rhs.c:646-663: the outlet ghost is zeroed behind a solid last cell and
+    # otherwise left as it was
+    cfg, coor, o, ucat, ucont = outlet_case(4)
+    mz, my, mx = cfg["mz"], cfg["my"], cfg["mx"]
+    before = ucat.copy().reshape(mz, my, mx, 3)
+    u = ucat.copy()
+    o.contra2cart(ucont.copy(), u)
+    u = u.reshape(mz, my, mx, 3)
+    nv = o.nvert.reshape(mz, my, mx)
+    solid = nv[mz - 2, 1:-1, 1:-1] > 0.1
+    assert solid.any() and (~solid).any()
+    ghost = u[mz - 1, 1:-1, 1:-1]
+    assert np.all(ghost[solid] == 0)
+    assert np.array_equal(ghost[~solid], before[mz - 1, 1:-1, 1:-1][~solid])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("bct5", [4, 1])
+def test_gpu_outlet_rules(bct5):
+    # the outlet face rule and the outlet ghost rule against the oracle, on a
+    # body at the outlet plus masked ghosts behind fluid last cells
+    cfg, coor, o, ucat, ucont = outlet_case(bct5)
+    mz, my, mx = cfg["mz"], cfg["my"], cfg["mx"]
+    nv = o.nvert.reshape(mz, my, mx)
+    nv[mz - 1, 3:7, 2:9] = np.where(nv[mz - 2, 3:7, 2:9] == 0, 2.0, nv[mz - 1, 3:7, 2:9])
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
+    d_uc, d_u, d_rhs = V.DeviceArray.from_host(ucont), V.DeviceArray.from_host(ucat), ctx.vec()
+    ctx.formfunction2(d_uc, d_u, ctx.sca(), d_rhs, 1.0)
+    uc, u, rhs = ucont.copy(), ucat.copy(), o.vec()
+    o.formfunction2(uc, u, o.sca(), rhs, 1.0)
+    assert np.array_equal(d_rhs.host(), rhs)
+    ctx.contra2cart(d_uc, d_u)
+    o.contra2cart(uc, u)
+    assert np.array_equal(d_u.host(), u)

@@ -105,8 +105,9 @@ static int check_config(const vfs_config *cfg)
         int bt = cfg->bctype[f];
         if (per[f / 2]) {
             if (bt != 7) { vfs_set_error("periodic faces must carry bctype 7"); return -3; }
-        } else if (bt != 1 && !(bt == -1 && f < 4)) {
-            vfs_set_error("unsupported bctype (supported: 1 no-slip, 7 periodic, -1 wall function on i/j walls)");
+        } else if (bt != 1 && !(bt == -1 && f < 4) && !(bt == 5 && f == 4) && !(bt == 4 && f == 5)) {
+            vfs_set_error("unsupported bctype (supported: 1 no-slip, 7 periodic, -1 wall function on i/j walls, "
+                          "5 inlet at k = 0, 4 outlet at k = mz-1)");
             return -3;
         }
     }

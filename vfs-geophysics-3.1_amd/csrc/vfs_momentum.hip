@@ -62,6 +62,11 @@ __global__ void k_c2c_bc_ghost(Dims d, const double *__restrict__ nvert, const d
         double x = L[3 * s], y = L[3 * s + 1], z = L[3 * s + 2];
         ucat[3 * id] = -1 * x; ucat[3 * id + 1] = -1 * y; ucat[3 * id + 2] = -1 * z;
     }
+    // outlet (rhs.c:646-663): zero behind a solid last cell; the inlet ghost
+    // is InflowFlux's (rhs.c:636-644 is disabled)
+    if (d.bc[5] == 4 && k == d.mz - 1 && (i != 0 && i != d.mx - 1 && j != 0 && j != d.my - 1) &&
+        nvert[IX(d, k - 1, j, i)] > 0.1)
+        ucat[3 * id] = ucat[3 * id + 1] = ucat[3 * id + 2] = 0;
 }
 
 // interior part of the boundary loop: solid cells and the 101220 corner rule

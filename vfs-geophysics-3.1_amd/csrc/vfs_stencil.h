@@ -229,6 +229,8 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     const S ucon_f = UCon(id);
     S ucon = ucon_f;
     if (per && (n == 0 || n == m - 2)) ucon = UCon((Idx)(base + (Idx)(m - 2) * st));
+    // outlet: the last face convects with the flux before it (momentum.c:1227-1229)
+    if (DIR == 2 && n == m - 2 && d.bc[5] == 4 && (int)NB(n) == 0) ucon = UCon((Idx)(id - st));
     const S up = -0.5 * (ucon + fabs(ucon));
     const S um = -0.5 * (ucon - fabs(ucon));
     if (d.immersed && n != m - 2 && NB(n) > 0.1) {

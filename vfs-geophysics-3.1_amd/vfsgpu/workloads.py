@@ -32,6 +32,12 @@ BASE = {
                 bottom=-1),
     "C2w": dict(N=(256, 128, 128), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1002,
                 bottom=-1),
+    # C1 / C2 with an inlet at k = 0 and an outlet at k = mz-1 (bctypes 5, 4)
+    # instead of k periodicity, as the turbine-wake case has
+    "C1o": dict(N=(64, 64, 64), beta=0.0, amp=0.0, ren=5000.0, les=0, immersed=0, field="poiseuille", seed=1001,
+                kopen=1),
+    "C2o": dict(N=(256, 128, 128), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1002,
+                kopen=1),
 }
 CS_LES = 0.01            # Compute_Smagorinsky_Constant_1, les.c:74-77 (Cs = 0.1^2)
 SPHERE = (LX / 2, 1.0, LZ / 2, 0.25)
@@ -44,8 +50,10 @@ def make_config(name, N=None, dt=1e-3):
     if N is not None:
         spec["N"] = tuple(N)
     Ni, Nj, Nk = spec["N"]
-    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=1,
-               bctype=[7, 7, spec.get("bottom", 1), 1, 7, 7], immersed=spec["immersed"], les=spec["les"], second_order=0,
+    kopen = spec.get("kopen", 0)
+    cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=0, k_periodic=0 if kopen else 1,
+               bctype=[7, 7, spec.get("bottom", 1), 1] + ([5, 4] if kopen else [7, 7]), immersed=spec["immersed"],
+               les=spec["les"], second_order=0,
                dpdz_set=1, weno=0, ren=spec["ren"], dt=dt, st=1.0, mean_pressure_gradient=0.0)
     spec["name"] = name
     return cfg, spec
