@@ -347,6 +347,49 @@ def ib_leg(ctx, cfg, spec, s, reps, with_cpu):
     return out
 
 
+def turbine_share_leg(steps, warmup):
+    """One GPU's share of the turbine-wake case (768 x 384 x 384 in 8 k
+    slabs: 768 x 384 x 48), with its boundary set: a log-law inflow at k = 0
+    (InflowFlux with the caller's profile), the outlet at k = mz-1
+    (OutflowFlux and FormBCS's flux correction before each step), a
+    wall-function bottom and LES; the fractional step in mixed precision."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    cfg, spec, ctx, ucat, ucont, nunk = setup("C5s")
+    Ni, Nj, Nk = spec["N"]
+    s = State(ctx, ucat, ucont)
+    coor = W.grid(spec)
+    dcoor = V.DeviceArray.from_host(coor)
+    plane = V.DeviceArray.from_host(W.loglaw_inflow_plane(cfg, coor))
+    ubcs = ctx.vec()
+    _, _, inlet_area = ctx.inlet_area()
+    balance = []
+
+    def step():
+        f_in = ctx.inflow_flux(dcoor, s.uc, s.u, ubcs, inletprofile=0, inlet_area=inlet_area, uin_plane=plane)
+        ctx.outlet_correction(f_in, s.u, s.uc, ubcs)
+        balance.append(abs(ctx.outflow_flux(s.uc) - f_in) / abs(f_in))
+        ctx.contra2cart(s.uc, s.u)
+        return fractional_step(ctx, cfg, s, 1)
+
+    for _ in range(warmup):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    out = [step() for _ in range(steps)]
+    ctx.sync()
+    step_ms = (time.perf_counter() - t0) / steps * 1e3
+    ctx.contra2cart(s.uc, s.u)
+    ctx.eddy_viscosity(s.u, W.CS_LES, s.nut)
+    rhs_ms = time_rhs(ctx, cfg, s, 10)
+    cells = Ni * Nj * Nk
+    return {"workload": f"turbine-wake share {Ni}x{Nj}x{Nk}: log-law inlet, outlet, wall-function bottom, LES",
+            "step_ms_mixed": round(step_ms, 3), "poisson_iters": int(np.median([o[0] for o in out])),
+            "max_div_after_projection": max(o[2] for o in out), "rhs_ms": round(rhs_ms, 4),
+            "rhs_gcells": round(cells / (rhs_ms * 1e-3) / 1e9, 3),
+            "flux_balance_after_outlet_correction": max(balance)}
+
+
 def traffic_from_profile(name):
     """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
@@ -440,6 +483,8 @@ def main():
     ap.add_argument("--no-dynamic", action="store_true", help="skip the dynamic Smagorinsky leg")
     ap.add_argument("--no-actuator", action="store_true", help="skip the actuator-disk leg")
     ap.add_argument("--no-ib", action="store_true", help="skip the CURVIB reconstruction leg")
+    ap.add_argument("--no-turbine-share", action="store_true",
+                    help="skip the 768x384x48 open-boundary leg (one GPU's share of the turbine-wake case)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -515,6 +560,12 @@ def main():
         ib = ib_leg(ctx, cfg, spec, s, 5, not args.no_cpu)
         log(f"[bench] CURVIB reconstruction: {ib}")
 
+    share = None
+    if not args.no_turbine_share and args.config == "C2":
+        del s, ctx
+        share = turbine_share_leg(2, 1)
+        log(f"[bench] turbine-wake share: {share}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -554,6 +605,7 @@ def main():
         "dynamic_les": dyn,
         "actuator_disk": act,
         "ib_reconstruction": ib,
+        "turbine_share": share,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -367,6 +367,9 @@ int vfs_restart_path(char *out, size_t cap, const char *dir, const char *stem, i
 double vfs_wl_uniform(uint64_t seed, uint64_t counter);
 int vfs_wl_channel_grid(int IM, int JM, int KM, double Lx, double Ly, double Lz,
                         double beta, double amp, double *coor_host);
+int vfs_wl_loglaw_k(int mx, int my, int mz, const int per[3], double Ly, double y0, double noise, uint64_t seed,
+                    const double *cent, const double *nvert, const double *icsi, const double *jeta,
+                    const double *kzet, double *ucat, double *ucont);
 int vfs_wl_poiseuille(int mx, int my, int mz, const int per[3], double Ly, double noise,
                       uint64_t seed, const double *cent, const double *nvert,
                       const double *icsi, const double *jeta, const double *kzet,

@@ -12,6 +12,7 @@ SMALL = {
     "C2w": (32, 24, 20),
     "C1o": (16, 12, 14),
     "C2o": (32, 24, 20),
+    "C1wo": (16, 12, 14),
 }
 
 

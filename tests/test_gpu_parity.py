@@ -17,7 +17,7 @@ import vfsgpu as V
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
-CASES = ["C1", "C2", "C1w", "C2w", "C1o", "C2o"]
+CASES = ["C1", "C2", "C1w", "C2w", "C1o", "C2o", "C1wo"]
 
 
 def say(*a):

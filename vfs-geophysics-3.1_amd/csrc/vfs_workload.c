@@ -134,6 +134,30 @@ int vfs_wl_poiseuille(int mx, int my, int mz, const int per[3], double Ly, doubl
     return 0;
 }
 
+/* Log-law streamwise-k profile of the turbine-wake case: w = (1/0.4)
+ * ln(y/y0) u* with u* = 1 / ((1/0.4) ln(Ly/y0)), so that w(Ly) = 1 (y is
+ * clamped to at least y0), plus U[-1,1) noise of amplitude `noise` on all
+ * three components at interior fluid cells; nvert > 0.1 gets zero. */
+int vfs_wl_loglaw_k(int mx, int my, int mz, const int per[3], double Ly, double y0, double noise, uint64_t seed,
+                    const double *cent, const double *nvert, const double *icsi, const double *jeta,
+                    const double *kzet, double *ucat, double *ucont)
+{
+    memset(ucat, 0, 3 * (size_t)mx * my * mz * sizeof(double));
+    const double us = 1.0 / ((1.0 / 0.4) * log(Ly / y0));
+    for (int k = 1; k < mz - 1; k++)
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++) {
+                size_t id = IX(k, j, i);
+                if (nvert && nvert[id] > 0.1) continue;
+                const double y = cent[3 * id + 1] > y0 ? cent[3 * id + 1] : y0;
+                ucat[3 * id] = noise * vfs_wl_uniform(seed, 3 * id + 0);
+                ucat[3 * id + 1] = noise * vfs_wl_uniform(seed, 3 * id + 1);
+                ucat[3 * id + 2] = (1.0 / 0.4) * log(y / y0) * us + noise * vfs_wl_uniform(seed, 3 * id + 2);
+            }
+    faces_from_ucat(mx, my, mz, per, ucat, icsi, jeta, kzet, ucont);
+    return 0;
+}
+
 /* Solenoidal random-Fourier velocity: u = sum_m A_m cos(k_m . x + phi_m) with
  * A_m perpendicular to k_m, |A_m| ~ |k_m|^(-11/6) (energy ~ k^-5/3),
  * integer mode numbers with 1 <= |n| <= nmax (kx = 2 pi nx/Lx, ky = pi ny/Ly,

@@ -457,6 +457,17 @@ def poiseuille(cfg, Ly, noise, seed, cent, nvert, icsi, jeta, kzet):
     return ucat, ucont
 
 
+def loglaw_k(cfg, Ly, y0, noise, seed, cent, nvert, icsi, jeta, kzet):
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    per = (C.c_int * 3)(cfg["i_periodic"], cfg["j_periodic"], cfg["k_periodic"])
+    ucat = np.zeros(3 * mx * my * mz)
+    ucont = np.zeros(3 * mx * my * mz)
+    _ck(lib().vfs_wl_loglaw_k(mx, my, mz, per, C.c_double(Ly), C.c_double(y0), C.c_double(noise), C.c_uint64(seed),
+                              _dp(cent), _dp(nvert), _dp(icsi), _dp(jeta), _dp(kzet), _dp(ucat), _dp(ucont)),
+        "loglaw_k")
+    return ucat, ucont
+
+
 def turbulent(cfg, Lx, Ly, Lz, nmodes, nmax, urms, seed, cent, nvert, icsi, jeta, kzet):
     mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
     per = (C.c_int * 3)(cfg["i_periodic"], cfg["j_periodic"], cfg["k_periodic"])
@@ -510,3 +521,9 @@ def ray_jitter(seed):
     v = C.c_double()
     _ck(lib().vfs_ib_ray_jitter(int(seed), C.byref(v)), "vfs_ib_ray_jitter")
     return v.value
+
+
+def _wl_uniform_array(seed, n):
+    """vfs_wl_uniform(seed, c) for c = 0..n-1 (the library's counter RNG)."""
+    f = lib().vfs_wl_uniform
+    return np.array([f(int(seed), c) for c in range(int(n))])

@@ -17,7 +17,7 @@ import math
 
 import numpy as np
 
-from . import channel_grid, poiseuille, random_scalar, random_vector, turbulent
+from . import channel_grid, loglaw_k, poiseuille, random_scalar, random_vector, turbulent
 
 LX, LY, LZ = 2 * math.pi, 2.0, math.pi
 
@@ -38,7 +38,17 @@ BASE = {
                 kopen=1),
     "C2o": dict(N=(256, 128, 128), beta=2.0, amp=0.1, ren=5000.0, les=1, immersed=1, field="turbulent", seed=1002,
                 kopen=1),
+    # C1 with a wall-function bottom and a k inlet / outlet (the boundary
+    # set of the turbine-wake case, at parity-test size)
+    "C1wo": dict(N=(64, 64, 64), beta=0.0, amp=0.0, ren=5000.0, les=0, immersed=0, field="poiseuille", seed=1001,
+                 bottom=-1, kopen=1),
+    # one GPU's share of the turbine-wake case (768 x 384 x 384 cut into 8
+    # k slabs): wall-function bottom, log-law inflow at k = 0, outlet at
+    # k = mz-1, LES
+    "C5s": dict(N=(768, 384, 48), beta=2.0, amp=0.0, ren=5000.0, les=1, immersed=0, field="loglaw", seed=1005,
+                bottom=-1, kopen=1),
 }
+LOGLAW_Y0 = 1e-3         # roughness height of the turbine-wake inflow
 CS_LES = 0.01            # Compute_Smagorinsky_Constant_1, les.c:74-77 (Cs = 0.1^2)
 SPHERE = (LX / 2, 1.0, LZ / 2, 0.25)
 
@@ -67,6 +77,9 @@ def grid(spec):
 def fields(cfg, spec, metrics, nvert):
     """Velocity (ucat, ucont) for the configuration from host metric arrays
     (dict with cent, icsi, jeta, kzet)."""
+    if spec["field"] == "loglaw":
+        return loglaw_k(cfg, LY, LOGLAW_Y0, 0.05, spec["seed"], metrics["cent"], nvert, metrics["icsi"],
+                        metrics["jeta"], metrics["kzet"])
     if spec["field"] == "poiseuille":
         return poiseuille(cfg, LY, 0.05, spec["seed"], metrics["cent"], nvert, metrics["icsi"], metrics["jeta"],
                           metrics["kzet"])
@@ -119,3 +132,17 @@ def icosphere(cx, cy, cz, r, level):
     e[out] = e[out][:, [0, 2, 1]]
     n[out] = -n[out]
     return x, e, n
+
+
+def loglaw_inflow_plane(cfg, coor, seed=1005, noise=0.05):
+    """Inflow speeds of the k = 0 faces (mx * my, j * mx + i) for the
+    caller-profile inlet: the log law at the face centre's y (the corners
+    InflowFlux averages, bcs.c:386-388) plus U[-1,1) noise."""
+    from . import _wl_uniform_array
+    mx, my = cfg["mx"], cfg["my"]
+    c = coor.reshape(cfg["mz"], my, mx, 3)
+    yc = np.zeros((my, mx))
+    yc[1:-1, 1:-1] = 0.25 * (c[1, 1:-1, 1:-1, 1] + c[1, :-2, 1:-1, 1] + c[1, 1:-1, :-2, 1] + c[1, :-2, :-2, 1])
+    us = 1.0 / ((1.0 / 0.4) * math.log(LY / LOGLAW_Y0))
+    w = (1.0 / 0.4) * np.log(np.maximum(yc, LOGLAW_Y0) / LOGLAW_Y0) * us
+    return (w + noise * _wl_uniform_array(seed + 1, mx * my).reshape(my, mx)).ravel()
