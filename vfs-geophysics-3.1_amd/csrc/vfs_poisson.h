@@ -13,6 +13,7 @@ struct MGLevel {
     long long n;
     int per_i, per_j, per_k;
     int nslot;                         // 19 (level 0) or 27
+    int sk;                            // k cells of the finer level per cell of this one (2, or 1 if k is kept)
     double *A64;  float *A32;          // slot-major coefficients, nslot*n
     // j-line Thomas factors: lower coupling, modified upper coupling, inverse pivot
     double *tl64, *tu64, *ti64;

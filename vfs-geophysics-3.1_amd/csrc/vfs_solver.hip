@@ -27,9 +27,10 @@ struct LV {
     long long n;
     int pi, pj, pk;
     int nslot;
+    int sk;   // k cells of the finer level per cell of this level: 2, or 1 where k is not coarsened
 };
 
-static LV lv_of(const MGLevel &l) { return LV{l.nx, l.ny, l.nz, l.n, l.per_i, l.per_j, l.per_k, l.nslot}; }
+static LV lv_of(const MGLevel &l) { return LV{l.nx, l.ny, l.nz, l.n, l.per_i, l.per_j, l.per_k, l.nslot, l.sk}; }
 
 // neighbour offsets (di, dj, dk) of the 19 slots of poisson.c:951-969; a
 // constexpr table so unrolled loops fold the offsets into immediates
@@ -82,11 +83,11 @@ __global__ void k_galerkin(LV F, LV C, const double *__restrict__ Af, double *Ac
     double acc[27];
 #pragma unroll
     for (int s = 0; s < 27; s++) acc[s] = 0;
-    for (int ck = 0; ck < 2; ck++)
+    for (int ck = 0; ck < C.sk; ck++)
         for (int cj = 0; cj < 2; cj++)
             for (int ci = 0; ci < 2; ci++) {
-                int fk = (F.nz == 1 ? K : 2 * K + ck), fj = (F.ny == 1 ? J : 2 * J + cj), fi = (F.nx == 1 ? I : 2 * I + ci);
-                if ((F.nz == 1 && ck) || (F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
+                int fk = (C.sk == 1 ? K : 2 * K + ck), fj = (F.ny == 1 ? J : 2 * J + cj), fi = (F.nx == 1 ? I : 2 * I + ci);
+                if ((F.ny == 1 && cj) || (F.nx == 1 && ci)) continue;
                 if (fk >= F.nz || fj >= F.ny || fi >= F.nx) continue;
                 long long f = cix(F, fk, fj, fi);
                 for (int s = 0; s < F.nslot; s++) {
@@ -96,7 +97,7 @@ __global__ void k_galerkin(LV F, LV C, const double *__restrict__ Af, double *Ac
                     slot_off(F.nslot, s, di, dj, dk);
                     int nk = fk + dk, nj = fj + dj, ni = fi + di;
                     if (!nb_ok(F, nk, nj, ni)) continue;
-                    int NK = F.nz == 1 ? nk : nk / 2, NJ = F.ny == 1 ? nj : nj / 2, NI = F.nx == 1 ? ni : ni / 2;
+                    int NK = C.sk == 1 ? nk : nk / 2, NJ = F.ny == 1 ? nj : nj / 2, NI = F.nx == 1 ? ni : ni / 2;
                     int oK = NK - K, oJ = NJ - J, oI = NI - I;
                     if (C.pk) { if (oK > 1) oK -= C.nz; if (oK < -1) oK += C.nz; }
                     if (C.pj) { if (oJ > 1) oJ -= C.ny; if (oJ < -1) oJ += C.ny; }
@@ -152,7 +153,7 @@ __device__ __forceinline__ T apply_row(const LV &L, const T *__restrict__ A, con
         long long nb = cix(L, nk, nj, ni);
         T xv = x[nb];
         if (e) {
-            int ck = L.nz == 1 ? nk : nk / 2, cj = L.ny == 1 ? nj : nj / 2, ci = L.nx == 1 ? ni : ni / 2;
+            int ck = C.sk == 1 ? nk : nk / 2, cj = L.ny == 1 ? nj : nj / 2, ci = L.nx == 1 ? ni : ni / 2;
             xv += alpha * e[cix(C, ck, cj, ci)];
         }
         s += a * xv;
@@ -240,7 +241,7 @@ __global__ void k_resid_restrict(LV F, LV C, const T *__restrict__ A, const T *_
                                   gridDim.x * gridDim.y * gridDim.z);
     const int i = (int)(lb % gridDim.x) * blockDim.x + threadIdx.x;
     const int J = (int)((lb / gridDim.x) % gridDim.y), K = (int)(lb / (gridDim.x * gridDim.y));
-    const int nj = F.ny > 1 ? 2 : 1, nk = F.nz > 1 ? 2 : 1;
+    const int nj = F.ny > 1 ? 2 : 1, nk = C.sk;
     T s = 0;
     if (i < F.nx) {
         for (int ck = 0; ck < nk; ck++) {
@@ -273,7 +274,7 @@ __device__ __forceinline__ T apply_row_pe(const LV &L, const LV &C, const T *__r
     const long long sy = L.nx, sz = (long long)L.nx * L.ny, cy = C.nx, cz = (long long)C.nx * C.ny;
     const long long I[3] = {im, i, ip}, J[3] = {jm * sy, j * sy, jp * sy}, K[3] = {km * sz, k * sz, kp * sz};
     const long long CI[3] = {im >> 1, i >> 1, ip >> 1}, CJ[3] = {(jm >> 1) * cy, (j >> 1) * cy, (jp >> 1) * cy},
-                    CK[3] = {(km >> 1) * cz, (k >> 1) * cz, (kp >> 1) * cz};
+                    CK[3] = {(km >> (C.sk - 1)) * cz, (k >> (C.sk - 1)) * cz, (kp >> (C.sk - 1)) * cz};
     T s = 0;
 #pragma unroll
     for (int q = 0; q < NS; q++) {
@@ -293,7 +294,7 @@ __global__ void k_resid_pe(LV L, LV C, const T *__restrict__ A, const T *__restr
 {
     ROW_IDX(L)
     r[t] = b[t] - apply_row_pe<T, NS>(L, C, A, x, e, alpha, t, i, j, k);
-    xe[t] = x[t] + alpha * e[cix(C, k >> 1, j >> 1, i >> 1)];
+    xe[t] = x[t] + alpha * e[cix(C, k >> (C.sk - 1), j >> 1, i >> 1)];
 }
 
 // x += alpha * P e  (piecewise-constant prolongation)
@@ -301,7 +302,7 @@ template <typename T>
 __global__ void k_prolong(LV L, LV C, const T *__restrict__ e, T alpha, T *__restrict__ x)
 {
     ROW_IDX(L)
-    x[t] += alpha * e[cix(C, k >> 1, j >> 1, i >> 1)];
+    x[t] += alpha * e[cix(C, k >> (C.sk - 1), j >> 1, i >> 1)];
 }
 
 // Damped j-line Jacobi update xn = xo + omega * T^{-1} r along every (k,i)
@@ -493,11 +494,11 @@ __global__ void __launch_bounds__(1024) k_small_cycle(SmallLevels<T> S)
         for (long long t = threadIdx.x; t < C.n; t += blockDim.x) {
             const int I = (int)(t % C.nx), J = (int)((t / C.nx) % C.ny), K = (int)(t / ((long long)C.nx * C.ny));
             T s = 0;
-            for (int ck = 0; ck < 2; ck++)
+            for (int ck = 0; ck < C.sk; ck++)
                 for (int cj = 0; cj < 2; cj++)
                     for (int ci = 0; ci < 2; ci++) {
-                        if ((L.nz == 1 && ck) || (L.ny == 1 && cj) || (L.nx == 1 && ci)) continue;
-                        const int fk = L.nz == 1 ? K : 2 * K + ck, fj = L.ny == 1 ? J : 2 * J + cj,
+                        if ((L.ny == 1 && cj) || (L.nx == 1 && ci)) continue;
+                        const int fk = C.sk == 1 ? K : 2 * K + ck, fj = L.ny == 1 ? J : 2 * J + cj,
                                   fi = L.nx == 1 ? I : 2 * I + ci;
                         if (fk >= L.nz || fj >= L.ny || fi >= L.nx) continue;
                         s += S.y[l][cix(L, fk, fj, fi)];
@@ -517,7 +518,7 @@ __global__ void __launch_bounds__(1024) k_small_cycle(SmallLevels<T> S)
         const T *e = S.x[l + 1];
         for (long long t = threadIdx.x; t < L.n; t += blockDim.x) {
             const int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
-            const int ck = L.nz == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
+            const int ck = C.sk == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
             x[t] += S.alpha * e[cix(C, ck, cj, ci)];
         }
         __syncthreads();
@@ -860,6 +861,38 @@ static int invert_dense(std::vector<double> &a, int n)
     return 0;
 }
 
+// mean |coefficient| of the i and of the k neighbours of a level (one
+// workgroup): the couplings that decide whether k is coarsened
+__global__ void __launch_bounds__(256) k_axis_strength(LV L, const double *__restrict__ A, double *out)
+{
+    __shared__ double si[256], sk[256];
+    const int i1 = L.nslot == 19 ? 1 : 14, i2 = L.nslot == 19 ? 2 : 12;
+    const int k1 = L.nslot == 19 ? 5 : 22, k2 = L.nslot == 19 ? 6 : 4;
+    double a = 0, b = 0;
+    for (long long t = threadIdx.x; t < L.n; t += 256) {
+        a += fabs(A[(long long)i1 * L.n + t]) + fabs(A[(long long)i2 * L.n + t]);
+        b += fabs(A[(long long)k1 * L.n + t]) + fabs(A[(long long)k2 * L.n + t]);
+    }
+    si[threadIdx.x] = a;
+    sk[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            si[threadIdx.x] += si[threadIdx.x + w];
+            sk[threadIdx.x] += sk[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = si[0];
+        out[1] = sk[0];
+    }
+}
+
+#ifndef VFS_MG_KEEPK
+#define VFS_MG_KEEPK 0.25   // k kept while its coupling is below this share of the i coupling
+#endif
+
 int vfs_mg_setup(vfs_ctx *c)
 {
     Poisson *ps = c->pois;
@@ -871,6 +904,7 @@ int vfs_mg_setup(vfs_ctx *c)
     l0.n = (long long)l0.nx * l0.ny * l0.nz;
     l0.per_i = d.ip; l0.per_j = d.jp; l0.per_k = d.kp;
     l0.nslot = 19;
+    l0.sk = 1;
     if (alloc_level(c, l0)) return -1;
     hipLaunchKernelGGL(k_gather_level0, dim3(grid_for(l0.n, 256)), dim3(256), 0, c->stream, d, lv_of(l0), ps->coef, l0.A64, l0.A32);
     VFS_CHECK_LAUNCH();
@@ -882,9 +916,22 @@ int vfs_mg_setup(vfs_ctx *c)
         MGLevel &f = ps->lv.back();
         MGLevel cl;
         std::memset(&cl, 0, sizeof(cl));
+        // semi-coarsening: k is kept while its couplings are under a quarter
+        // of the i couplings (cells more than twice as long in k as in i),
+        // where 2x2x2 aggregates would join cells the smoother leaves rough
+        int keep_k = f.nz <= 1;
+        if (!keep_k && f.nx > 1) {
+            hipLaunchKernelGGL(k_axis_strength, dim3(1), dim3(256), 0, c->stream, lv_of(f), f.A64, c->red);
+            VFS_CHECK_LAUNCH();
+            double h[2];
+            VFS_CHECK_HIP(hipMemcpyAsync(h, c->red, sizeof h, hipMemcpyDeviceToHost, c->stream));
+            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+            keep_k = h[1] < VFS_MG_KEEPK * h[0];
+        }
+        cl.sk = keep_k ? 1 : 2;
         cl.nx = f.nx > 1 ? (f.nx + 1) / 2 : 1;
         cl.ny = f.ny > 1 ? (f.ny + 1) / 2 : 1;
-        cl.nz = f.nz > 1 ? (f.nz + 1) / 2 : 1;
+        cl.nz = keep_k ? f.nz : (f.nz + 1) / 2;
         cl.n = (long long)cl.nx * cl.ny * cl.nz;
         cl.per_i = f.per_i; cl.per_j = f.per_j; cl.per_k = f.per_k;
         cl.nslot = 27;
