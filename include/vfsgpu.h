@@ -250,6 +250,19 @@ int vfs_outflow_flux(vfs_ctx *ctx, const double *ucont, double *flux_out);
 int vfs_outlet_correction(vfs_ctx *ctx, double flux_in, int scale, const double *ucat, double *ucont, double *ubcs,
                           double *flux_out, double *area);
 
+/* Options of ibm_interpolation_advanced beyond the defaults of
+ * vfs_ib_interpolate. */
+typedef struct {
+    int moving;             /* movefsi / rotatefsi: the fluid-support guard is off (ibm.c:3448) */
+    int wallfunction;       /* -wallfunction at ti > 0: the node velocity from the rough-wall
+                               law of wall_function_roughness_a (sediment_transport.c:18-96) */
+    double ks;              /* its roughness height (the reference passes 0.0001, ibm.c:3473) */
+} vfs_ib_options;
+/* vfs_ib_interpolate with options (ibm.c:3448-3478). */
+int vfs_ib_interpolate_opt(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
+                           const vfs_ib_options *opt, double *ucont, double *ucat, double *p, double *ustar,
+                           int *count, double *shear, double *pressure);
+
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

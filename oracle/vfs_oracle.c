@@ -2900,9 +2900,55 @@ int vo_ib_search(const vo_cfg *c, const vo_geom *g, const double *nvert, const v
     return 0;
 }
 
+/* sediment_transport.c:18-68: the rough-wall law of the IB wall model */
+static double hy_E(double utau, double ks, double nu)
+{
+    const double kplus = utau * ks / nu, kappa = 0.41, B = 5.5;
+    double dB;
+    if (kplus <= 2.25) dB = 0.0;
+    else if (kplus > 2.25 && kplus < 90.) dB = (B - 8.5 + 1. / kappa * log(kplus)) * sin(0.4258 * (log(kplus) - 0.811));
+    else dB = B - 8.5 + 1. / kappa * log(kplus);
+    return exp(kappa * (B - dB));
+}
+
+double vo_hydset_u(double nu, double y, double utau, double ks)
+{
+    const double y0plus = 11.53, kappa = 0.41, yplus = utau * y / nu;
+    if (yplus <= y0plus) return utau * yplus;
+    return utau / kappa * log(hy_E(utau, ks, nu) * yplus);
+}
+
+static double hy_f(double nu, double u, double y, double utau0, double ks)
+{
+    const double y0plus = 11.53, kappa = 0.41, yplus = utau0 * y / nu;
+    if (yplus <= y0plus) return utau0 * yplus - u;
+    return utau0 * (1. / kappa * log(hy_E(utau0, ks, nu) * yplus)) - u;
+}
+
+double vo_hydset_utau(double nu, double u, double y, double guess, double ks)
+{
+    double utau = guess, utau0 = guess;
+    for (int ii = 0; ii < 30; ii++) {
+        const double eps = 1.e-7;
+        const double df = (hy_f(nu, u, y, utau0 + eps, ks) - hy_f(nu, u, y, utau0 - eps, ks)) / (2. * eps);
+        utau = utau0 - hy_f(nu, u, y, utau0, ks) / df;
+        if (fabs(utau - utau0) < 1.e-7) break;
+        utau0 = utau;
+    }
+    return utau;
+}
+
 int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
                       const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
                       int *count, double *shear, double *pressure)
+{
+    const vo_ib_opt o = {0, 0, 0.0};
+    return vo_ib_interpolate_opt(c, g, nvert, s, nodes, n, &o, ucont, ucat, p, ustar, count, shear, pressure);
+}
+
+int vo_ib_interpolate_opt(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
+                          const vo_ib_node *nodes, int n, const vo_ib_opt *opt, double *ucont, double *ucat, double *p,
+                          double *ustar, int *count, double *shear, double *pressure)
 {
     DIMS;
     int rc = check_cfg(c);
@@ -2928,7 +2974,7 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
             for (int q = 0; q < 3; q++) Ua[q] = u1[q] * cs1 + u2[q] * cs2 + u3[q] * cs3;
         }
         for (int q = 0; q < 3; q++) Uc[q] = (lucat[3 * q1 + q] * sk1 + lucat[3 * q2 + q] * sk2 + lucat[3 * q3 + q] * sk3);
-        if (!c->inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) {
+        if (!opt->moving && !c->inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) {
             Ub[0] = Ub[1] = Ub[2] = 0;
             ustar[id] = 0;
         } else if (nvert[id] > 2.9) {
@@ -2937,8 +2983,17 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
         } else {
             const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
             const double un = u_c * nfx + v_c * nfy + w_c * nfz;
-            const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
-            if (c->inviscid) {   /* freeslip, wallfunction.c:152-166 */
+            double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+            if (opt->wallfunction) {   /* wall_function_roughness_a, sediment_transport.c:70-96 */
+                const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+                ustar[id] = vo_hydset_utau(1. / c->ren, ut_mag, sc, 0.01, opt->ks);
+                const double um = vo_hydset_u(1. / c->ren, sb, ustar[id], opt->ks);
+                if (ut_mag > 1.e-10) ut *= um / ut_mag, vt *= um / ut_mag, wt *= um / ut_mag;
+                else ut = vt = wt = 0;
+                Ub[0] = ut + sb / sc * un * nfx;
+                Ub[1] = vt + sb / sc * un * nfy;
+                Ub[2] = wt + sb / sc * un * nfz;
+            } else if (c->inviscid) {   /* freeslip, wallfunction.c:152-166 */
                 Ub[0] = ut + sb / sc * un * nfx;
                 Ub[1] = vt + sb / sc * un * nfy;
                 Ub[2] = wt + sb / sc * un * nfz;

@@ -258,6 +258,20 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
                       const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
                       int *count, double *shear, double *pressure);
 
+/* options of ibm_interpolation_advanced: moving (movefsi / rotatefsi: no
+ * fluid-support guard), wallfunction (-wallfunction with ti > 0: the
+ * rough-wall law of wall_function_roughness_a with roughness ks) */
+typedef struct {
+    int moving, wallfunction;
+    double ks;
+} vo_ib_opt;
+int vo_ib_interpolate_opt(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
+                          const vo_ib_node *nodes, int n, const vo_ib_opt *opt, double *ucont, double *ucat, double *p,
+                          double *ustar, int *count, double *shear, double *pressure);
+/* sediment_transport.c:29-68: the rough-wall law and its Newton inverse */
+double vo_hydset_u(double nu, double y, double utau, double ks);
+double vo_hydset_utau(double nu, double u, double y, double guess, double ks);
+
 /* ibm_search_advanced's marking (ibm.c:110-360, 508-516): nvert of one
  * triangulated body by ray casting (point_cell_advanced, ibm.c:1068-1183),
  * the 3x3x3 dilation and the relabelling to 3 / 1.  reset: the first body

@@ -39,6 +39,10 @@ def les_opt(les=2, ti=5, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0, k_
     return VoLesOpt(les, i_homo, j_homo, k_homo, ik, max_cs, ti, tistart, rstart)
 
 
+class VoIbOpt(C.Structure):
+    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double)]
+
+
 class VoInflowOpt(C.Structure):
     _fields_ = [("inletprofile", C.c_int), ("reserved", C.c_int), ("inlet_flux", C.c_double),
                 ("inlet_area", C.c_double)]
@@ -85,11 +89,13 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter", "vo_hydset_u", "vo_hydset_utau"]:
             getattr(_lib, name).restype = d
         for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
         _lib.vo_wf_table.restype = C.POINTER(d)
+        _lib.vo_hydset_u.argtypes = [d, d, d, d]
+        _lib.vo_hydset_utau.argtypes = [d, d, d, d, d]
     return _lib
 
 
@@ -263,15 +269,16 @@ class Oracle:
         self._call("vo_nvert_body", C.byref(self.geom), ptr(self.nvert), C.byref(body.c), C.c_int(int(bool(reset))))
         return self.nvert
 
-    def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar):
+    def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar, moving=False, wallfunction=False, ks=1e-4):
         """ibm_interpolation_advanced; ucont, ucat, p, ustar in/out.
         Returns the element (count, mean ustar^2, mean p)."""
         nodes = np.ascontiguousarray(nodes)
         cnt = np.zeros(body.n_elmt, np.int32)
         sh, pr = np.zeros(body.n_elmt), np.zeros(body.n_elmt)
-        self._call("vo_ib_interpolate", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
-                   nodes.ctypes.data_as(C.c_void_p), C.c_int(len(nodes)), ptr(ucont), ptr(ucat), ptr(p), ptr(ustar),
-                   ptr(cnt), ptr(sh), ptr(pr))
+        o = VoIbOpt(int(bool(moving)), int(bool(wallfunction)), float(ks))
+        self._call("vo_ib_interpolate_opt", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
+                   nodes.ctypes.data_as(C.c_void_p), C.c_int(len(nodes)), C.byref(o), ptr(ucont), ptr(ucat), ptr(p),
+                   ptr(ustar), ptr(cnt), ptr(sh), ptr(pr))
         return cnt, sh, pr
 
     def pressure_gradient(self, p, dp):

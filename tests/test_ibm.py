@@ -460,3 +460,84 @@ def test_gpu_ib_interpolate_skips_entries_outside_the_grid():
         outs.append((d_ucat.host(), d_ucont.host(), d_p.host()))
     for a, b in zip(*outs):
         assert np.array_equal(a, b)
+
+
+# ---------------------------------------------------------------------------
+# the IB wall model (wall_function_roughness_a) and moving bodies
+# ---------------------------------------------------------------------------
+def test_rough_wall_law_regimes():
+    # sediment_transport.c:18-68: viscous sublayer u = utau y+ (y+ <= 11.53);
+    # smooth log law u+ = ln(y+)/0.41 + 5.5 (k+ <= 2.25); fully rough
+    # u+ = ln(y/ks)/0.41 + 8.5 (k+ >= 90); the Newton inverse gives utau back
+    hu, ht = lib().vo_hydset_u, lib().vo_hydset_utau
+    nu = 1e-4
+    ut = 0.02
+    y = 5 * nu / ut
+    assert abs(hu(nu, y, ut, 1e-9) - ut * ut * y / nu) < 1e-15
+    y = 1e3 * nu / ut
+    assert abs(hu(nu, y, ut, 1e-9) - ut * (np.log(ut * y / nu) / 0.41 + 5.5)) < 1e-12
+    ks = 200 * nu / ut
+    assert abs(hu(nu, y, ut, ks) - ut * (np.log(y / ks) / 0.41 + 8.5)) < 1e-12
+    for (u, yy, k) in [(0.5, 0.01, 1e-4), (0.05, 0.002, 1e-4), (1.2, 0.05, 1e-3), (0.3, 0.004, 1e-6)]:
+        t = ht(nu, u, yy, 0.01, k)
+        assert abs(hu(nu, yy, t, k) - u) < 1e-6 * u
+
+
+@pytest.mark.parametrize("moving", [False, True])
+def test_plane_body_wall_model_and_moving(moving):
+    cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case()
+    body = IbBody(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    cent = o.g["cent"].reshape(-1, 3)
+    ucont, ustar = o.vec(), o.sca()
+    u = ucat.copy()
+    o.ib_interpolate(body, nodes, ucont, u, p.copy(), ustar, moving=moving, wallfunction=not moving, ks=1e-4)
+    nu = 1.0 / cfg["ren"]
+    hu, ht = lib().vo_hydset_u, lib().vo_hydset_utau
+    checked = 0
+    for nd in nodes:
+        if not (2 <= nd["i"] <= mx - 3 and 2 <= nd["k"] <= mz - 3 and nd["j"] >= 2):
+            continue
+        li = (nd["k"] * my + nd["j"]) * mx + nd["i"]
+        cells = [((nd[f"k{m}"] * my + nd[f"j{m}"]) * mx + nd[f"i{m}"]) for m in (1, 2, 3)]
+        sb = nd["d_s"]
+        sc = sb + nd["d_i"]
+        got = u[3 * li:3 * li + 3]
+        if not moving and sum(o.nvert[cm] for cm in cells) > 0.1:
+            assert np.all(got == 0)
+            continue
+        if moving:   # no guard: the linear field back at every node
+            want = U0 + (GAMMA * TANGENT + BETA * NORMAL) * sb
+        else:        # the law's speed at s_b for the u_tau that gives gamma s_c at s_c
+            t = ht(nu, abs(GAMMA) * sc, sc, 0.01, 1e-4)
+            assert abs(ustar[li] - t) < 1e-15
+            want = U0 + np.sign(GAMMA) * hu(nu, sb, t, 1e-4) * TANGENT + BETA * sb * NORMAL
+        assert np.abs(got - want).max() < 1e-11
+        checked += 1
+    assert checked > 50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("moving,wallfunction", [(True, False), (False, True), (True, True)])
+def test_gpu_ib_options(moving, wallfunction):
+    cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case()
+    body = IbBody(verts, elems, normals, uv)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
+    surf = gpu_surface(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    dn, nn = ctx.ib_search(surf)
+    ucont = np.sin(np.arange(3 * o.P) * 0.11)
+    d_ucat, d_ucont, d_p, d_us = (V.DeviceArray.from_host(a) for a in (ucat, ucont, p, np.zeros(o.P)))
+    st = ctx.ib_interpolate(surf, dn, nn, d_ucont, d_ucat, d_p, d_us, moving=moving, wallfunction=wallfunction,
+                            ks=1e-4)
+    uc, u, pp, us = ucont.copy(), ucat.copy(), p.copy(), np.zeros(o.P)
+    cnt, sh, pr = o.ib_interpolate(body, nodes, uc, u, pp, us, moving=moving, wallfunction=wallfunction, ks=1e-4)
+    # the wall law's log / exp / sin: the device's and the C library's
+    tol = 1e-12 if wallfunction else 0.0
+    for g, h in ((d_ucat.host(), u), (d_ucont.host(), uc), (d_p.host(), pp), (d_us.host(), us)):
+        assert np.abs(g - h).max() <= tol * max(1.0, np.abs(h).max())
+    assert np.array_equal(st[0].host(), cnt)
+    assert np.abs(st[1].host() - sh).max() <= tol * max(1.0, np.abs(sh).max())

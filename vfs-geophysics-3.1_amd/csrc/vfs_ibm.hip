@@ -346,6 +346,45 @@ __global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, cons
     nodes[l] = nd;
 }
 
+// sediment_transport.c:18-68: the rough-wall law of the IB wall model
+__device__ double hy_E(double utau, double ks, double nu)
+{
+    const double kplus = utau * ks / nu, kappa = 0.41, B = 5.5;
+    double dB;
+    if (kplus <= 2.25) dB = 0.0;
+    else if (kplus > 2.25 && kplus < 90.) dB = (B - 8.5 + 1. / kappa * log(kplus)) * sin(0.4258 * (log(kplus) - 0.811));
+    else dB = B - 8.5 + 1. / kappa * log(kplus);
+    return exp(kappa * (B - dB));
+}
+
+__device__ double hy_u(double nu, double y, double utau, double ks)
+{
+    const double y0plus = 11.53, kappa = 0.41, yplus = utau * y / nu;
+    if (yplus <= y0plus) return utau * yplus;
+    return utau / kappa * log(hy_E(utau, ks, nu) * yplus);
+}
+
+__device__ double hy_f(double nu, double u, double y, double utau0, double ks)
+{
+    const double y0plus = 11.53, kappa = 0.41, yplus = utau0 * y / nu;
+    if (yplus <= y0plus) return utau0 * yplus - u;
+    return utau0 * (1. / kappa * log(hy_E(utau0, ks, nu) * yplus)) - u;
+}
+
+// find_utau_hydset: Newton with a central-difference slope, 30 steps at most
+__device__ double hy_utau(double nu, double u, double y, double guess, double ks)
+{
+    double utau = guess, utau0 = guess;
+    for (int ii = 0; ii < 30; ii++) {
+        const double eps = 1.e-7;
+        const double df = (hy_f(nu, u, y, utau0 + eps, ks) - hy_f(nu, u, y, utau0 - eps, ks)) / (2. * eps);
+        utau = utau0 - hy_f(nu, u, y, utau0, ks) / df;
+        if (fabs(utau - utau0) < 1.e-7) break;
+        utau0 = utau;
+    }
+    return utau;
+}
+
 // a node record that fits the grid and the body (entries that do not are
 // skipped, so no kernel reads outside its arrays)
 __device__ __forceinline__ bool node_ok(const Dims &d, int n_elmt, const vfs_ib_node &nd)
@@ -362,7 +401,8 @@ __device__ __forceinline__ bool node_ok(const Dims &d, int n_elmt, const vfs_ib_
 __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restrict__ nvert, IbSurf s,
                                                   const vfs_ib_node *__restrict__ nodes, int n,
                                                   const double *__restrict__ ucat, const double *__restrict__ pf,
-                                                  double *__restrict__ ustar, double *__restrict__ stage)
+                                                  vfs_ib_options opt, double *__restrict__ ustar,
+                                                  double *__restrict__ stage)
 {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
@@ -381,14 +421,24 @@ __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restri
         for (int q = 0; q < 3; q++) Ua[q] = u1[q] * nd.cs1 + u2[q] * nd.cs2 + u3[q] * nd.cs3;
     }
     for (int q = 0; q < 3; q++) Uc[q] = (ucat[3 * q1 + q] * sk1 + ucat[3 * q2 + q] * sk2 + ucat[3 * q3 + q] * sk3);
-    if ((!d.inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) || nvert[id] > 2.9) {
+    if ((!opt.moving && !d.inviscid && nvert[q1] + nvert[q2] + nvert[q3] > 0.1) || nvert[id] > 2.9) {
         Ub[0] = Ub[1] = Ub[2] = 0;
         ustar[id] = 0;
     } else {
         const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
         const double un = u_c * nfx + v_c * nfy + w_c * nfz;
-        const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
-        if (d.inviscid) {   // freeslip, wallfunction.c:152-166
+        double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+        if (opt.wallfunction) {   // wall_function_roughness_a, sediment_transport.c:70-96
+            const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+            const double us = hy_utau(1. / d.ren, ut_mag, sc, 0.01, opt.ks);
+            ustar[id] = us;
+            const double um = hy_u(1. / d.ren, sb, us, opt.ks);
+            if (ut_mag > 1.e-10) ut *= um / ut_mag, vt *= um / ut_mag, wt *= um / ut_mag;
+            else ut = vt = wt = 0;
+            Ub[0] = ut + sb / sc * un * nfx;
+            Ub[1] = vt + sb / sc * un * nfy;
+            Ub[2] = wt + sb / sc * un * nfz;
+        } else if (d.inviscid) {   // freeslip, wallfunction.c:152-166
             Ub[0] = ut + sb / sc * un * nfx;
             Ub[1] = vt + sb / sc * un * nfy;
             Ub[2] = wt + sb / sc * un * nfz;
@@ -677,7 +727,18 @@ extern "C" int vfs_ib_interpolate(vfs_ctx *c, const vfs_ib_surface *surf, const 
                                   double *ucont, double *ucat, double *p, double *ustar, int *count, double *shear,
                                   double *pressure)
 {
+    const vfs_ib_options o = {0, 0, 0.0};
+    return vfs_ib_interpolate_opt(c, surf, nodes, n, &o, ucont, ucat, p, ustar, count, shear, pressure);
+}
+
+extern "C" int vfs_ib_interpolate_opt(vfs_ctx *c, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
+                                      const vfs_ib_options *opt, double *ucont, double *ucat, double *p, double *ustar,
+                                      int *count, double *shear, double *pressure)
+{
     const Dims d = c->d;
+    VFS_REQUIRE(opt && (opt->moving == 0 || opt->moving == 1) && (opt->wallfunction == 0 || opt->wallfunction == 1),
+                "bad IB options");
+    VFS_REQUIRE(!opt->wallfunction || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
     if (int rc = check_surface(surf)) return rc;
     VFS_REQUIRE(n >= 0 && (n == 0 || nodes) && ucont && ucat && p && ustar, "bad arguments");
     VFS_REQUIRE(4 * (long long)n <= 3 * d.P, "more IB nodes than cells");
@@ -685,7 +746,7 @@ extern "C" int vfs_ib_interpolate(vfs_ctx *c, const vfs_ib_surface *surf, const 
     double *stage = c->tmp3;   // 4n doubles
     if (n > 0) {
         hipLaunchKernelGGL(k_ib_nodes, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->nvert, s, nodes, n, ucat, p,
-                           ustar, stage);
+                           *opt, ustar, stage);
         hipLaunchKernelGGL(k_ib_scatter, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, s.n_elmt, nodes, n, stage,
                            ucat, p);
         VFS_CHECK_LAUNCH();

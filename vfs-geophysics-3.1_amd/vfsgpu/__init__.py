@@ -71,6 +71,10 @@ class InflowOptions(C.Structure):
                 ("inlet_area", C.c_double)]
 
 
+class IbOptions(C.Structure):
+    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double)]
+
+
 class IbSurfaceC(C.Structure):
     """vfs_ib_surface: device pointers of a triangulated body."""
     _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
@@ -270,15 +274,17 @@ class Ctx:
         _ck(lib().vfs_ib_search(self.h, C.byref(surface.c), nodes.ptr, C.c_int(cap), C.byref(n)), "vfs_ib_search")
         return nodes, n.value
 
-    def ib_interpolate(self, surface, nodes, n, ucont, ucat, p, ustar, stats=True):
+    def ib_interpolate(self, surface, nodes, n, ucont, ucat, p, ustar, stats=True, moving=False, wallfunction=False,
+                       ks=1e-4):
         """ibm_interpolation_advanced (ibm.c:3294-3698): ucont, ucat, p, ustar
         in/out.  Returns the element (count, mean ustar^2, mean p) as
         DeviceArrays, or None with stats=False."""
         ne = surface.n_elmt
         st = (DeviceArray(ne, np.int32), DeviceArray(ne), DeviceArray(ne)) if stats else (None, None, None)
-        _ck(lib().vfs_ib_interpolate(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), ucont.ptr, ucat.ptr,
-                                     p.ptr, ustar.ptr, *[None if a is None else a.ptr for a in st]),
-            "vfs_ib_interpolate")
+        o = IbOptions(int(bool(moving)), int(bool(wallfunction)), float(ks))
+        _ck(lib().vfs_ib_interpolate_opt(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), C.byref(o), ucont.ptr,
+                                         ucat.ptr, p.ptr, ustar.ptr, *[None if a is None else a.ptr for a in st]),
+            "vfs_ib_interpolate_opt")
         return st if stats else None
 
     def set_body_force(self, f_eul):
