@@ -180,6 +180,11 @@ typedef struct {
  * tries.  The context's mask is set; nvert_out (P doubles) may be NULL.
  * Synchronizes the stream. */
 int vfs_nvert_body(vfs_ctx *ctx, const vfs_ib_surface *surf, int reset, double *nvert_out);
+/* The same with -thin (ibm.c:281-317, point_cell_thin ibm.c:1186-1244): for
+ * a body of zero thickness (a plate, a blade), a cell whose segments to its
+ * +i, +j, +k and +ijk neighbours cut an element marks the cells of that
+ * 2x2x2 box below 0.5 with 2 (IB nodes after the relabelling). */
+int vfs_nvert_body_thin(vfs_ctx *ctx, const vfs_ib_surface *surf, int reset, int thin, double *nvert_out);
 /* The ray jitter of randomdirection: the first rand() after srand(seed) of
  * the C library, / (RAND_MAX + 1) - 0.5 (host only; the caller's rand()
  * sequence is left alone). */

@@ -3199,7 +3199,55 @@ static int ib_point_cell(const vo_ib_surface *s, const ib_bins *b, const double 
     return -1;
 }
 
+/* compgeom.c:301-370 ISLineTriangleIntp: does segment p1p2 cut element e */
+static int ib_line_cuts(const vo_ib_surface *s, const double p1[3], const double p2[3], int e)
+{
+    const int n1 = s->nv1[e], n2 = s->nv2[e], n3 = s->nv3[e];
+    const double xf1 = s->x_bp[n1], yf1 = s->y_bp[n1], zf1 = s->z_bp[n1];
+    const double xf2 = s->x_bp[n2], yf2 = s->y_bp[n2], zf2 = s->z_bp[n2];
+    const double xf3 = s->x_bp[n3], yf3 = s->y_bp[n3], zf3 = s->z_bp[n3];
+    const double nfx = s->nf_x[e], nfy = s->nf_y[e], nfz = s->nf_z[e];
+    const double tf1 = (p1[0] - xf1) * nfx + (p1[1] - yf1) * nfy + (p1[2] - zf1) * nfz;
+    const double tf2 = (p2[0] - xf1) * nfx + (p2[1] - yf1) * nfy + (p2[2] - zf1) * nfz;
+    if (!(tf1 * tf2 < 0)) return 0;
+    const double dx1 = p1[0] - xf1, dy1 = p1[1] - yf1, dz1 = p1[2] - zf1;
+    const double nftx = p2[0] - p1[0], nfty = p2[1] - p1[1], nftz = p2[2] - p1[2];
+    const double dx2 = xf2 - xf1, dy2 = yf2 - yf1, dz2 = zf2 - zf1;
+    const double dx3 = xf3 - xf1, dy3 = yf3 - yf1, dz3 = zf3 - zf1;
+    const double d = -(dx1 * (dy2 * dz3 - dz2 * dy3) - dy1 * (dx2 * dz3 - dz2 * dx3) + dz1 * (dx2 * dy3 - dy2 * dx3)) /
+                     (nftx * (dy2 * dz3 - dz2 * dy3) - nfty * (dx2 * dz3 - dz2 * dx3) + nftz * (dx2 * dy3 - dy2 * dx3));
+    const double pint[3] = {p1[0] + d * nftx, p1[1] + d * nfty, p1[2] + d * nftz};
+    if (!(d > 0 && d < 1.)) return 0;
+    const double pa[3] = {xf1, yf1, zf1}, pb[3] = {xf2, yf2, zf2}, pc[3] = {xf3, yf3, zf3};
+    return ib_point_in_triangle(pint, pa, pb, pc, nfx, nfy, nfz) == 1;
+}
+
+/* ibm.c:1186-1244 point_cell_thin: does a segment from p to p1..p4 cut an
+ * element listed in the bins around (ip, jp, kp) */
+static int ib_point_cell_thin(const vo_ib_surface *s, const ib_bins *b, const double *p, const double *const q[4],
+                              int ip, int jp, int kp, char *searched)
+{
+    memset(searched, 0, s->n_elmt);
+    const int ks = kp - 1 > 0 ? kp - 1 : 0, js = jp - 1 > 0 ? jp - 1 : 0, is = ip - 1 > 0 ? ip - 1 : 0;
+    for (int k = ks; k < kp + 2 && k < IB_NC; k++)
+        for (int j = js; j < jp + 2 && j < IB_NC; j++)
+            for (int i = is; i < ip + 2 && i < IB_NC; i++)
+                for (int m = b->head[(k * IB_NC + j) * IB_NC + i]; m >= 0; m = b->next[m]) {
+                    const int e = b->node[m];
+                    if (searched[e]) continue;
+                    searched[e] = 1;
+                    for (int w = 0; w < 4; w++)
+                        if (ib_line_cuts(s, p, q[w], e)) return 2;
+                }
+    return 0;
+}
+
 int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset)
+{
+    return vo_nvert_body_thin(c, g, nvert, s, reset, 0);
+}
+
+int vo_nvert_body_thin(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset, int thin)
 {
     DIMS;
     int rc = check_cfg(c);
@@ -3223,6 +3271,32 @@ int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_
                 nvert[id] = nvert[id] > val ? nvert[id] : val;
                 if (nvert[id] < 0) nvert[id] = 0;
             }
+    /* thin bodies (ibm.c:281-317): a segment from a cell centre to its +i,
+     * +j, +k and +ijk neighbours that cuts an element marks the eight cells
+     * of that box with 2 where they are below 0.5 */
+    if (!rc && thin)
+        for (int k = 1; k < mz - 1; k++)
+            for (int j = 1; j < my - 1; j++)
+                for (int i = 1; i < mx - 1; i++) {
+                    const double *p = &g->cent[3 * IX(k, j, i)];
+                    if (!(p[0] > b.xmin && p[0] < b.xmax && p[1] > b.ymin && p[1] < b.ymax && p[2] > b.zmin &&
+                          p[2] < b.zmax))
+                        continue;
+                    int open = 0;
+                    for (int dk = 0; dk < 2; dk++)
+                        for (int dj = 0; dj < 2; dj++)
+                            for (int di = 0; di < 2; di++) open |= nvert[IX(k + dk, j + dj, i + di)] < 0.5;
+                    if (!open) continue;
+                    const int ic = (int)floor((p[0] - b.xmin) / b.dcx), jc = (int)floor((p[1] - b.ymin) / b.dcy),
+                              kc = (int)floor((p[2] - b.zmin) / b.dcz);
+                    const double *q[4] = {&g->cent[3 * IX(k, j, i + 1)], &g->cent[3 * IX(k, j + 1, i)],
+                                          &g->cent[3 * IX(k + 1, j, i)], &g->cent[3 * IX(k + 1, j + 1, i + 1)]};
+                    if (!ib_point_cell_thin(s, &b, p, q, ic, jc, kc, searched)) continue;
+                    for (int dk = 0; dk < 2; dk++)
+                        for (int dj = 0; dj < 2; dj++)
+                            for (int di = 0; di < 2; di++)
+                                if (nvert[IX(k + dk, j + dj, i + di)] < 0.5) nvert[IX(k + dk, j + dj, i + di)] = 2.;
+                }
     free(searched);
     free(b.head); free(b.next); free(b.node);
     if (rc) return rc;

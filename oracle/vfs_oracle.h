@@ -277,6 +277,9 @@ double vo_hydset_utau(double nu, double u, double y, double guess, double ks);
  * the 3x3x3 dilation and the relabelling to 3 / 1.  reset: the first body
  * of a search (nvert < 5 cleared first).  -3: a ray never came clear. */
 int vo_nvert_body(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset);
+/* the same with -thin (ibm.c:281-317): cells around elements the box
+ * diagonals of the cell centres cut are marked too */
+int vo_nvert_body_thin(const vo_cfg *c, const vo_geom *g, double *nvert, const vo_ib_surface *s, int reset, int thin);
 /* the first rand() after srand(seed), scaled to [-0.5, 0.5) */
 double vo_ib_ray_jitter(int seed);
 

@@ -264,9 +264,10 @@ class Oracle:
                    ptr(ucat), ptr(ucont), ptr(ubcs), C.byref(f), C.byref(a))
         return f.value, a.value
 
-    def nvert_body(self, body, reset=True):
+    def nvert_body(self, body, reset=True, thin=False):
         """ibm_search_advanced's marking of one body into self.nvert."""
-        self._call("vo_nvert_body", C.byref(self.geom), ptr(self.nvert), C.byref(body.c), C.c_int(int(bool(reset))))
+        self._call("vo_nvert_body_thin", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
+                   C.c_int(int(bool(reset))), C.c_int(int(bool(thin))))
         return self.nvert
 
     def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar, moving=False, wallfunction=False, ks=1e-4):

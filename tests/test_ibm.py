@@ -541,3 +541,89 @@ def test_gpu_ib_options(moving, wallfunction):
         assert np.abs(g - h).max() <= tol * max(1.0, np.abs(h).max())
     assert np.array_equal(st[0].host(), cnt)
     assert np.abs(st[1].host() - sh).max() <= tol * max(1.0, np.abs(sh).max())
+
+
+# ---------------------------------------------------------------------------
+# thin bodies (-thin)
+# ---------------------------------------------------------------------------
+def thin_plate(n=12):
+    """A double-sided plate z = 0.4 + 0.35 x over the whole domain (each
+    triangle twice with opposite normals, so rays through it count two
+    crossings).  Its box spans the domain, so the 40^3 element bins are
+    about as large as the cells, which the bin neighbourhood of
+    point_cell_thin needs."""
+    xs, ys = np.linspace(-0.2, W.LX + 0.2, n + 1), np.linspace(0.2, 1.8, n + 1)
+    verts = np.array([(x, y, 0.4 + 0.35 * x) for y in ys for x in xs])
+    el = []
+    for b in range(n):
+        for a in range(n):
+            v = b * (n + 1) + a
+            el += [(v, v + 1, v + n + 2), (v, v + n + 2, v + n + 1)]
+    el = np.array(el, np.int32)
+    nrm = np.array([-0.35, 0.0, 1.0]) / np.hypot(0.35, 1.0)
+    elems = np.vstack([el, el[:, [0, 2, 1]]])
+    normals = np.vstack([np.tile(nrm, (len(el), 1)), np.tile(-nrm, (len(el), 1))])
+    # orient each copy so its normal follows the right-hand rule
+    x = verts
+    c = np.cross(x[elems[:, 1]] - x[elems[:, 0]], x[elems[:, 2]] - x[elems[:, 0]])
+    flip = np.einsum("ij,ij->i", c, normals) < 0
+    elems[flip] = elems[flip][:, [0, 2, 1]]
+    return verts, elems, normals
+
+
+def plate_case():
+    N = (24, 20, 18)
+    cfg = dict(mx=N[0] + 2, my=N[1] + 2, mz=N[2] + 2, i_periodic=1, j_periodic=0, k_periodic=1,
+               bctype=[7, 7, 1, 1, 7, 7], immersed=1, les=0, second_order=0, dpdz_set=1, ren=400.0, dt=1e-3, st=1.0,
+               mean_pressure_gradient=0.0)
+    coor = V.channel_grid(N[0] + 1, N[1] + 1, N[2] + 1, W.LX, W.LY, W.LZ, 0.0, 0.0)
+    return cfg, coor, thin_plate()
+
+
+def test_thin_plate_marks_only_cells_at_the_plate():
+    # every IB node of the -thin marking lies in the 2x2x2 box of a cell one of
+    # whose four segments (+i, +j, +k, +ijk) crosses the plate inside it
+    cfg, coor, (verts, elems, normals) = plate_case()
+    o = Oracle(cfg, coor)
+    o.nvert_body(IbBody(verts, elems, normals), thin=True)
+    mz, my, mx = cfg["mz"], cfg["my"], cfg["mx"]
+    nv = np.rint(o.nvert.reshape(mz, my, mx))
+    assert not (nv == 3).any()                  # double-sided: nothing inside
+    cent = o.g["cent"].reshape(mz, my, mx, 3)
+    f = lambda p: p[..., 2] - 0.4 - 0.35 * p[..., 0]
+    cross = np.zeros((mz, my, mx), bool)
+    for di, dj, dk in ((1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 1)):
+        a = cent[1:-1, 1:-1, 1:-1]
+        b = cent[1 + dk:mz - 1 + dk, 1 + dj:my - 1 + dj, 1 + di:mx - 1 + di]
+        fa, fb = f(a), f(b)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            t = fa / (fa - fb)
+        pt = a + t[..., None] * (b - a)
+        hit = (fa * fb < 0) & (pt[..., 1] > 0.2) & (pt[..., 1] < 1.8)
+        cross[1:-1, 1:-1, 1:-1] |= hit
+    allowed = np.zeros_like(cross)
+    for dk in (0, 1):
+        for dj in (0, 1):
+            for di in (0, 1):
+                allowed[dk:, dj:, di:] |= cross[:mz - dk, :my - dj, :mx - di]
+    marked = nv == 1
+    assert marked.sum() > 100
+    assert not (marked & ~allowed).any()
+    # without -thin a double-sided plate leaves the mask empty
+    o2 = Oracle(cfg, coor)
+    o2.nvert_body(IbBody(verts, elems, normals), thin=False)
+    assert np.all(o2.nvert == 0)
+
+
+@pytest.mark.gpu
+def test_gpu_nvert_thin_body():
+    cfg, coor, (verts, elems, normals) = plate_case()
+    o = Oracle(cfg, coor)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    x, e, n = W.icosphere(W.LX / 2, 1.0, 2.2, 0.3, 3)
+    for q, (vv, ee, nn, thin) in enumerate([(x, e, n, False), (verts, elems, normals, True)]):
+        o.nvert_body(IbBody(vv, ee, nn), reset=(q == 0), thin=thin)
+        got = ctx.nvert_body(V.IbSurface(vv, ee, nn), reset=(q == 0), thin=thin)
+    assert np.array_equal(got.host(), o.nvert)
+    assert (np.rint(o.nvert) == 1).sum() > 50 and (np.rint(o.nvert) == 3).sum() > 0

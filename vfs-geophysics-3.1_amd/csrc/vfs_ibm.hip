@@ -656,6 +656,86 @@ __global__ void __launch_bounds__(256) k_body_inside(Dims d, const double *__res
     nvert[t] = nv;
 }
 
+// compgeom.c:301-370 ISLineTriangleIntp: does segment p1p2 cut element e
+__device__ bool line_cuts(const IbSurf &s, const double p1[3], const double p2[3], int e)
+{
+    double f1[3], f2[3], f3[3];
+    vertex(s, s.nv1[e], f1);
+    vertex(s, s.nv2[e], f2);
+    vertex(s, s.nv3[e], f3);
+    const double nfx = s.nfx[e], nfy = s.nfy[e], nfz = s.nfz[e];
+    const double tf1 = (p1[0] - f1[0]) * nfx + (p1[1] - f1[1]) * nfy + (p1[2] - f1[2]) * nfz;
+    const double tf2 = (p2[0] - f1[0]) * nfx + (p2[1] - f1[1]) * nfy + (p2[2] - f1[2]) * nfz;
+    if (!(tf1 * tf2 < 0)) return false;
+    const double dx1 = p1[0] - f1[0], dy1 = p1[1] - f1[1], dz1 = p1[2] - f1[2];
+    const double nftx = p2[0] - p1[0], nfty = p2[1] - p1[1], nftz = p2[2] - p1[2];
+    const double dx2 = f2[0] - f1[0], dy2 = f2[1] - f1[1], dz2 = f2[2] - f1[2];
+    const double dx3 = f3[0] - f1[0], dy3 = f3[1] - f1[1], dz3 = f3[2] - f1[2];
+    const double d = -(dx1 * (dy2 * dz3 - dz2 * dy3) - dy1 * (dx2 * dz3 - dz2 * dx3) + dz1 * (dx2 * dy3 - dy2 * dx3)) /
+                     (nftx * (dy2 * dz3 - dz2 * dy3) - nfty * (dx2 * dz3 - dz2 * dx3) + nftz * (dx2 * dy3 - dy2 * dx3));
+    const double pint[3] = {p1[0] + d * nftx, p1[1] + d * nfty, p1[2] + d * nftz};
+    if (!(d > 0 && d < 1.)) return false;
+    return point_in_triangle(pint, f1, f2, f3, nfx, nfy, nfz) == 1;
+}
+
+// -thin (ibm.c:281-317), first pass: whether a segment from the cell centre
+// to its +i, +j, +k and +ijk neighbours cuts an element of the bins around
+// it (point_cell_thin, ibm.c:1186-1244), for cells in the box whose 2x2x2
+// box still has a cell below 0.5.  The marks go in a second pass: a cell that
+// the sequential loop would skip has only cells already marked in its box,
+// so marking from the unchanged mask gives the same result.
+__global__ void __launch_bounds__(256) k_thin_cut(Dims d, const double *__restrict__ cent, IbSurf s,
+                                                  const int *__restrict__ bins, BodyBox b,
+                                                  const double *__restrict__ nvert, int *__restrict__ cut)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    int c = 0;
+    const double p[3] = {cent[3 * t], cent[3 * t + 1], cent[3 * t + 2]};
+    if (i >= 1 && i <= d.mx - 2 && j >= 1 && j <= d.my - 2 && k >= 1 && k <= d.mz - 2 && p[0] > b.xmin &&
+        p[0] < b.xmax && p[1] > b.ymin && p[1] < b.ymax && p[2] > b.zmin && p[2] < b.zmax) {
+        bool open = false;
+        for (int dk = 0; dk < 2; dk++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int di = 0; di < 2; di++) open |= nvert[IX(d, k + dk, j + dj, i + di)] < 0.5;
+        if (open) {
+            const int ic = (int)floor((p[0] - b.xmin) / b.dcx), jc = (int)floor((p[1] - b.ymin) / b.dcy),
+                      kc = (int)floor((p[2] - b.zmin) / b.dcz);
+            const int is = ic - 1 > 0 ? ic - 1 : 0, js = jc - 1 > 0 ? jc - 1 : 0, ks = kc - 1 > 0 ? kc - 1 : 0;
+            const int ie = ic + 2 < NBIN ? ic + 2 : NBIN, je = jc + 2 < NBIN ? jc + 2 : NBIN,
+                      ke = kc + 2 < NBIN ? kc + 2 : NBIN;
+            const long long nb[4] = {IX(d, k, j, i + 1), IX(d, k, j + 1, i), IX(d, k + 1, j, i), IX(d, k + 1, j + 1, i + 1)};
+            for (int e = 0; e < s.n_elmt && !c; e++) {
+                const int *r = &bins[6 * e];
+                if (r[0] >= ie || r[1] <= is || r[2] >= je || r[3] <= js || r[4] >= ke || r[5] <= ks) continue;
+                for (int w = 0; w < 4 && !c; w++) {
+                    const double q[3] = {cent[3 * nb[w]], cent[3 * nb[w] + 1], cent[3 * nb[w] + 2]};
+                    if (line_cuts(s, p, q, e)) c = 1;
+                }
+            }
+        }
+    }
+    cut[t] = c;
+}
+
+// second pass: 2 where the cell is below 0.5 and a cut cell's box holds it
+__global__ void __launch_bounds__(256) k_thin_mark(Dims d, const int *__restrict__ cut, double *__restrict__ nvert)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    if (!(nvert[t] < 0.5)) return;
+    const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+    bool m = false;
+    for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                const int kk = k - dk, jj = j - dj, ii = i - di;
+                if (kk >= 1 && jj >= 1 && ii >= 1) m |= cut[IX(d, kk, jj, ii)] != 0;
+            }
+    if (m) nvert[t] = 2.;
+}
+
 int check_surface(const vfs_ib_surface *s)
 {
     VFS_REQUIRE(s && s->n_v > 0 && s->n_elmt > 0, "the body needs vertices and elements");
@@ -779,6 +859,11 @@ extern "C" int vfs_ib_ray_jitter(int seed, double *s)
 
 extern "C" int vfs_nvert_body(vfs_ctx *c, const vfs_ib_surface *surf, int reset, double *nvert_out)
 {
+    return vfs_nvert_body_thin(c, surf, reset, 0, nvert_out);
+}
+
+extern "C" int vfs_nvert_body_thin(vfs_ctx *c, const vfs_ib_surface *surf, int reset, int thin, double *nvert_out)
+{
     const Dims d = c->d;
     if (int rc = check_surface(surf)) return rc;
     const IbSurf s = to_surf(surf);
@@ -838,6 +923,13 @@ extern "C" int vfs_nvert_body(vfs_ctx *c, const vfs_ib_surface *surf, int reset,
     if (e == hipSuccess) {
         hipLaunchKernelGGL(k_body_inside, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g.cent, s, dbins, b,
                            djit, reset ? 1 : 0, c->nvert, fail);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && thin) {
+        int *cut = (int *)c->tmp3;   // P ints (tmp3 holds 3P doubles)
+        hipLaunchKernelGGL(k_thin_cut, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g.cent, s, dbins, b,
+                           c->nvert, cut);
+        hipLaunchKernelGGL(k_thin_mark, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, cut, c->nvert);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&nfail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream);

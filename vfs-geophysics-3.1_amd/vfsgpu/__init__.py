@@ -258,11 +258,13 @@ class Ctx:
                                         C.byref(f), C.byref(a)), "vfs_outlet_correction")
         return f.value, a.value
 
-    def nvert_body(self, surface, reset=True):
+    def nvert_body(self, surface, reset=True, thin=False):
         """ibm_search_advanced's marking (ibm.c:110-360) of one triangulated
-        body into the context's mask -> the mask as a DeviceArray."""
+        body into the context's mask (thin: -thin) -> the mask as a
+        DeviceArray."""
         out = self.sca()
-        _ck(lib().vfs_nvert_body(self.h, C.byref(surface.c), int(bool(reset)), out.ptr), "vfs_nvert_body")
+        _ck(lib().vfs_nvert_body_thin(self.h, C.byref(surface.c), int(bool(reset)), int(bool(thin)), out.ptr),
+            "vfs_nvert_body_thin")
         return out
 
     def ib_search(self, surface, cap=None):
