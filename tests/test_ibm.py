@@ -598,7 +598,7 @@ def test_thin_plate_marks_only_cells_at_the_plate():
         fa, fb = f(a), f(b)
         with np.errstate(divide="ignore", invalid="ignore"):
             t = fa / (fa - fb)
-        pt = a + t[..., None] * (b - a)
+            pt = a + t[..., None] * (b - a)
         hit = (fa * fb < 0) & (pt[..., 1] > 0.2) & (pt[..., 1] < 1.8)
         cross[1:-1, 1:-1, 1:-1] |= hit
     allowed = np.zeros_like(cross)
