@@ -268,6 +268,17 @@ int vfs_ib_interpolate_opt(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_i
                            const vfs_ib_options *opt, double *ucont, double *ucat, double *p, double *ustar,
                            int *count, double *shear, double *pressure);
 
+/* The averaging statistics of ibm_interpolation_advanced (ibm.c:3481-3521,
+ * 3556-3576): from the running sums of ucat (usum), of uv|vw|wu (u1sum) and
+ * of uu|vv|ww (u2sum) over N = ti - 1 steps (3P device fields each), per
+ * element (n_elmt device doubles each) the mean over its nodes of
+ * ustar_avg^2 (the wall model of opt, or noslip, on the mean velocity) and of
+ * the three contravariant Reynolds stresses of csi-eta, eta-zet and csi-zet
+ * (Contravariant_Reynolds_stress, ibm.c:3285-3292). */
+int vfs_ib_averages(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
+                    const vfs_ib_options *opt, const double *usum, const double *u1sum, const double *u2sum, double N,
+                    double *mean_shear, double *rs1, double *rs2, double *rs3);
+
 /* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */

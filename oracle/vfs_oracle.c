@@ -3475,3 +3475,73 @@ int vo_outlet_correction(const vo_cfg *c, const vo_geom *g, const double *nvert,
     *area = a;
     return 0;
 }
+
+/* ibm.c:3285-3292 Contravariant_Reynolds_stress */
+static double ib_crs(double uu, double uv, double uw, double vv, double vw, double ww, const double *a, const double *b)
+{
+    const double A = uu * a[0] * b[0] + vv * a[1] * b[1] + ww * a[2] * b[2] + uv * (a[0] * b[1] + a[1] * b[0]) +
+                     uw * (a[0] * b[2] + a[2] * b[0]) + vw * (a[1] * b[2] + a[2] * b[1]);
+    const double B = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) * sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    return A / B;
+}
+
+/* ibm.c:3481-3521 with 3556-3576: the averaging statistics of
+ * ibm_interpolation_advanced.  usum, u1sum (uv, vw, wu), u2sum (uu, vv, ww):
+ * the running sums over N = ti - 1 steps.  Per element: the mean of
+ * ustar_avg^2 (the wall model or noslip on the mean velocity) and of the
+ * three contravariant Reynolds stresses of the nodes. */
+int vo_ib_averages(const vo_cfg *c, const vo_geom *g, const vo_ib_surface *s, const vo_ib_node *nodes, int n,
+                   const vo_ib_opt *opt, const double *usum, const double *u1sum, const double *u2sum, double N,
+                   double *mean_shear, double *rs1, double *rs2, double *rs3)
+{
+    DIMS;
+    int rc = check_cfg(c);
+    if (rc) return rc;
+    int *cnt = calloc(s->n_elmt, sizeof(int));
+    for (int e = 0; e < s->n_elmt; e++) mean_shear[e] = rs1[e] = rs2[e] = rs3[e] = 0;
+    for (int l = 0; l < n; l++) {
+        const vo_ib_node *nd = &nodes[l];
+        const int ni = nd->cell;
+        const size_t id = IX(nd->k, nd->j, nd->i);
+        const size_t q1 = IX(nd->k1, nd->j1, nd->i1), q2 = IX(nd->k2, nd->j2, nd->i2), q3 = IX(nd->k3, nd->j3, nd->i3);
+        const double sb = nd->d_s, sc = sb + nd->d_i, sk1 = nd->cr1, sk2 = nd->cr2, sk3 = nd->cr3;
+        const double nfx = s->nf_x[ni], nfy = s->nf_y[ni], nfz = s->nf_z[ni];
+        double Ua[3] = {0, 0, 0}, Uc[3];
+        if (ni >= 0 && s->u) {
+            const double *u1 = &s->u[3 * s->nv1[ni]], *u2 = &s->u[3 * s->nv2[ni]], *u3 = &s->u[3 * s->nv3[ni]];
+            for (int q = 0; q < 3; q++) Ua[q] = u1[q] * nd->cs1 + u2[q] * nd->cs2 + u3[q] * nd->cs3;
+        }
+        for (int q = 0; q < 3; q++) Uc[q] = (usum[3 * q1 + q] * sk1 + usum[3 * q2 + q] * sk2 + usum[3 * q3 + q] * sk3) / N;
+        const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
+        const double un = u_c * nfx + v_c * nfy + w_c * nfz;
+        const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+        const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+        double us;
+        if (opt->wallfunction) us = vo_hydset_utau(1. / c->ren, ut_mag, sc, 0.01, opt->ks);
+        else us = sqrt(ut_mag / sc / c->ren);
+        double m[3], m2[3], m1[3];
+        for (int q = 0; q < 3; q++) {
+            m[q] = (usum[3 * q1 + q] * sk1 + usum[3 * q2 + q] * sk2 + usum[3 * q3 + q] * sk3) / N;
+            m2[q] = (u2sum[3 * q1 + q] * sk1 + u2sum[3 * q2 + q] * sk2 + u2sum[3 * q3 + q] * sk3) / N;
+            m1[q] = (u1sum[3 * q1 + q] * sk1 + u1sum[3 * q2 + q] * sk2 + u1sum[3 * q3 + q] * sk3) / N;
+        }
+        const double U = m[0], V = m[1], W = m[2];
+        const double uu = m2[0] - U * U, vv = m2[1] - V * V, ww = m2[2] - W * W;
+        const double uv = m1[0] - U * V, vw = m1[1] - V * W, uw = m1[2] - W * U;
+        const double *cs = &g->csi[3 * id], *et = &g->eta[3 * id], *ze = &g->zet[3 * id];
+        cnt[ni]++;
+        mean_shear[ni] += us * us;
+        rs1[ni] += ib_crs(uu, uv, uw, vv, vw, ww, cs, et);
+        rs2[ni] += ib_crs(uu, uv, uw, vv, vw, ww, et, ze);
+        rs3[ni] += ib_crs(uu, uv, uw, vv, vw, ww, cs, ze);
+    }
+    for (int e = 0; e < s->n_elmt; e++)
+        if (cnt[e]) {
+            mean_shear[e] /= (double)cnt[e];
+            rs1[e] /= (double)cnt[e];
+            rs2[e] /= (double)cnt[e];
+            rs3[e] /= (double)cnt[e];
+        }
+    free(cnt);
+    return 0;
+}

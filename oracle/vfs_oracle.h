@@ -268,6 +268,10 @@ typedef struct {
 int vo_ib_interpolate_opt(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
                           const vo_ib_node *nodes, int n, const vo_ib_opt *opt, double *ucont, double *ucat, double *p,
                           double *ustar, int *count, double *shear, double *pressure);
+/* ibm.c:3481-3521, 3556-3576: the averaging statistics per element */
+int vo_ib_averages(const vo_cfg *c, const vo_geom *g, const vo_ib_surface *s, const vo_ib_node *nodes, int n,
+                   const vo_ib_opt *opt, const double *usum, const double *u1sum, const double *u2sum, double N,
+                   double *mean_shear, double *rs1, double *rs2, double *rs3);
 /* sediment_transport.c:29-68: the rough-wall law and its Newton inverse */
 double vo_hydset_u(double nu, double y, double utau, double ks);
 double vo_hydset_utau(double nu, double u, double y, double guess, double ks);

@@ -264,6 +264,15 @@ class Oracle:
                    ptr(ucat), ptr(ucont), ptr(ubcs), C.byref(f), C.byref(a))
         return f.value, a.value
 
+    def ib_averages(self, body, nodes, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4):
+        nodes = np.ascontiguousarray(nodes)
+        out = [np.zeros(body.n_elmt) for _ in range(4)]
+        o = VoIbOpt(0, int(bool(wallfunction)), float(ks))
+        self._call("vo_ib_averages", C.byref(self.geom), C.byref(body.c), nodes.ctypes.data_as(C.c_void_p),
+                   C.c_int(len(nodes)), C.byref(o), ptr(usum), ptr(u1sum), ptr(u2sum), C.c_double(N),
+                   *[ptr(a) for a in out])
+        return out
+
     def nvert_body(self, body, reset=True, thin=False):
         """ibm_search_advanced's marking of one body into self.nvert."""
         self._call("vo_nvert_body_thin", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),

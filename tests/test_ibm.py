@@ -627,3 +627,67 @@ def test_gpu_nvert_thin_body():
         got = ctx.nvert_body(V.IbSurface(vv, ee, nn), reset=(q == 0), thin=thin)
     assert np.array_equal(got.host(), o.nvert)
     assert (np.rint(o.nvert) == 1).sum() > 50 and (np.rint(o.nvert) == 3).sum() > 0
+
+
+# ---------------------------------------------------------------------------
+# the averaging statistics (ibm.c:3481-3521)
+# ---------------------------------------------------------------------------
+TAU = dict(uu=0.3, vv=0.2, ww=0.1, uv=0.05, vw=-0.02, uw=0.07)
+
+
+def averaged_sums(o, N, linear):
+    """Running sums over N steps of a mean field (U0, or U0 plus the linear
+    shear of the plane case) with the constant fluctuation tensor TAU."""
+    cent = o.g["cent"].reshape(-1, 3)
+    dist = cent @ NORMAL - PLANE_C
+    U = U0 + ((GAMMA * TANGENT + BETA * NORMAL) * dist[:, None] if linear else 0 * dist[:, None])
+    u2 = np.stack([U[:, 0] ** 2 + TAU["uu"], U[:, 1] ** 2 + TAU["vv"], U[:, 2] ** 2 + TAU["ww"]], 1)
+    u1 = np.stack([U[:, 0] * U[:, 1] + TAU["uv"], U[:, 1] * U[:, 2] + TAU["vw"], U[:, 2] * U[:, 0] + TAU["uw"]], 1)
+    return (N * U).ravel(), (N * u1).ravel(), (N * u2).ravel()
+
+
+def test_averages_closed_form():
+    # a uniform mean stream carried by the body: ustar_avg = 0, and on a
+    # uniform grid the contravariant stresses of csi-eta, eta-zet, csi-zet are
+    # the Cartesian uv, vw, uw
+    cfg, coor, o, (verts, elems, normals, uv), _, _ = plane_case()
+    body = IbBody(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    N = 40.0
+    usum, u1sum, u2sum = averaged_sums(o, N, linear=False)
+    ms, r1, r2, r3 = o.ib_averages(body, nodes, usum, u1sum, u2sum, N)
+    hit = np.bincount(nodes["cell"], minlength=len(elems)) > 0
+    assert hit.sum() > 10
+    assert np.abs(ms[hit]).max() < 1e-15       # roundoff of U0 - Ua
+    assert np.abs(r1[hit] - TAU["uv"]).max() < 1e-12
+    assert np.abs(r2[hit] - TAU["vw"]).max() < 1e-12
+    assert np.abs(r3[hit] - TAU["uw"]).max() < 1e-12
+    # the linear shear: noslip on the mean gives ustar_avg^2 = |gamma| / Re
+    # (elements whose nodes' patches stay off the ghost cells)
+    usum, u1sum, u2sum = averaged_sums(o, N, linear=True)
+    ms, r1, r2, r3 = o.ib_averages(body, nodes, usum, u1sum, u2sum, N)
+    mx, mz = cfg["mx"], cfg["mz"]
+    edge = (nodes["i"] < 2) | (nodes["i"] > mx - 3) | (nodes["k"] < 2) | (nodes["k"] > mz - 3) | (nodes["j"] < 2)
+    clean = hit & (np.bincount(nodes["cell"][edge], minlength=len(elems)) == 0)
+    assert clean.sum() > 5
+    assert np.abs(ms[clean] - abs(GAMMA) / cfg["ren"]).max() < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("wallfunction", [False, True])
+def test_gpu_ib_averages(wallfunction):
+    cfg, coor, o, (verts, elems, normals, uv), _, _ = plane_case()
+    body = IbBody(verts, elems, normals, uv)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(coor))
+    ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
+    surf = gpu_surface(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    dn, nn = ctx.ib_search(surf)
+    N = 40.0
+    sums = averaged_sums(o, N, linear=True)
+    got = ctx.ib_averages(surf, dn, nn, *[V.DeviceArray.from_host(a) for a in sums], N, wallfunction=wallfunction)
+    want = o.ib_averages(body, nodes, *sums, N, wallfunction=wallfunction)
+    tol = 1e-12 if wallfunction else 0.0
+    for g, h in zip(got, want):
+        assert np.abs(g.host() - h).max() <= tol * max(1.0, np.abs(h).max())

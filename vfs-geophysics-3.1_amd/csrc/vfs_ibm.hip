@@ -515,6 +515,96 @@ __global__ void __launch_bounds__(STAT_BLOCK) k_ib_stats(Dims d, int n_elmt, con
     if (pres) pres[e] = pr;
 }
 
+// ibm.c:3285-3292 Contravariant_Reynolds_stress
+__device__ double crs(double uu, double uv, double uw, double vv, double vw, double ww, const double *a, const double *b)
+{
+    const double A = uu * a[0] * b[0] + vv * a[1] * b[1] + ww * a[2] * b[2] + uv * (a[0] * b[1] + a[1] * b[0]) +
+                     uw * (a[0] * b[2] + a[2] * b[0]) + vw * (a[1] * b[2] + a[2] * b[1]);
+    const double B = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]) * sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    return A / B;
+}
+
+// ibm.c:3481-3521: per node, ustar_avg^2 and the three contravariant
+// Reynolds stresses from the running sums, into the staging array (four
+// blocks of n)
+__global__ void __launch_bounds__(256) k_ib_avg_nodes(Dims d, Geom g, IbSurf s, const vfs_ib_node *__restrict__ nodes,
+                                                      int n, vfs_ib_options opt, const double *__restrict__ usum,
+                                                      const double *__restrict__ u1sum, const double *__restrict__ u2sum,
+                                                      double N, double *__restrict__ stage)
+{
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const vfs_ib_node nd = nodes[l];
+    if (!node_ok(d, s.n_elmt, nd)) return;
+    const int ni = nd.cell;
+    const long long id = IX(d, nd.k, nd.j, nd.i);
+    const long long q1 = IX(d, nd.k1, nd.j1, nd.i1), q2 = IX(d, nd.k2, nd.j2, nd.i2), q3 = IX(d, nd.k3, nd.j3, nd.i3);
+    const double sb = nd.d_s, sc = sb + nd.d_i, sk1 = nd.cr1, sk2 = nd.cr2, sk3 = nd.cr3;
+    const double nfx = s.nfx[ni], nfy = s.nfy[ni], nfz = s.nfz[ni];
+    double Ua[3] = {0, 0, 0}, m[3], m1[3], m2[3];
+    if (s.u) {
+        const double *u1 = &s.u[3 * (long long)s.nv1[ni]], *u2 = &s.u[3 * (long long)s.nv2[ni]],
+                     *u3 = &s.u[3 * (long long)s.nv3[ni]];
+        for (int q = 0; q < 3; q++) Ua[q] = u1[q] * nd.cs1 + u2[q] * nd.cs2 + u3[q] * nd.cs3;
+    }
+    for (int q = 0; q < 3; q++) {
+        m[q] = (usum[3 * q1 + q] * sk1 + usum[3 * q2 + q] * sk2 + usum[3 * q3 + q] * sk3) / N;
+        m2[q] = (u2sum[3 * q1 + q] * sk1 + u2sum[3 * q2 + q] * sk2 + u2sum[3 * q3 + q] * sk3) / N;
+        m1[q] = (u1sum[3 * q1 + q] * sk1 + u1sum[3 * q2 + q] * sk2 + u1sum[3 * q3 + q] * sk3) / N;
+    }
+    const double u_c = m[0] - Ua[0], v_c = m[1] - Ua[1], w_c = m[2] - Ua[2];
+    const double un = u_c * nfx + v_c * nfy + w_c * nfz;
+    const double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
+    const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+    const double us = opt.wallfunction ? hy_utau(1. / d.ren, ut_mag, sc, 0.01, opt.ks) : sqrt(ut_mag / sc / d.ren);
+    const double U = m[0], V = m[1], W = m[2];
+    const double uu = m2[0] - U * U, vv = m2[1] - V * V, ww = m2[2] - W * W;
+    const double uv = m1[0] - U * V, vw = m1[1] - V * W, uw = m1[2] - W * U;
+    const double *cs = &g.csi[3 * id], *et = &g.eta[3 * id], *ze = &g.zet[3 * id];
+    stage[l] = us * us;
+    stage[n + l] = crs(uu, uv, uw, vv, vw, ww, cs, et);
+    stage[2 * (long long)n + l] = crs(uu, uv, uw, vv, vw, ww, et, ze);
+    stage[3 * (long long)n + l] = crs(uu, uv, uw, vv, vw, ww, cs, ze);
+}
+
+// ibm.c:3556-3576 for the averaging statistics: per element, the means of
+// the four staged values over its nodes, added in list order
+__global__ void __launch_bounds__(STAT_BLOCK) k_ib_avg_stats(Dims d, int n_elmt, const vfs_ib_node *__restrict__ nodes,
+                                                             int n, const double *__restrict__ stage,
+                                                             double *__restrict__ o0, double *__restrict__ o1,
+                                                             double *__restrict__ o2, double *__restrict__ o3)
+{
+    __shared__ int sc[STAT_BLOCK];
+    __shared__ double sv[4][STAT_BLOCK];
+    const int e = blockIdx.x * STAT_BLOCK + threadIdx.x;
+    int cnt = 0;
+    double a[4] = {0, 0, 0, 0};
+    for (int l0 = 0; l0 < n; l0 += STAT_BLOCK) {
+        const int l = l0 + threadIdx.x;
+        if (l < n) {
+            const vfs_ib_node nd = nodes[l];
+            const bool ok = node_ok(d, n_elmt, nd);
+            sc[threadIdx.x] = ok ? nd.cell : -1;
+            for (int q = 0; q < 4; q++) sv[q][threadIdx.x] = ok ? stage[(long long)q * n + l] : 0.0;
+        }
+        __syncthreads();
+        const int m = min(STAT_BLOCK, n - l0);
+        for (int q = 0; q < m; q++)
+            if (sc[q] == e) {
+                cnt++;
+                for (int w = 0; w < 4; w++) a[w] += sv[w][q];
+            }
+        __syncthreads();
+    }
+    if (e >= n_elmt) return;
+    if (cnt)
+        for (int w = 0; w < 4; w++) a[w] /= (double)cnt;
+    o0[e] = a[0];
+    o1[e] = a[1];
+    o2[e] = a[2];
+    o3[e] = a[3];
+}
+
 // ibm.c:3619-3679: the fluxes of the faces of every nvert == 1 cell from the
 // averaged new velocity (immersed != 3: factor 1)
 __global__ void __launch_bounds__(256) k_ib_faces(Dims d, Geom g, const double *__restrict__ nvert,
@@ -945,5 +1035,29 @@ extern "C" int vfs_nvert_body_thin(vfs_ctx *c, const vfs_ib_surface *surf, int r
     c->nvert_f32 = 1;   // small integers
     if (nvert_out)
         VFS_CHECK_HIP(hipMemcpyAsync(nvert_out, c->nvert, (size_t)d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+extern "C" int vfs_ib_averages(vfs_ctx *c, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
+                               const vfs_ib_options *opt, const double *usum, const double *u1sum,
+                               const double *u2sum, double N, double *mean_shear, double *rs1, double *rs2,
+                               double *rs3)
+{
+    const Dims d = c->d;
+    if (int rc = check_surface(surf)) return rc;
+    VFS_REQUIRE(opt && n >= 0 && (n == 0 || nodes) && usum && u1sum && u2sum && mean_shear && rs1 && rs2 && rs3,
+                "bad arguments");
+    VFS_REQUIRE(!opt->wallfunction || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
+    VFS_REQUIRE(4 * (long long)n <= 3 * d.P, "more IB nodes than cells");
+    const IbSurf s = to_surf(surf);
+    double *stage = c->tmp3;
+    if (n > 0) {
+        hipLaunchKernelGGL(k_ib_avg_nodes, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->g, s, nodes, n, *opt,
+                           usum, u1sum, u2sum, N, stage);
+        VFS_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_ib_avg_stats, dim3(grid_for(s.n_elmt, STAT_BLOCK)), dim3(STAT_BLOCK), 0, c->stream, d,
+                       s.n_elmt, nodes, n, stage, mean_shear, rs1, rs2, rs3);
+    VFS_CHECK_LAUNCH();
     return 0;
 }

@@ -289,6 +289,17 @@ class Ctx:
             "vfs_ib_interpolate_opt")
         return st if stats else None
 
+    def ib_averages(self, surface, nodes, n, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4):
+        """The averaging statistics of ibm_interpolation_advanced -> the
+        element means (ustar_avg^2, and the three contravariant Reynolds
+        stresses) as DeviceArrays."""
+        ne = surface.n_elmt
+        out = [DeviceArray(ne) for _ in range(4)]
+        o = IbOptions(0, int(bool(wallfunction)), float(ks))
+        _ck(lib().vfs_ib_averages(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), C.byref(o), usum.ptr,
+                                  u1sum.ptr, u2sum.ptr, C.c_double(N), *[a.ptr for a in out]), "vfs_ib_averages")
+        return out
+
     def set_body_force(self, f_eul):
         _ck(lib().vfs_set_body_force(self.h, None if f_eul is None else f_eul.ptr), "vfs_set_body_force")
 
