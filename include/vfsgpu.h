@@ -259,9 +259,14 @@ int vfs_outlet_correction(vfs_ctx *ctx, double flux_in, int scale, const double 
  * vfs_ib_interpolate. */
 typedef struct {
     int moving;             /* movefsi / rotatefsi: the fluid-support guard is off (ibm.c:3448) */
-    int wallfunction;       /* -wallfunction at ti > 0: the node velocity from the rough-wall
-                               law of wall_function_roughness_a (sediment_transport.c:18-96) */
-    double ks;              /* its roughness height (the reference passes 0.0001, ibm.c:3473) */
+    int wallfunction;       /* -wallfunction at ti > 0: 1, the node velocity from the rough-wall
+                               law of wall_function_roughness_a (sediment_transport.c:18-96);
+                               2, with -roughness on body 0 (ibm.c:3468-3471), the rough log law
+                               u = u_tau (ln(y / roughness) / 0.41 + 8.5) of
+                               wall_function_roughness_loglaw (wallfunction.c:71-95), which keeps
+                               the tangential part only; the averages use law 1 for both */
+    double ks;              /* law 1's roughness height (the reference passes 0.0001, ibm.c:3473) */
+    double roughness;       /* -roughness (main.c:2073): law 2's roughness height */
 } vfs_ib_options;
 /* vfs_ib_interpolate with options (ibm.c:3448-3478). */
 int vfs_ib_interpolate_opt(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,

@@ -2938,11 +2938,32 @@ double vo_hydset_utau(double nu, double u, double y, double guess, double ks)
     return utau;
 }
 
+/* wallfunction.c:221-231 u_loglaw_roughness: linear in u_tau */
+double vo_roughlog_u(double y, double utau, double ks)
+{
+    return utau * (1. / WF_KAPPA * log(y / ks) + 8.5);
+}
+
+/* wallfunction.c:244-246, 264-269, 423-437: Newton on rl_u - u, slope by a
+ * central difference of step 1e-7, stop below 1e-10, 30 steps at most */
+double vo_roughlog_utau(double u, double y, double guess, double ks)
+{
+    double x = guess, x0 = guess;
+    for (int it = 0; it < 30; it++) {
+        const double h = 1.e-7;
+        const double fp = vo_roughlog_u(y, x0 + h, ks) - u, fm = vo_roughlog_u(y, x0 - h, ks) - u;
+        x = x0 - (vo_roughlog_u(y, x0, ks) - u) / ((fp - fm) / (2 * h));
+        if (fabs(x0 - x) < 1.e-10) break;
+        x0 = x;
+    }
+    return x;
+}
+
 int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
                       const vo_ib_node *nodes, int n, double *ucont, double *ucat, double *p, double *ustar,
                       int *count, double *shear, double *pressure)
 {
-    const vo_ib_opt o = {0, 0, 0.0};
+    const vo_ib_opt o = {0, 0, 0.0, 0.0};
     return vo_ib_interpolate_opt(c, g, nvert, s, nodes, n, &o, ucont, ucat, p, ustar, count, shear, pressure);
 }
 
@@ -2984,7 +3005,16 @@ int vo_ib_interpolate_opt(const vo_cfg *c, const vo_geom *g, const double *nvert
             const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
             const double un = u_c * nfx + v_c * nfy + w_c * nfz;
             double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
-            if (opt->wallfunction) {   /* wall_function_roughness_a, sediment_transport.c:70-96 */
+            if (opt->wallfunction == 2) {   /* wall_function_roughness_loglaw, wallfunction.c:71-95 */
+                const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+                ustar[id] = vo_roughlog_utau(ut_mag, sc, 0.01, opt->roughness);
+                const double um = vo_roughlog_u(sb, ustar[id], opt->roughness);
+                if (ut_mag > 1.e-10) ut *= um / ut_mag, vt *= um / ut_mag, wt *= um / ut_mag;
+                else ut = vt = wt = 0;
+                Ub[0] = ut;   /* the normal part is dropped (wallfunction.c:88-90) */
+                Ub[1] = vt;
+                Ub[2] = wt;
+            } else if (opt->wallfunction) {   /* wall_function_roughness_a, sediment_transport.c:70-96 */
                 const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
                 ustar[id] = vo_hydset_utau(1. / c->ren, ut_mag, sc, 0.01, opt->ks);
                 const double um = vo_hydset_u(1. / c->ren, sb, ustar[id], opt->ks);

@@ -260,10 +260,12 @@ int vo_ib_interpolate(const vo_cfg *c, const vo_geom *g, const double *nvert, co
 
 /* options of ibm_interpolation_advanced: moving (movefsi / rotatefsi: no
  * fluid-support guard), wallfunction (-wallfunction with ti > 0: the
- * rough-wall law of wall_function_roughness_a with roughness ks) */
+ * rough-wall law of wall_function_roughness_a with roughness ks; 2: with
+ * -roughness on body 0, wall_function_roughness_loglaw with roughness
+ * "roughness" at the nodes, law 1 in the averages) */
 typedef struct {
     int moving, wallfunction;
-    double ks;
+    double ks, roughness;
 } vo_ib_opt;
 int vo_ib_interpolate_opt(const vo_cfg *c, const vo_geom *g, const double *nvert, const vo_ib_surface *s,
                           const vo_ib_node *nodes, int n, const vo_ib_opt *opt, double *ucont, double *ucat, double *p,
@@ -275,6 +277,10 @@ int vo_ib_averages(const vo_cfg *c, const vo_geom *g, const vo_ib_surface *s, co
 /* sediment_transport.c:29-68: the rough-wall law and its Newton inverse */
 double vo_hydset_u(double nu, double y, double utau, double ks);
 double vo_hydset_utau(double nu, double u, double y, double guess, double ks);
+/* wallfunction.c:221-231 u_loglaw_roughness and 423-437
+ * find_utau_loglaw_roughness (wallfunction == 2 with roughness ks) */
+double vo_roughlog_u(double y, double utau, double ks);
+double vo_roughlog_utau(double u, double y, double guess, double ks);
 
 /* ibm_search_advanced's marking (ibm.c:110-360, 508-516): nvert of one
  * triangulated body by ray casting (point_cell_advanced, ibm.c:1068-1183),

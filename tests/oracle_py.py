@@ -40,7 +40,11 @@ def les_opt(les=2, ti=5, tistart=0, rstart=0, max_cs=0.5, i_homo=0, j_homo=0, k_
 
 
 class VoIbOpt(C.Structure):
-    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double)]
+    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double), ("roughness", C.c_double)]
+
+
+def _wall_law(wallfunction, roughness):
+    return 0 if not wallfunction else (1 if roughness is None else 2)
 
 
 class VoInflowOpt(C.Structure):
@@ -89,13 +93,16 @@ def lib():
         _build()
         _lib = C.CDLL(LIB)
         d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter", "vo_hydset_u", "vo_hydset_utau"]:
+        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter", "vo_hydset_u", "vo_hydset_utau",
+                     "vo_roughlog_u", "vo_roughlog_utau"]:
             getattr(_lib, name).restype = d
         for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
             getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
         _lib.vo_wf_table.restype = C.POINTER(d)
         _lib.vo_hydset_u.argtypes = [d, d, d, d]
         _lib.vo_hydset_utau.argtypes = [d, d, d, d, d]
+        _lib.vo_roughlog_u.argtypes = [d, d, d]
+        _lib.vo_roughlog_utau.argtypes = [d, d, d, d]
     return _lib
 
 
@@ -264,10 +271,10 @@ class Oracle:
                    ptr(ucat), ptr(ucont), ptr(ubcs), C.byref(f), C.byref(a))
         return f.value, a.value
 
-    def ib_averages(self, body, nodes, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4):
+    def ib_averages(self, body, nodes, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4, roughness=None):
         nodes = np.ascontiguousarray(nodes)
         out = [np.zeros(body.n_elmt) for _ in range(4)]
-        o = VoIbOpt(0, int(bool(wallfunction)), float(ks))
+        o = VoIbOpt(0, _wall_law(wallfunction, roughness), float(ks), float(roughness or 0.0))
         self._call("vo_ib_averages", C.byref(self.geom), C.byref(body.c), nodes.ctypes.data_as(C.c_void_p),
                    C.c_int(len(nodes)), C.byref(o), ptr(usum), ptr(u1sum), ptr(u2sum), C.c_double(N),
                    *[ptr(a) for a in out])
@@ -279,13 +286,14 @@ class Oracle:
                    C.c_int(int(bool(reset))), C.c_int(int(bool(thin))))
         return self.nvert
 
-    def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar, moving=False, wallfunction=False, ks=1e-4):
+    def ib_interpolate(self, body, nodes, ucont, ucat, p, ustar, moving=False, wallfunction=False, ks=1e-4,
+                       roughness=None):
         """ibm_interpolation_advanced; ucont, ucat, p, ustar in/out.
         Returns the element (count, mean ustar^2, mean p)."""
         nodes = np.ascontiguousarray(nodes)
         cnt = np.zeros(body.n_elmt, np.int32)
         sh, pr = np.zeros(body.n_elmt), np.zeros(body.n_elmt)
-        o = VoIbOpt(int(bool(moving)), int(bool(wallfunction)), float(ks))
+        o = VoIbOpt(int(bool(moving)), _wall_law(wallfunction, roughness), float(ks), float(roughness or 0.0))
         self._call("vo_ib_interpolate_opt", C.byref(self.geom), ptr(self.nvert), C.byref(body.c),
                    nodes.ctypes.data_as(C.c_void_p), C.c_int(len(nodes)), C.byref(o), ptr(ucont), ptr(ucat), ptr(p),
                    ptr(ustar), ptr(cnt), ptr(sh), ptr(pr))

@@ -518,8 +518,50 @@ def test_plane_body_wall_model_and_moving(moving):
     assert checked > 50
 
 
+def test_rough_loglaw_closed_form():
+    """u = u_tau (ln(y/ks)/0.41 + 8.5) is linear in u_tau, so the Newton
+    solve must land on u / (ln(y/ks)/0.41 + 8.5)."""
+    ru, rt = lib().vo_roughlog_u, lib().vo_roughlog_utau
+    for (u, y, k) in [(0.5, 0.01, 1e-3), (0.05, 0.002, 1e-4), (1.2, 0.05, 2e-3), (3e-4, 0.004, 1e-6)]:
+        a = np.log(y / k) / 0.41 + 8.5
+        assert abs(ru(y, 0.7, k) - 0.7 * a) < 1e-14 * a
+        assert abs(rt(u, y, 0.01, k) - u / a) < 1e-12 * u / a
+
+
+def test_plane_body_rough_loglaw():
+    """The rough log law at the nodes of the plane body: the tangential speed
+    gamma s_c at s_c becomes A(s_b)/A(s_c) of it at s_b, A(y) = ln(y/ks)/0.41
+    + 8.5, the normal part is dropped, and u_tau = gamma s_c / A(s_c)."""
+    cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case()
+    body = IbBody(verts, elems, normals, uv)
+    nodes = o.ib_search(body)
+    mx, my, mz = cfg["mx"], cfg["my"], cfg["mz"]
+    ucont, ustar = o.vec(), o.sca()
+    u = ucat.copy()
+    R = 2e-4
+    o.ib_interpolate(body, nodes, ucont, u, p.copy(), ustar, wallfunction=True, ks=1e-4, roughness=R)
+    A = lambda y: np.log(y / R) / 0.41 + 8.5
+    checked = 0
+    for nd in nodes:
+        if not (2 <= nd["i"] <= mx - 3 and 2 <= nd["k"] <= mz - 3 and nd["j"] >= 2):
+            continue
+        li = (nd["k"] * my + nd["j"]) * mx + nd["i"]
+        cells = [((nd[f"k{m}"] * my + nd[f"j{m}"]) * mx + nd[f"i{m}"]) for m in (1, 2, 3)]
+        sb = nd["d_s"]
+        sc = sb + nd["d_i"]
+        got = u[3 * li:3 * li + 3]
+        if sum(o.nvert[cm] for cm in cells) > 0.1:
+            assert np.all(got == 0)
+            continue
+        assert abs(ustar[li] - abs(GAMMA) * sc / A(sc)) < 1e-12
+        want = U0 + GAMMA * sc * A(sb) / A(sc) * TANGENT
+        assert np.abs(got - want).max() < 1e-11
+        checked += 1
+    assert checked > 50
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("moving,wallfunction", [(True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("moving,wallfunction", [(True, False), (False, True), (True, True), (False, "loglaw")])
 def test_gpu_ib_options(moving, wallfunction):
     cfg, coor, o, (verts, elems, normals, uv), ucat, p = plane_case()
     body = IbBody(verts, elems, normals, uv)
@@ -531,10 +573,12 @@ def test_gpu_ib_options(moving, wallfunction):
     dn, nn = ctx.ib_search(surf)
     ucont = np.sin(np.arange(3 * o.P) * 0.11)
     d_ucat, d_ucont, d_p, d_us = (V.DeviceArray.from_host(a) for a in (ucat, ucont, p, np.zeros(o.P)))
+    rough = 2e-4 if wallfunction == "loglaw" else None
     st = ctx.ib_interpolate(surf, dn, nn, d_ucont, d_ucat, d_p, d_us, moving=moving, wallfunction=wallfunction,
-                            ks=1e-4)
+                            ks=1e-4, roughness=rough)
     uc, u, pp, us = ucont.copy(), ucat.copy(), p.copy(), np.zeros(o.P)
-    cnt, sh, pr = o.ib_interpolate(body, nodes, uc, u, pp, us, moving=moving, wallfunction=wallfunction, ks=1e-4)
+    cnt, sh, pr = o.ib_interpolate(body, nodes, uc, u, pp, us, moving=moving, wallfunction=wallfunction, ks=1e-4,
+                                   roughness=rough)
     # the wall law's log / exp / sin: the device's and the C library's
     tol = 1e-12 if wallfunction else 0.0
     for g, h in ((d_ucat.host(), u), (d_ucont.host(), uc), (d_p.host(), pp), (d_us.host(), us)):

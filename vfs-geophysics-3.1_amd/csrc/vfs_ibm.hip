@@ -385,6 +385,28 @@ __device__ double hy_utau(double nu, double u, double y, double guess, double ks
     return utau;
 }
 
+// u_loglaw_roughness (wallfunction.c:221-231)
+__device__ double rl_u(double y, double utau, double ks)
+{
+    const double kappa = 0.41;
+    return utau * (1. / kappa * log(y / ks) + 8.5);
+}
+
+// find_utau_loglaw_roughness (wallfunction.c:423-437): Newton on
+// f = rl_u - u with a central-difference slope, 30 steps at most
+__device__ double rl_utau(double u, double y, double guess, double ks)
+{
+    double x = guess, x0 = guess;
+    for (int ii = 0; ii < 30; ii++) {
+        const double eps = 1.e-7;
+        const double df = ((rl_u(y, x0 + eps, ks) - u) - (rl_u(y, x0 - eps, ks) - u)) / (2 * eps);
+        x = x0 - (rl_u(y, x0, ks) - u) / df;
+        if (fabs(x0 - x) < 1.e-10) break;
+        x0 = x;
+    }
+    return x;
+}
+
 // a node record that fits the grid and the body (entries that do not are
 // skipped, so no kernel reads outside its arrays)
 __device__ __forceinline__ bool node_ok(const Dims &d, int n_elmt, const vfs_ib_node &nd)
@@ -428,7 +450,17 @@ __global__ void __launch_bounds__(256) k_ib_nodes(Dims d, const double *__restri
         const double u_c = Uc[0] - Ua[0], v_c = Uc[1] - Ua[1], w_c = Uc[2] - Ua[2];
         const double un = u_c * nfx + v_c * nfy + w_c * nfz;
         double ut = u_c - un * nfx, vt = v_c - un * nfy, wt = w_c - un * nfz;
-        if (opt.wallfunction) {   // wall_function_roughness_a, sediment_transport.c:70-96
+        if (opt.wallfunction == 2) {   // wall_function_roughness_loglaw, wallfunction.c:71-95
+            const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
+            const double us = rl_utau(ut_mag, sc, 0.01, opt.roughness);
+            ustar[id] = us;
+            const double um = rl_u(sb, us, opt.roughness);
+            if (ut_mag > 1.e-10) ut *= um / ut_mag, vt *= um / ut_mag, wt *= um / ut_mag;
+            else ut = vt = wt = 0;
+            Ub[0] = ut;
+            Ub[1] = vt;
+            Ub[2] = wt;
+        } else if (opt.wallfunction) {   // wall_function_roughness_a, sediment_transport.c:70-96
             const double ut_mag = sqrt(ut * ut + vt * vt + wt * wt);
             const double us = hy_utau(1. / d.ren, ut_mag, sc, 0.01, opt.ks);
             ustar[id] = us;
@@ -906,9 +938,10 @@ extern "C" int vfs_ib_interpolate_opt(vfs_ctx *c, const vfs_ib_surface *surf, co
                                       int *count, double *shear, double *pressure)
 {
     const Dims d = c->d;
-    VFS_REQUIRE(opt && (opt->moving == 0 || opt->moving == 1) && (opt->wallfunction == 0 || opt->wallfunction == 1),
+    VFS_REQUIRE(opt && (opt->moving == 0 || opt->moving == 1) && opt->wallfunction >= 0 && opt->wallfunction <= 2,
                 "bad IB options");
     VFS_REQUIRE(!opt->wallfunction || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
+    VFS_REQUIRE(opt->wallfunction != 2 || opt->roughness > 0, "the rough log law needs a roughness > 0");
     if (int rc = check_surface(surf)) return rc;
     VFS_REQUIRE(n >= 0 && (n == 0 || nodes) && ucont && ucat && p && ustar, "bad arguments");
     VFS_REQUIRE(4 * (long long)n <= 3 * d.P, "more IB nodes than cells");
@@ -1048,6 +1081,7 @@ extern "C" int vfs_ib_averages(vfs_ctx *c, const vfs_ib_surface *surf, const vfs
     VFS_REQUIRE(opt && n >= 0 && (n == 0 || nodes) && usum && u1sum && u2sum && mean_shear && rs1 && rs2 && rs3,
                 "bad arguments");
     VFS_REQUIRE(!opt->wallfunction || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
+    VFS_REQUIRE(opt->wallfunction != 2 || opt->roughness > 0, "the rough log law needs a roughness > 0");
     VFS_REQUIRE(4 * (long long)n <= 3 * d.P, "more IB nodes than cells");
     const IbSurf s = to_surf(surf);
     double *stage = c->tmp3;

@@ -72,7 +72,15 @@ class InflowOptions(C.Structure):
 
 
 class IbOptions(C.Structure):
-    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double)]
+    _fields_ = [("moving", C.c_int), ("wallfunction", C.c_int), ("ks", C.c_double), ("roughness", C.c_double)]
+
+
+def _wall_law(wallfunction, roughness):
+    """vfs_ib_options.wallfunction: 0 off, 1 wall_function_roughness_a, 2 the
+    rough log law of -roughness (ibm.c:3468-3474)."""
+    if not wallfunction:
+        return 0
+    return 1 if roughness is None else 2
 
 
 class IbSurfaceC(C.Structure):
@@ -277,25 +285,28 @@ class Ctx:
         return nodes, n.value
 
     def ib_interpolate(self, surface, nodes, n, ucont, ucat, p, ustar, stats=True, moving=False, wallfunction=False,
-                       ks=1e-4):
+                       ks=1e-4, roughness=None):
         """ibm_interpolation_advanced (ibm.c:3294-3698): ucont, ucat, p, ustar
         in/out.  Returns the element (count, mean ustar^2, mean p) as
-        DeviceArrays, or None with stats=False."""
+        DeviceArrays, or None with stats=False.  With wallfunction, a
+        roughness (-roughness, set for body 0 only in the reference) selects
+        the rough log law instead of wall_function_roughness_a."""
         ne = surface.n_elmt
         st = (DeviceArray(ne, np.int32), DeviceArray(ne), DeviceArray(ne)) if stats else (None, None, None)
-        o = IbOptions(int(bool(moving)), int(bool(wallfunction)), float(ks))
+        o = IbOptions(int(bool(moving)), _wall_law(wallfunction, roughness), float(ks),
+                      float(roughness or 0.0))
         _ck(lib().vfs_ib_interpolate_opt(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), C.byref(o), ucont.ptr,
                                          ucat.ptr, p.ptr, ustar.ptr, *[None if a is None else a.ptr for a in st]),
             "vfs_ib_interpolate_opt")
         return st if stats else None
 
-    def ib_averages(self, surface, nodes, n, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4):
+    def ib_averages(self, surface, nodes, n, usum, u1sum, u2sum, N, wallfunction=False, ks=1e-4, roughness=None):
         """The averaging statistics of ibm_interpolation_advanced -> the
         element means (ustar_avg^2, and the three contravariant Reynolds
         stresses) as DeviceArrays."""
         ne = surface.n_elmt
         out = [DeviceArray(ne) for _ in range(4)]
-        o = IbOptions(0, int(bool(wallfunction)), float(ks))
+        o = IbOptions(0, _wall_law(wallfunction, roughness), float(ks), float(roughness or 0.0))
         _ck(lib().vfs_ib_averages(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(n)), C.byref(o), usum.ptr,
                                   u1sum.ptr, u2sum.ptr, C.c_double(N), *[a.ptr for a in out]), "vfs_ib_averages")
         return out
