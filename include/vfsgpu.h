@@ -64,7 +64,10 @@ int vfs_memcpy_dtod(void *dst, const void *src, size_t bytes);
 
 /* ---- geometry ---------------------------------------------------------- */
 /* FormMetrics (metrics.c:4-1346, NEWMETRIC): cell and face metrics from node
- * coordinates coor (vector field, nodes i<mx-1, j<my-1, k<mz-1). */
+ * coordinates coor (vector field, nodes i<mx-1, j<my-1, k<mz-1).  Returns -7
+ * when an interior cell's Jacobian is not a positive finite number (a
+ * folded or degenerate grid; the reference prints "Negative jacobian",
+ * momentum.c:157); the metrics are still written. */
 int vfs_build_metrics(vfs_ctx *ctx, const double *coor);
 /* Copy one metric array out: which = 0..16 in the order csi, eta, zet, aj,
  * icsi, ieta, izet, iaj, jcsi, jeta, jzet, jaj, kcsi, keta, kzet, kaj, cent. */
@@ -341,7 +344,8 @@ int vfs_implicit_matrixfree(vfs_ctx *ctx, double *U, const double *U_o, const do
 /* setup_lidx2 + PoissonLHSNew (poisson.c:182-329, 971-1682): numbers the
  * unknowns (returned in *n_unknowns) and builds the 19-point coefficients
  * (with the 1e-10 drop rule) and the multigrid hierarchy, from the context's
- * metrics and nvert. */
+ * metrics and nvert.  Returns -8 when no cell is fluid (the system is
+ * empty). */
 int vfs_poisson_setup(vfs_ctx *ctx, int *n_unknowns);
 /* Unknown numbering on the DA layout (gid, -1 where no unknown). */
 int vfs_poisson_gid(vfs_ctx *ctx, double *gid);
@@ -361,7 +365,10 @@ int vfs_poisson_rhs(vfs_ctx *ctx, const double *ucont, double *b);
  * (poisson_hypre.c:498-630, 194-255; poisson.c:92-180).
  * precision 0: FP64 throughout; 1: FP32 stencil/smoother/Krylov inside an
  * FP64 defect-correction loop (FP64 residual, dot products, updates).
- * phi: in = initial guess on the DA layout, out = solution (ghosts set). */
+ * phi: in = initial guess on the DA layout, out = solution (ghosts set).
+ * Returns 0 when converged, 1 when maxit was reached above rtol (phi, iters
+ * and relres hold the last iterate), -6 when b or the residual is not finite
+ * (NaN / Inf).  The reference's HYPRE solve returns nothing in either case. */
 int vfs_poisson_solve(vfs_ctx *ctx, const double *b, double *phi, double rtol,
                       int precision, int maxit, int *iters, double *relres);
 /* UpdatePressure (poisson.c:3144-3304): p, phi in/out. */

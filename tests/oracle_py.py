@@ -3,6 +3,7 @@
 Test infrastructure only: the oracle is the checker that GPU results are
 compared against; nothing in the product imports this module.
 """
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -91,19 +92,36 @@ def lib():
     global _lib
     if _lib is None:
         _build()
-        _lib = C.CDLL(LIB)
-        d = C.c_double
-        for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter", "vo_hydset_u", "vo_hydset_utau",
-                     "vo_roughlog_u", "vo_roughlog_utau"]:
-            getattr(_lib, name).restype = d
-        for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
-            getattr(_lib, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
-        _lib.vo_wf_table.restype = C.POINTER(d)
-        _lib.vo_hydset_u.argtypes = [d, d, d, d]
-        _lib.vo_hydset_utau.argtypes = [d, d, d, d, d]
-        _lib.vo_roughlog_u.argtypes = [d, d, d]
-        _lib.vo_roughlog_utau.argtypes = [d, d, d, d]
+        _lib = _configure(C.CDLL(LIB))
     return _lib
+
+
+@contextlib.contextmanager
+def using(path):
+    """Run the enclosed block against another build of the oracle (a
+    mutant compiled by the pin tests); every wrapper calls lib()."""
+    global _lib
+    saved = lib()
+    _lib = _configure(C.CDLL(path))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _configure(L):
+    d = C.c_double
+    for name in ["vo_min_dt", "vo_divergence", "vo_wf_integrate_F", "vo_wf_find_utau", "vo_testfilter", "vo_testfilter_ik", "vo_dfunc", "vo_ib_ray_jitter", "vo_hydset_u", "vo_hydset_utau",
+                 "vo_roughlog_u", "vo_roughlog_utau"]:
+        getattr(L, name).restype = d
+    for name in ["vo_wf_integrate_F", "vo_wf_find_utau"]:
+        getattr(L, name).argtypes = [d, d, d] + ([d] if name == "vo_wf_find_utau" else [])
+    L.vo_wf_table.restype = C.POINTER(d)
+    L.vo_hydset_u.argtypes = [d, d, d, d]
+    L.vo_hydset_utau.argtypes = [d, d, d, d, d]
+    L.vo_roughlog_u.argtypes = [d, d, d]
+    L.vo_roughlog_utau.argtypes = [d, d, d, d]
+    return L
 
 
 def ptr(a):

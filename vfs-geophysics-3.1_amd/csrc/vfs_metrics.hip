@@ -142,6 +142,25 @@ __global__ void k_periodic_allmetrics(Dims d, Geom g)
     g.aj[o] = g.aj[s]; g.iaj[o] = g.iaj[s]; g.jaj[o] = g.jaj[s]; g.kaj[o] = g.kaj[s];
 }
 
+// interior cells whose Jacobian is not a positive finite number (the
+// reference prints "Negative jacobian" for aj < 0, momentum.c:157): per-block
+// count at part[b] and the largest such point index at part[VFS_RED_BLOCKS + b]
+__global__ void k_aj_check(Dims d, const double *__restrict__ aj, double *part)
+{
+    __shared__ double sh[4];
+    double cnt = 0, last = -1;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < d.P;
+         t += (long long)gridDim.x * blockDim.x) {
+        int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+        if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) continue;
+        const double a = aj[t];
+        if (!(a > 0.0 && a < INFINITY)) { cnt += 1.0; last = (double)t; }
+    }
+    double r = block_sum<256>(cnt, sh);
+    double m = block_max<256>(last, sh);
+    if (threadIdx.x == 0) { part[blockIdx.x] = r; part[VFS_RED_BLOCKS + blockIdx.x] = m; }
+}
+
 extern "C" int vfs_build_metrics(vfs_ctx *c, const double *coor)
 {
     const Dims d = c->d;
@@ -163,6 +182,21 @@ extern "C" int vfs_build_metrics(vfs_ctx *c, const double *coor)
     if (d.ip || d.jp || d.kp) {
         hipLaunchKernelGGL(k_periodic_allmetrics, dim3(grid_for(ns, 256)), dim3(256), 0, c->stream, d, c->g);
         VFS_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_aj_check, dim3(VFS_RED_BLOCKS), dim3(256), 0, c->stream, d, c->g.aj, c->red);
+    VFS_CHECK_LAUNCH();
+    double *res = c->red + (long long)VFS_RED_SLOTS * VFS_RED_BLOCKS;
+    if (vfs_reduce_finish_slots(c, 0, VFS_RED_BLOCKS, 1, 0, res)) return -1;
+    if (vfs_reduce_finish_slots(c, 1, VFS_RED_BLOCKS, 1, 1, res + 1)) return -1;
+    double h[2];
+    if (vfs_read_scalars(c, res, h, 2)) return -1;
+    if (h[0] != 0.0) {
+        const long long t = (long long)h[1];
+        const int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
+        vfs_set_error("vfs_build_metrics: " + std::to_string((long long)h[0]) +
+                      " cell(s) with a non-positive or non-finite Jacobian (a folded or degenerate grid), e.g. "
+                      "i,j,k = " + std::to_string(i) + "," + std::to_string(j) + "," + std::to_string(k));
+        return -7;
     }
     return 0;
 }

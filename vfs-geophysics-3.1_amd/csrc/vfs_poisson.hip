@@ -218,6 +218,10 @@ extern "C" int vfs_poisson_setup(vfs_ctx *c, int *n_unknowns)
     VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
     ps->n = total;
     if (n_unknowns) *n_unknowns = total;
+    if (total == 0) {
+        vfs_set_error("vfs_poisson_setup: no fluid cell (every cell has nvert >= 0.1), the pressure system is empty");
+        return -8;
+    }
     if (vfs_mg_setup(c)) return -1;
     ps->ready = 1;
     return 0;

@@ -1367,9 +1367,14 @@ extern "C" int vfs_poisson_solve(vfs_ctx *c, const double *b, double *phi, doubl
     VFS_CHECK_LAUNCH();
     double bn2;
     if (norm2_64(c, ps->bb64, n, dres, &bn2)) return -1;
+    if (!std::isfinite(bn2)) {
+        vfs_set_error("vfs_poisson_solve: the right-hand side is not finite (NaN or Inf)");
+        return -6;
+    }
     double bnorm = sqrt(bn2);
     int total = 0;
     double rel = 0;
+    bool finite = true;
     if (bnorm == 0.0) bnorm = 1.0;
     for (int outer = 0; outer < 50; outer++) {
         // FP64 defect r = b - A x (FP64 operator)
@@ -1377,6 +1382,7 @@ extern "C" int vfs_poisson_solve(vfs_ctx *c, const double *b, double *phi, doubl
         double rn2;
         if (norm2_64(c, ps->r64, n, dres, &rn2)) return -1;
         rel = sqrt(rn2) / bnorm;
+        if (!std::isfinite(rn2)) { finite = false; break; }
         if (rel <= rtol || total >= maxit || rn2 == 0.0) break;
         int it = 0;
         double est = 0;
@@ -1413,5 +1419,15 @@ extern "C" int vfs_poisson_solve(vfs_ctx *c, const double *b, double *phi, doubl
     if (vfs_periodic_sca(c, phi)) return -1;
     if (iters) *iters = total;
     if (relres) *relres = rel;
+    if (!finite) {
+        vfs_set_error("vfs_poisson_solve: the residual became non-finite after " + std::to_string(total) +
+                      " iterations");
+        return -6;
+    }
+    if (!(rel <= rtol)) {
+        vfs_set_error("vfs_poisson_solve: not converged, relres " + std::to_string(rel) + " > rtol " +
+                      std::to_string(rtol) + " after " + std::to_string(total) + " iterations");
+        return 1;
+    }
     return 0;
 }

@@ -33,12 +33,23 @@ def lib():
 
 
 class VfsError(RuntimeError):
-    pass
+    def __init__(self, msg, rc=-1):
+        super().__init__(msg)
+        self.rc = rc
+
+
+class VfsNotConverged(VfsError):
+    """vfs_poisson_solve stopped at maxit above rtol (return code 1); the
+    solution array holds the last iterate."""
+
+    def __init__(self, msg, iters, relres):
+        super().__init__(msg, 1)
+        self.iters, self.relres = iters, relres
 
 
 def _ck(rc, what):
     if rc != 0:
-        raise VfsError(f"{what} failed ({rc}): {lib().vfs_last_error().decode()}")
+        raise VfsError(f"{what} failed ({rc}): {lib().vfs_last_error().decode()}", rc)
 
 
 class Config(C.Structure):
@@ -390,9 +401,14 @@ class Ctx:
         _ck(lib().vfs_poisson_rhs(self.h, ucont.ptr, b.ptr), "vfs_poisson_rhs")
 
     def poisson_solve(self, b, phi, rtol=1e-8, precision=0, maxit=500):
+        """-> (iterations, relres); raises VfsNotConverged when maxit is
+        reached above rtol and VfsError on a non-finite residual."""
         it, rel = C.c_int(), C.c_double()
-        _ck(lib().vfs_poisson_solve(self.h, b.ptr, phi.ptr, C.c_double(rtol), C.c_int(precision), C.c_int(maxit),
-                                    C.byref(it), C.byref(rel)), "vfs_poisson_solve")
+        rc = lib().vfs_poisson_solve(self.h, b.ptr, phi.ptr, C.c_double(rtol), C.c_int(precision), C.c_int(maxit),
+                                     C.byref(it), C.byref(rel))
+        if rc == 1:
+            raise VfsNotConverged(f"vfs_poisson_solve: {lib().vfs_last_error().decode()}", it.value, rel.value)
+        _ck(rc, "vfs_poisson_solve")
         return it.value, rel.value
 
     def update_pressure(self, p, phi):
