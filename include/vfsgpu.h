@@ -41,7 +41,8 @@ typedef struct {
     int immersed;        /* -imm: 0 no IB, 1 CURVIB masks              */
     int les;             /* -les 1: constant-coefficient Smagorinsky    */
     int second_order;    /* -second_order                              */
-    int dpdz_set;        /* k-periodic forcing from mean_pressure_gradient */
+    int dpdz_set;        /* k-periodic forcing from mean_pressure_gradient; 0: the
+                            flux forcing (vfs_calc_k_flux, vfs_set_inlet_flux) */
     int weno;            /* -weno 1: WENO3 convective face values (level.c:1734) */
     int inviscid;        /* -inv: WENO3 convection, 2nd-order divergence, no
                             viscous term (momentum.c:757, 1582, 1621) */
@@ -287,7 +288,20 @@ int vfs_ib_averages(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node 
                     const vfs_ib_options *opt, const double *usum, const double *u1sum, const double *u2sum, double N,
                     double *mean_shear, double *rs1, double *rs2, double *rs3);
 
-/* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts). */
+/* The flux forcing of k-periodic channels without dpdz_set (momentum.c:
+ * 387-398): Pressure_Gradient adds dz (mean_k_flux - inlet_flux) / dt /
+ * mean_k_area to the k-face gradient.  vfs_set_inlet_flux sets -flux
+ * (main.c:198, 2067; the reference's default -1).  vfs_calc_k_flux is
+ * Calc_k_Flux (rhs2.c:1562-1645): the fluid k flux of each plane averaged
+ * over the planes, with the mean fluid |zet| of the planes of
+ * Calc_Inlet_Area (bcs.c:254-278); both stay on the device for
+ * Pressure_Gradient, and are returned when the host pointers are not NULL
+ * (then the stream is synchronized).  Each plane is summed in a fixed tree
+ * order, the planes in k order. */
+int vfs_set_inlet_flux(vfs_ctx *ctx, double inlet_flux);
+int vfs_calc_k_flux(vfs_ctx *ctx, const double *ucont, double *mean_k_flux, double *mean_k_area);
+/* Pressure_Gradient (momentum.c:197-440). p: in/out (periodic ghosts).  A
+ * k-periodic context without dpdz_set needs vfs_calc_k_flux first. */
 int vfs_pressure_gradient(vfs_ctx *ctx, double *p, double *dP);
 /* IB_BC (momentum.c:1982-2195). ucont in/out, ucat in/out. */
 int vfs_ib_bc(vfs_ctx *ctx, double *ucont, double *ucat);
@@ -308,6 +322,33 @@ int vfs_formfunction_snes(vfs_ctx *ctx, const double *U, const double *U_o,
  * rhs_scratch receives FF2(U) - dP. */
 int vfs_rk_stage(vfs_ctx *ctx, const double *u0, double *ucont, double *ucat,
                  const double *nu_t, const double *dP, double alfa, double *rhs_scratch);
+/* Boundary work of RungeKutta around its stages (rhs.c:1604-1614,
+ * 1685-1686).  Pointers the configuration does not need may be NULL. */
+typedef struct {
+    const vfs_inflow_options *inflow;   /* inlet at k = 0 (bctype[4] = 5): InflowFlux */
+    const double *coor;                 /* node coordinates (the inflow profile's y) */
+    const double *uin_plane;            /* inletprofile 0: mx*my speeds, else NULL */
+    double *ubcs;                       /* 3P boundary velocity, in/out, with an open k end */
+    const vfs_ib_surface *surf;         /* immersed body: ibm_interpolation_advanced first */
+    const vfs_ib_node *nodes;           /* its node list (vfs_ib_search) */
+    int n_nodes;
+    int add_body_force;                 /* 1: each stage also adds the context's body force
+                                           (vfs_set_body_force); the reference adds F_eul only
+                                           in FormFunction_SNES (momentum.c:2288), so 0 is the
+                                           reference's explicit step */
+    const vfs_ib_options *ib;           /* NULL: the defaults of vfs_ib_interpolate */
+    double *p, *ustar;                  /* P fields the reconstruction updates */
+} vfs_rk_bc;
+/* RungeKutta (rhs.c:1578-1700): InflowFlux, FormBCS's outlet branch with
+ * outflow_scale = 0 and the IB reconstruction, then four stages
+ * (alfa = 1/4, 1/3, 1/2, 1; U0 = the flux after the boundary work, returned
+ * in u0), then FormBCS's outlet branch with outflow_scale = 1.  ucont, ucat
+ * in/out; u0, rhs: 3P scratch (rhs ends holding the last stage's FF2 - dP);
+ * *flux_in / *flux_out (host, may be NULL): the inflow and the outflow
+ * before the final correction.  bc may be NULL for a periodic, body-free
+ * context.  Synchronizes the stream when a boundary routine runs. */
+int vfs_runge_kutta(vfs_ctx *ctx, const vfs_rk_bc *bc, double *ucont, double *ucat, const double *nu_t,
+                    const double *dP, double *u0, double *rhs, double *flux_in, double *flux_out);
 /* Calc_Minimum_dt (momentum.c:104-195): CFL-1 time step. */
 int vfs_min_dt(vfs_ctx *ctx, const double *ucont, double *dt_out);
 

@@ -1138,10 +1138,40 @@ int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nve
 int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          double *p, double *dp)
 {
+    if (c->k_periodic && !c->dpdz_set) return -5; /* the flux forcing needs vo_pressure_gradient_kf */
+    return vo_pressure_gradient_kf(c, g, nvert, p, dp, 0.0, -1.0, 1.0);
+}
+
+/* rhs2.c:1562-1645 Calc_k_Flux (no levelset): the fluid (nvert(k-1) +
+ * nvert(k) < 0.1) k flux summed over k = 1..mz-2, j, i, divided by mz-2; and
+ * the mean over the planes of their fluid |zet| (bcs.c:254-278) */
+int vo_calc_k_flux(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucont, double *mean_k_flux,
+                   double *mean_k_area)
+{
+    DIMS;
+    double f = 0, a = 0;
+    for (int k = 1; k < mz - 1; k++) {
+        double ak = 0;
+        for (int j = 1; j < my - 1; j++)
+            for (int i = 1; i < mx - 1; i++)
+                if (nvert[IX(k - 1, j, i)] + nvert[IX(k, j, i)] < 0.1) {
+                    f += ucont[3 * IX(k, j, i) + 2];
+                    const double *z = g->zet + 3 * IX(k, j, i);
+                    ak += sqrt(z[0] * z[0] + z[1] * z[1] + z[2] * z[2]);
+                }
+        a += ak;
+    }
+    *mean_k_flux = f / (mz - 2);
+    *mean_k_area = a / (double)(mz - 2);
+    return 0;
+}
+
+int vo_pressure_gradient_kf(const vo_cfg *c, const vo_geom *g, const double *nvert, double *p, double *dp,
+                            double mean_k_flux, double inlet_flux, double mean_k_area)
+{
     DIMS;
     int rc = check_cfg(c);
     if (rc) return rc;
-    if (c->k_periodic && !c->dpdz_set) return -5; /* flux forcing not restated */
     per_copy_sca(c, p);
     size_t P = (size_t)mx * my * mz;
     memset(dp, 0, 3 * P * sizeof(double));
@@ -1206,9 +1236,10 @@ int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
                 else dpde = (PP(k, j + 1, i) - PP(k, j - 1, i) + PP(k + 1, j + 1, i) - PP(k + 1, j - 1, i)) * 0.25;
                 if ((k == 0 || k == mz - 2) && kp) dpdz = PP(1, j, i) - PP(k, j, i);
                 else dpdz = (PP(k + 1, j, i) - PP(k, j, i));
-                if (kp) {
+                if (kp) { /* momentum.c:382-398 */
                     double dzz = 1. / g->kaj[id] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
-                    dpdz += c->mean_pressure_gradient * dzz;
+                    if (c->dpdz_set) dpdz += c->mean_pressure_gradient * dzz;
+                    else dpdz += dzz * (mean_k_flux - inlet_flux) / c->dt / mean_k_area;
                 }
                 VZ(dp, id) = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * g->kaj[id] / r3;
 
@@ -3573,5 +3604,56 @@ int vo_ib_averages(const vo_cfg *c, const vo_geom *g, const vo_ib_surface *s, co
             rs3[e] /= (double)cnt[e];
         }
     free(cnt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* rhs.c:1578-1700 RungeKutta: the boundary work before the stages           */
+/* (1604-1614), the four stages (1633-1662) and FormBCS with                 */
+/* outflow_scale = 1 after them (1685-1686)                                  */
+/* ------------------------------------------------------------------------ */
+int vo_runge_kutta(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor,
+                   const vo_inflow_opt *inflow, const double *uin_plane, double *ubcs, const vo_ib_surface *s,
+                   const vo_ib_node *nodes, int n_nodes, const vo_ib_opt *ib, double *p, double *ustar,
+                   const double *f_eul, double *ucont, double *ucat, const double *nu_t, const double *dP, double *u0,
+                   double *rhs, double *flux_in, double *flux_out)
+{
+    DIMS;
+    const size_t P = (size_t)mx * my * mz;
+    const int inlet = !c->k_periodic && c->bctype[4] == 5, outlet = !c->k_periodic && c->bctype[5] == 4;
+    double fin = 0, fout = 0, area = 0;
+    int rc = 0;
+    if ((inlet || outlet) && !ubcs) return -2;
+    if (inlet && (rc = vo_inflow_flux(c, g, nvert, coor, inflow, uin_plane, ucont, ucat, ubcs, &fin))) return rc;
+    if (outlet && (rc = vo_outlet_correction(c, g, nvert, fin, 0, ucat, ucont, ubcs, &fout, &area))) return rc;
+    if (c->immersed && s) {
+        /* the element statistics the reconstruction also forms (not used here) */
+        const vo_ib_opt dflt = {0, 0, 0.0, 0.0};
+        int *cnt = calloc((size_t)s->n_elmt, sizeof(int));
+        double *sh = calloc(2 * (size_t)s->n_elmt, sizeof(double));
+        if (!cnt || !sh) { free(cnt); free(sh); return -1; }
+        rc = vo_ib_interpolate_opt(c, g, nvert, s, nodes, n_nodes, ib ? ib : &dflt, ucont, ucat, p, ustar, cnt, sh,
+                                   sh + s->n_elmt);
+        free(cnt);
+        free(sh);
+        if (rc) return rc;
+    }
+    memcpy(u0, ucont, 3 * P * sizeof(double));
+    const double alfa[4] = {0.25, 1 / 3., 0.5, 1.};
+    for (int st = 0; st < 4; st++) {
+        if ((rc = vo_contra2cart(c, g, nvert, ucont, ucat))) return rc;
+        memset(rhs, 0, 3 * P * sizeof(double));
+        if ((rc = vo_formfunction2(c, g, nvert, ucont, ucat, nu_t, rhs, 1.0))) return rc;
+        const double a = alfa[st] * c->dt * c->st;
+        for (size_t n = 0; n < 3 * P; n++) {
+            rhs[n] += -1 * dP[n];
+            if (f_eul) rhs[n] += 1. * f_eul[n];
+            ucont[n] = a * rhs[n] + u0[n];
+        }
+        if ((rc = vo_contra2cart(c, g, nvert, ucont, ucat))) return rc;
+    }
+    if (outlet && (rc = vo_outlet_correction(c, g, nvert, fin, 1, ucat, ucont, ubcs, &fout, &area))) return rc;
+    if (flux_in) *flux_in = fin;
+    if (flux_out) *flux_out = fout;
     return 0;
 }

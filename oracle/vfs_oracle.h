@@ -123,6 +123,15 @@ int vo_dsm_smooth(const vo_cfg *c, const vo_geom *g, const double *nvert, const 
 int vo_pressure_gradient(const vo_cfg *c, const vo_geom *g, const double *nvert,
                          double *p, double *dp);
 
+/* the same with the flux forcing of a k-periodic context without
+ * dpdz_set (momentum.c:387-398): dz (mean_k_flux - inlet_flux) / dt /
+ * mean_k_area added to the k-face gradient */
+int vo_pressure_gradient_kf(const vo_cfg *c, const vo_geom *g, const double *nvert, double *p, double *dp,
+                            double mean_k_flux, double inlet_flux, double mean_k_area);
+/* rhs2.c:1562-1645 Calc_k_Flux and the mean plane area of bcs.c:254-278 */
+int vo_calc_k_flux(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucont, double *mean_k_flux,
+                   double *mean_k_area);
+
 /* momentum.c:1982-2195 IB_BC. ucont in/out, ucat in/out (periodic ghosts). */
 int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert,
              double *ucont, double *ucat);
@@ -318,6 +327,18 @@ int vo_outflow_flux(const vo_cfg *c, const double *nvert, const double *ucont, d
  * outflow_scale */
 int vo_outlet_correction(const vo_cfg *c, const vo_geom *g, const double *nvert, double flux_in, int scale,
                          const double *ucat, double *ucont, double *ubcs, double *flux_out, double *area);
+
+/* rhs.c:1578-1700 RungeKutta with its boundary work: InflowFlux (inlet at
+ * k = 0), FormBCS's outlet branch with outflow_scale 0, the IB
+ * reconstruction (when s is given), the four stages from U0 = the flux after
+ * that work (returned in u0), and FormBCS's outlet branch with
+ * outflow_scale 1.  f_eul (NULL: none) is added in every stage.  ubcs: 3P,
+ * needed with an open k end; flux_in / flux_out may be NULL. */
+int vo_runge_kutta(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *coor,
+                   const vo_inflow_opt *inflow, const double *uin_plane, double *ubcs, const vo_ib_surface *s,
+                   const vo_ib_node *nodes, int n_nodes, const vo_ib_opt *ib, double *p, double *ustar,
+                   const double *f_eul, double *ucont, double *ucat, const double *nu_t, const double *dP, double *u0,
+                   double *rhs, double *flux_in, double *flux_out);
 
 #ifdef __cplusplus
 }

@@ -317,8 +317,37 @@ class Oracle:
                    ptr(ustar), ptr(cnt), ptr(sh), ptr(pr))
         return cnt, sh, pr
 
-    def pressure_gradient(self, p, dp):
-        self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
+    def runge_kutta(self, ucont, ucat, nu_t, dP, coor=None, inflow=None, uin_plane=None, ubcs=None, body=None,
+                    nodes=None, ib=None, p=None, ustar=None, f_eul=None):
+        """vo_runge_kutta (rhs.c:1578-1700): ucont, ucat (and ubcs, p,
+        ustar) in/out -> (u0, rhs, flux_in, flux_out).  inflow: a
+        VoInflowOpt; ib: a VoIbOpt."""
+        u0, rhs = self.vec(), self.vec()
+        fi, fo = C.c_double(), C.c_double()
+        nodes = None if nodes is None else np.ascontiguousarray(nodes)
+        self._call("vo_runge_kutta", C.byref(self.geom), ptr(self.nvert), None if coor is None else ptr(coor),
+                   None if inflow is None else C.byref(inflow), None if uin_plane is None else ptr(uin_plane),
+                   None if ubcs is None else ptr(ubcs), None if body is None else C.byref(body.c),
+                   None if nodes is None else nodes.ctypes.data_as(C.c_void_p),
+                   C.c_int(0 if nodes is None else len(nodes)), None if ib is None else C.byref(ib),
+                   None if p is None else ptr(p), None if ustar is None else ptr(ustar),
+                   None if f_eul is None else ptr(f_eul), ptr(ucont), ptr(ucat), ptr(nu_t), ptr(dP), ptr(u0), ptr(rhs),
+                   C.byref(fi), C.byref(fo))
+        return u0, rhs, fi.value, fo.value
+
+    def pressure_gradient(self, p, dp, kflux=None, inlet_flux=-1.0):
+        """kflux: (mean_k_flux, mean_k_area) from calc_k_flux for the flux
+        forcing of a k-periodic context without dpdz_set."""
+        if kflux is None:
+            self._call("vo_pressure_gradient", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp))
+        else:
+            self._call("vo_pressure_gradient_kf", C.byref(self.geom), ptr(self.nvert), ptr(p), ptr(dp),
+                       C.c_double(kflux[0]), C.c_double(inlet_flux), C.c_double(kflux[1]))
+
+    def calc_k_flux(self, ucont):
+        f, a = C.c_double(), C.c_double()
+        self._call("vo_calc_k_flux", C.byref(self.geom), ptr(self.nvert), ptr(ucont), C.byref(f), C.byref(a))
+        return f.value, a.value
 
     def ib_bc(self, ucont, ucat):
         self._call("vo_ib_bc", C.byref(self.geom), ptr(self.nvert), ptr(ucont), ptr(ucat))

@@ -1,0 +1,90 @@
+"""The mixed-precision Poisson case C3: the C2 generator (tanh-stretched wavy
+channel, sphere) at 512x256x256 with a random right-hand side of zero
+aj-weighted mean (seed 1003) and the seeded k^-5/3 velocity.
+
+- At a reduced size the FP64 and the mixed solve are compared with the
+  oracle's 1e-13 FP64 solve of the same right-hand side.
+- At the full size both solves reach ||b - A phi|| / ||b|| <= 1e-8 in the
+  FP64 operator; the two solutions agree, and the velocity projected with
+  either differs by at most 1e-9 relative with max |div U| at the level the
+  residual allows.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from cases import gpu_case, oracle_case
+from conftest import rel_err
+from vfsgpu import workloads as W
+import vfsgpu as V
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_c3_reduced_against_the_oracle():
+    say("[c3] C3 generator at 48x24x24")
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C3", (48, 24, 24))
+    ctx = gpu_case(cfg, spec, coor)
+    ctx.poisson_setup()
+    b = W.poisson_rhs_random(cfg, spec["seed"], o.nvert, o.g["aj"])
+    phi_o = o.sca()
+    it_o, rel_o = o.poisson_solve(b.copy(), phi_o, rtol=1e-13)
+    assert rel_o <= 1e-13
+    db = V.DeviceArray.from_host(b)
+    for prec in (0, 1):
+        phi = ctx.sca()
+        it, rel = ctx.poisson_solve(db, phi, rtol=1e-10, precision=prec)
+        say(f"  precision {prec}: {it} iterations, relres {rel:.2e}")
+        assert rel <= 1e-10
+        assert rel_err(phi.host(), phi_o) < 1e-7
+
+
+def test_c3_full_size():
+    N = W.BASE["C3"]["N"]
+    say(f"[c3] C3 {N}")
+    cfg, spec = W.make_config("C3", N)
+    ctx = V.Ctx(cfg)
+    ctx.build_metrics(V.DeviceArray.from_host(W.grid(spec)))
+    ctx.nvert_sphere(*W.SPHERE)
+    n = ctx.poisson_setup()
+    say(f"  {n} unknowns")
+    nvert = ctx.nvert().host()
+    aj = ctx.metric("aj").host()
+    db = V.DeviceArray.from_host(W.poisson_rhs_random(cfg, spec["seed"], nvert, aj))
+    sols = []
+    for prec in (0, 1):
+        phi = ctx.sca()
+        it, rel = ctx.poisson_solve(db, phi, rtol=1e-8, precision=prec, maxit=1000)
+        say(f"  random b, precision {prec}: {it} iterations, relres {rel:.2e}")
+        assert rel <= 1e-8
+        sols.append(phi.host())
+    d = np.linalg.norm(sols[1] - sols[0]) / np.linalg.norm(sols[0])
+    say(f"  |phi_mixed - phi_fp64| / |phi_fp64| = {d:.2e}")
+    assert d < 1e-6
+    # the seeded velocity, projected with each solve
+    metrics = {m: ctx.metric(m).host() for m in ("cent", "icsi", "jeta", "kzet")}
+    ucat, ucont = W.fields(cfg, spec, metrics, nvert)
+    del metrics
+    us = []
+    for prec in (0, 1):
+        uc = V.DeviceArray.from_host(ucont)
+        b, phi = ctx.sca(), ctx.sca()
+        ctx.volume_flux(uc)
+        ctx.poisson_rhs(uc, b)
+        it, rel = ctx.poisson_solve(b, phi, rtol=1e-8, precision=prec, maxit=1000)
+        ctx.projection(phi, uc)
+        m0 = ctx.divergence(V.DeviceArray.from_host(ucont))
+        m1 = ctx.divergence(uc)
+        say(f"  velocity, precision {prec}: {it} iterations, relres {rel:.2e}, max|div U| {m0:.2e} -> {m1:.2e}")
+        assert rel <= 1e-8 and m1 <= 1e-4 * m0
+        us.append(uc.host())
+    du = np.linalg.norm(us[1] - us[0]) / np.linalg.norm(us[0])
+    say(f"  |U_mixed - U_fp64| / |U_fp64| = {du:.2e}")
+    assert du <= 1e-9
+    ctx.close()

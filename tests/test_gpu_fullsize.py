@@ -113,15 +113,23 @@ def test_poisson_full_size(big):
     a, m = sols[0].host(), sols[1].host()
     diff = np.linalg.norm(m - a) / np.linalg.norm(a)
     say(f"  |phi_mixed - phi_fp64| / |phi_fp64| = {diff:.2e}")
-    # both within 1e-10 relative residual: the solutions agree to cond * 1e-10
-    assert diff < 1e-5
-    u2 = V.DeviceArray.from_host(duc.host())
-    ctx.projection(sols[0], u2)
-    maxdiv = ctx.divergence(u2)
-    m0 = ctx.divergence(duc)
-    say(f"  max|div U| before {m0:.3e} after {maxdiv:.3e}")
-    # |r|_inf <= |r|_2 <= 1e-10 |b|_2 <= 1e-10 sqrt(n) |b|_inf
-    assert maxdiv < 1e-5 * m0
+    # both within 1e-10 relative residual: A (phi_m - phi_a) = r_a - r_m is at
+    # most 2e-10 |b|; the error this leaves in phi is far below the bound
+    # (measured about 1e-11; 3e-9 for solves to 1e-8)
+    assert diff < 1e-8
+    us = []
+    for prec in (0, 1):
+        u2 = V.DeviceArray.from_host(duc.host())
+        ctx.projection(sols[prec], u2)
+        us.append(u2)
+        maxdiv = ctx.divergence(u2)
+        m0 = ctx.divergence(duc)
+        say(f"  precision {prec}: max|div U| before {m0:.3e} after {maxdiv:.3e}")
+        # |r|_inf <= |r|_2 <= 1e-10 |b|_2 <= 1e-10 sqrt(n) |b|_inf
+        assert maxdiv < 1e-5 * m0
+    du = np.linalg.norm(us[1].host() - us[0].host()) / np.linalg.norm(us[0].host())
+    say(f"  |U_mixed - U_fp64| / |U_fp64| after projection = {du:.2e}")
+    assert du <= 1e-9
 
 
 def test_derivatives_full_size(big):

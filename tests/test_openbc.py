@@ -1,6 +1,9 @@
 """Open k boundaries: Calc_Inlet_Area (bcs.c:216-328), InflowFlux at a k = 0
 inlet (bcs.c:329-706), OutflowFlux at a k = mz-1 outlet (bcs.c:801-870) and
-the outlet branch of FormBCS (bcs.c:2285-2418).
+the outlet branch of FormBCS (bcs.c:2285-2418).  The device sums each plane
+in a fixed tree order and the oracle in the reference's j, i loop, so sums
+and what depends on them are compared to 1e-14 relative, the face values
+themselves bit for bit.
 
 CPU tests pin the oracle:
   - Contra2Cart_single (rhs.c:34-55) against a linear solve with the rows
@@ -18,6 +21,7 @@ GPU tests compare the HIP path with the oracle.
 import numpy as np
 import pytest
 
+from conftest import rel_err
 from oracle_py import Oracle, lib, ptr
 from vfsgpu import workloads as W
 import vfsgpu as V
@@ -122,7 +126,10 @@ def test_gpu_open_boundaries(profile, sphere):
     ctx.set_nvert(V.DeviceArray.from_host(o.nvert))
     ka, mean, inlet = ctx.inlet_area()
     oka, omean, oinlet = o.inlet_area()
-    assert np.array_equal(ka.host(), oka) and mean == omean and inlet == oinlet
+    # the device sums a plane in a fixed tree order, the oracle in the
+    # reference's j, i loop: equal up to the rounding of the association
+    assert rel_err(ka.host(), oka) <= 1e-14 and abs(mean - omean) <= 1e-14 * omean
+    assert abs(inlet - oinlet) <= 1e-14 * oinlet
     plane = np.linspace(0.2, 1.3, cfg["mx"] * cfg["my"]) if profile == 0 else None
     kw = dict(inletprofile=profile, inlet_flux=2.5 if profile == 2 else -1.0, inlet_area=oinlet)
     d_uc, d_u, d_b = V.DeviceArray.from_host(ucont), V.DeviceArray.from_host(ucat), ctx.vec()
@@ -132,13 +139,15 @@ def test_gpu_open_boundaries(profile, sphere):
     tol = 1e-15 if profile == 10 else 0.0   # pow of the power law: the device's and the C library's
     for g, h in ((d_uc.host(), uc), (d_u.host(), u), (d_b.host(), b)):
         assert np.abs(g - h).max() <= tol * np.abs(h).max()
-    assert abs(f - fo) <= tol * abs(fo)
-    assert ctx.outflow_flux(d_uc) == o.outflow_flux(uc)
+    # plane sums: fixed tree order on the device (see above)
+    assert abs(f - fo) <= 1e-14 * abs(fo)
+    assert abs(ctx.outflow_flux(d_uc) - o.outflow_flux(uc)) <= 1e-14 * np.abs(uc).max() * cfg["mx"] * cfg["my"]
     g_out = ctx.outlet_correction(fo, d_u, d_uc, d_b)
     o_out = o.outlet_correction(fo, u, uc, b)
-    assert g_out == o_out
+    assert abs(g_out[1] - o_out[1]) <= 1e-14 * o_out[1]
+    assert abs(g_out[0] - o_out[0]) <= 1e-14 * np.abs(uc).max() * cfg["mx"] * cfg["my"]
     for g, h in ((d_uc.host(), uc), (d_b.host(), b)):
-        assert np.abs(g - h).max() <= tol * np.abs(h).max()
+        assert np.abs(g - h).max() <= 1e-14 * np.abs(h).max()
 
 
 # ---------------------------------------------------------------------------

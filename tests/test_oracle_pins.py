@@ -52,7 +52,7 @@ M_AFF = np.array([[0.100, 0.031, -0.012],
 # affine grids
 # ---------------------------------------------------------------------------
 class Affine:
-    def __init__(self, N=(12, 11, 10), M=M_AFF, per=(0, 0, 0), ren=100.0, immersed=0, les=0, mpg=0.0):
+    def __init__(self, N=(12, 11, 10), M=M_AFF, per=(0, 0, 0), ren=100.0, immersed=0, les=0, mpg=0.0, dpdz_set=1):
         Ni, Nj, Nk = N
         self.M = np.asarray(M, float)
         self.mx, self.my, self.mz = Ni + 2, Nj + 2, Nk + 2
@@ -61,7 +61,7 @@ class Affine:
         coor = self.idx @ self.M.T
         bct = [7 if per[d // 2] else 1 for d in range(6)]
         self.cfg = dict(mx=self.mx, my=self.my, mz=self.mz, i_periodic=per[0], j_periodic=per[1], k_periodic=per[2],
-                        bctype=bct, immersed=immersed, les=les, second_order=0, dpdz_set=1, weno=0, inviscid=0,
+                        bctype=bct, immersed=immersed, les=les, second_order=0, dpdz_set=dpdz_set, weno=0, inviscid=0,
                         ren=ren, dt=1e-3, st=1.0, mean_pressure_gradient=mpg)
         self.o = Oracle(self.cfg, np.ascontiguousarray(coor.ravel()))
         e = [self.M[:, m] for m in range(3)]
@@ -123,15 +123,16 @@ def _sphere_mask(aff, r=0.2):
 def _unmasked(aff, nv, d):
     """Faces d of cell (k,j,i) that Formfunction_2 and Pressure_Gradient
     leave unmasked: interior cell, nvert sum <= 0.1, not the last cell of a
-    wall direction (momentum.c:1795-1803, 415-425)."""
+    non-periodic direction (momentum.c:1795-1803, 415-425)."""
     m = np.zeros(nv.shape, bool)
     m[1:-1, 1:-1, 1:-1] = True
     nb = np.roll(nv, -1, axis=2 - d)
     m &= (nv + nb) <= 0.1
     n = [aff.mx, aff.my, aff.mz][d]
-    sl = [slice(None)] * 3
-    sl[2 - d] = n - 2
-    m[tuple(sl)] = False
+    if not aff.cfg[("i_periodic", "j_periodic", "k_periodic")[d]]:
+        sl = [slice(None)] * 3
+        sl[2 - d] = n - 2
+        m[tuple(sl)] = False
     return m
 
 
@@ -185,6 +186,41 @@ def pin_mean_pressure_gradient():
     assert np.abs(b[..., 2] - g[2] * dx * dy).max() <= 1e-12 * abs(g[2]) * dx * dy
     assert np.abs(a[..., 2] - (g[2] - 1.7) * dx * dy).max() <= 1e-12 * dx * dy
     assert np.array_equal(a[..., :2], b[..., :2])
+
+
+def pin_flux_forcing():
+    """k-periodic uniform grid with a body (solid cells inside an IB layer),
+    dpdz_set off: Calc_k_Flux is the fluid k flux (nvert(k-1) + nvert(k) <
+    0.1) averaged over the planes and the mean area is dx dy times the fluid
+    faces per plane (rhs2.c:1606-1627, bcs.c:262-278); the forcing adds
+    (F - inlet_flux) dx dy / (dt A) to dP.z on every unmasked k face and
+    nothing to the other components (momentum.c:394-397)."""
+    Ni, Nj, Nk = 8, 9, 10
+    dx, dy, dz = 0.3, 0.2, 0.25
+    aff = Affine((Ni, Nj, Nk), np.diag([dx, dy, dz]), per=(1, 0, 1), immersed=1, dpdz_set=0)
+    o = aff.o
+    nv = np.zeros((aff.mz, aff.my, aff.mx))
+    nv[3:8, 3:7, 2:6] = 1.0
+    nv[4:7, 4:6, 3:5] = 3.0
+    o.nvert[:] = nv.ravel()
+    uc = np.random.default_rng(18).uniform(-1, 1, (aff.mz, aff.my, aff.mx, 3))
+    F, A = o.calc_k_flux(np.ascontiguousarray(uc.ravel()))
+    fluid = (np.roll(nv, 1, axis=0) + nv)[1:-1, 1:-1, 1:-1] < 0.1
+    F_ref = uc[1:-1, 1:-1, 1:-1, 2][fluid].sum() / Nk
+    A_ref = dx * dy * fluid.sum() / Nk
+    assert abs(F - F_ref) <= 1e-13 * np.abs(uc[..., 2]).sum() and abs(A - A_ref) <= 1e-13 * A_ref
+    p = np.zeros(o.P)
+    dp = {}
+    for fin in (F, F - 0.37):
+        d = o.vec()
+        o.pressure_gradient(p.copy(), d, kflux=(F, A), inlet_flux=fin)
+        dp[fin] = d.reshape(aff.mz, aff.my, aff.mx, 3)
+    assert not dp[F].any()   # at the target flux the forcing vanishes
+    D = dp[F - 0.37]
+    m = _unmasked(aff, nv, 2)
+    ref = 0.37 * dx * dy / (aff.cfg["dt"] * A)
+    assert m.sum() > 100 and np.abs(D[..., 2][m] - ref).max() <= 1e-13 * ref
+    assert not D[..., 2][~m].any() and not D[..., :2].any()
 
 
 def pin_ib_bc_face_rule():
@@ -510,7 +546,8 @@ def pin_taylor_green():
     assert e[0] / e[1] > 3.6 and e[1] < 1e-3, e
 
 
-PINS = {f.__name__: f for f in (pin_pressure_gradient_affine_with_body, pin_mean_pressure_gradient, pin_ib_bc_face_rule,
+PINS = {f.__name__: f for f in (pin_pressure_gradient_affine_with_body, pin_mean_pressure_gradient, pin_flux_forcing,
+                                pin_ib_bc_face_rule,
                                 pin_min_dt, pin_ff2_affine_convective, pin_ff2_affine_viscous,
                                 pin_ff2_body_uniform_stream, pin_ff2_body_viscous_linear, pin_rotation_invariance,
                                 pin_taylor_green)}
@@ -577,6 +614,17 @@ MUTANTS = [
     ("mean pressure gradient with the wrong sign",
      [("dpdz += c->mean_pressure_gradient * dzz;", "dpdz -= c->mean_pressure_gradient * dzz;", 1)],
      ["pin_mean_pressure_gradient"]),
+    ("flux forcing with the wrong sign",
+     [("dpdz += dzz * (mean_k_flux - inlet_flux) / c->dt / mean_k_area;",
+       "dpdz -= dzz * (mean_k_flux - inlet_flux) / c->dt / mean_k_area;", 1)],
+     ["pin_flux_forcing"]),
+    ("Calc_k_Flux not averaged over the planes",
+     [("*mean_k_flux = f / (mz - 2);", "*mean_k_flux = f;", 1)],
+     ["pin_flux_forcing"]),
+    ("Calc_k_Flux fluid test on one cell",
+     [("if (nvert[IX(k - 1, j, i)] + nvert[IX(k, j, i)] < 0.1) {\n                    f +=",
+       "if (nvert[IX(k, j, i)] < 0.1) {\n                    f +=", 1)],
+     ["pin_flux_forcing"]),
     ("IB_BC j face from the cell below",
      [("if ((int)(NV(k, j + 1, i)) == 1) {\n                        ucx = (U3(k, j + 1, i, 0) + U3(k, j, i, 0)) * 0.5;",
        "if ((int)(NV(k, j + 1, i)) == 1) {\n                        ucx = (U3(k, j - 1, i, 0) + U3(k, j, i, 0)) * 0.5;", 1)],

@@ -114,10 +114,6 @@ static int check_config(const vfs_config *cfg)
     if (cfg->immersed != 0 && cfg->immersed != 1) { vfs_set_error("immersed must be 0 or 1"); return -4; }
     if (cfg->weno != 0 && cfg->weno != 1) { vfs_set_error("weno must be 0 or 1"); return -4; }
     if (cfg->inviscid != 0 && cfg->inviscid != 1) { vfs_set_error("inviscid must be 0 or 1"); return -4; }
-    if (cfg->k_periodic && !cfg->dpdz_set) {
-        vfs_set_error("k-periodic flux forcing (inlet_flux) is not supported; set dpdz_set");
-        return -5;
-    }
     return 0;
 }
 
@@ -157,13 +153,16 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
               hipMalloc(&c->fp, 3 * P * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->snap, 3 * P * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->tmp3, 3 * P * sizeof(double)) == hipSuccess &&
-              hipMalloc(&c->red, (VFS_RED_SLOTS * VFS_RED_BLOCKS + 256) * sizeof(double)) == hipSuccess;
+              hipMalloc(&c->red, (VFS_RED_SLOTS * VFS_RED_BLOCKS + 256) * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->kflux, 4 * sizeof(double)) == hipSuccess;
     if (!ok) { vfs_ctx_destroy(c); vfs_set_error("hipMalloc scratch"); return -1; }
     hipMemset(c->nvert, 0, P * sizeof(double));
     hipMemset(c->face, 0, 18 * P * sizeof(double));
     hipMemset(c->fp, 0, 3 * P * sizeof(double));
     hipMemset(c->snap, 0, 3 * P * sizeof(double));
     hipMemset(c->tmp3, 0, 3 * P * sizeof(double));
+    hipMemset(c->kflux, 0, 4 * sizeof(double));
+    c->inlet_flux = -1.0;
     c->red_blocks = VFS_RED_BLOCKS;
     c->nvert_f32 = 1;
     if (hipDeviceSynchronize() != hipSuccess) { vfs_ctx_destroy(c); vfs_set_error("device sync"); return -1; }
@@ -182,12 +181,18 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     vfs_poisson_destroy(c);
     vfs_implicit_destroy(c);
     hipFree(c->geom_block); hipFree(c->nvert); hipFree(c->face); hipFree(c->fp);
-    hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red);
+    hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red); hipFree(c->kflux);
     if (c->ad) hipFree(c->ad);
     if (c->wf_table) hipFree(c->wf_table);
     if (c->ustar) hipFree(c->ustar);
     if (c->dsm) hipFree(c->dsm);
     delete c;
+    return 0;
+}
+
+extern "C" int vfs_set_inlet_flux(vfs_ctx *c, double inlet_flux)
+{
+    c->inlet_flux = inlet_flux;
     return 0;
 }
 

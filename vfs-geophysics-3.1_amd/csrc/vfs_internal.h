@@ -140,6 +140,9 @@ struct vfs_ctx {
     double *ustar;           // P: friction velocity of the wall-function cells
     double *dsm;             // 19*P: dynamic Smagorinsky scratch (allocated on first use)
     const double *body_force;   // F_eul added to the implicit residual (caller's array, may be null)
+    double *kflux;           // 2 doubles: mean_k_flux (Calc_k_Flux) and mean_k_area (Calc_Inlet_Area)
+    int kflux_ready;         // vfs_calc_k_flux has run
+    double inlet_flux;       // -flux (main.c:198, 2067): the target k flux of the forcing, default -1
     int red_blocks;
     Poisson *pois;
 };

@@ -167,8 +167,10 @@ extern "C" int vfs_eddy_viscosity_cs(vfs_ctx *c, const double *ucat, const doubl
 // Pressure_Gradient
 // ---------------------------------------------------------------------------
 #define IB1(x) ((int)((x) + 0.5) == 1)
+// kf: {mean_k_flux, mean_k_area} of the flux forcing (k-periodic without
+// dpdz_set), fin the target flux (-flux)
 __global__ void k_pgrad(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ p,
-                        double *__restrict__ dp)
+                        double *__restrict__ dp, const double *__restrict__ kf, double fin)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
@@ -235,9 +237,10 @@ __global__ void k_pgrad(Dims d, Geom g, const double *__restrict__ nvert, const 
     else dpde = (PP(k, j + 1, i) - PP(k, j - 1, i) + PP(k + 1, j + 1, i) - PP(k + 1, j - 1, i)) * 0.25;
     if ((k == 0 || k == mz - 2) && kp) dpdz = PP(1, j, i) - PP(k, j, i);
     else dpdz = (PP(k + 1, j, i) - PP(k, j, i));
-    if (kp) {
+    if (kp) {   // momentum.c:382-398
         double dzz = 1. / g.kaj[t] / sqrt(kz[0] * kz[0] + kz[1] * kz[1] + kz[2] * kz[2]);
-        dpdz += d.mpg * dzz;
+        if (d.dpdz_set) dpdz += d.mpg * dzz;
+        else dpdz += dzz * (kf[0] - fin) / d.dt / kf[1];
     }
     oz = (dpdc * g31_k + dpde * g32_k + dpdz * g33_k) * g.kaj[t] / r3;
 
@@ -252,8 +255,11 @@ __global__ void k_pgrad(Dims d, Geom g, const double *__restrict__ nvert, const 
 extern "C" int vfs_pressure_gradient(vfs_ctx *c, double *p, double *dP)
 {
     const Dims d = c->d;
+    VFS_REQUIRE(!d.kp || d.dpdz_set || c->kflux_ready,
+                "k-periodic flux forcing: call vfs_calc_k_flux (Calc_k_Flux) before vfs_pressure_gradient");
     if (vfs_periodic_sca(c, p)) return -1;
-    hipLaunchKernelGGL(k_pgrad, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, p, dP);
+    hipLaunchKernelGGL(k_pgrad, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, p, dP,
+                       c->kflux, c->inlet_flux);
     VFS_CHECK_LAUNCH();
     return 0;
 }

@@ -4,10 +4,15 @@
 // and the outlet branch of FormBCS (bcs.c:2285-2418), without levelset or
 // inflow fluctuations.  threshold = 0.1 (bcs.c:5).
 //
-// The face writes run one thread a face.  Every sum runs in one thread in
-// the reference's j, i order, so the fluxes and areas are the reference's
-// bit for bit (a plane of 128 x 256 faces takes some tens of microseconds).
+// The face writes run one thread a face.  Each plane sum runs in one
+// workgroup of 1024: thread t adds the faces t, t + 1024, ... of the plane
+// (rows of mx-2 faces, j-major as the reference walks them), then the
+// threads' partials meet in a fixed butterfly-and-wave order.  The order is
+// the same on every run, so the sums are deterministic; they differ from the
+// reference's single j, i loop only by the rounding of the association
+// (about 1e-16 relative).
 #include "vfs_internal.h"
+#include "vfs_reduce.h"
 
 namespace {
 
@@ -31,36 +36,80 @@ __device__ void c2c_single(const double *csi, const double *eta, const double *z
     u[2] = det2 / det;
 }
 
-// bcs.c:262-274: fluid |zet| of each k plane, one thread a plane
-__global__ void k_plane_areas(Dims d, const double *__restrict__ zet, const double *__restrict__ nvert,
-                              double *__restrict__ k_area)
+constexpr int PLANE_BS = 1024;
+
+// bcs.c:262-274: fluid |zet| of each k plane, one workgroup a plane
+__global__ void __launch_bounds__(PLANE_BS) k_plane_areas(Dims d, const double *__restrict__ zet,
+                                                          const double *__restrict__ nvert, double *__restrict__ k_area)
 {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= d.mz) return;
+    __shared__ double sh[PLANE_BS / 64];
+    const int k = blockIdx.x;
+    const long long nf = (long long)(d.mx - 2) * (d.my - 2);
     double a = 0;
     if (k >= 1 && k <= d.mz - 2)
-        for (int j = 1; j <= d.my - 2; j++)
-            for (int i = 1; i <= d.mx - 2; i++) {
-                const long long t = IX(d, k, j, i);
-                if (nvert[t - (long long)d.mx * d.my] + nvert[t] < 0.1) a += norm3(&zet[3 * t]);
-            }
-    k_area[k] = a;
+        for (long long q = threadIdx.x; q < nf; q += PLANE_BS) {
+            const int j = 1 + (int)(q / (d.mx - 2)), i = 1 + (int)(q % (d.mx - 2));
+            const long long t = IX(d, k, j, i);
+            if (nvert[t - (long long)d.mx * d.my] + nvert[t] < 0.1) a += norm3(&zet[3 * t]);
+        }
+    a = block_sum<PLANE_BS>(a, sh);
+    if (threadIdx.x == 0) k_area[k] = a;
 }
 
-// bcs.c:275-310: k_area[0] = k_area[1], the mean over 1..mz-2, and the
-// fluid area of the inlet plane (one thread)
-__global__ void k_area_finish(Dims d, const double *__restrict__ zet, const double *__restrict__ nvert,
-                              double *__restrict__ k_area, double *__restrict__ out)
+// bcs.c:275-310: k_area[0] = k_area[1], the mean over 1..mz-2 (in k order)
+// and the fluid area of the inlet plane (one workgroup)
+__global__ void __launch_bounds__(PLANE_BS) k_area_finish(Dims d, const double *__restrict__ zet,
+                                                          const double *__restrict__ nvert,
+                                                          double *__restrict__ k_area, double *__restrict__ out)
 {
-    k_area[0] = k_area[1];
-    double m = 0;
-    for (int k = 1; k <= d.mz - 2; k++) m += k_area[k];
-    out[0] = m / (double)(d.mz - 2);
+    __shared__ double sh[PLANE_BS / 64];
+    const long long nf = (long long)(d.mx - 2) * (d.my - 2);
     double a = 0;
-    for (int j = 1; j <= d.my - 2; j++)
-        for (int i = 1; i <= d.mx - 2; i++)
-            if (nvert[IX(d, 1, j, i)] < THR) a += norm3(&zet[3 * IX(d, 0, j, i)]);
-    out[1] = a;
+    for (long long q = threadIdx.x; q < nf; q += PLANE_BS) {
+        const int j = 1 + (int)(q / (d.mx - 2)), i = 1 + (int)(q % (d.mx - 2));
+        if (nvert[IX(d, 1, j, i)] < THR) a += norm3(&zet[3 * IX(d, 0, j, i)]);
+    }
+    a = block_sum<PLANE_BS>(a, sh);
+    if (threadIdx.x == 0) {
+        k_area[0] = k_area[1];
+        double m = 0;
+        for (int k = 1; k <= d.mz - 2; k++) m += k_area[k];
+        out[0] = m / (double)(d.mz - 2);
+        out[1] = a;
+    }
+}
+
+// rhs2.c:1606-1619: the fluid flux through the k faces of each plane
+// (fluid: nvert(k-1) + nvert(k) < 0.1), one workgroup a plane
+__global__ void __launch_bounds__(PLANE_BS) k_kflux_planes(Dims d, const double *__restrict__ nvert,
+                                                           const double *__restrict__ ucont, double *__restrict__ kf)
+{
+    __shared__ double sh[PLANE_BS / 64];
+    const int k = blockIdx.x;
+    const long long nf = (long long)(d.mx - 2) * (d.my - 2);
+    double f = 0;
+    if (k >= 1 && k <= d.mz - 2)
+        for (long long q = threadIdx.x; q < nf; q += PLANE_BS) {
+            const int j = 1 + (int)(q / (d.mx - 2)), i = 1 + (int)(q % (d.mx - 2));
+            const long long t = IX(d, k, j, i);
+            if (nvert[t - (long long)d.mx * d.my] + nvert[t] < 0.1) f += ucont[3 * t + 2];
+        }
+    f = block_sum<PLANE_BS>(f, sh);
+    if (threadIdx.x == 0) kf[k] = f;
+}
+
+// the plane sums over k = 1..mz-2 in k order: Flux / (mz-2) (rhs2.c:1627)
+// and the mean fluid area (bcs.c:277-278)
+__global__ void k_kflux_finish(Dims d, const double *__restrict__ kf, const double *__restrict__ ka,
+                               double *__restrict__ out)
+{
+    double f = 0, a = 0;
+    for (int k = 1; k <= d.mz - 2; k++) {
+        f += kf[k];
+        a += ka[k];
+    }
+    out[0] = f / (d.mz - 2);
+    out[1] = a / (double)(d.mz - 2);
 }
 
 // bcs.c:394-470: the inflow speed of the profiles
@@ -114,20 +163,30 @@ __global__ void k_inflow_faces(Dims d, Geom g, const double *__restrict__ nvert,
 
 // the fluid flux through a k plane of faces (kf) whose fluid test reads
 // plane kn: bcs.c:686-705 (inlet: kf = 0, kn = 1), 845-857 (outlet: kf = kn =
-// mz-2); with area, also the fluid |zet| of plane kf (bcs.c:2299-2330)
-__global__ void k_plane_flux(Dims d, const double *__restrict__ zet, const double *__restrict__ nvert,
-                             const double *__restrict__ ucont, int kf, int kn, int with_area, double *__restrict__ out)
+// mz-2); with area, also the fluid |zet| of plane kf (bcs.c:2299-2330).  One
+// workgroup of PLANE_BS.
+__global__ void __launch_bounds__(PLANE_BS) k_plane_flux(Dims d, const double *__restrict__ zet,
+                                                         const double *__restrict__ nvert,
+                                                         const double *__restrict__ ucont, int kf, int kn,
+                                                         int with_area, double *__restrict__ out)
 {
+    __shared__ double sh[PLANE_BS / 64];
+    const long long nf = (long long)(d.mx - 2) * (d.my - 2);
     double f = 0, a = 0;
-    for (int j = 1; j <= d.my - 2; j++)
-        for (int i = 1; i <= d.mx - 2; i++)
-            if (nvert[IX(d, kn, j, i)] < THR) {
-                const long long t = IX(d, kf, j, i);
-                f += ucont[3 * t + 2];
-                if (with_area) a += norm3(&zet[3 * t]);
-            }
-    out[0] = f;
-    out[1] = a;
+    for (long long q = threadIdx.x; q < nf; q += PLANE_BS) {
+        const int j = 1 + (int)(q / (d.mx - 2)), i = 1 + (int)(q % (d.mx - 2));
+        if (nvert[IX(d, kn, j, i)] < THR) {
+            const long long t = IX(d, kf, j, i);
+            f += ucont[3 * t + 2];
+            if (with_area) a += norm3(&zet[3 * t]);
+        }
+    }
+    f = block_sum<PLANE_BS>(f, sh);
+    a = block_sum<PLANE_BS>(a, sh);
+    if (threadIdx.x == 0) {
+        out[0] = f;
+        out[1] = a;
+    }
 }
 
 // bcs.c:2290-2297: ubcs of the outlet plane from the last cells
@@ -169,13 +228,33 @@ extern "C" int vfs_inlet_area(vfs_ctx *c, double *k_area, double *mean_k_area, d
     const Dims d = c->d;
     VFS_REQUIRE(!d.kp, "open k boundaries need k_periodic = 0");
     VFS_REQUIRE(k_area && mean_k_area && inlet_area, "bad arguments");
-    hipLaunchKernelGGL(k_plane_areas, dim3(grid_for(d.mz, 64)), dim3(64), 0, c->stream, d, c->g.zet, c->nvert, k_area);
-    hipLaunchKernelGGL(k_area_finish, dim3(1), dim3(1), 0, c->stream, d, c->g.zet, c->nvert, k_area, c->red);
+    hipLaunchKernelGGL(k_plane_areas, dim3(d.mz), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, k_area);
+    hipLaunchKernelGGL(k_area_finish, dim3(1), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, k_area, c->red);
     VFS_CHECK_LAUNCH();
     double h[2];
     if (read_back(c, h, 2)) return -1;
     *mean_k_area = h[0];
     *inlet_area = h[1];
+    return 0;
+}
+
+extern "C" int vfs_calc_k_flux(vfs_ctx *c, const double *ucont, double *mean_k_flux, double *mean_k_area)
+{
+    const Dims d = c->d;
+    VFS_REQUIRE(ucont, "bad arguments");
+    double *ka = c->tmp3, *kf = c->tmp3 + d.mz;   // 2 mz doubles of scratch
+    hipLaunchKernelGGL(k_plane_areas, dim3(d.mz), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, ka);
+    hipLaunchKernelGGL(k_kflux_planes, dim3(d.mz), dim3(PLANE_BS), 0, c->stream, d, c->nvert, ucont, kf);
+    hipLaunchKernelGGL(k_kflux_finish, dim3(1), dim3(1), 0, c->stream, d, kf, ka, c->kflux);
+    VFS_CHECK_LAUNCH();
+    c->kflux_ready = 1;
+    if (mean_k_flux || mean_k_area) {
+        double h[2];
+        VFS_CHECK_HIP(hipMemcpyAsync(h, c->kflux, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+        if (mean_k_flux) *mean_k_flux = h[0];
+        if (mean_k_area) *mean_k_area = h[1];
+    }
     return 0;
 }
 
@@ -192,7 +271,7 @@ extern "C" int vfs_inflow_flux(vfs_ctx *c, const vfs_inflow_options *o, const do
                 "a set inlet_flux needs the inlet area (vfs_inlet_area)");
     hipLaunchKernelGGL(k_inflow_faces, plane_grid(d), dim3(64), 0, c->stream, d, c->g, c->nvert, coor, *o, uin_plane,
                        ucont, ucat, ubcs);
-    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(1), 0, c->stream, d, c->g.zet, c->nvert, ucont, 0, 1, 0, c->red);
+    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, ucont, 0, 1, 0, c->red);
     VFS_CHECK_LAUNCH();
     double h[2];
     if (read_back(c, h, 2)) return -1;
@@ -205,7 +284,7 @@ extern "C" int vfs_outflow_flux(vfs_ctx *c, const double *ucont, double *flux_ou
     const Dims d = c->d;
     VFS_REQUIRE(!d.kp, "open k boundaries need k_periodic = 0");
     VFS_REQUIRE(ucont && flux_out, "bad arguments");
-    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(1), 0, c->stream, d, c->g.zet, c->nvert, ucont, d.mz - 2, d.mz - 2, 0,
+    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, ucont, d.mz - 2, d.mz - 2, 0,
                        c->red);
     VFS_CHECK_LAUNCH();
     double h[2];
@@ -221,7 +300,7 @@ extern "C" int vfs_outlet_correction(vfs_ctx *c, double flux_in, int scale, cons
     VFS_REQUIRE(!d.kp, "open k boundaries need k_periodic = 0");
     VFS_REQUIRE(ucat && ucont && ubcs && flux_out && area, "bad arguments");
     hipLaunchKernelGGL(k_outlet_ubcs, plane_grid(d), dim3(64), 0, c->stream, d, c->nvert, ucat, ubcs);
-    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(1), 0, c->stream, d, c->g.zet, c->nvert, ucont, d.mz - 2, d.mz - 2, 1,
+    hipLaunchKernelGGL(k_plane_flux, dim3(1), dim3(PLANE_BS), 0, c->stream, d, c->g.zet, c->nvert, ucont, d.mz - 2, d.mz - 2, 1,
                        c->red);
     if (scale)
         hipLaunchKernelGGL(k_outlet_scale, plane_grid(d), dim3(64), 0, c->stream, d, c->g.zet, c->nvert, flux_in, c->red,

@@ -94,6 +94,13 @@ def _wall_law(wallfunction, roughness):
     return 1 if roughness is None else 2
 
 
+class RkBc(C.Structure):
+    """vfs_rk_bc: the boundary work of vfs_runge_kutta."""
+    _fields_ = [("inflow", C.c_void_p), ("coor", C.c_void_p), ("uin_plane", C.c_void_p), ("ubcs", C.c_void_p),
+                ("surf", C.c_void_p), ("nodes", C.c_void_p), ("n_nodes", C.c_int), ("add_body_force", C.c_int),
+                ("ib", C.c_void_p), ("p", C.c_void_p), ("ustar", C.c_void_p)]
+
+
 class IbSurfaceC(C.Structure):
     """vfs_ib_surface: device pointers of a triangulated body."""
     _fields_ = [("n_v", C.c_int), ("n_elmt", C.c_int),
@@ -334,6 +341,19 @@ class Ctx:
         _ck(lib().vfs_smagorinsky_constant(self.h, ucat.ptr, C.byref(o), int(ti), int(tistart), int(rstart), cs.ptr),
             "vfs_smagorinsky_constant")
 
+    def set_inlet_flux(self, inlet_flux):
+        """-flux: the target k flux of the forcing of a k-periodic context
+        without dpdz_set (the reference's default -1)."""
+        _ck(lib().vfs_set_inlet_flux(self.h, C.c_double(inlet_flux)), "vfs_set_inlet_flux")
+
+    def calc_k_flux(self, ucont, read=True):
+        """Calc_k_Flux (rhs2.c:1562-1645) -> (mean_k_flux, mean_k_area), kept
+        on the device for Pressure_Gradient (None, None with read=False)."""
+        f, a = C.c_double(), C.c_double()
+        _ck(lib().vfs_calc_k_flux(self.h, ucont.ptr, C.byref(f) if read else None, C.byref(a) if read else None),
+            "vfs_calc_k_flux")
+        return (f.value, a.value) if read else (None, None)
+
     def pressure_gradient(self, p, dp):
         _ck(lib().vfs_pressure_gradient(self.h, p.ptr, dp.ptr), "vfs_pressure_gradient")
 
@@ -351,6 +371,26 @@ class Ctx:
     def rk_stage(self, u0, ucont, ucat, nu_t, dP, alfa, rhs):
         _ck(lib().vfs_rk_stage(self.h, u0.ptr, ucont.ptr, ucat.ptr, _p(nu_t), dP.ptr, C.c_double(alfa), rhs.ptr),
             "vfs_rk_stage")
+
+    def runge_kutta(self, ucont, ucat, nu_t, dP, u0, rhs, inflow=None, coor=None, uin_plane=None, ubcs=None,
+                    surface=None, nodes=None, n_nodes=0, ib=None, p=None, ustar=None, add_body_force=False):
+        """RungeKutta (rhs.c:1578-1700) with its boundary work -> (flux_in,
+        flux_out).  inflow: an InflowOptions for an inlet at k = 0; ib: an
+        IbOptions (None: the defaults)."""
+        keep = []
+
+        def ref(x):
+            if x is None:
+                return None
+            keep.append(x)
+            return C.cast(C.byref(x), C.c_void_p)
+
+        bc = RkBc(ref(inflow), _p(coor), _p(uin_plane), _p(ubcs), ref(surface.c) if surface is not None else None,
+                  _p(nodes), int(n_nodes), int(bool(add_body_force)), ref(ib), _p(p), _p(ustar))
+        fi, fo = C.c_double(), C.c_double()
+        _ck(lib().vfs_runge_kutta(self.h, C.byref(bc), ucont.ptr, ucat.ptr, _p(nu_t), dP.ptr, u0.ptr, rhs.ptr,
+                                  C.byref(fi), C.byref(fo)), "vfs_runge_kutta")
+        return fi.value, fo.value
 
     def min_dt(self, ucont):
         v = C.c_double()
