@@ -205,6 +205,15 @@ int vfs_ib_ray_jitter(int seed, double *s);
  * a node has no element or no interception (the reference prints "IBM
  * Searching Fail!"), -4 if cap is too small.  Synchronizes the stream. */
 int vfs_ib_search(vfs_ctx *ctx, const vfs_ib_surface *surf, vfs_ib_node *nodes, int cap, int *n_nodes);
+/* vfs_ib_search without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (tight loops that move a body
+ * every step, harnesses that forbid hipMalloc).  Same arguments, results
+ * and error codes, plus -5 when the job needs more than the library's static
+ * 8 MiB scratch on the device (32 bytes per element plus 4 bytes per 256
+ * grid points).  Every call on a device shares that device's scratch, which
+ * keeps the last call's contents: calls that overlap on one device race.
+ * Synchronizes the stream. */
+int vfs_ib_search_noalloc(vfs_ctx *ctx, const vfs_ib_surface *surf, vfs_ib_node *nodes, int cap, int *n_nodes);
 /* ibm_interpolation_advanced (ibm.c:3294-3698) without averaging, levelset,
  * FSI or an IB wall model: ucat at each node from the interpolated velocity
  * along the normal (noslip, or freeslip with -inv, wallfunction.c:134-166;

@@ -136,6 +136,7 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     d.ren = cfg->ren; d.dt = cfg->dt; d.st = cfg->st; d.mpg = cfg->mean_pressure_gradient;
     d.P = (long long)d.mx * d.my * d.mz;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; vfs_set_error("no HIP device"); return -1; }
+    if (vfs_ib_scratch_setup(c->device)) { delete c; return -1; }
     c->stream = 0;
     const size_t P = (size_t)d.P;
     // metrics: 13 vector arrays + 4 scalar arrays

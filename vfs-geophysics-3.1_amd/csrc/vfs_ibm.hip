@@ -17,6 +17,7 @@
 // Ucat / P.
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
 #include "vfs_internal.h"
@@ -919,6 +920,124 @@ extern "C" int vfs_ib_search(vfs_ctx *c, const vfs_ib_surface *surf, vfs_ib_node
     }
     if (nfail) {
         vfs_set_error("vfs_ib_search: " + std::to_string(nfail) +
+                      " IB nodes have no nearest element or no interception point");
+        return -3;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// vfs_ib_search_noalloc: the same node list with its scratch (the element
+// bounding spheres, the per-block node counts turned into offsets in place,
+// the node total and the failure count) in a static device array instead of
+// a per-call allocation.  Each device holds its own copy of the array until
+// the process ends; calls that overlap on one device share it and race.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t IB_SCRATCH_BYTES = size_t(8) << 20;
+constexpr int IB_SCRATCH_DEVICES = 64;
+
+static __device__ __align__(256) char scratch[IB_SCRATCH_BYTES];
+
+// the array's device address on each device (the host-side name of the array
+// is not a device address), written once under the mutex
+char *g_scratch_dev[IB_SCRATCH_DEVICES];
+std::mutex g_scratch_mu;
+
+// exclusive scan of cnt[0..nb) in place, with the total at *total (one
+// workgroup of 256 in chunks of 256)
+__global__ void __launch_bounds__(256) k_ib_scan(int nb, int *cnt, int *total)
+{
+    __shared__ int sh[256];
+    int carry = 0;
+    for (int base = 0; base < nb; base += 256) {
+        const int v = base + (int)threadIdx.x < nb ? cnt[base + threadIdx.x] : 0;
+        sh[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int add = (int)threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += add;
+            __syncthreads();
+        }
+        if (base + (int)threadIdx.x < nb) cnt[base + threadIdx.x] = carry + sh[threadIdx.x] - v;
+        __syncthreads();
+        carry += sh[255];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void k_ib_zero_ints(int n, int *p)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0;
+}
+
+}   // namespace
+
+// Called by vfs_ctx_create with the context's device current: records that
+// device's address of the static scratch once.
+int vfs_ib_scratch_setup(int device)
+{
+    VFS_REQUIRE(device >= 0 && device < IB_SCRATCH_DEVICES, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_dev[device]) return 0;
+    void *p = nullptr;
+    VFS_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = (char *)p;
+    return 0;
+}
+
+extern "C" int vfs_ib_search_noalloc(vfs_ctx *c, const vfs_ib_surface *surf, vfs_ib_node *nodes, int cap,
+                                     int *n_nodes)
+{
+    const Dims d = c->d;
+    if (int rc = check_surface(surf)) return rc;
+    VFS_REQUIRE(n_nodes && cap >= 0 && (cap == 0 || nodes), "bad node list");
+    char *buf = nullptr;
+    if (c->device >= 0 && c->device < IB_SCRATCH_DEVICES) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        buf = g_scratch_dev[c->device];
+    }
+    VFS_REQUIRE(buf, "vfs_ib_search_noalloc: the context's device has no scratch address");
+    const IbSurf s = to_surf(surf);
+    const int nb = (int)grid_for(d.P, LIST_BLOCK);
+    // layout: qv (3 n_elmt doubles), rad (n_elmt doubles), cnt (nb ints), total, fail
+    const size_t bytes = (size_t)s.n_elmt * 4 * sizeof(double) + ((size_t)nb + 2) * sizeof(int);
+    if (bytes > IB_SCRATCH_BYTES) {
+        vfs_set_error("vfs_ib_search_noalloc: " + std::to_string(s.n_elmt) + " elements and " + std::to_string(d.P) +
+                      " grid points need " + std::to_string(bytes) + " bytes of scratch; the static scratch holds " +
+                      std::to_string(IB_SCRATCH_BYTES) + " (32 bytes an element plus 4 bytes per 256 points)");
+        return -5;
+    }
+    double *qv = (double *)buf, *rad = qv + 3 * (size_t)s.n_elmt;
+    int *cnt = (int *)(rad + s.n_elmt), *total_d = cnt + nb, *fail = total_d + 1;
+    hipLaunchKernelGGL(k_ib_zero_ints, dim3(1), dim3(64), 0, c->stream, 2, total_d);
+    hipLaunchKernelGGL(k_ib_bound, dim3(grid_for(s.n_elmt, 256)), dim3(256), 0, c->stream, s, qv, rad);
+    hipLaunchKernelGGL(k_ib_count, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, cnt);
+    hipLaunchKernelGGL(k_ib_scan, dim3(1), dim3(256), 0, c->stream, nb, cnt, total_d);
+    VFS_CHECK_LAUNCH();
+    int total = 0, nfail = 0;
+    VFS_CHECK_HIP(hipMemcpyAsync(&total, total_d, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    *n_nodes = total;
+    if (total > cap) {
+        vfs_set_error("vfs_ib_search_noalloc: " + std::to_string(total) + " IB nodes do not fit the list of " +
+                      std::to_string(cap));
+        return -4;
+    }
+    if (total > 0) {
+        hipLaunchKernelGGL(k_ib_place, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, cnt, total, nodes);
+        hipLaunchKernelGGL(k_ib_search, dim3(grid_for(total, 64)), dim3(64), 0, c->stream, d, c->g, s, qv, rad, total,
+                           nodes, fail);
+        VFS_CHECK_LAUNCH();
+        VFS_CHECK_HIP(hipMemcpyAsync(&nfail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+    }
+    if (nfail) {
+        vfs_set_error("vfs_ib_search_noalloc: " + std::to_string(nfail) +
                       " IB nodes have no nearest element or no interception point");
         return -3;
     }

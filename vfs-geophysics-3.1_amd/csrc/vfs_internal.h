@@ -150,6 +150,10 @@ struct vfs_ctx {
 // the 3x3x3 dilation and 3 / 1 relabelling of an inside-marked mask (vfs_metrics.hip)
 int vfs_mask_dilate_relabel(vfs_ctx *c);
 
+// the device address of the static scratch of vfs_ib_search_noalloc,
+// recorded once per device (vfs_ibm.hip)
+int vfs_ib_scratch_setup(int device);
+
 // wall-function walls (vfs_wall.hip)
 bool vfs_has_wallfn(const Dims &d);
 int vfs_wall_setup(vfs_ctx *c);

@@ -302,6 +302,14 @@ class Ctx:
         _ck(lib().vfs_ib_search(self.h, C.byref(surface.c), nodes.ptr, C.c_int(cap), C.byref(n)), "vfs_ib_search")
         return nodes, n.value
 
+    def ib_search_noalloc(self, surface, nodes, cap):
+        """vfs_ib_search_noalloc into a caller's node list (a DeviceArray of
+        cap 144-byte records) -> the node count."""
+        n = C.c_int(0)
+        _ck(lib().vfs_ib_search_noalloc(self.h, C.byref(surface.c), nodes.ptr, C.c_int(int(cap)), C.byref(n)),
+            "vfs_ib_search_noalloc")
+        return n.value
+
     def ib_interpolate(self, surface, nodes, n, ucont, ucat, p, ustar, stats=True, moving=False, wallfunction=False,
                        ks=1e-4, roughness=None):
         """ibm_interpolation_advanced (ibm.c:3294-3698): ucont, ucat, p, ustar
