@@ -413,6 +413,69 @@ def turbine_share_leg(steps, warmup):
                                                         for o in out)}
 
 
+def turbine_c5_leg(steps, warmup, precision=1):
+    """The turbine-wake case at full size (768 x 384 x 384, 113 M cells):
+    log-law inflow with fluctuations at k = 0 (InflowFlux with the caller's
+    profile), the outlet at k = mz-1, a wall-function bottom, LES, and the
+    actuator ring spread once with the Gaussian delta function and carried in
+    every Runge-Kutta stage; FlowSolver steps with the mixed-precision
+    Poisson solve to 1e-8.  Reports the step time and, for every step, that
+    the fields stay finite, the outflow equals the inflow after the outlet
+    correction, and the divergence left by the projection."""
+    import vfsgpu as V
+    from vfsgpu import workloads as W
+    from vfsgpu.step import FlowSolver
+    t_setup = time.perf_counter()
+    log("[bench] C5: building the grid, metrics, fields and the Poisson hierarchy")
+    cfg, spec, ctx, ucat, ucont, nunk = setup("C5")
+    log(f"[bench] C5: {nunk} unknowns after {time.perf_counter() - t_setup:.1f} s")
+    Ni, Nj, Nk = spec["N"]
+    coor = W.grid(spec)
+    dcoor = V.DeviceArray.from_host(coor)
+    plane = V.DeviceArray.from_host(W.loglaw_inflow_plane(cfg, coor))
+    del coor
+    _, _, inlet_area = ctx.inlet_area()
+    ring = W.ACTUATOR_RING
+    cent, F, dA = W.actuator_ring()
+    dcent = V.DeviceArray.from_host(cent.reshape(-1))
+    box, _ = ctx.rotor_elements(dcoor, dcent)
+    f_eul = ctx.vec()
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.calc_f_eul(dcoor, dcent, V.DeviceArray.from_host(F.reshape(-1)), V.DeviceArray.from_host(dA), box, f_eul,
+                   deltafunc=ring["deltafunc"], halfwidth=ring["halfwidth"])
+    ctx.sync()
+    spread_ms = (time.perf_counter() - t0) * 1e3
+    ctx.set_body_force(f_eul)
+    log(f"[bench] C5: actuator force spread in {spread_ms:.1f} ms")
+    fs = FlowSolver(ctx, V.DeviceArray.from_host(ucont), V.DeviceArray.from_host(ucat), coor=dcoor,
+                    inflow=V.InflowOptions(0, 0, -1.0, inlet_area), uin_plane=plane, cs=W.CS_LES,
+                    precision=precision, rtol=1e-8, add_body_force=True)
+    del ucat, ucont
+    setup_s = time.perf_counter() - t_setup
+    for _ in range(warmup):
+        fs.step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    out = []
+    for _ in range(steps):
+        out.append(fs.step())
+        log(f"[bench] C5 step {fs.ti}: {out[-1]}")
+    ctx.sync()
+    step_ms = (time.perf_counter() - t0) / max(1, steps) * 1e3
+    finite = all(np.isfinite(o["maxdiv"]) and np.isfinite(o["relres"]) for o in out)
+    res = {"workload": f"turbine wake {Ni}x{Nj}x{Nk}: log-law inlet, outlet, wall-function bottom, LES, "
+                       f"actuator ring of {cent.shape[1]} elements", "cells": Ni * Nj * Nk, "steps": steps,
+           "warmup": warmup, "precision": "mixed" if precision else "fp64", "setup_s": round(setup_s, 1),
+           "f_eul_spread_ms": round(spread_ms, 2), "step_ms": round(step_ms, 1),
+           "poisson_iters": [o["iters"] for o in out], "max_relres": max(o["relres"] for o in out),
+           "max_div_after_projection": max(o["maxdiv"] for o in out),
+           "flux_balance_after_outlet_correction": max(abs(o["flux_out_corrected"] - o["flux_in"]) / abs(o["flux_in"])
+                                                       for o in out),
+           "finite": bool(finite and np.isfinite(fs.uc.host()).all())}
+    return res
+
+
 def traffic_from_profile(name):
     """HBM bytes per Formfunction_2 call from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE of the Formfunction_2 kernels, each
@@ -508,6 +571,8 @@ def main():
     ap.add_argument("--no-ib", action="store_true", help="skip the CURVIB reconstruction leg")
     ap.add_argument("--no-turbine-share", action="store_true",
                     help="skip the 768x384x48 open-boundary leg (one GPU's share of the turbine-wake case)")
+    ap.add_argument("--c5-steps", type=int, default=0,
+                    help="also run the full 768x384x384 turbine-wake case for this many steps (about 200 GB of HBM)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -581,6 +646,13 @@ def main():
         share = turbine_share_leg(2, 1)
         log(f"[bench] turbine-wake share: {share}")
 
+    c5 = None
+    if args.c5_steps > 0:
+        if share is None:
+            del s, ctx
+        c5 = turbine_c5_leg(args.c5_steps, 1)
+        log(f"[bench] turbine wake C5: {c5}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.config, args.cpu_budget)
@@ -623,6 +695,7 @@ def main():
         "actuator_disk": act,
         "ib_reconstruction": ib,
         "turbine_share": share,
+        "turbine_c5": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -9,8 +9,10 @@ C2  256x128x128 tanh-stretched (beta=2) channel with a sinusoidal bottom wall
     random-Fourier k^-5/3 field (seed 1002) on a Poiseuille mean.
 C3  the C2 generator at 512x256x256 with a random Poisson RHS (seed 1003).
 C4  C2 fields with unit-norm random tangent/cotangent (seeds 14001/24001).
-C5  768x384x384 turbine wake: needs inflow/outflow and wall-function
-    boundaries (bctypes 5/4/-1), which this build does not implement yet.
+C5  768x384x384 turbine wake: tanh-stretched channel (beta=2, flat bottom)
+    with a wall-function bottom (bctype -1), a log-law inflow at k = 0 with
+    U[-1,1)*0.05 fluctuations (seed 1005, bctype 5), an outlet at k = mz-1
+    (bctype 4), Smagorinsky LES and an actuator-disk ring (ACTUATOR_RING).
 Box: Lx = 2 pi, Ly = 2, Lz = pi in every configuration.
 """
 import math
@@ -42,6 +44,10 @@ BASE = {
     # set of the turbine-wake case, at parity-test size)
     "C1wo": dict(N=(64, 64, 64), beta=0.0, amp=0.0, ren=5000.0, les=0, immersed=0, field="poiseuille", seed=1001,
                  bottom=-1, kopen=1),
+    # the turbine-wake case: wall-function bottom, log-law inflow at k = 0,
+    # outlet at k = mz-1, LES
+    "C5": dict(N=(768, 384, 384), beta=2.0, amp=0.0, ren=5000.0, les=1, immersed=0, field="loglaw", seed=1005,
+               bottom=-1, kopen=1),
     # one GPU's share of the turbine-wake case (768 x 384 x 384 cut into 8
     # k slabs): wall-function bottom, log-law inflow at k = 0, outlet at
     # k = mz-1, LES
@@ -132,6 +138,27 @@ def icosphere(cx, cy, cz, r, level):
     e[out] = e[out][:, [0, 2, 1]]
     n[out] = -n[out]
     return x, e, n
+
+
+# the actuator disk of the turbine-wake case: a ring of radius 0.5 about
+# (Lx/2, 1, Lz/4) in the x-y plane (normal to the stream, which runs in k),
+# thickness 0.1, thrust -1 per unit area along z, Gaussian spreading
+ACTUATOR_RING = dict(n=1024, radius=0.5, thickness=0.1, centre=(LX / 2, 1.0, LZ / 4), deltafunc=7, halfwidth=4.0)
+
+
+def actuator_ring(n=None, radius=None, thickness=None, centre=None, **_):
+    """Element centres (3, n), forces (3, n) and areas (n) of the actuator
+    ring: n equal arcs, each of area 2 pi r / n * thickness."""
+    r = ACTUATOR_RING
+    n = n or r["n"]
+    radius = radius or r["radius"]
+    thickness = thickness or r["thickness"]
+    cx, cy, cz = centre or r["centre"]
+    th = 2 * np.pi * np.arange(n) / n
+    cent = np.stack([cx + radius * np.cos(th), cy + radius * np.sin(th), np.full(n, cz)])
+    F = np.stack([np.zeros(n), np.zeros(n), -np.ones(n)])
+    dA = np.full(n, 2 * np.pi * radius / n * thickness)
+    return cent, F, dA
 
 
 def loglaw_inflow_plane(cfg, coor, seed=1005, noise=0.05):
