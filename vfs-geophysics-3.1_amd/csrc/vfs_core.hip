@@ -187,6 +187,7 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     if (c->wf_table) hipFree(c->wf_table);
     if (c->ustar) hipFree(c->ustar);
     if (c->dsm) hipFree(c->dsm);
+    if (c->ff2_flags) hipFree(c->ff2_flags);
     delete c;
     return 0;
 }

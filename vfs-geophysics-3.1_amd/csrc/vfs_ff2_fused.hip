@@ -25,30 +25,15 @@ int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const do
 int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
 int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale);
 
-// Build-time shape parameters (tuned on MI355X; see the Makefile).
+// Shape parameters (tuned on MI355X).
 #ifndef VFS_FACES_KCHUNK
 #define VFS_FACES_KCHUNK 32      // k-planes per workgroup of the face kernel
-#endif
-#ifndef VFS_FACES_SPLIT
-#define VFS_FACES_SPLIT 0        // 1: convective and viscous fluxes in separate waves
 #endif
 #ifndef VFS_FACES_WAVES
 #define VFS_FACES_WAVES 2        // target waves per SIMD of the face kernel
 #endif
-#ifndef VFS_CELLS_KMARCH
-#define VFS_CELLS_KMARCH 1       // k-marching cell divergence
-#endif
-#ifndef VFS_RHS_KMARCH
-#define VFS_RHS_KMARCH 0         // 1: k-marching projection (measured no faster)
-#endif
 #ifndef VFS_CELLS_KCHUNK
 #define VFS_CELLS_KCHUNK 16
-#endif
-#ifndef VFS_CELLRHS
-#define VFS_CELLRHS 0            // 1: fused divergence+projection, 0: two passes
-#endif
-#ifndef VFS_CELLRHS_KCHUNK
-#define VFS_CELLRHS_KCHUNK 16
 #endif
 
 namespace {
@@ -62,8 +47,7 @@ constexpr int BI = TI + 3;      // staged columns: i0-1 .. i0+TI+1
 constexpr int BJ = TJ + 3;      // staged rows:    j0-1 .. j0+TJ+1
 constexpr int BP = BI * BJ;     // points per staged plane
 constexpr int NT = TI * TJ;     // threads per face group
-constexpr int NG = VFS_FACES_SPLIT ? 2 : 1;  // groups: (convective, viscous) or both in one
-constexpr int NTB = NT * NG;    // threads per workgroup
+constexpr int NTB = NT;         // threads per workgroup
 constexpr int NU = (3 * BP + NTB - 1) / NTB;   // ucat doubles per thread per plane
 constexpr int NN = (BP + NTB - 1) / NTB;       // nvert values per thread per plane
 
@@ -148,63 +132,78 @@ __device__ __forceinline__ void store_plane(int k, const StageMap &m, const doub
 }  // namespace
 
 // The three face fluxes of point (k,j,i) from the staged planes, written to
-// the face array.  PARTS: 1 convective only, 2 viscous only, 3 both (the
-// unused part is removed at compile time).  The periodic wrap-around faces
-// are left to k_ff2_faces_bnd.
-template <int PARTS, bool WENO>
-__device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const double *__restrict__ ucont,
+// the face array.  The periodic wrap-around faces are left to
+// k_ff2_faces_bnd.  PLAIN: no cell within reach of the tile's faces of this
+// plane is marked, so every face except the last of its direction takes the
+// mask-free specialisation of face_flux_t (bit-identical results).
+template <int DIR, bool WENO, bool PLAIN>
+__device__ __forceinline__ void face_dir(const Dims &d, const Geom &g, const double *__restrict__ ucont,
                                          const double *__restrict__ nu_t, double *__restrict__ face, int i0, int j0,
                                          int i, int j, int k)
 {
     const LdsU U{i0, j0};
-    const LdsN N{i0, j0};
     const long long P = d.P;
     const unsigned t = IX32(d, k, j, i);
+    const int n = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int m = DIR == 0 ? d.mx : DIR == 1 ? d.my : d.mz;
     double D[3], V[3];
+    if (PLAIN && n != m - 2)
+        face_flux_t<DIR, double, LdsU, UConPlain, ZeroN, StoredFaceMetric, WENO, unsigned, true>(
+            d, g, ZeroN{}, UConPlain{ucont, DIR}, U, nu_t, i, j, k, D, V);
+    else
+        face_flux_t<DIR, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(
+            d, g, LdsN{i0, j0}, UConPlain{ucont, DIR}, U, nu_t, i, j, k, D, V);
     // slot s of the face array starts at face + s * P (a uniform pointer)
-    if (!(d.ip && i == d.mx - 2)) {
-        face_flux_t<0, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 0}, U,
-                                                                                        nu_t, i, j, k, D, V);
 #pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) st32(face + (0 * 3 + q) * P, t, D[q]);
-            if (PARTS & 2) st32(face + (3 * 3 + q) * P, t, V[q]);
-        }
-    }
-    // one face at a time: keeps the live registers of a single face
-    __builtin_amdgcn_sched_barrier(0);
-    if (!(d.jp && j == d.my - 2)) {
-        face_flux_t<1, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 1}, U,
-                                                                                        nu_t, i, j, k, D, V);
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) st32(face + (1 * 3 + q) * P, t, D[q]);
-            if (PARTS & 2) st32(face + (4 * 3 + q) * P, t, V[q]);
-        }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (!(d.kp && k == d.mz - 2)) {
-        face_flux_t<2, double, LdsU, UConPlain, LdsN, StoredFaceMetric, WENO, unsigned>(d, g, N, UConPlain{ucont, 2}, U,
-                                                                                        nu_t, i, j, k, D, V);
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            if (PARTS & 1) st32(face + (2 * 3 + q) * P, t, D[q]);
-            if (PARTS & 2) st32(face + (5 * 3 + q) * P, t, V[q]);
-        }
+    for (int q = 0; q < 3; q++) {
+        st32(face + (DIR * 3 + q) * P, t, D[q]);
+        st32(face + ((3 + DIR) * 3 + q) * P, t, V[q]);
     }
 }
 
+template <bool WENO, bool PLAIN>
+__device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const double *__restrict__ ucont,
+                                         const double *__restrict__ nu_t, double *__restrict__ face, int i0, int j0,
+                                         int i, int j, int k)
+{
+    if (!(d.ip && i == d.mx - 2)) face_dir<0, WENO, PLAIN>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+    // one face at a time: keeps the live registers of a single face
+    __builtin_amdgcn_sched_barrier(0);
+    if (!(d.jp && j == d.my - 2)) face_dir<1, WENO, PLAIN>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+    __builtin_amdgcn_sched_barrier(0);
+    if (!(d.kp && k == d.mz - 2)) face_dir<2, WENO, PLAIN>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+}
+
+// Plane flags of the face kernel: flag[(k * gy + by) * gx + bx] = 1 when no
+// cell of the tile's staged box on planes k-1..k+2 (one-cell i/j halo, two on
+// the high side, widened by one more each way) is marked.
+__global__ void k_ff2_plain_flags(Dims d, const double *__restrict__ nvert, int gx, int gy,
+                                  unsigned char *__restrict__ flag)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)gx * gy * d.mz) return;
+    const int bx = (int)(t % gx), by = (int)((t / gx) % gy), k = (int)(t / ((long long)gx * gy));
+    const int i0 = 1 + bx * TI, j0 = 1 + by * TJ;
+    const int ia = max(i0 - 3, 0), ib = min(i0 + TI + 2, d.mx - 1);
+    const int ja = max(j0 - 3, 0), jb = min(j0 + TJ + 2, d.my - 1);
+    const int ka = max(k - 1, 0), kb = min(k + 2, d.mz - 1);
+    int clean = 1;
+    for (int kk = ka; kk <= kb && clean; kk++)
+        for (int jj = ja; jj <= jb && clean; jj++)
+            for (int ii = ia; ii <= ib; ii++)
+                if (nvert[IX(d, kk, jj, ii)] != 0.0) { clean = 0; break; }
+    flag[t] = (unsigned char)clean;
+}
+
 // Face fluxes of all three directions at every interior point
-// (1..mx-2, 1..my-2, k in [kb, ke)).  grid: (ceil((mx-2)/64), ceil((my-2)/4), k chunks)
+// (1..mx-2, 1..my-2, k in [kb, ke)).  grid: (ceil((mx-2)/64), ceil((my-2)/TJ), k chunks)
 template <bool WENO>
-__global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FACES_WAVES, VFS_FACES_WAVES))) k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert,
-                                                      const double *__restrict__ ucont,
-                                                      const double *__restrict__ ucat,
-                                                      const double *__restrict__ nu_t, double *__restrict__ face,
-                                                      int kchunk)
+__global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FACES_WAVES, VFS_FACES_WAVES)))
+k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
+                const double *__restrict__ ucat, const double *__restrict__ nu_t, double *__restrict__ face,
+                int kchunk, const unsigned char *__restrict__ plain)
 {
     const int tx = threadIdx.x % TI, ty = (threadIdx.x / TI) % TJ;
-    const int grp = __builtin_amdgcn_readfirstlane(threadIdx.x / NT);   // 0: convective (or both), 1: viscous
     // logical tile of this workgroup: XCD x takes a contiguous range (x fastest)
     const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
     const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
@@ -214,6 +213,7 @@ __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FA
     const int ke = min(kb + kchunk, d.mz - 1);
     const int i = i0 + tx, j = j0 + ty;
     const bool active = i <= d.mx - 2 && j <= d.my - 2;
+    const unsigned fstride = gridDim.x * gridDim.y, fbase = bx + gridDim.x * by;
 
     double ru[NU];
     float rn[NN];
@@ -227,9 +227,8 @@ __global__ void __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(VFS_FA
         // prefetch plane k+3 while plane k is computed
         fetch_plane(d, ucat, nvert, sm, k + 3, ru, rn);
         if (active) {
-            if (!VFS_FACES_SPLIT) faces_at<3, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
-            else if (grp == 0) faces_at<1, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
-            else faces_at<2, WENO>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            if (plain[fbase + fstride * (unsigned)k]) faces_at<WENO, true>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
+            else faces_at<WENO, false>(d, g, ucont, nu_t, face, i0, j0, i, j, k);
         }
         __syncthreads();   // every read of plane k-1 is done
         store_plane(k + 3, sm, ru, rn);
@@ -355,131 +354,18 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
 #undef FI
 }
 
-// Projection onto the faces with the masks (momentum.c:1673-1900),
-// k-marching: the fp, zeta metric and nvert of the k+1 neighbour read at
-// plane k are the cell's own values at plane k+1, so they stay in registers.
-// Same expressions and order as project_rhs; interior cells only (the ghost
-// shell is zeroed by k_zero_shell_vec).
-__global__ void __launch_bounds__(DT) k_ff2_rhs_km(Dims d, Geom g, const double *__restrict__ nvert,
-                                                  const double *__restrict__ fp, double *__restrict__ rhs,
-                                                  double scale, int kchunk)
+// the plane flags of the face kernel, recomputed when the mask has changed
+static int ff2_plain_flags(vfs_ctx *c, int gx, int gy)
 {
-    const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
-    const unsigned lb = xcd_block(blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z), nb);
-    const int bx = lb % gridDim.x, by = (lb / gridDim.x) % gridDim.y, bz = lb / (gridDim.x * gridDim.y);
-    const int i = 1 + bx * DI + (int)(threadIdx.x % DI), j = 1 + by * DJ + (int)(threadIdx.x / DI);
-    if (i > d.mx - 2 || j > d.my - 2) return;
-    const int kb = 1 + bz * kchunk, ke = min(kb + kchunk, d.mz - 1);
-    const long long sk = (long long)d.mx * d.my;
-    const int ia = wrap_cell(i + 1, d.mx, d.ip), jb = wrap_cell(j + 1, d.my, d.jp);
-    const double *cs = g.csi, *et = g.eta, *ze = g.zet;
-    long long id = IX(d, kb, j, i);
-    double fc[3] = {fp[3 * id], fp[3 * id + 1], fp[3 * id + 2]};
-    double zc[3] = {ze[3 * id], ze[3 * id + 1], ze[3 * id + 2]};
-    double n0 = nvert[id];
-    for (int k = kb; k < ke; k++, id += sk) {
-        const int kc = wrap_cell(k + 1, d.mz, d.kp);
-        const long long a = IX(d, k, j, ia), b = IX(d, k, jb, i), c = IX(d, kc, j, i);
-        const long long ga = id + 1, gb = id + d.mx, gc = id + sk;   // metric ghosts are stored
-        // next plane's own values (the k+1 neighbour of this plane)
-        const double fn[3] = {fp[3 * c], fp[3 * c + 1], fp[3 * c + 2]};
-        const double zn[3] = {ze[3 * gc], ze[3 * gc + 1], ze[3 * gc + 2]};
-        const double nn = nvert[gc];
-        double r[3] = {rhs[3 * id], rhs[3 * id + 1], rhs[3 * id + 2]};
-        r[0] += scale * (0.5 * (cs[3 * id] * fc[0] + cs[3 * id + 1] * fc[1] + cs[3 * id + 2] * fc[2]) +
-                         0.5 * (cs[3 * ga] * fp[3 * a] + cs[3 * ga + 1] * fp[3 * a + 1] + cs[3 * ga + 2] * fp[3 * a + 2])) *
-                g.iaj[id];
-        r[1] += scale * (0.5 * (et[3 * id] * fc[0] + et[3 * id + 1] * fc[1] + et[3 * id + 2] * fc[2]) +
-                         0.5 * (et[3 * gb] * fp[3 * b] + et[3 * gb + 1] * fp[3 * b + 1] + et[3 * gb + 2] * fp[3 * b + 2])) *
-                g.jaj[id];
-        r[2] += scale * (0.5 * (zc[0] * fc[0] + zc[1] * fc[1] + zc[2] * fc[2]) +
-                         0.5 * (zn[0] * fn[0] + zn[1] * fn[1] + zn[2] * fn[2])) * g.kaj[id];
-        if (n0 + nvert[id + 1] > 0.1 || (!d.ip && i == d.mx - 2)) r[0] = 0;
-        if (n0 + nvert[id + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) r[1] = 0;
-        if (n0 + nn > 0.1 || (!d.kp && k == d.mz - 2)) r[2] = 0;
-        rhs[3 * id] = r[0]; rhs[3 * id + 1] = r[1]; rhs[3 * id + 2] = r[2];
-        // the k+1 neighbour is the cell of the next step (a periodic wrap to
-        // plane 1 only happens at the last plane, where the loop ends)
-        fc[0] = fn[0]; fc[1] = fn[1]; fc[2] = fn[2];
-        zc[0] = zn[0]; zc[1] = zn[1]; zc[2] = zn[2];
-        n0 = nn;
-    }
-}
-
-// Cell divergence (momentum.c:1506-1636) and projection onto the faces
-// (momentum.c:1673-1900) in one k-marching pass over a 64 x 4 column of
-// cells.  Each thread keeps fp of its own cell at planes k and k+1; the fp of
-// the i+1 column and the j+1 row of the tile (periodic-wrapped) are computed
-// by the first 68 threads; neighbours are exchanged through LDS.
-constexpr int CI = 64, CJ = 4, CT = CI * CJ;
-
-__global__ void __launch_bounds__(CT) __attribute__((amdgpu_waves_per_eu(2, 2))) k_ff2_cellrhs(Dims d, Geom g, const double *__restrict__ nvert,
-                                                   const double *__restrict__ face, double *__restrict__ rhs,
-                                                   double scale, int kchunk)
-{
-    __shared__ double sfp[3][CJ + 1][CI + 1];
-    const int tx = threadIdx.x % CI, ty = threadIdx.x / CI;
-    const int i0 = 1 + blockIdx.x * CI, j0 = 1 + blockIdx.y * CJ;
-    const int kb = 1 + blockIdx.z * kchunk;
-    const int ke = min(kb + kchunk, d.mz - 1);
-    const int i = i0 + tx, j = j0 + ty;
-    const bool active = i <= d.mx - 2 && j <= d.my - 2;
-    // last tile column / row that holds an interior cell
-    const int ilast = min(i0 + CI - 1, d.mx - 2), jlast = min(j0 + CJ - 1, d.my - 2);
-    // halo work: threads 0..CI-1 the j+1 row, CI..CI+CJ-1 the i+1 column
-    int hi = -1, hj = -1, hslot_i = 0, hslot_j = 0;
-    if (threadIdx.x < CI) {
-        hi = i0 + threadIdx.x; hj = wrap_cell(jlast + 1, d.my, d.jp);
-        hslot_i = threadIdx.x; hslot_j = CJ;
-        if (hi > d.mx - 2) hi = -1;
-    } else if (threadIdx.x < CI + CJ) {
-        hj = j0 + threadIdx.x - CI; hi = wrap_cell(ilast + 1, d.mx, d.ip);
-        hslot_i = CI; hslot_j = threadIdx.x - CI;
-        if (hj > d.my - 2) hi = -1;
-    }
-    // a wrapped neighbour that is a ghost (non-periodic) only feeds masked faces
-    const bool hvalid = hi >= 1 && hi <= d.mx - 2 && hj >= 1 && hj <= d.my - 2;
-
-    double f0[3] = {0, 0, 0}, f1[3] = {0, 0, 0};
-    if (active) cell_fp(d, nvert, face, i, j, kb, f0);
-    for (int k = kb; k < ke; k++) {
-        const int kn = wrap_cell(k + 1, d.mz, d.kp);
-        if (active) {
-            if (kn <= d.mz - 2) cell_fp(d, nvert, face, i, j, kn, f1);
-            else f1[0] = f1[1] = f1[2] = 0;
-            sfp[0][ty][tx] = f0[0]; sfp[1][ty][tx] = f0[1]; sfp[2][ty][tx] = f0[2];
-        }
-        if (hvalid) {
-            double h[3];
-            cell_fp(d, nvert, face, hi, hj, k, h);
-            sfp[0][hslot_j][hslot_i] = h[0]; sfp[1][hslot_j][hslot_i] = h[1]; sfp[2][hslot_j][hslot_i] = h[2];
-        } else if (threadIdx.x < CI + CJ) {
-            sfp[0][hslot_j][hslot_i] = 0; sfp[1][hslot_j][hslot_i] = 0; sfp[2][hslot_j][hslot_i] = 0;
-        }
-        __syncthreads();
-        if (active) {
-            const long long id = IX(d, k, j, i);
-            const int ai = i < ilast ? tx + 1 : CI, bj = j < jlast ? ty + 1 : CJ;
-            const int ay = i < ilast ? ty : ty, bx = j < jlast ? tx : tx;
-            double fa[3], fb[3];
-            for (int q = 0; q < 3; q++) { fa[q] = sfp[q][ay][ai]; fb[q] = sfp[q][bj][bx]; }
-            double r[3] = {rhs[3 * id], rhs[3 * id + 1], rhs[3 * id + 2]};
-            project_rhs_v(d, g, nvert, i, j, k, id, scale, f0, fa, fb, f1, r);
-            rhs[3 * id] = r[0]; rhs[3 * id + 1] = r[1]; rhs[3 * id + 2] = r[2];
-        }
-        __syncthreads();
-        f0[0] = f1[0]; f0[1] = f1[1]; f0[2] = f1[2];
-    }
-}
-
-// rhs = 0 on the ghost shell (the boundary planes of k_ff2_rhs)
-__global__ void k_zero_shell_vec(Dims d, double *v)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int k, j, i;
-    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
-    long long o = IX(d, k, j, i);
-    v[3 * o] = v[3 * o + 1] = v[3 * o + 2] = 0;
+    const Dims d = c->d;
+    const size_t n = (size_t)gx * gy * d.mz;
+    if (c->ff2_flags && c->ff2_flags_version == c->nvert_version) return 0;
+    if (!c->ff2_flags) VFS_CHECK_HIP(hipMalloc(&c->ff2_flags, n));
+    hipLaunchKernelGGL(k_ff2_plain_flags, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->nvert, gx, gy,
+                       c->ff2_flags);
+    VFS_CHECK_LAUNCH();
+    c->ff2_flags_version = c->nvert_version;
+    return 0;
 }
 
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale)
@@ -490,12 +376,13 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     const Dims d = c->d;
     const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);
+    if (ff2_plain_flags(c, (int)grid.x, (int)grid.y)) return -1;
     if (d.weno || d.inviscid)
         hipLaunchKernelGGL(k_ff2_faces_lds<true>, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t,
-                           c->face, kchunk);
+                           c->face, kchunk, c->ff2_flags);
     else
         hipLaunchKernelGGL(k_ff2_faces_lds<false>, grid, dim3(NTB), 0, c->stream, d, c->g, c->nvert, ucont, ucat, nu_t,
-                           c->face, kchunk);
+                           c->face, kchunk, c->ff2_flags);
     VFS_CHECK_LAUNCH();
     const long long n3[3] = {(long long)(d.my - 2) * (d.mz - 2), (long long)(d.mx - 2) * (d.mz - 2),
                              (long long)(d.mx - 2) * (d.my - 2)};
@@ -503,30 +390,9 @@ int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const dou
     hipLaunchKernelGGL(k_ff2_faces_bnd, dim3(grid_for(nb, 256)), dim3(256), 0, c->stream, d, c->g, c->nvert, ucont,
                        ucat, nu_t, c->face);
     VFS_CHECK_LAUNCH();
-    if (!VFS_CELLRHS) {
-#if VFS_CELLS_KMARCH
-        const int kc3 = VFS_CELLS_KCHUNK;
-        const dim3 g3((d.mx - 2 + DI - 1) / DI, (d.my - 2 + DJ - 1) / DJ, (d.mz - 2 + kc3 - 1) / kc3);
-        hipLaunchKernelGGL(k_ff2_cells_km, g3, dim3(DT), 0, c->stream, d, c->nvert, c->face, c->fp, kc3);
-        VFS_CHECK_LAUNCH();
-#if VFS_RHS_KMARCH
-        hipLaunchKernelGGL(k_ff2_rhs_km, g3, dim3(DT), 0, c->stream, d, c->g, c->nvert, c->fp, rhs, scale, kc3);
-        VFS_CHECK_LAUNCH();
-        hipLaunchKernelGGL(k_zero_shell_vec, dim3(grid_for(shell_count(d), 256)), dim3(256), 0, c->stream, d, rhs);
-        VFS_CHECK_LAUNCH();
-        return 0;
-#else
-        return vfs_ff2_rhs_only(c, rhs, scale);
-#endif
-#else
-        return vfs_ff2_from_faces(c, rhs, scale);
-#endif
-    }
-    const int kc2 = VFS_CELLRHS_KCHUNK;
-    const dim3 g2((d.mx - 2 + CI - 1) / CI, (d.my - 2 + CJ - 1) / CJ, (d.mz - 2 + kc2 - 1) / kc2);
-    hipLaunchKernelGGL(k_ff2_cellrhs, g2, dim3(CT), 0, c->stream, d, c->g, c->nvert, c->face, rhs, scale, kc2);
+    const int kc3 = VFS_CELLS_KCHUNK;
+    const dim3 g3((d.mx - 2 + DI - 1) / DI, (d.my - 2 + DJ - 1) / DJ, (d.mz - 2 + kc3 - 1) / kc3);
+    hipLaunchKernelGGL(k_ff2_cells_km, g3, dim3(DT), 0, c->stream, d, c->nvert, c->face, c->fp, kc3);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_zero_shell_vec, dim3(grid_for(shell_count(d), 256)), dim3(256), 0, c->stream, d, rhs);
-    VFS_CHECK_LAUNCH();
-    return 0;
+    return vfs_ff2_rhs_only(c, rhs, scale);
 }

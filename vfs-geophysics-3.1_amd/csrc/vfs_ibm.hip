@@ -1106,6 +1106,7 @@ extern "C" int vfs_nvert_body(vfs_ctx *c, const vfs_ib_surface *surf, int reset,
 
 extern "C" int vfs_nvert_body_thin(vfs_ctx *c, const vfs_ib_surface *surf, int reset, int thin, double *nvert_out)
 {
+    c->nvert_version++;
     const Dims d = c->d;
     if (int rc = check_surface(surf)) return rc;
     const IbSurf s = to_surf(surf);

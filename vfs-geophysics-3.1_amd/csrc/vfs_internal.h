@@ -143,6 +143,9 @@ struct vfs_ctx {
     double *kflux;           // 2 doubles: mean_k_flux (Calc_k_Flux) and mean_k_area (Calc_Inlet_Area)
     int kflux_ready;         // vfs_calc_k_flux has run
     double inlet_flux;       // -flux (main.c:198, 2067): the target k flux of the forcing, default -1
+    int nvert_version;       // bumped by every entry point that writes nvert
+    unsigned char *ff2_flags;   // mask-free planes of the face kernel's tiles (allocated on first use)
+    int ff2_flags_version;   // nvert_version the flags were computed for
     int red_blocks;
     Poisson *pois;
 };

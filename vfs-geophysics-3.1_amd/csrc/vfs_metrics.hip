@@ -225,6 +225,7 @@ __global__ void k_nvert_f32_check(long long n, const double *__restrict__ nvert,
 
 extern "C" int vfs_set_nvert(vfs_ctx *c, const double *nvert)
 {
+    c->nvert_version++;
     VFS_CHECK_HIP(hipMemcpyAsync(c->nvert, nvert, (size_t)c->d.P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     if (vfs_wall_mask(c)) return -1;   // the wall-function cells stay IB nodes
     hipLaunchKernelGGL(k_nvert_f32_check, dim3(VFS_RED_BLOCKS), dim3(256), 0, c->stream, c->d.P, c->nvert, c->red);
@@ -300,6 +301,7 @@ __global__ void k_sphere_relabel(Dims d, double *nvert)
 // the dilation and the relabelling that follow the inside test of any body
 int vfs_mask_dilate_relabel(vfs_ctx *c)
 {
+    c->nvert_version++;
     const Dims d = c->d;
     unsigned gb = grid_for(d.P, 256);
     hipLaunchKernelGGL(k_sphere_dilate, dim3(gb), dim3(256), 0, c->stream, d, c->nvert);
@@ -311,6 +313,7 @@ int vfs_mask_dilate_relabel(vfs_ctx *c)
 
 extern "C" int vfs_nvert_sphere(vfs_ctx *c, double cx, double cy, double cz, double r, double *nvert_out)
 {
+    c->nvert_version++;
     const Dims d = c->d;
     unsigned gb = grid_for(d.P, 256);
     hipLaunchKernelGGL(k_sphere_inside, dim3(gb), dim3(256), 0, c->stream, d, c->g.cent, cx, cy, cz, r, c->nvert);

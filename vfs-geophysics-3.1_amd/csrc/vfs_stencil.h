@@ -167,11 +167,22 @@ struct StoredFaceMetric {
     }
 };
 
+// A mask reader for faces with no marked cell in reach: every nvert test of
+// the face is false, so the compiler removes them.
+struct ZeroN {
+    __device__ __forceinline__ double operator()(int, int, int) const { return 0.0; }
+};
+
 // WENO: false compiles the WENO3 branch out (for kernels launched only when
 // the weno flag is off).
 // Idx: index type of the global arrays (unsigned: 32-bit offsets).
+// PLAIN: the caller guarantees that no cell the face reads is marked
+// (nvert = 0) and that the face is neither the first nor the last of its
+// direction (0 < n < m-2); every branch those conditions decide is then the
+// interior one, and the checks are compiled out.  Same expressions, so the
+// results are bit-identical to the general path on such faces.
 template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric, bool WENO = true,
-          typename Idx = long long>
+          typename Idx = long long, bool PLAIN = false>
 __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
                                             const FC &UCon, const FU &U, const double *__restrict__ nu_t,
                                             int i, int j, int k, S D[3], S V[3], const FM &fm = FM())
@@ -219,26 +230,30 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
 #define NB(tt) N(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i)
 #define UB(tt, q) U(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i, q)
     int L = n - 1, R = n + 2;
-    if (n == 0 || n == m - 2) {
-        if (per) L = m - 3, R = 2;
-        else L = n, R = n + 1;
-    } else if (NB(L) + NB(R) > 0.1) {
-        L = n, R = n + 1;
+    if (!PLAIN) {
+        if (n == 0 || n == m - 2) {
+            if (per) L = m - 3, R = 2;
+            else L = n, R = n + 1;
+        } else if (NB(L) + NB(R) > 0.1) {
+            L = n, R = n + 1;
+        }
     }
     if (d.second_order) L = n, R = n + 1;
     const S ucon_f = UCon(id);
     S ucon = ucon_f;
-    if (per && (n == 0 || n == m - 2)) ucon = UCon((Idx)(base + (Idx)(m - 2) * st));
-    // outlet: the last face convects with the flux before it (momentum.c:1227-1229)
-    if (DIR == 2 && n == m - 2 && d.bc[5] == 4 && (int)NB(n) == 0) ucon = UCon((Idx)(id - st));
+    if (!PLAIN) {
+        if (per && (n == 0 || n == m - 2)) ucon = UCon((Idx)(base + (Idx)(m - 2) * st));
+        // outlet: the last face convects with the flux before it (momentum.c:1227-1229)
+        if (DIR == 2 && n == m - 2 && d.bc[5] == 4 && (int)NB(n) == 0) ucon = UCon((Idx)(id - st));
+    }
     const S up = -0.5 * (ucon + fabs(ucon));
     const S um = -0.5 * (ucon - fabs(ucon));
-    if (d.immersed && n != m - 2 && NB(n) > 0.1) {
+    if (!PLAIN && d.immersed && n != m - 2 && NB(n) > 0.1) {
 #pragma unroll
         for (int q = 0; q < 3; q++)
             D[q] = um * (0.125 * (-UB(n + 2, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
                    up * (0.125 * (-UB(n, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q));
-    } else if (d.immersed && n != 0 && NB(n + 1) > 0.1) {
+    } else if (!PLAIN && d.immersed && n != 0 && NB(n + 1) > 0.1) {
 #pragma unroll
         for (int q = 0; q < 3; q++)
             D[q] = um * (0.125 * (-UB(n + 1, q) - 2. * UB(n + 1, q) + 3. * UB(n, q)) + UB(n + 1, q)) +
@@ -256,15 +271,15 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
     }
-    if (NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = S(0.0);
+    if (!PLAIN && NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = S(0.0);
 
     const double nu = 1. / d.ren;
     S v0, v1, v2;
     if (d.les) {
         double nut;
         const Idx id1 = id + st;   // the point across the face
-        if ((n == 0 && !per) || NB(n) > 0.1) nut = ldx(nu_t, id1);
-        else if ((n == m - 2 && !per) || NB(n + 1) > 0.1) nut = ldx(nu_t, id);
+        if (!PLAIN && ((n == 0 && !per) || NB(n) > 0.1)) nut = ldx(nu_t, id1);
+        else if (!PLAIN && ((n == m - 2 && !per) || NB(n + 1) > 0.1)) nut = ldx(nu_t, id);
         else nut = 0.5 * (ldx(nu_t, id) + ldx(nu_t, id1));
         v0 = (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nut);
         v1 = (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nut);

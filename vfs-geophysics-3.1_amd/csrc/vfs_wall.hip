@@ -74,6 +74,7 @@ __global__ void k_wall_mask(Dims d, double *nvert)
 
 int vfs_wall_mask(vfs_ctx *c)
 {
+    c->nvert_version++;
     const Dims d = c->d;
     if (d.immersed || !vfs_has_wallfn(d)) return 0;
     hipLaunchKernelGGL(k_wall_mask, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->nvert);
