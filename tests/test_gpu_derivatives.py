@@ -9,7 +9,8 @@ What pins them:
   with |U| (momentum.c:731-732).  A central difference of the oracle's F is
   therefore exact up to rounding, so J w must match it to ~1e-9.
 - VJP: the adjoint identity <v, J w> = <J^T v, w> to 1e-12 of |v| |J w|
-  for random v, w.
+  for random v, w, single entries against the JVP's, and the same bits on
+  repeated calls.
 - Newton-Krylov: the oracle's F at the returned U is below the requested
   relative tolerance (the reference's PETSc iterates are not reproducible
   without PETSc 3.1: the converged state is the check).
@@ -142,6 +143,21 @@ def test_vjp_columns_against_jvp(dcase):
             c["ctx"].rhs_jvp(dU, du, dn, V.DeviceArray.from_host(ec), jw)
             a, b = jw.host()[r], row[col]
             assert abs(a - b) <= 1e-12 * scale, (r, col, a, b)
+
+
+def test_vjp_is_deterministic(dcase):
+    """J^T v gathers every adjoint element in a fixed order (no atomics): calls
+    on the same inputs give the same bits."""
+    c = dcase
+    v = V.DeviceArray.from_host(W.unit_vector(c["cfg"], 24007))
+    dU, du, dn = dev(c)
+    out = []
+    for _ in range(3):
+        jtv = c["ctx"].vec()
+        c["ctx"].rhs_vjp(dU, du, dn, v, jtv)
+        out.append(jtv.host())
+    assert np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
 
 
 def test_newton_krylov_converges(dcase):

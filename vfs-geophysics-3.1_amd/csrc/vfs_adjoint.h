@@ -1,98 +1,14 @@
 // Transposes of the per-point stencil bodies of the momentum right-hand side.
 // Each function mirrors the branch structure of its forward counterpart in
-// vfs_stencil.h (cited there against momentum.c / k-omega.c) and scatters
-// the adjoint of its outputs into the adjoints of the inputs it read.
+// vfs_stencil.h (cited there against momentum.c / k-omega.c).
 //
-// The adjoint uses hardware FP64 atomics on device memory from hipMalloc
-// (coarse grained): sums are exact up to the order of additions.
+// The transpose of a face flux does not scatter into the adjoints of the
+// points it read.  It returns the adjoint per stencil position of the face
+// (FaceAdj), and every point later gathers the positions that name it from
+// the faces around it in a fixed order, so J^T v is the same bits on every
+// run.
 #pragma once
 #include "vfs_stencil.h"
-
-__device__ __forceinline__ void adj_add(double *p, double v)
-{
-    if (v != 0.0) unsafeAtomicAdd(p, v);
-}
-
-// Scatter into the adjoint of a cell-centred vector field at (k,j,i).
-struct VecAdj {
-    double *p;
-    Dims d;
-    __device__ __forceinline__ void operator()(int k, int j, int i, int q, double v) const
-    {
-        adj_add(p + 3 * IX(d, k, j, i) + q, v);
-    }
-};
-
-// Transpose of du_face: dcb/deb/dzb are the adjoints of dc/de/dz.
-template <typename Acc>
-__device__ __forceinline__ void du_face_adj(int dir, const Acc &A, const double *__restrict__ nvert, const Dims &d,
-                                            int i, int j, int k, const double dcb[3], const double deb[3],
-                                            const double dzb[3])
-{
-#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
-// (U(a) + U(b) - U(c) - U(e)) * s
-#define T4(s, bar, ka, ja, ia, kb, jb, ib, kc, jc, ic, ke, je, ie)                 \
-    for (int q = 0; q < 3; q++) {                                                  \
-        double w = (bar)[q] * (s);                                                 \
-        A(ka, ja, ia, q, w); A(kb, jb, ib, q, w);                                  \
-        A(kc, jc, ic, q, -w); A(ke, je, ie, q, -w);                                \
-    }
-// U(a) - U(b)
-#define T2(bar, ka, ja, ia, kb, jb, ib)                                            \
-    for (int q = 0; q < 3; q++) { A(ka, ja, ia, q, (bar)[q]); A(kb, jb, ib, q, -(bar)[q]); }
-    if (dir == 0) {
-        T2(dcb, k, j, i + 1, k, j, i);
-        if (NV(k, j + 1, i) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
-            T4(0.5, deb, k, j, i + 1, k, j, i, k, j - 1, i + 1, k, j - 1, i);
-        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k, j - 1, i + 1) > VFS_SOLID) {
-            T4(0.5, deb, k, j + 1, i + 1, k, j + 1, i, k, j, i + 1, k, j, i);
-        } else {
-            T4(0.25, deb, k, j + 1, i + 1, k, j + 1, i, k, j - 1, i + 1, k, j - 1, i);
-        }
-        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
-            T4(0.5, dzb, k, j, i + 1, k, j, i, k - 1, j, i + 1, k - 1, j, i);
-        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j, i + 1) > VFS_SOLID) {
-            T4(0.5, dzb, k + 1, j, i + 1, k + 1, j, i, k, j, i + 1, k, j, i);
-        } else {
-            T4(0.25, dzb, k + 1, j, i + 1, k + 1, j, i, k - 1, j, i + 1, k - 1, j, i);
-        }
-    } else if (dir == 1) {
-        if (NV(k, j, i + 1) > VFS_SOLID || NV(k, j + 1, i + 1) > VFS_SOLID) {
-            T4(0.5, dcb, k, j + 1, i, k, j, i, k, j + 1, i - 1, k, j, i - 1);
-        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k, j + 1, i - 1) > VFS_SOLID) {
-            T4(0.5, dcb, k, j + 1, i + 1, k, j, i + 1, k, j + 1, i, k, j, i);
-        } else {
-            T4(0.25, dcb, k, j + 1, i + 1, k, j, i + 1, k, j + 1, i - 1, k, j, i - 1);
-        }
-        T2(deb, k, j + 1, i, k, j, i);
-        if (NV(k + 1, j, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
-            T4(0.5, dzb, k, j + 1, i, k, j, i, k - 1, j + 1, i, k - 1, j, i);
-        } else if (NV(k - 1, j, i) > VFS_SOLID || NV(k - 1, j + 1, i) > VFS_SOLID) {
-            T4(0.5, dzb, k + 1, j + 1, i, k + 1, j, i, k, j + 1, i, k, j, i);
-        } else {
-            T4(0.25, dzb, k + 1, j + 1, i, k + 1, j, i, k - 1, j + 1, i, k - 1, j, i);
-        }
-    } else {
-        if (NV(k, j, i + 1) > VFS_SOLID || NV(k + 1, j, i + 1) > VFS_SOLID) {
-            T4(0.5, dcb, k + 1, j, i, k, j, i, k + 1, j, i - 1, k, j, i - 1);
-        } else if (NV(k, j, i - 1) > VFS_SOLID || NV(k + 1, j, i - 1) > VFS_SOLID) {
-            T4(0.5, dcb, k + 1, j, i + 1, k, j, i + 1, k + 1, j, i, k, j, i);
-        } else {
-            T4(0.25, dcb, k + 1, j, i + 1, k, j, i + 1, k + 1, j, i - 1, k, j, i - 1);
-        }
-        if (NV(k, j + 1, i) > VFS_SOLID || NV(k + 1, j + 1, i) > VFS_SOLID) {
-            T4(0.5, deb, k + 1, j, i, k, j, i, k + 1, j - 1, i, k, j - 1, i);
-        } else if (NV(k, j - 1, i) > VFS_SOLID || NV(k + 1, j - 1, i) > VFS_SOLID) {
-            T4(0.5, deb, k + 1, j + 1, i, k, j + 1, i, k + 1, j, i, k, j, i);
-        } else {
-            T4(0.25, deb, k + 1, j + 1, i, k, j + 1, i, k + 1, j - 1, i, k, j - 1, i);
-        }
-        T2(dzb, k + 1, j, i, k, j, i);
-    }
-#undef T4
-#undef T2
-#undef NV
-}
 
 // weno3 (level.c:1734-1771) and its gradient with respect to f0..f3 (the
 // wave speed only selects the upwind side).  With fL, fC, fR the upwind
@@ -121,6 +37,24 @@ __device__ __forceinline__ double weno3_grad(double f0, double f1, double f2, do
     return w0 * u0 + w1 * u1;
 }
 
+// The adjoint of one face flux, by stencil position.  For a face of stored
+// index n along DIR (cells c0 = n and c1 = n+1 along DIR):
+//  S[r][q]     ucat component q at the face's own row, along-DIR position
+//              n + r - 1 (r = 0..3); in a periodic direction the first face
+//              (n = 0) has r = 0 at m-3 and the last one (n = m-2) has r = 3
+//              at 2, the positions its 4th-order stencil wraps to;
+//  X[tt][s][q] ucat component q one row to the - (s = 0) or + (s = 1) side in
+//              the tt-th transverse direction, at positions n and n+1 alike;
+//  uself       ucont component DIR of the face itself;
+//  usrc        ucont component DIR of face m-2, which the first face of a
+//              periodic direction convects with.
+struct FaceAdj {
+    double S[4][3];
+    double X[2][2][3];
+    double uself, usrc;
+};
+constexpr int FACE_ADJ_SLOTS = 26;
+
 // Transpose of face_flux_t<DIR, double> with respect to ucont (component DIR)
 // and ucat, at the linearisation point (ucont, ucat); nu_t is frozen.
 // Db/Vb are the adjoints of the convective and viscous face fluxes.
@@ -128,8 +62,19 @@ template <int DIR>
 __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, const double *__restrict__ nvert,
                                               const double *__restrict__ ucont, const double *__restrict__ ucat,
                                               const double *__restrict__ nu_t, int i, int j, int k,
-                                              const double Db[3], const double Vb[3], double *ucont_b, double *ucat_b)
+                                              const double Db[3], const double Vb[3], FaceAdj &F)
 {
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) F.S[r][q] = 0;
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) F.X[a][s][q] = 0;
+    F.uself = F.usrc = 0;
     const long long id = IX(d, k, j, i);
     const double *fc = DIR == 0 ? g.icsi : DIR == 1 ? g.jcsi : g.kcsi;
     const double *fe = DIR == 0 ? g.ieta : DIR == 1 ? g.jeta : g.keta;
@@ -151,7 +96,6 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
     const long long base = id - (long long)n * st;
 #define NB(tt) nvert[base + (long long)(tt) * st]
 #define UB(tt, q) ucat[3 * (base + (long long)(tt) * st) + (q)]
-#define AU(tt, q, v) adj_add(ucat_b + 3 * (base + (long long)(tt) * st) + (q), (v))
     int L = n - 1, R = n + 2;
     if (n == 0 || n == m - 2) {
         if (per) L = m - 3, R = 2;
@@ -160,6 +104,8 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
         L = n, R = n + 1;
     }
     if (d.second_order) L = n, R = n + 1;
+    // stencil slots of L and R: the upwind neighbours, or the face's own cells
+    const int sL = L == n ? 1 : 0, sR = R == n + 1 ? 2 : 3;
     const long long of = base + (long long)n * st;
     const long long ou = (per && (n == 0 || n == m - 2)) ? base + (long long)(m - 2) * st : of;
     const double ucon_f = ucont[3 * of + DIR];
@@ -177,9 +123,9 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
                 const double b = 0.125 * (-UB(n, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q);
                 umb += Db[q] * a;
                 upb += Db[q] * b;
-                AU(n + 2, q, Db[q] * um * -0.125);
-                AU(n + 1, q, Db[q] * (um * 0.75 + up * 0.375));
-                AU(n, q, Db[q] * (um * 0.375 + up * 0.625));
+                F.S[3][q] += Db[q] * um * -0.125;
+                F.S[2][q] += Db[q] * (um * 0.75 + up * 0.375);
+                F.S[1][q] += Db[q] * (um * 0.375 + up * 0.625);
             }
             uconb += upb * (-0.5 * (1. + sg)) + umb * (-0.5 * (1. - sg));
         } else if (d.immersed && n != 0 && NB(n + 1) > 0.1) {
@@ -189,16 +135,16 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
                 const double b = 0.125 * (-UB(n - 1, q) - 2. * UB(n, q) + 3. * UB(n + 1, q)) + UB(n, q);
                 umb += Db[q] * a;
                 upb += Db[q] * b;
-                AU(n + 1, q, Db[q] * (um * 0.625 + up * 0.375));
-                AU(n, q, Db[q] * (um * 0.375 + up * 0.75));
-                AU(n - 1, q, Db[q] * up * -0.125);
+                F.S[2][q] += Db[q] * (um * 0.625 + up * 0.375);
+                F.S[1][q] += Db[q] * (um * 0.375 + up * 0.75);
+                F.S[0][q] += Db[q] * up * -0.125;
             }
             uconb += upb * (-0.5 * (1. + sg)) + umb * (-0.5 * (1. - sg));
         } else if (d.second_order) {
             for (int q = 0; q < 3; q++) {
                 ucon_fb += Db[q] * (-0.5 * (UB(n, q) + UB(n + 1, q)));
-                AU(n, q, Db[q] * -ucon_f * 0.5);
-                AU(n + 1, q, Db[q] * -ucon_f * 0.5);
+                F.S[1][q] += Db[q] * -ucon_f * 0.5;
+                F.S[2][q] += Db[q] * -ucon_f * 0.5;
             }
         } else if (d.weno) {
             for (int q = 0; q < 3; q++) {
@@ -206,23 +152,27 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
                 const double W = weno3_grad(UB(L, q), UB(n, q), UB(n + 1, q), UB(R, q), ucon_f, gw);
                 ucon_fb += Db[q] * -W;
                 const double c = Db[q] * -ucon_f;
-                AU(L, q, c * gw[0]);
-                AU(n, q, c * gw[1]);
-                AU(n + 1, q, c * gw[2]);
-                AU(R, q, c * gw[3]);
+                F.S[sL][q] += c * gw[0];
+                F.S[1][q] += c * gw[1];
+                F.S[2][q] += c * gw[2];
+                F.S[sR][q] += c * gw[3];
             }
         } else {
             const double c = -ucon_f * 0.0625;
             for (int q = 0; q < 3; q++) {
                 ucon_fb += Db[q] * (-0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q)));
-                AU(L, q, -c * Db[q]);
-                AU(n, q, 9. * c * Db[q]);
-                AU(n + 1, q, 9. * c * Db[q]);
-                AU(R, q, -c * Db[q]);
+                F.S[sL][q] += -c * Db[q];
+                F.S[1][q] += 9. * c * Db[q];
+                F.S[2][q] += 9. * c * Db[q];
+                F.S[sR][q] += -c * Db[q];
             }
         }
-        adj_add(ucont_b + 3 * of + DIR, ucon_fb);
-        adj_add(ucont_b + 3 * ou + DIR, uconb);
+        if (ou == of) {
+            F.uself = ucon_fb + uconb;
+        } else {
+            F.uself = ucon_fb;
+            F.usrc = uconb;
+        }
     }
 
     double W = ajc * (1. / d.ren);
@@ -245,8 +195,37 @@ __device__ __forceinline__ void face_flux_adj(const Dims &d, const Geom &g, cons
             dzb[a] += vb * nn[a] * ze[q];
         }
     }
-    du_face_adj(DIR, VecAdj{ucat_b, d}, nvert, d, i, j, k, dcb, deb, dzb);
+    // transpose of du_face: the normal difference U(c1) - U(c0), then the two
+    // transverse differences (U(c1 + p) + U(c0 + p) - U(c1 + m) - U(c0 + m)) * s
+    // with du_face's choice of (p, m, s)
+    const double *dnb = DIR == 0 ? dcb : DIR == 1 ? deb : dzb;
+#pragma unroll
+    for (int q = 0; q < 3; q++) { F.S[2][q] += dnb[q]; F.S[1][q] -= dnb[q]; }
+    const int i1 = i + (DIR == 0), j1 = j + (DIR == 1), k1 = k + (DIR == 2);
+#define NV(kk, jj, ii) nvert[IX(d, kk, jj, ii)]
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int t = DIR == 0 ? 1 + tt : DIR == 1 ? 2 * tt : tt;
+        const int ti = t == 0, tj = t == 1, tk = t == 2;
+        int p = 1, mm = -1;
+        double sc = 0.25;
+        if (NV(k + tk, j + tj, i + ti) > VFS_SOLID || NV(k1 + tk, j1 + tj, i1 + ti) > VFS_SOLID) {
+            p = 0; mm = -1; sc = 0.5;
+        } else if (NV(k - tk, j - tj, i - ti) > VFS_SOLID || NV(k1 - tk, j1 - tj, i1 - ti) > VFS_SOLID) {
+            p = 1; mm = 0; sc = 0.5;
+        }
+        const double *tb = t == 0 ? dcb : t == 1 ? deb : dzb;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const double w = tb[q] * sc;
+            // + w at transverse offset p, - w at offset mm, at c1 and c0 alike
+            if (p == 0) { F.S[2][q] += w; F.S[1][q] += w; }
+            else F.X[tt][1][q] += w;
+            if (mm == 0) { F.S[2][q] -= w; F.S[1][q] -= w; }
+            else F.X[tt][0][q] -= w;
+        }
+    }
+#undef NV
 #undef NB
 #undef UB
-#undef AU
 }

@@ -184,6 +184,7 @@ extern "C" int vfs_ctx_destroy(vfs_ctx *c)
     hipFree(c->geom_block); hipFree(c->nvert); hipFree(c->face); hipFree(c->fp);
     hipFree(c->snap); hipFree(c->tmp3); hipFree(c->red); hipFree(c->kflux);
     if (c->ad) hipFree(c->ad);
+    if (c->vjp_ws) hipFree(c->vjp_ws);
     if (c->wf_table) hipFree(c->wf_table);
     if (c->ustar) hipFree(c->ustar);
     if (c->dsm) hipFree(c->dsm);

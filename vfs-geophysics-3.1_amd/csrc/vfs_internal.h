@@ -135,6 +135,7 @@ struct vfs_ctx {
     double *tmp3;            // 3*P general
     double *red;             // reduction partials + result scalars
     double *ad;              // 12*P: primal and adjoint state of the JVP/VJP (allocated on first use)
+    double *vjp_ws;          // 26*P: the face adjoints of one direction in the VJP (allocated on first use)
     void *nk;                // Newton-Krylov workspace (allocated on first use)
     double *wf_table;        // wall-function y+ integral table (bctype -1 only)
     double *ustar;           // P: friction velocity of the wall-function cells

@@ -14,7 +14,8 @@
 //
 // JVP: the linear stages run the forward kernels on the tangent, and the
 // face fluxes run in dual numbers (value, tangent).
-// VJP: every stage is transposed by hand in reverse order (vfs_adjoint.h).
+// VJP: every stage is transposed by hand in reverse order (vfs_adjoint.h),
+// each as a gather with a fixed summation order.
 #include "vfs_adjoint.h"
 #include "vfs_internal.h"
 
@@ -157,8 +158,29 @@ extern "C" int vfs_rhs_jvp(vfs_ctx *c, const double *U, const double *ucat, cons
 // ---------------------------------------------------------------------------
 // VJP
 // ---------------------------------------------------------------------------
-// Transpose of the rhs projection with masks (k_ff2_rhs, momentum.c:1673-1900)
-// and of the -Mb U/dt term: out = -vb/dt, fpb += scale * P^T vb.
+// Every transpose is a gather: each adjoint element sums, in a fixed order,
+// the contributions of the forward outputs that read it, so no two threads
+// write one element, no atomics are used and J^T v is the same bits on every
+// run.
+
+// v at cell t, component dir, with the face masks of the projection
+// (momentum.c:1795-1803)
+__device__ __forceinline__ double vb_masked(const Dims &d, const double *__restrict__ nvert,
+                                            const double *__restrict__ v, long long t, int i, int j, int k, int dir)
+{
+    const long long s = dir == 0 ? 1 : dir == 1 ? (long long)d.mx : (long long)d.mx * d.my;
+    const int c = dir == 0 ? i : dir == 1 ? j : k;
+    const int m = dir == 0 ? d.mx : dir == 1 ? d.my : d.mz;
+    const int per = dir == 0 ? d.ip : dir == 1 ? d.jp : d.kp;
+    if (nvert[t] + nvert[t + s] > 0.1 || (!per && c == m - 2)) return 0.0;
+    return v[3 * t + dir];
+}
+
+// Transpose of the rhs projection with masks (k_ff2_rhs, momentum.c:
+// 1673-1900) and of the -Mb U/dt term: out = -vb/dt and fpb = scale P^T vb.
+// A cell's fp is read by its own three faces and by the i / j / k face of the
+// cell before it (across a periodic seam, the last cell), with that cell's
+// metric at its +1 neighbour.
 __global__ void k_vjp_rhs(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ v,
                           double *__restrict__ fpb, double *__restrict__ out, double scale)
 {
@@ -166,116 +188,261 @@ __global__ void k_vjp_rhs(Dims d, Geom g, const double *__restrict__ nvert, cons
     if (t >= d.P) return;
     POINT_IJK(t);
     if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) {
-        out[3 * t] = out[3 * t + 1] = out[3 * t + 2] = 0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) out[3 * t + q] = fpb[3 * t + q] = 0;
         return;
     }
-    double vb[3] = {v[3 * t], v[3 * t + 1], v[3 * t + 2]};
-    const double n0 = nvert[t];
-    if (n0 + nvert[t + 1] > 0.1 || (!d.ip && i == d.mx - 2)) vb[0] = 0;
-    if (n0 + nvert[t + d.mx] > 0.1 || (!d.jp && j == d.my - 2)) vb[1] = 0;
-    if (n0 + nvert[t + (long long)d.mx * d.my] > 0.1 || (!d.kp && k == d.mz - 2)) vb[2] = 0;
+    double vb[3];
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) vb[dir] = vb_masked(d, nvert, v, t, i, j, k, dir);
     const double dt = d.dt;
 #pragma unroll
     for (int q = 0; q < 3; q++) out[3 * t + q] = -1. / dt * vb[q];
-    const long long a = IX(d, k, j, wrap_cell(i + 1, d.mx, d.ip)), b = IX(d, k, wrap_cell(j + 1, d.my, d.jp), i),
-                    cc = IX(d, wrap_cell(k + 1, d.mz, d.kp), j, i);
-    const long long ga = t + 1, gb = t + d.mx, gc = t + (long long)d.mx * d.my;
     const double w0 = scale * 0.5 * g.iaj[t] * vb[0], w1 = scale * 0.5 * g.jaj[t] * vb[1],
                  w2 = scale * 0.5 * g.kaj[t] * vb[2];
+    double f[3];
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        adj_add(fpb + 3 * t + q, g.csi[3 * t + q] * w0 + g.eta[3 * t + q] * w1 + g.zet[3 * t + q] * w2);
-        adj_add(fpb + 3 * a + q, g.csi[3 * ga + q] * w0);
-        adj_add(fpb + 3 * b + q, g.eta[3 * gb + q] * w1);
-        adj_add(fpb + 3 * cc + q, g.zet[3 * gc + q] * w2);
+    for (int q = 0; q < 3; q++) f[q] = g.csi[3 * t + q] * w0 + g.eta[3 * t + q] * w1 + g.zet[3 * t + q] * w2;
+    const int ib = i > 1 ? i - 1 : (d.ip ? d.mx - 2 : 0);
+    if (ib >= 1) {
+        const long long c = IX(d, k, j, ib);
+        const double w = scale * 0.5 * g.iaj[c] * vb_masked(d, nvert, v, c, ib, j, k, 0);
+#pragma unroll
+        for (int q = 0; q < 3; q++) f[q] += g.csi[3 * (c + 1) + q] * w;
     }
+    const int jb = j > 1 ? j - 1 : (d.jp ? d.my - 2 : 0);
+    if (jb >= 1) {
+        const long long c = IX(d, k, jb, i);
+        const double w = scale * 0.5 * g.jaj[c] * vb_masked(d, nvert, v, c, i, jb, k, 1);
+#pragma unroll
+        for (int q = 0; q < 3; q++) f[q] += g.eta[3 * (c + d.mx) + q] * w;
+    }
+    const int kb = k > 1 ? k - 1 : (d.kp ? d.mz - 2 : 0);
+    if (kb >= 1) {
+        const long long c = IX(d, kb, j, i);
+        const double w = scale * 0.5 * g.kaj[c] * vb_masked(d, nvert, v, c, i, j, kb, 2);
+#pragma unroll
+        for (int q = 0; q < 3; q++) f[q] += g.zet[3 * (c + (long long)d.mx * d.my) + q] * w;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) fpb[3 * t + q] = f[q];
 }
 
-// Transpose of cell_fp (momentum.c:1506-1636)
+// Transpose of cell_fp (momentum.c:1506-1636): the adjoint of each face flux
+// from the cells whose divergence reads it.  Along its direction a face f is
+// the own face of cell f, the upwind face of cell f+1 and a far face of the
+// 4th-order correction of cells f-1 .. f+2 (periodic-wrapped).
 __global__ void k_vjp_cells(Dims d, const double *__restrict__ nvert, const double *__restrict__ fpb,
                             double *__restrict__ face)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
-    if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
     const long long P = d.P;
-    const CellFaces c = cell_faces(d, nvert, i, j, k);
     const double cdv = d.second_order ? 1.0 : 9. / 8.;
-#define FA(kind, q, kk, jj, ii, v) adj_add(face + (long long)(3 * (kind) + (q)) * P + IX(d, kk, jj, ii), (v))
 #pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const double fb = fpb[3 * t + q];
-        if (fb == 0.0) continue;
-        FA(0, q, k, j, i, cdv * fb); FA(0, q, k, j, c.im, -cdv * fb);
-        FA(1, q, k, j, i, cdv * fb); FA(1, q, k, c.jm, i, -cdv * fb);
-        FA(2, q, k, j, i, cdv * fb); FA(2, q, c.km, j, i, -cdv * fb);
-        if (!d.second_order) {
-            const double wi = -1. / 8. * (1. / c.den_i) * fb, wj = -1. / 8. * (1. / c.den_j) * fb,
-                         wk = -1. / 8. * (1. / c.den_k) * fb;
-            FA(0, q, k, j, c.iR, wi); FA(0, q, k, j, c.iL, -wi);
-            FA(1, q, k, c.jR, i, wj); FA(1, q, k, c.jL, i, -wj);
-            FA(2, q, c.kR, j, i, wk); FA(2, q, c.kL, j, i, -wk);
+    for (int dir = 0; dir < 3; dir++) {
+        double D[3] = {0, 0, 0}, Vv[3] = {0, 0, 0};
+        const int f = dir == 0 ? i : dir == 1 ? j : k;
+        const int m = dir == 0 ? d.mx : dir == 1 ? d.my : d.mz;
+        const int per = dir == 0 ? d.ip : dir == 1 ? d.jp : d.kp;
+        const bool row = (dir == 0 || (i >= 1 && i <= d.mx - 2)) && (dir == 1 || (j >= 1 && j <= d.my - 2)) &&
+                         (dir == 2 || (k >= 1 && k <= d.mz - 2));
+        if (row && f <= m - 2) {
+            for (int o = -1; o <= 2; o++) {
+                int cn = f + o;
+                if (cn < 1 || cn > m - 2) {
+                    if (!per) continue;
+                    cn = cn < 1 ? cn + (m - 2) : cn - (m - 2);
+                }
+                const int ci = dir == 0 ? cn : i, cj = dir == 1 ? cn : j, ck = dir == 2 ? cn : k;
+                const CellFaces cf = cell_faces(d, nvert, ci, cj, ck);
+                const int im = dir == 0 ? cf.im : dir == 1 ? cf.jm : cf.km;
+                const int fR = dir == 0 ? cf.iR : dir == 1 ? cf.jR : cf.kR;
+                const int fL = dir == 0 ? cf.iL : dir == 1 ? cf.jL : cf.kL;
+                const double den = dir == 0 ? cf.den_i : dir == 1 ? cf.den_j : cf.den_k;
+                const long long c = IX(d, ck, cj, ci);
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    const double fb = fpb[3 * c + q];
+                    if (cn == f) { D[q] += cdv * fb; Vv[q] += fb; }
+                    if (im == f) { D[q] += -cdv * fb; Vv[q] += -fb; }
+                    if (!d.second_order) {
+                        const double w = -1. / 8. * (1. / den) * fb;
+                        if (fR == f) D[q] += w;
+                        if (fL == f) D[q] += -w;
+                    }
+                }
+            }
         }
-        FA(3, q, k, j, i, fb); FA(3, q, k, j, c.im, -fb);
-        FA(4, q, k, j, i, fb); FA(4, q, k, c.jm, i, -fb);
-        FA(5, q, k, j, i, fb); FA(5, q, c.km, j, i, -fb);
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            face[(long long)(3 * dir + q) * P + t] = D[q];
+            face[(long long)(3 * (3 + dir) + q) * P + t] = Vv[q];
+        }
     }
-#undef FA
 }
 
-// Transpose of the face fluxes (same face set as k_ff2_faces)
+// The adjoints of the DIR faces by stencil position (FaceAdj, slot-major:
+// ws[s * P + t]), zero at points that hold no such face.  The faces are
+// those of k_ff2_faces.
+template <int DIR>
 __global__ void k_vjp_faces(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
                             const double *__restrict__ ucat, const double *__restrict__ nu_t,
-                            const double *__restrict__ face, double *ucont_b, double *ucat_b)
+                            const double *__restrict__ face, double *__restrict__ ws)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
-    if (i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
     const long long P = d.P;
-    double Db[3], Vb[3];
-    if (j >= 1 && k >= 1) {
-        for (int q = 0; q < 3; q++) { Db[q] = face[(0 * 3 + q) * P + t]; Vb[q] = face[(3 * 3 + q) * P + t]; }
-        face_flux_adj<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, Db, Vb, ucont_b, ucat_b);
+    FaceAdj F;
+    const bool isface = i <= d.mx - 2 && j <= d.my - 2 && k <= d.mz - 2 && (DIR == 0 || i >= 1) &&
+                        (DIR == 1 || j >= 1) && (DIR == 2 || k >= 1);
+    double Db[3] = {0, 0, 0}, Vb[3] = {0, 0, 0};
+    bool any = false;
+    if (isface) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            Db[q] = face[(long long)(DIR * 3 + q) * P + t];
+            Vb[q] = face[(long long)((3 + DIR) * 3 + q) * P + t];
+            any = any || Db[q] != 0.0 || Vb[q] != 0.0;
+        }
     }
-    if (i >= 1 && k >= 1) {
-        for (int q = 0; q < 3; q++) { Db[q] = face[(1 * 3 + q) * P + t]; Vb[q] = face[(4 * 3 + q) * P + t]; }
-        face_flux_adj<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, Db, Vb, ucont_b, ucat_b);
+    if (any) {
+        face_flux_adj<DIR>(d, g, nvert, ucont, ucat, nu_t, i, j, k, Db, Vb, F);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) F.S[r][q] = 0;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+#pragma unroll
+                for (int q = 0; q < 3; q++) F.X[a][s][q] = 0;
+        F.uself = F.usrc = 0;
     }
-    if (i >= 1 && j >= 1) {
-        for (int q = 0; q < 3; q++) { Db[q] = face[(2 * 3 + q) * P + t]; Vb[q] = face[(5 * 3 + q) * P + t]; }
-        face_flux_adj<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, Db, Vb, ucont_b, ucat_b);
-    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 3; q++) ws[(long long)(3 * r + q) * P + t] = F.S[r][q];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) ws[(long long)(12 + (2 * a + s) * 3 + q) * P + t] = F.X[a][s][q];
+    ws[24 * P + t] = F.uself;
+    ws[25 * P + t] = F.usrc;
 }
 
-// Transpose of the periodic ghost copy of a vector field (k_periodic_vec):
-// ghost adjoints move onto their sources; sources are never ghosts.
-__global__ void k_periodic_vec_T(Dims d, double *v)
+// ucat_b (+)= the DIR-face adjoints at this point, cb[DIR] = the ucont
+// adjoint of the DIR face here.  Order: the along-DIR slots of the faces
+// x+1, x, x-1, x-2 (x: this point's DIR index), the wrapped slots of a
+// periodic direction, then per transverse direction the faces of the row
+// above and below, each at positions n and n+1.
+template <int DIR>
+__global__ void k_vjp_gather(Dims d, const double *__restrict__ ws, double *__restrict__ ub,
+                             double *__restrict__ cb, int accumulate)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int k, j, i;
-    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
-    int a, b, c;
-    if (!per_src(d, i, j, k, a, b, c)) return;
-    long long s = IX(d, c, b, a), o = IX(d, k, j, i);
+    if (t >= d.P) return;
+    POINT_IJK(t);
+    const long long P = d.P;
+    const int x = DIR == 0 ? i : DIR == 1 ? j : k;
+    const int m = DIR == 0 ? d.mx : DIR == 1 ? d.my : d.mz;
+    const int per = DIR == 0 ? d.ip : DIR == 1 ? d.jp : d.kp;
+    const long long st = DIR == 0 ? 1 : DIR == 1 ? (long long)d.mx : (long long)d.mx * d.my;
+    double acc[3];
+#pragma unroll
+    for (int q = 0; q < 3; q++) acc[q] = accumulate ? ub[3 * t + q] : 0.0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int n = x + 1 - r;
+        if (n < 0 || n > m - 1) continue;
+        const long long f = t + (long long)(n - x) * st;
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += ws[(long long)(3 * r + q) * P + f];
+    }
+    if (per && x == m - 3) {
+        const long long f = t - (long long)x * st;   // face 0: its slot 0 is cell m-3
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += ws[(long long)q * P + f];
+    }
+    if (per && x == 2) {
+        const long long f = t + (long long)(m - 2 - x) * st;   // face m-2: its slot 3 is cell 2
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += ws[(long long)(9 + q) * P + f];
+    }
+#pragma unroll
+    for (int tt = 0; tt < 2; tt++) {
+        const int td = DIR == 0 ? 1 + tt : DIR == 1 ? 2 * tt : tt;
+        const int y = td == 0 ? i : td == 1 ? j : k;
+        const int my_ = td == 0 ? d.mx : td == 1 ? d.my : d.mz;
+        const long long sy = td == 0 ? 1 : td == 1 ? (long long)d.mx : (long long)d.mx * d.my;
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            const int yf = s == 0 ? y + 1 : y - 1;   // the face row this point is the -/+ neighbour of
+            if (yf < 0 || yf > my_ - 1) continue;
+#pragma unroll
+            for (int sn = 0; sn < 2; sn++) {
+                const int n = x - sn;
+                if (n < 0) continue;
+                const long long f = t + (long long)(yf - y) * sy - (long long)sn * st;
+#pragma unroll
+                for (int q = 0; q < 3; q++) acc[q] += ws[(long long)(12 + (2 * tt + s) * 3 + q) * P + f];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) ub[3 * t + q] = acc[q];
+    double u = ws[24 * P + t];
+    if (per && x == m - 2) u += ws[25 * P + (t - (long long)x * st)];
+    cb[3 * t + DIR] = u;
+}
+
+// Transpose of the periodic ghost copy of a vector field (k_periodic_vec)
+// along one direction: each ghost plane's adjoint moves onto the plane it
+// copies.  The copy of k_periodic_vec equals the copies along i, then j, then
+// k, so its transpose is these moves along k, then j, then i; each move has
+// one source per destination.
+__global__ void k_periodic_vec_T_dir(Dims d, int dir, double *v)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long A = dir == 0 ? (long long)d.my * d.mz : dir == 1 ? (long long)d.mx * d.mz : (long long)d.mx * d.my;
+    if (t >= 2 * A) return;
+    const bool hi = t >= A;
+    const long long r = hi ? t - A : t;
+    int i, j, k;
+    if (dir == 0) { j = (int)(r % d.my); k = (int)(r / d.my); i = hi ? d.mx - 1 : 0; }
+    else if (dir == 1) { i = (int)(r % d.mx); k = (int)(r / d.mx); j = hi ? d.my - 1 : 0; }
+    else { i = (int)(r % d.mx); j = (int)(r / d.mx); k = hi ? d.mz - 1 : 0; }
+    const int src = hi ? 1 : (dir == 0 ? d.mx : dir == 1 ? d.my : d.mz) - 2;
+    const long long o = IX(d, k, j, i);
+    const long long s = dir == 0 ? IX(d, k, j, src) : dir == 1 ? IX(d, k, src, i) : IX(d, src, j, i);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
-        adj_add(v + 3 * s + q, v[3 * o + q]);
+        v[3 * s + q] += v[3 * o + q];
         v[3 * o + q] = 0;
     }
 }
 
 static int periodic_vec_T(vfs_ctx *c, double *v)
 {
-    if (!(c->d.ip || c->d.jp || c->d.kp)) return 0;
-    long long n = shell_count(c->d);
-    hipLaunchKernelGGL(k_periodic_vec_T, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, c->d, v);
-    VFS_CHECK_LAUNCH();
+    const Dims d = c->d;
+    const int per[3] = {d.ip, d.jp, d.kp};
+    for (int dir = 2; dir >= 0; dir--) {
+        if (!per[dir]) continue;
+        const long long A = dir == 0 ? (long long)d.my * d.mz : dir == 1 ? (long long)d.mx * d.mz : (long long)d.mx * d.my;
+        hipLaunchKernelGGL(k_periodic_vec_T_dir, dim3(grid_for(2 * A, 256)), dim3(256), 0, c->stream, d, dir, v);
+        VFS_CHECK_LAUNCH();
+    }
     return 0;
 }
 
-// Transpose of the periodic face copies of IB_BC (momentum.c:2167-2181)
+// Transpose of the periodic face copies of IB_BC (momentum.c:2167-2181); each
+// face source has one ghost copy
 __global__ void k_periodic_faces_T(Dims d, double *u)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -284,7 +451,7 @@ __global__ void k_periodic_faces_T(Dims d, double *u)
     long long o = IX(d, k, j, i);
 #define MOVE(q, src)                                                  \
     {                                                                 \
-        adj_add(u + 3 * (src) + (q), u[3 * o + (q)]);                 \
+        u[3 * (src) + (q)] += u[3 * o + (q)];                         \
         u[3 * o + (q)] = 0;                                           \
     }
     if (d.ip && i == 0) MOVE(0, IX(d, k, j, d.mx - 2));
@@ -296,29 +463,54 @@ __global__ void k_periodic_faces_T(Dims d, double *u)
 #undef MOVE
 }
 
-// Transpose of the IB face rule of IB_BC (momentum.c:2088-2124)
-__global__ void k_ibbc_T(Dims d, Geom g, const double *__restrict__ nvert, double *ucont_b, double *ucat_b)
+// Transpose of the IB face rule of IB_BC (momentum.c:2088-2124), in two
+// passes: w = the ucont adjoint of the IB faces (then zeroed there); ucat_b
+// gathers 0.5 * metric * w of its own faces and of those of its -1
+// neighbours.
+__global__ void k_ibbc_T_take(Dims d, const double *__restrict__ nvert, double *cb, double *__restrict__ w)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
-    if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
-    const long long sj = d.mx, sk = (long long)d.mx * d.my;
-    const bool self = (int)nvert[t] == 1;
-    const long long nb[3] = {t + 1, t + sj, t + sk};
-    const double *mt[3] = {g.icsi + 3 * t, g.jeta + 3 * t, g.kzet + 3 * t};
+    const bool in = i >= 1 && j >= 1 && k >= 1 && i <= d.mx - 2 && j <= d.my - 2 && k <= d.mz - 2;
+    const long long nb[3] = {t + 1, t + d.mx, t + (long long)d.mx * d.my};
+    const bool self = in && (int)nvert[t] == 1;
 #pragma unroll
     for (int dir = 0; dir < 3; dir++) {
-        if (!(self || (int)(nvert[nb[dir]]) == 1)) continue;
-        const double val = ucont_b[3 * t + dir];
-        ucont_b[3 * t + dir] = 0;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const double w = 0.5 * mt[dir][q] * val;
-            adj_add(ucat_b + 3 * t + q, w);
-            adj_add(ucat_b + 3 * nb[dir] + q, w);
+        double val = 0;
+        if (in && (self || (int)(nvert[nb[dir]]) == 1)) {
+            val = cb[3 * t + dir];
+            cb[3 * t + dir] = 0;
         }
+        w[3 * t + dir] = val;
     }
+}
+
+__global__ void k_ibbc_T_give(Dims d, Geom g, const double *__restrict__ w, double *__restrict__ ub)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    POINT_IJK(t);
+    const long long sd[3] = {1, d.mx, (long long)d.mx * d.my};
+    const int cd[3] = {i, j, k};
+    const double *mt[3] = {g.icsi, g.jeta, g.kzet};
+    double acc[3] = {ub[3 * t], ub[3 * t + 1], ub[3 * t + 2]};
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        const double val = w[3 * t + dir];
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += 0.5 * mt[dir][3 * t + q] * val;
+    }
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        if (cd[dir] < 1) continue;
+        const long long c = t - sd[dir];
+        const double val = w[3 * c + dir];
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += 0.5 * mt[dir][3 * c + q] * val;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) ub[3 * t + q] = acc[q];
 }
 
 // Transpose of k_c2c_bc_interior: zero the adjoint where ucat was zeroed
@@ -363,25 +555,48 @@ __global__ void k_c2c_bc_ghost_T_zero(Dims d, const double *__restrict__ nvert, 
     ub[3 * id] = ub[3 * id + 1] = ub[3 * id + 2] = 0;
 }
 
-__global__ void k_c2c_bc_ghost_T_add(Dims d, const double *__restrict__ nvert, const double *__restrict__ snap, double *ub)
-{
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    int k, j, i;
-    if (t >= shell_count(d) || !shell_point(d, t, k, j, i)) return;
-    long long s = c2c_ghost_rule(d, nvert, i, j, k);
-    if (s < 0) return;
-    long long id = IX(d, k, j, i);
-#pragma unroll
-    for (int q = 0; q < 3; q++) adj_add(ub + 3 * s + q, -1 * snap[3 * id + q]);
-}
-
-// Transpose of the Cramer solve of k_c2c_interior (rhs.c:203-231)
-__global__ void k_c2c_interior_T(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ub,
-                                 double *cb)
+// ub at a mirror source += -1 * (the adjoint of each ghost mirroring it, from
+// the snapshot), the ghosts taken in the order -i, +i, -j, +j, -k, +k
+__global__ void k_c2c_bc_ghost_T_gather(Dims d, const double *__restrict__ nvert, const double *__restrict__ snap,
+                                        double *__restrict__ ub)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
+    double acc[3] = {0, 0, 0};
+    bool any = false;
+#pragma unroll
+    for (int e = 0; e < 6; e++) {
+        const int gi = i + (e == 0 ? -1 : e == 1 ? 1 : 0), gj = j + (e == 2 ? -1 : e == 3 ? 1 : 0),
+                  gk = k + (e == 4 ? -1 : e == 5 ? 1 : 0);
+        if (gi < 0 || gj < 0 || gk < 0 || gi > d.mx - 1 || gj > d.my - 1 || gk > d.mz - 1) continue;
+        if (gi > 0 && gj > 0 && gk > 0 && gi < d.mx - 1 && gj < d.my - 1 && gk < d.mz - 1) continue;
+        if (c2c_ghost_rule(d, nvert, gi, gj, gk) != t) continue;
+        const long long g = IX(d, gk, gj, gi);
+        if (!any) {
+#pragma unroll
+            for (int q = 0; q < 3; q++) acc[q] = ub[3 * t + q];
+            any = true;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; q++) acc[q] += -1 * snap[3 * g + q];
+    }
+    if (any) {
+#pragma unroll
+        for (int q = 0; q < 3; q++) ub[3 * t + q] = acc[q];
+    }
+}
+
+// Transpose of the Cramer solve of k_c2c_interior (rhs.c:203-231), in two
+// passes: qb = M^-T ub at the fluid cells; the ucont adjoint of each face
+// gathers 0.5 qb of the two cells that average it.
+__global__ void k_c2c_interior_T_q(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ub,
+                                   double *__restrict__ qb)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    POINT_IJK(t);
+    qb[3 * t] = qb[3 * t + 1] = qb[3 * t + 2] = 0;
     if (i < 1 || j < 1 || k < 1 || i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
     if (!(nvert[t] < 0.1)) return;
     const double u0 = ub[3 * t], u1 = ub[3 * t + 1], u2 = ub[3 * t + 2];
@@ -394,15 +609,25 @@ __global__ void k_c2c_interior_T(Dims d, Geom g, const double *__restrict__ nver
     const double i00 = m11 * m22 - m12 * m21, i01 = -(m01 * m22 - m02 * m21), i02 = m01 * m12 - m02 * m11;
     const double i10 = -(m10 * m22 - m12 * m20), i11 = m00 * m22 - m02 * m20, i12 = -(m00 * m12 - m02 * m10);
     const double i20 = m10 * m21 - m11 * m20, i21 = -(m00 * m21 - m01 * m20), i22 = m00 * m11 - m01 * m10;
-    const double q0 = (i00 * u0 + i10 * u1 + i20 * u2) / det;
-    const double q1 = (i01 * u0 + i11 * u1 + i21 * u2) / det;
-    const double q2 = (i02 * u0 + i12 * u1 + i22 * u2) / det;
-    adj_add(cb + 3 * (t - 1), 0.5 * q0);
-    adj_add(cb + 3 * t, 0.5 * q0);
-    adj_add(cb + 3 * (t - d.mx) + 1, 0.5 * q1);
-    adj_add(cb + 3 * t + 1, 0.5 * q1);
-    adj_add(cb + 3 * (t - (long long)d.mx * d.my) + 2, 0.5 * q2);
-    adj_add(cb + 3 * t + 2, 0.5 * q2);
+    qb[3 * t] = (i00 * u0 + i10 * u1 + i20 * u2) / det;
+    qb[3 * t + 1] = (i01 * u0 + i11 * u1 + i21 * u2) / det;
+    qb[3 * t + 2] = (i02 * u0 + i12 * u1 + i22 * u2) / det;
+}
+
+__global__ void k_c2c_interior_T_faces(Dims d, const double *__restrict__ qb, double *__restrict__ cb)
+{
+    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= d.P) return;
+    POINT_IJK(t);
+    const long long sd[3] = {1, d.mx, (long long)d.mx * d.my};
+    const int cd[3] = {i, j, k}, md[3] = {d.mx, d.my, d.mz};
+#pragma unroll
+    for (int dir = 0; dir < 3; dir++) {
+        double a = cb[3 * t + dir];
+        a += 0.5 * qb[3 * t + dir];
+        if (cd[dir] + 1 <= md[dir] - 1) a += 0.5 * qb[3 * (t + sd[dir]) + dir];
+        cb[3 * t + dir] = a;
+    }
 }
 
 // out = Mb (out + cb)
@@ -429,18 +654,29 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_REQUIRE(v != jtv, "v and jtv must not alias");
     if (ensure_ad(c)) return -1;
     const size_t P = (size_t)d.P;
-    double *lu = c->ad, *uc = c->ad + 3 * P, *cb = c->ad + 6 * P, *ub = c->ad + 9 * P;
+    if (!c->vjp_ws) VFS_CHECK_HIP(hipMalloc(&c->vjp_ws, FACE_ADJ_SLOTS * P * sizeof(double)));
+    double *lu = c->ad, *uc = c->ad + 3 * P, *cb = c->ad + 6 * P, *ub = c->ad + 9 * P, *ws = c->vjp_ws;
     if (primal_state(c, U, ucat, lu, uc)) return -1;
     const dim3 gp(grid_for(d.P, 256)), bk(256);
     const unsigned gs = grid_for(shell_count(d), 256);
-    VFS_CHECK_HIP(hipMemsetAsync(c->fp, 0, 3 * P * sizeof(double), c->stream));
-    VFS_CHECK_HIP(hipMemsetAsync(c->face, 0, 18 * P * sizeof(double), c->stream));
-    VFS_CHECK_HIP(hipMemsetAsync(cb, 0, 6 * P * sizeof(double), c->stream));   // cb and ub
+    // projection and divergence (every element written)
     hipLaunchKernelGGL(k_vjp_rhs, gp, bk, 0, c->stream, d, c->g, c->nvert, v, c->fp, jtv, 0.5);
     VFS_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_vjp_cells, gp, bk, 0, c->stream, d, c->nvert, c->fp, c->face);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_vjp_faces, gp, bk, 0, c->stream, d, c->g, c->nvert, lu, uc, nu_t, c->face, cb, ub);
+    // face fluxes, one direction at a time: ub = sum over i, j, k faces,
+    // cb[dir] = the ucont adjoint of the dir faces
+    hipLaunchKernelGGL(k_vjp_faces<0>, gp, bk, 0, c->stream, d, c->g, c->nvert, lu, uc, nu_t, c->face, ws);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vjp_gather<0>, gp, bk, 0, c->stream, d, ws, ub, cb, 0);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vjp_faces<1>, gp, bk, 0, c->stream, d, c->g, c->nvert, lu, uc, nu_t, c->face, ws);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vjp_gather<1>, gp, bk, 0, c->stream, d, ws, ub, cb, 1);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vjp_faces<2>, gp, bk, 0, c->stream, d, c->g, c->nvert, lu, uc, nu_t, c->face, ws);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_vjp_gather<2>, gp, bk, 0, c->stream, d, ws, ub, cb, 1);
     VFS_CHECK_LAUNCH();
     // periodic copy of lucont at the start of Formfunction_2
     if (periodic_vec_T(c, cb)) return -1;
@@ -450,7 +686,9 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
         VFS_CHECK_LAUNCH();
     }
     if (d.immersed) {
-        hipLaunchKernelGGL(k_ibbc_T, gp, bk, 0, c->stream, d, c->g, c->nvert, cb, ub);
+        hipLaunchKernelGGL(k_ibbc_T_take, gp, bk, 0, c->stream, d, c->nvert, cb, ws);
+        VFS_CHECK_LAUNCH();
+        hipLaunchKernelGGL(k_ibbc_T_give, gp, bk, 0, c->stream, d, c->g, ws, ub);
         VFS_CHECK_LAUNCH();
     }
     // Contra2Cart_2 in reverse: periodic copy, interior zeroing, ghost rules,
@@ -461,10 +699,12 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_CHECK_HIP(hipMemcpyAsync(c->snap, ub, 3 * P * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(k_c2c_bc_ghost_T_zero, dim3(gs), bk, 0, c->stream, d, c->nvert, ub);
     VFS_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_c2c_bc_ghost_T_add, dim3(gs), bk, 0, c->stream, d, c->nvert, c->snap, ub);
+    hipLaunchKernelGGL(k_c2c_bc_ghost_T_gather, gp, bk, 0, c->stream, d, c->nvert, c->snap, ub);
     VFS_CHECK_LAUNCH();
     if (periodic_vec_T(c, ub)) return -1;
-    hipLaunchKernelGGL(k_c2c_interior_T, gp, bk, 0, c->stream, d, c->g, c->nvert, ub, cb);
+    hipLaunchKernelGGL(k_c2c_interior_T_q, gp, bk, 0, c->stream, d, c->g, c->nvert, ub, ws);
+    VFS_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_c2c_interior_T_faces, gp, bk, 0, c->stream, d, ws, cb);
     VFS_CHECK_LAUNCH();
     if (periodic_vec_T(c, cb)) return -1;
     hipLaunchKernelGGL(k_vjp_final, gp, bk, 0, c->stream, d, cb, jtv);
