@@ -54,13 +54,32 @@ __device__ __forceinline__ void wave_first_min(double &d, long long &o)
     }
 }
 
-// One wavefront per element: the closest node, first among every 20th node
-// in i, j, k loop order, then in the +-21 window around it; the widths of
-// that cell and the box +- itpwidth clamped to the interior.
-__global__ void __launch_bounds__(64) k_rotor_elements(Dims d, Geom g, const double *__restrict__ coor, int n,
-                                                       const double *__restrict__ cent, int itpwidth,
-                                                       int *__restrict__ box, double *__restrict__ dh)
+// first minimum of (d, order) over a workgroup of RE_WAVES wavefronts
+constexpr int RE_WAVES = 4;
+__device__ __forceinline__ void block_first_min(double &d, long long &o, double *sd, long long *so)
 {
+    wave_first_min(d, o);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) sd[w] = d, so[w] = o;
+    __syncthreads();
+    d = sd[0], o = so[0];
+    for (int q = 1; q < RE_WAVES; q++)
+        if (sd[q] < d || (sd[q] == d && so[q] < o)) d = sd[q], o = so[q];
+    __syncthreads();
+}
+
+// One workgroup per element: the closest node, first among every 20th node
+// in i, j, k loop order, then in the +-21 window around it; the widths of
+// that cell and the box +- itpwidth clamped to the interior.  The first
+// minimum (smallest distance, then earliest loop position) does not depend
+// on how the positions are spread over the threads.
+__global__ void __launch_bounds__(64 * RE_WAVES) k_rotor_elements(Dims d, Geom g, const double *__restrict__ coor,
+                                                                  int n, const double *__restrict__ cent,
+                                                                  int itpwidth, int *__restrict__ box,
+                                                                  double *__restrict__ dh)
+{
+    __shared__ double sd[RE_WAVES];
+    __shared__ long long so[RE_WAVES];
     const int l = blockIdx.x, lane = threadIdx.x;
     if (l >= n) return;
     const int lxs = 1, lxe = d.mx - 1, lys = 1, lye = d.my - 1, lzs = 1, lze = d.mz - 1;
@@ -74,12 +93,12 @@ __global__ void __launch_bounds__(64) k_rotor_elements(Dims d, Geom g, const dou
     const int na = (lxe - lxs + 19) / 20, nb = (lye - lys + 19) / 20, nc = (lze - lzs + 19) / 20;
     double dm = 1.e6;
     long long om = -1;
-    for (long long o = lane; o < (long long)na * nb * nc; o += 64) {
+    for (long long o = lane; o < (long long)na * nb * nc; o += 64 * RE_WAVES) {
         const int a = (int)(o / ((long long)nb * nc)), b = (int)((o / nc) % nb), c = (int)(o % nc);
         const double d1 = dist(lxs + 20 * a, lys + 20 * b, lzs + 20 * c);
         if (d1 < dm) { dm = d1; om = o; }
     }
-    wave_first_min(dm, om);
+    block_first_min(dm, om, sd, so);
     int icc = lxs, jcc = lys, kcc = lzs;
     if (om >= 0) {
         icc = lxs + 20 * (int)(om / ((long long)nb * nc));
@@ -91,12 +110,12 @@ __global__ void __launch_bounds__(64) k_rotor_elements(Dims d, Geom g, const dou
     const int wi = max(i1 - i0, 0), wj = max(j1 - j0, 0), wk = max(k1 - k0, 0);
     dm = 1.e13;
     om = -1;
-    for (long long o = lane; o < (long long)wi * wj * wk; o += 64) {
+    for (long long o = lane; o < (long long)wi * wj * wk; o += 64 * RE_WAVES) {
         const int a = (int)(o / ((long long)wj * wk)), b = (int)((o / wk) % wj), c = (int)(o % wk);
         const double d1 = dist(i0 + a, j0 + b, k0 + c);
         if (d1 < dm) { dm = d1; om = o; }
     }
-    wave_first_min(dm, om);
+    block_first_min(dm, om, sd, so);
     if (lane != 0) return;
     int ic = 0, jc = 0, kc = 0;   // the reference's initial values when the window is empty
     if (om >= 0) {
@@ -131,7 +150,7 @@ extern "C" int vfs_rotor_elements(vfs_ctx *c, const double *coor, int n, const d
 {
     VFS_REQUIRE(n >= 0 && (n == 0 || (coor && cent && box && dh)), "bad element arrays");
     if (n == 0) return 0;
-    hipLaunchKernelGGL(k_rotor_elements, dim3(n), dim3(64), 0, c->stream, c->d, c->g, coor, n, cent, itpwidth, box, dh);
+    hipLaunchKernelGGL(k_rotor_elements, dim3(n), dim3(64 * RE_WAVES), 0, c->stream, c->d, c->g, coor, n, cent, itpwidth, box, dh);
     VFS_CHECK_LAUNCH();
     return 0;
 }
@@ -180,15 +199,36 @@ __device__ __forceinline__ void normals(const double *csi, const double *eta, co
 }
 
 // rotor_model.c:4879-4994 for the cells of the box union: one thread a
-// cell, elements in order
-__global__ void k_f_eul(Dims d, Geom g, const double *__restrict__ coor, int n, const double *__restrict__ cent,
-                        const double *__restrict__ F, const double *__restrict__ dA, const int *__restrict__ box,
-                        int df, double halfwidth, int width_mode, double fx, double fy, double fz, int i0, int j0,
-                        int k0, int ni_, int nj_, int nk_, double *__restrict__ f_eul)
+// cell, elements in order.  A workgroup first lists, in order, the elements
+// of each batch of 256 whose box meets the box of its cells; a cell then
+// runs the reference's per-element test on those only, so its sum takes the
+// same terms in the same order.
+constexpr int FEUL_BLOCK = 256;
+__global__ void __launch_bounds__(FEUL_BLOCK) k_f_eul(Dims d, Geom g, const double *__restrict__ coor, int n,
+                                                      const double *__restrict__ cent, const double *__restrict__ F,
+                                                      const double *__restrict__ dA, const int *__restrict__ box,
+                                                      int df, double halfwidth, int width_mode, double fx, double fy,
+                                                      double fz, int i0, int j0, int k0, int ni_, int nj_, int nk_,
+                                                      double *__restrict__ f_eul)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)ni_ * nj_ * nk_) return;
-    const int i = i0 + (int)(t % ni_), j = j0 + (int)((t / ni_) % nj_), k = k0 + (int)(t / ((long long)ni_ * nj_));
+    __shared__ int sl[FEUL_BLOCK];
+    const long long nt = (long long)ni_ * nj_ * nk_;
+    const long long t0 = (long long)blockIdx.x * FEUL_BLOCK, t = t0 + threadIdx.x;
+    // the box of the workgroup's cells (whole rows / planes once it wraps)
+    int bi0, bi1, bj0, bj1, bk0, bk1;
+    {
+        const long long tl = (t0 + FEUL_BLOCK < nt ? t0 + FEUL_BLOCK : nt) - 1;
+        const int a0 = (int)(t0 % ni_), b0 = (int)((t0 / ni_) % nj_), c0 = (int)(t0 / ((long long)ni_ * nj_));
+        const int a1 = (int)(tl % ni_), b1 = (int)((tl / ni_) % nj_), c1 = (int)(tl / ((long long)ni_ * nj_));
+        bk0 = k0 + c0, bk1 = k0 + c1;
+        if (c0 == c1) bj0 = j0 + b0, bj1 = j0 + b1;
+        else bj0 = j0, bj1 = j0 + nj_ - 1;
+        if (c0 == c1 && b0 == b1) bi0 = i0 + a0, bi1 = i0 + a1;
+        else bi0 = i0, bi1 = i0 + ni_ - 1;
+    }
+    const bool live = t < nt;
+    const long long tc = live ? t : t0;   // threads past the last cell work on the first one and write nothing
+    const int i = i0 + (int)(tc % ni_), j = j0 + (int)((tc / ni_) % nj_), k = k0 + (int)(tc / ((long long)ni_ * nj_));
     const long long id = IX(d, k, j, i);
 #define CO(kk, jj, ii, q) coor[3 * IX(d, kk, jj, ii) + (q)]
     double xi[3], xj[3], xk[3];
@@ -213,7 +253,25 @@ __global__ void k_f_eul(Dims d, Geom g, const double *__restrict__ coor, int n, 
     }
     const double *m[3] = {cs, et, ze};
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int l = 0; l < n; l++) {
+    for (int l0 = 0; l0 < n; l0 += FEUL_BLOCK) {
+        // order-keeping compaction of the batch's elements that meet the box
+        const int le = l0 + (int)threadIdx.x;
+        const int meet = le < n && !(bi1 < box[le] || bi0 >= box[n + le] || bj1 < box[2 * n + le] ||
+                                     bj0 >= box[3 * n + le] || bk1 < box[4 * n + le] || bk0 >= box[5 * n + le]);
+        sl[threadIdx.x] = meet;
+        __syncthreads();
+        for (int w = 1; w < FEUL_BLOCK; w <<= 1) {   // inclusive scan
+            const int v = threadIdx.x >= (unsigned)w ? sl[threadIdx.x - w] : 0;
+            __syncthreads();
+            sl[threadIdx.x] += v;
+            __syncthreads();
+        }
+        const int cnt = sl[FEUL_BLOCK - 1], pos = sl[threadIdx.x] - 1;
+        __syncthreads();
+        if (meet) sl[pos] = le;
+        __syncthreads();
+        for (int ql = 0; live && ql < cnt; ql++) {
+        const int l = sl[ql];
         if (i < box[l] || i >= box[n + l] || j < box[2 * n + l] || j >= box[3 * n + l] || k < box[4 * n + l] ||
             k >= box[5 * n + l])
             continue;
@@ -241,7 +299,10 @@ __global__ void k_f_eul(Dims d, Geom g, const double *__restrict__ coor, int n, 
                                dfunc_dev(df, r3, halfwidth);
             acc[f] += Fx * dfn * a * m[f][0] + Fy * dfn * a * m[f][1] + Fz * dfn * a * m[f][2];
         }
+        }
+        __syncthreads();   // sl is rewritten by the next batch
     }
+    if (!live) return;
     f_eul[3 * id] = acc[0];
     f_eul[3 * id + 1] = acc[1];
     f_eul[3 * id + 2] = acc[2];
@@ -305,7 +366,7 @@ extern "C" int vfs_calc_f_eul(vfs_ctx *c, const double *coor, int n, const doubl
             const long long cells = (long long)ni * nj * nk;
             const double fx = width_mode == 1 ? dh_fixed[0] : 0, fy = width_mode == 1 ? dh_fixed[1] : 0,
                          fz = width_mode == 1 ? dh_fixed[2] : 0;
-            hipLaunchKernelGGL(k_f_eul, dim3(grid_for(cells, 128)), dim3(128), 0, c->stream, d, c->g, coor, n, cent,
+            hipLaunchKernelGGL(k_f_eul, dim3(grid_for(cells, FEUL_BLOCK)), dim3(FEUL_BLOCK), 0, c->stream, d, c->g, coor, n, cent,
                                F, dA, box, df, halfwidth, width_mode, fx, fy, fz, h[0], h[2], h[4], ni, nj, nk, f_eul);
             VFS_CHECK_LAUNCH();
         }

@@ -229,47 +229,75 @@ __global__ void __launch_bounds__(LIST_BLOCK) k_ib_place(Dims d, const double *_
 }
 
 // nearestcell1 (ibm.c:1366-1511), InterceptionPoint (ibm.c:1759-2250) with
-// ICP (ibm.c:1659-1757), and the boundary fallback (ibm.c:413-493)
+// ICP (ibm.c:1659-1757), and the boundary fallback (ibm.c:413-493).
+// One wavefront a node.  The reference scans the elements in order, skips
+// one whose bounding sphere is no nearer than the running minimum, and
+// takes a candidate only when it is strictly nearer (the first of equal
+// candidates wins).  Here the lanes take 64 elements at a time: each one
+// evaluates its element against the minimum at the start of the batch (a
+// larger bound, so no element the scan would evaluate is left out) and
+// keeps its nearest candidate, the first of equal ones.  The elements that
+// could still win are then replayed in element order with the scan's two
+// tests, so the node gets the element, point and distance of the scan.
 __global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, const double *__restrict__ qv,
                                                   const double *__restrict__ rad, int n,
                                                   vfs_ib_node *__restrict__ nodes, int *__restrict__ fail)
 {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int l = blockIdx.x, lane = threadIdx.x;
     if (l >= n) return;
     vfs_ib_node nd = nodes[l];
     const int i = nd.i, j = nd.j, k = nd.k;
     const long long id = IX(d, k, j, i);
     const double p[3] = {g.cent[3 * id], g.cent[3 * id + 1], g.cent[3 * id + 2]};
 
-    double dmin = 1.e20, pmin[3] = {0, 0, 0};
+    double dmin = 1.e20, pmin[3] = {0, 0, 0};   // the same in every lane
     int cell_min = -100;
-    for (int e = 0; e < s.n_elmt; e++) {
-        const double q[3] = {qv[3 * e], qv[3 * e + 1], qv[3 * e + 2]};
-        if (!(dist3(p, q) - rad[e] < dmin)) continue;
-        const int n1 = s.nv1[e];
-        const double nfx = s.nfx[e], nfy = s.nfy[e], nfz = s.nfz[e];
-        double p1[3], p2[3], p3[3], po[3], dd;
-        vertex(s, n1, p1);
-        vertex(s, s.nv2[e], p2);
-        vertex(s, s.nv3[e], p3);
-        double tf = ((p[0] - p1[0]) * nfx + (p[1] - p1[1]) * nfy + (p[2] - p1[2]) * nfz);
-        if (fabs(tf) < 1.e-10) tf = 1.e-10;
-        if (!(tf >= 0)) continue;
-        const double pj[3] = {p[0] - tf * nfx, p[1] - tf * nfy, p[2] - tf * nfz};
-        if (point_in_triangle(pj, p1, p2, p3, nfx, nfy, nfz) == 1) {
-            if (tf < dmin) dmin = tf, pmin[0] = pj[0], pmin[1] = pj[1], pmin[2] = pj[2], cell_min = e;
-        } else {
-            dis_p_line(p, p1, p2, po, dd);
-            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
-            dis_p_line(p, p2, p3, po, dd);
-            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
-            dis_p_line(p, p3, p1, po, dd);
-            if (dd < dmin) dmin = dd, pmin[0] = po[0], pmin[1] = po[1], pmin[2] = po[2], cell_min = e;
+    for (int e0 = 0; e0 < s.n_elmt; e0 += 64) {
+        const int e = e0 + lane;
+        double lb = 0, v = INFINITY, po[3] = {0, 0, 0};
+        bool cand = false;
+        if (e < s.n_elmt) {
+            const double q[3] = {qv[3 * e], qv[3 * e + 1], qv[3 * e + 2]};
+            lb = dist3(p, q) - rad[e];
+            if (lb < dmin) {
+                const double nfx = s.nfx[e], nfy = s.nfy[e], nfz = s.nfz[e];
+                double p1[3], p2[3], p3[3];
+                vertex(s, s.nv1[e], p1);
+                vertex(s, s.nv2[e], p2);
+                vertex(s, s.nv3[e], p3);
+                double tf = ((p[0] - p1[0]) * nfx + (p[1] - p1[1]) * nfy + (p[2] - p1[2]) * nfz);
+                if (fabs(tf) < 1.e-10) tf = 1.e-10;
+                if (tf >= 0) {
+                    const double pj[3] = {p[0] - tf * nfx, p[1] - tf * nfy, p[2] - tf * nfz};
+                    if (point_in_triangle(pj, p1, p2, p3, nfx, nfy, nfz) == 1) {
+                        v = tf, po[0] = pj[0], po[1] = pj[1], po[2] = pj[2];
+                    } else {
+                        double pe[3], dd;
+                        dis_p_line(p, p1, p2, pe, dd);
+                        if (dd < v) v = dd, po[0] = pe[0], po[1] = pe[1], po[2] = pe[2];
+                        dis_p_line(p, p2, p3, pe, dd);
+                        if (dd < v) v = dd, po[0] = pe[0], po[1] = pe[1], po[2] = pe[2];
+                        dis_p_line(p, p3, p1, pe, dd);
+                        if (dd < v) v = dd, po[0] = pe[0], po[1] = pe[1], po[2] = pe[2];
+                    }
+                    cand = v < dmin;
+                }
+            }
+        }
+        unsigned long long bal = __ballot(cand);
+        while (bal) {
+            const int src = __ffsll(bal) - 1;   // the earliest element left
+            bal &= bal - 1;
+            const double lbs = __shfl(lb, src), vs = __shfl(v, src);
+            const double x0 = __shfl(po[0], src), x1 = __shfl(po[1], src), x2 = __shfl(po[2], src);
+            if (lbs < dmin && vs < dmin) dmin = vs, pmin[0] = x0, pmin[1] = x1, pmin[2] = x2, cell_min = e0 + src;
         }
     }
     if (cell_min == -100) {
-        atomicAdd(fail, 1);
-        nodes[l] = nd;
+        if (lane == 0) {
+            atomicAdd(fail, 1);
+            nodes[l] = nd;
+        }
         return;
     }
     nd.cell = cell_min;
@@ -332,8 +360,10 @@ __global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, cons
         if (i == 1 || i == d.mx - 2 || j == 1 || j == d.my - 2) ti = i, tj = j, tk = s.nfz[cell_min] > 0 ? k + 1 : k - 1;
         else if (k == 1 || k == d.mz - 2) ti = i, tj = j + 1, tk = k;
         else {
-            atomicAdd(fail, 1);
-            nodes[l] = nd;
+            if (lane == 0) {
+                atomicAdd(fail, 1);
+                nodes[l] = nd;
+            }
             return;
         }
         const long long c = IX(d, tk, tj, ti);
@@ -344,7 +374,7 @@ __global__ void __launch_bounds__(64) k_ib_search(Dims d, Geom g, IbSurf s, cons
         nd.j1 = nd.j2 = nd.j3 = tj;
         nd.k1 = nd.k2 = nd.k3 = tk;
     }
-    nodes[l] = nd;
+    if (lane == 0) nodes[l] = nd;
 }
 
 // sediment_transport.c:18-68: the rough-wall law of the IB wall model
@@ -722,9 +752,12 @@ __device__ int intsect(const double o[3], const double dir[3], const double v0[3
 // (ic, jc, k >= kc) are those whose own bin range holds that column, tested
 // once each.  The result (the parity of the crossings, or a coincidence that
 // asks for the next ray) does not depend on the order the elements are
-// tested in.  reset clears nvert < 5 first (the first body of a search).
+// tested in.  The elements whose bin range holds column (ic, jc) are listed
+// per column (cstart / celem, element order), so a cell tests those only.
+// reset clears nvert < 5 first (the first body of a search).
 __global__ void __launch_bounds__(256) k_body_inside(Dims d, const double *__restrict__ cent, IbSurf s,
-                                                     const int *__restrict__ bins, BodyBox b,
+                                                     const int *__restrict__ bins, const int *__restrict__ cstart,
+                                                     const int *__restrict__ celem, BodyBox b,
                                                      const double *__restrict__ jitter, int reset,
                                                      double *__restrict__ nvert, int *__restrict__ fail)
 {
@@ -748,7 +781,9 @@ __global__ void __launch_bounds__(256) k_body_inside(Dims d, const double *__res
             double dt[MAXHIT];
             int de[MAXHIT];
             int nintp = 0, sing = 0;
-            for (int e = 0; e < s.n_elmt && !sing; e++) {
+            const int col = ic * NBIN + jc;
+            for (int qe = cstart[col]; qe < cstart[col + 1] && !sing; qe++) {
+                const int e = celem[qe];
                 const int *r = &bins[6 * e];
                 if (!(r[0] <= ic && ic < r[1] && r[2] <= jc && jc < r[3] && (r[4] > kc ? r[4] : kc) < r[5])) continue;
                 double v0[3], v1[3], v2[3], tt;
@@ -902,7 +937,7 @@ extern "C" int vfs_ib_search(vfs_ctx *c, const vfs_ib_surface *surf, vfs_ib_node
             e = hipMemcpyAsync(off, h.data(), nb * sizeof(int), hipMemcpyHostToDevice, c->stream);
             if (e == hipSuccess) {
                 hipLaunchKernelGGL(k_ib_place, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, off, total, nodes);
-                hipLaunchKernelGGL(k_ib_search, dim3(grid_for(total, 64)), dim3(64), 0, c->stream, d, c->g, s, qv, rad,
+                hipLaunchKernelGGL(k_ib_search, dim3(total), dim3(64), 0, c->stream, d, c->g, s, qv, rad,
                                    total, nodes, fail);
                 e = hipGetLastError();
             }
@@ -1030,7 +1065,7 @@ extern "C" int vfs_ib_search_noalloc(vfs_ctx *c, const vfs_ib_surface *surf, vfs
     }
     if (total > 0) {
         hipLaunchKernelGGL(k_ib_place, dim3(nb), dim3(LIST_BLOCK), 0, c->stream, d, c->nvert, cnt, total, nodes);
-        hipLaunchKernelGGL(k_ib_search, dim3(grid_for(total, 64)), dim3(64), 0, c->stream, d, c->g, s, qv, rad, total,
+        hipLaunchKernelGGL(k_ib_search, dim3(total), dim3(64), 0, c->stream, d, c->g, s, qv, rad, total,
                            nodes, fail);
         VFS_CHECK_LAUNCH();
         VFS_CHECK_HIP(hipMemcpyAsync(&nfail, fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -1150,22 +1185,38 @@ extern "C" int vfs_nvert_body_thin(vfs_ctx *c, const vfs_ib_surface *surf, int r
         }
         for (int q = 0; q < 6; q++) bins[6 * (size_t)e + q] = r[q];
     }
+    // the elements of each (ic, jc) column, in element order
+    std::vector<int> cstart(NBIN * NBIN + 1, 0), celem;
+    for (int e = 0; e < s.n_elmt; e++)
+        for (int ic = bins[6 * (size_t)e]; ic < bins[6 * (size_t)e + 1]; ic++)
+            for (int jc = bins[6 * (size_t)e + 2]; jc < bins[6 * (size_t)e + 3]; jc++) cstart[ic * NBIN + jc + 1]++;
+    for (int q = 0; q < NBIN * NBIN; q++) cstart[q + 1] += cstart[q];
+    celem.resize(std::max(cstart[NBIN * NBIN], 1));
+    {
+        std::vector<int> at(cstart.begin(), cstart.end() - 1);
+        for (int e = 0; e < s.n_elmt; e++)
+            for (int ic = bins[6 * (size_t)e]; ic < bins[6 * (size_t)e + 1]; ic++)
+                for (int jc = bins[6 * (size_t)e + 2]; jc < bins[6 * (size_t)e + 3]; jc++) celem[at[ic * NBIN + jc]++] = e;
+    }
     double jit[NTRY];
     for (int q = 0; q < NTRY; q++)
         if (vfs_ib_ray_jitter(q + 1, &jit[q])) return -1;
-    const size_t bytes = bins.size() * sizeof(int) + sizeof jit + sizeof(int);
+    const size_t bytes = bins.size() * sizeof(int) + sizeof jit + sizeof(int) + (cstart.size() + celem.size()) * sizeof(int);
     char *buf = nullptr;
     VFS_CHECK_HIP(hipMalloc(&buf, bytes));
     int *dbins = (int *)buf;
     double *djit = (double *)(buf + bins.size() * sizeof(int));   // 24 bytes an element: 8-aligned
     int *fail = (int *)(djit + NTRY);
+    int *dcs = fail + 1, *dce = dcs + cstart.size();
     int nfail = 0;
     hipError_t e = hipMemcpyAsync(dbins, bins.data(), bins.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(djit, jit, sizeof jit, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dcs, cstart.data(), cstart.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dce, celem.data(), celem.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
     if (e == hipSuccess) e = hipMemsetAsync(fail, 0, sizeof(int), c->stream);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_body_inside, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g.cent, s, dbins, b,
-                           djit, reset ? 1 : 0, c->nvert, fail);
+        hipLaunchKernelGGL(k_body_inside, dim3(grid_for(d.P, 256)), dim3(256), 0, c->stream, d, c->g.cent, s, dbins,
+                           dcs, dce, b, djit, reset ? 1 : 0, c->nvert, fail);
         e = hipGetLastError();
     }
     if (e == hipSuccess && thin) {
