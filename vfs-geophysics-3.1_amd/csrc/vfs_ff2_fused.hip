@@ -4,25 +4,26 @@
 // The face fluxes dominate the right-hand side: each face reads about 54 ucat
 // values and 10 nvert values on a 2x4x3 neighbourhood.  Read from global
 // memory as AoS doubles, those loads are bound by the L1/TA rate long before
-// HBM.  Here a workgroup owns a 64(i) x 4(j) column of cells and marches in
+// HBM.  Here a workgroup owns a 64(i) x 8(j) column of cells and marches in
 // k.  It keeps four k-planes of ucat (SoA, FP64) and nvert (FP32, exact for
 // the small-integer masks) with a one-cell i/j halo (two on the high i/j
 // side) in LDS: planes k-1..k+2 are what the i-, j- and k-faces of plane k
 // read.  The next plane is prefetched into registers while the current one
-// is computed.  Reads that fall outside the staged box (the periodic
-// wrap-around of the 4th-order stencil at the first and last faces) go to
-// global memory.
+// is computed.  On the tile planes whose box holds no marked cell (a flag per
+// tile and plane, recomputed when the mask changes) the faces take the
+// mask-free specialisation of face_flux_t.
 //
 // Every expression is the one in vfs_stencil.h, so results are bit-identical
-// to the reference-structured pass k_ff2_faces.  Face points on the
-// low boundary planes (i = 0, j = 0, k = 0) are computed by a small separate
-// kernel.  The cell divergence and the projection then run as in the
-// reference-structured path.
+// to the reference-structured pass k_ff2_faces.  The faces on the low
+// boundary planes (i = 0, j = 0, k = 0) and, in periodic directions, the last
+// faces, whose 4th-order stencil wraps around the seam, are computed by a
+// small separate kernel.  The cell divergence marches in k with the k-face
+// fluxes in registers, and the projection runs as in the reference-structured
+// path.
 #include "vfs_internal.h"
 #include "vfs_stencil.h"
 
 int vfs_ff2_passes(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale);
-int vfs_ff2_from_faces(vfs_ctx *c, double *rhs, double scale);
 int vfs_ff2_rhs_only(vfs_ctx *c, double *rhs, double scale);
 
 // Shape parameters (tuned on MI355X).
@@ -177,22 +178,23 @@ __device__ __forceinline__ void faces_at(const Dims &d, const Geom &g, const dou
 // Plane flags of the face kernel: flag[(k * gy + by) * gx + bx] = 1 when no
 // cell of the tile's staged box on planes k-1..k+2 (one-cell i/j halo, two on
 // the high side, widened by one more each way) is marked.
-__global__ void k_ff2_plain_flags(Dims d, const double *__restrict__ nvert, int gx, int gy,
-                                  unsigned char *__restrict__ flag)
+// One wavefront a flag (grid gx * gy * mz, 64 threads): the lanes take the
+// rows of the box in turn, a lane a point.
+__global__ void __launch_bounds__(64) k_ff2_plain_flags(Dims d, const double *__restrict__ nvert, int gx, int gy,
+                                                        unsigned char *__restrict__ flag)
 {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)gx * gy * d.mz) return;
+    const long long t = blockIdx.x;
     const int bx = (int)(t % gx), by = (int)((t / gx) % gy), k = (int)(t / ((long long)gx * gy));
     const int i0 = 1 + bx * TI, j0 = 1 + by * TJ;
     const int ia = max(i0 - 3, 0), ib = min(i0 + TI + 2, d.mx - 1);
     const int ja = max(j0 - 3, 0), jb = min(j0 + TJ + 2, d.my - 1);
     const int ka = max(k - 1, 0), kb = min(k + 2, d.mz - 1);
-    int clean = 1;
-    for (int kk = ka; kk <= kb && clean; kk++)
-        for (int jj = ja; jj <= jb && clean; jj++)
-            for (int ii = ia; ii <= ib; ii++)
-                if (nvert[IX(d, kk, jj, ii)] != 0.0) { clean = 0; break; }
-    flag[t] = (unsigned char)clean;
+    bool marked = false;
+    for (int kk = ka; kk <= kb; kk++)
+        for (int jj = ja; jj <= jb; jj++)
+            for (int ii = ia + (int)threadIdx.x; ii <= ib; ii += 64) marked |= nvert[IX(d, kk, jj, ii)] != 0.0;
+    const bool any = __ballot(marked) != 0;
+    if (threadIdx.x == 0) flag[t] = (unsigned char)!any;
 }
 
 // Face fluxes of all three directions at every interior point
@@ -361,7 +363,7 @@ static int ff2_plain_flags(vfs_ctx *c, int gx, int gy)
     const size_t n = (size_t)gx * gy * d.mz;
     if (c->ff2_flags && c->ff2_flags_version == c->nvert_version) return 0;
     if (!c->ff2_flags) VFS_CHECK_HIP(hipMalloc(&c->ff2_flags, n));
-    hipLaunchKernelGGL(k_ff2_plain_flags, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, d, c->nvert, gx, gy,
+    hipLaunchKernelGGL(k_ff2_plain_flags, dim3((unsigned)n), dim3(64), 0, c->stream, d, c->nvert, gx, gy,
                        c->ff2_flags);
     VFS_CHECK_LAUNCH();
     c->ff2_flags_version = c->nvert_version;
