@@ -281,7 +281,12 @@ typedef struct {
     double ks;              /* law 1's roughness height (the reference passes 0.0001, ibm.c:3473) */
     double roughness;       /* -roughness (main.c:2073): law 2's roughness height */
 } vfs_ib_options;
-/* vfs_ib_interpolate with options (ibm.c:3448-3478). */
+/* vfs_ib_interpolate with options (ibm.c:3448-3478).  ks is read by law 1
+ * only.  The reference takes its wall laws from ti = 1 on; at ti = 0 a
+ * -wallfunction run takes noslip, even with -inv (ibm.c:3453-3477).  Here
+ * wallfunction = 0 gives noslip on a viscous context and freeslip on an
+ * inviscid one, so that one step of an inviscid -wallfunction run has no
+ * counterpart. */
 int vfs_ib_interpolate_opt(vfs_ctx *ctx, const vfs_ib_surface *surf, const vfs_ib_node *nodes, int n,
                            const vfs_ib_options *opt, double *ucont, double *ucat, double *p, double *ustar,
                            int *count, double *shear, double *pressure);

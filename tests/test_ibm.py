@@ -574,10 +574,12 @@ def test_gpu_ib_options(moving, wallfunction):
     ucont = np.sin(np.arange(3 * o.P) * 0.11)
     d_ucat, d_ucont, d_p, d_us = (V.DeviceArray.from_host(a) for a in (ucat, ucont, p, np.zeros(o.P)))
     rough = 2e-4 if wallfunction == "loglaw" else None
+    # ks is read by the smooth-wall law only: the rough log law runs with none
+    ks = 0.0 if wallfunction == "loglaw" else 1e-4
     st = ctx.ib_interpolate(surf, dn, nn, d_ucont, d_ucat, d_p, d_us, moving=moving, wallfunction=wallfunction,
-                            ks=1e-4, roughness=rough)
+                            ks=ks, roughness=rough)
     uc, u, pp, us = ucont.copy(), ucat.copy(), p.copy(), np.zeros(o.P)
-    cnt, sh, pr = o.ib_interpolate(body, nodes, uc, u, pp, us, moving=moving, wallfunction=wallfunction, ks=1e-4,
+    cnt, sh, pr = o.ib_interpolate(body, nodes, uc, u, pp, us, moving=moving, wallfunction=wallfunction, ks=ks,
                                    roughness=rough)
     # the wall law's log / exp / sin: the device's and the C library's
     tol = 1e-12 if wallfunction else 0.0

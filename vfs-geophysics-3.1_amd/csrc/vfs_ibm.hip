@@ -1094,7 +1094,7 @@ extern "C" int vfs_ib_interpolate_opt(vfs_ctx *c, const vfs_ib_surface *surf, co
     const Dims d = c->d;
     VFS_REQUIRE(opt && (opt->moving == 0 || opt->moving == 1) && opt->wallfunction >= 0 && opt->wallfunction <= 2,
                 "bad IB options");
-    VFS_REQUIRE(!opt->wallfunction || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
+    VFS_REQUIRE(opt->wallfunction != 1 || opt->ks > 0, "the IB wall model needs a roughness ks > 0");
     VFS_REQUIRE(opt->wallfunction != 2 || opt->roughness > 0, "the rough log law needs a roughness > 0");
     if (int rc = check_surface(surf)) return rc;
     VFS_REQUIRE(n >= 0 && (n == 0 || nodes) && ucont && ucat && p && ustar, "bad arguments");
