@@ -46,6 +46,12 @@ typedef struct {
     int weno;            /* -weno 1: WENO3 convective face values (level.c:1734) */
     int inviscid;        /* -inv: WENO3 convection, 2nd-order divergence, no
                             viscous term (momentum.c:757, 1582, 1621) */
+    int skew;            /* -skew: skew-symmetric convection, half the divergence
+                            form plus half the advective form interpolated to the
+                            cells (momentum.c:798-811, 1060-1071, 1298-1309,
+                            1596-1608) */
+    int clark;           /* -clark: the Clark gradient-model stress in the viscous
+                            face fluxes (momentum.c:907-925, 1142-1160, 1385-1403) */
     double ren, dt, st;  /* Reynolds number, time step, Strouhal factor */
     double mean_pressure_gradient;
 } vfs_config;

@@ -82,6 +82,8 @@ static int check_cfg(const vo_cfg *c)
     if (c->immersed != 0 && c->immersed != 1) return -4;
     if (c->weno != 0 && c->weno != 1) return -4;
     if (c->inviscid != 0 && c->inviscid != 1) return -4;
+    if (c->skew != 0 && c->skew != 1) return -4;
+    if (c->clark != 0 && c->clark != 1) return -4;
     return 0;
 }
 
@@ -669,6 +671,24 @@ static void du_dxyz(const vo_geom *g, size_t id, const double dc[3], const doubl
     for (int a = 0; a < 3; a++)
         for (int b = 0; b < 3; b++)
             D[a][b] = (dc[a] * g->csi[3 * id + b] + de[a] * g->eta[3 * id + b] + dz[a] * g->zet[3 * id + b]) * ajc;
+}
+
+/* Compute_du_dxyz (k-omega.c:597-610) with given metrics: the Cartesian
+ * gradient at a face from its computational-space differences */
+static void du_dxyz_face(double csi0, double csi1, double csi2, double eta0, double eta1, double eta2, double zet0,
+                         double zet1, double zet2, double ajc, const double dc[3], const double de[3],
+                         const double dz[3], double *du_dx, double *dv_dx, double *dw_dx, double *du_dy,
+                         double *dv_dy, double *dw_dy, double *du_dz, double *dv_dz, double *dw_dz)
+{
+    *du_dx = (dc[0] * csi0 + de[0] * eta0 + dz[0] * zet0) * ajc;
+    *du_dy = (dc[0] * csi1 + de[0] * eta1 + dz[0] * zet1) * ajc;
+    *du_dz = (dc[0] * csi2 + de[0] * eta2 + dz[0] * zet2) * ajc;
+    *dv_dx = (dc[1] * csi0 + de[1] * eta0 + dz[1] * zet0) * ajc;
+    *dv_dy = (dc[1] * csi1 + de[1] * eta1 + dz[1] * zet1) * ajc;
+    *dv_dz = (dc[1] * csi2 + de[1] * eta2 + dz[1] * zet2) * ajc;
+    *dw_dx = (dc[2] * csi0 + de[2] * eta0 + dz[2] * zet0) * ajc;
+    *dw_dy = (dc[2] * csi1 + de[2] * eta1 + dz[2] * zet1) * ajc;
+    *dw_dz = (dc[2] * csi2 + de[2] * eta2 + dz[2] * zet2) * ajc;
 }
 
 /* |S| = sqrt(2 S:S) in the reference's term order (les.c:199-203) */
@@ -1331,6 +1351,24 @@ int vo_ib_bc(const vo_cfg *c, const vo_geom *g, const double *nvert, double *uco
 /* ------------------------------------------------------------------------ */
 /* momentum.c:456-1979 Formfunction_2                                        */
 /* ------------------------------------------------------------------------ */
+/* rhs2.c:592-643 Calculate_normal_and_area + Calculate_dxdydz: the length
+ * scales of a cell along x, y, z (volume 1/ajc over the face areas, along the
+ * unit covariant base vectors) */
+static void dxdydz(double ajc, const double *csi, const double *eta, const double *zet, double *dx, double *dy,
+                   double *dz)
+{
+    double ni[3], nj[3], nk[3];
+    const double vol = 1. / ajc;
+    calc_normal(csi, eta, zet, ni, nj, nk);
+    const double Ai = sqrt(csi[0] * csi[0] + csi[1] * csi[1] + csi[2] * csi[2]);
+    const double Aj = sqrt(eta[0] * eta[0] + eta[1] * eta[1] + eta[2] * eta[2]);
+    const double Ak = sqrt(zet[0] * zet[0] + zet[1] * zet[1] + zet[2] * zet[2]);
+    const double Li = vol / Ai, Lj = vol / Aj, Lk = vol / Ak;
+    *dx = fabs(Li * ni[0] + Lj * nj[0] + Lk * nk[0]);
+    *dy = fabs(Li * ni[1] + Lj * nj[1] + Lk * nk[1]);
+    *dz = fabs(Li * ni[2] + Lj * nj[2] + Lk * nk[2]);
+}
+
 int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                      double *ucont, const double *ucat, const double *nu_t,
                      double *rhs, double scale)
@@ -1340,10 +1378,12 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
     if (rc) return rc;
     const int ip = c->i_periodic, jp = c->j_periodic, kp = c->k_periodic;
     size_t P = (size_t)mx * my * mz;
-    double *buf = (double *)calloc(21 * P, sizeof(double));
+    double *buf = (double *)calloc(30 * P, sizeof(double));
     if (!buf) return -1;
     double *div1 = buf, *div2 = buf + 3 * P, *div3 = buf + 6 * P;
     double *visc1 = buf + 9 * P, *visc2 = buf + 12 * P, *visc3 = buf + 15 * P, *fp = buf + 18 * P;
+    /* -skew: the advective face fluxes adv1..3 (momentum.c:610-618) */
+    double *adv1 = buf + 21 * P, *adv2 = buf + 24 * P, *adv3 = buf + 27 * P;
 
     per_copy_vec(c, ucont); /* momentum.c:642-669 */
 
@@ -1354,6 +1394,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
         const double *faj = dir == 0 ? g->iaj : dir == 1 ? g->jaj : g->kaj;
         double *dv = dir == 0 ? div1 : dir == 1 ? div2 : div3;
         double *vs = dir == 0 ? visc1 : dir == 1 ? visc2 : visc3;
+        double *av = dir == 0 ? adv1 : dir == 1 ? adv2 : adv3;
         /* face loops: i-faces momentum.c:672-931, j 934-1166, k 1169-1409 */
 #pragma omp parallel for schedule(static)
         for (int k = (dir == 2 ? 0 : 1); k < mz - 1; k++)
@@ -1400,14 +1441,21 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                     int m = dir == 0 ? mx : dir == 1 ? my : mz;
                     int per = dir == 0 ? ip : dir == 1 ? jp : kp;
                     int L = n - 1, R = n + 2;
+                    /* denom: the divisor of the 4th-order ucat difference of -skew;
+                     * next to a body only the k faces reset it (momentum.c:705-723,
+                     * 967-985, 1202-1219) */
+                    double denom = 3.;
 #define NB(t) (dir == 0 ? NV(k, j, t) : dir == 1 ? NV(k, t, i) : NV(t, j, i))
 #define UB(t, q) (dir == 0 ? U3(k, j, t, q) : dir == 1 ? U3(k, t, i, q) : U3(t, j, i, q))
 #define UC(t) (dir == 0 ? ucont[3 * IX(k, j, t) + 0] : dir == 1 ? ucont[3 * IX(k, t, i) + 1] : ucont[3 * IX(t, j, i) + 2])
                     if (n == 0 || n == m - 2) {
                         if (per) L = m - 3, R = 2;
-                        else L = n, R = n + 1;
-                    } else if (NB(L) + NB(R) > 0.1) L = n, R = n + 1;
-                    if (c->second_order) L = n, R = n + 1;
+                        else L = n, R = n + 1, denom = 1.;
+                    } else if (NB(L) + NB(R) > 0.1) {
+                        L = n, R = n + 1;
+                        if (dir == 2) denom = 1.;
+                    }
+                    if (c->second_order) L = n, R = n + 1, denom = 1.;
                     double ucon_f = UC(n);
                     double ucon = ucon_f;
                     if (per && (n == 0 || n == m - 2)) ucon = UC(m - 2);
@@ -1436,10 +1484,20 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                         for (int q = 0; q < 3; q++)
                             d3v[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
                     }
+                    double a3v[3] = {0, 0, 0};
+                    if (c->skew) {   /* momentum.c:798-811, 1060-1071, 1298-1309 */
+                        const double *dn = dir == 0 ? dc : dir == 1 ? de : dz;
+                        for (int q = 0; q < 3; q++) {
+                            d3v[q] *= 0.5;
+                            const double d4 = (UB(R, q) - UB(L, q)) * (1. / denom);
+                            a3v[q] = -0.5 * ucon_f * (9. / 8. * dn[q] - 1. / 8. * d4);
+                        }
+                    }
                     if (NB(n) + NB(n + 1) > 0.1) {
-                        if (!c->immersed) d3v[0] = d3v[1] = d3v[2] = 0;
+                        if (!c->immersed) d3v[0] = d3v[1] = d3v[2] = a3v[0] = a3v[1] = a3v[2] = 0;
                     }
                     dv[3 * id] = d3v[0]; dv[3 * id + 1] = d3v[1]; dv[3 * id + 2] = d3v[2];
+                    if (c->skew) av[3 * id] = a3v[0], av[3 * id + 1] = a3v[1], av[3 * id + 2] = a3v[2];
 
                     double nu = 1. / c->ren, nut = 0;
                     double v0, v1, v2;
@@ -1456,6 +1514,26 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                     v0 += (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * nrm0 + r21 * nrm1 + r31 * nrm2) * ajc * (nu);
                     v1 += (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * nrm0 + r22 * nrm1 + r32 * nrm2) * ajc * (nu);
                     v2 += (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * nrm0 + r23 * nrm1 + r33 * nrm2) * ajc * (nu);
+                    if (c->clark) {   /* momentum.c:907-925, 1142-1160, 1385-1403 */
+                        double dx, dy, dzl;
+                        /* the cell metrics of point (k,j,i) with the face's Jacobian */
+                        dxdydz(ajc, &g->csi[3 * id], &g->eta[3 * id], &g->zet[3 * id], &dx, &dy, &dzl);
+                        const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dzl * dzl;
+                        double du_dx, du_dy, du_dz, dv_dx, dv_dy, dv_dz, dw_dx, dw_dy, dw_dz;
+                        du_dxyz_face(csi0, csi1, csi2, eta0, eta1, eta2, zet0, zet1, zet2, ajc, dc, de, dz, &du_dx, &dv_dx,
+                                     &dw_dx, &du_dy, &dv_dy, &dw_dy, &du_dz, &dv_dz, &dw_dz);
+                        const double t11 = (du_dx * du_dx * dx2 + du_dy * du_dy * dy2 + du_dz * du_dz * dz2);
+                        const double t12 = (du_dx * dv_dx * dx2 + du_dy * dv_dy * dy2 + du_dz * dv_dz * dz2);
+                        const double t13 = (du_dx * dw_dx * dx2 + du_dy * dw_dy * dy2 + du_dz * dw_dz * dz2);
+                        const double t21 = t12;
+                        const double t22 = (dv_dx * dv_dx * dx2 + dv_dy * dv_dy * dy2 + dv_dz * dv_dz * dz2);
+                        const double t23 = (dv_dx * dw_dx * dx2 + dv_dy * dw_dy * dy2 + dv_dz * dw_dz * dz2);
+                        const double t31 = t13, t32 = t23;
+                        const double t33 = (dw_dx * dw_dx * dx2 + dw_dy * dw_dy * dy2 + dw_dz * dw_dz * dz2);
+                        v0 -= (t11 * nrm0 + t12 * nrm1 + t13 * nrm2) / 12.;
+                        v1 -= (t21 * nrm0 + t22 * nrm1 + t23 * nrm2) / 12.;
+                        v2 -= (t31 * nrm0 + t32 * nrm1 + t33 * nrm2) / 12.;
+                    }
                     vs[3 * id] = v0; vs[3 * id + 1] = v1; vs[3 * id + 2] = v2;
 #undef NB
 #undef UB
@@ -1465,6 +1543,7 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
     /* periodic copies of the face arrays, momentum.c:1463-1504 */
     per_copy_vec(c, div1); per_copy_vec(c, div2); per_copy_vec(c, div3);
     per_copy_vec(c, visc1); per_copy_vec(c, visc2); per_copy_vec(c, visc3);
+    if (c->skew) { per_copy_vec(c, adv1); per_copy_vec(c, adv2); per_copy_vec(c, adv3); }
 
     /* cell divergence with 4th-order deferred correction, momentum.c:1506-1636 */
 #pragma omp parallel for schedule(static)
@@ -1501,6 +1580,16 @@ int vo_formfunction2(const vo_cfg *c, const vo_geom *g, const double *nvert,
                         d4[q] += (div2[3 * a2 + q] - div2[3 * b2 + q]) * (1. / den_j);
                         d4[q] += (div3[3 * a3 + q] - div3[3 * b3 + q]) * (1. / den_k);
                         f[q] = 9. / 8. * dv[q] + -1. / 8. * d4[q];
+                    }
+                }
+                if (c->skew && !c->inviscid) {   /* momentum.c:1596-1608 */
+                    if (c->second_order) iR = i, iL = i - 1, jR = j, jL = j - 1, kR = k, kL = k - 1;
+                    const size_t a1 = IX(k, j, iR), b1 = IX(k, j, iL), a2 = IX(k, jR, i), b2 = IX(k, jL, i);
+                    const size_t a3 = IX(kR, j, i), b3 = IX(kL, j, i);
+                    for (int q = 0; q < 3; q++) {
+                        f[q] += 9. / 8. * 0.5 * (adv1[3 * id + q] + adv1[3 * im + q]) - 1. / 8. * 0.5 * (adv1[3 * a1 + q] + adv1[3 * b1 + q]);
+                        f[q] += 9. / 8. * 0.5 * (adv2[3 * id + q] + adv2[3 * jm + q]) - 1. / 8. * 0.5 * (adv2[3 * a2 + q] + adv2[3 * b2 + q]);
+                        f[q] += 9. / 8. * 0.5 * (adv3[3 * id + q] + adv3[3 * km + q]) - 1. / 8. * 0.5 * (adv3[3 * a3 + q] + adv3[3 * b3 + q]);
                     }
                 }
                 if (!c->inviscid)   /* momentum.c:1621-1627 */

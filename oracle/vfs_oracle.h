@@ -31,6 +31,8 @@ typedef struct {
     int dpdz_set;          /* 1: k-periodic forcing uses mean_pressure_gradient     */
     int weno;              /* 1: WENO3 convective face values (momentum.c:782-786)  */
     int inviscid;          /* -inv: WENO3 convection, 2nd-order divergence, no viscous term */
+    int skew;              /* -skew: skew-symmetric convection (momentum.c:798-811, 1596-1608) */
+    int clark;             /* -clark: Clark model stress in the viscous fluxes (momentum.c:907-925) */
     double ren, dt, st;
     double mean_pressure_gradient;
 } vo_cfg;

@@ -32,5 +32,5 @@ def test_workload_rng_is_deterministic_and_uniform():
 
 
 def test_config_layout_matches_header():
-    # vfs_config: 3 ints + 3 ints + 6 ints + 4 ints, then 4 doubles
-    assert C.sizeof(V.Config) == 18 * 4 + 4 * 8
+    # vfs_config: 3 ints + 3 ints + 6 ints + 8 ints, then 4 doubles
+    assert C.sizeof(V.Config) == 20 * 4 + 4 * 8

@@ -98,12 +98,17 @@ def test_formfunction2_reference_passes(case, monkeypatch):
     test_formfunction2(case)
 
 
-@pytest.mark.parametrize("scheme", [dict(weno=1), dict(inviscid=1), dict(inviscid=1, second_order=1)],
-                         ids=["weno", "inviscid", "inviscid-2nd"])
+@pytest.mark.parametrize("scheme", [dict(weno=1), dict(inviscid=1), dict(inviscid=1, second_order=1), dict(skew=1),
+                                    dict(clark=1), dict(skew=1, clark=1, second_order=1), dict(skew=1, inviscid=1)],
+                         ids=["weno", "inviscid", "inviscid-2nd", "skew", "clark", "skew-clark-2nd", "skew-inviscid"])
 def test_formfunction2_weno(case, scheme):
     """-weno 1: WENO3 convective face values (momentum.c:782-786 and the j/k
     counterparts); -inv: WENO3 whatever else is set, the 2nd-order divergence
-    and no viscous term (momentum.c:757-773, 1582, 1621) on the same fields."""
+    and no viscous term (momentum.c:757-773, 1582, 1621); -skew: half the
+    divergence form plus the advective face fluxes interpolated to the cells
+    (momentum.c:798-811, 1596-1608), whose cell part -inv skips; -clark: the
+    Clark model stress in the viscous face fluxes (momentum.c:907-925); all
+    on the same fields."""
     cfg = dict(case["cfg"], **scheme)
     _, spec, coor, o, ucat, ucont = oracle_case(case["name"])
     o = __import__("oracle_py").Oracle(cfg, coor)
@@ -123,6 +128,27 @@ def test_formfunction2_weno(case, scheme):
     ctx.formfunction2(duc, du, dn, drhs, 1.0)
     assert rel_err(drhs.host(), rhs) < TOL
     ctx.close()
+
+
+def test_derivatives_reject_skew_and_clark(case):
+    """The JVP / VJP do not differentiate -skew or -clark, and the dynamic
+    procedure has no Clark term: such contexts are refused before any work
+    is enqueued."""
+    _, spec, coor, o, ucat, ucont = oracle_case(case["name"])
+    for opt in ("skew", "clark"):
+        ctx = gpu_case(dict(case["cfg"], **{opt: 1}), spec, coor)
+        U, u, w, jw, nut = ctx.vec(), ctx.vec(), ctx.vec(), ctx.vec(), ctx.sca()
+        for f in (ctx.rhs_jvp, ctx.rhs_vjp):
+            with pytest.raises(V.VfsError) as e:
+                f(U, u, nut, w, jw)
+            assert e.value.rc == -2
+            # wall-function walls are refused first, for their own reason
+            assert opt in str(e.value) or -1 in case["cfg"]["bctype"]
+        if opt == "clark":
+            with pytest.raises(V.VfsError) as e:
+                ctx.smagorinsky_constant(u, nut, les=2, ti=5)
+            assert e.value.rc == -2 and "Clark" in str(e.value)
+        ctx.close()
 
 
 def test_pressure_gradient(case):

@@ -315,6 +315,70 @@ def pin_ff2_affine_viscous():
         assert np.abs(D[inner][..., d] - ref).max() <= 1e-9 * abs(ref), d
 
 
+def pin_ff2_skew_affine():
+    """-skew with a linear velocity on the sheared walled grid, viscosity
+    off: half the divergence form plus half the advective form,
+    rhs_f = -S_f.(G u + u tr G / 2)(x_f), on every unmasked face, with the
+    4th-order and the second-order scheme (the face differences and the
+    interpolation to the cells are exact for linear fields, the wall rows'
+    second-order ones included)."""
+    rng = np.random.default_rng(15)
+    G = rng.uniform(-1, 1, (3, 3))
+    u0 = rng.uniform(-1, 1, 3)
+    uf = lambda x: x @ G.T + u0  # noqa: E731
+    for second in (0, 1):
+        aff = Affine(ren=1e300)
+        nv = np.zeros((aff.mz, aff.my, aff.mx))
+        aff.o.c.skew = 1
+        aff.o.c.second_order = second
+        R = _ff2(aff.o, aff.face_flux(uf), aff.cells(uf))
+        for d in range(3):
+            u = uf(aff.xf[d])
+            ref = -((u @ G.T) + 0.5 * u * np.trace(G)) @ aff.S[d]
+            m = _unmasked(aff, nv, d)
+            assert np.abs(R[..., d][m] - ref[m]).max() <= 1e-12 * np.abs(ref).max(), (second, d)
+
+
+def _cell_lengths(M):
+    """The cell length scales along x, y, z that the Clark term uses on an
+    affine grid: |sum_m (V / |S_m|) e_m / |e_m||, the heights of the cell
+    over its three face pairs along the unit lattice vectors."""
+    e = [M[:, m] for m in range(3)]
+    S = [np.cross(e[1], e[2]), np.cross(e[2], e[0]), np.cross(e[0], e[1])]
+    V = float(np.linalg.det(M))
+    return np.abs(sum(V / np.linalg.norm(S[m]) * e[m] / np.linalg.norm(e[m]) for m in range(3)))
+
+
+def pin_ff2_clark_affine():
+    """-clark with a quadratic velocity u = H x x / 2 + G x on the sheared
+    grid: the face gradients are exact, so the model stress
+    T_ab = sum_m du_a/dx_m du_b/dx_m dx_m^2 is quadratic along the lattice
+    and the term's share of the right-hand side, rhs(clark) - rhs, equals
+    -S_f.(div T)(x_f) / 12 on the faces away from the walls."""
+    rng = np.random.default_rng(16)
+    aff = Affine(N=(13, 12, 11))
+    H = rng.uniform(-1, 1, (3, 3, 3))
+    H = 0.5 * (H + H.transpose(0, 2, 1))
+    G = rng.uniform(-1, 1, (3, 3))
+
+    def uf(x):
+        return 0.5 * np.einsum("abc,...b,...c->...a", H, x, x) + x @ G.T
+
+    dl2 = _cell_lengths(aff.M) ** 2
+    uc, u = aff.face_flux(uf), aff.cells(uf)
+    R0 = _ff2(aff.o, uc, u)
+    aff.o.c.clark = 1
+    R1 = _ff2(aff.o, uc, u)
+    D = R1 - R0
+    inner = (slice(3, -4), slice(3, -4), slice(3, -4))
+    for d in range(3):
+        x = aff.xf[d][inner]
+        J = np.einsum("abc,...c->...ab", H, x) + G                      # du_a / dx_b
+        divT = np.einsum("amb,...bm,m->...a", H, J, dl2) + np.einsum("...am,bmb,m->...a", J, H, dl2)
+        ref = -(divT @ aff.S[d]) / 12.0
+        assert np.abs(D[inner][..., d] - ref).max() <= 1e-9 * np.abs(ref).max(), d
+
+
 def _body_fields(aff, nv, uf, rng):
     """Cell velocity uf in fluid and IB cells, garbage in solid cells; face
     fluxes uf.S except on faces that touch a solid cell (garbage)."""
@@ -549,6 +613,7 @@ def pin_taylor_green():
 PINS = {f.__name__: f for f in (pin_pressure_gradient_affine_with_body, pin_mean_pressure_gradient, pin_flux_forcing,
                                 pin_ib_bc_face_rule,
                                 pin_min_dt, pin_ff2_affine_convective, pin_ff2_affine_viscous,
+                                pin_ff2_skew_affine, pin_ff2_clark_affine,
                                 pin_ff2_body_uniform_stream, pin_ff2_body_viscous_linear, pin_rotation_invariance,
                                 pin_taylor_green)}
 
@@ -636,6 +701,30 @@ MUTANTS = [
     ("Calc_Minimum_dt pairs the eta flux with the wrong Jacobian index",
      [("a = fabs(1. / VY(ucont, id) / g->aj[id]);", "a = fabs(1. / VX(ucont, id) / g->aj[id]);", 1)],
      ["pin_min_dt"]),
+    ("skew: divergence form not halved",
+     [("d3v[q] *= 0.5;", "d3v[q] *= 1.0;", 1)],
+     ["pin_ff2_skew_affine"]),
+    ("skew: 4th-order velocity difference reversed",
+     [("const double d4 = (UB(R, q) - UB(L, q)) * (1. / denom);",
+       "const double d4 = (UB(L, q) - UB(R, q)) * (1. / denom);", 1)],
+     ["pin_ff2_skew_affine"]),
+    ("skew: wall faces keep the divisor 3",
+     [("else L = n, R = n + 1, denom = 1.;", "else L = n, R = n + 1;", 1)],
+     ["pin_ff2_skew_affine"]),
+    ("skew: interpolation to the cell from the wrong side",
+     [("f[q] += 9. / 8. * 0.5 * (adv1[3 * id + q] + adv1[3 * im + q])",
+       "f[q] += 9. / 8. * 0.5 * (adv1[3 * id + q] + adv1[3 * a1 + q])", 1)],
+     ["pin_ff2_skew_affine"]),
+    ("clark: the y length scale taken from x",
+     [("const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dzl * dzl;",
+       "const double dx2 = dx * dx, dy2 = dx * dx, dz2 = dzl * dzl;", 1)],
+     ["pin_ff2_clark_affine"]),
+    ("clark: t23 from the u gradient",
+     [("const double t23 = (dv_dx * dw_dx * dx2", "const double t23 = (du_dx * dw_dx * dx2", 1)],
+     ["pin_ff2_clark_affine"]),
+    ("clark: the stress applied with the wrong sign",
+     [("v0 -= (t11 * nrm0 + t12 * nrm1 + t13 * nrm2) / 12.;", "v0 += (t11 * nrm0 + t12 * nrm1 + t13 * nrm2) / 12.;", 1)],
+     ["pin_ff2_clark_affine"]),
 ]
 
 

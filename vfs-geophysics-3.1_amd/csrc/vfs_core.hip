@@ -114,6 +114,8 @@ static int check_config(const vfs_config *cfg)
     if (cfg->immersed != 0 && cfg->immersed != 1) { vfs_set_error("immersed must be 0 or 1"); return -4; }
     if (cfg->weno != 0 && cfg->weno != 1) { vfs_set_error("weno must be 0 or 1"); return -4; }
     if (cfg->inviscid != 0 && cfg->inviscid != 1) { vfs_set_error("inviscid must be 0 or 1"); return -4; }
+    if (cfg->skew != 0 && cfg->skew != 1) { vfs_set_error("skew must be 0 or 1"); return -4; }
+    if (cfg->clark != 0 && cfg->clark != 1) { vfs_set_error("clark must be 0 or 1"); return -4; }
     return 0;
 }
 
@@ -133,6 +135,8 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     d.dpdz_set = cfg->dpdz_set;
     d.weno = cfg->weno;
     d.inviscid = cfg->inviscid;
+    d.skew = cfg->skew;
+    d.clark = cfg->clark;
     d.ren = cfg->ren; d.dt = cfg->dt; d.st = cfg->st; d.mpg = cfg->mean_pressure_gradient;
     d.P = (long long)d.mx * d.my * d.mz;
     if (hipGetDevice(&c->device) != hipSuccess) { delete c; vfs_set_error("no HIP device"); return -1; }
@@ -150,7 +154,7 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
     double **scas[4] = {&c->g.aj, &c->g.iaj, &c->g.jaj, &c->g.kaj};
     for (int n = 0; n < 4; n++) { *scas[n] = p; p += P; }
     bool ok = hipMalloc(&c->nvert, P * sizeof(double)) == hipSuccess &&
-              hipMalloc(&c->face, 18 * P * sizeof(double)) == hipSuccess &&
+              hipMalloc(&c->face, VFS_FACE_SLOTS(d) * P * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->fp, 3 * P * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->snap, 3 * P * sizeof(double)) == hipSuccess &&
               hipMalloc(&c->tmp3, 3 * P * sizeof(double)) == hipSuccess &&
@@ -158,7 +162,7 @@ extern "C" int vfs_ctx_create(const vfs_config *cfg, vfs_ctx **out)
               hipMalloc(&c->kflux, 4 * sizeof(double)) == hipSuccess;
     if (!ok) { vfs_ctx_destroy(c); vfs_set_error("hipMalloc scratch"); return -1; }
     hipMemset(c->nvert, 0, P * sizeof(double));
-    hipMemset(c->face, 0, 18 * P * sizeof(double));
+    hipMemset(c->face, 0, VFS_FACE_SLOTS(d) * P * sizeof(double));
     hipMemset(c->fp, 0, 3 * P * sizeof(double));
     hipMemset(c->snap, 0, 3 * P * sizeof(double));
     hipMemset(c->tmp3, 0, 3 * P * sizeof(double));

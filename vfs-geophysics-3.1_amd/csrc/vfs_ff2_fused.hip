@@ -244,7 +244,7 @@ k_ff2_faces_lds(Dims d, Geom g, const double *__restrict__ nvert, const double *
 // 4th-order stencil wraps around to cell 2 (momentum.c:699-705).
 // Segment s of the launch covers face direction s % 3 on plane 0 (s < 3) or
 // on plane m-2 (s >= 3).
-__global__ void k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
+__global__ void __launch_bounds__(256) k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
                                 const double *__restrict__ ucat, const double *__restrict__ nu_t,
                                 double *__restrict__ face)
 {
@@ -266,10 +266,10 @@ __global__ void k_ff2_faces_bnd(Dims d, Geom g, const double *__restrict__ nvert
             k = s < 3 ? 0 : d.mz - 2; i = 1 + (int)(t % (d.mx - 2)); j = 1 + (int)(t / (d.mx - 2));
         }
         const long long o = IX(d, k, j, i);
-        double D[3], V[3];
-        if (dir == 0) face_flux<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
-        else if (dir == 1) face_flux<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
-        else face_flux<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        double D[3], V[3], A[3];   // A: -skew only, which takes the passes
+        if (dir == 0) face_flux<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
+        else if (dir == 1) face_flux<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
+        else face_flux<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
         for (int q = 0; q < 3; q++) { face[(dir * 3 + q) * P + o] = D[q]; face[((3 + dir) * 3 + q) * P + o] = V[q]; }
         return;
     }
@@ -373,8 +373,10 @@ static int ff2_plain_flags(vfs_ctx *c, int gx, int gy)
 int vfs_ff2_fused(vfs_ctx *c, const double *ucont, const double *ucat, const double *nu_t, double *rhs, double scale)
 {
     // the staged kernels hold nvert as FP32 (exact for the integer masks) and
-    // address the arrays with 32-bit byte offsets
-    if (!c->nvert_f32 || !VFS_IX32_OK(c->d.P)) return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
+    // address the arrays with 32-bit byte offsets; -skew and -clark take the
+    // reference-structured passes
+    if (!c->nvert_f32 || !VFS_IX32_OK(c->d.P) || c->d.skew || c->d.clark)
+        return vfs_ff2_passes(c, ucont, ucat, nu_t, rhs, scale);
     const Dims d = c->d;
     const int kchunk = VFS_FACES_KCHUNK;
     const dim3 grid((d.mx - 2 + TI - 1) / TI, (d.my - 2 + TJ - 1) / TJ, (d.mz - 2 + kchunk - 1) / kchunk);

@@ -230,6 +230,7 @@ extern "C" int vfs_implicit_matrixfree(vfs_ctx *c, double *U, const double *U_o,
     VFS_REQUIRE(!vfs_has_wallfn(d),
                 "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(!d.inviscid, "the derivatives do not cover the inviscid scheme (-inv)");
+    VFS_REQUIRE(!d.skew && !d.clark, "the derivatives do not cover -skew or -clark");
     VFS_REQUIRE(d.bc[4] != 5 && d.bc[5] != 4, "the derivatives do not cover open k boundaries (bctypes 5 and 4)");
     VFS_REQUIRE(restart >= 1 && restart <= VFS_RED_SLOTS - 4, "restart must be in [1, 60]");
     VFS_REQUIRE(rtol > 0 && max_newton >= 0 && max_krylov >= 1, "bad tolerance or iteration limits");

@@ -19,7 +19,7 @@ struct Dims {
     int mx, my, mz;
     int ip, jp, kp;          // i/j/k_periodic
     int bc[6];               // bctype
-    int immersed, les, second_order, dpdz_set, weno, inviscid;
+    int immersed, les, second_order, dpdz_set, weno, inviscid, skew, clark;
     double ren, dt, st, mpg;
     long long P;             // mx*my*mz
 };
@@ -55,6 +55,9 @@ template <typename Idx> __device__ __forceinline__ Idx ixt(const Dims &d, int k,
 template <> __device__ __forceinline__ long long ixt<long long>(const Dims &d, int k, int j, int i) { return IX(d, k, j, i); }
 template <> __device__ __forceinline__ unsigned ixt<unsigned>(const Dims &d, int k, int j, int i) { return IX32(d, k, j, i); }
 #define VFS_IX32_OK(P) (24ull * (unsigned long long)(P) < (1ull << 32))
+
+// slots of the face array: div1..3, visc1..3 and, under -skew, adv1..3 (3 each)
+#define VFS_FACE_SLOTS(d) ((d).skew ? 27 : 18)
 
 // Periodic ghost source of point (k,j,i): the reference's `flag/a/b/c` idiom
 // (e.g. momentum.c:642-669).  Sources are never flagged themselves.
@@ -129,7 +132,7 @@ struct vfs_ctx {
     double *nvert;
     int nvert_f32;           // every nvert value is exactly representable in FP32
     // scratch (device)
-    double *face;            // 18*P: div1..3 and visc1..3 (vector each)
+    double *face;            // 18*P: div1..3 and visc1..3 (vector each); 27*P with adv1..3 under -skew
     double *fp;              // 3*P
     double *snap;            // 3*P: Contra2Cart boundary snapshot / SNES copies
     double *tmp3;            // 3*P general

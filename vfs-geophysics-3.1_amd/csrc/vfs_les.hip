@@ -593,6 +593,7 @@ extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, const vf
     VFS_REQUIRE(o, "options required");
     VFS_REQUIRE(o->les >= 1 && o->les <= 3, "les must be 1 (constant Cs), 2 (dynamic) or 3 (dynamic, relaxed)");
     VFS_REQUIRE(ucat && cs, "ucat and cs are required");
+    VFS_REQUIRE(!c->d.clark || o->les == 1, "the dynamic procedure does not cover the Clark term (-clark, les.c:648)");
     const int nh = (o->i_homo_filter != 0) + (o->j_homo_filter != 0) + (o->k_homo_filter != 0);
     VFS_REQUIRE(nh <= 1 || (nh == 2 && o->i_homo_filter && o->k_homo_filter),
                 "homogeneous filters: one direction, or i and k together");

@@ -334,7 +334,7 @@ extern "C" int vfs_ib_bc(vfs_ctx *c, double *ucont, double *ucat)
 // Formfunction_2, reference-structured passes
 // ---------------------------------------------------------------------------
 // face fluxes of all three directions at point (k,j,i): div and visc
-__global__ void k_ff2_faces(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
+__global__ void __launch_bounds__(256) k_ff2_faces(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
                             const double *__restrict__ ucat, const double *__restrict__ nu_t, double *__restrict__ face)
 {
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -342,21 +342,27 @@ __global__ void k_ff2_faces(Dims d, Geom g, const double *__restrict__ nvert, co
     int i = (int)(t % d.mx), j = (int)((t / d.mx) % d.my), k = (int)(t / ((long long)d.mx * d.my));
     if (i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
     const long long P = d.P;
-    double D[3], V[3];
+    double D[3], V[3], A[3];
     if (j >= 1 && k >= 1) {
-        face_flux<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        face_flux<0>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
 #pragma unroll
         for (int q = 0; q < 3; q++) { face[(0 * 3 + q) * P + t] = D[q]; face[(3 * 3 + q) * P + t] = V[q]; }
+        if (d.skew)
+            for (int q = 0; q < 3; q++) face[(6 * 3 + q) * P + t] = A[q];
     }
     if (i >= 1 && k >= 1) {
-        face_flux<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        face_flux<1>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
 #pragma unroll
         for (int q = 0; q < 3; q++) { face[(1 * 3 + q) * P + t] = D[q]; face[(4 * 3 + q) * P + t] = V[q]; }
+        if (d.skew)
+            for (int q = 0; q < 3; q++) face[(7 * 3 + q) * P + t] = A[q];
     }
     if (i >= 1 && j >= 1) {
-        face_flux<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V);
+        face_flux<2>(d, g, nvert, ucont, ucat, nu_t, i, j, k, D, V, A);
 #pragma unroll
         for (int q = 0; q < 3; q++) { face[(2 * 3 + q) * P + t] = D[q]; face[(5 * 3 + q) * P + t] = V[q]; }
+        if (d.skew)
+            for (int q = 0; q < 3; q++) face[(8 * 3 + q) * P + t] = A[q];
     }
 }
 

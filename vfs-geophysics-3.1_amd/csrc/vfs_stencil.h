@@ -173,6 +173,36 @@ struct ZeroN {
     __device__ __forceinline__ double operator()(int, int, int) const { return 0.0; }
 };
 
+// Calculate_normal_and_area + Calculate_dxdydz (rhs2.c:592-643): the cell's
+// length scales along x, y and z from the metrics cs / et / ze and the
+// Jacobian ajc (volume 1 / ajc divided by the face areas, along the unit
+// covariant base vectors).
+__device__ __forceinline__ void dxdydz(double ajc, const double cs[3], const double et[3], const double ze[3],
+                                       double &dx, double &dy, double &dz)
+{
+    const double vol = 1. / ajc;
+    const double a11 = cs[0], a12 = cs[1], a13 = cs[2];
+    const double a21 = et[0], a22 = et[1], a23 = et[2];
+    const double a31 = ze[0], a32 = ze[1], a33 = ze[2];
+    const double det = a11 * (a33 * a22 - a32 * a23) - a21 * (a33 * a12 - a32 * a13) + a31 * (a23 * a12 - a22 * a13);
+    double nx_i = (a33 * a22 - a32 * a23) / det, nx_j = -(a33 * a12 - a32 * a13) / det, nx_k = (a23 * a12 - a22 * a13) / det;
+    double ny_i = -(a33 * a21 - a31 * a23) / det, ny_j = (a33 * a11 - a31 * a13) / det, ny_k = -(a23 * a11 - a21 * a13) / det;
+    double nz_i = (a32 * a21 - a31 * a22) / det, nz_j = -(a32 * a11 - a31 * a12) / det, nz_k = (a22 * a11 - a21 * a12) / det;
+    const double sum_i = sqrt(nx_i * nx_i + ny_i * ny_i + nz_i * nz_i);
+    const double sum_j = sqrt(nx_j * nx_j + ny_j * ny_j + nz_j * nz_j);
+    const double sum_k = sqrt(nx_k * nx_k + ny_k * ny_k + nz_k * nz_k);
+    const double Ai = sqrt(a11 * a11 + a12 * a12 + a13 * a13);
+    const double Aj = sqrt(a21 * a21 + a22 * a22 + a23 * a23);
+    const double Ak = sqrt(a31 * a31 + a32 * a32 + a33 * a33);
+    nx_i /= sum_i, ny_i /= sum_i, nz_i /= sum_i;
+    nx_j /= sum_j, ny_j /= sum_j, nz_j /= sum_j;
+    nx_k /= sum_k, ny_k /= sum_k, nz_k /= sum_k;
+    const double Li = vol / Ai, Lj = vol / Aj, Lk = vol / Ak;
+    dx = fabs(Li * nx_i + Lj * nx_j + Lk * nx_k);
+    dy = fabs(Li * ny_i + Lj * ny_j + Lk * ny_k);
+    dz = fabs(Li * nz_i + Lj * nz_j + Lk * nz_k);
+}
+
 // WENO: false compiles the WENO3 branch out (for kernels launched only when
 // the weno flag is off).
 // Idx: index type of the global arrays (unsigned: 32-bit offsets).
@@ -181,11 +211,13 @@ struct ZeroN {
 // direction (0 < n < m-2); every branch those conditions decide is then the
 // interior one, and the checks are compiled out.  Same expressions, so the
 // results are bit-identical to the general path on such faces.
+// OPTS: the -skew and -clark terms are compiled in (A: the advective flux of
+// -skew, written when d.skew is set).
 template <int DIR, typename S, typename FU, typename FC, typename FN, typename FM = StoredFaceMetric, bool WENO = true,
-          typename Idx = long long, bool PLAIN = false>
+          typename Idx = long long, bool PLAIN = false, bool OPTS = false>
 __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const FN &N,
                                             const FC &UCon, const FU &U, const double *__restrict__ nu_t,
-                                            int i, int j, int k, S D[3], S V[3], const FM &fm = FM())
+                                            int i, int j, int k, S D[3], S V[3], const FM &fm = FM(), S *A = nullptr)
 {
     const Idx id = ixt<Idx>(d, k, j, i);
     double mcs[3], met[3], mze[3], ajc;
@@ -229,16 +261,20 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     const Idx base = id - (Idx)n * st;   // point with index 0 along DIR
 #define NB(tt) N(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i)
 #define UB(tt, q) U(DIR == 2 ? (tt) : k, DIR == 1 ? (tt) : j, DIR == 0 ? (tt) : i, q)
+    // den: divisor of the 4th-order ucat difference of -skew; next to a body
+    // only the k faces reset it (momentum.c:705-723, 967-985, 1202-1219)
     int L = n - 1, R = n + 2;
+    double den = 3.;
     if (!PLAIN) {
         if (n == 0 || n == m - 2) {
             if (per) L = m - 3, R = 2;
-            else L = n, R = n + 1;
+            else L = n, R = n + 1, den = 1.;
         } else if (NB(L) + NB(R) > 0.1) {
             L = n, R = n + 1;
+            if (DIR == 2) den = 1.;
         }
     }
-    if (d.second_order) L = n, R = n + 1;
+    if (d.second_order) L = n, R = n + 1, den = 1.;
     const S ucon_f = UCon(id);
     S ucon = ucon_f;
     if (!PLAIN) {
@@ -271,7 +307,19 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
 #pragma unroll
         for (int q = 0; q < 3; q++) D[q] = -ucon_f * 0.0625 * (-UB(L, q) + 9. * UB(n, q) + 9. * UB(n + 1, q) - UB(R, q));
     }
-    if (!PLAIN && NB(n) + NB(n + 1) > 0.1 && !d.immersed) D[0] = D[1] = D[2] = S(0.0);
+    if (OPTS && d.skew) {   // momentum.c:798-811, 1060-1071, 1298-1309
+        const S *dn = DIR == 0 ? dc : DIR == 1 ? de : dz;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            D[q] = D[q] * 0.5;
+            const S d4 = (UB(R, q) - UB(L, q)) * (1. / den);
+            A[q] = -0.5 * ucon_f * (9. / 8. * dn[q] - 1. / 8. * d4);
+        }
+    }
+    if (!PLAIN && NB(n) + NB(n + 1) > 0.1 && !d.immersed) {
+        D[0] = D[1] = D[2] = S(0.0);
+        if (OPTS && d.skew) A[0] = A[1] = A[2] = S(0.0);
+    }
 
     const double nu = 1. / d.ren;
     S v0, v1, v2;
@@ -290,6 +338,32 @@ __device__ __forceinline__ void face_flux_t(const Dims &d, const Geom &g, const 
     v0 += (g11 * dc[0] + g21 * de[0] + g31 * dz[0] + r11 * n0 + r21 * n1 + r31 * n2) * ajc * (nu);
     v1 += (g11 * dc[1] + g21 * de[1] + g31 * dz[1] + r12 * n0 + r22 * n1 + r32 * n2) * ajc * (nu);
     v2 += (g11 * dc[2] + g21 * de[2] + g31 * dz[2] + r13 * n0 + r23 * n1 + r33 * n2) * ajc * (nu);
+    if (OPTS && d.clark) {   // momentum.c:907-925, 1142-1160, 1385-1403
+        // length scales from the cell metrics of point (k,j,i) and the face's
+        // Jacobian, as the reference passes them
+        double cc[3], ce[3], cz[3], dx, dy, dzl;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            cc[q] = ldx(g.csi, (Idx)(3 * id + q)); ce[q] = ldx(g.eta, (Idx)(3 * id + q));
+            cz[q] = ldx(g.zet, (Idx)(3 * id + q));
+        }
+        dxdydz(ajc, cc, ce, cz, dx, dy, dzl);
+        const double dx2 = dx * dx, dy2 = dy * dy, dz2 = dzl * dzl;
+        // Compute_du_dxyz (k-omega.c:597-610): r_ab * ajc
+        const S du_dx = r11 * ajc, du_dy = r12 * ajc, du_dz = r13 * ajc;
+        const S dv_dx = r21 * ajc, dv_dy = r22 * ajc, dv_dz = r23 * ajc;
+        const S dw_dx = r31 * ajc, dw_dy = r32 * ajc, dw_dz = r33 * ajc;
+        const S t11 = du_dx * du_dx * dx2 + du_dy * du_dy * dy2 + du_dz * du_dz * dz2;
+        const S t12 = du_dx * dv_dx * dx2 + du_dy * dv_dy * dy2 + du_dz * dv_dz * dz2;
+        const S t13 = du_dx * dw_dx * dx2 + du_dy * dw_dy * dy2 + du_dz * dw_dz * dz2;
+        const S t22 = dv_dx * dv_dx * dx2 + dv_dy * dv_dy * dy2 + dv_dz * dv_dz * dz2;
+        const S t23 = dv_dx * dw_dx * dx2 + dv_dy * dw_dy * dy2 + dv_dz * dw_dz * dz2;
+        const S t33 = dw_dx * dw_dx * dx2 + dw_dy * dw_dy * dy2 + dw_dz * dw_dz * dz2;
+        const S t21 = t12, t31 = t13, t32 = t23;
+        v0 = v0 - (t11 * n0 + t12 * n1 + t13 * n2) / 12.;
+        v1 = v1 - (t21 * n0 + t22 * n1 + t23 * n2) / 12.;
+        v2 = v2 - (t31 * n0 + t32 * n1 + t33 * n2) / 12.;
+    }
     V[0] = v0; V[1] = v1; V[2] = v2;
 #undef NB
 #undef UB
@@ -307,12 +381,15 @@ struct UConDual {
     __device__ __forceinline__ Dual operator()(long long o) const { return Dual(p[3 * o + dir], t[3 * o + dir]); }
 };
 
+// the reference-structured face pass, with -skew and -clark
 template <int DIR>
 __device__ __forceinline__ void face_flux(const Dims &d, const Geom &g, const double *__restrict__ nvert,
                                           const double *__restrict__ ucont, const double *__restrict__ ucat,
-                                          const double *__restrict__ nu_t, int i, int j, int k, double D[3], double V[3])
+                                          const double *__restrict__ nu_t, int i, int j, int k, double D[3], double V[3],
+                                          double A[3])
 {
-    face_flux_t<DIR, double>(d, g, GlobalSca{nvert, d}, UConPlain{ucont, DIR}, GlobalVec{ucat, d}, nu_t, i, j, k, D, V);
+    face_flux_t<DIR, double, GlobalVec, UConPlain, GlobalSca, StoredFaceMetric, true, long long, false, true>(
+        d, g, GlobalSca{nvert, d}, UConPlain{ucont, DIR}, GlobalVec{ucat, d}, nu_t, i, j, k, D, V, StoredFaceMetric(), A);
 }
 
 // index of a face along a periodic direction as stored: face 0 is a copy of
@@ -356,7 +433,8 @@ __device__ __forceinline__ CellFaces cell_faces(const Dims &d, const double *__r
 }
 
 // fp of cell (k,j,i) from the face arrays (momentum.c:1506-1636).
-// face: slot-major SoA, slot s = 3*kind + comp with kind 0..2 div1..3, 3..5 visc1..3.
+// face: slot-major SoA, slot s = 3*kind + comp with kind 0..2 div1..3, 3..5
+// visc1..3 and, under -skew, 6..8 adv1..3.
 __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict__ nvert,
                                         const double *__restrict__ face, int i, int j, int k, double f[3])
 {
@@ -377,6 +455,13 @@ __device__ __forceinline__ void cell_fp(const Dims &d, const double *__restrict_
             d4 += (FI(1, q, k, c.jR, i) - FI(1, q, k, c.jL, i)) * (1. / c.den_j);
             d4 += (FI(2, q, c.kR, j, i) - FI(2, q, c.kL, j, i)) * (1. / c.den_k);
             r = 9. / 8. * dv + -1. / 8. * d4;
+        }
+        if (d.skew && !d.inviscid) {   // momentum.c:1596-1608: the advective fluxes, 4th-order interpolated
+            int iR = c.iR, iL = c.iL, jR = c.jR, jL = c.jL, kR = c.kR, kL = c.kL;
+            if (d.second_order) iR = i, iL = im, jR = j, jL = jm, kR = k, kL = km;
+            r += 9. / 8. * 0.5 * (FI(6, q, k, j, i) + FI(6, q, k, j, im)) - 1. / 8. * 0.5 * (FI(6, q, k, j, iR) + FI(6, q, k, j, iL));
+            r += 9. / 8. * 0.5 * (FI(7, q, k, j, i) + FI(7, q, k, jm, i)) - 1. / 8. * 0.5 * (FI(7, q, k, jR, i) + FI(7, q, k, jL, i));
+            r += 9. / 8. * 0.5 * (FI(8, q, k, j, i) + FI(8, q, km, j, i)) - 1. / 8. * 0.5 * (FI(8, q, kR, j, i) + FI(8, q, kL, j, i));
         }
         if (!d.inviscid)   // momentum.c:1621-1627
             r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) +

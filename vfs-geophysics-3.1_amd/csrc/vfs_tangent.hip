@@ -148,6 +148,7 @@ extern "C" int vfs_rhs_jvp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_REQUIRE(!vfs_has_wallfn(c->d),
                 "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(!c->d.inviscid, "the derivatives do not cover the inviscid scheme (-inv)");
+    VFS_REQUIRE(!c->d.skew && !c->d.clark, "the derivatives do not cover -skew or -clark");
     VFS_REQUIRE(c->d.bc[4] != 5 && c->d.bc[5] != 4, "the derivatives do not cover open k boundaries (bctypes 5 and 4)");
     VFS_REQUIRE(U && ucat && w && jw, "null argument");
     VFS_REQUIRE(w != jw, "w and jw must not alias");
@@ -649,6 +650,7 @@ extern "C" int vfs_rhs_vjp(vfs_ctx *c, const double *U, const double *ucat, cons
     VFS_REQUIRE(!vfs_has_wallfn(d),
                 "the derivatives do not cover the wall model of wall-function walls (bctype -1)");
     VFS_REQUIRE(!d.inviscid, "the derivatives do not cover the inviscid scheme (-inv)");
+    VFS_REQUIRE(!d.skew && !d.clark, "the derivatives do not cover -skew or -clark");
     VFS_REQUIRE(d.bc[4] != 5 && d.bc[5] != 4, "the derivatives do not cover open k boundaries (bctypes 5 and 4)");
     VFS_REQUIRE(U && ucat && v && jtv, "null argument");
     VFS_REQUIRE(v != jtv, "v and jtv must not alias");

@@ -57,7 +57,7 @@ class Config(C.Structure):
                 ("i_periodic", C.c_int), ("j_periodic", C.c_int), ("k_periodic", C.c_int),
                 ("bctype", C.c_int * 6), ("immersed", C.c_int), ("les", C.c_int),
                 ("second_order", C.c_int), ("dpdz_set", C.c_int), ("weno", C.c_int), ("inviscid", C.c_int),
-                ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
+                ("skew", C.c_int), ("clark", C.c_int), ("ren", C.c_double), ("dt", C.c_double), ("st", C.c_double),
                 ("mean_pressure_gradient", C.c_double)]
 
     @classmethod
