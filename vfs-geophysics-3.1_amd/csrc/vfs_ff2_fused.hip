@@ -275,13 +275,34 @@ __global__ void __launch_bounds__(256) k_ff2_faces_bnd(Dims d, Geom g, const dou
     }
 }
 
-// Cell divergence (momentum.c:1506-1636), k-marching: each thread owns a
-// cell column and keeps the k-direction face fluxes it reuses (convective at
-// planes k-2..k+1, viscous at k-1..k) in registers, so every k-face value is
-// read once per column instead of up to four times.  Planes outside that
-// window (the periodic wrap-around of the first and last cells) are read
-// from memory.  Same expressions and order as cell_fp.
+// Cell divergence (momentum.c:1506-1636), k-marching, with every load
+// address known before any value arrives.  Along each direction the faces a
+// cell can read are the four at cyclic offsets -2, -1, 0, +1 (the far faces
+// of the 4th-order correction, the upwind face, the own face; across a
+// periodic seam face f stands for face 1 + (f - 1) mod (m - 2), the
+// reference's ghost copies): the 4th-order pair takes offsets (+1, -2),
+// the second-order fallback (0, -1).  All of them, and the seven nvert
+// values that decide, are loaded at the top of a step; the k faces move
+// through a register ring.  Same operands in the same order as cell_fp.
 constexpr int DI = 64, DJ = 4, DT = DI * DJ;
+
+// face index f of a direction with m points, as stored: periodic seams wrap
+// cyclically, other out-of-range indices are clamped (never selected)
+__device__ __forceinline__ int face_cyc(int f, int m, int per)
+{
+    if (per) {
+        const int p = m - 2, r = (f - 1) % p;
+        return 1 + (r < 0 ? r + p : r);
+    }
+    return f < 0 ? 0 : (f > m - 1 ? m - 1 : f);
+}
+
+// the 4th-order correction keeps its default faces at cell n (momentum.c:1530-1580)
+__device__ __forceinline__ bool far_faces(int n, int m, int per, double nsum)
+{
+    if (!per && (n == 1 || n == 2 || n == m - 3 || n == m - 2)) return false;
+    return !(nsum > 0.1);
+}
 
 __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__restrict__ nvert,
                                                     const double *__restrict__ face, double *__restrict__ fp,
@@ -294,66 +315,74 @@ __global__ void __launch_bounds__(DT) k_ff2_cells_km(Dims d, const double *__res
     if (i > d.mx - 2 || j > d.my - 2) return;
     const int kb = 1 + bz * kchunk, ke = min(kb + kchunk, d.mz - 1);
     const long long P = d.P;
-    const unsigned sk = (unsigned)d.mx * (unsigned)d.my, col = IX32(d, 0, j, i);
-    // slot s of the face array starts at face + s * P (a uniform pointer)
-#define FI(kind, q, kk, jj, ii) ld32(face + (3 * (kind) + (q)) * P, IX32(d, kk, jj, ii))
-    // ring: convective k-flux at planes k-2..k+1, viscous at k-1..k
+    const unsigned sk = (unsigned)d.mx * (unsigned)d.my, sy = (unsigned)d.mx;
+    // i and j candidate faces (offsets -2, -1, 0, +1) within a plane
+    const unsigned ia = IX32(d, 0, j, face_cyc(i - 2, d.mx, d.ip)), ib = IX32(d, 0, j, face_cyc(i - 1, d.mx, d.ip)),
+                   ic = IX32(d, 0, j, i), id = IX32(d, 0, j, face_cyc(i + 1, d.mx, d.ip));
+    const unsigned ja = IX32(d, 0, face_cyc(j - 2, d.my, d.jp), i), jb = IX32(d, 0, face_cyc(j - 1, d.my, d.jp), i),
+                   jd = IX32(d, 0, face_cyc(j + 1, d.my, d.jp), i);
+    const bool fo = !(d.second_order || d.inviscid);
+    const double *D0 = face, *D1 = face + 3 * P, *D2 = face + 6 * P;
+    const double *V0 = face + 9 * P, *V1 = face + 12 * P, *V2 = face + 15 * P;
+    // ring: convective k-flux at faces k-2..k+1, viscous at k-1..k (cyclic indices)
     double c0[3], c1[3], c2[3], c3[3], v0[3], v1[3];
-#pragma unroll
-    for (int q = 0; q < 3; q++) {
-        const double *D2 = face + (6 + q) * P, *V2 = face + (15 + q) * P;
-        c0[q] = kb - 2 >= 0 ? ld32(D2, col + (kb - 2) * sk) : 0.0;
-        c1[q] = ld32(D2, col + (kb - 1) * sk);
-        c2[q] = ld32(D2, col + kb * sk);
-        c3[q] = kb + 1 <= d.mz - 2 ? ld32(D2, col + (kb + 1) * sk) : 0.0;
-        v0[q] = ld32(V2, col + (kb - 1) * sk);
-        v1[q] = ld32(V2, col + kb * sk);
-    }
-    for (int k = kb; k < ke; k++) {
-        // prefetch the planes the next step brings into the window
-        double nc[3], nv[3];
+    {
+        const unsigned p0 = face_cyc(kb - 2, d.mz, d.kp) * sk, p1 = face_cyc(kb - 1, d.mz, d.kp) * sk,
+                       p2 = (unsigned)kb * sk, p3 = face_cyc(kb + 1, d.mz, d.kp) * sk;
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            nc[q] = k + 2 <= d.mz - 2 ? ld32(face + (6 + q) * P, col + (k + 2) * sk) : 0.0;
-            nv[q] = ld32(face + (15 + q) * P, col + (k + 1) * sk);
+            c0[q] = ld32(D2 + q * P, ic + p0); c1[q] = ld32(D2 + q * P, ic + p1);
+            c2[q] = ld32(D2 + q * P, ic + p2); c3[q] = ld32(D2 + q * P, ic + p3);
+            v0[q] = ld32(V2 + q * P, ic + p1); v1[q] = ld32(V2 + q * P, ic + p2);
         }
-        const CellFaces cf = cell_faces<unsigned>(d, nvert, i, j, k);
-        const int im = cf.im, jm = cf.jm, km = cf.km;
+    }
+    for (int k = kb; k < ke; k++) {
+        const unsigned pk = (unsigned)k * sk;
+        // every load of the step first
+        const double nc = ld32(nvert, ic + pk), nim = ld32(nvert, ic + pk - 1), nip = ld32(nvert, ic + pk + 1),
+                     njm = ld32(nvert, ic + pk - sy), njp = ld32(nvert, ic + pk + sy),
+                     nkm = ld32(nvert, ic + pk - sk), nkp = ld32(nvert, ic + pk + sk);
+        double ai[3][4], aj[3][4], vi[3][2], vj[3][2], nc3[3], nv3[3];
+        const unsigned pn = face_cyc(k + 2, d.mz, d.kp) * sk, pv = face_cyc(k + 1, d.mz, d.kp) * sk;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            ai[q][0] = ld32(D0 + q * P, ia + pk); ai[q][1] = ld32(D0 + q * P, ib + pk);
+            ai[q][2] = ld32(D0 + q * P, ic + pk); ai[q][3] = ld32(D0 + q * P, id + pk);
+            aj[q][0] = ld32(D1 + q * P, ja + pk); aj[q][1] = ld32(D1 + q * P, jb + pk);
+            aj[q][2] = ld32(D1 + q * P, ic + pk); aj[q][3] = ld32(D1 + q * P, jd + pk);
+            vi[q][0] = ld32(V0 + q * P, ib + pk); vi[q][1] = ld32(V0 + q * P, ic + pk);
+            vj[q][0] = ld32(V1 + q * P, jb + pk); vj[q][1] = ld32(V1 + q * P, ic + pk);
+            nc3[q] = ld32(D2 + q * P, ic + pn); nv3[q] = ld32(V2 + q * P, ic + pv);
+        }
+        const bool fi = far_faces(i, d.mx, d.ip, nim + nc + nip);
+        const bool fj = far_faces(j, d.my, d.jp, njm + nc + njp);
+        const bool fk = far_faces(k, d.mz, d.kp, nkm + nc + nkp);
+        const double den_i = fi ? 3. : 1., den_j = fj ? 3. : 1., den_k = fk ? 3. : 1.;
         double f[3];
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            // k-direction fluxes from the window, or memory outside it
-            auto K2 = [&](int kk) -> double {
-                return kk == k - 2 ? c0[q] : kk == k - 1 ? c1[q] : kk == k ? c2[q] : kk == k + 1 ? c3[q]
-                                                                                     : FI(2, q, kk, j, i);
-            };
-            auto K5 = [&](int kk) -> double { return kk == k - 1 ? v0[q] : kk == k ? v1[q] : FI(5, q, kk, j, i); };
-            double dv = (FI(0, q, k, j, i) - FI(0, q, k, j, im) + FI(1, q, k, j, i) - FI(1, q, k, jm, i) +
-                         c2[q] - K2(km));
+            const double dv = (ai[q][2] - ai[q][1] + aj[q][2] - aj[q][1] + c2[q] - c1[q]);
             double r;
-            if (d.second_order || d.inviscid) {
+            if (!fo) {
                 r = dv;
             } else {
                 double d4 = 0;
-                d4 += (FI(0, q, k, j, cf.iR) - FI(0, q, k, j, cf.iL)) * (1. / cf.den_i);
-                d4 += (FI(1, q, k, cf.jR, i) - FI(1, q, k, cf.jL, i)) * (1. / cf.den_j);
-                d4 += (K2(cf.kR) - K2(cf.kL)) * (1. / cf.den_k);
+                d4 += ((fi ? ai[q][3] : ai[q][2]) - (fi ? ai[q][0] : ai[q][1])) * (1. / den_i);
+                d4 += ((fj ? aj[q][3] : aj[q][2]) - (fj ? aj[q][0] : aj[q][1])) * (1. / den_j);
+                d4 += ((fk ? c3[q] : c2[q]) - (fk ? c0[q] : c1[q])) * (1. / den_k);
                 r = 9. / 8. * dv + -1. / 8. * d4;
             }
-            if (!d.inviscid)
-                r += (FI(3, q, k, j, i) - FI(3, q, k, j, im) + FI(4, q, k, j, i) - FI(4, q, k, jm, i) + v1[q] -
-                      K5(km)) / 1.0;
+            if (!d.inviscid) r += (vi[q][1] - vi[q][0] + vj[q][1] - vj[q][0] + v1[q] - v0[q]) / 1.0;
             f[q] = r;
         }
-        const unsigned t = col + k * sk;
+        const unsigned t = ic + pk;
         st32(fp, 3 * t, f[0]); st32(fp, 3 * t + 1, f[1]); st32(fp, 3 * t + 2, f[2]);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
-            c0[q] = c1[q]; c1[q] = c2[q]; c2[q] = c3[q]; c3[q] = nc[q];
-            v0[q] = v1[q]; v1[q] = nv[q];
+            c0[q] = c1[q]; c1[q] = c2[q]; c2[q] = c3[q]; c3[q] = nc3[q];
+            v0[q] = v1[q]; v1[q] = nv3[q];
         }
     }
-#undef FI
 }
 
 // the plane flags of the face kernel, recomputed when the mask has changed
