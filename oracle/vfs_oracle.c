@@ -895,9 +895,16 @@ static void covariant_metrics(const double gg[3][3], double G[3][3])
     G[2][0] = (a32 * a21 - a31 * a22) / det, G[2][1] = -(a32 * a11 - a31 * a12) / det, G[2][2] = (a22 * a11 - a21 * a12) / det;
 }
 
+static void dxdydz(double ajc, const double *csi, const double *eta, const double *zet, double *dx, double *dy,
+                   double *dz);
+
 /* les.c:306-687, second pass: the Germano identity contracted in the
  * contravariant form, LM = L_ba M_aq G_bq and MM = M_nm M_nl G_ml, at the
- * non-solid interior cells (0 elsewhere), then periodic copies. */
+ * non-solid interior cells (0 elsewhere), then periodic copies.  With
+ * -clark, LM also takes - N_ab M_aq G_bq, N = 4 R - <T> the Clark model's
+ * Germano term (les.c:334-340, 410-420, 489-548, 648): T from the gradient
+ * at each box point and R from the gradient of the filtered velocity, both
+ * with the length scales of the centre cell. */
 int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat, const vo_les_opt *o,
                  const double *grad, const double *sabs, const double *ucat_f, double *LM, double *MM)
 {
@@ -917,7 +924,10 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
             for (int i = 1; i < mx - 1; i++) {
                 const size_t id = IX(k, j, i);
                 if (nvert[id] > 1.1) continue;
-                double w[3][3][3], UU[3][3][3][3][3], SS[3][3][3][3][3];
+                double w[3][3][3], UU[3][3][3][3][3], SS[3][3][3][3][3], TT[3][3][3][3][3];
+                double lx, ly, lz, dl2[3];   /* les.c:339-340: the centre cell's length scales */
+                dxdydz(g->aj[id], &g->csi[3 * id], &g->eta[3 * id], &g->zet[3 * id], &lx, &ly, &lz);
+                dl2[0] = lx * lx, dl2[1] = ly * ly, dl2[2] = lz * lz;
                 get_weight(c, g, nvert, i, j, k, w);
                 for (int r = 0; r < 27; r++) {
                     const int R = r / 9, Q = (r / 3) % 3, Pp = r % 3;
@@ -935,6 +945,11 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                         for (int b = 0; b < 3; b++) S[a][b] = 0.5 * (D[a][b] + D[b][a]);
                     for (int a = 0; a < 3; a++)
                         for (int b = 0; b < 3; b++) SS[a][b][R][Q][Pp] = S[a][b] * sabs[n];
+                    if (c->clark)   /* les.c:412-420 */
+                        for (int a = 0; a < 3; a++)
+                            for (int b = 0; b < 3; b++)
+                                TT[a][b][R][Q][Pp] = (D[a][0] * D[b][0] * dl2[0] + D[a][1] * D[b][1] * dl2[1] +
+                                                      D[a][2] * D[b][2] * dl2[2]) / 12.;
                 }
                 double sum_weight = 0;
                 for (int p = -1; p <= 1; p++)
@@ -972,10 +987,26 @@ int vo_dsm_pass2(const vo_cfg *c, const vo_geom *g, const double *nvert, const d
                 for (int a = 0; a < 3; a++)
                     for (int q = 0; q < 3; q++)
                         M[a][q] = Mc[a][0] * gg[q][0] + Mc[a][1] * gg[q][1] + Mc[a][2] * gg[q][2];
+                double N[3][3] = {{0}};
+                if (c->clark) {   /* les.c:489-548: N = 4 R - <T>, contravariant in its second index */
+                    double Nc[3][3];
+                    for (int a = 0; a < 3; a++)
+                        for (int b = 0; b < 3; b++) {
+                            const double Rab = (D[a][0] * D[b][0] * dl2[0] + D[a][1] * D[b][1] * dl2[1] +
+                                                D[a][2] * D[b][2] * dl2[2]) / 12.;
+                            Nc[a][b] = 4.0 * Rab - tfilter(TT[a][b], w, o->testfilter_ik);
+                        }
+                    for (int a = 0; a < 3; a++)
+                        for (int q = 0; q < 3; q++)
+                            N[a][q] = Nc[a][0] * gg[q][0] + Nc[a][1] * gg[q][1] + Nc[a][2] * gg[q][2];
+                }
                 double num = 0, denom = 0;
                 for (int q = 0; q < 3; q++)
                     for (int a = 0; a < 3; a++)
-                        for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
+                        for (int b = 0; b < 3; b++) {
+                            num += L[b][a] * M[a][q] * G[b][q];
+                            if (c->clark) num -= N[a][b] * M[a][q] * G[b][q];   /* les.c:648 */
+                        }
                 for (int m = 0; m < 3; m++)
                     for (int n = 0; n < 3; n++)
                         for (int l = 0; l < 3; l++) denom += M[n][m] * M[n][l] * G[m][l];

@@ -32,7 +32,8 @@ typedef struct {
     int weno;              /* 1: WENO3 convective face values (momentum.c:782-786)  */
     int inviscid;          /* -inv: WENO3 convection, 2nd-order divergence, no viscous term */
     int skew;              /* -skew: skew-symmetric convection (momentum.c:798-811, 1596-1608) */
-    int clark;             /* -clark: Clark model stress in the viscous fluxes (momentum.c:907-925) */
+    int clark;             /* -clark: Clark model stress in the viscous fluxes (momentum.c:907-925)
+                              and its Germano term in the dynamic procedure (les.c:648) */
     double ren, dt, st;
     double mean_pressure_gradient;
 } vo_cfg;
@@ -99,8 +100,8 @@ typedef struct {
     int ti, tistart, rstart;
 } vo_les_opt;
 
-/* les.c:67-1133 Compute_Smagorinsky_Constant_1 without the Clark term
- * (o NULL: -les 2 with the default options at ti = 2).  Cs = 0 while ti < 2
+/* les.c:67-1133 Compute_Smagorinsky_Constant_1, with the Clark term when
+ * c->clark is set (o NULL: -les 2 with the default options at ti = 2).  Cs = 0 while ti < 2
  * on a fresh start.  Cs: in (the previous Cs, read by -les 3) and out. */
 int vo_smagorinsky_constant(const vo_cfg *c, const vo_geom *g, const double *nvert, const double *ucat,
                             const vo_les_opt *o, double *Cs);

@@ -131,9 +131,8 @@ def test_formfunction2_weno(case, scheme):
 
 
 def test_derivatives_reject_skew_and_clark(case):
-    """The JVP / VJP do not differentiate -skew or -clark, and the dynamic
-    procedure has no Clark term: such contexts are refused before any work
-    is enqueued."""
+    """The JVP / VJP do not differentiate -skew or -clark: such contexts are
+    refused before any work is enqueued."""
     _, spec, coor, o, ucat, ucont = oracle_case(case["name"])
     for opt in ("skew", "clark"):
         ctx = gpu_case(dict(case["cfg"], **{opt: 1}), spec, coor)
@@ -144,10 +143,6 @@ def test_derivatives_reject_skew_and_clark(case):
             assert e.value.rc == -2
             # wall-function walls are refused first, for their own reason
             assert opt in str(e.value) or -1 in case["cfg"]["bctype"]
-        if opt == "clark":
-            with pytest.raises(V.VfsError) as e:
-                ctx.smagorinsky_constant(u, nut, les=2, ti=5)
-            assert e.value.rc == -2 and "Clark" in str(e.value)
         ctx.close()
 
 

@@ -27,11 +27,11 @@ import vfsgpu as V
 LES_EPS = 1e-4
 
 
-def uniform_box(N, all_periodic=True):
+def uniform_box(N, all_periodic=True, clark=0):
     Ni, Nj, Nk = N
     cfg = dict(mx=Ni + 2, my=Nj + 2, mz=Nk + 2, i_periodic=1, j_periodic=1 if all_periodic else 0, k_periodic=1,
                bctype=[7] * 6 if all_periodic else [7, 7, 1, 1, 7, 7], immersed=0, les=1, second_order=0,
-               dpdz_set=1, ren=1000.0, dt=1e-3, st=1.0, mean_pressure_gradient=0.0)
+               dpdz_set=1, ren=1000.0, dt=1e-3, st=1.0, mean_pressure_gradient=0.0, clark=clark)
     coor = V.channel_grid(Ni + 1, Nj + 1, Nk + 1, W.LX, W.LY, W.LZ, 0.0, 0.0)
     return cfg, Oracle(cfg, coor)
 
@@ -51,8 +51,10 @@ def box(f):
     return f
 
 
-def textbook_germano_lilly(u, h, max_cs):
-    """u: (3, Nk, Nj, Ni) periodic; h = (dx, dy, dz).  Returns LM, MM, Cs."""
+def textbook_germano_lilly(u, h, max_cs, clark=False):
+    """u: (3, Nk, Nj, Ni) periodic; h = (dx, dy, dz).  Returns LM, MM, Cs.
+    clark: the mixed model's Germano term, L - N with N = 4 R(u^) - <T(u)>
+    and the Clark stress T_ab = sum_m d_m u_a d_m u_b h_m^2 / 12."""
     axes = (-1, -2, -3)
 
     def grad(v):
@@ -71,6 +73,14 @@ def textbook_germano_lilly(u, h, max_cs):
     Sh, Sha = strain(uf)
     M = np.array([[-(8 * vol) ** (2 / 3) * Sha * Sh[a, b] + vol ** (2 / 3) * box(Sa * S[a, b]) for b in range(3)]
                   for a in range(3)])
+    if clark:
+        def clark_stress(v):
+            D = grad(v)
+            return np.array([[sum(D[a, m] * D[b, m] * h[m] ** 2 for m in range(3)) / 12 for b in range(3)]
+                             for a in range(3)])
+        T = clark_stress(u)
+        N = 4 * clark_stress(uf) - np.array([[box(T[a, b]) for b in range(3)] for a in range(3)])
+        L = L - N
     LM = (L * M).sum(axis=(0, 1))
     MM = (M * M).sum(axis=(0, 1))
     C = 0.5 * box(LM) / (box(MM) + LES_EPS)
@@ -97,11 +107,13 @@ def test_testfilter_closed_form_and_weights():
     assert abs(integrate_testfilter(v, w) - want) < 1e-14 * abs(want) + 1e-15
 
 
+@pytest.mark.parametrize("clark", [0, 1], ids=["smagorinsky", "clark"])
 @pytest.mark.parametrize("N", [(10, 8, 12), (9, 11, 7)])
-def test_reduces_to_textbook_germano_lilly_on_a_uniform_box(N):
+def test_reduces_to_textbook_germano_lilly_on_a_uniform_box(N, clark):
     """Triply periodic uniform box with unequal spacings: the oracle's LM, MM
-    (pass 2) and final Cs equal the Cartesian textbook model."""
-    cfg, o = uniform_box(N)
+    (pass 2) and final Cs equal the Cartesian textbook model, with and
+    without the Clark term (-clark, les.c:648)."""
+    cfg, o = uniform_box(N, clark=clark)
     Ni, Nj, Nk = N
     h = (W.LX / Ni, W.LY / Nj, W.LZ / Nk)
     rng = np.random.default_rng(7)
@@ -119,7 +131,10 @@ def test_reduces_to_textbook_germano_lilly_on_a_uniform_box(N):
     ucat = np.zeros(3 * o.P)
     ucat.reshape(o.mz, o.my, o.mx, 3)[1:-1, 1:-1, 1:-1] = np.moveaxis(u, 0, -1)
     periodic_ghosts(o, ucat, 3)
-    LM_t, MM_t, Cs_t = textbook_germano_lilly(u, h, 0.5)
+    LM_t, MM_t, Cs_t = textbook_germano_lilly(u, h, 0.5, clark=bool(clark))
+    if clark:   # the Clark term moves LM by a share of its size
+        LM_s = textbook_germano_lilly(u, h, 0.5)[0]
+        assert np.abs(LM_t - LM_s).max() > 0.05 * np.abs(LM_s).max()
     grad, sabs, uf = o.dsm_pass1(ucat)
     LM, MM = o.dsm_pass2(ucat, grad, sabs, uf)
     inner = (slice(1, -1),) * 3
@@ -333,6 +348,43 @@ def test_gpu_dynamic_options(name, opt):
     ctx.close()
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["C2", "C1w"])
+@pytest.mark.parametrize("opt", [dict(), dict(ik=1), dict(les=3, ti=150), dict(i_homo=1, k_homo=1)],
+                         ids=["box", "ik", "les3", "ik-homo"])
+def test_gpu_dynamic_clark(name, opt):
+    """-clark: the Clark term of the Germano identity (les.c:648) on the
+    parity cases, with the box and the i-k plane filters."""
+    print(f"[dsm clark] {name} {opt}", flush=True)
+    N = SMALL[name]
+    cfg, spec = W.make_config(name, N)
+    cfg["clark"] = 1
+    coor = W.grid(spec)
+    o = Oracle(cfg, coor)
+    if spec["immersed"]:
+        o.nvert_sphere(*W.SPHERE)
+    o.wall_mask()
+    metrics = {m: o.g[m] for m in ("cent", "icsi", "jeta", "kzet")}
+    ucat, ucont = W.fields(cfg, spec, metrics, o.nvert)
+    o.contra2cart(ucont, ucat)
+    ctx = gpu_case(cfg, spec, coor)
+    assert np.array_equal(ctx.nvert().host(), o.nvert)
+    du = V.DeviceArray.from_host(ucat)
+    prev = np.abs(np.random.default_rng(4).standard_normal(o.P)) * 0.05
+    want = o.smagorinsky_constant(ucat, cs_prev=prev, max_cs=0.3, **opt)
+    # the term changes the model's answer on this field (by 10-70 %; under
+    # -les 3 the relaxation towards the previous Cs keeps 1e-5 to 1e-3 of it)
+    o.c.clark = 0
+    assert rel_err(o.smagorinsky_constant(ucat, cs_prev=prev, max_cs=0.3, **opt), want) > 1e-5
+    cs = V.DeviceArray.from_host(prev)
+    ctx.smagorinsky_constant(du, cs, max_cs=0.3, **opt)
+    got = cs.host()
+    assert np.count_nonzero(want) > want.size // 4
+    assert rel_err(got, want) < TOL
+    assert np.array_equal(got == 0, want == 0)
+    ctx.close()
+
+
 # ---------------------------------------------------------------------------
 # the other options of the routine: -testfilter_ik, -i/j/k_homo_filter, -les 3
 # ---------------------------------------------------------------------------
@@ -359,11 +411,13 @@ def simpson_xz(f):
     return f
 
 
-def test_testfilter_ik_reduces_to_the_textbook_2d_filter():
+@pytest.mark.parametrize("clark", [0, 1], ids=["smagorinsky", "clark"])
+def test_testfilter_ik_reduces_to_the_textbook_2d_filter(clark):
     """Triply periodic uniform box: with -testfilter_ik the model is Germano-
-    Lilly with the 2-D Simpson filter in x-z and test width 5^(1/3) Delta."""
+    Lilly with the 2-D Simpson filter in x-z and test width 5^(1/3) Delta,
+    and the Clark term's <T> takes the same filter."""
     N = (10, 8, 12)
-    cfg, o = uniform_box(N)
+    cfg, o = uniform_box(N, clark=clark)
     h = (W.LX / N[0], W.LY / N[1], W.LZ / N[2])
     rng = np.random.default_rng(17)
     u = rng.standard_normal((3, N[2], N[1], N[0]))
@@ -384,6 +438,14 @@ def test_testfilter_ik_reduces_to_the_textbook_2d_filter():
     Sh, Sha = strain(uf)
     M = np.array([[-5 ** (2 / 3) * vol ** (2 / 3) * Sha * Sh[a, b] + vol ** (2 / 3) * simpson_xz(Sa * S[a, b])
                    for b in range(3)] for a in range(3)])
+    if clark:
+        def clark_stress(v):
+            D = np.array([[(np.roll(v[a], -1, axes[m]) - np.roll(v[a], 1, axes[m])) / (2 * h[m]) for m in range(3)]
+                          for a in range(3)])
+            return np.array([[sum(D[a, m] * D[b, m] * h[m] ** 2 for m in range(3)) / 12 for b in range(3)]
+                             for a in range(3)])
+        T = clark_stress(u)
+        L = L - (4 * clark_stress(uf) - np.array([[simpson_xz(T[a, b]) for b in range(3)] for a in range(3)]))
     LM_t, MM_t = (L * M).sum(axis=(0, 1)), (M * M).sum(axis=(0, 1))
     grad, sabs, ufo = o.dsm_pass1(ucat, ik=1)
     LM, MM = o.dsm_pass2(ucat, grad, sabs, ufo, ik=1)

@@ -1,7 +1,7 @@
-// The dynamic Smagorinsky procedure: Compute_Smagorinsky_Constant_1 with
-// -les 2 (les.c:67-1133) and the default filter options (3-D box test
-// filter, testfilter_ik 0; no homogeneous-direction averaging; no Clark
-// term), and -les 1 / the fresh-start rule of the same routine.
+// The dynamic Smagorinsky procedure: Compute_Smagorinsky_Constant_1
+// (les.c:67-1133) with its options: -les 1 / 2 / 3 and the fresh-start rule,
+// the homogeneous-direction averages, -testfilter_ik and, on a -clark
+// context, the Clark model's Germano term.
 //
 // Five kernels over the padded grid, each followed by the periodic ghost
 // copies the reference makes at that point:
@@ -9,14 +9,16 @@
 //   1. k_dsm_grad   velocity gradient at the non-solid interior cells,
 //                   stored as the six products S_ab |S| the test filter of
 //                   the second pass reads (the reference keeps the nine
-//                   gradients and |S| and forms these products there), and
-//                   the test-filtered velocity;
+//                   gradients and |S| and forms these products there), the
+//                   test-filtered velocity and, under -clark, the nine
+//                   gradients;
 //   2. k_dsm_lm     the Germano identity in the reference's contravariant
-//                   form: LM = L_ba M_aq G_bq, MM = M_nm M_nl G_ml;
+//                   form: LM = L_ba M_aq G_bq (- N_ab M_aq G_bq under
+//                   -clark), MM = M_nm M_nl G_ml;
 //   3. k_dsm_cs     Cs = max(0.5 <LM> / (<MM> + 1e-4), 0) and the clipping;
 //   4. k_dsm_smooth the box filter of the clipped Cs at the fluid cells.
-// Scratch: 19 doubles a point, allocated on first use and kept by the
-// context.  Built without FMA contraction; every sum keeps the reference's
+// Scratch: 19 doubles a point (28 under -clark), allocated on first use and
+// kept by the context.  Built without FMA contraction; every sum keeps the reference's
 // order, so results differ from the oracle only through pow(x, 1/3).
 #include "vfs_internal.h"
 #include "vfs_stencil.h"
@@ -192,12 +194,13 @@ __device__ __forceinline__ int ss_plane(int a, int b)
     return lo == 0 ? hi : (lo == 1 ? 2 + hi : 5);
 }
 
-// les.c:188-257 (first pass)
-template <bool IK>
+// les.c:188-257 (first pass).  CL (-clark): the nine gradient components
+// too, as planes gr + (3 a + b) P (the reference's Ax, Ay, Az).
+template <bool IK, bool CL>
 __global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *__restrict__ nvert,
                                                   const double *__restrict__ ucat, const double *__restrict__ uq,
                                                   const double *__restrict__ wt, double *__restrict__ ss,
-                                                  double *__restrict__ uf)
+                                                  double *__restrict__ uf, double *__restrict__ gr)
 {
     DSM_PREAMBLE;
     double *pl[6];
@@ -208,12 +211,18 @@ __global__ void __launch_bounds__(256) k_dsm_grad(Dims d, Geom g, const double *
         for (int q = 0; q < 6; q++) st32(pl[q], t, 0.0);
 #pragma unroll
         for (int q = 0; q < 3; q++) st32(uf, 3 * t + q, 0.0);
+        if (CL)
+#pragma unroll
+            for (int q = 0; q < 9; q++) st32(gr + (size_t)q * d.P, t, 0.0);
         return;
     }
     double dc[3], de[3], dz[3], D[3][3], S[3][3];
     du_center(d, i, j, k, ucat, nvert, dc, de, dz);
     du_dxyz(g, t, dc, de, dz, D);
     const double sabs = strain(D, S);
+    if (CL)
+#pragma unroll
+        for (int q = 0; q < 9; q++) st32(gr + (size_t)q * d.P, t, D[q / 3][q % 3]);
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
@@ -261,16 +270,40 @@ __device__ __forceinline__ void covariant_metrics(const double gg[3][3], double 
 // <S|S|> plane b; wave 3: planes 3-5 and the weight sums), so a wave's role
 // is uniform and its loads run along i.  The raw octant totals meet in LDS,
 // and wave 0 finishes each point: the divisions by the weight total, the
-// Germano terms and the contractions.
+// Germano terms and the contractions.  CL (-clark): two more waves filter
+// the Clark stresses T_ab of the box points (with the centre cell's length
+// scales, les.c:339-340, 410-420), three planes each, and wave 0 adds the
+// term - N_ab M_aq G_bq, N = 4 R - <T> (les.c:509-548, 648).
 #define DSM_LM_PTS 64
-template <bool IK>
-__global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__restrict__ nvert,
-                                                const double *__restrict__ uq, const double *__restrict__ Ub,
-                                                const double *__restrict__ wt, const double *__restrict__ ss,
-                                                const double *__restrict__ uf, double *__restrict__ LM,
-                                                double *__restrict__ MM)
+
+// the Clark stress T_ab / 12 of point n from the gradient planes
+__device__ __forceinline__ double clark_t(const double *__restrict__ gr, size_t Pz, unsigned n, int a, int b,
+                                          const double dl2[3])
 {
-    __shared__ double sh[17][DSM_LM_PTS];   // 0-8 <U_b u_a> (3b+a), 9-14 <S|S|> planes, 15 w total, 16 sum_weight
+    const double *ga = gr + 3 * a * Pz, *gb = gr + 3 * b * Pz;
+    return (ld32(ga, n) * ld32(gb, n) * dl2[0] + ld32(ga + Pz, n) * ld32(gb + Pz, n) * dl2[1] +
+            ld32(ga + 2 * Pz, n) * ld32(gb + 2 * Pz, n) * dl2[2]) / 12.;
+}
+
+// the squared length scales of cell t (Calculate_dxdydz, les.c:339-340)
+__device__ __forceinline__ void clark_lengths(const Geom &g, unsigned t, double dl2[3])
+{
+    double cs[3], ce[3], cz[3], lx, ly, lz;
+#pragma unroll
+    for (int q = 0; q < 3; q++) cs[q] = ld32(g.csi, 3 * t + q), ce[q] = ld32(g.eta, 3 * t + q), cz[q] = ld32(g.zet, 3 * t + q);
+    dxdydz(ld32(g.aj, t), cs, ce, cz, lx, ly, lz);
+    dl2[0] = lx * lx, dl2[1] = ly * ly, dl2[2] = lz * lz;
+}
+
+template <bool IK, bool CL>
+__global__ void __launch_bounds__(CL ? 384 : 256) k_dsm_lm(Dims d, Geom g, const double *__restrict__ nvert,
+                                                          const double *__restrict__ uq, const double *__restrict__ Ub,
+                                                          const double *__restrict__ wt, const double *__restrict__ ss,
+                                                          const double *__restrict__ uf, const double *__restrict__ gr,
+                                                          double *__restrict__ LM, double *__restrict__ MM)
+{
+    // 0-8 <U_b u_a> (3b+a), 9-14 <S|S|> planes, 15 w total, 16 sum_weight, 17-22 <T> planes (-clark)
+    __shared__ double sh[CL ? 23 : 17][DSM_LM_PTS];
     const unsigned lane = threadIdx.x & 63, role = threadIdx.x >> 6;
     const unsigned t = blockIdx.x * DSM_LM_PTS + lane;
     const unsigned mx = (unsigned)d.mx, my = (unsigned)d.my;
@@ -288,7 +321,7 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
                     simpson_ik([&](int R, int P) { return ld32(Upl, at(R, P)) * ld32(ua, at(R, P)); });
             }
             sh[9 + role][lane] = simpson_ik([&](int R, int P) { return ld32(sp, at(R, P)); });
-        } else {
+        } else if (role == 3) {
 #pragma unroll
             for (int q = 3; q < 6; q++) {
                 const double *pq = ss + q * Pz;
@@ -296,6 +329,16 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
             }
             sh[15][lane] = 1.0;
             sh[16][lane] = 0.0;
+        } else if (CL) {   // <T_ab>: (0,0) (0,1) (0,2) on wave 4, (1,1) (1,2) (2,2) on wave 5
+            double dl2[3];
+            clark_lengths(g, t, dl2);
+            const int a = role == 4 ? 0 : 1;
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                if (b < a) continue;
+                sh[17 + ss_plane(a, b)][lane] = simpson_ik([&](int R, int P) { return clark_t(gr, Pz, at(R, P), a, b, dl2); });
+            }
+            if (role == 5) sh[17 + ss_plane(2, 2)][lane] = simpson_ik([&](int R, int P) { return clark_t(gr, Pz, at(R, P), 2, 2, dl2); });
         }
     } else if (act) {
         const size_t Pz = (size_t)d.P;
@@ -315,7 +358,7 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
 #pragma unroll
             for (int a = 0; a < 3; a++) sh[3 * role + a][lane] = oct_total(v[a]);
             sh[9 + role][lane] = oct_total(v[3]);
-        } else {          // S|S| planes 3-5 and the weight sums
+        } else if (role == 3) {   // S|S| planes 3-5 and the weight sums
             const double *p3 = ss + 3 * Pz, *p4 = p3 + Pz, *p5 = p4 + Pz;
             Oct v[3], w;
             OCT_FOR
@@ -342,6 +385,33 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
                         sum_weight += ld32(wt, t + (unsigned)slot_off(d, r, q, pp)) * cf;
                     }
             sh[16][lane] = sum_weight;
+        } else if (CL) {   // <T_ab>: (0,0) (0,1) (0,2) on wave 4, (1,1) (1,2) (2,2) on wave 5
+            double dl2[3];
+            clark_lengths(g, t, dl2);
+            Oct v[3];
+            if (role == 4) {
+                OCT_FOR
+                {
+                    const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+                    const double wn = ld32(wt, n);
+#pragma unroll
+                    for (int b = 0; b < 3; b++) oct_add(v[b], R, Q, P, clark_t(gr, Pz, n, 0, b, dl2) * wn);
+                }
+#pragma unroll
+                for (int b = 0; b < 3; b++) sh[17 + ss_plane(0, b)][lane] = oct_total(v[b]);
+            } else {
+                OCT_FOR
+                {
+                    const unsigned n = t + (unsigned)slot_off(d, R, Q, P);
+                    const double wn = ld32(wt, n);
+                    oct_add(v[0], R, Q, P, clark_t(gr, Pz, n, 1, 1, dl2) * wn);
+                    oct_add(v[1], R, Q, P, clark_t(gr, Pz, n, 1, 2, dl2) * wn);
+                    oct_add(v[2], R, Q, P, clark_t(gr, Pz, n, 2, 2, dl2) * wn);
+                }
+                sh[17 + ss_plane(1, 1)][lane] = oct_total(v[0]);
+                sh[17 + ss_plane(1, 2)][lane] = oct_total(v[1]);
+                sh[17 + ss_plane(2, 2)][lane] = oct_total(v[2]);
+            }
         }
     }
     __syncthreads();
@@ -392,13 +462,32 @@ __global__ void __launch_bounds__(256) k_dsm_lm(Dims d, Geom g, const double *__
     for (int a = 0; a < 3; a++)
 #pragma unroll
         for (int q = 0; q < 3; q++) M[a][q] = Mc[a][0] * gg[q][0] + Mc[a][1] * gg[q][1] + Mc[a][2] * gg[q][2];
+    double N[3][3];
+    if (CL) {   // N = 4 R - <T> with R from the filtered velocity's gradient (les.c:509-548)
+        double dl2[3], Nc[3][3];
+        clark_lengths(g, t, dl2);
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) {
+                const double Rab = (D[a][0] * D[b][0] * dl2[0] + D[a][1] * D[b][1] * dl2[1] + D[a][2] * D[b][2] * dl2[2]) / 12.;
+                Nc[a][b] = 4.0 * Rab - sh[17 + ss_plane(a, b)][lane] / ws;
+            }
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int q = 0; q < 3; q++) N[a][q] = Nc[a][0] * gg[q][0] + Nc[a][1] * gg[q][1] + Nc[a][2] * gg[q][2];
+    }
     double num = 0, denom = 0;
 #pragma unroll
     for (int q = 0; q < 3; q++)
 #pragma unroll
         for (int a = 0; a < 3; a++)
 #pragma unroll
-            for (int b = 0; b < 3; b++) num += L[b][a] * M[a][q] * G[b][q];
+            for (int b = 0; b < 3; b++) {
+                num += L[b][a] * M[a][q] * G[b][q];
+                if (CL) num -= N[a][b] * M[a][q] * G[b][q];   // les.c:648
+            }
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
@@ -593,7 +682,6 @@ extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, const vf
     VFS_REQUIRE(o, "options required");
     VFS_REQUIRE(o->les >= 1 && o->les <= 3, "les must be 1 (constant Cs), 2 (dynamic) or 3 (dynamic, relaxed)");
     VFS_REQUIRE(ucat && cs, "ucat and cs are required");
-    VFS_REQUIRE(!c->d.clark || o->les == 1, "the dynamic procedure does not cover the Clark term (-clark, les.c:648)");
     const int nh = (o->i_homo_filter != 0) + (o->j_homo_filter != 0) + (o->k_homo_filter != 0);
     VFS_REQUIRE(nh <= 1 || (nh == 2 && o->i_homo_filter && o->k_homo_filter),
                 "homogeneous filters: one direction, or i and k together");
@@ -605,21 +693,31 @@ extern "C" int vfs_smagorinsky_constant(vfs_ctx *c, const double *ucat, const vf
         VFS_CHECK_LAUNCH();
         return 0;
     }
-    if (!c->dsm) VFS_CHECK_HIP(hipMalloc(&c->dsm, (size_t)19 * d.P * sizeof(double)));
+    // 19 planes, and the 9 gradient planes of -clark
+    const bool cl = d.clark != 0;
+    if (!c->dsm) VFS_CHECK_HIP(hipMalloc(&c->dsm, (size_t)(cl ? 28 : 19) * d.P * sizeof(double)));
     const size_t P = (size_t)d.P;
     double *ss = c->dsm, *uf = ss + 6 * P, *LM = uf + 3 * P, *MM = LM + P, *cst = MM + P, *wt = cst + P;
-    double *uq = wt + P, *Ub = uq + 3 * P;
+    double *uq = wt + P, *Ub = uq + 3 * P, *gr = cl ? Ub + 3 * P : nullptr;
     const bool ik = o->testfilter_ik != 0, les3 = o->les == 3, homo = nh > 0;
     const dim3 blk(256), grd(grid_for(d.P, 256));
     hipLaunchKernelGGL(k_dsm_point, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, wt, uq, Ub);
     VFS_CHECK_LAUNCH();
-    if (ik) hipLaunchKernelGGL(k_dsm_grad<true>, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
-    else hipLaunchKernelGGL(k_dsm_grad<false>, grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf);
+#define DSM_GRAD(IKV, CLV) \
+    hipLaunchKernelGGL((k_dsm_grad<IKV, CLV>), grd, blk, 0, c->stream, d, c->g, c->nvert, ucat, uq, wt, ss, uf, gr)
+    if (ik) { if (cl) DSM_GRAD(true, true); else DSM_GRAD(true, false); }
+    else { if (cl) DSM_GRAD(false, true); else DSM_GRAD(false, false); }
+#undef DSM_GRAD
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, ss, 6) || vfs_periodic_vec(c, uf)) return -1;
+    if (cl && periodic_planes(c, gr, 9)) return -1;
     const dim3 glm(grid_for(d.P, DSM_LM_PTS));
-    if (ik) hipLaunchKernelGGL(k_dsm_lm<true>, glm, dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub, wt, ss, uf, LM, MM);
-    else hipLaunchKernelGGL(k_dsm_lm<false>, glm, dim3(256), 0, c->stream, d, c->g, c->nvert, uq, Ub, wt, ss, uf, LM, MM);
+#define DSM_LM(IKV, CLV) \
+    hipLaunchKernelGGL((k_dsm_lm<IKV, CLV>), glm, dim3(CLV ? 384 : 256), 0, c->stream, d, c->g, c->nvert, uq, Ub, wt, \
+                       ss, uf, gr, LM, MM)
+    if (ik) { if (cl) DSM_LM(true, true); else DSM_LM(true, false); }
+    else { if (cl) DSM_LM(false, true); else DSM_LM(false, false); }
+#undef DSM_LM
     VFS_CHECK_LAUNCH();
     if (periodic_planes(c, LM, 2)) return -1;   // LM and MM are adjacent planes
     const double max_cs = o->max_cs;
