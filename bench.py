@@ -571,6 +571,8 @@ def main():
     ap.add_argument("--no-ib", action="store_true", help="skip the CURVIB reconstruction leg")
     ap.add_argument("--no-turbine-share", action="store_true",
                     help="skip the 768x384x48 open-boundary leg (one GPU's share of the turbine-wake case)")
+    ap.add_argument("--share-only", action="store_true",
+                    help="only the 768x384x48 open-boundary leg (--steps steps after --warmup), for kernel traces")
     ap.add_argument("--c5-steps", type=int, default=0,
                     help="also run the full 768x384x384 turbine-wake case for this many steps (about 200 GB of HBM)")
     args = ap.parse_args()
@@ -579,6 +581,10 @@ def main():
               f"(the multi-GPU k-slab decomposition is a design, not an implementation)", file=sys.stderr)
         sys.exit(2)
 
+    if args.share_only:
+        log("[bench] turbine-wake share only")
+        print(json.dumps({"turbine_share": turbine_share_leg(args.steps, args.warmup)}), flush=True)
+        return
     cfg, spec, ctx, ucat, ucont, nunk = setup(args.config, poisson=not args.rhs_only)
     Ni, Nj, Nk = spec["N"]
     cells = Ni * Nj * Nk
