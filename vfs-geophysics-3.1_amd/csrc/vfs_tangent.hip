@@ -75,7 +75,7 @@ static int primal_state(vfs_ctx *c, const double *U, const double *ucat_in, doub
 // ---------------------------------------------------------------------------
 // JVP
 // ---------------------------------------------------------------------------
-__global__ void k_ff2_faces_jvp(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
+__global__ void __launch_bounds__(256) k_ff2_faces_jvp(Dims d, Geom g, const double *__restrict__ nvert, const double *__restrict__ ucont,
                                 const double *__restrict__ ucat, const double *__restrict__ tucont,
                                 const double *__restrict__ tucat, const double *__restrict__ nu_t,
                                 double *__restrict__ face)
