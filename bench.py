@@ -76,6 +76,20 @@ def setup(name, N=None, poisson=True):
     return cfg, spec, ctx, ucat, ucont, n_unknowns
 
 
+def time_poisson_setup(ctx, reps):
+    """Median wall time of vfs_poisson_setup: the unknown numbering, the
+    19-point coefficients and the multigrid hierarchy, built again from the
+    current mask (what a step pays when a moving body changes the mask)."""
+    ts = []
+    for _ in range(reps):
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.poisson_setup()
+        ctx.sync()
+        ts.append(time.perf_counter() - t0)
+    return round(float(np.median(ts)) * 1e3, 2)
+
+
 class State:
     def __init__(self, ctx, ucat, ucont):
         import vfsgpu as V
@@ -625,6 +639,8 @@ def main():
     bytes_per_cell = RHS_BYTES_PER_CELL[cfg["les"]]
     achieved = bytes_per_cell * cells / (rhs_ms * 1e-3) / 1e9
     log(f"[bench] Formfunction_2 {rhs_ms:.3f} ms -> {gcells:.2f} Gcell/s, {achieved:.0f} GB/s")
+    setup_ms = time_poisson_setup(ctx, 3)
+    log(f"[bench] Poisson setup {setup_ms} ms")
 
     implicit = None
     if not args.no_implicit:
@@ -689,7 +705,8 @@ def main():
                     "fp64_iters": res["fp64"]["iters"], "mixed_iters": res["mixed"]["iters"],
                     "max_relres": max(res["fp64"]["relres"], res["mixed"]["relres"]),
                     "max_div_after_projection": max(res["fp64"]["maxdiv"], res["mixed"]["maxdiv"]),
-                    "phi_rel_diff_mixed_vs_fp64": acc["phi_rel_diff_l2"]},
+                    "phi_rel_diff_mixed_vs_fp64": acc["phi_rel_diff_l2"],
+                    "setup_ms": setup_ms},
         "accuracy_mixed_vs_fp64": acc,
         "step_ms": {"fp64": round(res["fp64"]["step_ms"], 3), "mixed": round(res["mixed"]["step_ms"], 3),
                     "speedup_mixed_vs_fp64": round(res["fp64"]["step_ms"] / res["mixed"]["step_ms"], 3)},

@@ -317,6 +317,41 @@ def test_poisson_solve_converged_solution(case, poisson, precision):
     assert rel_err(dphi.host(), phi_o) < 1e-7
 
 
+@pytest.mark.parametrize("block", [False, True])
+def test_poisson_single_level_dense_inverse(block):
+    """A grid of at most 1024 unknowns keeps one multigrid level, which the
+    preconditioner solves with the dense inverse built at set-up (Gauss-Jordan
+    with partial pivoting on the device; with the marked block, 27 cells are
+    not unknowns).  With that exact preconditioner FP64 GMRES converges in one
+    iteration (two at most); a wrong inverse would need many."""
+    say(f"[case] single-level C1 N=(10, 8, 9) block={block}")
+    cfg, spec, coor, o, ucat, ucont = oracle_case("C1", N=(10, 8, 9))
+    ctx = gpu_case(cfg, spec, coor)
+    if block:
+        nv = np.zeros((o.mz, o.my, o.mx))
+        nv[3:6, 2:5, 4:7] = 1.0
+        o.nvert[:] = nv.ravel()
+        o.wall_mask()
+        ctx.set_nvert(V.DeviceArray.from_host(nv.ravel()))
+    n = ctx.poisson_setup()
+    o.setup_gid()
+    o.poisson_lhs()
+    assert n == o.n and n <= 1024
+    uc = ucont.copy()
+    o.volume_flux(uc)
+    b = o.sca()
+    o.poisson_rhs(uc, b)
+    phi_o = o.sca()
+    o.poisson_solve(b, phi_o, rtol=1e-13)
+    for precision, max_it, tol in ((0, 2, 1e-12), (1, 10, 1e-10)):
+        dphi = ctx.sca()
+        it, rel = ctx.poisson_solve(V.DeviceArray.from_host(b), dphi, rtol=tol, precision=precision)
+        say(f"  precision={precision} iters={it} relres={rel:.2e}")
+        assert it <= max_it and rel <= tol
+        assert rel_err(dphi.host(), phi_o) < 1e-7
+    ctx.close()
+
+
 def test_projection_update_divergence(case, poisson):
     o, ctx = case["o"], case["ctx"]
     cfg = case["cfg"]

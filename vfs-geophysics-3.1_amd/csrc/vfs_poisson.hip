@@ -53,8 +53,8 @@ __device__ __forceinline__ bool is_unknown(const Dims &d, const double *nvert, i
 }
 
 // One fluid row of PoissonLHSNew (poisson.c:1160-1555), literal order.
-__device__ void stencil_row(const Dims &d, const Geom &g, const double *__restrict__ nvert, int i, int j, int k,
-                            double vol[19])
+__device__ __forceinline__ void stencil_row(const Dims &d, const Geom &g, const double *__restrict__ nvert, int i,
+                                            int j, int k, double vol[19])
 {
     const int ip = d.ip, jp = d.jp, kp = d.kp, mx = d.mx, my = d.my, mz = d.mz;
     const double th = 0.1, r = 1.0;
@@ -82,7 +82,8 @@ __global__ void k_poisson_coef(Dims d, Geom g, const double *__restrict__ nvert,
         else if (!(nvert[t] > 0.1)) {
             stencil_row(d, g, nvert, i, j, k, vol);
             flag = 1;
-            // drop rule (poisson.c:1604-1608)
+            // drop rule (poisson.c:1604-1608); unrolled, so the row stays in registers
+#pragma unroll
             for (int m = 1; m < 19; m++) {
                 bool keep = fabs(vol[m]) > 1.e-10 &&
                             is_unknown(d, nvert, k + c_slot[m][2], j + c_slot[m][1], i + c_slot[m][0]);
@@ -90,6 +91,7 @@ __global__ void k_poisson_coef(Dims d, Geom g, const double *__restrict__ nvert,
             }
         }
     }
+#pragma unroll
     for (int m = 0; m < 19; m++) {
         coef[(long long)m * d.P + t] = vol[m];
         coef32[(long long)m * d.P + t] = (float)vol[m];

@@ -833,62 +833,115 @@ static int alloc_level(vfs_ctx *c, MGLevel &l)
     return 0;
 }
 
-// Gauss-Jordan inverse with partial pivoting (host, n <= a few hundred)
-static int invert_dense(std::vector<double> &a, int n)
+// Gauss-Jordan inverse with partial pivoting of the dense coarsest operator,
+// on the augmented matrix [M | I] (row-major, n rows of 2n), one column at a
+// time: k_gj_pivot then k_gj_update.  Columns left of the pivot column are
+// zero in every row not yet used as a pivot, so both kernels start there.
+//
+// Pivot of column col (one workgroup): the largest |a[r][col]|, r >= col,
+// the lowest such r on ties (what a sequential scan picks), is swapped into
+// row col, which is then scaled by 1/pivot; f[r] = a[r][col] is kept for
+// the update (f[col] = 0).  A zero or NaN pivot sets *bad, after which every
+// later launch returns at once.
+__global__ void __launch_bounds__(1024) k_gj_pivot(double *__restrict__ a, int n, int col, double *__restrict__ f,
+                                                   int *__restrict__ bad)
 {
-    std::vector<double> inv((size_t)n * n, 0.0);
-    for (int q = 0; q < n; q++) inv[(size_t)q * n + q] = 1.0;
-    for (int col = 0; col < n; col++) {
-        int piv = col;
-        double best = fabs(a[(size_t)col * n + col]);
-        for (int r = col + 1; r < n; r++)
-            if (fabs(a[(size_t)r * n + col]) > best) best = fabs(a[(size_t)r * n + col]), piv = r;
-        if (best == 0.0) return -1;
-        if (piv != col)
-            for (int q = 0; q < n; q++) {
-                std::swap(a[(size_t)piv * n + q], a[(size_t)col * n + q]);
-                std::swap(inv[(size_t)piv * n + q], inv[(size_t)col * n + q]);
-            }
-        double s = 1.0 / a[(size_t)col * n + col];
-        for (int q = col; q < n; q++) a[(size_t)col * n + q] *= s;
-        for (int q = 0; q < n; q++) inv[(size_t)col * n + q] *= s;
-        for (int r = 0; r < n; r++) {
-            if (r == col) continue;
-            double f = a[(size_t)r * n + col];
-            if (f == 0.0) continue;
-            for (int q = col; q < n; q++) a[(size_t)r * n + q] -= f * a[(size_t)col * n + q];
-            for (int q = 0; q < n; q++) inv[(size_t)r * n + q] -= f * inv[(size_t)col * n + q];
-        }
+    __shared__ double sv[16];
+    __shared__ int si[16];
+    __shared__ int s_piv, s_bad;
+    __shared__ double s_sc;
+    const long long w = 2LL * n;
+    if (threadIdx.x == 0) s_bad = *bad;
+    __syncthreads();
+    if (s_bad) return;
+    double best = -1.0;
+    int bi = n;
+    for (int r = col + (int)threadIdx.x; r < n; r += 1024) {
+        const double v = fabs(a[(long long)r * w + col]);
+        if (v > best) { best = v; bi = r; }
     }
-    a.swap(inv);
-    return 0;
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ov = __shfl_xor(best, o, 64);
+        const int oi = __shfl_xor(bi, o, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+    }
+    if ((threadIdx.x & 63) == 0) { sv[threadIdx.x >> 6] = best; si[threadIdx.x >> 6] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = sv[0];
+        int p = si[0];
+        for (int q = 1; q < 16; q++)
+            if (sv[q] > b || (sv[q] == b && si[q] < p)) { b = sv[q]; p = si[q]; }
+        s_piv = p;
+        s_bad = !(b > 0.0);
+        if (s_bad) *bad = 1;
+        else s_sc = 1.0 / a[(long long)p * w + col];   // read before any thread swaps
+    }
+    __syncthreads();
+    if (s_bad) return;
+    const int p = s_piv;
+    double *rc = a + (long long)col * w, *rp = a + (long long)p * w;
+    const double sc = s_sc;
+    for (long long q = col + threadIdx.x; q < w; q += 1024) {
+        const double x = rp[q];
+        if (p != col) rp[q] = rc[q];
+        rc[q] = x * sc;
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += 1024) f[r] = r == col ? 0.0 : a[(long long)r * w + col];
 }
 
-// mean |coefficient| of the i and of the k neighbours of a level (one
-// workgroup): the couplings that decide whether k is coarsened
-__global__ void __launch_bounds__(256) k_axis_strength(LV L, const double *__restrict__ A, double *out)
+// a[r][q] -= f[r] * a[col][q] for q >= col; rows with f[r] == 0 unchanged
+// grid: (ceil((2n - col) / 256), n)
+__global__ void __launch_bounds__(256) k_gj_update(double *__restrict__ a, int n, int col,
+                                                   const double *__restrict__ f, const int *__restrict__ bad)
 {
-    __shared__ double si[256], sk[256];
+    if (*bad) return;
+    const int r = blockIdx.y;
+    const long long w = 2LL * n, q = col + (long long)blockIdx.x * 256 + threadIdx.x;
+    const double fr = f[r];
+    if (q >= w || fr == 0.0) return;
+    a[(long long)r * w + q] -= fr * a[(long long)col * w + q];
+}
+
+// The inverse (right half of the augmented matrix) in the layouts of the
+// coarse solves: column-major nc x nc (cinv, rows of a solve one per thread)
+// and scattered to all ncell cells of the level (cinvr, row-major, zero
+// elsewhere), FP64 and FP32.
+__global__ void __launch_bounds__(256) k_gj_out(const double *__restrict__ a, int n, const long long *__restrict__ cidx,
+                                                long long ncell, double *__restrict__ cinv64, float *__restrict__ cinv32,
+                                                double *__restrict__ cinvr64, float *__restrict__ cinvr32)
+{
+    const long long t = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long long)n * n) return;
+    const int r = (int)(t / n), q = (int)(t % n);
+    const double v = a[(long long)r * 2 * n + n + q];
+    cinv64[(long long)q * n + r] = v;
+    cinv32[(long long)q * n + r] = (float)v;
+    const long long o = cidx[r] * ncell + cidx[q];
+    cinvr64[o] = v;
+    cinvr32[o] = (float)v;
+}
+
+// sums of |coefficient| of the i and of the k neighbours of a level: the
+// couplings that decide whether k is coarsened.  One partial per workgroup
+// at part[b] (i) and part[VFS_RED_BLOCKS + b] (k); k_reduce_finish adds them
+// in a fixed order, so the decision does not depend on the schedule.
+__global__ void __launch_bounds__(256) k_axis_strength(LV L, const double *__restrict__ A, double *part)
+{
+    __shared__ double sh[4];
     const int i1 = L.nslot == 19 ? 1 : 14, i2 = L.nslot == 19 ? 2 : 12;
     const int k1 = L.nslot == 19 ? 5 : 22, k2 = L.nslot == 19 ? 6 : 4;
     double a = 0, b = 0;
-    for (long long t = threadIdx.x; t < L.n; t += 256) {
+    for (long long t = (long long)blockIdx.x * 256 + threadIdx.x; t < L.n; t += (long long)gridDim.x * 256) {
         a += fabs(A[(long long)i1 * L.n + t]) + fabs(A[(long long)i2 * L.n + t]);
         b += fabs(A[(long long)k1 * L.n + t]) + fabs(A[(long long)k2 * L.n + t]);
     }
-    si[threadIdx.x] = a;
-    sk[threadIdx.x] = b;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            si[threadIdx.x] += si[threadIdx.x + w];
-            sk[threadIdx.x] += sk[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
+    a = block_sum<256>(a, sh);
+    b = block_sum<256>(b, sh);
     if (threadIdx.x == 0) {
-        out[0] = si[0];
-        out[1] = sk[0];
+        part[blockIdx.x] = a;
+        part[VFS_RED_BLOCKS + blockIdx.x] = b;
     }
 }
 
@@ -924,11 +977,13 @@ int vfs_mg_setup(vfs_ctx *c)
         // where 2x2x2 aggregates would join cells the smoother leaves rough
         int keep_k = f.nz <= 1;
         if (!keep_k && f.nx > 1) {
-            hipLaunchKernelGGL(k_axis_strength, dim3(1), dim3(256), 0, c->stream, lv_of(f), f.A64, c->red);
+            const int nb = (int)std::min<long long>(VFS_RED_BLOCKS, grid_for(f.n, 256));
+            hipLaunchKernelGGL(k_axis_strength, dim3(nb), dim3(256), 0, c->stream, lv_of(f), f.A64, c->red);
             VFS_CHECK_LAUNCH();
+            double *res = c->red + (long long)VFS_RED_SLOTS * VFS_RED_BLOCKS;
+            if (vfs_reduce_finish_slots(c, 0, nb, 2, 0, res)) return -1;
             double h[2];
-            VFS_CHECK_HIP(hipMemcpyAsync(h, c->red, sizeof h, hipMemcpyDeviceToHost, c->stream));
-            VFS_CHECK_HIP(hipStreamSynchronize(c->stream));
+            if (vfs_read_scalars(c, res, h, 2)) return -1;
             keep_k = h[1] < VFS_MG_KEEPK * h[0];
         }
         cl.sk = keep_k ? 1 : 2;
@@ -990,7 +1045,6 @@ int vfs_mg_setup(vfs_ctx *c)
             M[(size_t)r * nc + pos[nbq]] += a;
         }
     }
-    if (invert_dense(M, nc)) { vfs_set_error("singular coarsest Poisson operator"); return -1; }
     ps->nc = nc;
     // the first level (below the finest) small enough for the one-workgroup cycle
     ps->lsmall = (int)ps->lv.size();
@@ -1001,28 +1055,49 @@ int vfs_mg_setup(vfs_ctx *c)
     DMALLOC(ps->cidx, (size_t)nc * sizeof(long long));
     DMALLOC(ps->cinvr64, (size_t)cl.n * cl.n * sizeof(double));
     DMALLOC(ps->cinvr32, (size_t)cl.n * cl.n * sizeof(float));
-    VFS_CHECK_HIP(hipStreamSynchronize(c->stream));   // the zero fills before the uploads
     {
-        // the inverse scattered to all cells of the level, row-major
-        std::vector<double> F((size_t)cl.n * cl.n, 0.0);
-        for (int r = 0; r < nc; r++)
-            for (int q = 0; q < nc; q++) F[(size_t)list[r] * cl.n + list[q]] = M[(size_t)r * nc + q];
-        std::vector<float> F32(F.begin(), F.end());
-        VFS_CHECK_HIP(hipMemcpy(ps->cinvr64, F.data(), F.size() * sizeof(double), hipMemcpyHostToDevice));
-        VFS_CHECK_HIP(hipMemcpy(ps->cinvr32, F32.data(), F32.size() * sizeof(float), hipMemcpyHostToDevice));
+        // (M | I) inverted on the device (2 nc launches of a few microseconds)
+        std::vector<double> aug((size_t)nc * 2 * nc, 0.0);
+        for (int r = 0; r < nc; r++) {
+            std::memcpy(&aug[(size_t)r * 2 * nc], &M[(size_t)r * nc], (size_t)nc * sizeof(double));
+            aug[(size_t)r * 2 * nc + nc + r] = 1.0;
+        }
+        double *da = nullptr, *df = nullptr;
+        int *dbad = nullptr;
+        auto release = [&]() { hipFree(da); hipFree(df); hipFree(dbad); };
+        if (hipMalloc((void **)&da, aug.size() * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&df, (size_t)nc * sizeof(double)) != hipSuccess ||
+            hipMalloc((void **)&dbad, sizeof(int)) != hipSuccess) {
+            release();
+            vfs_set_error("hipMalloc failed in the Poisson set-up");
+            return -1;
+        }
+        int bad = 0;
+        if (hipMemcpyAsync(da, aug.data(), aug.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipMemsetAsync(dbad, 0, sizeof(int), c->stream) != hipSuccess ||
+            hipMemcpyAsync(ps->cidx, list.data(), list.size() * sizeof(long long), hipMemcpyHostToDevice,
+                           c->stream) != hipSuccess) {
+            release();
+            vfs_set_error("copy to the device failed in the Poisson set-up");
+            return -1;
+        }
+        for (int col = 0; col < nc; col++) {
+            hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(1024), 0, c->stream, da, nc, col, df, dbad);
+            hipLaunchKernelGGL(k_gj_update, dim3(grid_for(2LL * nc - col, 256), nc), dim3(256), 0, c->stream, da, nc,
+                               col, df, dbad);
+        }
+        hipLaunchKernelGGL(k_gj_out, dim3(grid_for((long long)nc * nc, 256)), dim3(256), 0, c->stream, da, nc,
+                           ps->cidx, cl.n, ps->cinv64, ps->cinv32, ps->cinvr64, ps->cinvr32);
+        const hipError_t le = hipGetLastError();
+        const hipError_t ce = hipMemcpyAsync(&bad, dbad, sizeof(int), hipMemcpyDeviceToHost, c->stream);
+        const hipError_t se = hipStreamSynchronize(c->stream);
+        release();
+        if (le != hipSuccess || ce != hipSuccess || se != hipSuccess) {
+            vfs_set_error("the coarsest Poisson inverse failed on the device");
+            return -1;
+        }
+        if (bad) { vfs_set_error("singular coarsest Poisson operator"); return -1; }
     }
-    // column-major (cinv[c * nc + r]) so that the rows of a solve, one per
-    // thread, read consecutive addresses for each column
-    {
-        std::vector<double> T((size_t)nc * nc);
-        for (int r = 0; r < nc; r++)
-            for (int q = 0; q < nc; q++) T[(size_t)q * nc + r] = M[(size_t)r * nc + q];
-        M.swap(T);
-    }
-    std::vector<float> M32(M.begin(), M.end());
-    VFS_CHECK_HIP(hipMemcpy(ps->cinv64, M.data(), M.size() * sizeof(double), hipMemcpyHostToDevice));
-    VFS_CHECK_HIP(hipMemcpy(ps->cinv32, M32.data(), M32.size() * sizeof(float), hipMemcpyHostToDevice));
-    VFS_CHECK_HIP(hipMemcpy(ps->cidx, list.data(), list.size() * sizeof(long long), hipMemcpyHostToDevice));
     // Krylov space
     const size_t n0 = (size_t)ps->lv[0].n;
     DMALLOC(ps->V64, (size_t)(ps->m + 1) * n0 * sizeof(double));
