@@ -42,7 +42,8 @@ struct Poisson {
     float *w32, *z32, *x32, *bb32;
     double *dots;                       // device scalars for multi-dot results
     double *hdots;                      // pinned host mirror
-    double omega, alpha;
+    double omega;                       // line-smoother damping
+    double alpha, alpha_semi;           // scale of the coarse correction from 2x2x2 / 2x2x1 (k kept) aggregates
     int wdepth;                         // levels l < wdepth visit level l+1 twice (W-cycle)
     int lsmall;                         // first level cycled by the one-workgroup kernel
 };

@@ -22,6 +22,20 @@
 #include "vfs_poisson.h"
 #include "vfs_reduce.h"
 
+// Scale of the prolongated coarse correction (over-correction: the
+// piecewise-constant prolongation of an aggregate undershoots the smooth
+// error), for 2x2x2 aggregates and for 2x2x1 ones (k kept).  Measured with
+// GMRES to 1e-8: 1.4 / 1.2 take 22 iterations at 256x128x128 (25 at 1 / 1),
+// 23 at 512x256x256 (31) and 41 on the 768x384x48 turbine slab (59).  The
+// hierarchy loses stability above them: 1.9 on 2x2x2 aggregates, and 1.3 on
+// 2x2x1 ones with 1.4 on the others, take more iterations than 1 / 1.
+#ifndef VFS_MG_ALPHA
+#define VFS_MG_ALPHA 1.4
+#endif
+#ifndef VFS_MG_ALPHA_SEMI
+#define VFS_MG_ALPHA_SEMI 1.2
+#endif
+
 struct LV {
     int nx, ny, nz;
     long long n;
@@ -419,7 +433,8 @@ struct SmallLevels {
     int nc;
     const long long *cidx;
     const T *cinv;
-    T omega, alpha;
+    T omega;
+    T alpha[MG_MAXL];   // coarse-correction scale into level l from level l + 1
     int zero;    // the first level starts from x = 0 (else from the given x)
 };
 
@@ -519,7 +534,7 @@ __global__ void __launch_bounds__(1024) k_small_cycle(SmallLevels<T> S)
         for (long long t = threadIdx.x; t < L.n; t += blockDim.x) {
             const int i = (int)(t % L.nx), j = (int)((t / L.nx) % L.ny), k = (int)(t / ((long long)L.nx * L.ny));
             const int ck = C.sk == 1 ? k : k / 2, cj = L.ny == 1 ? j : j / 2, ci = L.nx == 1 ? i : i / 2;
-            x[t] += S.alpha * e[cix(C, ck, cj, ci)];
+            x[t] += S.alpha[l] * e[cix(C, ck, cj, ci)];
         }
         __syncthreads();
         wg_resid<T>(L, S.A[l], x, S.b[l], S.y[l]);
@@ -813,7 +828,8 @@ int vfs_poisson_alloc(vfs_ctx *c)
     // line-Jacobi damping: 0.9 took 25 GMRES iterations to 1e-8 on the
     // 256x128x128 channel against 26 for 0.8; 1.0 does not converge
     ps->omega = 0.9;
-    ps->alpha = 1.0;
+    ps->alpha = VFS_MG_ALPHA;
+    ps->alpha_semi = VFS_MG_ALPHA_SEMI;
     ps->wdepth = 2;
     ps->m = 40;
     return 0;
@@ -1167,13 +1183,19 @@ static int launch_thomas(vfs_ctx *c, const MGLevel &L, const T *r, const T *xo, 
 // the piecewise-constant aggregation hierarchy), the residual of the
 // corrected iterate, and line smoothing with the prolongation folded in.
 // Scratch: the level's y and d vectors and the coarser levels' b and x.
+// the coarse-correction scale into level l, by the shape of the aggregates of level l + 1
+static double mg_alpha(const Poisson *ps, int l)
+{
+    return l + 1 < (int)ps->lv.size() && ps->lv[l + 1].sk == 1 ? ps->alpha_semi : ps->alpha;
+}
+
 template <typename T>
 static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
 {
     Poisson *ps = c->pois;
     MGLevel &L = ps->lv[l];
     LV Lv = lv_of(L);
-    const T alpha = (T)ps->alpha;
+    const T alpha = (T)mg_alpha(ps, l);
     T *y = vec_of<T>(L, 2);
     const T *A = A_of<T>(L);
     if (l == (int)ps->lv.size() - 1) {
@@ -1200,7 +1222,7 @@ static int vcycle(vfs_ctx *c, int l, const T *b, T *x, bool zero_guess)
         S.cidx = ps->cidx;
         S.cinv = std::is_same<T, double>::value ? (const T *)ps->cinv64 : (const T *)ps->cinv32;
         S.omega = (T)ps->omega;
-        S.alpha = alpha;
+        for (int q = l; q < S.last; q++) S.alpha[q] = (T)mg_alpha(ps, q);
         S.zero = zero_guess ? 1 : 0;
         hipLaunchKernelGGL(k_small_cycle<T>, dim3(1), dim3(1024), 0, c->stream, S);
         VFS_CHECK_LAUNCH();
