@@ -29,6 +29,9 @@
 // 23 at 512x256x256 (31) and 41 on the 768x384x48 turbine slab (59).  The
 // hierarchy loses stability above them: 1.9 on 2x2x2 aggregates, and 1.3 on
 // 2x2x1 ones with 1.4 on the others, take more iterations than 1 / 1.
+#ifndef VFS_MG_WDEPTH
+#define VFS_MG_WDEPTH 2     // levels l < this visit level l + 1 twice (W-cycle)
+#endif
 #ifndef VFS_MG_ALPHA
 #define VFS_MG_ALPHA 1.4
 #endif
@@ -830,7 +833,7 @@ int vfs_poisson_alloc(vfs_ctx *c)
     ps->omega = 0.9;
     ps->alpha = VFS_MG_ALPHA;
     ps->alpha_semi = VFS_MG_ALPHA_SEMI;
-    ps->wdepth = 2;
+    ps->wdepth = VFS_MG_WDEPTH;
     ps->m = 40;
     return 0;
 }
