@@ -29,6 +29,9 @@
 // 23 at 512x256x256 (31) and 41 on the 768x384x48 turbine slab (59).  The
 // hierarchy loses stability above them: 1.9 on 2x2x2 aggregates, and 1.3 on
 // 2x2x1 ones with 1.4 on the others, take more iterations than 1 / 1.
+#ifndef VFS_MG_OMEGA
+#define VFS_MG_OMEGA 0.9    // damping of the j-line Jacobi smoother
+#endif
 #ifndef VFS_MG_WDEPTH
 #define VFS_MG_WDEPTH 2     // levels l < this visit level l + 1 twice (W-cycle)
 #endif
@@ -830,7 +833,7 @@ int vfs_poisson_alloc(vfs_ctx *c)
     DMALLOC(ps->rowflag, P);
     // line-Jacobi damping: 0.9 took 25 GMRES iterations to 1e-8 on the
     // 256x128x128 channel against 26 for 0.8; 1.0 does not converge
-    ps->omega = 0.9;
+    ps->omega = VFS_MG_OMEGA;
     ps->alpha = VFS_MG_ALPHA;
     ps->alpha_semi = VFS_MG_ALPHA_SEMI;
     ps->wdepth = VFS_MG_WDEPTH;
