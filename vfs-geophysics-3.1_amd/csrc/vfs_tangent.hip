@@ -43,7 +43,7 @@ __device__ __forceinline__ void noslip_mask(const Dims &d, int i, int j, int k, 
 // a = Mb U; optionally out = -1/dt * Mb U
 __global__ void k_mask_init(Dims d, const double *__restrict__ U, double *__restrict__ a, double *__restrict__ out)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
     double u[3] = {U[3 * t], U[3 * t + 1], U[3 * t + 2]};
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) k_ff2_faces_jvp(Dims d, Geom g, const dou
                                 const double *__restrict__ tucat, const double *__restrict__ nu_t,
                                 double *__restrict__ face)
 {
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long t = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= d.P) return;
     POINT_IJK(t);
     if (i > d.mx - 2 || j > d.my - 2 || k > d.mz - 2) return;
