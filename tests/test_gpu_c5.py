@@ -63,6 +63,9 @@ def test_c5_twenty_steps():
         out = fs.step()
         say(f"[c5] step {step}: {out}")
         assert np.isfinite(out["maxdiv"]) and out["relres"] <= 1e-8
+        # measured: 30-34 iterations on the first three steps, 18-24 after
+        # (semi-coarsened hierarchy with the over-corrected W-cycle)
+        assert out["iters"] <= (45 if step < 3 else 32)
         assert abs(out["flux_out_corrected"] - out["flux_in"]) <= 1e-12 * abs(out["flux_in"])
         # measured: 1e-5 and 2e-6 on the first two steps (the seeded field is
         # not solenoidal), then 1e-7 to 1e-9

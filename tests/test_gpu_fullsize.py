@@ -109,6 +109,9 @@ def test_poisson_full_size(big):
         it, rel = ctx.poisson_solve(db, phi, rtol=1e-10, precision=prec, maxit=1000)
         say(f"  precision {prec}: {it} iterations, rel res {rel:.2e}")
         assert rel <= 1e-10
+        # measured with the over-corrected W-cycle: 22 / 22 (C1), 26 / 28 (C2);
+        # a weaker hierarchy or cycle shows here before any residual test fails
+        assert it <= {"C1": 30, "C2": 36}[big["name"]]
         sols[prec] = phi
     a, m = sols[0].host(), sols[1].host()
     diff = np.linalg.norm(m - a) / np.linalg.norm(a)
